@@ -1,0 +1,225 @@
+"""fcl_amd — MI355X engine for FCL's BVH-vs-BVH collide()/distance() path.
+
+Python plumbing over the C ABI in include/fclx.h: contexts, mesh upload,
+single-query and batched calls, and the seeded workload generators.  The
+compute runs in libfclx.so's HIP kernels; nothing here computes a result.
+Device buffers for batched calls are torch tensors (device memory and streams
+only).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+OBBRSS = N.FCLX_BV_OBBRSS
+RSS = N.FCLX_BV_RSS
+ENGINE_PLAIN = N.FCLX_ENGINE_PLAIN
+ENGINE_FAST = N.FCLX_ENGINE_FAST
+
+
+class FclxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("fclx error %d: %s" % (code, msg))
+        self.code = code
+
+
+# ---------------------------------------------------------------- generators
+def icosphere(k, noise=0.15, seed=0):
+    """Frequency-k geodesic icosphere, radii perturbed by U(-noise, noise)."""
+    L = N.lib()
+    nv, nt = N.c_i32(), N.c_i32()
+    L.fclx_synth_icosphere_size(k, ctypes.byref(nv), ctypes.byref(nt))
+    V = np.empty((nv.value, 3), np.float64)
+    T = np.empty((nt.value, 3), np.int32)
+    rc = L.fclx_synth_icosphere(k, noise, seed, N.ptr(V, N.c_dbl), N.ptr(T, N.c_i32))
+    if rc != 0:
+        raise ValueError("bad icosphere parameters")
+    return V, T
+
+
+def heightfield(g, height_frac=0.05, seed=0):
+    """g x g heightfield over [-1,1]^2, heights U(-h,h), h = 2*height_frac."""
+    L = N.lib()
+    nv, nt = N.c_i32(), N.c_i32()
+    L.fclx_synth_heightfield_size(g, ctypes.byref(nv), ctypes.byref(nt))
+    V = np.empty((nv.value, 3), np.float64)
+    T = np.empty((nt.value, 3), np.int32)
+    rc = L.fclx_synth_heightfield(g, height_frac, seed, N.ptr(V, N.c_dbl), N.ptr(T, N.c_i32))
+    if rc != 0:
+        raise ValueError("bad heightfield parameters")
+    return V, T
+
+
+def scale_to_radius(V, r=1.0):
+    V = np.ascontiguousarray(V, np.float64)
+    N.lib().fclx_synth_scale_to_radius(N.ptr(V, N.c_dbl), V.shape[0], r)
+    return V
+
+
+def poses(n, kind=0, p0=2.2, p1=0.0, seed=0):
+    """(tf1, tf2) arrays of shape (n, 12); tf1 = identity.  kind 0: translation
+    uniform in [-p0, p0]^3; kind 1: |t| ~ U[p0, p1] along a uniform direction."""
+    tf1 = np.empty((n, 12), np.float64)
+    tf2 = np.empty((n, 12), np.float64)
+    rc = N.lib().fclx_synth_poses(n, kind, p0, p1, seed, N.ptr(tf1, N.c_dbl),
+                                  N.ptr(tf2, N.c_dbl))
+    if rc != 0:
+        raise ValueError("bad pose parameters")
+    return tf1, tf2
+
+
+def uniform(n, seed):
+    out = np.empty(n, np.float64)
+    N.lib().fclx_synth_uniform(n, seed, N.ptr(out, N.c_dbl))
+    return out
+
+
+# ------------------------------------------------------------- host builder
+def bvh_build(V, T, bv_type=OBBRSS):
+    """Host-only hierarchy build (no device): returns dict of numpy arrays
+    bv (nodes x 31), links (nodes x 3), sizes (nodes), depth."""
+    V = np.ascontiguousarray(V, np.float64)
+    T = np.ascontiguousarray(T, np.int32)
+    nb = max(2 * T.shape[0] - 1, 1)
+    bv = np.zeros((nb, 31), np.float64)
+    links = np.zeros((nb, 3), np.int32)
+    sizes = np.zeros(nb, np.float64)
+    depth = N.c_i32()
+    rc = N.lib().fclx_bvh_build(bv_type, N.ptr(V, N.c_dbl), V.shape[0], N.ptr(T, N.c_i32),
+                                T.shape[0], N.ptr(bv, N.c_dbl), N.ptr(links, N.c_i32),
+                                N.ptr(sizes, N.c_dbl), ctypes.byref(depth))
+    if rc != 0:
+        raise FclxError(rc, "bvh build failed")
+    return {"bv": bv, "links": links, "sizes": sizes, "depth": depth.value}
+
+
+# ------------------------------------------------------------------- context
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """One GPU.  Meshes are uploaded once and named by integer ids."""
+
+    def __init__(self, device=0):
+        self._L = N.lib()
+        h = ctypes.c_void_p()
+        rc = self._L.fclx_init(device, ctypes.byref(h))
+        if rc != 0:
+            raise FclxError(rc, "fclx_init failed (is a GPU visible?)")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            self._L.fclx_shutdown(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FclxError(rc, (self._L.fclx_last_error(self._h) or b"").decode())
+        return rc
+
+    # meshes
+    def mesh(self, V, T, bv_type=OBBRSS):
+        V = np.ascontiguousarray(V, np.float64)
+        T = np.ascontiguousarray(T, np.int32)
+        mid = N.c_i32()
+        self._check(self._L.fclx_mesh_create(self._h, bv_type, N.ptr(V, N.c_dbl), V.shape[0],
+                                             N.ptr(T, N.c_i32), T.shape[0],
+                                             ctypes.byref(mid)))
+        return mid.value
+
+    def mesh_destroy(self, mid):
+        self._check(self._L.fclx_mesh_destroy(self._h, mid))
+
+    def mesh_info(self, mid):
+        info = N.MeshInfo()
+        self._check(self._L.fclx_mesh_get_info(self._h, mid, ctypes.byref(info)))
+        return {f: getattr(info, f) for f, _ in N.MeshInfo._fields_}
+
+    def export_nodes(self, mid):
+        info = self.mesh_info(mid)
+        nb = info["num_bvs"]
+        bv = np.zeros((nb, 31), np.float64)
+        links = np.zeros((nb, 3), np.int32)
+        sizes = np.zeros(nb, np.float64)
+        self._check(self._L.fclx_mesh_export_nodes(self._h, mid, N.ptr(bv, N.c_dbl),
+                                                   N.ptr(links, N.c_i32),
+                                                   N.ptr(sizes, N.c_dbl)))
+        return {"bv": bv, "links": links, "sizes": sizes, "depth": info["depth"]}
+
+    # single queries (host buffers)
+    def collide(self, m1, tf1, m2, tf2, cap=1):
+        tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
+        tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
+        pairs = np.zeros((cap, 2), np.int32)
+        n = N.c_i32()
+        stats = np.zeros(2, np.int64)
+        self._check(self._L.fclx_collide(self._h, m1, N.ptr(tf1, N.c_dbl), m2,
+                                         N.ptr(tf2, N.c_dbl), cap, N.ptr(pairs, N.c_i32),
+                                         ctypes.byref(n), N.ptr(stats, N.c_i64)))
+        return pairs[: n.value].copy(), stats
+
+    def distance(self, m1, tf1, m2, tf2, nearest=True):
+        tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
+        tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
+        d = N.c_double()
+        pts = np.zeros(6, np.float64)
+        ids = np.zeros(2, np.int32)
+        stats = np.zeros(2, np.int64)
+        self._check(self._L.fclx_distance(self._h, m1, N.ptr(tf1, N.c_dbl), m2,
+                                          N.ptr(tf2, N.c_dbl), 1 if nearest else 0,
+                                          ctypes.byref(d), N.ptr(pts, N.c_dbl),
+                                          N.ptr(ids, N.c_i32), N.ptr(stats, N.c_i64)))
+        return d.value, pts.reshape(2, 3), ids, stats
+
+    # batched calls on device tensors
+    def collide_batch_async(self, query_meshes, tf1, tf2, num_max_contacts, counts, pairs,
+                            num_pairs, engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,
+                            stream=None):
+        a = N.CollideBatchArgs()
+        a.num_queries = int(query_meshes.shape[0])
+        a.query_meshes = query_meshes.data_ptr()
+        a.tf1 = tf1.data_ptr()
+        a.tf2 = tf2.data_ptr()
+        a.num_max_contacts = int(num_max_contacts)
+        a.engine = int(engine)
+        a.counts = counts.data_ptr()
+        a.pairs = pairs.data_ptr() if pairs is not None else None
+        a.pair_capacity = int(pairs.shape[0]) if pairs is not None else 0
+        a.num_pairs = num_pairs.data_ptr()
+        a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
+        a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
+        self._check(self._L.fclx_collide_batch(self._h, ctypes.byref(a), stream))
+
+    def distance_batch_async(self, query_meshes, tf1, tf2, distance, points=None, ids=None,
+                             engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,
+                             stream=None, nearest=True):
+        a = N.DistanceBatchArgs()
+        a.num_queries = int(query_meshes.shape[0])
+        a.query_meshes = query_meshes.data_ptr()
+        a.tf1 = tf1.data_ptr()
+        a.tf2 = tf2.data_ptr()
+        a.enable_nearest_points = 1 if nearest else 0
+        a.engine = int(engine)
+        a.distance = distance.data_ptr()
+        a.points = points.data_ptr() if points is not None else None
+        a.ids = ids.data_ptr() if ids is not None else None
+        a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
+        a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
+        self._check(self._L.fclx_distance_batch(self._h, ctypes.byref(a), stream))
+
+    def sync(self, stream=None):
+        return self._check(self._L.fclx_sync(self._h, stream))
+
+    def sync_status(self, stream=None):
+        """Like sync() but returns the status code instead of raising."""
+        return self._L.fclx_sync(self._h, stream)

@@ -1,0 +1,107 @@
+"""ctypes binding of libfclx.so (the C ABI declared in include/fclx.h and
+include/fclx_synth.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+it is missing the import of any entry point raises NativeLibraryMissing.
+"""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfclx.so")
+
+FCLX_OK = 0
+FCLX_ERR_INVALID_ARGUMENT = -1
+FCLX_ERR_OUT_OF_MEMORY = -2
+FCLX_ERR_CAPACITY = -3
+FCLX_ERR_DEVICE = -4
+FCLX_ERR_UNSUPPORTED = -5
+FCLX_ERR_EMPTY_MODEL = -6
+FCLX_ERR_STACK_OVERFLOW = -7
+
+FCLX_BV_OBBRSS = 0
+FCLX_BV_RSS = 1
+FCLX_ENGINE_PLAIN = 0
+FCLX_ENGINE_FAST = 1
+
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+P_i32 = ctypes.POINTER(c_i32)
+P_i64 = ctypes.POINTER(c_i64)
+P_dbl = ctypes.POINTER(c_dbl)
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("num_bvs", c_i32), ("num_tris", c_i32), ("num_vertices", c_i32),
+                ("depth", c_i32), ("bv_type", c_i32), ("reserved", c_i32),
+                ("device_bytes", c_i64)]
+
+
+class CollideBatchArgs(ctypes.Structure):
+    _fields_ = [("num_queries", c_i32), ("query_meshes", c_vp), ("tf1", c_vp), ("tf2", c_vp),
+                ("num_max_contacts", c_i32), ("engine", c_i32), ("counts", c_vp),
+                ("pairs", c_vp), ("pair_capacity", c_i64), ("num_pairs", c_vp),
+                ("bv_tests", c_vp), ("leaf_tests", c_vp)]
+
+
+class DistanceBatchArgs(ctypes.Structure):
+    _fields_ = [("num_queries", c_i32), ("query_meshes", c_vp), ("tf1", c_vp), ("tf2", c_vp),
+                ("enable_nearest_points", c_i32), ("engine", c_i32), ("distance", c_vp),
+                ("points", c_vp), ("ids", c_vp), ("bv_tests", c_vp), ("leaf_tests", c_vp)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "fclx_init": (c_i32, [c_i32, ctypes.POINTER(c_vp)]),
+    "fclx_shutdown": (None, [c_vp]),
+    "fclx_last_error": (ctypes.c_char_p, [c_vp]),
+    "fclx_mesh_create": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_i32, c_i32, P_i32]),
+    "fclx_mesh_destroy": (c_i32, [c_vp, c_i32]),
+    "fclx_mesh_get_info": (c_i32, [c_vp, c_i32, ctypes.POINTER(MeshInfo)]),
+    "fclx_mesh_export_nodes": (c_i32, [c_vp, c_i32, P_dbl, P_i32, P_dbl]),
+    "fclx_bvh_build": (c_i32, [c_i32, P_dbl, c_i32, P_i32, c_i32, P_dbl, P_i32, P_dbl,
+                               P_i32]),
+    "fclx_collide_batch": (c_i32, [c_vp, ctypes.POINTER(CollideBatchArgs), c_vp]),
+    "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
+    "fclx_sync": (c_i32, [c_vp, c_vp]),
+    "fclx_collide": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, P_i32, P_i64]),
+    "fclx_distance": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_dbl, P_dbl, P_i32,
+                              P_i64]),
+    "fclx_synth_icosphere_size": (None, [c_i32, P_i32, P_i32]),
+    "fclx_synth_icosphere": (c_i32, [c_i32, c_dbl, c_u64, P_dbl, P_i32]),
+    "fclx_synth_heightfield_size": (None, [c_i32, P_i32, P_i32]),
+    "fclx_synth_heightfield": (c_i32, [c_i32, c_dbl, c_u64, P_dbl, P_i32]),
+    "fclx_synth_scale_to_radius": (None, [P_dbl, c_i32, c_dbl]),
+    "fclx_synth_poses": (c_i32, [c_i32, c_i32, c_dbl, c_dbl, c_u64, P_dbl, P_dbl]),
+    "fclx_synth_uniform": (None, [c_i64, c_u64, P_dbl]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libfclx.so with argtypes set; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                "libfclx.so not found at %s; build it with __graft_entry__.build()" % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def ptr(arr, ctype):
+    """ctypes pointer to a C-contiguous numpy array."""
+    assert arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))

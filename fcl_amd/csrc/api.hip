@@ -1,0 +1,604 @@
+// C ABI of the engine: contexts, mesh upload (host BVH build -> HBM), and
+// the batched / single-query collide and distance entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fclx.h"
+#include "engine.h"
+#include "host/bvh_build.h"
+#include "kernels.h"
+
+using namespace fclx;
+
+namespace {
+
+struct MeshSlot {
+  bool live = false;
+  HostBVH bvh;
+  ObbNode64* obb64 = nullptr;
+  RssNode64* rss64 = nullptr;
+  double* size64 = nullptr;
+  ObbNode32* obb32 = nullptr;
+  double* verts = nullptr;
+  int32_t* tris = nullptr;
+  int64_t bytes = 0;
+  MeshDev dev{};
+};
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t n) {
+  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * std::max<size_t>(n, 1));
+}
+
+__global__ void zero_i64(long long* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+struct fclx_context {
+  int device = 0;
+  std::mutex mu;          // meshes, table, single-query staging
+  std::string err;
+  std::vector<MeshSlot> meshes;
+  MeshDev* d_table = nullptr;
+  int table_cap = 0;
+  int32_t* d_status = nullptr;
+  hipStream_t stream = nullptr;  // single-query calls
+  // single-query staging
+  double* s_tf = nullptr;        // 24
+  int32_t* s_qm = nullptr;       // 2
+  int32_t* s_count = nullptr;    // 1
+  long long* s_num = nullptr;    // 1
+  long long* s_stats = nullptr;  // 2
+  int32_t* s_pairs = nullptr;
+  long long s_pairs_cap = 0;
+  double* s_dist = nullptr;      // 1 + 6 points
+  int32_t* s_ids = nullptr;      // 2
+  EngineWorkspace* ws = nullptr;
+};
+
+namespace {
+
+int fail(fclx_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(fclx_context* ctx, hipError_t e, const char* what) {
+  return fail(ctx, e == hipErrorOutOfMemory ? FCLX_ERR_OUT_OF_MEMORY : FCLX_ERR_DEVICE,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                              \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+void free_slot(MeshSlot& s) {
+  (void)hipFree(s.obb64);
+  (void)hipFree(s.rss64);
+  (void)hipFree(s.size64);
+  (void)hipFree(s.obb32);
+  (void)hipFree(s.verts);
+  (void)hipFree(s.tris);
+  s = MeshSlot{};
+}
+
+int table_sync(fclx_context* ctx) {
+  const int n = static_cast<int>(ctx->meshes.size());
+  if (n > ctx->table_cap) {
+    int cap = std::max(16, ctx->table_cap * 2);
+    while (cap < n) cap *= 2;
+    MeshDev* t = nullptr;
+    HIP_TRY(ctx, dmalloc(&t, cap));
+    (void)hipFree(ctx->d_table);
+    ctx->d_table = t;
+    ctx->table_cap = cap;
+  }
+  std::vector<MeshDev> host(n);
+  for (int i = 0; i < n; ++i) host[i] = ctx->meshes[i].live ? ctx->meshes[i].dev : MeshDev{};
+  HIP_TRY(ctx, hipMemcpy(ctx->d_table, host.data(), sizeof(MeshDev) * n, hipMemcpyHostToDevice));
+  return FCLX_OK;
+}
+
+bool valid_mesh(const fclx_context* ctx, int32_t id) {
+  return id >= 0 && id < static_cast<int32_t>(ctx->meshes.size()) && ctx->meshes[id].live;
+}
+
+int max_depth(const fclx_context* ctx) {
+  int d = 0;
+  for (const auto& m : ctx->meshes)
+    if (m.live) d = std::max(d, m.bvh.depth);
+  return d;
+}
+
+// Largest |value| rounding error of a double -> float conversion over the
+// OBB coordinates of a mesh: half a float ulp of the largest magnitude.
+float obb32_error(const std::vector<HostNode>& nodes) {
+  double mx = 0;
+  for (const auto& n : nodes) {
+    for (double v : n.obb_to) mx = std::max(mx, std::fabs(v));
+    for (double v : n.obb_extent) mx = std::max(mx, std::fabs(v));
+  }
+  return static_cast<float>(std::ldexp(std::max(mx, 1.0), -24));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fclx_init(int32_t device, fclx_context** out) {
+  if (!out) return FCLX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FCLX_ERR_DEVICE;
+  if (device < 0 || device >= ndev) return FCLX_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return FCLX_ERR_DEVICE;
+  auto* ctx = new (std::nothrow) fclx_context();
+  if (!ctx) return FCLX_ERR_OUT_OF_MEMORY;
+  ctx->device = device;
+  bool ok = dmalloc(&ctx->d_status, 1) == hipSuccess &&
+            hipMemset(ctx->d_status, 0, sizeof(int32_t)) == hipSuccess &&
+            hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+            dmalloc(&ctx->s_tf, 24) == hipSuccess && dmalloc(&ctx->s_qm, 2) == hipSuccess &&
+            dmalloc(&ctx->s_count, 1) == hipSuccess && dmalloc(&ctx->s_num, 1) == hipSuccess &&
+            dmalloc(&ctx->s_stats, 2) == hipSuccess && dmalloc(&ctx->s_dist, 7) == hipSuccess &&
+            dmalloc(&ctx->s_ids, 2) == hipSuccess;
+  if (!ok) {
+    fclx_shutdown(ctx);
+    return FCLX_ERR_DEVICE;
+  }
+  ctx->ws = engine_workspace_create();
+  *out = ctx;
+  return FCLX_OK;
+}
+
+void fclx_shutdown(fclx_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  for (auto& m : ctx->meshes)
+    if (m.live) free_slot(m);
+  engine_workspace_destroy(ctx->ws);
+  (void)hipFree(ctx->d_table);
+  (void)hipFree(ctx->d_status);
+  (void)hipFree(ctx->s_tf);
+  (void)hipFree(ctx->s_qm);
+  (void)hipFree(ctx->s_count);
+  (void)hipFree(ctx->s_num);
+  (void)hipFree(ctx->s_stats);
+  (void)hipFree(ctx->s_pairs);
+  (void)hipFree(ctx->s_dist);
+  (void)hipFree(ctx->s_ids);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* fclx_last_error(const fclx_context* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
+                     int32_t num_vertices, const int32_t* tris, int32_t num_tris,
+                     int32_t* mesh_id) {
+  if (!ctx || !mesh_id) return FCLX_ERR_INVALID_ARGUMENT;
+  if (bv_type != FCLX_BV_OBBRSS && bv_type != FCLX_BV_RSS)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "only OBBRSS and RSS hierarchies are supported");
+  if (num_tris <= 0 || num_vertices <= 0 || !vertices || !tris)
+    return fail(ctx, FCLX_ERR_EMPTY_MODEL, "model has no triangles or vertices");
+  for (int64_t i = 0; i < 3LL * num_tris; ++i)
+    if (tris[i] < 0 || tris[i] >= num_vertices)
+      return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "triangle index out of range");
+  for (int64_t i = 0; i < 3LL * num_vertices; ++i)
+    if (!std::isfinite(vertices[i]))
+      return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "non-finite vertex coordinate");
+  MeshSlot s;
+  if (build_bvh(s.bvh, bv_type, vertices, num_vertices, tris, num_tris) != 0)
+    return fail(ctx, FCLX_ERR_EMPTY_MODEL, "empty model");
+  const auto& nodes = s.bvh.nodes;
+  const int nb = static_cast<int>(nodes.size());
+  std::vector<ObbNode64> o64(nb);
+  std::vector<RssNode64> r64(nb);
+  std::vector<ObbNode32> o32(nb);
+  std::vector<double> sz(nb);
+  for (int i = 0; i < nb; ++i) {
+    const HostNode& h = nodes[i];
+    ObbNode64& o = o64[i];
+    std::memcpy(o.axis, h.obb_axis, sizeof(o.axis));
+    std::memcpy(o.to, h.obb_to, sizeof(o.to));
+    std::memcpy(o.extent, h.obb_extent, sizeof(o.extent));
+    o.first_child = h.first_child;
+    o.pad = 0;
+    RssNode64& r = r64[i];
+    std::memcpy(r.axis, h.rss_axis, sizeof(r.axis));
+    std::memcpy(r.to, h.rss_to, sizeof(r.to));
+    r.l[0] = h.rss_l[0];
+    r.l[1] = h.rss_l[1];
+    r.r = h.rss_r;
+    r.first_child = h.first_child;
+    r.pad = 0;
+    ObbNode32& f = o32[i];
+    for (int k = 0; k < 9; ++k) f.axis[k] = static_cast<float>(h.obb_axis[k]);
+    for (int k = 0; k < 3; ++k) {
+      f.to[k] = static_cast<float>(h.obb_to[k]);
+      f.extent[k] = static_cast<float>(h.obb_extent[k]);
+    }
+    f.first_child = h.first_child;
+    sz[i] = s.bvh.node_size(i);
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  bool ok = dmalloc(&s.obb64, nb) == hipSuccess && dmalloc(&s.rss64, nb) == hipSuccess &&
+            dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32, nb) == hipSuccess &&
+            dmalloc(&s.verts, 3 * static_cast<size_t>(num_vertices)) == hipSuccess &&
+            dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess;
+  if (ok)
+    ok = hipMemcpy(s.obb64, o64.data(), sizeof(ObbNode64) * nb, hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.rss64, r64.data(), sizeof(RssNode64) * nb, hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.size64, sz.data(), sizeof(double) * nb, hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.obb32, o32.data(), sizeof(ObbNode32) * nb, hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.verts, vertices, sizeof(double) * 3 * num_vertices,
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(s.tris, tris, sizeof(int32_t) * 3 * num_tris, hipMemcpyHostToDevice) ==
+             hipSuccess;
+  if (!ok) {
+    free_slot(s);
+    return fail(ctx, FCLX_ERR_OUT_OF_MEMORY, "device allocation or upload failed");
+  }
+  s.bytes = static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
+                                        sizeof(ObbNode32)) +
+            24LL * num_vertices + 12LL * num_tris;
+  s.live = true;
+  MeshDev& d = s.dev;
+  d.obb64 = s.obb64;
+  d.rss64 = s.rss64;
+  d.size64 = s.size64;
+  d.obb32 = s.obb32;
+  d.verts = s.verts;
+  d.tris = s.tris;
+  d.num_bvs = nb;
+  d.num_tris = num_tris;
+  d.num_verts = num_vertices;
+  d.depth = s.bvh.depth;
+  d.bv_type = bv_type;
+  d.obb32_err = obb32_error(nodes);
+  int id = -1;
+  for (size_t i = 0; i < ctx->meshes.size(); ++i)
+    if (!ctx->meshes[i].live) {
+      id = static_cast<int>(i);
+      break;
+    }
+  if (id < 0) {
+    id = static_cast<int>(ctx->meshes.size());
+    ctx->meshes.emplace_back();
+  }
+  ctx->meshes[id] = std::move(s);
+  int rc = table_sync(ctx);
+  if (rc != FCLX_OK) {
+    free_slot(ctx->meshes[id]);
+    return rc;
+  }
+  *mesh_id = id;
+  return FCLX_OK;
+}
+
+int fclx_mesh_destroy(fclx_context* ctx, int32_t mesh_id) {
+  if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh_id)) return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();  // no kernel may still read the mesh
+  free_slot(ctx->meshes[mesh_id]);
+  return table_sync(ctx);
+}
+
+int fclx_mesh_get_info(fclx_context* ctx, int32_t mesh_id, fclx_mesh_info* info) {
+  if (!ctx || !info) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh_id)) return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  const MeshSlot& s = ctx->meshes[mesh_id];
+  info->num_bvs = static_cast<int32_t>(s.bvh.nodes.size());
+  info->num_tris = static_cast<int32_t>(s.bvh.tris.size() / 3);
+  info->num_vertices = static_cast<int32_t>(s.bvh.vertices.size() / 3);
+  info->depth = s.bvh.depth;
+  info->bv_type = s.bvh.bv_type;
+  info->reserved = 0;
+  info->device_bytes = s.bytes;
+  return FCLX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+void export_nodes(const HostBVH& b, double* bv_data, int32_t* links, double* sizes) {
+  for (size_t i = 0; i < b.nodes.size(); ++i) {
+    const HostNode& n = b.nodes[i];
+    if (bv_data) {
+      double* o = bv_data + 31 * i;
+      std::memcpy(o, n.obb_axis, 9 * sizeof(double));
+      std::memcpy(o + 9, n.obb_to, 3 * sizeof(double));
+      std::memcpy(o + 12, n.obb_extent, 3 * sizeof(double));
+      std::memcpy(o + 15, n.rss_axis, 9 * sizeof(double));
+      std::memcpy(o + 24, n.rss_to, 3 * sizeof(double));
+      o[27] = n.rss_l[0];
+      o[28] = n.rss_l[1];
+      o[29] = n.rss_r;
+      o[30] = 0;
+    }
+    if (links) {
+      links[3 * i] = n.first_child;
+      links[3 * i + 1] = n.first_primitive;
+      links[3 * i + 2] = n.num_primitives;
+    }
+    if (sizes) sizes[i] = b.node_size(static_cast<int>(i));
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int fclx_mesh_export_nodes(fclx_context* ctx, int32_t mesh_id, double* bv_data, int32_t* links,
+                           double* sizes) {
+  if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh_id)) return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  export_nodes(ctx->meshes[mesh_id].bvh, bv_data, links, sizes);
+  return FCLX_OK;
+}
+
+int fclx_bvh_build(int32_t bv_type, const double* vertices, int32_t num_vertices,
+                   const int32_t* tris, int32_t num_tris, double* bv_data, int32_t* links,
+                   double* sizes, int32_t* depth) {
+  if (bv_type != FCLX_BV_OBBRSS && bv_type != FCLX_BV_RSS) return FCLX_ERR_UNSUPPORTED;
+  if (num_tris <= 0 || num_vertices <= 0 || !vertices || !tris) return FCLX_ERR_EMPTY_MODEL;
+  for (int64_t i = 0; i < 3LL * num_tris; ++i)
+    if (tris[i] < 0 || tris[i] >= num_vertices) return FCLX_ERR_INVALID_ARGUMENT;
+  HostBVH b;
+  if (build_bvh(b, bv_type, vertices, num_vertices, tris, num_tris) != 0)
+    return FCLX_ERR_EMPTY_MODEL;
+  export_nodes(b, bv_data, links, sizes);
+  if (depth) *depth = b.depth;
+  return FCLX_OK;
+}
+
+int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->num_queries < 0 || a->num_max_contacts < 1 || !a->counts || !a->num_pairs ||
+      (a->num_queries > 0 && (!a->query_meshes || !a->tf1 || !a->tf2)) ||
+      (a->pair_capacity > 0 && !a->pairs))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid collide batch arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int depth, nmesh;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    depth = max_depth(ctx);
+    nmesh = static_cast<int>(ctx->meshes.size());
+  }
+  if (2 * depth > kMaxStack)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
+  (void)hipSetDevice(ctx->device);
+  zero_i64<<<1, 64, 0, s>>>(reinterpret_cast<long long*>(a->num_pairs), 1);
+  CollideLaunch L{};
+  L.meshes = ctx->d_table;
+  L.num_meshes = nmesh;
+  L.num_queries = a->num_queries;
+  L.query_meshes = a->query_meshes;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.cap = a->num_max_contacts;
+  L.counts = a->counts;
+  L.pairs = a->pairs;
+  L.pair_capacity = a->pairs ? a->pair_capacity : 0;
+  L.num_pairs = reinterpret_cast<long long*>(a->num_pairs);
+  L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
+  L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
+  L.status = ctx->d_status;
+  hipError_t e;
+  if (a->engine == FCLX_ENGINE_PLAIN) {
+    e = launch_collide_plain(L, 2 * depth, s);
+  } else if (a->engine == FCLX_ENGINE_FAST) {
+    e = engine_collide(ctx->ws, L, 2 * depth, s);
+  } else {
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "collide launch");
+  return FCLX_OK;
+}
+
+int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->num_queries < 0 || !a->distance ||
+      (a->num_queries > 0 && (!a->query_meshes || !a->tf1 || !a->tf2)))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid distance batch arguments");
+  if (a->engine != FCLX_ENGINE_PLAIN && a->engine != FCLX_ENGINE_FAST)
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int depth, nmesh;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    depth = max_depth(ctx);
+    nmesh = static_cast<int>(ctx->meshes.size());
+  }
+  if (2 * depth > kMaxStack)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
+  (void)hipSetDevice(ctx->device);
+  DistanceLaunch L{};
+  L.meshes = ctx->d_table;
+  L.num_meshes = nmesh;
+  L.num_queries = a->num_queries;
+  L.query_meshes = a->query_meshes;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.distance = a->distance;
+  L.points = a->enable_nearest_points ? a->points : nullptr;
+  L.ids = a->ids;
+  L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
+  L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
+  L.status = ctx->d_status;
+  hipError_t e = launch_distance_plain(L, 2 * depth, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "distance launch");
+  return FCLX_OK;
+}
+
+int fclx_sync(fclx_context* ctx, void* stream) {
+  if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  HIP_TRY(ctx, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  int32_t st = 0;
+  HIP_TRY(ctx, hipMemcpy(&st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st != 0) {
+    HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
+    if (st & kStatusStackOverflow)
+      return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "a traversal exceeded its stack bound");
+    if (st & kStatusBadQuery)
+      return fail(ctx, FCLX_ERR_INVALID_ARGUMENT,
+                  "a query named a missing mesh or an unsupported BV pair");
+    if (st & (kStatusPairOverflow | kStatusQueueOverflow))
+      return fail(ctx, FCLX_ERR_CAPACITY, "pair output exceeded pair_capacity");
+  }
+  return FCLX_OK;
+}
+
+int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t* num_contacts,
+                 int64_t* stats) {
+  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1)
+    return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh1) || !valid_mesh(ctx, mesh2))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  if (ctx->meshes[mesh1].bvh.bv_type != FCLX_BV_OBBRSS ||
+      ctx->meshes[mesh2].bvh.bv_type != FCLX_BV_OBBRSS)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "mesh collide needs OBBRSS hierarchies");
+  (void)hipSetDevice(ctx->device);
+  if (cap > ctx->s_pairs_cap) {
+    (void)hipFree(ctx->s_pairs);
+    ctx->s_pairs = nullptr;
+    ctx->s_pairs_cap = 0;
+    HIP_TRY(ctx, dmalloc(&ctx->s_pairs, 3 * static_cast<size_t>(cap)));
+    ctx->s_pairs_cap = cap;
+  }
+  double tf[24];
+  std::memcpy(tf, tf1, 12 * sizeof(double));
+  std::memcpy(tf + 12, tf2, 12 * sizeof(double));
+  const int32_t qm[2] = {mesh1, mesh2};
+  hipStream_t s = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->s_tf, tf, sizeof(tf), hipMemcpyHostToDevice, s));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->s_qm, qm, sizeof(qm), hipMemcpyHostToDevice, s));
+  zero_i64<<<1, 64, 0, s>>>(ctx->s_num, 1);
+  CollideLaunch L{};
+  L.meshes = ctx->d_table;
+  L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+  L.num_queries = 1;
+  L.query_meshes = ctx->s_qm;
+  L.tf1 = ctx->s_tf;
+  L.tf2 = ctx->s_tf + 12;
+  L.cap = cap;
+  L.counts = ctx->s_count;
+  L.pairs = ctx->s_pairs;
+  L.pair_capacity = cap;
+  L.num_pairs = ctx->s_num;
+  L.bv_tests = ctx->s_stats;
+  L.leaf_tests = ctx->s_stats + 1;
+  L.status = ctx->d_status;
+  const int depth = ctx->meshes[mesh1].bvh.depth + ctx->meshes[mesh2].bvh.depth;
+  HIP_TRY(ctx, launch_collide_plain(L, depth, s));
+  int32_t count = 0;
+  long long st[2];
+  HIP_TRY(ctx, hipMemcpyAsync(&count, ctx->s_count, sizeof(count), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (count > 0 && pairs) {
+    std::vector<int32_t> rec(3 * static_cast<size_t>(count));
+    HIP_TRY(ctx, hipMemcpy(rec.data(), ctx->s_pairs, sizeof(int32_t) * rec.size(),
+                           hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; ++i) {
+      pairs[2 * i] = rec[3 * i + 1];
+      pairs[2 * i + 1] = rec[3 * i + 2];
+    }
+  }
+  *num_contacts = count;
+  if (stats) {
+    stats[0] = st[0];
+    stats[1] = st[1];
+  }
+  int32_t flag = 0;
+  HIP_TRY(ctx, hipMemcpy(&flag, ctx->d_status, sizeof(flag), hipMemcpyDeviceToHost));
+  if (flag & kStatusStackOverflow) {
+    HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
+    return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "traversal stack overflow");
+  }
+  return FCLX_OK;
+}
+
+int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                  const double tf2[12], int32_t enable_nearest_points, double* min_distance,
+                  double* points, int32_t* ids, int64_t* stats) {
+  if (!ctx || !tf1 || !tf2 || !min_distance) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh1) || !valid_mesh(ctx, mesh2))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  if (ctx->meshes[mesh1].bvh.bv_type != ctx->meshes[mesh2].bvh.bv_type)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "both meshes must use the same BV type");
+  (void)hipSetDevice(ctx->device);
+  double tf[24];
+  std::memcpy(tf, tf1, 12 * sizeof(double));
+  std::memcpy(tf + 12, tf2, 12 * sizeof(double));
+  const int32_t qm[2] = {mesh1, mesh2};
+  hipStream_t s = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->s_tf, tf, sizeof(tf), hipMemcpyHostToDevice, s));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->s_qm, qm, sizeof(qm), hipMemcpyHostToDevice, s));
+  DistanceLaunch L{};
+  L.meshes = ctx->d_table;
+  L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+  L.num_queries = 1;
+  L.query_meshes = ctx->s_qm;
+  L.tf1 = ctx->s_tf;
+  L.tf2 = ctx->s_tf + 12;
+  L.distance = ctx->s_dist;
+  L.points = enable_nearest_points ? ctx->s_dist + 1 : nullptr;
+  L.ids = ctx->s_ids;
+  L.bv_tests = ctx->s_stats;
+  L.leaf_tests = ctx->s_stats + 1;
+  L.status = ctx->d_status;
+  const int depth = ctx->meshes[mesh1].bvh.depth + ctx->meshes[mesh2].bvh.depth;
+  HIP_TRY(ctx, launch_distance_plain(L, depth, s));
+  double out[7];
+  int32_t idv[2];
+  long long st[2];
+  HIP_TRY(ctx, hipMemcpyAsync(out, ctx->s_dist, sizeof(out), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(idv, ctx->s_ids, sizeof(idv), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  *min_distance = out[0];
+  if (points && enable_nearest_points) std::memcpy(points, out + 1, 6 * sizeof(double));
+  if (ids) {
+    ids[0] = idv[0];
+    ids[1] = idv[1];
+  }
+  if (stats) {
+    stats[0] = st[0];
+    stats[1] = st[1];
+  }
+  int32_t flag = 0;
+  HIP_TRY(ctx, hipMemcpy(&flag, ctx->d_status, sizeof(flag), hipMemcpyDeviceToHost));
+  if (flag & kStatusStackOverflow) {
+    HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
+    return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "traversal stack overflow");
+  }
+  return FCLX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Device-resident layout of an uploaded mesh.  Shared by the host upload code
+// and the kernels; plain structs only.
+//
+// A mesh lives in HBM as structure-of-node-records:
+//   obb64[n]  128 B: OBB axis (9 doubles, row-major, column j = box axis j),
+//                    centre To (3), half extents (3), first_child link.
+//   rss64[n]  128 B: RSS axis (9), origin To (3), rectangle l (2), radius r,
+//                    first_child link.
+//   size64[n]   8 B: BV::size() of the model's BV type (OBBRSS: squared norm
+//                    of the OBB extents; RSS: sqrt(l0^2+l1^2)+2r), computed on
+//                    the host so that no device sqrt enters a descent decision.
+//   obb32[n]   64 B: the OBB rounded to float, for conservative culling.
+//   verts      24 B each (x,y,z doubles); tris 12 B each (3 x int32).
+// Node records are 128-B aligned so that one lane's node fetch is one L2
+// line; a wavefront of lanes on different nodes then issues 64 whole-line
+// requests instead of 16x as many partial ones.
+#pragma once
+#include <stdint.h>
+
+namespace fclx {
+
+struct alignas(128) ObbNode64 {
+  double axis[9];
+  double to[3];
+  double extent[3];
+  int32_t first_child;  // >= 0: left child (right = left+1); < 0: -(tri+1)
+  int32_t pad;
+};
+static_assert(sizeof(ObbNode64) == 128, "ObbNode64 must be one 128-B line");
+
+struct alignas(128) RssNode64 {
+  double axis[9];
+  double to[3];
+  double l[2];
+  double r;
+  int32_t first_child;
+  int32_t pad;
+};
+static_assert(sizeof(RssNode64) == 128, "RssNode64 must be one 128-B line");
+
+struct alignas(64) ObbNode32 {
+  float axis[9];
+  float to[3];
+  float extent[3];
+  int32_t first_child;
+};
+static_assert(sizeof(ObbNode32) == 64, "ObbNode32 must be 64 B");
+
+// Per-mesh device descriptor (pointers into HBM).  Kernels receive an array
+// of these and index it with the query's mesh ids.
+struct MeshDev {
+  const ObbNode64* obb64;
+  const RssNode64* rss64;
+  const double* size64;
+  const ObbNode32* obb32;
+  const double* verts;  // 3 per vertex
+  const int32_t* tris;  // 3 per triangle
+  int32_t num_bvs;
+  int32_t num_tris;
+  int32_t num_verts;
+  int32_t depth;        // levels from root to deepest leaf (root = 1)
+  int32_t bv_type;      // 0 = OBBRSS, 1 = RSS
+  float obb32_err;      // bound on |float - double| of any obb32 coordinate
+};
+
+}  // namespace fclx

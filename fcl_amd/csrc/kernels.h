@@ -1,0 +1,57 @@
+// Launch descriptors shared by the C ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fclx_layout.h"
+
+namespace fclx {
+
+// bits of the context's device status word
+enum : int32_t {
+  kStatusStackOverflow = 1,
+  kStatusPairOverflow = 2,
+  kStatusQueueOverflow = 4,
+  kStatusBadQuery = 8,  // a query named a missing mesh or an unsupported BV pair
+};
+
+// Largest per-lane stack the traversal kernels are instantiated for; the
+// host rejects meshes whose summed depths exceed it.
+constexpr int kMaxStack = 256;
+
+struct CollideLaunch {
+  const MeshDev* meshes;       // device table indexed by mesh id
+  int32_t num_meshes;
+  int32_t num_queries;
+  const int32_t* query_meshes; // 2 per query
+  const double* tf1;           // 12 per query
+  const double* tf2;
+  int32_t cap;                 // num_max_contacts
+  int32_t* counts;             // per query
+  int32_t* pairs;              // 3 per record: query, b1, b2
+  long long pair_capacity;
+  long long* num_pairs;        // running record count (may exceed capacity)
+  long long* bv_tests;         // optional, per query
+  long long* leaf_tests;       // optional, per query
+  int32_t* status;             // device status word (kStatus* bits)
+};
+
+struct DistanceLaunch {
+  const MeshDev* meshes;
+  int32_t num_meshes;
+  int32_t num_queries;
+  const int32_t* query_meshes;
+  const double* tf1;
+  const double* tf2;
+  double* distance;            // per query
+  double* points;              // optional, 6 per query (world frame)
+  int32_t* ids;                // optional, 2 per query
+  long long* bv_tests;
+  long long* leaf_tests;
+  int32_t* status;
+};
+
+hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);
+hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s);
+
+}  // namespace fclx

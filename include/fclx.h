@@ -1,0 +1,157 @@
+/*
+ * fclx.h — C ABI of the MI355X proximity-query engine.
+ *
+ * This is the boundary the FCL mesh hot path crosses.  It replaces, for
+ * BVHModel<OBBRSS>/BVHModel<RSS> geometry:
+ *   - fcl::collide  (include/fcl/narrowphase/collision-inl.h:93-149, 178-207)
+ *     through the dispatch-table cell collision_matrix[BV_OBBRSS][BV_OBBRSS]
+ *     (narrowphase/detail/collision_func_matrix-inl.h:571-626, 851);
+ *   - fcl::distance (include/fcl/narrowphase/distance-inl.h:90-193, 222-246)
+ *     through distance_matrix[BV_RSS][BV_RSS] and [BV_OBBRSS][BV_OBBRSS]
+ *     (narrowphase/detail/distance_func_matrix-inl.h:386-458, 661-663);
+ *   - BVHModel<BV>::beginModel/addSubModel/endModel
+ *     (include/fcl/geometry/bvh/BVH_model-inl.h:206-253, 382-517), whose
+ *     built hierarchy is flattened and kept resident in device memory.
+ * Batched entry points run many queries in one launch on a caller stream.
+ *
+ * Conventions: plain pointers and sizes only.  A pose is 12 doubles: the
+ * rotation row-major (9) followed by the translation (3).  Pointers named
+ * "device" must point to device memory of the context's GPU; all others are
+ * host memory.  Every function returns FCLX_OK (0) or a negative FCLX_ERR_*
+ * code; no exception crosses the boundary.  fclx_last_error() gives text.
+ * A context may be used from several threads; calls that share the
+ * single-query staging buffers (fclx_collide, fclx_distance) serialise on an
+ * internal lock, batched calls do not.
+ */
+#ifndef FCLX_H
+#define FCLX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCLX_OK 0
+#define FCLX_ERR_INVALID_ARGUMENT (-1)
+#define FCLX_ERR_OUT_OF_MEMORY (-2)
+#define FCLX_ERR_CAPACITY (-3)       /* output buffer too small; see fclx_sync */
+#define FCLX_ERR_DEVICE (-4)         /* HIP runtime error */
+#define FCLX_ERR_UNSUPPORTED (-5)
+#define FCLX_ERR_EMPTY_MODEL (-6)    /* cf. BVH_ERR_BUILD_EMPTY_MODEL (BVH_internal.h:58-69) */
+#define FCLX_ERR_STACK_OVERFLOW (-7) /* a traversal exceeded its stack bound */
+
+#define FCLX_BV_OBBRSS 0
+#define FCLX_BV_RSS 1
+
+/* Traversal engines.  PLAIN: one query per lane, FP64 throughout, the
+ * reference's depth-first order (contacts come out in the reference's order
+ * for every num_max_contacts).  FAST: wavefront-cooperative traversal with
+ * FP32 conservative culling and FP64 leaves; the same contact set as PLAIN
+ * when a query's contact count stays within num_max_contacts, otherwise the
+ * query is re-run by PLAIN so the truncated list is the reference's. */
+#define FCLX_ENGINE_PLAIN 0
+#define FCLX_ENGINE_FAST 1
+
+typedef struct fclx_context fclx_context;
+
+typedef struct {
+  int32_t num_bvs;
+  int32_t num_tris;
+  int32_t num_vertices;
+  int32_t depth;          /* levels, root = 1 */
+  int32_t bv_type;
+  int32_t reserved;
+  int64_t device_bytes;   /* HBM held by the mesh */
+} fclx_mesh_info;
+
+typedef struct {
+  int32_t num_queries;
+  const int32_t* query_meshes; /* device: 2 mesh ids per query */
+  const double* tf1;           /* device: 12 per query */
+  const double* tf2;           /* device: 12 per query */
+  int32_t num_max_contacts;    /* CollisionRequest::num_max_contacts (>= 1) */
+  int32_t engine;              /* FCLX_ENGINE_* */
+  int32_t* counts;             /* device: contacts per query */
+  int32_t* pairs;              /* device: 3 int32 per record (query, b1, b2); may be NULL */
+  int64_t pair_capacity;       /* records that fit in pairs */
+  int64_t* num_pairs;          /* device: 1 int64, records produced (set by the call) */
+  int64_t* bv_tests;           /* device, optional: per-query num_bv_tests */
+  int64_t* leaf_tests;         /* device, optional: per-query num_leaf_tests */
+} fclx_collide_batch_args;
+
+typedef struct {
+  int32_t num_queries;
+  const int32_t* query_meshes; /* device: 2 mesh ids per query */
+  const double* tf1;           /* device */
+  const double* tf2;           /* device */
+  int32_t enable_nearest_points;
+  int32_t engine;
+  double* distance;            /* device: min_distance per query */
+  double* points;              /* device, optional: 6 per query, world frame */
+  int32_t* ids;                /* device, optional: (b1, b2) per query */
+  int64_t* bv_tests;           /* device, optional */
+  int64_t* leaf_tests;         /* device, optional */
+} fclx_distance_batch_args;
+
+/* Context on one GPU (device ordinal).  No counterpart in the reference,
+ * which has no device state. */
+int fclx_init(int32_t device, fclx_context** out);
+void fclx_shutdown(fclx_context* ctx);
+const char* fclx_last_error(const fclx_context* ctx);
+
+/* Build the BVH on the host exactly as BVHModel<BV>::endModel() does
+ * (BVH_model-inl.h:449-517, 832-938; mean split, default fitter) and upload
+ * it.  vertices: 3 doubles each; tris: 3 int32 vertex indices each.
+ * *mesh_id receives the id that queries use. */
+int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
+                     int32_t num_vertices, const int32_t* tris, int32_t num_tris,
+                     int32_t* mesh_id);
+int fclx_mesh_destroy(fclx_context* ctx, int32_t mesh_id);
+int fclx_mesh_get_info(fclx_context* ctx, int32_t mesh_id, fclx_mesh_info* info);
+/* Host copy of the built hierarchy (BVHModel::getBV, BVH_model.h:86-92):
+ * per node 31 doubles = OBB axis(9, row-major) To(3) extent(3), RSS axis(9)
+ * To(3) l(2) r(1); links = first_child, first_primitive, num_primitives;
+ * sizes = BV::size().  Any output pointer may be NULL. */
+int fclx_mesh_export_nodes(fclx_context* ctx, int32_t mesh_id, double* bv_data,
+                           int32_t* links, double* sizes);
+
+/* Host-only BVH build with no device and no context: the same hierarchy
+ * fclx_mesh_create uploads, written to caller arrays sized for 2*num_tris-1
+ * nodes in the fclx_mesh_export_nodes layout.  *depth may be NULL. */
+int fclx_bvh_build(int32_t bv_type, const double* vertices, int32_t num_vertices,
+                   const int32_t* tris, int32_t num_tris, double* bv_data, int32_t* links,
+                   double* sizes, int32_t* depth);
+
+/* Batched collide()/distance() on a HIP stream (NULL = default stream).
+ * Asynchronous: completion and overflow status come from fclx_sync. */
+int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* args,
+                       void* stream);
+int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
+                        void* stream);
+/* Wait for the stream, then report (and clear) the device status: FCLX_OK,
+ * FCLX_ERR_CAPACITY (pair records beyond pair_capacity were dropped) or
+ * FCLX_ERR_STACK_OVERFLOW. */
+int fclx_sync(fclx_context* ctx, void* stream);
+
+/* One query with host buffers: the drop-in path behind fcl::collide for two
+ * BVHModel<OBBRSS> (orientedMeshCollide, collision_func_matrix-inl.h:571-590).
+ * cap = num_max_contacts minus contacts already in the caller's result.
+ * pairs receives up to cap (b1, b2) pairs in the reference's order;
+ * stats (optional) = {num_bv_tests, num_leaf_tests}. */
+int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t* num_contacts,
+                 int64_t* stats);
+/* One query: the drop-in path behind fcl::distance for two BVHModel<RSS> or
+ * BVHModel<OBBRSS> (orientedMeshDistance, distance_func_matrix-inl.h:386-404).
+ * points (optional) = nearest points on mesh1 and mesh2 in the world frame;
+ * ids (optional) = (b1, b2). */
+int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                  const double tf2[12], int32_t enable_nearest_points, double* min_distance,
+                  double* points, int32_t* ids, int64_t* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,151 @@
+/*
+ * fclo — CPU oracle for the FCL mesh collide()/distance() hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of FCL 0.7.0's
+ * double-precision BVH build, BVH-vs-BVH collision traversal and BVH-vs-BVH
+ * distance traversal.  It exists to check the HIP engine in fcl_amd/; only
+ * the test suite, the smoke check and the CPU-baseline leg of the benchmark
+ * may link or call it.  The product never does.
+ *
+ * Every function names the reference file:line it restates (paths relative to
+ * the FCL source tree).  The reference evaluates small vector expressions
+ * through Eigen, whose association order is not fixed by FCL; this
+ * restatement fixes one order and uses it everywhere:
+ *   dot(a,b)       = (a0*b0 + a1*b1) + a2*b2
+ *   (M*v)_i        = dot(row i of M, v)
+ *   (A*B)_ij       = (A_i0*B_0j + A_i1*B_1j) + A_i2*B_2j
+ *   (A^T*B)_ij     = (A_0i*B_0j + A_1i*B_1j) + A_2i*B_2j
+ *   cross(a,b)     = (a1*b2 - a2*b1, a2*b0 - a0*b2, a0*b1 - a1*b0)
+ *   squaredNorm(v) = (v0*v0 + v1*v1) + v2*v2
+ * with no fused multiply-add (built with -ffp-contract=off).
+ *
+ * Matrices are 9 doubles, row-major: M(i,j) = m[3*i+j].  A pose is 12
+ * doubles: rotation (row-major) followed by translation.
+ */
+#ifndef FCLO_H
+#define FCLO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FCLO_BV_OBBRSS = 0, FCLO_BV_RSS = 1 };
+
+/* One BVH node: an OBB and an RSS sharing the same fitted axes, plus the
+ * reference's child/primitive link (BV_node_base.h:49-76). */
+typedef struct {
+  double obb_axis[9];
+  double obb_to[3];
+  double obb_extent[3];
+  double rss_axis[9];
+  double rss_to[3];
+  double rss_l[2];
+  double rss_r;
+  int first_child;      /* >=0: left child index; <0: -(primitive+1) */
+  int first_primitive;
+  int num_primitives;
+} fclo_node;
+
+typedef struct {
+  int bv_type;          /* FCLO_BV_OBBRSS or FCLO_BV_RSS */
+  int num_vertices;
+  int num_tris;
+  int num_bvs;
+  double *vertices;     /* 3*num_vertices */
+  int *tris;            /* 3*num_tris */
+  fclo_node *bvs;       /* 2*num_tris-1 */
+  unsigned *primitive_indices;
+} fclo_model;
+
+/* ---- math ---------------------------------------------------------------- */
+double fclo_dot(const double a[3], const double b[3]);
+void fclo_cross(const double a[3], const double b[3], double out[3]);
+void fclo_eigen_old(const double m[9], double dout[3], double vout[9]);
+void fclo_relative_transform(const double tf1[12], const double tf2[12],
+                             double R[9], double T[3]);
+void fclo_inverse_compose(const double tf1[12], const double tf2[12],
+                          double out[12]);
+
+/* ---- BVH build ----------------------------------------------------------- */
+/* Returns 0 on success; negative for an empty model. */
+int fclo_model_build(fclo_model *m, int bv_type, const double *vertices,
+                     int num_vertices, const int *tris, int num_tris);
+void fclo_model_free(fclo_model *m);
+int fclo_model_depth(const fclo_model *m);
+double fclo_node_size(const fclo_model *m, int b);
+
+/* ---- primitive tests ----------------------------------------------------- */
+int fclo_obb_disjoint(const double B[9], const double T[3], const double a[3],
+                      const double b[3]);
+int fclo_obb_overlap(const double R0[9], const double T0[3],
+                     const fclo_node *b1, const fclo_node *b2);
+int fclo_intersect_triangle(const double P1[3], const double P2[3],
+                            const double P3[3], const double Q1[3],
+                            const double Q2[3], const double Q3[3],
+                            const double R[9], const double T[3]);
+double fclo_rect_distance(const double Rab[9], const double Tab[3],
+                          const double a[2], const double b[2], double P[3],
+                          double Q[3]);
+double fclo_rss_distance(const double R0[9], const double T0[3],
+                         const fclo_node *b1, const fclo_node *b2);
+double fclo_tri_distance(const double S[9], const double T[9], double P[3],
+                         double Q[3]);
+
+/* ---- collision ----------------------------------------------------------- */
+typedef struct {
+  int num_contacts;
+  int64_t num_bv_tests;
+  int64_t num_leaf_tests;
+} fclo_collide_stats;
+
+/* One mesh-mesh collide() query (collision-inl.h:93-149 -> orientedMeshCollide).
+ * cap: number of contacts that may still be added (num_max_contacts minus the
+ * contacts already in the result).  Pairs (b1,b2) are written in traversal
+ * order into pairs[2*k]; at most cap pairs are written. */
+void fclo_collide(const fclo_model *m1, const double tf1[12],
+                  const fclo_model *m2, const double tf2[12], int cap,
+                  int *pairs, fclo_collide_stats *stats);
+
+/* ---- distance ------------------------------------------------------------ */
+typedef struct {
+  double min_distance;
+  double p1[3], p2[3];   /* nearest points, world frame */
+  int b1, b2;
+  int64_t num_bv_tests;
+  int64_t num_leaf_tests;
+} fclo_distance_out;
+
+void fclo_distance(const fclo_model *m1, const double tf1[12],
+                   const fclo_model *m2, const double tf2[12],
+                   int enable_nearest_points, fclo_distance_out *out);
+
+/* ---- brute force over all triangle pairs (no hierarchy) ------------------ */
+int64_t fclo_collide_all_pairs(const fclo_model *m1, const double tf1[12],
+                               const fclo_model *m2, const double tf2[12],
+                               int *pairs, int64_t store);
+double fclo_distance_all_pairs(const fclo_model *m1, const double tf1[12],
+                               const fclo_model *m2, const double tf2[12],
+                               int ids[2]);
+
+/* ---- batch drivers (OpenMP over queries; the CPU baseline) ---------------- */
+/* Query q collides models[q_m1[q]] vs models[q_m2[q]].  counts[q] receives
+ * the number of contacts (<= cap); the first min(counts[q], store) pairs are
+ * written to pairs[2*q*store ...].  store may be 0. */
+void fclo_collide_batch(const fclo_model *const *models, int num_queries,
+                        const int *q_m1, const int *q_m2, const double *tf1s,
+                        const double *tf2s, int cap, int store, int *counts,
+                        int *pairs, int64_t *bv_tests, int64_t *leaf_tests,
+                        int threads);
+void fclo_distance_batch(const fclo_model *const *models, int num_queries,
+                         const int *q_m1, const int *q_m2, const double *tf1s,
+                         const double *tf2s, int enable_nearest_points,
+                         double *dist, double *points, int *ids,
+                         int64_t *bv_tests, int64_t *leaf_tests, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,286 @@
+"""ctypes binding of the CPU oracle (oracle/libfclo.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfclo.so")
+
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+P_i32 = ctypes.POINTER(c_i32)
+P_i64 = ctypes.POINTER(c_i64)
+P_dbl = ctypes.POINTER(c_dbl)
+
+OBBRSS = 0
+RSS = 1
+
+
+class Node(ctypes.Structure):
+    _fields_ = [("obb_axis", c_dbl * 9), ("obb_to", c_dbl * 3), ("obb_extent", c_dbl * 3),
+                ("rss_axis", c_dbl * 9), ("rss_to", c_dbl * 3), ("rss_l", c_dbl * 2),
+                ("rss_r", c_dbl), ("first_child", c_i32), ("first_primitive", c_i32),
+                ("num_primitives", c_i32)]
+
+
+class Model(ctypes.Structure):
+    _fields_ = [("bv_type", c_i32), ("num_vertices", c_i32), ("num_tris", c_i32),
+                ("num_bvs", c_i32), ("vertices", P_dbl), ("tris", P_i32),
+                ("bvs", ctypes.POINTER(Node)), ("primitive_indices", ctypes.POINTER(ctypes.c_uint))]
+
+
+class CollideStats(ctypes.Structure):
+    _fields_ = [("num_contacts", c_i32), ("num_bv_tests", c_i64), ("num_leaf_tests", c_i64)]
+
+
+class DistanceOut(ctypes.Structure):
+    _fields_ = [("min_distance", c_dbl), ("p1", c_dbl * 3), ("p2", c_dbl * 3), ("b1", c_i32),
+                ("b2", c_i32), ("num_bv_tests", c_i64), ("num_leaf_tests", c_i64)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        PM = ctypes.POINTER(Model)
+        sig = {
+            "fclo_dot": (c_dbl, [P_dbl, P_dbl]),
+            "fclo_eigen_old": (None, [P_dbl, P_dbl, P_dbl]),
+            "fclo_relative_transform": (None, [P_dbl, P_dbl, P_dbl, P_dbl]),
+            "fclo_inverse_compose": (None, [P_dbl, P_dbl, P_dbl]),
+            "fclo_model_build": (c_i32, [PM, c_i32, P_dbl, c_i32, P_i32, c_i32]),
+            "fclo_model_free": (None, [PM]),
+            "fclo_model_depth": (c_i32, [PM]),
+            "fclo_node_size": (c_dbl, [PM, c_i32]),
+            "fclo_obb_disjoint": (c_i32, [P_dbl, P_dbl, P_dbl, P_dbl]),
+            "fclo_intersect_triangle": (c_i32, [P_dbl] * 8),
+            "fclo_rect_distance": (c_dbl, [P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
+            "fclo_rss_distance": (c_dbl, [P_dbl, P_dbl, ctypes.POINTER(Node),
+                                          ctypes.POINTER(Node)]),
+            "fclo_tri_distance": (c_dbl, [P_dbl, P_dbl, P_dbl, P_dbl]),
+            "fclo_collide": (None, [PM, P_dbl, PM, P_dbl, c_i32, P_i32,
+                                    ctypes.POINTER(CollideStats)]),
+            "fclo_distance": (None, [PM, P_dbl, PM, P_dbl, c_i32, ctypes.POINTER(DistanceOut)]),
+            "fclo_collide_all_pairs": (c_i64, [PM, P_dbl, PM, P_dbl, P_i32, c_i64]),
+            "fclo_distance_all_pairs": (c_dbl, [PM, P_dbl, PM, P_dbl, P_i32]),
+            "fclo_collide_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl, P_dbl,
+                                          c_i32, c_i32, P_i32, P_i32, P_i64, P_i64, c_i32]),
+            "fclo_distance_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
+                                           P_dbl, c_i32, P_dbl, P_dbl, P_i32, P_i64, P_i64,
+                                           c_i32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+class OracleMesh:
+    """A BVHModel built by the oracle's restatement of the reference builder."""
+
+    def __init__(self, V, T, bv_type=OBBRSS):
+        self.V = _d(V)
+        self.T = np.ascontiguousarray(T, np.int32)
+        self.m = Model()
+        rc = lib().fclo_model_build(ctypes.byref(self.m), bv_type, _p(self.V, c_dbl),
+                                    self.V.shape[0], _p(self.T, c_i32), self.T.shape[0])
+        if rc != 0:
+            raise ValueError("empty model")
+
+    def __del__(self):
+        try:
+            lib().fclo_model_free(ctypes.byref(self.m))
+        except Exception:
+            pass
+
+    @property
+    def num_bvs(self):
+        return self.m.num_bvs
+
+    def depth(self):
+        return lib().fclo_model_depth(ctypes.byref(self.m))
+
+    def nodes(self):
+        """(bv[nodes,31], links[nodes,3], sizes[nodes]) in the layout of
+        fclx_mesh_export_nodes."""
+        n = self.m.num_bvs
+        raw = np.ctypeslib.as_array(ctypes.cast(self.m.bvs, ctypes.POINTER(c_dbl)),
+                                    shape=(n * ctypes.sizeof(Node) // 8,))
+        rec = raw.reshape(n, ctypes.sizeof(Node) // 8)
+        bv = np.zeros((n, 31), np.float64)
+        bv[:, :30] = rec[:, :30]
+        ints = np.ctypeslib.as_array(ctypes.cast(self.m.bvs, P_i32),
+                                     shape=(n * ctypes.sizeof(Node) // 4,))
+        ints = ints.reshape(n, ctypes.sizeof(Node) // 4)
+        links = ints[:, 60:63].copy()
+        sizes = np.array([lib().fclo_node_size(ctypes.byref(self.m), i) for i in range(n)])
+        return bv.copy(), links, sizes
+
+
+def collide(m1, tf1, m2, tf2, cap=1):
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    pairs = np.zeros((max(cap, 1), 2), np.int32)
+    st = CollideStats()
+    lib().fclo_collide(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m), _p(tf2, c_dbl),
+                       cap, _p(pairs, c_i32), ctypes.byref(st))
+    return pairs[: st.num_contacts].copy(), (st.num_bv_tests, st.num_leaf_tests)
+
+
+def distance(m1, tf1, m2, tf2, nearest=True):
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    o = DistanceOut()
+    lib().fclo_distance(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m),
+                        _p(tf2, c_dbl), 1 if nearest else 0, ctypes.byref(o))
+    pts = np.array([list(o.p1), list(o.p2)])
+    return o.min_distance, pts, (o.b1, o.b2), (o.num_bv_tests, o.num_leaf_tests)
+
+
+def collide_all_pairs(m1, tf1, m2, tf2, store=1 << 20):
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    pairs = np.zeros((store, 2), np.int32)
+    n = lib().fclo_collide_all_pairs(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m),
+                                     _p(tf2, c_dbl), _p(pairs, c_i32), store)
+    if n > store:
+        raise ValueError("more intersecting pairs than store")
+    return pairs[:n].copy()
+
+
+def distance_all_pairs(m1, tf1, m2, tf2):
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    ids = np.zeros(2, np.int32)
+    d = lib().fclo_distance_all_pairs(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m),
+                                      _p(tf2, c_dbl), _p(ids, c_i32))
+    return d, (int(ids[0]), int(ids[1]))
+
+
+def _model_array(meshes):
+    PM = ctypes.POINTER(Model)
+    arr = (PM * len(meshes))(*[ctypes.pointer(m.m) for m in meshes])
+    return arr
+
+
+def collide_batch(meshes, qm, tf1, tf2, cap, store=0, threads=None):
+    """Batch collide over queries; qm (n,2) indexes meshes.  Returns counts,
+    pairs (n, store, 2), bv_tests, leaf_tests."""
+    qm = np.ascontiguousarray(qm, np.int32)
+    n = qm.shape[0]
+    q1 = np.ascontiguousarray(qm[:, 0])
+    q2 = np.ascontiguousarray(qm[:, 1])
+    tf1 = _d(tf1)
+    tf2 = _d(tf2)
+    counts = np.zeros(n, np.int32)
+    pairs = np.zeros((n, max(store, 1), 2), np.int32)
+    bvt = np.zeros(n, np.int64)
+    lft = np.zeros(n, np.int64)
+    arr = _model_array(meshes)
+    lib().fclo_collide_batch(arr, n, _p(q1, c_i32), _p(q2, c_i32), _p(tf1, c_dbl),
+                             _p(tf2, c_dbl), cap, store, _p(counts, c_i32), _p(pairs, c_i32),
+                             _p(bvt, c_i64), _p(lft, c_i64), threads or os.cpu_count())
+    return counts, pairs[:, :store], bvt, lft
+
+
+def distance_batch(meshes, qm, tf1, tf2, nearest=True, threads=None):
+    qm = np.ascontiguousarray(qm, np.int32)
+    n = qm.shape[0]
+    q1 = np.ascontiguousarray(qm[:, 0])
+    q2 = np.ascontiguousarray(qm[:, 1])
+    tf1 = _d(tf1)
+    tf2 = _d(tf2)
+    dist = np.zeros(n)
+    pts = np.zeros((n, 6))
+    ids = np.zeros((n, 2), np.int32)
+    bvt = np.zeros(n, np.int64)
+    lft = np.zeros(n, np.int64)
+    arr = _model_array(meshes)
+    lib().fclo_distance_batch(arr, n, _p(q1, c_i32), _p(q2, c_i32), _p(tf1, c_dbl),
+                              _p(tf2, c_dbl), 1 if nearest else 0, _p(dist, c_dbl),
+                              _p(pts, c_dbl), _p(ids, c_i32), _p(bvt, c_i64), _p(lft, c_i64),
+                              threads or os.cpu_count())
+    return dist, pts, ids, bvt, lft
+
+
+# primitive tests
+def obb_disjoint(B, T, a, b):
+    return lib().fclo_obb_disjoint(_p(_d(B), c_dbl), _p(_d(T), c_dbl), _p(_d(a), c_dbl),
+                                   _p(_d(b), c_dbl))
+
+
+def intersect_triangle(P, Q, R=None, T=None):
+    """P, Q: (3,3) vertex arrays; (R,T) moves Q into P's frame (identity by default)."""
+    P = _d(P)
+    Q = _d(Q)
+    R = _d(np.eye(3) if R is None else R)
+    T = _d(np.zeros(3) if T is None else T)
+    args = [_p(np.ascontiguousarray(P[i]), c_dbl) for i in range(3)]
+    qs = [np.ascontiguousarray(Q[i]) for i in range(3)]
+    args += [_p(q, c_dbl) for q in qs] + [_p(R, c_dbl), _p(T, c_dbl)]
+    return lib().fclo_intersect_triangle(*args)
+
+
+def rect_distance(Rab, Tab, a, b):
+    P = np.zeros(3)
+    Q = np.zeros(3)
+    d = lib().fclo_rect_distance(_p(_d(Rab), c_dbl), _p(_d(Tab), c_dbl), _p(_d(a), c_dbl),
+                                 _p(_d(b), c_dbl), _p(P, c_dbl), _p(Q, c_dbl))
+    return d, P, Q
+
+
+def rss_distance_nodes(axis1, to1, l1, r1, axis2, to2, l2, r2, R0=None, T0=None):
+    """RSS-RSS distance (RSS-inl.h:1956-1974) for hand-built rectangles."""
+    n1, n2 = Node(), Node()
+    for n, ax, to, l, r in ((n1, axis1, to1, l1, r1), (n2, axis2, to2, l2, r2)):
+        n.rss_axis[:] = list(_d(ax).reshape(9))
+        n.rss_to[:] = list(_d(to))
+        n.rss_l[:] = list(_d(l))
+        n.rss_r = float(r)
+    R0 = _d(np.eye(3) if R0 is None else R0)
+    T0 = _d(np.zeros(3) if T0 is None else T0)
+    return lib().fclo_rss_distance(_p(R0, c_dbl), _p(T0, c_dbl), ctypes.byref(n1),
+                                   ctypes.byref(n2))
+
+
+def tri_distance(S, T):
+    S = _d(S).reshape(9)
+    T = _d(T).reshape(9)
+    P = np.zeros(3)
+    Q = np.zeros(3)
+    d = lib().fclo_tri_distance(_p(S, c_dbl), _p(T, c_dbl), _p(P, c_dbl), _p(Q, c_dbl))
+    return d, P, Q
+
+
+def eigen_old(M):
+    M = _d(M).reshape(9)
+    d = np.zeros(3)
+    v = np.zeros(9)
+    lib().fclo_eigen_old(_p(M, c_dbl), _p(d, c_dbl), _p(v, c_dbl))
+    return d, v.reshape(3, 3)

@@ -1,0 +1,188 @@
+"""HIP engine vs the CPU oracle on the same seeded inputs, through the C ABI.
+
+Bar: collision counts, contact pairs (in the reference's order) and the
+traversal statistics num_bv_tests / num_leaf_tests are bit-identical;
+distances, nearest points and triangle ids are bit-identical too (the FP64
+device path uses the oracle's evaluation order with no fused multiply-add).
+"""
+import numpy as np
+import pytest
+import torch
+
+import fcl_amd as F
+import fclo
+import util
+from fcl_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV, dtype)
+
+
+def collide_gpu(ctx, qm, tf1, tf2, cap, engine=F.ENGINE_PLAIN, capacity=None):
+    n = qm.shape[0]
+    capacity = capacity if capacity is not None else max(1, n * min(cap, 64))
+    d_qm, d_tf1, d_tf2 = _t(qm, torch.int32), _t(tf1, torch.float64), _t(tf2, torch.float64)
+    counts = torch.zeros(n, dtype=torch.int32, device=DEV)
+    pairs = torch.zeros((capacity, 3), dtype=torch.int32, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    bvt = torch.zeros(n, dtype=torch.int64, device=DEV)
+    lft = torch.zeros(n, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.collide_batch_async(d_qm, d_tf1, d_tf2, cap, counts, pairs, num, engine=engine,
+                            bv_tests=bvt, leaf_tests=lft, stream=s)
+    rc = ctx.sync_status(s)
+    nrec = int(num.item())
+    rec = pairs[: min(nrec, capacity)].cpu().numpy()
+    # per-query lists in production order (records of one query keep order)
+    order = np.argsort(rec[:, 0], kind="stable")
+    rec = rec[order]
+    per_q = {}
+    if len(rec):
+        qs, starts = np.unique(rec[:, 0], return_index=True)
+        ends = list(starts[1:]) + [len(rec)]
+        for q, a, b in zip(qs, starts, ends):
+            per_q[int(q)] = rec[a:b, 1:]
+    return rc, counts.cpu().numpy(), per_q, bvt.cpu().numpy(), lft.cpu().numpy(), nrec
+
+
+@pytest.fixture(scope="module")
+def c0_meshes(ctx):
+    """The plain mesh-vs-mesh collide workload at reduced query count:
+    two perturbed icospheres, 2,420 triangles each."""
+    (V1, T1), (V2, T2) = F.icosphere(11, 0.15, 0xA0), F.icosphere(11, 0.15, 0xB0)
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    return o, g
+
+
+def test_first_contact_parity(ctx, c0_meshes):
+    o, g = c0_meshes
+    n = 3000
+    tf1, tf2 = F.poses(n, 0, 2.2, 0, 0xC0)
+    oc, op, obv, olf = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 1, store=1)
+    print("case: first contact, 2x2420 tris, n=%d" % n, flush=True)
+    rc, gc, gp, gbv, glf, _ = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 1)
+    assert rc == 0
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gbv, obv) and np.array_equal(glf, olf)
+    for q in range(n):
+        if oc[q]:
+            assert np.array_equal(gp[q], op[q, :1])
+    assert 0.3 < (oc > 0).mean() < 0.7
+
+
+@pytest.mark.parametrize("cap", [3, 100000])
+def test_exhaustive_and_capped_parity(ctx, cap):
+    (V1, T1), (V2, T2) = F.icosphere(7, 0.15, 11), F.heightfield(30, 0.05, 12)
+    V2 = F.scale_to_radius(V2, 1.0)
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    n = 600
+    tf1, tf2 = F.poses(n, 0, 1.8, 0, 0xC1)
+    store = min(cap, 4000)
+    oc, op, obv, olf = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, cap, store=store)
+    print("case: exhaustive cap=%d n=%d" % (cap, n), flush=True)
+    rc, gc, gp, gbv, glf, nrec = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, cap,
+                                             capacity=int(oc.sum()) + 16)
+    assert rc == 0 and nrec == oc.sum()
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gbv, obv) and np.array_equal(glf, olf)
+    for q in range(n):
+        if oc[q]:
+            assert np.array_equal(gp[q][:store], op[q, : min(oc[q], store)])
+    assert oc.max() > 10
+
+
+def test_env_rob_exhaustive(ctx):
+    """The reference's mesh_mesh fixture (env.obj vs rob.obj) with its pose
+    distribution: GPU contact sets equal the oracle's and brute force."""
+    (V1, T1), (V2, T2) = util.load_obj("env.obj"), util.load_obj("rob.obj")
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    n = 10
+    tf1 = util.random_transforms(n, [-3000, -3000, 0, 3000, 3000, 3000], 2024)
+    tf2 = np.tile(util.identity_pose(), (n, 1))
+    print("case: env/rob exhaustive", flush=True)
+    rc, gc, gp, _, _, _ = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 1 << 30,
+                                      capacity=1 << 20)
+    assert rc == 0
+    for q in range(n):
+        bf = fclo.collide_all_pairs(o[0], tf1[q], o[1], tf2[q])
+        got = gp.get(q, np.zeros((0, 2), np.int32))
+        key = lambda p: sorted(map(tuple, p))
+        assert key(got) == key(bf)
+
+
+def test_single_query_entry_point(ctx, c0_meshes):
+    o, g = c0_meshes
+    tf1, tf2 = F.poses(40, 0, 1.5, 0, 0xC2)
+    for q in range(40):
+        for cap in (1, 4):
+            gp, gst = ctx.collide(g[0], tf1[q], g[1], tf2[q], cap=cap)
+            op, ost = fclo.collide(o[0], tf1[q], o[1], tf2[q], cap=cap)
+            assert np.array_equal(gp, op) and tuple(gst) == tuple(ost)
+
+
+@pytest.mark.parametrize("bv_type", [F.RSS, F.OBBRSS])
+def test_distance_parity(ctx, bv_type):
+    """Mesh distance at reduced size (the distance workload uses 20,480-tri
+    spheres; here 3,920): every output bit-identical to the oracle."""
+    (V1, T1), (V2, T2) = F.icosphere(14, 0.15, 0xD0), F.icosphere(14, 0.15, 0xD1)
+    o = [fclo.OracleMesh(V1, T1, bv_type), fclo.OracleMesh(V2, T2, bv_type)]
+    g = [ctx.mesh(V1, T1, bv_type), ctx.mesh(V2, T2, bv_type)]
+    n = 400
+    tf1, tf2 = F.poses(n, 1, 2.1, 4.0, 0xD2)
+    od, opts, oids, obv, olf = fclo.distance_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2)
+    print("case: distance bv=%d n=%d" % (bv_type, n), flush=True)
+    qm = np.tile(g, (n, 1))
+    dist = torch.zeros(n, dtype=torch.float64, device=DEV)
+    pts = torch.zeros((n, 6), dtype=torch.float64, device=DEV)
+    ids = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
+    bvt = torch.zeros(n, dtype=torch.int64, device=DEV)
+    lft = torch.zeros(n, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.distance_batch_async(_t(qm, torch.int32), _t(tf1, torch.float64), _t(tf2, torch.float64),
+                             dist, pts, ids, bv_tests=bvt, leaf_tests=lft, stream=s)
+    assert ctx.sync_status(s) == 0
+    gd = dist.cpu().numpy()
+    assert np.array_equal(gd.view(np.uint64), od.view(np.uint64))
+    assert np.array_equal(ids.cpu().numpy(), oids)
+    assert np.array_equal(pts.cpu().numpy().view(np.uint64), opts.view(np.uint64))
+    assert np.array_equal(bvt.cpu().numpy(), obv) and np.array_equal(lft.cpu().numpy(), olf)
+    # single-query entry point
+    for q in range(0, n, 50):
+        d, p, i, st = ctx.distance(g[0], tf1[q], g[1], tf2[q])
+        assert d == od[q] and tuple(i) == tuple(oids[q]) and tuple(st) == (obv[q], olf[q])
+
+
+def test_bad_queries_are_reported(ctx, c0_meshes):
+    o, g = c0_meshes
+    tf1, tf2 = F.poses(4, 0, 2.2, 0, 1)
+    qm = np.array([[g[0], g[1]], [g[0], 999], [-1, g[1]], [g[0], g[1]]], np.int32)
+    rc, gc, _, _, _, _ = collide_gpu(ctx, qm, tf1, tf2, 1)
+    assert rc == N.FCLX_ERR_INVALID_ARGUMENT
+    assert gc[1] == 0 and gc[2] == 0
+    # the status was cleared: a good batch is clean again
+    rc, _, _, _, _, _ = collide_gpu(ctx, np.tile(g, (4, 1)), tf1, tf2, 1)
+    assert rc == 0
+
+
+def test_pair_capacity_overflow_reported(ctx, c0_meshes):
+    o, g = c0_meshes
+    n = 200
+    tf1, tf2 = F.poses(n, 0, 0.5, 0, 2)
+    oc, _, _, _ = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 100000)
+    rc, gc, _, _, _, nrec = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 100000, capacity=10)
+    assert rc == N.FCLX_ERR_CAPACITY
+    assert np.array_equal(gc, oc) and nrec == oc.sum()
+
+
+def test_empty_model_rejected(ctx):
+    with pytest.raises(F.FclxError) as e:
+        ctx.mesh(np.zeros((0, 3)), np.zeros((0, 3), np.int32))
+    assert e.value.code == N.FCLX_ERR_EMPTY_MODEL
