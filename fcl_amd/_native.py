@@ -85,6 +85,19 @@ SIGNATURES = {
 _lib = None
 
 
+def _one_hip_runtime():
+    """torch ships its own HIP runtime with the same soname as the system's
+    (libamdhip64.so.7) but links it by file name.  Loading torch first makes
+    libfclx.so bind to that already-loaded runtime, so the process holds one
+    HIP runtime and torch streams/pointers are valid in both."""
+    if os.environ.get("FCLX_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded libfclx.so with argtypes set; raises if it was not built."""
     global _lib
@@ -92,6 +105,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise NativeLibraryMissing(
                 "libfclx.so not found at %s; build it with __graft_entry__.build()" % LIB_PATH)
+        _one_hip_runtime()
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

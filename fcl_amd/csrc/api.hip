@@ -123,15 +123,13 @@ int max_depth(const fclx_context* ctx) {
   return d;
 }
 
-// Largest |value| rounding error of a double -> float conversion over the
-// OBB coordinates of a mesh: half a float ulp of the largest magnitude.
-float obb32_error(const std::vector<HostNode>& nodes) {
+// Largest L1 norm of an OBB centre over the mesh's nodes; the FP32 cull's
+// translation error term scales with it.
+double centre_bound(const std::vector<HostNode>& nodes) {
   double mx = 0;
-  for (const auto& n : nodes) {
-    for (double v : n.obb_to) mx = std::max(mx, std::fabs(v));
-    for (double v : n.obb_extent) mx = std::max(mx, std::fabs(v));
-  }
-  return static_cast<float>(std::ldexp(std::max(mx, 1.0), -24));
+  for (const auto& n : nodes)
+    mx = std::max(mx, std::fabs(n.obb_to[0]) + std::fabs(n.obb_to[1]) + std::fabs(n.obb_to[2]));
+  return mx;
 }
 
 }  // namespace
@@ -273,7 +271,8 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   d.num_verts = num_vertices;
   d.depth = s.bvh.depth;
   d.bv_type = bv_type;
-  d.obb32_err = obb32_error(nodes);
+  d.pad = 0;
+  d.cmax = centre_bound(nodes);
   int id = -1;
   for (size_t i = 0; i < ctx->meshes.size(); ++i)
     if (!ctx->meshes[i].live) {

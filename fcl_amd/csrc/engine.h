@@ -8,7 +8,23 @@ struct EngineWorkspace;
 
 EngineWorkspace* engine_workspace_create();
 void engine_workspace_destroy(EngineWorkspace* ws);
+
+// Collide batch with the fast engine.  num_max_contacts <= kOrderedCapMax
+// runs the per-lane ordered traversal with FP32 culling; larger caps run the
+// breadth-first front expansion + persistent wavefront DFS, and re-run any
+// query whose contact count reaches the cap with the plain kernel so that
+// its truncated list is the reference's depth-first prefix.
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                           hipStream_t s);
+
+// Tunables of the exhaustive path (defaults chosen by measurement).
+struct EngineTuning {
+  int max_levels = 24;          // breadth-first levels before the DFS phase
+  long long front_target = 1 << 22;  // stop expanding once the front is this large
+  int dfs_blocks_per_cu = 8;    // persistent grid: blocks of 4 wavefronts
+};
+EngineTuning& engine_tuning();
+
+constexpr int kOrderedCapMax = 64;
 
 }  // namespace fclx

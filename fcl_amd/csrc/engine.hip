@@ -1,17 +1,819 @@
-// Wavefront-cooperative traversal engine (FCLX_ENGINE_FAST).
+// Fast collide engine.
+//
+// Exhaustive contacts (num_max_contacts > kOrderedCapMax):
+//   1. setup: per-query FP64 and FP32 relative transforms, margins, and the
+//      root BVTT task (q, 0, 0) of every valid query;
+//   2. breadth-first front expansion: one lane per BVTT task, FP32
+//      conservative OBB cull (FP64 only in the uncertain band), children
+//      appended with one atomic per wavefront (ballot + popcount prefix);
+//      leaf-pair candidates go to a separate buffer.  This splits a large
+//      query over many wavefronts before the depth-first phase starts;
+//   3. leaf kernel: FP64 triangle SAT over the candidates of phase 2;
+//   4. persistent depth-first phase: every wavefront owns a LIFO task stack
+//      in HBM; each step pops up to 64 tasks (one per lane), tests them, and
+//      pushes the children of overlapping internal pairs with one prefix sum.
+//      Leaf candidates collect in an LDS queue and run 64 at a time, so the
+//      FP64 triangle tests execute with the wavefront converged.  Idle
+//      wavefronts refill from the front of phase 2;
+//   5. any query whose contact count reached the cap is re-run by the plain
+//      depth-first kernel and its records replaced (the reference stops at
+//      the cap-th contact in depth-first order).
+// Without early stop the set of BVTT nodes visited is order-independent, so
+// the contact set and the per-query test counts equal the reference's.
+//
+// Small caps (<= kOrderedCapMax, e.g. first contact): per-lane depth-first
+// traversal in the reference's order with FP32 culling, lanes refilling with
+// new queries as they finish.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "device/cull32.h"
+#include "device/exact64.h"
 #include "engine.h"
 
 namespace fclx {
 
-struct EngineWorkspace {
-  int unused;
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kLeafQ = 128;
+
+struct QueryXf64 {
+  double R[9];
+  double T[3];
 };
 
-EngineWorkspace* engine_workspace_create() { return new EngineWorkspace{}; }
-void engine_workspace_destroy(EngineWorkspace* ws) { delete ws; }
+// task = (query:22 | b1:21 | b2:21)
+__device__ __forceinline__ unsigned long long pack_task(int q, int b1, int b2) {
+  return (static_cast<unsigned long long>(q) << 42) |
+         (static_cast<unsigned long long>(b1) << 21) | static_cast<unsigned long long>(b2);
+}
+__device__ __forceinline__ void unpack_task(unsigned long long t, int& q, int& b1, int& b2) {
+  q = static_cast<int>(t >> 42);
+  b1 = static_cast<int>((t >> 21) & 0x1FFFFF);
+  b2 = static_cast<int>(t & 0x1FFFFF);
+}
 
-hipError_t engine_collide(EngineWorkspace*, const CollideLaunch&, int, hipStream_t) {
-  return hipErrorNotSupported;
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long lanes_below() {
+  return (1ULL << lane_id()) - 1ULL;
+}
+
+// One atomic per wavefront: returns this lane's slot among the lanes with
+// pred set (slot values base .. base+popcount-1, in lane order).
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* counter,
+                                                           bool pred, int per_lane) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned long long base = 0;
+  if (lane_id() == leader)
+    base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)) * per_lane);
+  base = __shfl(base, leader);
+  return base + static_cast<unsigned long long>(__popcll(mask & lanes_below())) * per_lane;
+}
+
+__device__ __forceinline__ bool query_ok(const MeshDev* meshes, int n, int i1, int i2) {
+  if (i1 < 0 || i2 < 0 || i1 >= n || i2 >= n) return false;
+  const MeshDev& a = meshes[i1];
+  const MeshDev& b = meshes[i2];
+  return a.num_bvs > 0 && b.num_bvs > 0 && a.bv_type == 0 && b.bv_type == 0;
+}
+
+__device__ __forceinline__ V3 vtx(const MeshDev& m, int i) {
+  const double* p = m.verts + 3 * static_cast<size_t>(i);
+  return v3(p[0], p[1], p[2]);
+}
+
+// R = R1^T R2, T = R1^T (t2 - t1)
+__device__ __forceinline__ void rel_pose64(const double* tf1, const double* tf2, double* R,
+                                           double* T) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      R[3 * i + j] = (tf1[i] * tf2[j] + tf1[3 + i] * tf2[3 + j]) + tf1[6 + i] * tf2[6 + j];
+  const V3 d = v3(tf2[9] - tf1[9], tf2[10] - tf1[10], tf2[11] - tf1[11]);
+  T[0] = dot(col(tf1, 0), d);
+  T[1] = dot(col(tf1, 1), d);
+  T[2] = dot(col(tf1, 2), d);
+}
+
+__device__ __forceinline__ void make_xf32(const double* R, const double* T, double cmax1,
+                                          double cmax2, QueryXf32& X) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) X.R[i] = static_cast<float>(R[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) X.T[i] = static_cast<float>(T[i]);
+  const double st = cmax1 + (fabs(T[0]) + fabs(T[1]) + fabs(T[2])) + cmax2;
+  // margin rounded up: the float conversion may round down by <= 2^-24
+  X.ms = static_cast<float>(static_cast<double>(kCullKS) * static_cast<double>(kCullU) * st *
+                            (1.0 + 1.0 / 1048576.0));
+}
+
+struct EngineArgs {
+  CollideLaunch L;
+  QueryXf32* qx32;
+  QueryXf64* qx64;
+  int32_t* qflag;
+  unsigned long long* nflag;
+  unsigned long long* front_in;
+  unsigned long long* front_out;
+  unsigned long long* n_out;
+  unsigned long long* leafc;
+  unsigned long long* n_leafc;
+  long long n_in;
+  unsigned long long* head;       // DFS refill cursor
+  unsigned long long* stacks;
+  int stack_cap;
+};
+
+// ---------------------------------------------------------------- setup
+__global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
+  const CollideLaunch& L = A.L;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = false;
+  if (q < L.num_queries) {
+    const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
+    valid = query_ok(L.meshes, L.num_meshes, i1, i2);
+    L.counts[q] = 0;
+    A.qflag[q] = 0;
+    if (L.bv_tests) L.bv_tests[q] = 0;
+    if (L.leaf_tests) L.leaf_tests[q] = 0;
+    QueryXf32 X;
+    if (valid) {
+      QueryXf64 x;
+      rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q), x.R,
+                 x.T);
+      A.qx64[q] = x;
+      make_xf32(x.R, x.T, L.meshes[i1].cmax, L.meshes[i2].cmax, X);
+      X.m1 = i1;
+      X.m2 = i2;
+      X.valid = 1;
+    } else {
+      X = QueryXf32{};
+      X.valid = 0;
+      atomicOr(L.status, kStatusBadQuery);
+    }
+    A.qx32[q] = X;
+  }
+  const unsigned long long slot = wave_reserve(A.n_out, valid, 1);
+  if (valid) A.front_out[slot] = pack_task(q, 0, 0);
+}
+
+// BV test of task (q,b1,b2): FP32 cull, FP64 only when unsure.
+__device__ __forceinline__ bool bv_overlap(const EngineArgs& A, const QueryXf32& X, int q,
+                                           const MeshDev& m1, const MeshDev& m2, int b1,
+                                           int b2) {
+  const int r = obb_cull32(X, m1.obb32[b1], m2.obb32[b2]);
+  if (r != kCullUnsure) return r == kCullOverlap;
+  const QueryXf64& x = A.qx64[q];
+  M3 R;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
+  return obb_overlap64(R, v3(x.T[0], x.T[1], x.T[2]), m1.obb64[b1], m2.obb64[b2]);
+}
+
+// Record one intersecting pair; flags the query once its count reaches the cap.
+__device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q, int id1,
+                                            int id2) {
+  const CollideLaunch& L = A.L;
+  const unsigned long long slot = wave_reserve(reinterpret_cast<unsigned long long*>(L.num_pairs),
+                                               hit, 1);
+  if (!hit) return;
+  if (static_cast<long long>(slot) < L.pair_capacity) {
+    L.pairs[3 * slot] = q;
+    L.pairs[3 * slot + 1] = id1;
+    L.pairs[3 * slot + 2] = id2;
+  } else {
+    atomicOr(L.status, kStatusPairOverflow);
+  }
+  const int old = atomicAdd(&L.counts[q], 1);
+  if (old + 1 >= L.cap) {
+    if (atomicExch(&A.qflag[q], 1) == 0) atomicAdd(A.nflag, 1ULL);
+  }
+}
+
+__device__ __forceinline__ bool leaf_hit(const EngineArgs& A, unsigned long long task, int& q,
+                                         int& id1, int& id2) {
+  int b1, b2;
+  unpack_task(task, q, b1, b2);
+  const QueryXf32& X = A.qx32[q];
+  const MeshDev& m1 = A.L.meshes[X.m1];
+  const MeshDev& m2 = A.L.meshes[X.m2];
+  id1 = -(m1.obb64[b1].first_child + 1);
+  id2 = -(m2.obb64[b2].first_child + 1);
+  const int32_t* t1 = m1.tris + 3 * static_cast<size_t>(id1);
+  const int32_t* t2 = m2.tris + 3 * static_cast<size_t>(id2);
+  const QueryXf64& x = A.qx64[q];
+  M3 R;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
+  if (A.L.leaf_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.leaf_tests[q]), 1ULL);
+  return tri_intersect64(vtx(m1, t1[0]), vtx(m1, t1[1]), vtx(m1, t1[2]), vtx(m2, t2[0]),
+                         vtx(m2, t2[1]), vtx(m2, t2[2]), R, v3(x.T[0], x.T[1], x.T[2]));
+}
+
+// ------------------------------------------------------- front expansion
+__global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = i < A.n_in;
+  bool push = false, leafc = false;
+  unsigned long long c0 = 0, c1 = 0, task = 0;
+  if (active) {
+    task = A.front_in[i];
+    int q, b1, b2;
+    unpack_task(task, q, b1, b2);
+    const QueryXf32 X = A.qx32[q];
+    const MeshDev& m1 = A.L.meshes[X.m1];
+    const MeshDev& m2 = A.L.meshes[X.m2];
+    if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+    if (bv_overlap(A, X, q, m1, m2, b1, b2)) {
+      const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
+      if (fc1 < 0 && fc2 < 0) {
+        leafc = true;
+      } else {
+        push = true;
+        if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
+          c0 = pack_task(q, fc1, b2);
+          c1 = pack_task(q, fc1 + 1, b2);
+        } else {
+          c0 = pack_task(q, b1, fc2);
+          c1 = pack_task(q, b1, fc2 + 1);
+        }
+      }
+    }
+  }
+  const unsigned long long so = wave_reserve(A.n_out, push, 2);
+  if (push) {
+    A.front_out[so] = c0;
+    A.front_out[so + 1] = c1;
+  }
+  const unsigned long long sl = wave_reserve(A.n_leafc, leafc, 1);
+  if (leafc) A.leafc[sl] = task;
+}
+
+__global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool hit = false;
+  int q = 0, id1 = 0, id2 = 0;
+  if (i < A.n_in) hit = leaf_hit(A, A.leafc[i], q, id1, id2);
+  record_hits(A, hit, q, id1, id2);
+}
+
+// ----------------------------------------------- persistent depth-first
+__global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
+  __shared__ unsigned long long leafq_all[kWavesPerBlock][kLeafQ];
+  const int wave = threadIdx.x / kWave;
+  const int lane = lane_id();
+  unsigned long long* leafq = leafq_all[wave];
+  unsigned long long* stack =
+      A.stacks + (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave) * A.stack_cap;
+  int sp = 0;      // wavefront-uniform
+  int nleaf = 0;   // wavefront-uniform
+  bool input_left = true;
+
+  auto flush_leaves = [&](int k) {
+    bool hit = false;
+    int q = 0, id1 = 0, id2 = 0;
+    if (lane < k) hit = leaf_hit(A, leafq[nleaf - k + lane], q, id1, id2);
+    __builtin_amdgcn_wave_barrier();
+    nleaf -= k;
+    record_hits(A, hit, q, id1, id2);
+  };
+
+  while (true) {
+    if (sp < kWave && input_left) {
+      const int want = kWave - sp;
+      unsigned long long start = 0;
+      if (lane == 0) start = atomicAdd(A.head, static_cast<unsigned long long>(want));
+      start = __shfl(start, 0);
+      const long long avail = A.n_in - static_cast<long long>(start);
+      const int got = avail <= 0 ? 0 : static_cast<int>(avail < want ? avail : want);
+      if (got < want) input_left = false;
+      if (lane < got) stack[sp + lane] = A.front_in[start + lane];
+      sp += got;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (sp == 0) break;
+    const int n = sp < kWave ? sp : kWave;
+    sp -= n;
+    const bool active = lane < n;
+    unsigned long long task = 0;
+    if (active) task = stack[sp + lane];
+    bool push = false, leafc = false;
+    unsigned long long c0 = 0, c1 = 0;
+    if (active) {
+      int q, b1, b2;
+      unpack_task(task, q, b1, b2);
+      const QueryXf32 X = A.qx32[q];
+      const MeshDev& m1 = A.L.meshes[X.m1];
+      const MeshDev& m2 = A.L.meshes[X.m2];
+      if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+      if (bv_overlap(A, X, q, m1, m2, b1, b2)) {
+        const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
+        if (fc1 < 0 && fc2 < 0) {
+          leafc = true;
+        } else {
+          push = true;
+          if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
+            c0 = pack_task(q, fc1, b2);
+            c1 = pack_task(q, fc1 + 1, b2);
+          } else {
+            c0 = pack_task(q, b1, fc2);
+            c1 = pack_task(q, b1, fc2 + 1);
+          }
+        }
+      }
+    }
+    // children onto the stack (one prefix sum)
+    const unsigned long long pm = __ballot(push);
+    const int npush = 2 * __popcll(pm);
+    if (sp + npush > A.stack_cap) {
+      if (lane == 0) atomicOr(A.L.status, kStatusQueueOverflow);
+      break;  // never write past the stack; the overflow is reported
+    }
+    if (push) {
+      const int o = sp + 2 * __popcll(pm & lanes_below());
+      stack[o] = c0;
+      stack[o + 1] = c1;
+    }
+    sp += npush;
+    // leaf candidates into the LDS queue, tested 64 at a time
+    const unsigned long long lm = __ballot(leafc);
+    const int nl = __popcll(lm);
+    if (nleaf + nl > kLeafQ) flush_leaves(kWave);
+    if (leafc) leafq[nleaf + __popcll(lm & lanes_below())] = task;
+    __builtin_amdgcn_wave_barrier();
+    nleaf += nl;
+    if (nleaf >= kWave) flush_leaves(kWave);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  while (nleaf > 0) flush_leaves(nleaf < kWave ? nleaf : kWave);
+}
+
+// ---------------------------------- fix-up of queries that reached the cap
+__global__ void engine_collect_flagged(const int32_t* qflag, int nq, int32_t* list,
+                                       unsigned long long* n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool f = q < nq && qflag[q] != 0;
+  const unsigned long long s = wave_reserve(n, f, 1);
+  if (f) list[s] = q;
+}
+
+// copy records of unflagged queries to out (unordered), count in n
+__global__ void engine_drop_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
+                                    int32_t* out, unsigned long long* n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool keep = i < nrec && qflag[rec[3 * i]] == 0;
+  const unsigned long long s = wave_reserve(n, keep, 1);
+  if (keep) {
+    out[3 * s] = rec[3 * i];
+    out[3 * s + 1] = rec[3 * i + 1];
+    out[3 * s + 2] = rec[3 * i + 2];
+  }
+}
+
+// gather the flagged queries' inputs into a compact batch
+__global__ void engine_gather_flagged(const int32_t* list, int n, const int32_t* qm,
+                                      const double* tf1, const double* tf2, int32_t* qm_o,
+                                      double* tf1_o, double* tf2_o) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int q = list[i];
+  qm_o[2 * i] = qm[2 * q];
+  qm_o[2 * i + 1] = qm[2 * q + 1];
+  for (int k = 0; k < 12; ++k) {
+    tf1_o[12 * i + k] = tf1[12 * static_cast<size_t>(q) + k];
+    tf2_o[12 * i + k] = tf2[12 * static_cast<size_t>(q) + k];
+  }
+}
+
+// scatter the re-run results back: counts/stats per query, records appended
+__global__ void engine_scatter_flagged(const int32_t* list, int n, const int32_t* counts_s,
+                                       const long long* bvt_s, const long long* lft_s,
+                                       int32_t* counts, long long* bvt, long long* lft) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int q = list[i];
+  counts[q] = counts_s[i];
+  if (bvt) bvt[q] = bvt_s[i];
+  if (lft) lft[q] = lft_s[i];
+}
+
+__global__ void engine_append_records(const int32_t* rec, long long nrec, const int32_t* list,
+                                      int32_t* out, long long out_base, long long cap,
+                                      int32_t* status) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const long long o = out_base + i;
+  if (o >= cap) {
+    atomicOr(status, kStatusPairOverflow);
+    return;
+  }
+  out[3 * o] = list[rec[3 * i]];
+  out[3 * o + 1] = rec[3 * i + 1];
+  out[3 * o + 2] = rec[3 * i + 2];
+}
+
+__global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// ---------------------------------------------- ordered, small caps
+template <int STACK>
+__global__ void __launch_bounds__(kBlock) ordered_collide(CollideLaunch L,
+                                                          unsigned long long* next) {
+  // per-lane state
+  int q = -1;
+  int count = 0;
+  long long bvt = 0, lft = 0;
+  int sp = 0, b1 = 0, b2 = 0;
+  bool overflow = false;
+  M3 R;
+  V3 T;
+  QueryXf32 X;
+  MeshDev m1{}, m2{};
+  int2 stack[STACK];
+  while (true) {
+    const bool need = q == -1;  // -2: the batch is exhausted for this lane
+    const unsigned long long nm = __ballot(need);
+    if (nm != 0) {
+      const int leader = __ffsll(static_cast<long long>(nm)) - 1;
+      unsigned long long base = 0;
+      if (lane_id() == leader) base = atomicAdd(next, static_cast<unsigned long long>(__popcll(nm)));
+      base = __shfl(base, leader);
+      if (need) {
+        const long long cand = static_cast<long long>(base + __popcll(nm & lanes_below()));
+        q = cand < L.num_queries ? static_cast<int>(cand) : -2;
+        if (q >= 0) {
+          const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
+          count = 0;
+          bvt = lft = 0;
+          sp = 0;
+          b1 = b2 = 0;
+          if (!query_ok(L.meshes, L.num_meshes, i1, i2)) {
+            L.counts[q] = 0;
+            if (L.bv_tests) L.bv_tests[q] = 0;
+            if (L.leaf_tests) L.leaf_tests[q] = 0;
+            atomicOr(L.status, kStatusBadQuery);
+            q = -1;  // fetch again next round
+          } else {
+            m1 = L.meshes[i1];
+            m2 = L.meshes[i2];
+            double Rd[9], Td[3];
+            rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q),
+                       Rd, Td);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) R.m[k] = Rd[k];
+            T = v3(Td[0], Td[1], Td[2]);
+            make_xf32(Rd, Td, m1.cmax, m2.cmax, X);
+          }
+        }
+      }
+    }
+    if (__ballot(q != -2) == 0) break;
+    if (q < 0) continue;
+    // one BVTT node visit, as in the plain kernel
+    const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
+    ++bvt;
+    bool descend = false;
+    int r = obb_cull32(X, m1.obb32[b1], m2.obb32[b2]);
+    if (r == kCullUnsure) r = obb_overlap64(R, T, m1.obb64[b1], m2.obb64[b2]) ? 1 : 0;
+    if (r == kCullOverlap) {
+      if (fc1 < 0 && fc2 < 0) {
+        ++lft;
+        const int id1 = -(fc1 + 1), id2 = -(fc2 + 1);
+        const int32_t* t1 = m1.tris + 3 * static_cast<size_t>(id1);
+        const int32_t* t2 = m2.tris + 3 * static_cast<size_t>(id2);
+        if (tri_intersect64(vtx(m1, t1[0]), vtx(m1, t1[1]), vtx(m1, t1[2]), vtx(m2, t2[0]),
+                            vtx(m2, t2[1]), vtx(m2, t2[2]), R, T)) {
+          if (count < L.cap) {
+            const long long slot = static_cast<long long>(
+                atomicAdd(reinterpret_cast<unsigned long long*>(L.num_pairs), 1ULL));
+            if (slot < L.pair_capacity) {
+              L.pairs[3 * slot] = q;
+              L.pairs[3 * slot + 1] = id1;
+              L.pairs[3 * slot + 2] = id2;
+            } else {
+              atomicOr(L.status, kStatusPairOverflow);
+            }
+            ++count;
+          }
+        }
+      } else {
+        descend = true;
+      }
+    }
+    bool done = false;
+    if (descend) {
+      if (sp >= STACK) {
+        overflow = true;
+        done = true;
+      } else if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
+        stack[sp++] = make_int2(fc1 + 1, b2);
+        b1 = fc1;
+      } else {
+        stack[sp++] = make_int2(b1, fc2 + 1);
+        b2 = fc2;
+      }
+    } else if (sp == 0 || (count > 0 && count >= L.cap)) {
+      done = true;
+    } else {
+      const int2 e = stack[--sp];
+      b1 = e.x;
+      b2 = e.y;
+    }
+    if (done) {
+      L.counts[q] = count;
+      if (L.bv_tests) L.bv_tests[q] = bvt;
+      if (L.leaf_tests) L.leaf_tests[q] = lft;
+      if (overflow) atomicOr(L.status, kStatusStackOverflow);
+      overflow = false;
+      q = -1;
+    }
+  }
+}
+
+int device_cus() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 256;
+    return n > 0 ? n : 256;
+  }();
+  return cus;
+}
+
+template <typename T>
+hipError_t grow(T*& p, size_t& cap, size_t need) {
+  if (need <= cap) return hipSuccess;
+  size_t ncap = std::max(need, cap * 2);
+  (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * ncap);
+  if (e == hipSuccess) cap = ncap;
+  return e;
+}
+
+#define ENG_TRY(x)                       \
+  do {                                   \
+    hipError_t e_ = (x);                 \
+    if (e_ != hipSuccess) return e_;     \
+  } while (0)
+
+}  // namespace
+
+struct EngineWorkspace {
+  QueryXf32* qx32 = nullptr;
+  size_t qx32_cap = 0;
+  QueryXf64* qx64 = nullptr;
+  size_t qx64_cap = 0;
+  int32_t* qflag = nullptr;
+  size_t qflag_cap = 0;
+  unsigned long long* front[2] = {nullptr, nullptr};
+  size_t front_cap[2] = {0, 0};
+  unsigned long long* leafc = nullptr;
+  size_t leafc_cap = 0;
+  unsigned long long* stacks = nullptr;
+  size_t stacks_cap = 0;
+  unsigned long long* counters = nullptr;  // 8 device counters
+  unsigned long long* host_counters = nullptr;
+};
+
+EngineTuning& engine_tuning() {
+  static EngineTuning t;
+  return t;
+}
+
+EngineWorkspace* engine_workspace_create() {
+  auto* ws = new EngineWorkspace();
+  if (hipMalloc(reinterpret_cast<void**>(&ws->counters), 8 * sizeof(unsigned long long)) !=
+          hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&ws->host_counters),
+                    8 * sizeof(unsigned long long)) != hipSuccess) {
+    engine_workspace_destroy(ws);
+    return nullptr;
+  }
+  return ws;
+}
+
+void engine_workspace_destroy(EngineWorkspace* ws) {
+  if (!ws) return;
+  (void)hipFree(ws->qx32);
+  (void)hipFree(ws->qx64);
+  (void)hipFree(ws->qflag);
+  (void)hipFree(ws->front[0]);
+  (void)hipFree(ws->front[1]);
+  (void)hipFree(ws->leafc);
+  (void)hipFree(ws->stacks);
+  (void)hipFree(ws->counters);
+  if (ws->host_counters) (void)hipHostFree(ws->host_counters);
+  delete ws;
+}
+
+namespace {
+
+hipError_t read_counter(EngineWorkspace* ws, int idx, unsigned long long& v, hipStream_t s) {
+  ENG_TRY(hipMemcpyAsync(ws->host_counters + idx, ws->counters + idx, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, s));
+  ENG_TRY(hipStreamSynchronize(s));
+  v = ws->host_counters[idx];
+  return hipSuccess;
+}
+
+inline unsigned grid_for(long long n) {
+  return static_cast<unsigned>((n + kBlock - 1) / kBlock);
+}
+
+// Re-run the flagged queries with the plain depth-first kernel.
+hipError_t fixup_flagged(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
+                         unsigned long long nflag, hipStream_t s) {
+  const int nq = L.num_queries;
+  int32_t* list = nullptr;
+  int32_t* qm = nullptr;
+  double* tf = nullptr;
+  int32_t* cnt = nullptr;
+  long long* st = nullptr;
+  int32_t* rec = nullptr;
+  int32_t* kept = nullptr;
+  long long* nrun = nullptr;
+  hipError_t e = hipSuccess;
+  const long long cap_rec = static_cast<long long>(nflag) * L.cap;
+  unsigned long long nkept = 0;
+  do {
+    if ((e = hipMalloc(&list, sizeof(int32_t) * nflag)) != hipSuccess) break;
+    if ((e = hipMalloc(&qm, sizeof(int32_t) * 2 * nflag)) != hipSuccess) break;
+    if ((e = hipMalloc(&tf, sizeof(double) * 24 * nflag)) != hipSuccess) break;
+    if ((e = hipMalloc(&cnt, sizeof(int32_t) * nflag)) != hipSuccess) break;
+    if ((e = hipMalloc(&st, sizeof(long long) * 2 * nflag)) != hipSuccess) break;
+    if ((e = hipMalloc(&rec, sizeof(int32_t) * 3 * std::max<long long>(cap_rec, 1))) != hipSuccess)
+      break;
+    if ((e = hipMalloc(&nrun, sizeof(long long))) != hipSuccess) break;
+    set_u64<<<1, 1, 0, s>>>(ws->counters + 5, 0);
+    engine_collect_flagged<<<grid_for(nq), kBlock, 0, s>>>(ws->qflag, nq, list, ws->counters + 5);
+    engine_gather_flagged<<<grid_for(nflag), kBlock, 0, s>>>(list, static_cast<int>(nflag),
+                                                             L.query_meshes, L.tf1, L.tf2, qm, tf,
+                                                             tf + 12 * nflag);
+    set_u64<<<1, 1, 0, s>>>(reinterpret_cast<unsigned long long*>(nrun), 0);
+    CollideLaunch P = L;
+    P.num_queries = static_cast<int32_t>(nflag);
+    P.query_meshes = qm;
+    P.tf1 = tf;
+    P.tf2 = tf + 12 * nflag;
+    P.counts = cnt;
+    P.pairs = rec;
+    P.pair_capacity = cap_rec;
+    P.num_pairs = nrun;
+    P.bv_tests = st;
+    P.leaf_tests = st + nflag;
+    if ((e = launch_collide_plain(P, stack_need, s)) != hipSuccess) break;
+  } while (false);
+  if (e == hipSuccess) {
+    // records produced by the fast engine
+    unsigned long long produced = 0;
+    e = hipMemcpyAsync(&produced, L.num_pairs, sizeof(produced), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const long long have = std::min<long long>(static_cast<long long>(produced), L.pair_capacity);
+    if (e == hipSuccess && have > 0 && L.pairs) {
+      e = hipMalloc(&kept, sizeof(int32_t) * 3 * have);
+      if (e == hipSuccess) {
+        set_u64<<<1, 1, 0, s>>>(ws->counters + 6, 0);
+        engine_drop_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, kept,
+                                                              ws->counters + 6);
+        e = read_counter(ws, 6, nkept, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(L.pairs, kept, sizeof(int32_t) * 3 * nkept, hipMemcpyDeviceToDevice,
+                             s);
+      }
+    }
+    unsigned long long nplain = 0;
+    if (e == hipSuccess) {
+      e = hipMemcpyAsync(&nplain, nrun, sizeof(nplain), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e == hipSuccess) {
+      engine_scatter_flagged<<<grid_for(nflag), kBlock, 0, s>>>(
+          list, static_cast<int>(nflag), cnt, st, st + nflag, L.counts, L.bv_tests,
+          L.leaf_tests);
+      const unsigned long long dropped = static_cast<unsigned long long>(have) - nkept;
+      const unsigned long long total = produced - dropped + nplain;
+      if (nplain > 0 && L.pairs)
+        engine_append_records<<<grid_for(nplain), kBlock, 0, s>>>(
+            rec, static_cast<long long>(nplain), list, L.pairs, static_cast<long long>(nkept),
+            L.pair_capacity, L.status);
+      set_u64<<<1, 1, 0, s>>>(reinterpret_cast<unsigned long long*>(L.num_pairs), total);
+      e = hipStreamSynchronize(s);
+    }
+  }
+  (void)hipFree(list);
+  (void)hipFree(qm);
+  (void)hipFree(tf);
+  (void)hipFree(cnt);
+  (void)hipFree(st);
+  (void)hipFree(rec);
+  (void)hipFree(kept);
+  (void)hipFree(nrun);
+  return e;
+}
+
+}  // namespace
+
+hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
+                          hipStream_t s) {
+  if (!ws) return hipErrorInvalidValue;
+  const int nq = L.num_queries;
+  if (nq == 0) return hipSuccess;
+  const int cus = device_cus();
+  if (L.cap <= kOrderedCapMax) {
+    set_u64<<<1, 1, 0, s>>>(ws->counters + 0, 0);
+    const unsigned grid = static_cast<unsigned>(
+        std::min<long long>(grid_for(nq), static_cast<long long>(cus) * 8));
+    if (stack_need <= 64)
+      ordered_collide<64><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
+    else if (stack_need <= 128)
+      ordered_collide<128><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
+    else
+      ordered_collide<256><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
+    return hipGetLastError();
+  }
+  if (nq >= (1 << 22)) return hipErrorInvalidValue;  // task packing limit
+  const EngineTuning& tune = engine_tuning();
+  ENG_TRY(grow(ws->qx32, ws->qx32_cap, nq));
+  ENG_TRY(grow(ws->qx64, ws->qx64_cap, nq));
+  ENG_TRY(grow(ws->qflag, ws->qflag_cap, nq));
+  ENG_TRY(grow(ws->front[0], ws->front_cap[0], std::max<size_t>(nq, 1024)));
+
+  EngineArgs A{};
+  A.L = L;
+  A.qx32 = ws->qx32;
+  A.qx64 = ws->qx64;
+  A.qflag = ws->qflag;
+  A.nflag = ws->counters + 4;
+  for (int i = 0; i < 8; ++i) set_u64<<<1, 1, 0, s>>>(ws->counters + i, 0);
+  // 1. setup -> front[0]
+  A.front_out = ws->front[0];
+  A.n_out = ws->counters + 0;
+  engine_setup<<<grid_for(nq), kBlock, 0, s>>>(A);
+  ENG_TRY(hipGetLastError());
+  unsigned long long nfront = 0;
+  ENG_TRY(read_counter(ws, 0, nfront, s));
+  // 2. breadth-first expansion while the front is small
+  int cur = 0;
+  unsigned long long nleafc_total = 0;
+  for (int level = 0; level < tune.max_levels && nfront > 0 &&
+                      static_cast<long long>(nfront) < tune.front_target;
+       ++level) {
+    const int nxt = cur ^ 1;
+    ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], 2 * nfront));
+    ENG_TRY(grow(ws->leafc, ws->leafc_cap, nleafc_total + nfront));
+    A.front_in = ws->front[cur];
+    A.n_in = static_cast<long long>(nfront);
+    A.front_out = ws->front[nxt];
+    A.n_out = ws->counters + 1;
+    A.leafc = ws->leafc;
+    A.n_leafc = ws->counters + 2;
+    set_u64<<<1, 1, 0, s>>>(ws->counters + 1, 0);
+    engine_expand<<<grid_for(nfront), kBlock, 0, s>>>(A);
+    ENG_TRY(hipGetLastError());
+    ENG_TRY(read_counter(ws, 1, nfront, s));
+    ENG_TRY(read_counter(ws, 2, nleafc_total, s));
+    cur = nxt;
+  }
+  // 3. leaf candidates of the breadth-first phase
+  if (nleafc_total > 0) {
+    A.leafc = ws->leafc;
+    A.n_in = static_cast<long long>(nleafc_total);
+    engine_leaves<<<grid_for(nleafc_total), kBlock, 0, s>>>(A);
+    ENG_TRY(hipGetLastError());
+  }
+  // 4. persistent depth-first phase over the remaining front
+  if (nfront > 0) {
+    const long long waves_needed = (static_cast<long long>(nfront) + kWave - 1) / kWave;
+    const long long max_blocks = static_cast<long long>(cus) * tune.dfs_blocks_per_cu;
+    const unsigned blocks = static_cast<unsigned>(
+        std::max<long long>(1, std::min<long long>(max_blocks, (waves_needed + 3) / 4)));
+    const int stack_cap = 2 * kWave * (stack_need + 2) + 2 * kWave;
+    ENG_TRY(grow(ws->stacks, ws->stacks_cap,
+                 static_cast<size_t>(blocks) * kWavesPerBlock * stack_cap));
+    A.front_in = ws->front[cur];
+    A.n_in = static_cast<long long>(nfront);
+    A.head = ws->counters + 3;
+    A.stacks = ws->stacks;
+    A.stack_cap = stack_cap;
+    set_u64<<<1, 1, 0, s>>>(ws->counters + 3, 0);
+    engine_dfs<<<blocks, kBlock, 0, s>>>(A);
+    ENG_TRY(hipGetLastError());
+  }
+  // 5. queries that reached the cap
+  unsigned long long nflag = 0;
+  ENG_TRY(read_counter(ws, 4, nflag, s));
+  if (nflag > 0) ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
+  return hipSuccess;
 }
 
 }  // namespace fclx

@@ -60,7 +60,8 @@ struct MeshDev {
   int32_t num_verts;
   int32_t depth;        // levels from root to deepest leaf (root = 1)
   int32_t bv_type;      // 0 = OBBRSS, 1 = RSS
-  float obb32_err;      // bound on |float - double| of any obb32 coordinate
+  int32_t pad;
+  double cmax;          // max over nodes of |OBB centre|_1 (bounds the FP32 cull error)
 };
 
 }  // namespace fclx

@@ -60,13 +60,14 @@ def c0_meshes(ctx):
     return o, g
 
 
-def test_first_contact_parity(ctx, c0_meshes):
+@pytest.mark.parametrize("engine", [F.ENGINE_PLAIN, F.ENGINE_FAST])
+def test_first_contact_parity(ctx, c0_meshes, engine):
     o, g = c0_meshes
     n = 3000
     tf1, tf2 = F.poses(n, 0, 2.2, 0, 0xC0)
     oc, op, obv, olf = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 1, store=1)
-    print("case: first contact, 2x2420 tris, n=%d" % n, flush=True)
-    rc, gc, gp, gbv, glf, _ = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 1)
+    print("case: first contact, 2x2420 tris, n=%d engine=%d" % (n, engine), flush=True)
+    rc, gc, gp, gbv, glf, _ = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 1, engine=engine)
     assert rc == 0
     assert np.array_equal(gc, oc)
     assert np.array_equal(gbv, obv) and np.array_equal(glf, olf)
@@ -95,7 +96,50 @@ def test_exhaustive_and_capped_parity(ctx, cap):
     for q in range(n):
         if oc[q]:
             assert np.array_equal(gp[q][:store], op[q, : min(oc[q], store)])
-    assert oc.max() > 10
+    assert oc.max() == cap if cap < 10 else oc.max() > 10
+
+
+def _set(p):
+    return sorted(map(tuple, np.asarray(p).reshape(-1, 2).tolist()))
+
+
+@pytest.mark.parametrize("cap", [100000, 70])
+def test_fast_engine_exhaustive(ctx, cap):
+    """FAST engine, large caps: per query the same contact set and the same
+    num_bv_tests / num_leaf_tests as the oracle; queries that reach the cap
+    (cap=70 here) come back as the reference's depth-first prefix."""
+    (V1, T1), (V2, T2) = F.icosphere(9, 0.15, 21), F.heightfield(24, 0.05, 22)
+    V2 = F.scale_to_radius(V2, 1.0)
+    (V3, T3) = F.icosphere(6, 0.15, 23)
+    meshes = [(V1, T1), (V2, T2), (V3, T3)]
+    o = [fclo.OracleMesh(V, T) for V, T in meshes]
+    g = [ctx.mesh(V, T) for V, T in meshes]
+    n = 900
+    rng = np.random.default_rng(31)
+    pairs = np.array([(0, 1), (1, 2), (0, 2), (2, 2)], np.int32)[rng.integers(0, 4, n)]
+    tf1, tf2 = F.poses(n, 0, 1.6, 0, 0xC3)
+    store = min(cap, 3000)
+    oc, op, obv, olf = fclo.collide_batch(o, pairs, tf1, tf2, cap, store=store)
+    gq = np.array(g, np.int32)[pairs]
+    print("case: fast exhaustive cap=%d n=%d" % (cap, n), flush=True)
+    rc, gc, gp, gbv, glf, nrec = collide_gpu(ctx, gq, tf1, tf2, cap, engine=F.ENGINE_FAST,
+                                             capacity=int(oc.sum()) * 2 + 64)
+    assert rc == 0
+    assert np.array_equal(gc, oc)
+    assert nrec == oc.sum()
+    assert np.array_equal(gbv, obv) and np.array_equal(glf, olf)
+    flagged = 0
+    for q in range(n):
+        got = gp.get(q, np.zeros((0, 2), np.int32))
+        want = op[q, : min(oc[q], store)]
+        if oc[q] >= cap:
+            flagged += 1
+            assert np.array_equal(got, want)  # depth-first prefix, in order
+        else:
+            assert _set(got) == _set(want)
+    if cap < 1000:
+        assert flagged > 0
+    assert oc.max() > 50
 
 
 def test_env_rob_exhaustive(ctx):
