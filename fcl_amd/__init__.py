@@ -171,7 +171,7 @@ class Context:
     def distance(self, m1, tf1, m2, tf2, nearest=True):
         tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
         tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
-        d = N.c_double()
+        d = N.c_dbl()
         pts = np.zeros(6, np.float64)
         ids = np.zeros(2, np.int32)
         stats = np.zeros(2, np.int64)
