@@ -217,6 +217,14 @@ class Context:
         a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
         self._check(self._L.fclx_distance_batch(self._h, ctypes.byref(a), stream))
 
+    def set_profiling(self, enable=True):
+        self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))
+
+    def engine_report(self):
+        r = N.EngineReport()
+        self._check(self._L.fclx_engine_last_report(self._h, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in N.EngineReport._fields_}
+
     def sync(self, stream=None):
         return self._check(self._L.fclx_sync(self._h, stream))
 

@@ -56,8 +56,18 @@ class DistanceBatchArgs(ctypes.Structure):
                 ("points", c_vp), ("ids", c_vp), ("bv_tests", c_vp), ("leaf_tests", c_vp)]
 
 
+class EngineReport(ctypes.Structure):
+    _fields_ = [("setup_ms", c_dbl), ("expand_ms", c_dbl), ("leaves_ms", c_dbl),
+                ("dfs_ms", c_dbl), ("fixup_ms", c_dbl), ("expand_bv_tests", c_i64),
+                ("expand_leaf_tests", c_i64), ("dfs_bv_tests", c_i64), ("dfs_leaf_tests", c_i64),
+                ("fp64_fallbacks", c_i64), ("front_size", c_i64), ("levels", c_i32),
+                ("flagged", c_i32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "fclx_set_profiling": (c_i32, [c_vp, c_i32]),
+    "fclx_engine_last_report": (c_i32, [c_vp, ctypes.POINTER(EngineReport)]),
     "fclx_init": (c_i32, [c_i32, ctypes.POINTER(c_vp)]),
     "fclx_shutdown": (None, [c_vp]),
     "fclx_last_error": (ctypes.c_char_p, [c_vp]),

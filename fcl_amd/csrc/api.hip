@@ -451,6 +451,33 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
   return FCLX_OK;
 }
 
+int fclx_set_profiling(fclx_context* ctx, int32_t enable) {
+  if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  hipError_t e = engine_set_profiling(ctx->ws, enable != 0);
+  if (e != hipSuccess) return hip_fail(ctx, e, "engine_set_profiling");
+  return FCLX_OK;
+}
+
+int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
+  if (!ctx || !out || !ctx->ws) return FCLX_ERR_INVALID_ARGUMENT;
+  const EngineReport& r = engine_report(ctx->ws);
+  out->setup_ms = r.setup_ms;
+  out->expand_ms = r.expand_ms;
+  out->leaves_ms = r.leaves_ms;
+  out->dfs_ms = r.dfs_ms;
+  out->fixup_ms = r.fixup_ms;
+  out->expand_bv_tests = r.expand_bv_tests;
+  out->expand_leaf_tests = r.expand_leaf_tests;
+  out->dfs_bv_tests = r.dfs_bv_tests;
+  out->dfs_leaf_tests = r.dfs_leaf_tests;
+  out->fp64_fallbacks = r.fp64_fallbacks;
+  out->front_size = r.front_size;
+  out->levels = r.levels;
+  out->flagged = r.flagged;
+  return FCLX_OK;
+}
+
 int fclx_sync(fclx_context* ctx, void* stream) {
   if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);

@@ -27,4 +27,13 @@ EngineTuning& engine_tuning();
 
 constexpr int kOrderedCapMax = 64;
 
+struct EngineReport {
+  double setup_ms, expand_ms, leaves_ms, dfs_ms, fixup_ms;
+  long long expand_bv_tests, expand_leaf_tests, dfs_bv_tests, dfs_leaf_tests, fp64_fallbacks;
+  long long front_size;
+  int levels, flagged;
+};
+hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
+const EngineReport& engine_report(const EngineWorkspace* ws);
+
 }  // namespace fclx

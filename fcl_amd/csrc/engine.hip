@@ -130,7 +130,17 @@ struct EngineArgs {
   unsigned long long* head;       // DFS refill cursor
   unsigned long long* stacks;
   int stack_cap;
+  unsigned long long* work;  // kWork* counters
 };
+
+enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback };
+
+// add popcount(pred) over the wavefront to *counter with one atomic
+__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m != 0 && lane_id() == __ffsll(static_cast<long long>(m)) - 1)
+    atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
+}
 
 // ---------------------------------------------------------------- setup
 __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
@@ -168,9 +178,10 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
 // BV test of task (q,b1,b2): FP32 cull, FP64 only when unsure.
 __device__ __forceinline__ bool bv_overlap(const EngineArgs& A, const QueryXf32& X, int q,
                                            const MeshDev& m1, const MeshDev& m2, int b1,
-                                           int b2) {
+                                           int b2, bool& fallback) {
   const int r = obb_cull32(X, m1.obb32[b1], m2.obb32[b2]);
-  if (r != kCullUnsure) return r == kCullOverlap;
+  fallback = r == kCullUnsure;
+  if (!fallback) return r == kCullOverlap;
   const QueryXf64& x = A.qx64[q];
   M3 R;
 #pragma unroll
@@ -222,7 +233,7 @@ __device__ __forceinline__ bool leaf_hit(const EngineArgs& A, unsigned long long
 __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = i < A.n_in;
-  bool push = false, leafc = false;
+  bool push = false, leafc = false, fb = false;
   unsigned long long c0 = 0, c1 = 0, task = 0;
   if (active) {
     task = A.front_in[i];
@@ -232,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
     const MeshDev& m1 = A.L.meshes[X.m1];
     const MeshDev& m2 = A.L.meshes[X.m2];
     if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
-    if (bv_overlap(A, X, q, m1, m2, b1, b2)) {
+    if (bv_overlap(A, X, q, m1, m2, b1, b2, fb)) {
       const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
       if (fc1 < 0 && fc2 < 0) {
         leafc = true;
@@ -248,6 +259,8 @@ __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
       }
     }
   }
+  wave_count(A.work + kWorkExpandBV, active);
+  wave_count(A.work + kWorkFallback, fb);
   const unsigned long long so = wave_reserve(A.n_out, push, 2);
   if (push) {
     A.front_out[so] = c0;
@@ -262,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
   bool hit = false;
   int q = 0, id1 = 0, id2 = 0;
   if (i < A.n_in) hit = leaf_hit(A, A.leafc[i], q, id1, id2);
+  wave_count(A.work + kWorkLeafKernel, i < A.n_in);
   record_hits(A, hit, q, id1, id2);
 }
 
@@ -276,8 +290,10 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
   int sp = 0;      // wavefront-uniform
   int nleaf = 0;   // wavefront-uniform
   bool input_left = true;
+  unsigned long long n_bv = 0, n_leaf = 0, n_fb = 0;  // wavefront-uniform
 
   auto flush_leaves = [&](int k) {
+    n_leaf += k;
     bool hit = false;
     int q = 0, id1 = 0, id2 = 0;
     if (lane < k) hit = leaf_hit(A, leafq[nleaf - k + lane], q, id1, id2);
@@ -305,8 +321,9 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     const bool active = lane < n;
     unsigned long long task = 0;
     if (active) task = stack[sp + lane];
-    bool push = false, leafc = false;
+    bool push = false, leafc = false, fb = false;
     unsigned long long c0 = 0, c1 = 0;
+    n_bv += n;
     if (active) {
       int q, b1, b2;
       unpack_task(task, q, b1, b2);
@@ -314,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
       const MeshDev& m1 = A.L.meshes[X.m1];
       const MeshDev& m2 = A.L.meshes[X.m2];
       if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
-      if (bv_overlap(A, X, q, m1, m2, b1, b2)) {
+      if (bv_overlap(A, X, q, m1, m2, b1, b2, fb)) {
         const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
         if (fc1 < 0 && fc2 < 0) {
           leafc = true;
@@ -330,6 +347,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
         }
       }
     }
+    n_fb += __popcll(__ballot(fb));
     // children onto the stack (one prefix sum)
     const unsigned long long pm = __ballot(push);
     const int npush = 2 * __popcll(pm);
@@ -354,6 +372,11 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
   while (nleaf > 0) flush_leaves(nleaf < kWave ? nleaf : kWave);
+  if (lane == 0) {
+    atomicAdd(A.work + kWorkDfsBV, n_bv);
+    atomicAdd(A.work + kWorkDfsLeaf, n_leaf);
+    if (n_fb) atomicAdd(A.work + kWorkFallback, n_fb);
+  }
 }
 
 // ---------------------------------- fix-up of queries that reached the cap
@@ -581,8 +604,11 @@ struct EngineWorkspace {
   size_t leafc_cap = 0;
   unsigned long long* stacks = nullptr;
   size_t stacks_cap = 0;
-  unsigned long long* counters = nullptr;  // 8 device counters
+  unsigned long long* counters = nullptr;  // 16 device counters (8.. = work)
   unsigned long long* host_counters = nullptr;
+  bool profile = false;
+  hipEvent_t ev[6] = {};
+  EngineReport report{};
 };
 
 EngineTuning& engine_tuning() {
@@ -592,10 +618,10 @@ EngineTuning& engine_tuning() {
 
 EngineWorkspace* engine_workspace_create() {
   auto* ws = new EngineWorkspace();
-  if (hipMalloc(reinterpret_cast<void**>(&ws->counters), 8 * sizeof(unsigned long long)) !=
+  if (hipMalloc(reinterpret_cast<void**>(&ws->counters), 16 * sizeof(unsigned long long)) !=
           hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ws->host_counters),
-                    8 * sizeof(unsigned long long)) != hipSuccess) {
+                    16 * sizeof(unsigned long long)) != hipSuccess) {
     engine_workspace_destroy(ws);
     return nullptr;
   }
@@ -613,8 +639,20 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
   (void)hipFree(ws->stacks);
   (void)hipFree(ws->counters);
   if (ws->host_counters) (void)hipHostFree(ws->host_counters);
+  for (auto& e : ws->ev)
+    if (e) (void)hipEventDestroy(e);
   delete ws;
 }
+
+hipError_t engine_set_profiling(EngineWorkspace* ws, bool on) {
+  if (!ws) return hipErrorInvalidValue;
+  if (on && !ws->ev[0])
+    for (auto& e : ws->ev) ENG_TRY(hipEventCreate(&e));
+  ws->profile = on;
+  return hipSuccess;
+}
+
+const EngineReport& engine_report(const EngineWorkspace* ws) { return ws->report; }
 
 namespace {
 
@@ -754,7 +792,14 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   A.qx64 = ws->qx64;
   A.qflag = ws->qflag;
   A.nflag = ws->counters + 4;
-  for (int i = 0; i < 8; ++i) set_u64<<<1, 1, 0, s>>>(ws->counters + i, 0);
+  A.work = ws->counters + 8;
+  for (int i = 0; i < 16; ++i) set_u64<<<1, 1, 0, s>>>(ws->counters + i, 0);
+  EngineReport& rep = ws->report;
+  rep = EngineReport{};
+  auto mark = [&](int k) {
+    if (ws->profile) (void)hipEventRecord(ws->ev[k], s);
+  };
+  mark(0);
   // 1. setup -> front[0]
   A.front_out = ws->front[0];
   A.n_out = ws->counters + 0;
@@ -762,6 +807,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   ENG_TRY(hipGetLastError());
   unsigned long long nfront = 0;
   ENG_TRY(read_counter(ws, 0, nfront, s));
+  mark(1);
   // 2. breadth-first expansion while the front is small
   int cur = 0;
   unsigned long long nleafc_total = 0;
@@ -783,7 +829,9 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     ENG_TRY(read_counter(ws, 1, nfront, s));
     ENG_TRY(read_counter(ws, 2, nleafc_total, s));
     cur = nxt;
+    ++rep.levels;
   }
+  mark(2);
   // 3. leaf candidates of the breadth-first phase
   if (nleafc_total > 0) {
     A.leafc = ws->leafc;
@@ -791,6 +839,8 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     engine_leaves<<<grid_for(nleafc_total), kBlock, 0, s>>>(A);
     ENG_TRY(hipGetLastError());
   }
+  mark(3);
+  rep.front_size = static_cast<long long>(nfront);
   // 4. persistent depth-first phase over the remaining front
   if (nfront > 0) {
     const long long waves_needed = (static_cast<long long>(nfront) + kWave - 1) / kWave;
@@ -809,10 +859,29 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     engine_dfs<<<blocks, kBlock, 0, s>>>(A);
     ENG_TRY(hipGetLastError());
   }
+  mark(4);
   // 5. queries that reached the cap
   unsigned long long nflag = 0;
   ENG_TRY(read_counter(ws, 4, nflag, s));
   if (nflag > 0) ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
+  mark(5);
+  rep.flagged = static_cast<int>(nflag);
+  ENG_TRY(hipMemcpyAsync(ws->host_counters + 8, ws->counters + 8, 5 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, s));
+  ENG_TRY(hipStreamSynchronize(s));
+  rep.expand_bv_tests = static_cast<long long>(ws->host_counters[8 + kWorkExpandBV]);
+  rep.expand_leaf_tests = static_cast<long long>(ws->host_counters[8 + kWorkLeafKernel]);
+  rep.dfs_bv_tests = static_cast<long long>(ws->host_counters[8 + kWorkDfsBV]);
+  rep.dfs_leaf_tests = static_cast<long long>(ws->host_counters[8 + kWorkDfsLeaf]);
+  rep.fp64_fallbacks = static_cast<long long>(ws->host_counters[8 + kWorkFallback]);
+  if (ws->profile) {
+    float ms = 0;
+    double* out[5] = {&rep.setup_ms, &rep.expand_ms, &rep.leaves_ms, &rep.dfs_ms, &rep.fixup_ms};
+    for (int k = 0; k < 5; ++k) {
+      ENG_TRY(hipEventElapsedTime(&ms, ws->ev[k], ws->ev[k + 1]));
+      *out[k] = ms;
+    }
+  }
   return hipSuccess;
 }
 

@@ -129,6 +129,30 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* args,
                        void* stream);
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
                         void* stream);
+/* Breakdown of the last FCLX_ENGINE_FAST collide batch with a cap above the
+ * ordered limit (no counterpart in the reference, which only keeps
+ * num_bv_tests/num_leaf_tests per traversal, bvh_collision_traversal_node.h:
+ * 91-94).  Times come from HIP events on the caller's stream and are filled
+ * only while profiling is enabled; the work counts always are. */
+typedef struct {
+  double setup_ms;
+  double expand_ms;          /* all breadth-first levels */
+  double leaves_ms;          /* leaf kernel over the breadth-first candidates */
+  double dfs_ms;             /* persistent depth-first kernel */
+  double fixup_ms;           /* re-run of queries that reached the cap */
+  int64_t expand_bv_tests;   /* BV tests done by the breadth-first levels */
+  int64_t expand_leaf_tests; /* triangle tests of the leaf kernel */
+  int64_t dfs_bv_tests;      /* BV tests done by the depth-first kernel */
+  int64_t dfs_leaf_tests;    /* triangle tests done by the depth-first kernel */
+  int64_t fp64_fallbacks;    /* BV tests the FP32 cull left to FP64 */
+  int64_t front_size;        /* tasks handed to the depth-first kernel */
+  int32_t levels;            /* breadth-first levels run */
+  int32_t flagged;           /* queries re-run because they reached the cap */
+} fclx_engine_report;
+
+int fclx_set_profiling(fclx_context* ctx, int32_t enable);
+int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
+
 /* Wait for the stream, then report (and clear) the device status: FCLX_OK,
  * FCLX_ERR_CAPACITY (pair records beyond pair_capacity were dropped) or
  * FCLX_ERR_STACK_OVERFLOW. */
