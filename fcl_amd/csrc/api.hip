@@ -29,6 +29,8 @@ struct MeshSlot {
   ObbNode32* obb32 = nullptr;
   double* verts = nullptr;
   int32_t* tris = nullptr;
+  int32_t* span = nullptr;
+  int32_t* prim_pos = nullptr;
   int64_t bytes = 0;
   MeshDev dev{};
 };
@@ -92,6 +94,8 @@ void free_slot(MeshSlot& s) {
   (void)hipFree(s.obb32);
   (void)hipFree(s.verts);
   (void)hipFree(s.tris);
+  (void)hipFree(s.span);
+  (void)hipFree(s.prim_pos);
   s = MeshSlot{};
 }
 
@@ -232,14 +236,29 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
     f.first_child = h.first_child;
     sz[i] = s.bvh.node_size(i);
   }
+  // primitive ranges of the nodes and each triangle's position in the build
+  // order: they locate a triangle's leaf from the root (depth-first keys)
+  std::vector<int32_t> span(2 * static_cast<size_t>(nb));
+  std::vector<int32_t> pos(num_tris);
+  for (int i = 0; i < nb; ++i) {
+    span[2 * i] = nodes[i].first_primitive;
+    span[2 * i + 1] = nodes[i].num_primitives;
+  }
+  for (int i = 0; i < num_tris; ++i) pos[s.bvh.primitive_indices[i]] = i;
   std::lock_guard<std::mutex> lock(ctx->mu);
   (void)hipSetDevice(ctx->device);
   bool ok = dmalloc(&s.obb64, nb) == hipSuccess && dmalloc(&s.rss64, nb) == hipSuccess &&
             dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32, nb) == hipSuccess &&
             dmalloc(&s.verts, 3 * static_cast<size_t>(num_vertices)) == hipSuccess &&
-            dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess;
+            dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess &&
+            dmalloc(&s.span, 2 * static_cast<size_t>(nb)) == hipSuccess &&
+            dmalloc(&s.prim_pos, static_cast<size_t>(num_tris)) == hipSuccess;
   if (ok)
-    ok = hipMemcpy(s.obb64, o64.data(), sizeof(ObbNode64) * nb, hipMemcpyHostToDevice) ==
+    ok = hipMemcpy(s.span, span.data(), sizeof(int32_t) * span.size(), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.prim_pos, pos.data(), sizeof(int32_t) * pos.size(),
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(s.obb64, o64.data(), sizeof(ObbNode64) * nb, hipMemcpyHostToDevice) ==
              hipSuccess &&
          hipMemcpy(s.rss64, r64.data(), sizeof(RssNode64) * nb, hipMemcpyHostToDevice) ==
              hipSuccess &&
@@ -257,9 +276,11 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   }
   s.bytes = static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
                                         sizeof(ObbNode32)) +
-            24LL * num_vertices + 12LL * num_tris;
+            24LL * num_vertices + 16LL * num_tris + 8LL * nb;
   s.live = true;
   MeshDev& d = s.dev;
+  d.span = s.span;
+  d.prim_pos = s.prim_pos;
   d.obb64 = s.obb64;
   d.rss64 = s.rss64;
   d.size64 = s.size64;

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <hipcub/hipcub.hpp>
 #include <vector>
 
 #include "device/cull32.h"
@@ -380,12 +381,95 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
 }
 
 // ---------------------------------- fix-up of queries that reached the cap
-__global__ void engine_collect_flagged(const int32_t* qflag, int nq, int32_t* list,
+// list[s] = q for every flagged query; qflag[q] becomes s + 1
+__global__ void engine_collect_flagged(int32_t* qflag, int nq, int32_t* list,
                                        unsigned long long* n) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool f = q < nq && qflag[q] != 0;
   const unsigned long long s = wave_reserve(n, f, 1);
-  if (f) list[s] = q;
+  if (f) {
+    list[s] = q;
+    qflag[q] = static_cast<int32_t>(s) + 1;
+  }
+}
+
+// records per flagged query
+__global__ void engine_count_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
+                                     int32_t* seg_count) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const int f = qflag[rec[3 * i]];
+  if (f) atomicAdd(&seg_count[f - 1], 1);
+}
+
+// group the flagged queries' record indices by query
+__global__ void engine_group_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
+                                     const long long* seg_off, int32_t* fill,
+                                     long long* grouped) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const int f = qflag[rec[3 * i]];
+  if (f) grouped[seg_off[f - 1] + atomicAdd(&fill[f - 1], 1)] = i;
+}
+
+// Depth-first position of a leaf pair: the BVTT path from (0,0) to the
+// pair's leaf nodes, one bit per descent (0 = first child), most significant
+// first.  The descent rule is the traversal's (firstOverSecond); the child
+// holding a triangle is the one whose primitive range contains the
+// triangle's build position.  Distinct leaf pairs have paths that differ
+// before either ends, so keys order pairs exactly as depth-first search.
+__global__ void engine_leaf_keys(const int32_t* rec, const long long* grouped, long long n,
+                                 const CollideLaunch L, unsigned long long* keys) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t* r = rec + 3 * grouped[i];
+  const int q = r[0];
+  const MeshDev& m1 = L.meshes[L.query_meshes[2 * q]];
+  const MeshDev& m2 = L.meshes[L.query_meshes[2 * q + 1]];
+  const int p1 = m1.prim_pos[r[1]], p2 = m2.prim_pos[r[2]];
+  int x1 = 0, x2 = 0, len = 0;
+  unsigned long long key = 0;
+  while (true) {
+    const int fc1 = m1.obb64[x1].first_child, fc2 = m2.obb64[x2].first_child;
+    if (fc1 < 0 && fc2 < 0) break;
+    int bit;
+    if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[x1] > m2.size64[x2]))) {
+      bit = p1 < m1.span[2 * fc1] + m1.span[2 * fc1 + 1] ? 0 : 1;
+      x1 = fc1 + bit;
+    } else {
+      bit = p2 < m2.span[2 * fc2] + m2.span[2 * fc2 + 1] ? 0 : 1;
+      x2 = fc2 + bit;
+    }
+    if (len < 64) key |= static_cast<unsigned long long>(bit) << (63 - len);
+    ++len;
+  }
+  keys[i] = key;
+}
+
+// first `take` sorted records of each segment -> out[base + out_off[seg] ...]
+__global__ void engine_emit_prefix(const int32_t* rec, const long long* sorted,
+                                   const long long* seg_off, const long long* out_off,
+                                   const int32_t* take, int nseg, int32_t* out, long long base,
+                                   long long cap, int32_t* status) {
+  const int seg = blockIdx.x;
+  if (seg >= nseg) return;
+  for (int j = threadIdx.x; j < take[seg]; j += blockDim.x) {
+    const long long o = base + out_off[seg] + j;
+    if (o >= cap) {
+      atomicOr(status, kStatusPairOverflow);
+      return;
+    }
+    const int32_t* r = rec + 3 * sorted[seg_off[seg] + j];
+    out[3 * o] = r[0];
+    out[3 * o + 1] = r[1];
+    out[3 * o + 2] = r[2];
+  }
+}
+
+__global__ void engine_set_counts(const int32_t* list, const int32_t* take, int n,
+                                  int32_t* counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counts[list[i]] = take[i];
 }
 
 // copy records of unflagged queries to out (unordered), count in n
@@ -444,6 +528,7 @@ __global__ void engine_append_records(const int32_t* rec, long long nrec, const 
 }
 
 __global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void or_status(int32_t* status, int32_t bits) { atomicOr(status, bits); }
 
 // ---------------------------------------------- ordered, small caps
 template <int STACK>
@@ -759,6 +844,125 @@ hipError_t fixup_flagged(EngineWorkspace* ws, const CollideLaunch& L, int stack_
   return e;
 }
 
+// Device buffers of one fix-up, freed together.
+struct Scratch {
+  std::vector<void*> ptrs;
+  template <typename T>
+  hipError_t alloc(T*& p, size_t n) {
+    p = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(n, 1));
+    if (e == hipSuccess) ptrs.push_back(p);
+    return e;
+  }
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+// Truncate each flagged query's recorded contact set to its first `cap`
+// pairs in depth-first order (keys from engine_leaf_keys, one segmented
+// radix sort), replacing the query's records in the output buffer.
+hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long long nflag,
+                      hipStream_t s) {
+  const int nq = L.num_queries;
+  const int nf = static_cast<int>(nflag);
+  unsigned long long produced = 0;
+  ENG_TRY(hipMemcpyAsync(&produced, L.num_pairs, sizeof(produced), hipMemcpyDeviceToHost, s));
+  ENG_TRY(hipStreamSynchronize(s));
+  const long long have = std::min<long long>(static_cast<long long>(produced), L.pair_capacity);
+  Scratch S;
+  int32_t *list, *seg_count, *fill, *take;
+  long long *seg_off, *out_off;
+  ENG_TRY(S.alloc(list, nf));
+  ENG_TRY(S.alloc(seg_count, nf));
+  ENG_TRY(S.alloc(fill, nf));
+  ENG_TRY(S.alloc(take, nf));
+  ENG_TRY(S.alloc(seg_off, nf));
+  ENG_TRY(S.alloc(out_off, nf));
+  set_u64<<<1, 1, 0, s>>>(ws->counters + 5, 0);
+  engine_collect_flagged<<<grid_for(nq), kBlock, 0, s>>>(ws->qflag, nq, list, ws->counters + 5);
+  ENG_TRY(hipMemsetAsync(seg_count, 0, sizeof(int32_t) * nf, s));
+  ENG_TRY(hipMemsetAsync(fill, 0, sizeof(int32_t) * nf, s));
+  if (have > 0)
+    engine_count_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, seg_count);
+  std::vector<int32_t> h_count(nf), h_take(nf);
+  std::vector<long long> h_off(nf), h_out(nf);
+  std::vector<int> h_begin(nf), h_end(nf);
+  ENG_TRY(hipMemcpyAsync(h_count.data(), seg_count, sizeof(int32_t) * nf, hipMemcpyDeviceToHost,
+                         s));
+  ENG_TRY(hipStreamSynchronize(s));
+  long long total = 0, total_take = 0;
+  for (int i = 0; i < nf; ++i) {
+    h_off[i] = total;
+    h_begin[i] = static_cast<int>(total);
+    total += h_count[i];
+    h_end[i] = static_cast<int>(total);
+    h_take[i] = std::min(h_count[i], L.cap);
+    h_out[i] = total_take;
+    total_take += h_take[i];
+  }
+  if (total >= (1LL << 31)) return hipErrorInvalidValue;  // sort offsets are int
+  ENG_TRY(hipMemcpyAsync(seg_off, h_off.data(), sizeof(long long) * nf, hipMemcpyHostToDevice,
+                         s));
+  ENG_TRY(hipMemcpyAsync(out_off, h_out.data(), sizeof(long long) * nf, hipMemcpyHostToDevice,
+                         s));
+  ENG_TRY(hipMemcpyAsync(take, h_take.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
+  long long *grouped, *sorted;
+  unsigned long long *keys, *keys_sorted;
+  int *d_begin, *d_end;
+  ENG_TRY(S.alloc(grouped, total));
+  ENG_TRY(S.alloc(sorted, total));
+  ENG_TRY(S.alloc(keys, total));
+  ENG_TRY(S.alloc(keys_sorted, total));
+  ENG_TRY(S.alloc(d_begin, nf));
+  ENG_TRY(S.alloc(d_end, nf));
+  ENG_TRY(hipMemcpyAsync(d_begin, h_begin.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+  ENG_TRY(hipMemcpyAsync(d_end, h_end.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+  if (total > 0) {
+    engine_group_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, seg_off,
+                                                           fill, grouped);
+    engine_leaf_keys<<<grid_for(total), kBlock, 0, s>>>(L.pairs, grouped, total, L, keys);
+    size_t tmp_bytes = 0;
+    ENG_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        nullptr, tmp_bytes, keys, keys_sorted, grouped, sorted, static_cast<int>(total), nf,
+        d_begin, d_end, 0, 64, s));
+    unsigned char* tmp;
+    ENG_TRY(S.alloc(tmp, tmp_bytes));
+    ENG_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        tmp, tmp_bytes, keys, keys_sorted, grouped, sorted, static_cast<int>(total), nf, d_begin,
+        d_end, 0, 64, s));
+  }
+  // keep every other query's records, then append the truncated lists
+  unsigned long long nkept = 0;
+  int32_t* kept;
+  ENG_TRY(S.alloc(kept, 3 * std::max<long long>(have, 1)));
+  set_u64<<<1, 1, 0, s>>>(ws->counters + 6, 0);
+  if (have > 0)
+    engine_drop_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, kept,
+                                                          ws->counters + 6);
+  ENG_TRY(read_counter(ws, 6, nkept, s));
+  int32_t* prefix;
+  ENG_TRY(S.alloc(prefix, 3 * std::max<long long>(total_take, 1)));
+  if (total_take > 0)
+    engine_emit_prefix<<<nf, kBlock, 0, s>>>(L.pairs, sorted, seg_off, out_off, take, nf, prefix,
+                                             0, total_take, L.status);
+  if (nkept > 0)
+    ENG_TRY(hipMemcpyAsync(L.pairs, kept, sizeof(int32_t) * 3 * nkept, hipMemcpyDeviceToDevice,
+                           s));
+  const long long room = L.pair_capacity - static_cast<long long>(nkept);
+  const long long fit = std::max<long long>(0, std::min(room, total_take));
+  if (fit > 0)
+    ENG_TRY(hipMemcpyAsync(L.pairs + 3 * nkept, prefix, sizeof(int32_t) * 3 * fit,
+                           hipMemcpyDeviceToDevice, s));
+  if (fit < total_take) or_status<<<1, 1, 0, s>>>(L.status, kStatusPairOverflow);
+  engine_set_counts<<<grid_for(nf), kBlock, 0, s>>>(list, take, nf, L.counts);
+  const unsigned long long lost = produced - static_cast<unsigned long long>(have);
+  set_u64<<<1, 1, 0, s>>>(reinterpret_cast<unsigned long long*>(L.num_pairs),
+                          nkept + static_cast<unsigned long long>(total_take) + lost);
+  ENG_TRY(hipStreamSynchronize(s));
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
@@ -863,7 +1067,15 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   // 5. queries that reached the cap
   unsigned long long nflag = 0;
   ENG_TRY(read_counter(ws, 4, nflag, s));
-  if (nflag > 0) ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
+  if (nflag > 0) {
+    // per-query statistics of a truncated traversal need the depth-first
+    // re-run; otherwise the recorded sets are truncated by depth-first keys
+    // (which fit 64 bits while the summed tree depths do)
+    if (L.bv_tests || L.leaf_tests || stack_need > 64)
+      ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
+    else
+      ENG_TRY(fixup_keys(ws, L, nflag, s));
+  }
   mark(5);
   rep.flagged = static_cast<int>(nflag);
   ENG_TRY(hipMemcpyAsync(ws->host_counters + 8, ws->counters + 8, 5 * sizeof(unsigned long long),

@@ -62,6 +62,8 @@ struct MeshDev {
   int32_t bv_type;      // 0 = OBBRSS, 1 = RSS
   int32_t pad;
   double cmax;          // max over nodes of |OBB centre|_1 (bounds the FP32 cull error)
+  const int32_t* span;      // 2 per node: first_primitive, num_primitives
+  const int32_t* prim_pos;  // per triangle: its position in the build's primitive order
 };
 
 }  // namespace fclx
