@@ -160,13 +160,19 @@ class Context:
     def collide(self, m1, tf1, m2, tf2, cap=1):
         tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
         tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
-        pairs = np.zeros((cap, 2), np.int32)
-        n = N.c_i32()
-        stats = np.zeros(2, np.int64)
-        self._check(self._L.fclx_collide(self._h, m1, N.ptr(tf1, N.c_dbl), m2,
-                                         N.ptr(tf2, N.c_dbl), cap, N.ptr(pairs, N.c_i32),
-                                         ctypes.byref(n), N.ptr(stats, N.c_i64)))
-        return pairs[: n.value].copy(), stats
+        room = min(cap, 4096)
+        while True:
+            pairs = np.zeros((room, 2), np.int32)
+            n = N.c_i32()
+            stats = np.zeros(2, np.int64)
+            rc = self._L.fclx_collide(self._h, m1, N.ptr(tf1, N.c_dbl), m2, N.ptr(tf2, N.c_dbl),
+                                      cap, N.ptr(pairs, N.c_i32), room, ctypes.byref(n),
+                                      N.ptr(stats, N.c_i64))
+            if rc == N.FCLX_ERR_CAPACITY:
+                room = n.value
+                continue
+            self._check(rc)
+            return pairs[: n.value].copy(), stats
 
     def distance(self, m1, tf1, m2, tf2, nearest=True):
         tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)

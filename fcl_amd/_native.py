@@ -80,7 +80,8 @@ SIGNATURES = {
     "fclx_collide_batch": (c_i32, [c_vp, ctypes.POINTER(CollideBatchArgs), c_vp]),
     "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
     "fclx_sync": (c_i32, [c_vp, c_vp]),
-    "fclx_collide": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, P_i32, P_i64]),
+    "fclx_collide": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, c_i32, P_i32,
+                              P_i64]),
     "fclx_distance": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_dbl, P_dbl, P_i32,
                               P_i64]),
     "fclx_synth_icosphere_size": (None, [c_i32, P_i32, P_i32]),

@@ -519,9 +519,10 @@ int fclx_sync(fclx_context* ctx, void* stream) {
 }
 
 int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
-                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t* num_contacts,
-                 int64_t* stats) {
-  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1)
+                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t pairs_capacity,
+                 int32_t* num_contacts, int64_t* stats) {
+  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1 || pairs_capacity < 0 ||
+      (pairs_capacity > 0 && !pairs))
     return FCLX_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (!valid_mesh(ctx, mesh1) || !valid_mesh(ctx, mesh2))
@@ -530,13 +531,6 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
       ctx->meshes[mesh2].bvh.bv_type != FCLX_BV_OBBRSS)
     return fail(ctx, FCLX_ERR_UNSUPPORTED, "mesh collide needs OBBRSS hierarchies");
   (void)hipSetDevice(ctx->device);
-  if (cap > ctx->s_pairs_cap) {
-    (void)hipFree(ctx->s_pairs);
-    ctx->s_pairs = nullptr;
-    ctx->s_pairs_cap = 0;
-    HIP_TRY(ctx, dmalloc(&ctx->s_pairs, 3 * static_cast<size_t>(cap)));
-    ctx->s_pairs_cap = cap;
-  }
   double tf[24];
   std::memcpy(tf, tf1, 12 * sizeof(double));
   std::memcpy(tf + 12, tf2, 12 * sizeof(double));
@@ -544,49 +538,69 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->s_tf, tf, sizeof(tf), hipMemcpyHostToDevice, s));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->s_qm, qm, sizeof(qm), hipMemcpyHostToDevice, s));
-  zero_i64<<<1, 64, 0, s>>>(ctx->s_num, 1);
-  CollideLaunch L{};
-  L.meshes = ctx->d_table;
-  L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
-  L.num_queries = 1;
-  L.query_meshes = ctx->s_qm;
-  L.tf1 = ctx->s_tf;
-  L.tf2 = ctx->s_tf + 12;
-  L.cap = cap;
-  L.counts = ctx->s_count;
-  L.pairs = ctx->s_pairs;
-  L.pair_capacity = cap;
-  L.num_pairs = ctx->s_num;
-  L.bv_tests = ctx->s_stats;
-  L.leaf_tests = ctx->s_stats + 1;
-  L.status = ctx->d_status;
   const int depth = ctx->meshes[mesh1].bvh.depth + ctx->meshes[mesh2].bvh.depth;
-  HIP_TRY(ctx, launch_collide_plain(L, depth, s));
+  if (depth > kMaxStack)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
+  // device staging for the pairs starts small and grows to the count found
+  long long want = std::min<long long>(cap, 4096);
   int32_t count = 0;
-  long long st[2];
-  HIP_TRY(ctx, hipMemcpyAsync(&count, ctx->s_count, sizeof(count), hipMemcpyDeviceToHost, s));
-  HIP_TRY(ctx, hipMemcpyAsync(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost, s));
-  HIP_TRY(ctx, hipStreamSynchronize(s));
-  if (count > 0 && pairs) {
-    std::vector<int32_t> rec(3 * static_cast<size_t>(count));
-    HIP_TRY(ctx, hipMemcpy(rec.data(), ctx->s_pairs, sizeof(int32_t) * rec.size(),
-                           hipMemcpyDeviceToHost));
-    for (int i = 0; i < count; ++i) {
-      pairs[2 * i] = rec[3 * i + 1];
-      pairs[2 * i + 1] = rec[3 * i + 2];
+  long long st[2] = {0, 0};
+  long long produced = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (want > ctx->s_pairs_cap) {
+      (void)hipFree(ctx->s_pairs);
+      ctx->s_pairs = nullptr;
+      ctx->s_pairs_cap = 0;
+      HIP_TRY(ctx, dmalloc(&ctx->s_pairs, 3 * static_cast<size_t>(want)));
+      ctx->s_pairs_cap = want;
     }
+    zero_i64<<<1, 64, 0, s>>>(ctx->s_num, 1);
+    CollideLaunch L{};
+    L.meshes = ctx->d_table;
+    L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+    L.num_queries = 1;
+    L.query_meshes = ctx->s_qm;
+    L.tf1 = ctx->s_tf;
+    L.tf2 = ctx->s_tf + 12;
+    L.cap = cap;
+    L.counts = ctx->s_count;
+    L.pairs = ctx->s_pairs;
+    L.pair_capacity = ctx->s_pairs_cap;
+    L.num_pairs = ctx->s_num;
+    L.bv_tests = ctx->s_stats;
+    L.leaf_tests = ctx->s_stats + 1;
+    L.status = ctx->d_status;
+    HIP_TRY(ctx, launch_collide_plain(L, depth, s));
+    HIP_TRY(ctx, hipMemcpyAsync(&count, ctx->s_count, sizeof(count), hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(&produced, ctx->s_num, sizeof(produced), hipMemcpyDeviceToHost,
+                                s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (produced <= ctx->s_pairs_cap) break;
+    want = produced;  // every contact of this query fits on the second run
   }
+  int32_t flag = 0;
+  HIP_TRY(ctx, hipMemcpy(&flag, ctx->d_status, sizeof(flag), hipMemcpyDeviceToHost));
+  if (flag) HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
+  if (flag & kStatusStackOverflow)
+    return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "traversal stack overflow");
   *num_contacts = count;
   if (stats) {
     stats[0] = st[0];
     stats[1] = st[1];
   }
-  int32_t flag = 0;
-  HIP_TRY(ctx, hipMemcpy(&flag, ctx->d_status, sizeof(flag), hipMemcpyDeviceToHost));
-  if (flag & kStatusStackOverflow) {
-    HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
-    return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "traversal stack overflow");
+  const int32_t ncopy = std::min(count, pairs_capacity);
+  if (ncopy > 0) {
+    std::vector<int32_t> rec(3 * static_cast<size_t>(ncopy));
+    HIP_TRY(ctx, hipMemcpy(rec.data(), ctx->s_pairs, sizeof(int32_t) * rec.size(),
+                           hipMemcpyDeviceToHost));
+    for (int i = 0; i < ncopy; ++i) {
+      pairs[2 * i] = rec[3 * i + 1];
+      pairs[2 * i + 1] = rec[3 * i + 2];
+    }
   }
+  if (count > pairs_capacity)
+    return fail(ctx, FCLX_ERR_CAPACITY, "pairs buffer smaller than the contact count");
   return FCLX_OK;
 }
 

@@ -1,0 +1,277 @@
+// fcl::collide / fcl::distance / BVHModel drop-in over the C ABI.
+//
+// Dispatch mirrors collide<S,Solver> (narrowphase/collision-inl.h:93-149)
+// and distance<Solver> (distance-inl.h:90-193): an invalid solver type
+// returns (size_t)-1 / -1, num_max_contacts == 0 warns and returns 0, a
+// pre-satisfied result short-circuits (collision_func_matrix-inl.h:580,
+// distance_func_matrix-inl.h:396), and a geometry pair outside the mesh
+// path prints the reference's "not supported" warning with the neutral
+// return.  The mesh-mesh cells run on the GPU through one process-wide
+// engine context (device 0, or FCLX_DEVICE).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <mutex>
+
+#include "../../../include/fcl/fcl.h"
+#include "../../../include/fclx.h"
+
+namespace fcl {
+namespace detail {
+
+namespace {
+std::once_flag g_once;
+fclx_context* g_ctx = nullptr;
+std::atomic<bool> g_warned_contact{false}, g_warned_cost{false};
+
+fclx_context* engine() {
+  std::call_once(g_once, [] {
+    const char* d = std::getenv("FCLX_DEVICE");
+    const int dev = d ? std::atoi(d) : 0;
+    if (fclx_init(dev, &g_ctx) != FCLX_OK) {
+      std::cerr << "fcl: GPU engine initialisation failed on device " << dev << "\n";
+      g_ctx = nullptr;
+    }
+  });
+  return g_ctx;
+}
+}  // namespace
+
+struct DeviceMesh {
+  int32_t id = -1;
+  ~DeviceMesh() {
+    if (g_ctx && id >= 0) fclx_mesh_destroy(g_ctx, id);
+  }
+};
+
+int device_mesh_id(const DeviceMesh* m) { return m ? m->id : -1; }
+
+int upload_mesh(int bv_type, const std::vector<Vector3d>& vertices,
+                const std::vector<Triangle>& tris, std::shared_ptr<DeviceMesh>& out) {
+  fclx_context* ctx = engine();
+  if (!ctx) return BVH_ERR_UNKNOWN;
+  std::vector<double> v(3 * vertices.size());
+  for (size_t i = 0; i < vertices.size(); ++i)
+    for (int k = 0; k < 3; ++k) v[3 * i + k] = vertices[i][k];
+  std::vector<int32_t> t(3 * tris.size());
+  for (size_t i = 0; i < tris.size(); ++i)
+    for (int k = 0; k < 3; ++k) {
+      if (tris[i][k] >= vertices.size()) return BVH_ERR_INCORRECT_DATA;
+      t[3 * i + k] = static_cast<int32_t>(tris[i][k]);
+    }
+  auto m = std::make_shared<DeviceMesh>();
+  const int rc = fclx_mesh_create(ctx, bv_type, v.data(), static_cast<int32_t>(vertices.size()),
+                                  t.data(), static_cast<int32_t>(tris.size()), &m->id);
+  switch (rc) {
+    case FCLX_OK:
+      out = m;
+      return BVH_OK;
+    case FCLX_ERR_EMPTY_MODEL:
+      return BVH_ERR_BUILD_EMPTY_MODEL;
+    case FCLX_ERR_OUT_OF_MEMORY:
+      return BVH_ERR_MODEL_OUT_OF_MEMORY;
+    case FCLX_ERR_INVALID_ARGUMENT:
+      return BVH_ERR_INCORRECT_DATA;
+    default:
+      std::cerr << "fcl: mesh upload failed: " << fclx_last_error(ctx) << "\n";
+      return BVH_ERR_UNKNOWN;
+  }
+}
+
+namespace {
+
+void to_pose(const Transform3d& tf, double out[12]) {
+  for (int i = 0; i < 9; ++i) out[i] = tf.R.m[i];
+  for (int i = 0; i < 3; ++i) out[9 + i] = tf.t[i];
+}
+
+// built BVHModel of the given node type, or nullptr
+int mesh_id_of(const CollisionGeometry<double>* g, NODE_TYPE want) {
+  if (!g || g->getObjectType() != OT_BVH || g->getNodeType() != want) return -1;
+  if (want == BV_OBBRSS) return static_cast<const BVHModel<OBBRSSd>*>(g)->deviceMeshId();
+  if (want == BV_RSS) return static_cast<const BVHModel<RSSd>*>(g)->deviceMeshId();
+  return -1;
+}
+
+void unsupported(const char* what, const CollisionGeometry<double>* o1,
+                 const CollisionGeometry<double>* o2) {
+  std::cerr << "Warning: " << what << " function between node type "
+            << (o1 ? o1->getNodeType() : BV_UNKNOWN) << " and node type "
+            << (o2 ? o2->getNodeType() : BV_UNKNOWN) << " is not supported\n";
+}
+
+}  // namespace
+}  // namespace detail
+
+template <>
+std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform3d& tf1,
+                            const CollisionGeometry<double>* o2, const Transform3d& tf2,
+                            const CollisionRequest<double>& request,
+                            CollisionResult<double>& result) {
+  if (request.gjk_solver_type != GST_LIBCCD && request.gjk_solver_type != GST_INDEP) {
+    std::cerr << "Warning! Invalid GJK solver\n";
+    return static_cast<std::size_t>(-1);
+  }
+  if (request.num_max_contacts == 0) {
+    std::cerr << "Warning: should stop early as num_max_contact is " << request.num_max_contacts
+              << " !\n";
+    return 0;
+  }
+  const int id1 = detail::mesh_id_of(o1, BV_OBBRSS);
+  const int id2 = detail::mesh_id_of(o2, BV_OBBRSS);
+  if (id1 < 0 || id2 < 0) {
+    detail::unsupported("collision", o1, o2);
+    return 0;
+  }
+  if (request.isSatisfied(result)) return result.numContacts();
+  if (request.enable_contact && !detail::g_warned_contact.exchange(true))
+    std::cerr << "fcl: contact points/normals are not computed on the GPU mesh path; "
+                 "contacts carry primitive ids only\n";
+  if (request.enable_cost && !detail::g_warned_cost.exchange(true))
+    std::cerr << "fcl: cost sources are not computed on the GPU mesh path\n";
+  const std::size_t have = result.numContacts();
+  const std::size_t room = request.num_max_contacts - have;
+  const int32_t cap = room > static_cast<std::size_t>(INT_MAX) ? INT_MAX : static_cast<int32_t>(room);
+  double p1[12], p2[12];
+  detail::to_pose(tf1, p1);
+  detail::to_pose(tf2, p2);
+  fclx_context* ctx = detail::engine();
+  std::vector<int32_t> pairs(2 * std::min<int32_t>(cap, 4096));
+  int32_t n = 0;
+  int rc = fclx_collide(ctx, id1, p1, id2, p2, cap, pairs.data(),
+                        static_cast<int32_t>(pairs.size() / 2), &n, nullptr);
+  if (rc == FCLX_ERR_CAPACITY) {
+    pairs.resize(2 * static_cast<size_t>(n));
+    rc = fclx_collide(ctx, id1, p1, id2, p2, cap, pairs.data(), n, &n, nullptr);
+  }
+  if (rc != FCLX_OK) {
+    std::cerr << "fcl: GPU collide failed: " << fclx_last_error(ctx) << "\n";
+    return result.numContacts();
+  }
+  for (int32_t i = 0; i < n; ++i)
+    result.addContact(Contact<double>(o1, o2, pairs[2 * i], pairs[2 * i + 1]));
+  return result.numContacts();
+}
+
+template <>
+double distance<double>(const CollisionGeometry<double>* o1, const Transform3d& tf1,
+                        const CollisionGeometry<double>* o2, const Transform3d& tf2,
+                        const DistanceRequest<double>& request, DistanceResult<double>& result) {
+  if (request.gjk_solver_type != GST_LIBCCD && request.gjk_solver_type != GST_INDEP) return -1;
+  const NODE_TYPE nt = o1 ? o1->getNodeType() : BV_UNKNOWN;
+  const int id1 = detail::mesh_id_of(o1, nt == BV_RSS ? BV_RSS : BV_OBBRSS);
+  const int id2 = detail::mesh_id_of(o2, nt == BV_RSS ? BV_RSS : BV_OBBRSS);
+  if (id1 < 0 || id2 < 0) {
+    detail::unsupported("distance", o1, o2);
+    return std::numeric_limits<double>::max();
+  }
+  if (request.isSatisfied(result)) return result.min_distance;
+  double p1[12], p2[12];
+  detail::to_pose(tf1, p1);
+  detail::to_pose(tf2, p2);
+  fclx_context* ctx = detail::engine();
+  double d = 0, pts[6];
+  int32_t ids[2];
+  const int rc = fclx_distance(ctx, id1, p1, id2, p2, request.enable_nearest_points ? 1 : 0, &d,
+                               pts, ids, nullptr);
+  if (rc != FCLX_OK) {
+    std::cerr << "fcl: GPU distance failed: " << fclx_last_error(ctx) << "\n";
+    return result.min_distance;
+  }
+  if (request.enable_nearest_points)
+    result.update(d, o1, o2, ids[0], ids[1], Vector3d(pts[0], pts[1], pts[2]),
+                  Vector3d(pts[3], pts[4], pts[5]));
+  else
+    result.update(d, o1, o2, ids[0], ids[1]);
+  return result.min_distance;
+}
+
+std::size_t collideBatch(const std::vector<CollideQuery>& queries,
+                         const CollisionRequest<double>& request,
+                         std::vector<CollisionResult<double>>& results) {
+  results.assign(queries.size(), CollisionResult<double>());
+  if (queries.empty()) return 0;
+  if (request.num_max_contacts == 0) return 0;
+  fclx_context* ctx = detail::engine();
+  if (!ctx) return static_cast<std::size_t>(-1);
+  const size_t n = queries.size();
+  std::vector<int32_t> qm(2 * n);
+  std::vector<double> tf(24 * n);
+  for (size_t i = 0; i < n; ++i) {
+    qm[2 * i] = detail::mesh_id_of(queries[i].o1, BV_OBBRSS);
+    qm[2 * i + 1] = detail::mesh_id_of(queries[i].o2, BV_OBBRSS);
+    if (qm[2 * i] < 0 || qm[2 * i + 1] < 0) {
+      detail::unsupported("collision", queries[i].o1, queries[i].o2);
+      return static_cast<std::size_t>(-1);
+    }
+    detail::to_pose(queries[i].tf1, &tf[12 * i]);
+    detail::to_pose(queries[i].tf2, &tf[12 * (n + i)]);
+  }
+  const int32_t cap = request.num_max_contacts > static_cast<std::size_t>(INT_MAX)
+                          ? INT_MAX
+                          : static_cast<int32_t>(request.num_max_contacts);
+  int32_t *d_qm = nullptr, *d_counts = nullptr, *d_pairs = nullptr;
+  double* d_tf = nullptr;
+  int64_t* d_num = nullptr;
+  long long capacity = std::max<long long>(1024, 64LL * n);
+  std::size_t ret = static_cast<std::size_t>(-1);
+  hipStream_t s = nullptr;
+  do {
+    if (hipMalloc(&d_qm, sizeof(int32_t) * 2 * n) != hipSuccess) break;
+    if (hipMalloc(&d_tf, sizeof(double) * 24 * n) != hipSuccess) break;
+    if (hipMalloc(&d_counts, sizeof(int32_t) * n) != hipSuccess) break;
+    if (hipMalloc(&d_num, sizeof(int64_t)) != hipSuccess) break;
+    if (hipMemcpy(d_qm, qm.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice) != hipSuccess)
+      break;
+    if (hipMemcpy(d_tf, tf.data(), sizeof(double) * 24 * n, hipMemcpyHostToDevice) != hipSuccess)
+      break;
+    int rc = FCLX_ERR_CAPACITY;
+    int64_t produced = 0;
+    for (int attempt = 0; attempt < 3 && rc == FCLX_ERR_CAPACITY; ++attempt) {
+      (void)hipFree(d_pairs);
+      d_pairs = nullptr;
+      if (hipMalloc(&d_pairs, sizeof(int32_t) * 3 * capacity) != hipSuccess) break;
+      fclx_collide_batch_args a{};
+      a.num_queries = static_cast<int32_t>(n);
+      a.query_meshes = d_qm;
+      a.tf1 = d_tf;
+      a.tf2 = d_tf + 12 * n;
+      a.num_max_contacts = cap;
+      a.engine = FCLX_ENGINE_FAST;
+      a.counts = d_counts;
+      a.pairs = d_pairs;
+      a.pair_capacity = capacity;
+      a.num_pairs = d_num;
+      rc = fclx_collide_batch(ctx, &a, s);
+      if (rc == FCLX_OK) rc = fclx_sync(ctx, s);
+      if (hipMemcpy(&produced, d_num, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = FCLX_ERR_DEVICE;
+        break;
+      }
+      if (rc == FCLX_ERR_CAPACITY) capacity = produced + 1024;
+    }
+    if (rc != FCLX_OK) break;
+    std::vector<int32_t> rec(3 * static_cast<size_t>(produced));
+    if (produced > 0 && hipMemcpy(rec.data(), d_pairs, sizeof(int32_t) * rec.size(),
+                                  hipMemcpyDeviceToHost) != hipSuccess)
+      break;
+    for (int64_t i = 0; i < produced; ++i) {
+      const size_t q = static_cast<size_t>(rec[3 * i]);
+      results[q].addContact(
+          Contact<double>(queries[q].o1, queries[q].o2, rec[3 * i + 1], rec[3 * i + 2]));
+    }
+    ret = n;
+  } while (false);
+  (void)hipFree(d_qm);
+  (void)hipFree(d_tf);
+  (void)hipFree(d_counts);
+  (void)hipFree(d_num);
+  (void)hipFree(d_pairs);
+  return ret;
+}
+
+}  // namespace fcl

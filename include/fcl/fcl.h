@@ -1,0 +1,501 @@
+// fcl/fcl.h — drop-in C++ API of the MI355X engine for FCL's mesh path.
+//
+// Mirrors the parts of FCL 0.7.0's public API that the BVH-vs-BVH
+// collide()/distance() path uses, with the same names, argument meanings,
+// defaults and neutral error returns:
+//   fcl::collide   include/fcl/narrowphase/collision.h:55-66, collision-inl.h:93-207
+//   fcl::distance  include/fcl/narrowphase/distance.h, distance-inl.h:90-246
+//   CollisionRequest / CollisionResult / Contact   collision_request.h:51-106,
+//                  collision_result.h:51-93, contact.h:47-91
+//   DistanceRequest / DistanceResult   distance_request.h:51-113, distance_result.h:50-107
+//   BVHModel<BV>   include/fcl/geometry/bvh/BVH_model.h:56-227 (build API)
+//   CollisionObject  include/fcl/narrowphase/collision_object.h
+// Eigen is not required: Vector3/Matrix3/Transform3 are small value types
+// with the subset of Eigen's interface that callers of this path use.
+// Mesh-mesh queries on BVHModel<OBBRSS<double>> (collide, distance) and
+// BVHModel<RSS<double>> (distance) run on the GPU; any other pair of
+// geometry types is reported unsupported with the reference's neutral
+// return (0 contacts / max distance).  Only S = double is instantiated.
+#ifndef FCL_AMD_FCL_H
+#define FCL_AMD_FCL_H
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <vector>
+
+namespace fcl {
+
+// ------------------------------------------------------------------ math
+template <typename S>
+struct Vector3 {
+  S v[3];
+  Vector3() : v{0, 0, 0} {}
+  Vector3(S x, S y, S z) : v{x, y, z} {}
+  static Vector3 Zero() { return Vector3(); }
+  S& operator[](int i) { return v[i]; }
+  const S& operator[](int i) const { return v[i]; }
+  S& operator()(int i) { return v[i]; }
+  const S& operator()(int i) const { return v[i]; }
+  Vector3 operator+(const Vector3& o) const { return {v[0] + o[0], v[1] + o[1], v[2] + o[2]}; }
+  Vector3 operator-(const Vector3& o) const { return {v[0] - o[0], v[1] - o[1], v[2] - o[2]}; }
+  Vector3 operator-() const { return {-v[0], -v[1], -v[2]}; }
+  Vector3 operator*(S s) const { return {v[0] * s, v[1] * s, v[2] * s}; }
+  Vector3 operator/(S s) const { return {v[0] / s, v[1] / s, v[2] / s}; }
+  Vector3& operator+=(const Vector3& o) { return *this = *this + o; }
+  Vector3& operator-=(const Vector3& o) { return *this = *this - o; }
+  bool operator==(const Vector3& o) const { return v[0] == o[0] && v[1] == o[1] && v[2] == o[2]; }
+  S dot(const Vector3& o) const { return (v[0] * o[0] + v[1] * o[1]) + v[2] * o[2]; }
+  Vector3 cross(const Vector3& o) const {
+    return {v[1] * o[2] - v[2] * o[1], v[2] * o[0] - v[0] * o[2], v[0] * o[1] - v[1] * o[0]};
+  }
+  S squaredNorm() const { return dot(*this); }
+  S norm() const { return std::sqrt(squaredNorm()); }
+  S x() const { return v[0]; }
+  S y() const { return v[1]; }
+  S z() const { return v[2]; }
+  const S* data() const { return v; }
+};
+template <typename S>
+Vector3<S> operator*(S s, const Vector3<S>& a) {
+  return a * s;
+}
+
+template <typename S>
+struct Matrix3 {
+  S m[9];  // row-major
+  Matrix3() : m{0, 0, 0, 0, 0, 0, 0, 0, 0} {}
+  static Matrix3 Identity() {
+    Matrix3 r;
+    r.m[0] = r.m[4] = r.m[8] = 1;
+    return r;
+  }
+  static Matrix3 Zero() { return Matrix3(); }
+  S& operator()(int i, int j) { return m[3 * i + j]; }
+  const S& operator()(int i, int j) const { return m[3 * i + j]; }
+  Matrix3 transpose() const {
+    Matrix3 r;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r(i, j) = (*this)(j, i);
+    return r;
+  }
+  Vector3<S> row(int i) const { return {m[3 * i], m[3 * i + 1], m[3 * i + 2]}; }
+  Vector3<S> col(int j) const { return {m[j], m[3 + j], m[6 + j]}; }
+  Vector3<S> operator*(const Vector3<S>& p) const { return {row(0).dot(p), row(1).dot(p), row(2).dot(p)}; }
+  Matrix3 operator*(const Matrix3& o) const {
+    Matrix3 r;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r(i, j) = row(i).dot(o.col(j));
+    return r;
+  }
+  bool operator==(const Matrix3& o) const {
+    for (int i = 0; i < 9; ++i)
+      if (m[i] != o.m[i]) return false;
+    return true;
+  }
+};
+
+// Rigid transform (Eigen Isometry3 subset): p -> R p + t.
+template <typename S>
+struct Transform3 {
+  Matrix3<S> R = Matrix3<S>::Identity();
+  Vector3<S> t;
+  static Transform3 Identity() { return Transform3(); }
+  void setIdentity() { *this = Transform3(); }
+  Matrix3<S>& linear() { return R; }
+  const Matrix3<S>& linear() const { return R; }
+  Matrix3<S>& rotation() { return R; }
+  const Matrix3<S>& rotation() const { return R; }
+  Vector3<S>& translation() { return t; }
+  const Vector3<S>& translation() const { return t; }
+  Vector3<S> operator*(const Vector3<S>& p) const { return R * p + t; }
+  Transform3 operator*(const Transform3& o) const {
+    Transform3 r;
+    r.R = R * o.R;
+    r.t = R * o.t + t;
+    return r;
+  }
+  Transform3 inverse() const {
+    Transform3 r;
+    r.R = R.transpose();
+    r.t = -(r.R * t);
+    return r;
+  }
+};
+
+using Vector3d = Vector3<double>;
+using Matrix3d = Matrix3<double>;
+using Transform3d = Transform3<double>;
+
+// -------------------------------------------------------------- geometry
+enum OBJECT_TYPE { OT_UNKNOWN, OT_BVH, OT_GEOM, OT_OCTREE, OT_COUNT };
+enum NODE_TYPE {
+  BV_UNKNOWN, BV_AABB, BV_OBB, BV_RSS, BV_kIOS, BV_OBBRSS, BV_KDOP16, BV_KDOP18, BV_KDOP24,
+  GEOM_BOX, GEOM_SPHERE, GEOM_ELLIPSOID, GEOM_CAPSULE, GEOM_CONE, GEOM_CYLINDER, GEOM_CONVEX,
+  GEOM_PLANE, GEOM_HALFSPACE, GEOM_TRIANGLE, GEOM_OCTREE, NODE_COUNT
+};
+enum GJKSolverType { GST_LIBCCD, GST_INDEP };
+enum BVHBuildState {
+  BVH_BUILD_STATE_EMPTY, BVH_BUILD_STATE_BEGUN, BVH_BUILD_STATE_PROCESSED,
+  BVH_BUILD_STATE_UPDATE_BEGUN, BVH_BUILD_STATE_UPDATED, BVH_BUILD_STATE_REPLACE_BEGUN
+};
+enum BVHReturnCode {
+  BVH_OK = 0, BVH_ERR_MODEL_OUT_OF_MEMORY = -1, BVH_ERR_BUILD_OUT_OF_SEQUENCE = -2,
+  BVH_ERR_BUILD_EMPTY_MODEL = -3, BVH_ERR_BUILD_EMPTY_PREVIOUS_FRAME = -4,
+  BVH_ERR_UNSUPPORTED_FUNCTION = -5, BVH_ERR_UNUPDATED_MODEL = -6, BVH_ERR_INCORRECT_DATA = -7,
+  BVH_ERR_UNKNOWN = -8
+};
+enum BVHModelType { BVH_MODEL_UNKNOWN, BVH_MODEL_TRIANGLES, BVH_MODEL_POINTCLOUD };
+
+template <typename S>
+class CollisionGeometry {
+ public:
+  virtual ~CollisionGeometry() = default;
+  virtual OBJECT_TYPE getObjectType() const { return OT_UNKNOWN; }
+  virtual NODE_TYPE getNodeType() const { return BV_UNKNOWN; }
+  void* getUserData() const { return user_data; }
+  void setUserData(void* d) { user_data = d; }
+  void* user_data = nullptr;
+};
+
+// Bounding-volume tags of the two supported hierarchies.
+template <typename S_>
+struct OBBRSS {
+  using S = S_;
+  static constexpr NODE_TYPE node_type = BV_OBBRSS;
+};
+template <typename S_>
+struct RSS {
+  using S = S_;
+  static constexpr NODE_TYPE node_type = BV_RSS;
+};
+using OBBRSSd = OBBRSS<double>;
+using RSSd = RSS<double>;
+
+struct Triangle {
+  std::size_t vids[3] = {0, 0, 0};
+  Triangle() = default;
+  Triangle(std::size_t a, std::size_t b, std::size_t c) : vids{a, b, c} {}
+  void set(std::size_t a, std::size_t b, std::size_t c) {
+    vids[0] = a;
+    vids[1] = b;
+    vids[2] = c;
+  }
+  std::size_t operator[](int i) const { return vids[i]; }
+  std::size_t& operator[](int i) { return vids[i]; }
+};
+
+namespace detail {
+// device-resident copy of a built hierarchy (released with the last owner)
+struct DeviceMesh;
+int upload_mesh(int bv_type, const std::vector<Vector3d>& vertices,
+                const std::vector<Triangle>& tris, std::shared_ptr<DeviceMesh>& out);
+int device_mesh_id(const DeviceMesh* m);
+}  // namespace detail
+
+// BVHModel<BV>: beginModel / addVertex / addTriangle / addSubModel /
+// endModel as in BVH_model-inl.h:206-517; endModel() builds the hierarchy
+// (mean split) on the host and uploads it.
+template <typename BV>
+class BVHModel : public CollisionGeometry<typename BV::S> {
+ public:
+  using S = typename BV::S;
+  BVHModel() = default;
+  // copies share the immutable device hierarchy; raw arrays point at the copy's storage
+  BVHModel(const BVHModel& o)
+      : CollisionGeometry<S>(o), build_state(o.build_state), verts_(o.verts_), tris_(o.tris_),
+        dev_(o.dev_) {
+    sync();
+  }
+  BVHModel& operator=(const BVHModel& o) {
+    if (this != &o) {
+      CollisionGeometry<S>::operator=(o);
+      build_state = o.build_state;
+      verts_ = o.verts_;
+      tris_ = o.tris_;
+      dev_ = o.dev_;
+      sync();
+    }
+    return *this;
+  }
+
+  OBJECT_TYPE getObjectType() const override { return OT_BVH; }
+  NODE_TYPE getNodeType() const override { return BV::node_type; }
+  BVHModelType getModelType() const {
+    if (num_tris && num_vertices) return BVH_MODEL_TRIANGLES;
+    if (num_vertices) return BVH_MODEL_POINTCLOUD;
+    return BVH_MODEL_UNKNOWN;
+  }
+  int getNumBVs() const { return build_state == BVH_BUILD_STATE_PROCESSED ? 2 * num_tris - 1 : 0; }
+
+  int beginModel(int num_tris_ = 0, int num_vertices_ = 0) {
+    const bool was_empty = build_state == BVH_BUILD_STATE_EMPTY;
+    verts_.clear();
+    tris_.clear();
+    dev_.reset();
+    if (num_tris_ > 0) tris_.reserve(num_tris_);
+    if (num_vertices_ > 0) verts_.reserve(num_vertices_);
+    sync();
+    if (!was_empty) {
+      build_state = BVH_BUILD_STATE_EMPTY;
+      return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    }
+    build_state = BVH_BUILD_STATE_BEGUN;
+    return BVH_OK;
+  }
+  int addVertex(const Vector3<S>& p) {
+    if (build_state != BVH_BUILD_STATE_BEGUN) return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    verts_.push_back(p);
+    sync();
+    return BVH_OK;
+  }
+  int addTriangle(const Vector3<S>& p1, const Vector3<S>& p2, const Vector3<S>& p3) {
+    if (build_state == BVH_BUILD_STATE_PROCESSED) return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    const std::size_t o = verts_.size();
+    verts_.push_back(p1);
+    verts_.push_back(p2);
+    verts_.push_back(p3);
+    tris_.emplace_back(o, o + 1, o + 2);
+    sync();
+    return BVH_OK;
+  }
+  int addSubModel(const std::vector<Vector3<S>>& ps, const std::vector<Triangle>& ts) {
+    if (build_state == BVH_BUILD_STATE_PROCESSED) return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    const std::size_t o = verts_.size();
+    verts_.insert(verts_.end(), ps.begin(), ps.end());
+    for (const Triangle& t : ts) tris_.emplace_back(t[0] + o, t[1] + o, t[2] + o);
+    sync();
+    return BVH_OK;
+  }
+  int addSubModel(const std::vector<Vector3<S>>& ps) {
+    if (build_state == BVH_BUILD_STATE_PROCESSED) return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    verts_.insert(verts_.end(), ps.begin(), ps.end());
+    sync();
+    return BVH_OK;
+  }
+  int endModel() {
+    if (build_state != BVH_BUILD_STATE_BEGUN) return BVH_ERR_BUILD_OUT_OF_SEQUENCE;
+    if (tris_.empty() && verts_.empty()) return BVH_ERR_BUILD_EMPTY_MODEL;
+    if (tris_.empty()) return BVH_ERR_UNSUPPORTED_FUNCTION;  // point clouds: out of scope
+    const int rc = detail::upload_mesh(BV::node_type == BV_OBBRSS ? 0 : 1, verts_, tris_, dev_);
+    if (rc != BVH_OK) return rc;
+    build_state = BVH_BUILD_STATE_PROCESSED;
+    return BVH_OK;
+  }
+
+  // device handle of the built model (-1 before endModel)
+  int deviceMeshId() const { return dev_ ? detail::device_mesh_id(dev_.get()) : -1; }
+
+  // the reference exposes the raw arrays; these point into the model's storage
+  Vector3<S>* vertices = nullptr;
+  Triangle* tri_indices = nullptr;
+  int num_tris = 0;
+  int num_vertices = 0;
+  BVHBuildState build_state = BVH_BUILD_STATE_EMPTY;
+
+ private:
+  void sync() {
+    vertices = verts_.empty() ? nullptr : verts_.data();
+    tri_indices = tris_.empty() ? nullptr : tris_.data();
+    num_tris = static_cast<int>(tris_.size());
+    num_vertices = static_cast<int>(verts_.size());
+  }
+  std::vector<Vector3<S>> verts_;
+  std::vector<Triangle> tris_;
+  std::shared_ptr<detail::DeviceMesh> dev_;
+};
+
+// ------------------------------------------------------ requests/results
+template <typename S>
+struct Contact {
+  const CollisionGeometry<S>* o1 = nullptr;
+  const CollisionGeometry<S>* o2 = nullptr;
+  intptr_t b1 = NONE;
+  intptr_t b2 = NONE;
+  Vector3<S> normal;
+  Vector3<S> pos;
+  S penetration_depth = 0;
+  static const int NONE = -1;
+  Contact() = default;
+  Contact(const CollisionGeometry<S>* o1_, const CollisionGeometry<S>* o2_, int b1_, int b2_)
+      : o1(o1_), o2(o2_), b1(b1_), b2(b2_) {}
+  bool operator<(const Contact& o) const {
+    if (b1 == o.b1) return b2 < o.b2;
+    return b1 < o.b1;
+  }
+  bool operator==(const Contact& o) const {
+    return o1 == o.o1 && o2 == o.o2 && b1 == o.b1 && b2 == o.b2;
+  }
+};
+
+template <typename S>
+class CollisionResult;
+
+template <typename S>
+struct CollisionRequest {
+  std::size_t num_max_contacts;
+  bool enable_contact;
+  std::size_t num_max_cost_sources;
+  bool enable_cost;
+  bool use_approximate_cost;
+  GJKSolverType gjk_solver_type;
+  bool enable_cached_gjk_guess = false;
+  Vector3<S> cached_gjk_guess = Vector3<S>(1, 0, 0);
+  S gjk_tolerance{1e-6};
+  CollisionRequest(std::size_t num_max_contacts_ = 1, bool enable_contact_ = false,
+                   std::size_t num_max_cost_sources_ = 1, bool enable_cost_ = false,
+                   bool use_approximate_cost_ = true,
+                   GJKSolverType gjk_solver_type_ = GST_LIBCCD, S gjk_tolerance_ = 1e-6)
+      : num_max_contacts(num_max_contacts_), enable_contact(enable_contact_),
+        num_max_cost_sources(num_max_cost_sources_), enable_cost(enable_cost_),
+        use_approximate_cost(use_approximate_cost_), gjk_solver_type(gjk_solver_type_),
+        gjk_tolerance(gjk_tolerance_) {}
+  // collision_request-inl.h:77-82
+  bool isSatisfied(const CollisionResult<S>& result) const {
+    return (!enable_cost) && result.isCollision() && (num_max_contacts <= result.numContacts());
+  }
+};
+
+template <typename S>
+class CollisionResult {
+ public:
+  void addContact(const Contact<S>& c) { contacts.push_back(c); }
+  bool isCollision() const { return !contacts.empty(); }
+  std::size_t numContacts() const { return contacts.size(); }
+  const Contact<S>& getContact(std::size_t i) const {
+    return i < contacts.size() ? contacts[i] : contacts.back();
+  }
+  void getContacts(std::vector<Contact<S>>& out) const { out = contacts; }
+  void clear() { contacts.clear(); }
+  std::vector<Contact<S>> contacts;
+};
+
+template <typename S>
+class DistanceResult;
+
+template <typename S>
+struct DistanceRequest {
+  bool enable_nearest_points;
+  bool enable_signed_distance;
+  S rel_err;
+  S abs_err;
+  S distance_tolerance;
+  GJKSolverType gjk_solver_type;
+  explicit DistanceRequest(bool enable_nearest_points_ = false,
+                           bool enable_signed_distance_ = false, S rel_err_ = 0.0,
+                           S abs_err_ = 0.0, S distance_tolerance_ = 1e-6,
+                           GJKSolverType gjk_solver_type_ = GST_LIBCCD)
+      : enable_nearest_points(enable_nearest_points_),
+        enable_signed_distance(enable_signed_distance_), rel_err(rel_err_), abs_err(abs_err_),
+        distance_tolerance(distance_tolerance_), gjk_solver_type(gjk_solver_type_) {}
+  // distance_request-inl.h:73-77
+  bool isSatisfied(const DistanceResult<S>& result) const { return result.min_distance <= 0; }
+};
+
+template <typename S>
+class DistanceResult {
+ public:
+  S min_distance = std::numeric_limits<S>::max();
+  Vector3<S> nearest_points[2];
+  const CollisionGeometry<S>* o1 = nullptr;
+  const CollisionGeometry<S>* o2 = nullptr;
+  intptr_t b1 = NONE;
+  intptr_t b2 = NONE;
+  static const int NONE = -1;
+  // distance_result-inl.h:58-105: keep the first strict improvement
+  void update(S distance, const CollisionGeometry<S>* o1_, const CollisionGeometry<S>* o2_,
+              int b1_, int b2_) {
+    if (min_distance > distance) {
+      min_distance = distance;
+      o1 = o1_;
+      o2 = o2_;
+      b1 = b1_;
+      b2 = b2_;
+    }
+  }
+  void update(S distance, const CollisionGeometry<S>* o1_, const CollisionGeometry<S>* o2_,
+              int b1_, int b2_, const Vector3<S>& p1, const Vector3<S>& p2) {
+    if (min_distance > distance) {
+      min_distance = distance;
+      o1 = o1_;
+      o2 = o2_;
+      b1 = b1_;
+      b2 = b2_;
+      nearest_points[0] = p1;
+      nearest_points[1] = p2;
+    }
+  }
+  void clear() { *this = DistanceResult(); }
+};
+
+template <typename S>
+class CollisionObject {
+ public:
+  explicit CollisionObject(const std::shared_ptr<CollisionGeometry<S>>& geom,
+                           const Transform3<S>& tf = Transform3<S>::Identity())
+      : geom_(geom), tf_(tf) {}
+  const std::shared_ptr<CollisionGeometry<S>>& collisionGeometry() const { return geom_; }
+  const Transform3<S>& getTransform() const { return tf_; }
+  void setTransform(const Transform3<S>& tf) { tf_ = tf; }
+  OBJECT_TYPE getObjectType() const { return geom_->getObjectType(); }
+  NODE_TYPE getNodeType() const { return geom_->getNodeType(); }
+
+ private:
+  std::shared_ptr<CollisionGeometry<S>> geom_;
+  Transform3<S> tf_;
+};
+
+// ------------------------------------------------------------- queries
+template <typename S>
+std::size_t collide(const CollisionGeometry<S>* o1, const Transform3<S>& tf1,
+                    const CollisionGeometry<S>* o2, const Transform3<S>& tf2,
+                    const CollisionRequest<S>& request, CollisionResult<S>& result);
+template <typename S>
+std::size_t collide(const CollisionObject<S>* o1, const CollisionObject<S>* o2,
+                    const CollisionRequest<S>& request, CollisionResult<S>& result) {
+  return collide(o1->collisionGeometry().get(), o1->getTransform(), o2->collisionGeometry().get(),
+                 o2->getTransform(), request, result);
+}
+
+template <typename S>
+S distance(const CollisionGeometry<S>* o1, const Transform3<S>& tf1,
+           const CollisionGeometry<S>* o2, const Transform3<S>& tf2,
+           const DistanceRequest<S>& request, DistanceResult<S>& result);
+template <typename S>
+S distance(const CollisionObject<S>* o1, const CollisionObject<S>* o2,
+           const DistanceRequest<S>& request, DistanceResult<S>& result) {
+  return distance(o1->collisionGeometry().get(), o1->getTransform(),
+                  o2->collisionGeometry().get(), o2->getTransform(), request, result);
+}
+
+// the double instantiations live in libfclx.so
+template <>
+std::size_t collide<double>(const CollisionGeometry<double>*, const Transform3d&,
+                            const CollisionGeometry<double>*, const Transform3d&,
+                            const CollisionRequest<double>&, CollisionResult<double>&);
+template <>
+double distance<double>(const CollisionGeometry<double>*, const Transform3d&,
+                        const CollisionGeometry<double>*, const Transform3d&,
+                        const DistanceRequest<double>&, DistanceResult<double>&);
+
+// ------------------------------------------------------------- batched
+// Many mesh-mesh queries in one engine launch (no counterpart in the
+// reference).  results[i] receives query i's contacts as collide() would
+// append them (the set; for capped queries the reference's depth-first
+// prefix).  Returns the number of queries processed, or (size_t)-1 when a
+// query names unsupported geometry.
+struct CollideQuery {
+  const CollisionGeometry<double>* o1;
+  Transform3d tf1;
+  const CollisionGeometry<double>* o2;
+  Transform3d tf2;
+};
+std::size_t collideBatch(const std::vector<CollideQuery>& queries,
+                         const CollisionRequest<double>& request,
+                         std::vector<CollisionResult<double>>& results);
+
+}  // namespace fcl
+
+#endif

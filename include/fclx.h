@@ -161,11 +161,14 @@ int fclx_sync(fclx_context* ctx, void* stream);
 /* One query with host buffers: the drop-in path behind fcl::collide for two
  * BVHModel<OBBRSS> (orientedMeshCollide, collision_func_matrix-inl.h:571-590).
  * cap = num_max_contacts minus contacts already in the caller's result.
- * pairs receives up to cap (b1, b2) pairs in the reference's order;
+ * *num_contacts receives the number of contacts (<= cap); the first
+ * min(*num_contacts, pairs_capacity) (b1, b2) pairs, in the reference's
+ * order, go to pairs.  Returns FCLX_ERR_CAPACITY when pairs was too small
+ * (call again with room for *num_contacts pairs).
  * stats (optional) = {num_bv_tests, num_leaf_tests}. */
 int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
-                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t* num_contacts,
-                 int64_t* stats);
+                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t pairs_capacity,
+                 int32_t* num_contacts, int64_t* stats);
 /* One query: the drop-in path behind fcl::distance for two BVHModel<RSS> or
  * BVHModel<OBBRSS> (orientedMeshDistance, distance_func_matrix-inl.h:386-404).
  * points (optional) = nearest points on mesh1 and mesh2 in the world frame;
