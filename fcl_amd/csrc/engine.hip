@@ -116,9 +116,19 @@ __device__ __forceinline__ void make_xf32(const double* R, const double* T, doub
                             (1.0 + 1.0 / 1048576.0));
 }
 
+// Node and size arrays of a query's two meshes, read in parallel with the
+// query's transform so no mesh-descriptor load sits on the critical path.
+struct QueryPtr {
+  const ObbNode32* n1;
+  const ObbNode32* n2;
+  const double* s1;
+  const double* s2;
+};
+
 struct EngineArgs {
   CollideLaunch L;
   QueryXf32* qx32;
+  QueryPtr* qptr;
   QueryXf64* qx64;
   int32_t* qflag;
   unsigned long long* nflag;
@@ -165,6 +175,8 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
       X.m1 = i1;
       X.m2 = i2;
       X.valid = 1;
+      A.qptr[q] = QueryPtr{L.meshes[i1].obb32, L.meshes[i2].obb32, L.meshes[i1].size64,
+                           L.meshes[i2].size64};
     } else {
       X = QueryXf32{};
       X.valid = 0;
@@ -176,18 +188,54 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
   if (valid) A.front_out[slot] = pack_task(q, 0, 0);
 }
 
-// BV test of task (q,b1,b2): FP32 cull, FP64 only when unsure.
-__device__ __forceinline__ bool bv_overlap(const EngineArgs& A, const QueryXf32& X, int q,
-                                           const MeshDev& m1, const MeshDev& m2, int b1,
-                                           int b2, bool& fallback) {
-  const int r = obb_cull32(X, m1.obb32[b1], m2.obb32[b2]);
-  fallback = r == kCullUnsure;
-  if (!fallback) return r == kCullOverlap;
-  const QueryXf64& x = A.qx64[q];
-  M3 R;
+// Outcome of testing one BVTT task.
+struct Visit {
+  bool push = false;   // overlapping internal pair: children c0, c1
+  bool leafc = false;  // overlapping leaf pair: triangle test pending
+  bool fallback = false;
+  unsigned long long c0 = 0, c1 = 0;
+};
+
+// BV test of task (q,b1,b2): FP32 cull, FP64 only when unsure; then the
+// reference's descent rule (firstOverSecond) for the children.  The query
+// record, both nodes and both sizes are loaded before any is used.
+__device__ __forceinline__ Visit visit_task(const EngineArgs& A, unsigned long long task) {
+  int q, b1, b2;
+  unpack_task(task, q, b1, b2);
+  const QueryPtr P = A.qptr[q];
+  const QueryXf32& X = A.qx32[q];
+  const ObbNode32& n1 = P.n1[b1];
+  const ObbNode32& n2 = P.n2[b2];
+  if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+  Visit v;
+  int r = obb_cull32(X, n1, n2);
+  if (r == kCullUnsure) {
+    v.fallback = true;
+    const QueryXf64& x = A.qx64[q];
+    M3 R;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
-  return obb_overlap64(R, v3(x.T[0], x.T[1], x.T[2]), m1.obb64[b1], m2.obb64[b2]);
+    for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
+    const MeshDev& m1 = A.L.meshes[X.m1];
+    const MeshDev& m2 = A.L.meshes[X.m2];
+    r = obb_overlap64(R, v3(x.T[0], x.T[1], x.T[2]), m1.obb64[b1], m2.obb64[b2]) ? kCullOverlap
+                                                                                   : kCullDisjoint;
+  }
+  if (r == kCullOverlap) {
+    const int fc1 = n1.first_child, fc2 = n2.first_child;
+    if (fc1 < 0 && fc2 < 0) {
+      v.leafc = true;
+    } else {
+      v.push = true;
+      if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+        v.c0 = pack_task(q, fc1, b2);
+        v.c1 = pack_task(q, fc1 + 1, b2);
+      } else {
+        v.c0 = pack_task(q, b1, fc2);
+        v.c1 = pack_task(q, b1, fc2 + 1);
+      }
+    }
+  }
+  return v;
 }
 
 // Record one intersecting pair; flags the query once its count reaches the cap.
@@ -234,41 +282,21 @@ __device__ __forceinline__ bool leaf_hit(const EngineArgs& A, unsigned long long
 __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = i < A.n_in;
-  bool push = false, leafc = false, fb = false;
-  unsigned long long c0 = 0, c1 = 0, task = 0;
+  unsigned long long task = 0;
+  Visit v;
   if (active) {
     task = A.front_in[i];
-    int q, b1, b2;
-    unpack_task(task, q, b1, b2);
-    const QueryXf32 X = A.qx32[q];
-    const MeshDev& m1 = A.L.meshes[X.m1];
-    const MeshDev& m2 = A.L.meshes[X.m2];
-    if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
-    if (bv_overlap(A, X, q, m1, m2, b1, b2, fb)) {
-      const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
-      if (fc1 < 0 && fc2 < 0) {
-        leafc = true;
-      } else {
-        push = true;
-        if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
-          c0 = pack_task(q, fc1, b2);
-          c1 = pack_task(q, fc1 + 1, b2);
-        } else {
-          c0 = pack_task(q, b1, fc2);
-          c1 = pack_task(q, b1, fc2 + 1);
-        }
-      }
-    }
+    v = visit_task(A, task);
   }
   wave_count(A.work + kWorkExpandBV, active);
-  wave_count(A.work + kWorkFallback, fb);
-  const unsigned long long so = wave_reserve(A.n_out, push, 2);
-  if (push) {
-    A.front_out[so] = c0;
-    A.front_out[so + 1] = c1;
+  wave_count(A.work + kWorkFallback, v.fallback);
+  const unsigned long long so = wave_reserve(A.n_out, v.push, 2);
+  if (v.push) {
+    A.front_out[so] = v.c0;
+    A.front_out[so + 1] = v.c1;
   }
-  const unsigned long long sl = wave_reserve(A.n_leafc, leafc, 1);
-  if (leafc) A.leafc[sl] = task;
+  const unsigned long long sl = wave_reserve(A.n_leafc, v.leafc, 1);
+  if (v.leafc) A.leafc[sl] = task;
 }
 
 __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
@@ -322,33 +350,12 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     const bool active = lane < n;
     unsigned long long task = 0;
     if (active) task = stack[sp + lane];
-    bool push = false, leafc = false, fb = false;
-    unsigned long long c0 = 0, c1 = 0;
     n_bv += n;
-    if (active) {
-      int q, b1, b2;
-      unpack_task(task, q, b1, b2);
-      const QueryXf32 X = A.qx32[q];
-      const MeshDev& m1 = A.L.meshes[X.m1];
-      const MeshDev& m2 = A.L.meshes[X.m2];
-      if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
-      if (bv_overlap(A, X, q, m1, m2, b1, b2, fb)) {
-        const int fc1 = m1.obb32[b1].first_child, fc2 = m2.obb32[b2].first_child;
-        if (fc1 < 0 && fc2 < 0) {
-          leafc = true;
-        } else {
-          push = true;
-          if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
-            c0 = pack_task(q, fc1, b2);
-            c1 = pack_task(q, fc1 + 1, b2);
-          } else {
-            c0 = pack_task(q, b1, fc2);
-            c1 = pack_task(q, b1, fc2 + 1);
-          }
-        }
-      }
-    }
-    n_fb += __popcll(__ballot(fb));
+    Visit v;
+    if (active) v = visit_task(A, task);
+    const bool push = v.push, leafc = v.leafc;
+    const unsigned long long c0 = v.c0, c1 = v.c1;
+    n_fb += __popcll(__ballot(v.fallback));
     // children onto the stack (one prefix sum)
     const unsigned long long pm = __ballot(push);
     const int npush = 2 * __popcll(pm);
@@ -393,23 +400,78 @@ __global__ void engine_collect_flagged(int32_t* qflag, int nq, int32_t* list,
   }
 }
 
-// records per flagged query
-__global__ void engine_count_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
-                                     int32_t* seg_count) {
+// positions of the records of flagged queries (one atomic per wavefront)
+__global__ void engine_find_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
+                                    long long* pos, unsigned long long* n) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nrec) return;
-  const int f = qflag[rec[3 * i]];
-  if (f) atomicAdd(&seg_count[f - 1], 1);
+  const bool f = i < nrec && qflag[rec[3 * i]] != 0;
+  const unsigned long long s = wave_reserve(n, f, 1);
+  if (f) pos[s] = i;
 }
 
-// group the flagged queries' record indices by query
-__global__ void engine_group_flagged(const int32_t* rec, long long nrec, const int32_t* qflag,
-                                     const long long* seg_off, int32_t* fill,
-                                     long long* grouped) {
+// segment (flagged-query index) of each flagged record
+__global__ void engine_segment_of(const int32_t* rec, const long long* pos, long long n,
+                                  const int32_t* qflag, uint32_t* seg) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nrec) return;
-  const int f = qflag[rec[3 * i]];
-  if (f) grouped[seg_off[f - 1] + atomicAdd(&fill[f - 1], 1)] = i;
+  if (i < n) seg[i] = static_cast<uint32_t>(qflag[rec[3 * pos[i]]] - 1);
+}
+
+// flagged positions below `limit` (holes to refill)
+__global__ void engine_holes(const long long* pos, long long n, long long limit, long long* out,
+                             unsigned long long* cnt) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool h = i < n && pos[i] < limit;
+  const unsigned long long s = wave_reserve(cnt, h, 1);
+  if (h) out[s] = pos[i];
+}
+
+// unflagged positions in [from, nrec) (records to move into holes)
+__global__ void engine_tail_keep(const int32_t* rec, long long from, long long nrec,
+                                 const int32_t* qflag, long long* out, unsigned long long* cnt) {
+  const long long i = from + static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool k = i < nrec && qflag[rec[3 * i]] == 0;
+  const unsigned long long s = wave_reserve(cnt, k, 1);
+  if (k) out[s] = i;
+}
+
+// stage the records that will fill the holes: truncated lists first
+// (segment-major, depth-first order), then the kept tail records
+__global__ void engine_stage_fill(const int32_t* rec, const long long* sorted,
+                                  const long long* seg_off, const long long* out_off,
+                                  const int32_t* take, int nseg, const long long* tail,
+                                  long long ntail, long long ntake, int32_t* stage) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < ntake) {
+    // segment of output slot i: last seg with out_off[seg] <= i
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (out_off[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const long long j = i - out_off[lo];
+    if (j < take[lo]) {
+      const int32_t* r = rec + 3 * sorted[seg_off[lo] + j];
+      stage[3 * i] = r[0];
+      stage[3 * i + 1] = r[1];
+      stage[3 * i + 2] = r[2];
+    }
+  } else if (i < ntake + ntail) {
+    const int32_t* r = rec + 3 * tail[i - ntake];
+    stage[3 * i] = r[0];
+    stage[3 * i + 1] = r[1];
+    stage[3 * i + 2] = r[2];
+  }
+}
+
+__global__ void engine_fill_holes(const int32_t* stage, const long long* holes, long long n,
+                                  int32_t* rec) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long o = holes[i];
+  rec[3 * o] = stage[3 * i];
+  rec[3 * o + 1] = stage[3 * i + 1];
+  rec[3 * o + 2] = stage[3 * i + 2];
 }
 
 // Depth-first position of a leaf pair: the BVTT path from (0,0) to the
@@ -446,24 +508,11 @@ __global__ void engine_leaf_keys(const int32_t* rec, const long long* grouped, l
   keys[i] = key;
 }
 
-// first `take` sorted records of each segment -> out[base + out_off[seg] ...]
-__global__ void engine_emit_prefix(const int32_t* rec, const long long* sorted,
-                                   const long long* seg_off, const long long* out_off,
-                                   const int32_t* take, int nseg, int32_t* out, long long base,
-                                   long long cap, int32_t* status) {
-  const int seg = blockIdx.x;
-  if (seg >= nseg) return;
-  for (int j = threadIdx.x; j < take[seg]; j += blockDim.x) {
-    const long long o = base + out_off[seg] + j;
-    if (o >= cap) {
-      atomicOr(status, kStatusPairOverflow);
-      return;
-    }
-    const int32_t* r = rec + 3 * sorted[seg_off[seg] + j];
-    out[3 * o] = r[0];
-    out[3 * o + 1] = r[1];
-    out[3 * o + 2] = r[2];
-  }
+
+__global__ void engine_gather_counts(const int32_t* list, int n, const int32_t* counts,
+                                     int32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = counts[list[i]];
 }
 
 __global__ void engine_set_counts(const int32_t* list, const int32_t* take, int n,
@@ -528,7 +577,6 @@ __global__ void engine_append_records(const int32_t* rec, long long nrec, const 
 }
 
 __global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
-__global__ void or_status(int32_t* status, int32_t bits) { atomicOr(status, bits); }
 
 // ---------------------------------------------- ordered, small caps
 template <int STACK>
@@ -679,6 +727,8 @@ hipError_t grow(T*& p, size_t& cap, size_t need) {
 struct EngineWorkspace {
   QueryXf32* qx32 = nullptr;
   size_t qx32_cap = 0;
+  QueryPtr* qptr = nullptr;
+  size_t qptr_cap = 0;
   QueryXf64* qx64 = nullptr;
   size_t qx64_cap = 0;
   int32_t* qflag = nullptr;
@@ -716,6 +766,7 @@ EngineWorkspace* engine_workspace_create() {
 void engine_workspace_destroy(EngineWorkspace* ws) {
   if (!ws) return;
   (void)hipFree(ws->qx32);
+  (void)hipFree(ws->qptr);
   (void)hipFree(ws->qx64);
   (void)hipFree(ws->qflag);
   (void)hipFree(ws->front[0]);
@@ -860,8 +911,11 @@ struct Scratch {
 };
 
 // Truncate each flagged query's recorded contact set to its first `cap`
-// pairs in depth-first order (keys from engine_leaf_keys, one segmented
-// radix sort), replacing the query's records in the output buffer.
+// pairs in depth-first order and rewrite the output buffer in place.  Work is
+// proportional to the flagged records F (one scan of the buffer finds them):
+// path keys (engine_leaf_keys), a radix sort by key then a stable one by
+// query, and the freed slots below the new end are refilled with the
+// truncated lists and with the unflagged records from past the new end.
 hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long long nflag,
                       hipStream_t s) {
   const int nq = L.num_queries;
@@ -869,96 +923,110 @@ hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long
   unsigned long long produced = 0;
   ENG_TRY(hipMemcpyAsync(&produced, L.num_pairs, sizeof(produced), hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
-  const long long have = std::min<long long>(static_cast<long long>(produced), L.pair_capacity);
   Scratch S;
-  int32_t *list, *seg_count, *fill, *take;
+  int32_t *list, *seg_count, *take;
   long long *seg_off, *out_off;
   ENG_TRY(S.alloc(list, nf));
   ENG_TRY(S.alloc(seg_count, nf));
-  ENG_TRY(S.alloc(fill, nf));
   ENG_TRY(S.alloc(take, nf));
   ENG_TRY(S.alloc(seg_off, nf));
   ENG_TRY(S.alloc(out_off, nf));
   set_u64<<<1, 1, 0, s>>>(ws->counters + 5, 0);
   engine_collect_flagged<<<grid_for(nq), kBlock, 0, s>>>(ws->qflag, nq, list, ws->counters + 5);
-  ENG_TRY(hipMemsetAsync(seg_count, 0, sizeof(int32_t) * nf, s));
-  ENG_TRY(hipMemsetAsync(fill, 0, sizeof(int32_t) * nf, s));
-  if (have > 0)
-    engine_count_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, seg_count);
+  engine_gather_counts<<<grid_for(nf), kBlock, 0, s>>>(list, nf, L.counts, seg_count);
   std::vector<int32_t> h_count(nf), h_take(nf);
   std::vector<long long> h_off(nf), h_out(nf);
-  std::vector<int> h_begin(nf), h_end(nf);
   ENG_TRY(hipMemcpyAsync(h_count.data(), seg_count, sizeof(int32_t) * nf, hipMemcpyDeviceToHost,
                          s));
   ENG_TRY(hipStreamSynchronize(s));
-  long long total = 0, total_take = 0;
+  long long F = 0, T = 0;
   for (int i = 0; i < nf; ++i) {
-    h_off[i] = total;
-    h_begin[i] = static_cast<int>(total);
-    total += h_count[i];
-    h_end[i] = static_cast<int>(total);
+    h_off[i] = F;
+    F += h_count[i];
     h_take[i] = std::min(h_count[i], L.cap);
-    h_out[i] = total_take;
-    total_take += h_take[i];
+    h_out[i] = T;
+    T += h_take[i];
   }
-  if (total >= (1LL << 31)) return hipErrorInvalidValue;  // sort offsets are int
+  if (static_cast<long long>(produced) > L.pair_capacity || !L.pairs) {
+    // records were dropped (capacity status already set): report capped
+    // counts and leave the buffer for the caller's retry
+    ENG_TRY(hipMemcpyAsync(take, h_take.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
+    engine_set_counts<<<grid_for(nf), kBlock, 0, s>>>(list, take, nf, L.counts);
+    ENG_TRY(hipStreamSynchronize(s));
+    return hipGetLastError();
+  }
+  if (F >= (1LL << 31)) return hipErrorInvalidValue;  // radix sort item count is int
+  const long long N = static_cast<long long>(produced);
+  const long long Nn = N - F + T;  // new end of the buffer
   ENG_TRY(hipMemcpyAsync(seg_off, h_off.data(), sizeof(long long) * nf, hipMemcpyHostToDevice,
                          s));
   ENG_TRY(hipMemcpyAsync(out_off, h_out.data(), sizeof(long long) * nf, hipMemcpyHostToDevice,
                          s));
   ENG_TRY(hipMemcpyAsync(take, h_take.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
-  long long *grouped, *sorted;
+  long long *pos, *pos_a, *pos_b, *holes, *tail;
   unsigned long long *keys, *keys_sorted;
-  int *d_begin, *d_end;
-  ENG_TRY(S.alloc(grouped, total));
-  ENG_TRY(S.alloc(sorted, total));
-  ENG_TRY(S.alloc(keys, total));
-  ENG_TRY(S.alloc(keys_sorted, total));
-  ENG_TRY(S.alloc(d_begin, nf));
-  ENG_TRY(S.alloc(d_end, nf));
-  ENG_TRY(hipMemcpyAsync(d_begin, h_begin.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
-  ENG_TRY(hipMemcpyAsync(d_end, h_end.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
-  if (total > 0) {
-    engine_group_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, seg_off,
-                                                           fill, grouped);
-    engine_leaf_keys<<<grid_for(total), kBlock, 0, s>>>(L.pairs, grouped, total, L, keys);
-    size_t tmp_bytes = 0;
-    ENG_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        nullptr, tmp_bytes, keys, keys_sorted, grouped, sorted, static_cast<int>(total), nf,
-        d_begin, d_end, 0, 64, s));
-    unsigned char* tmp;
-    ENG_TRY(S.alloc(tmp, tmp_bytes));
-    ENG_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        tmp, tmp_bytes, keys, keys_sorted, grouped, sorted, static_cast<int>(total), nf, d_begin,
-        d_end, 0, 64, s));
-  }
-  // keep every other query's records, then append the truncated lists
-  unsigned long long nkept = 0;
-  int32_t* kept;
-  ENG_TRY(S.alloc(kept, 3 * std::max<long long>(have, 1)));
+  uint32_t *seg, *seg_sorted;
+  ENG_TRY(S.alloc(pos, F));
+  ENG_TRY(S.alloc(pos_a, F));
+  ENG_TRY(S.alloc(pos_b, F));
+  ENG_TRY(S.alloc(keys, F));
+  ENG_TRY(S.alloc(keys_sorted, F));
+  ENG_TRY(S.alloc(seg, F));
+  ENG_TRY(S.alloc(seg_sorted, F));
+  ENG_TRY(S.alloc(holes, F));
+  ENG_TRY(S.alloc(tail, F));
+  // 1. where the flagged records are
   set_u64<<<1, 1, 0, s>>>(ws->counters + 6, 0);
-  if (have > 0)
-    engine_drop_flagged<<<grid_for(have), kBlock, 0, s>>>(L.pairs, have, ws->qflag, kept,
-                                                          ws->counters + 6);
-  ENG_TRY(read_counter(ws, 6, nkept, s));
-  int32_t* prefix;
-  ENG_TRY(S.alloc(prefix, 3 * std::max<long long>(total_take, 1)));
-  if (total_take > 0)
-    engine_emit_prefix<<<nf, kBlock, 0, s>>>(L.pairs, sorted, seg_off, out_off, take, nf, prefix,
-                                             0, total_take, L.status);
-  if (nkept > 0)
-    ENG_TRY(hipMemcpyAsync(L.pairs, kept, sizeof(int32_t) * 3 * nkept, hipMemcpyDeviceToDevice,
-                           s));
-  const long long room = L.pair_capacity - static_cast<long long>(nkept);
-  const long long fit = std::max<long long>(0, std::min(room, total_take));
-  if (fit > 0)
-    ENG_TRY(hipMemcpyAsync(L.pairs + 3 * nkept, prefix, sizeof(int32_t) * 3 * fit,
-                           hipMemcpyDeviceToDevice, s));
-  if (fit < total_take) or_status<<<1, 1, 0, s>>>(L.status, kStatusPairOverflow);
+  engine_find_flagged<<<grid_for(N), kBlock, 0, s>>>(L.pairs, N, ws->qflag, pos, ws->counters + 6);
+  unsigned long long found = 0;
+  ENG_TRY(read_counter(ws, 6, found, s));
+  if (static_cast<long long>(found) != F) return hipErrorUnknown;  // counts disagree with records
+  // 2. their depth-first keys, ordered within each query
+  if (F > 0) {
+    engine_leaf_keys<<<grid_for(F), kBlock, 0, s>>>(L.pairs, pos, F, L, keys);
+    int seg_bits = 1;
+    while ((1 << seg_bits) < nf) ++seg_bits;
+    size_t b1 = 0, b2 = 0;
+    ENG_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, keys, keys_sorted, pos, pos_a,
+                                               static_cast<int>(F), 0, 64, s));
+    ENG_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b2, seg, seg_sorted, pos_a, pos_b,
+                                               static_cast<int>(F), 0, seg_bits, s));
+    unsigned char* tmp;
+    ENG_TRY(S.alloc(tmp, std::max(b1, b2)));
+    size_t bt = std::max(b1, b2);
+    ENG_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bt, keys, keys_sorted, pos, pos_a,
+                                               static_cast<int>(F), 0, 64, s));
+    // segment ids follow the key order: regather them for the stable sort
+    engine_segment_of<<<grid_for(F), kBlock, 0, s>>>(L.pairs, pos_a, F, ws->qflag, seg);
+    bt = std::max(b1, b2);
+    ENG_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bt, seg, seg_sorted, pos_a, pos_b,
+                                               static_cast<int>(F), 0, seg_bits, s));
+  }
+  // 3. slots freed below the new end, and the unflagged records beyond it
+  set_u64<<<1, 1, 0, s>>>(ws->counters + 6, 0);
+  set_u64<<<1, 1, 0, s>>>(ws->counters + 7, 0);
+  if (F > 0) engine_holes<<<grid_for(F), kBlock, 0, s>>>(pos, F, Nn, holes, ws->counters + 6);
+  if (N > Nn)
+    engine_tail_keep<<<grid_for(N - Nn), kBlock, 0, s>>>(L.pairs, Nn, N, ws->qflag, tail,
+                                                         ws->counters + 7);
+  unsigned long long nholes = 0, ntail = 0;
+  ENG_TRY(read_counter(ws, 6, nholes, s));
+  ENG_TRY(read_counter(ws, 7, ntail, s));
+  if (static_cast<long long>(nholes) != T + static_cast<long long>(ntail))
+    return hipErrorUnknown;  // bookkeeping invariant of the in-place rewrite
+  // 4. stage every record that moves, then write it into a freed slot
+  if (nholes > 0) {
+    int32_t* stage;
+    ENG_TRY(S.alloc(stage, 3 * nholes));
+    engine_stage_fill<<<grid_for(nholes), kBlock, 0, s>>>(L.pairs, pos_b, seg_off, out_off, take,
+                                                          nf, tail, static_cast<long long>(ntail),
+                                                          T, stage);
+    engine_fill_holes<<<grid_for(nholes), kBlock, 0, s>>>(stage, holes,
+                                                          static_cast<long long>(nholes), L.pairs);
+  }
   engine_set_counts<<<grid_for(nf), kBlock, 0, s>>>(list, take, nf, L.counts);
-  const unsigned long long lost = produced - static_cast<unsigned long long>(have);
   set_u64<<<1, 1, 0, s>>>(reinterpret_cast<unsigned long long*>(L.num_pairs),
-                          nkept + static_cast<unsigned long long>(total_take) + lost);
+                          static_cast<unsigned long long>(Nn));
   ENG_TRY(hipStreamSynchronize(s));
   return hipGetLastError();
 }
@@ -986,6 +1054,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   if (nq >= (1 << 22)) return hipErrorInvalidValue;  // task packing limit
   const EngineTuning& tune = engine_tuning();
   ENG_TRY(grow(ws->qx32, ws->qx32_cap, nq));
+  ENG_TRY(grow(ws->qptr, ws->qptr_cap, nq));
   ENG_TRY(grow(ws->qx64, ws->qx64_cap, nq));
   ENG_TRY(grow(ws->qflag, ws->qflag_cap, nq));
   ENG_TRY(grow(ws->front[0], ws->front_cap[0], std::max<size_t>(nq, 1024)));
@@ -993,6 +1062,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   EngineArgs A{};
   A.L = L;
   A.qx32 = ws->qx32;
+  A.qptr = ws->qptr;
   A.qx64 = ws->qx64;
   A.qflag = ws->qflag;
   A.nflag = ws->counters + 4;
