@@ -722,6 +722,29 @@ hipError_t grow(T*& p, size_t& cap, size_t need) {
     if (e_ != hipSuccess) return e_;     \
   } while (0)
 
+// grow() for a buffer whose first `used` elements must survive: the new
+// allocation receives them (stream-ordered) before the old one is freed
+template <typename T>
+hipError_t grow_keep(T*& p, size_t& cap, size_t need, size_t used, hipStream_t s) {
+  if (need <= cap) return hipSuccess;
+  const size_t ncap = std::max(need, cap * 2);
+  T* np = nullptr;
+  ENG_TRY(hipMalloc(reinterpret_cast<void**>(&np), sizeof(T) * ncap));
+  hipError_t e = hipSuccess;
+  if (used > 0) {
+    e = hipMemcpyAsync(np, p, sizeof(T) * used, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(np);
+    return e;
+  }
+  (void)hipFree(p);
+  p = np;
+  cap = ncap;
+  return hipSuccess;
+}
+
 }  // namespace
 
 struct EngineWorkspace {
@@ -1090,7 +1113,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
        ++level) {
     const int nxt = cur ^ 1;
     ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], 2 * nfront));
-    ENG_TRY(grow(ws->leafc, ws->leafc_cap, nleafc_total + nfront));
+    ENG_TRY(grow_keep(ws->leafc, ws->leafc_cap, nleafc_total + nfront, nleafc_total, s));
     A.front_in = ws->front[cur];
     A.n_in = static_cast<long long>(nfront);
     A.front_out = ws->front[nxt];

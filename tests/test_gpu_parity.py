@@ -167,6 +167,34 @@ def test_env_rob_exhaustive(ctx):
         assert key(got) == key(bf)
 
 
+def test_fast_engine_fresh_context():
+    """FAST engine in a new context, whose work buffers start empty and grow
+    while the breadth-first phase accumulates leaf candidates over several
+    levels: the env/rob contact sets equal the oracle's (regression for
+    candidates lost when that buffer was reallocated)."""
+    (V1, T1), (V2, T2) = util.load_obj("env.obj"), util.load_obj("rob.obj")
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    ctx = F.Context(0)
+    try:
+        g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+        n = 10
+        tf1 = util.random_transforms(n, [-3000, -3000, 0, 3000, 3000, 3000], 2024)
+        tf2 = np.tile(util.identity_pose(), (n, 1))
+        print("case: fast engine, fresh context, env/rob", flush=True)
+        rc, gc, gp, gbv, glf, nrec = collide_gpu(ctx, np.tile(g, (n, 1)), tf1, tf2, 100000,
+                                                 engine=F.ENGINE_FAST, capacity=1 << 16)
+    finally:
+        ctx.close()
+    assert rc == 0
+    oc, op, obv, olf = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 100000,
+                                          store=4096)
+    assert np.array_equal(gc, oc) and nrec == oc.sum() and oc.sum() > 0
+    assert np.array_equal(gbv, obv) and np.array_equal(glf, olf)
+    for q in range(n):
+        got = gp.get(q, np.zeros((0, 2), np.int32))
+        assert _set(got) == _set(op[q, : oc[q]])
+
+
 def test_single_query_entry_point(ctx, c0_meshes):
     o, g = c0_meshes
     tf1, tf2 = F.poses(40, 0, 1.5, 0, 0xC2)
