@@ -1037,14 +1037,27 @@ hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long
   ENG_TRY(read_counter(ws, 7, ntail, s));
   if (static_cast<long long>(nholes) != T + static_cast<long long>(ntail))
     return hipErrorUnknown;  // bookkeeping invariant of the in-place rewrite
-  // 4. stage every record that moves, then write it into a freed slot
+  // 4. stage every record that moves, then write it into a freed slot; the
+  // slots are taken in ascending order so each truncated list stays in
+  // depth-first order in the buffer
   if (nholes > 0) {
+    long long* holes_sorted;
+    ENG_TRY(S.alloc(holes_sorted, nholes));
+    int pos_bits = 1;
+    while (pos_bits < 63 && (1LL << pos_bits) < Nn) ++pos_bits;
+    size_t bh = 0;
+    ENG_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bh, holes, holes_sorted,
+                                              static_cast<int>(nholes), 0, pos_bits, s));
+    unsigned char* tmp_h;
+    ENG_TRY(S.alloc(tmp_h, bh));
+    ENG_TRY(hipcub::DeviceRadixSort::SortKeys(tmp_h, bh, holes, holes_sorted,
+                                              static_cast<int>(nholes), 0, pos_bits, s));
     int32_t* stage;
     ENG_TRY(S.alloc(stage, 3 * nholes));
     engine_stage_fill<<<grid_for(nholes), kBlock, 0, s>>>(L.pairs, pos_b, seg_off, out_off, take,
                                                           nf, tail, static_cast<long long>(ntail),
                                                           T, stage);
-    engine_fill_holes<<<grid_for(nholes), kBlock, 0, s>>>(stage, holes,
+    engine_fill_holes<<<grid_for(nholes), kBlock, 0, s>>>(stage, holes_sorted,
                                                           static_cast<long long>(nholes), L.pairs);
   }
   engine_set_counts<<<grid_for(nf), kBlock, 0, s>>>(list, take, nf, L.counts);
