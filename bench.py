@@ -31,6 +31,12 @@ OPS_LEAF = 923         # FP64 ops of one triangle SAT
 BYTES_BV = 128         # two 64-B FP32 node records
 BYTES_LEAF = 144       # six FP64 vertices
 
+# HBM bytes per engine_dfs launch, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over this workload, each scaled by the reading of the same
+# counter for a device copy of known size (python bench.py --copy-calibration);
+# keyed by (queries, pairs).  profiles/engine_dfs_traffic.md holds the runs.
+TRAFFIC = {}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -60,6 +66,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run three device copies of MIB MiB (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -70,6 +79,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
+        return
 
     import fcl_amd as F
     from fcl_amd import workloads
@@ -155,22 +174,32 @@ def main():
         log("plain: %.1f ms, %.3g queries/s, exact match %s" % (plain_s * 1e3, nq / plain_s, exact))
 
     # ------------------------------------------------------------- roofline
+    # The depth-first kernel's algorithmic work is its share of the exact
+    # traversal (the oracle's num_bv_tests / num_leaf_tests minus what the
+    # breadth-first phase did).  VALU fraction prices it at the full-test op
+    # counts, HBM fraction at the bytes of the records one test reads.  The
+    # reported bound is the HBM one (the larger fraction on this workload);
+    # there is no matrix-core work, and the VALU fraction is given beside it.
     t_dfs = rep["dfs_ms"] * 1e-3
-    frac = None
+    alg_bytes = rep["dfs_bv_tests"] * BYTES_BV + rep["dfs_leaf_tests"] * BYTES_LEAF
+    valu_frac = hbm_frac = None
     if t_dfs > 0:
-        frac = (rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 +
-                rep["dfs_leaf_tests"] * OPS_LEAF / PEAK_FP64) / t_dfs
-        hbm_frac = (rep["dfs_bv_tests"] * BYTES_BV + rep["dfs_leaf_tests"] * BYTES_LEAF) / \
-            t_dfs / PEAK_HBM
+        valu_frac = (rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 +
+                     rep["dfs_leaf_tests"] * OPS_LEAF / PEAK_FP64) / t_dfs
+        hbm_frac = alg_bytes / t_dfs / PEAK_HBM
+    traffic = TRAFFIC.get((nq, args.pairs))
     roofline = {
         "kernel": "engine_dfs",
-        "bound": "valu",
-        "achieved": frac * PEAK_FP32 / 1e12 if frac is not None else None,
-        "peak": PEAK_FP32 / 1e12,
-        "unit": "TFLOP/s",
-        "frac": frac,
-        "hbm_frac": hbm_frac if frac is not None else None,
-        "traffic": None,
+        "bound": "hbm",
+        "achieved": alg_bytes / t_dfs / 1e9 if t_dfs > 0 else None,
+        "peak": PEAK_HBM / 1e9,
+        "unit": "GB/s",
+        "frac": hbm_frac,
+        "traffic": traffic["bytes"] if traffic else None,
+        "traffic_source": traffic["source"] if traffic else None,
+        "algorithmic_bytes": alg_bytes,
+        "valu_frac": valu_frac,
+        "valu_peak_tflops": PEAK_FP32 / 1e12,
         "kernel_ms": rep["dfs_ms"],
         "bv_tests": rep["dfs_bv_tests"],
         "leaf_tests": rep["dfs_leaf_tests"],
