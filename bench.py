@@ -35,7 +35,11 @@ BYTES_LEAF = 144       # six FP64 vertices
 # WRITE_SIZE passes over this workload, each scaled by the reading of the same
 # counter for a device copy of known size (python bench.py --copy-calibration);
 # keyed by (queries, pairs).  profiles/engine_dfs_traffic.md holds the runs.
-TRAFFIC = {}
+TRAFFIC = {
+    (1_000_000, 64): {"bytes": 4.978e12,
+                      "source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), "
+                                "mean of 4 engine_dfs dispatches"},
+}
 
 
 def log(*a):

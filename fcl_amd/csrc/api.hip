@@ -67,6 +67,7 @@ struct fclx_context {
   double* s_dist = nullptr;      // 1 + 6 points
   int32_t* s_ids = nullptr;      // 2
   EngineWorkspace* ws = nullptr;
+  std::mutex ws_mu;       // the FAST engine's work buffers (one batch at a time)
 };
 
 namespace {
@@ -401,17 +402,19 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid collide batch arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int depth, nmesh;
+  const MeshDev* table;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     depth = max_depth(ctx);
     nmesh = static_cast<int>(ctx->meshes.size());
+    table = ctx->d_table;
   }
   if (2 * depth > kMaxStack)
     return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
   (void)hipSetDevice(ctx->device);
   zero_i64<<<1, 64, 0, s>>>(reinterpret_cast<long long*>(a->num_pairs), 1);
   CollideLaunch L{};
-  L.meshes = ctx->d_table;
+  L.meshes = table;
   L.num_meshes = nmesh;
   L.num_queries = a->num_queries;
   L.query_meshes = a->query_meshes;
@@ -429,6 +432,7 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
   if (a->engine == FCLX_ENGINE_PLAIN) {
     e = launch_collide_plain(L, 2 * depth, s);
   } else if (a->engine == FCLX_ENGINE_FAST) {
+    std::lock_guard<std::mutex> lock(ctx->ws_mu);
     e = engine_collide(ctx->ws, L, 2 * depth, s);
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
@@ -446,16 +450,18 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int depth, nmesh;
+  const MeshDev* table;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     depth = max_depth(ctx);
     nmesh = static_cast<int>(ctx->meshes.size());
+    table = ctx->d_table;
   }
   if (2 * depth > kMaxStack)
     return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
   (void)hipSetDevice(ctx->device);
   DistanceLaunch L{};
-  L.meshes = ctx->d_table;
+  L.meshes = table;
   L.num_meshes = nmesh;
   L.num_queries = a->num_queries;
   L.query_meshes = a->query_meshes;

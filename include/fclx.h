@@ -19,9 +19,12 @@
  * "device" must point to device memory of the context's GPU; all others are
  * host memory.  Every function returns FCLX_OK (0) or a negative FCLX_ERR_*
  * code; no exception crosses the boundary.  fclx_last_error() gives text.
- * A context may be used from several threads; calls that share the
- * single-query staging buffers (fclx_collide, fclx_distance) serialise on an
- * internal lock, batched calls do not.
+ * A context may be used from several threads.  Single-query calls
+ * (fclx_collide, fclx_distance) share staging buffers and serialise on an
+ * internal lock; FAST-engine collide batches with num_max_contacts above 64
+ * share the context's work buffers and serialise on a second lock (they
+ * return once the batch is done); other batched calls only enqueue work.
+ * The status word that fclx_sync reports and clears is per context.
  */
 #ifndef FCLX_H
 #define FCLX_H
@@ -48,8 +51,11 @@ extern "C" {
  * reference's depth-first order (contacts come out in the reference's order
  * for every num_max_contacts).  FAST: wavefront-cooperative traversal with
  * FP32 conservative culling and FP64 leaves; the same contact set as PLAIN
- * when a query's contact count stays within num_max_contacts, otherwise the
- * query is re-run by PLAIN so the truncated list is the reference's. */
+ * when a query's contact count stays below num_max_contacts (records of such
+ * a query come out in no particular order).  A query that reaches the cap
+ * gets the reference's depth-first prefix, in order: its records are ordered
+ * by depth-first path keys and cut, or, when per-query statistics are
+ * requested, the query is re-run by PLAIN. */
 #define FCLX_ENGINE_PLAIN 0
 #define FCLX_ENGINE_FAST 1
 
@@ -139,7 +145,7 @@ typedef struct {
   double expand_ms;          /* all breadth-first levels */
   double leaves_ms;          /* leaf kernel over the breadth-first candidates */
   double dfs_ms;             /* persistent depth-first kernel */
-  double fixup_ms;           /* re-run of queries that reached the cap */
+  double fixup_ms;           /* truncation of queries that reached the cap */
   int64_t expand_bv_tests;   /* BV tests done by the breadth-first levels */
   int64_t expand_leaf_tests; /* triangle tests of the leaf kernel */
   int64_t dfs_bv_tests;      /* BV tests done by the depth-first kernel */
@@ -147,7 +153,7 @@ typedef struct {
   int64_t fp64_fallbacks;    /* BV tests the FP32 cull left to FP64 */
   int64_t front_size;        /* tasks handed to the depth-first kernel */
   int32_t levels;            /* breadth-first levels run */
-  int32_t flagged;           /* queries re-run because they reached the cap */
+  int32_t flagged;           /* queries whose contact count reached the cap */
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
