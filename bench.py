@@ -211,6 +211,8 @@ def main():
                                                            rep["expand_bv_tests"]),
         "phase_ms": {k: rep[k] for k in ("setup_ms", "expand_ms", "leaves_ms", "dfs_ms",
                                          "fixup_ms")},
+        "dfs_balance": {k: rep[k] for k in ("dfs_waves", "dfs_wave_busy", "dfs_end_p50_ms",
+                                            "dfs_end_p90_ms", "dfs_end_max_ms")},
     }
 
     # ------------------------------------------------------------ CPU oracle

@@ -61,7 +61,9 @@ class EngineReport(ctypes.Structure):
                 ("dfs_ms", c_dbl), ("fixup_ms", c_dbl), ("expand_bv_tests", c_i64),
                 ("expand_leaf_tests", c_i64), ("dfs_bv_tests", c_i64), ("dfs_leaf_tests", c_i64),
                 ("fp64_fallbacks", c_i64), ("front_size", c_i64), ("levels", c_i32),
-                ("flagged", c_i32)]
+                ("flagged", c_i32), ("dfs_waves", c_i32), ("reserved", c_i32),
+                ("dfs_wave_busy", c_dbl), ("dfs_end_p50_ms", c_dbl), ("dfs_end_p90_ms", c_dbl),
+                ("dfs_end_max_ms", c_dbl)]
 
 
 # name -> (restype, argtypes)

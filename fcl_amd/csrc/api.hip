@@ -502,6 +502,12 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   out->front_size = r.front_size;
   out->levels = r.levels;
   out->flagged = r.flagged;
+  out->dfs_waves = r.dfs_waves;
+  out->reserved = 0;
+  out->dfs_wave_busy = r.dfs_wave_busy;
+  out->dfs_end_p50_ms = r.dfs_end_p50_ms;
+  out->dfs_end_p90_ms = r.dfs_end_p90_ms;
+  out->dfs_end_max_ms = r.dfs_end_max_ms;
   return FCLX_OK;
 }
 

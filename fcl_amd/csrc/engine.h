@@ -32,6 +32,8 @@ struct EngineReport {
   long long expand_bv_tests, expand_leaf_tests, dfs_bv_tests, dfs_leaf_tests, fp64_fallbacks;
   long long front_size;
   int levels, flagged;
+  int dfs_waves;
+  double dfs_wave_busy, dfs_end_p50_ms, dfs_end_p90_ms, dfs_end_max_ms;
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
 const EngineReport& engine_report(const EngineWorkspace* ws);

@@ -142,6 +142,7 @@ struct EngineArgs {
   unsigned long long* stacks;
   int stack_cap;
   unsigned long long* work;  // kWork* counters
+  unsigned long long* wave_prof;  // profiling: (start, end, bv tests) per DFS wavefront, or null
 };
 
 enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback };
@@ -320,6 +321,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
   int nleaf = 0;   // wavefront-uniform
   bool input_left = true;
   unsigned long long n_bv = 0, n_leaf = 0, n_fb = 0;  // wavefront-uniform
+  const unsigned long long t_start = A.wave_prof ? wall_clock64() : 0;
 
   auto flush_leaves = [&](int k) {
     n_leaf += k;
@@ -380,6 +382,13 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
   while (nleaf > 0) flush_leaves(nleaf < kWave ? nleaf : kWave);
+  if (A.wave_prof && lane == 0) {
+    unsigned long long* w =
+        A.wave_prof + 3 * (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave);
+    w[0] = t_start;
+    w[1] = wall_clock64();
+    w[2] = n_bv;
+  }
   if (lane == 0) {
     atomicAdd(A.work + kWorkDfsBV, n_bv);
     atomicAdd(A.work + kWorkDfsLeaf, n_leaf);
@@ -766,6 +775,8 @@ struct EngineWorkspace {
   unsigned long long* host_counters = nullptr;
   bool profile = false;
   hipEvent_t ev[6] = {};
+  unsigned long long* wave_prof = nullptr;  // profiling only
+  size_t wave_prof_cap = 0;
   EngineReport report{};
 };
 
@@ -797,6 +808,7 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
   (void)hipFree(ws->leafc);
   (void)hipFree(ws->stacks);
   (void)hipFree(ws->counters);
+  (void)hipFree(ws->wave_prof);
   if (ws->host_counters) (void)hipHostFree(ws->host_counters);
   for (auto& e : ws->ev)
     if (e) (void)hipEventDestroy(e);
@@ -820,6 +832,44 @@ hipError_t read_counter(EngineWorkspace* ws, int idx, unsigned long long& v, hip
                          hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
   v = ws->host_counters[idx];
+  return hipSuccess;
+}
+
+// Load balance of the depth-first kernel from its per-wavefront (start, end,
+// BV tests) records: mean busy time of the wavefronts that did work over the
+// kernel's span, and when 50 % / 90 % / all of them had finished.
+hipError_t summarize_waves(EngineWorkspace* ws, size_t nwaves, hipStream_t s) {
+  std::vector<unsigned long long> h(3 * nwaves);
+  ENG_TRY(hipMemcpyAsync(h.data(), ws->wave_prof, sizeof(unsigned long long) * h.size(),
+                         hipMemcpyDeviceToHost, s));
+  ENG_TRY(hipStreamSynchronize(s));
+  int dev = 0, khz = 0;
+  ENG_TRY(hipGetDevice(&dev));
+  ENG_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) return hipSuccess;
+  const double ms_per_tick = 1.0 / khz;
+  unsigned long long t0 = ~0ULL;
+  std::vector<unsigned long long> ends;
+  double busy = 0;
+  for (size_t w = 0; w < nwaves; ++w) {
+    if (h[3 * w + 2] == 0) continue;
+    t0 = std::min(t0, h[3 * w]);
+    ends.push_back(h[3 * w + 1]);
+    busy += static_cast<double>(h[3 * w + 1] - h[3 * w]);
+  }
+  EngineReport& r = ws->report;
+  r.dfs_waves = static_cast<int>(ends.size());
+  if (ends.empty()) return hipSuccess;
+  std::sort(ends.begin(), ends.end());
+  const double span = static_cast<double>(ends.back() - t0);
+  r.dfs_wave_busy = span > 0 ? busy / (span * static_cast<double>(ends.size())) : 1.0;
+  auto at = [&](double f) {
+    const size_t i = std::min(ends.size() - 1, static_cast<size_t>(f * (ends.size() - 1)));
+    return static_cast<double>(ends[i] - t0) * ms_per_tick;
+  };
+  r.dfs_end_p50_ms = at(0.5);
+  r.dfs_end_p90_ms = at(0.9);
+  r.dfs_end_max_ms = at(1.0);
   return hipSuccess;
 }
 
@@ -1152,6 +1202,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   mark(3);
   rep.front_size = static_cast<long long>(nfront);
   // 4. persistent depth-first phase over the remaining front
+  size_t dfs_waves = 0;
   if (nfront > 0) {
     const long long waves_needed = (static_cast<long long>(nfront) + kWave - 1) / kWave;
     const long long max_blocks = static_cast<long long>(cus) * tune.dfs_blocks_per_cu;
@@ -1165,11 +1216,19 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     A.head = ws->counters + 3;
     A.stacks = ws->stacks;
     A.stack_cap = stack_cap;
+    const size_t nwaves = static_cast<size_t>(blocks) * kWavesPerBlock;
+    A.wave_prof = nullptr;
+    if (ws->profile) {
+      ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * nwaves));
+      A.wave_prof = ws->wave_prof;
+    }
     set_u64<<<1, 1, 0, s>>>(ws->counters + 3, 0);
     engine_dfs<<<blocks, kBlock, 0, s>>>(A);
     ENG_TRY(hipGetLastError());
+    dfs_waves = nwaves;
   }
   mark(4);
+  if (ws->profile && dfs_waves > 0) ENG_TRY(summarize_waves(ws, dfs_waves, s));
   // 5. queries that reached the cap
   unsigned long long nflag = 0;
   ENG_TRY(read_counter(ws, 4, nflag, s));

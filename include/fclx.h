@@ -154,6 +154,15 @@ typedef struct {
   int64_t front_size;        /* tasks handed to the depth-first kernel */
   int32_t levels;            /* breadth-first levels run */
   int32_t flagged;           /* queries whose contact count reached the cap */
+  /* depth-first load balance (profiling only): wavefronts that did work,
+   * their mean busy time over the kernel's span, and the times at which
+   * 50 % / 90 % / all of them had finished, from the first wave's start */
+  int32_t dfs_waves;
+  int32_t reserved;
+  double dfs_wave_busy;
+  double dfs_end_p50_ms;
+  double dfs_end_p90_ms;
+  double dfs_end_max_ms;
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);

@@ -31,6 +31,34 @@ def test_every_declared_symbol_is_exported():
     assert declared <= set(N.SIGNATURES)
 
 
+def test_binding_structs_match_the_header(tmp_path):
+    """Every argument/report struct of fclx.h has the size and field offsets
+    the ctypes binding assumes (compiled with the host C compiler)."""
+    structs = {"fclx_mesh_info": N.MeshInfo,
+               "fclx_collide_batch_args": N.CollideBatchArgs,
+               "fclx_distance_batch_args": N.DistanceBatchArgs,
+               "fclx_engine_report": N.EngineReport}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "fclx.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for f, _ in py._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,
+                                          check=True).stdout.split()]
+    want = []
+    for py in structs.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want
+
+
 def test_icosphere_is_closed_and_sized():
     for k in (1, 2, 5, 11):
         V, T = F.icosphere(k, 0.15, 7)
