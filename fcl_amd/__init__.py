@@ -226,6 +226,10 @@ class Context:
     def set_profiling(self, enable=True):
         self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))
 
+    def set_engine_param(self, name, value):
+        """Tunables of the FAST engine's exhaustive path (fclx_set_engine_param)."""
+        self._check(self._L.fclx_set_engine_param(self._h, name.encode(), float(value)))
+
     def engine_report(self):
         r = N.EngineReport()
         self._check(self._L.fclx_engine_last_report(self._h, ctypes.byref(r)))

@@ -61,7 +61,7 @@ class EngineReport(ctypes.Structure):
                 ("dfs_ms", c_dbl), ("fixup_ms", c_dbl), ("expand_bv_tests", c_i64),
                 ("expand_leaf_tests", c_i64), ("dfs_bv_tests", c_i64), ("dfs_leaf_tests", c_i64),
                 ("fp64_fallbacks", c_i64), ("front_size", c_i64), ("levels", c_i32),
-                ("flagged", c_i32), ("dfs_waves", c_i32), ("reserved", c_i32),
+                ("flagged", c_i32), ("dfs_waves", c_i32), ("dfs_rounds", c_i32),
                 ("dfs_wave_busy", c_dbl), ("dfs_end_p50_ms", c_dbl), ("dfs_end_p90_ms", c_dbl),
                 ("dfs_end_max_ms", c_dbl)]
 
@@ -70,6 +70,7 @@ class EngineReport(ctypes.Structure):
 SIGNATURES = {
     "fclx_set_profiling": (c_i32, [c_vp, c_i32]),
     "fclx_engine_last_report": (c_i32, [c_vp, ctypes.POINTER(EngineReport)]),
+    "fclx_set_engine_param": (c_i32, [c_vp, ctypes.c_char_p, c_dbl]),
     "fclx_init": (c_i32, [c_i32, ctypes.POINTER(c_vp)]),
     "fclx_shutdown": (None, [c_vp]),
     "fclx_last_error": (ctypes.c_char_p, [c_vp]),

@@ -486,6 +486,30 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable) {
   return FCLX_OK;
 }
 
+int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
+  if (!ctx || !name || !ctx->ws) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->ws_mu);
+  EngineTuning& t = engine_tuning(ctx->ws);
+  const std::string n(name);
+  auto in = [&](double lo, double hi) { return value >= lo && value <= hi; };
+  if (n == "max_levels" && in(1, 64)) {
+    t.max_levels = static_cast<int>(value);
+  } else if (n == "front_target" && in(64, 1LL << 40)) {
+    t.front_target = static_cast<long long>(value);
+  } else if (n == "dfs_blocks_per_cu" && in(1, 32)) {
+    t.dfs_blocks_per_cu = static_cast<int>(value);
+  } else if (n == "defer_leaf" && in(64, 1LL << 34)) {
+    t.defer_leaf = static_cast<long long>(value);
+  } else if (n == "defer_uncertain" && in(64, 1LL << 34)) {
+    t.defer_uncertain = static_cast<long long>(value);
+  } else if (n == "idle_stop" && in(0, 1)) {
+    t.idle_stop = value;
+  } else {
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter or value out of range");
+  }
+  return FCLX_OK;
+}
+
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   if (!ctx || !out || !ctx->ws) return FCLX_ERR_INVALID_ARGUMENT;
   const EngineReport& r = engine_report(ctx->ws);
@@ -503,7 +527,7 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   out->levels = r.levels;
   out->flagged = r.flagged;
   out->dfs_waves = r.dfs_waves;
-  out->reserved = 0;
+  out->dfs_rounds = r.dfs_rounds;
   out->dfs_wave_busy = r.dfs_wave_busy;
   out->dfs_end_p50_ms = r.dfs_end_p50_ms;
   out->dfs_end_p90_ms = r.dfs_end_p90_ms;
@@ -519,12 +543,12 @@ int fclx_sync(fclx_context* ctx, void* stream) {
   HIP_TRY(ctx, hipMemcpy(&st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost));
   if (st != 0) {
     HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
-    if (st & kStatusStackOverflow)
+    if (st & (kStatusStackOverflow | kStatusQueueOverflow))
       return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "a traversal exceeded its stack bound");
     if (st & kStatusBadQuery)
       return fail(ctx, FCLX_ERR_INVALID_ARGUMENT,
                   "a query named a missing mesh or an unsupported BV pair");
-    if (st & (kStatusPairOverflow | kStatusQueueOverflow))
+    if (st & kStatusPairOverflow)
       return fail(ctx, FCLX_ERR_CAPACITY, "pair output exceeded pair_capacity");
   }
   return FCLX_OK;

@@ -17,13 +17,16 @@ void engine_workspace_destroy(EngineWorkspace* ws);
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                           hipStream_t s);
 
-// Tunables of the exhaustive path (defaults chosen by measurement).
+// Tunables of the exhaustive path, per context (defaults chosen by measurement).
 struct EngineTuning {
-  int max_levels = 24;          // breadth-first levels before the DFS phase
+  int max_levels = 24;          // breadth-first levels before a depth-first round
   long long front_target = 1 << 22;  // stop expanding once the front is this large
-  int dfs_blocks_per_cu = 8;    // persistent grid: blocks of 4 wavefronts
+  int dfs_blocks_per_cu = 8;    // persistent grid cap: blocks of 4 wavefronts per CU
+  long long defer_leaf = 1LL << 27;       // deferred leaf-pair buffer (entries)
+  long long defer_uncertain = 1LL << 24;  // deferred uncertain-task buffer (entries)
+  double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
 };
-EngineTuning& engine_tuning();
+EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 
 constexpr int kOrderedCapMax = 64;
 
@@ -32,7 +35,7 @@ struct EngineReport {
   long long expand_bv_tests, expand_leaf_tests, dfs_bv_tests, dfs_leaf_tests, fp64_fallbacks;
   long long front_size;
   int levels, flagged;
-  int dfs_waves;
+  int dfs_waves, dfs_rounds;
   double dfs_wave_busy, dfs_end_p50_ms, dfs_end_p90_ms, dfs_end_max_ms;
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);

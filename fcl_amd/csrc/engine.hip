@@ -43,6 +43,23 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kLeafQ = 128;
 
+// Device counters of the exhaustive path (ws->counters[...]).
+enum : int {
+  kCtrSetupOut = 0,   // root tasks
+  kCtrExpandOut = 1,  // breadth-first level output
+  kCtrLeafc = 2,      // breadth-first leaf candidates
+  kCtrHead = 3,       // depth-first refill cursor
+  kCtrFlagged = 4,    // queries that reached the cap
+  // 5..7: cap fix-up
+  kCtrWork = 8,       // 8 .. 8+kWork*: work counters
+  kCtrDefer0 = 16,    // leaf pairs: reservation, written prefix end, dumps
+  kCtrDefer1 = 19,    // uncertain tasks: the same three
+  kCtrStop = 22,
+  kCtrIdle = 23,
+  kCtrNext = 24,      // next round's front
+  kNumCounters = 32
+};
+
 struct QueryXf64 {
   double R[9];
   double T[3];
@@ -142,7 +159,20 @@ struct EngineArgs {
   unsigned long long* stacks;
   int stack_cap;
   unsigned long long* work;  // kWork* counters
+  unsigned long long* leaf_work;  // work counter the leaf kernel adds to
   unsigned long long* wave_prof;  // profiling: (start, end, bv tests) per DFS wavefront, or null
+  // depth-first rounds: tasks that need FP64 work are deferred to two
+  // buffers (0: overlapping leaf pairs, 1: tasks the FP32 cull left
+  // uncertain).  Flushes fill [0, defer_cap); a flush that does not fit
+  // fails and stops the round.  Per-wavefront leftovers are dumped after
+  // defer_cap.  defer_ctl[w]: reservation counter, end of the written
+  // prefix, dump counter.
+  unsigned long long* defer[2];
+  long long defer_cap[2];
+  unsigned long long* defer_ctl[2];
+  unsigned long long* stop;   // nonzero: every wavefront dumps its state and exits
+  unsigned long long* idle;   // wavefronts that ran out of work
+  unsigned long long idle_limit;  // idle count that stops the round (0: never)
 };
 
 enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback };
@@ -200,6 +230,7 @@ struct Visit {
 // BV test of task (q,b1,b2): FP32 cull, FP64 only when unsure; then the
 // reference's descent rule (firstOverSecond) for the children.  The query
 // record, both nodes and both sizes are loaded before any is used.
+template <bool kCount = true>
 __device__ __forceinline__ Visit visit_task(const EngineArgs& A, unsigned long long task) {
   int q, b1, b2;
   unpack_task(task, q, b1, b2);
@@ -207,7 +238,8 @@ __device__ __forceinline__ Visit visit_task(const EngineArgs& A, unsigned long l
   const QueryXf32& X = A.qx32[q];
   const ObbNode32& n1 = P.n1[b1];
   const ObbNode32& n2 = P.n2[b2];
-  if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+  if (kCount && A.L.bv_tests)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
   Visit v;
   int r = obb_cull32(X, n1, n2);
   if (r == kCullUnsure) {
@@ -305,35 +337,129 @@ __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
   bool hit = false;
   int q = 0, id1 = 0, id2 = 0;
   if (i < A.n_in) hit = leaf_hit(A, A.leafc[i], q, id1, id2);
-  wave_count(A.work + kWorkLeafKernel, i < A.n_in);
+  wave_count(A.leaf_work, i < A.n_in);
+  record_hits(A, hit, q, id1, id2);
+}
+
+// Deferred uncertain tasks: the FP64 test decides (the FP32 pass already
+// counted the BV test); overlapping leaf pairs run their triangle test here,
+// children of overlapping internal pairs go to the next round's front.
+__global__ void __launch_bounds__(kBlock) engine_uncertain(EngineArgs A) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = i < A.n_in;
+  unsigned long long task = 0;
+  Visit v;
+  if (active) {
+    task = A.leafc[i];
+    v = visit_task<false>(A, task);
+  }
+  const unsigned long long so = wave_reserve(A.n_out, v.push, 2);
+  if (v.push) {
+    A.front_out[so] = v.c0;
+    A.front_out[so + 1] = v.c1;
+  }
+  bool hit = false;
+  int q = 0, id1 = 0, id2 = 0;
+  if (v.leafc) hit = leaf_hit(A, task, q, id1, id2);
+  wave_count(A.leaf_work, v.leafc);
   record_hits(A, hit, q, id1, id2);
 }
 
 // ----------------------------------------------- persistent depth-first
+// Outcome of the FP32 pass over one task.
+enum : int { kStepNone = 0, kStepPush = 1, kStepLeaf = 2, kStepUncertain = 3 };
+
+// FP32-only BV test of task (q,b1,b2) and the descent rule; nothing FP64 is
+// inlined here, which keeps the depth-first kernel's register count low.
+__device__ __forceinline__ int step32(const EngineArgs& A, unsigned long long task,
+                                      unsigned long long& c0, unsigned long long& c1) {
+  int q, b1, b2;
+  unpack_task(task, q, b1, b2);
+  const QueryPtr P = A.qptr[q];
+  const QueryXf32& X = A.qx32[q];
+  const ObbNode32& n1 = P.n1[b1];
+  const ObbNode32& n2 = P.n2[b2];
+  if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+  const int r = obb_cull32(X, n1, n2);
+  if (r == kCullDisjoint) return kStepNone;
+  if (r == kCullUnsure) return kStepUncertain;
+  const int fc1 = n1.first_child, fc2 = n2.first_child;
+  if (fc1 < 0 && fc2 < 0) return kStepLeaf;
+  if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+    c0 = pack_task(q, fc1, b2);
+    c1 = pack_task(q, fc1 + 1, b2);
+  } else {
+    c0 = pack_task(q, b1, fc2);
+    c1 = pack_task(q, b1, fc2 + 1);
+  }
+  return kStepPush;
+}
+
+// Move the top k entries of a wavefront's LDS queue to deferred buffer w.
+// Returns false (and moves nothing) when the buffer has no room for them.
+__device__ __forceinline__ bool flush_deferred(const EngineArgs& A, int w,
+                                               const unsigned long long* q, int& n, int k) {
+  unsigned long long s = 0;
+  if (lane_id() == 0) s = atomicAdd(A.defer_ctl[w], static_cast<unsigned long long>(k));
+  s = __shfl(s, 0);
+  if (static_cast<long long>(s + k) > A.defer_cap[w]) return false;
+  if (lane_id() < k) A.defer[w][s + lane_id()] = q[n - k + lane_id()];
+  if (lane_id() == 0) atomicMax(A.defer_ctl[w] + 1, s + k);
+  __builtin_amdgcn_wave_barrier();
+  n -= k;
+  return true;
+}
+
+// A stopped or finishing wavefront's queue leftovers, after defer_cap
+// (room for kLeafQ per wavefront is reserved there).
+__device__ __forceinline__ void dump_deferred(const EngineArgs& A, int w,
+                                              const unsigned long long* q, int n) {
+  if (n == 0) return;
+  unsigned long long s = 0;
+  if (lane_id() == 0) s = atomicAdd(A.defer_ctl[w] + 2, static_cast<unsigned long long>(n));
+  s = __shfl(s, 0);
+  for (int i = lane_id(); i < n; i += kWave) A.defer[w][A.defer_cap[w] + s + i] = q[i];
+}
+
+// One depth-first round.  Every wavefront owns a LIFO stack in device
+// memory; each step pops up to 64 tasks (one per lane), runs the FP32 test,
+// pushes the children of overlapping internal pairs with one prefix sum and
+// queues overlapping leaf pairs and uncertain tasks in LDS, moved 64 at a
+// time to the deferred buffers for the FP64 kernels.  Idle wavefronts
+// refill from the front.  The round stops early when a deferred buffer is
+// full or when idle_limit wavefronts have run out of work; then every
+// wavefront appends its stack to front_out and its queue leftovers to the
+// dump regions, and the host runs the FP64 kernels and the next round.
 __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
-  __shared__ unsigned long long leafq_all[kWavesPerBlock][kLeafQ];
+  __shared__ unsigned long long q_all[2][kWavesPerBlock][kLeafQ];
   const int wave = threadIdx.x / kWave;
   const int lane = lane_id();
-  unsigned long long* leafq = leafq_all[wave];
+  unsigned long long* lq = q_all[0][wave];
+  unsigned long long* uq = q_all[1][wave];
   unsigned long long* stack =
       A.stacks + (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave) * A.stack_cap;
-  int sp = 0;      // wavefront-uniform
-  int nleaf = 0;   // wavefront-uniform
+  int sp = 0;                 // wavefront-uniform
+  int nl = 0, nu = 0;         // queue fill, wavefront-uniform
   bool input_left = true;
-  unsigned long long n_bv = 0, n_leaf = 0, n_fb = 0;  // wavefront-uniform
+  bool stopped = false;
+  unsigned long long n_bv = 0, n_unc = 0;  // wavefront-uniform
   const unsigned long long t_start = A.wave_prof ? wall_clock64() : 0;
 
-  auto flush_leaves = [&](int k) {
-    n_leaf += k;
-    bool hit = false;
-    int q = 0, id1 = 0, id2 = 0;
-    if (lane < k) hit = leaf_hit(A, leafq[nleaf - k + lane], q, id1, id2);
-    __builtin_amdgcn_wave_barrier();
-    nleaf -= k;
-    record_hits(A, hit, q, id1, id2);
-  };
-
   while (true) {
+    const int stop_seen =
+        __hip_atomic_load(A.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1 : 0;
+    if (__builtin_amdgcn_readfirstlane(stop_seen)) {  // one decision per wavefront
+      stopped = true;
+      break;
+    }
+    // room for one full step in both queues
+    const bool full = (nl >= kWave && !flush_deferred(A, 0, lq, nl, kWave)) ||
+                      (nu >= kWave && !flush_deferred(A, 1, uq, nu, kWave));
+    if (full) {
+      if (lane == 0) __hip_atomic_store(A.stop, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      stopped = true;
+      break;
+    }
     if (sp < kWave && input_left) {
       const int want = kWave - sp;
       unsigned long long start = 0;
@@ -350,38 +476,53 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     const int n = sp < kWave ? sp : kWave;
     sp -= n;
     const bool active = lane < n;
-    unsigned long long task = 0;
+    unsigned long long task = 0, c0 = 0, c1 = 0;
     if (active) task = stack[sp + lane];
     n_bv += n;
-    Visit v;
-    if (active) v = visit_task(A, task);
-    const bool push = v.push, leafc = v.leafc;
-    const unsigned long long c0 = v.c0, c1 = v.c1;
-    n_fb += __popcll(__ballot(v.fallback));
+    const int r = active ? step32(A, task, c0, c1) : kStepNone;
     // children onto the stack (one prefix sum)
-    const unsigned long long pm = __ballot(push);
+    const unsigned long long pm = __ballot(r == kStepPush);
     const int npush = 2 * __popcll(pm);
     if (sp + npush > A.stack_cap) {
       if (lane == 0) atomicOr(A.L.status, kStatusQueueOverflow);
       break;  // never write past the stack; the overflow is reported
     }
-    if (push) {
+    if (r == kStepPush) {
       const int o = sp + 2 * __popcll(pm & lanes_below());
       stack[o] = c0;
       stack[o + 1] = c1;
     }
     sp += npush;
-    // leaf candidates into the LDS queue, tested 64 at a time
-    const unsigned long long lm = __ballot(leafc);
-    const int nl = __popcll(lm);
-    if (nleaf + nl > kLeafQ) flush_leaves(kWave);
-    if (leafc) leafq[nleaf + __popcll(lm & lanes_below())] = task;
+    // leaf pairs and uncertain tasks into the LDS queues
+    const unsigned long long lm = __ballot(r == kStepLeaf);
+    const unsigned long long um = __ballot(r == kStepUncertain);
+    if (r == kStepLeaf) lq[nl + __popcll(lm & lanes_below())] = task;
+    if (r == kStepUncertain) uq[nu + __popcll(um & lanes_below())] = task;
     __builtin_amdgcn_wave_barrier();
-    nleaf += nl;
-    if (nleaf >= kWave) flush_leaves(kWave);
+    nl += __popcll(lm);
+    nu += __popcll(um);
+    n_unc += __popcll(um);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
-  while (nleaf > 0) flush_leaves(nleaf < kWave ? nleaf : kWave);
+  if (stopped) {
+    // hand the stack to the next round
+    if (sp > 0) {
+      unsigned long long s = 0;
+      if (lane == 0) s = atomicAdd(A.n_out, static_cast<unsigned long long>(sp));
+      s = __shfl(s, 0);
+      for (int i = lane; i < sp; i += kWave) A.front_out[s + i] = stack[i];
+    }
+    dump_deferred(A, 0, lq, nl);
+    dump_deferred(A, 1, uq, nu);
+  } else {
+    if (nl > 0 && !flush_deferred(A, 0, lq, nl, nl)) dump_deferred(A, 0, lq, nl);
+    if (nu > 0 && !flush_deferred(A, 1, uq, nu, nu)) dump_deferred(A, 1, uq, nu);
+    if (lane == 0) {
+      const unsigned long long idle = atomicAdd(A.idle, 1ULL) + 1;
+      if (A.idle_limit != 0 && idle == A.idle_limit)
+        __hip_atomic_store(A.stop, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
   if (A.wave_prof && lane == 0) {
     unsigned long long* w =
         A.wave_prof + 3 * (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave);
@@ -391,8 +532,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
   }
   if (lane == 0) {
     atomicAdd(A.work + kWorkDfsBV, n_bv);
-    atomicAdd(A.work + kWorkDfsLeaf, n_leaf);
-    if (n_fb) atomicAdd(A.work + kWorkFallback, n_fb);
+    if (n_unc) atomicAdd(A.work + kWorkFallback, n_unc);
   }
 }
 
@@ -586,6 +726,9 @@ __global__ void engine_append_records(const int32_t* rec, long long nrec, const 
 }
 
 __global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void zero_u64(unsigned long long* p, int n) {
+  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
 
 // ---------------------------------------------- ordered, small caps
 template <int STACK>
@@ -771,26 +914,30 @@ struct EngineWorkspace {
   size_t leafc_cap = 0;
   unsigned long long* stacks = nullptr;
   size_t stacks_cap = 0;
-  unsigned long long* counters = nullptr;  // 16 device counters (8.. = work)
+  unsigned long long* defer[2] = {nullptr, nullptr};
+  size_t defer_cap[2] = {0, 0};
+  unsigned long long* counters = nullptr;  // kNumCounters device counters
   unsigned long long* host_counters = nullptr;
   bool profile = false;
-  hipEvent_t ev[6] = {};
-  unsigned long long* wave_prof = nullptr;  // profiling only
+  // profiling: event pairs around launches, summed per phase at sync points
+  std::vector<hipEvent_t> ev;
+  std::vector<std::pair<int, size_t>> spans;  // (phase, index of the start event)
+  size_t ev_used = 0;
+  unsigned long long* wave_prof = nullptr;
   size_t wave_prof_cap = 0;
+  double busy_sum = 0, busy_den = 0;
   EngineReport report{};
+  EngineTuning tune{};
 };
 
-EngineTuning& engine_tuning() {
-  static EngineTuning t;
-  return t;
-}
+EngineTuning& engine_tuning(EngineWorkspace* ws) { return ws->tune; }
 
 EngineWorkspace* engine_workspace_create() {
   auto* ws = new EngineWorkspace();
-  if (hipMalloc(reinterpret_cast<void**>(&ws->counters), 16 * sizeof(unsigned long long)) !=
-          hipSuccess ||
+  if (hipMalloc(reinterpret_cast<void**>(&ws->counters),
+                kNumCounters * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ws->host_counters),
-                    16 * sizeof(unsigned long long)) != hipSuccess) {
+                    kNumCounters * sizeof(unsigned long long)) != hipSuccess) {
     engine_workspace_destroy(ws);
     return nullptr;
   }
@@ -809,6 +956,8 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
   (void)hipFree(ws->stacks);
   (void)hipFree(ws->counters);
   (void)hipFree(ws->wave_prof);
+  (void)hipFree(ws->defer[0]);
+  (void)hipFree(ws->defer[1]);
   if (ws->host_counters) (void)hipHostFree(ws->host_counters);
   for (auto& e : ws->ev)
     if (e) (void)hipEventDestroy(e);
@@ -817,8 +966,6 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
 
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on) {
   if (!ws) return hipErrorInvalidValue;
-  if (on && !ws->ev[0])
-    for (auto& e : ws->ev) ENG_TRY(hipEventCreate(&e));
   ws->profile = on;
   return hipSuccess;
 }
@@ -832,6 +979,43 @@ hipError_t read_counter(EngineWorkspace* ws, int idx, unsigned long long& v, hip
                          hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
   v = ws->host_counters[idx];
+  return hipSuccess;
+}
+
+enum : int { kPhaseSetup = 0, kPhaseExpand, kPhaseLeaves, kPhaseDfs, kPhaseFixup };
+
+// Profiling clock: an event pair around each timed launch sequence, summed
+// per phase once the stream has passed them.  No-ops unless profiling.
+hipError_t clock_begin(EngineWorkspace* ws, hipStream_t s, size_t& idx) {
+  if (!ws->profile) return hipSuccess;
+  while (ws->ev.size() < ws->ev_used + 2) {
+    hipEvent_t e;
+    ENG_TRY(hipEventCreate(&e));
+    ws->ev.push_back(e);
+  }
+  idx = ws->ev_used;
+  ws->ev_used += 2;
+  return hipEventRecord(ws->ev[idx], s);
+}
+
+void clock_end(EngineWorkspace* ws, hipStream_t s, size_t idx, int phase) {
+  if (!ws->profile) return;
+  (void)hipEventRecord(ws->ev[idx + 1], s);
+  ws->spans.emplace_back(phase, idx);
+}
+
+hipError_t clock_collect(EngineWorkspace* ws) {
+  if (!ws->profile) return hipSuccess;
+  EngineReport& r = ws->report;
+  double* out[5] = {&r.setup_ms, &r.expand_ms, &r.leaves_ms, &r.dfs_ms, &r.fixup_ms};
+  for (const auto& sp : ws->spans) {
+    ENG_TRY(hipEventSynchronize(ws->ev[sp.second + 1]));
+    float ms = 0;
+    ENG_TRY(hipEventElapsedTime(&ms, ws->ev[sp.second], ws->ev[sp.second + 1]));
+    *out[sp.first] += ms;
+  }
+  ws->spans.clear();
+  ws->ev_used = 0;
   return hipSuccess;
 }
 
@@ -858,11 +1042,14 @@ hipError_t summarize_waves(EngineWorkspace* ws, size_t nwaves, hipStream_t s) {
     busy += static_cast<double>(h[3 * w + 1] - h[3 * w]);
   }
   EngineReport& r = ws->report;
-  r.dfs_waves = static_cast<int>(ends.size());
   if (ends.empty()) return hipSuccess;
   std::sort(ends.begin(), ends.end());
   const double span = static_cast<double>(ends.back() - t0);
-  r.dfs_wave_busy = span > 0 ? busy / (span * static_cast<double>(ends.size())) : 1.0;
+  ws->busy_sum += busy;
+  ws->busy_den += span * static_cast<double>(ends.size());
+  r.dfs_wave_busy = ws->busy_den > 0 ? ws->busy_sum / ws->busy_den : 1.0;
+  if (r.dfs_waves > 0) return hipSuccess;  // percentiles describe the first round
+  r.dfs_waves = static_cast<int>(ends.size());
   auto at = [&](double f) {
     const size_t i = std::min(ends.size() - 1, static_cast<size_t>(f * (ends.size() - 1)));
     return static_cast<double>(ends[i] - t0) * ms_per_tick;
@@ -1138,12 +1325,13 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     return hipGetLastError();
   }
   if (nq >= (1 << 22)) return hipErrorInvalidValue;  // task packing limit
-  const EngineTuning& tune = engine_tuning();
+  const EngineTuning& tune = ws->tune;
   ENG_TRY(grow(ws->qx32, ws->qx32_cap, nq));
   ENG_TRY(grow(ws->qptr, ws->qptr_cap, nq));
   ENG_TRY(grow(ws->qx64, ws->qx64_cap, nq));
   ENG_TRY(grow(ws->qflag, ws->qflag_cap, nq));
   ENG_TRY(grow(ws->front[0], ws->front_cap[0], std::max<size_t>(nq, 1024)));
+  unsigned long long* C = ws->counters;
 
   EngineArgs A{};
   A.L = L;
@@ -1151,88 +1339,185 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   A.qptr = ws->qptr;
   A.qx64 = ws->qx64;
   A.qflag = ws->qflag;
-  A.nflag = ws->counters + 4;
-  A.work = ws->counters + 8;
-  for (int i = 0; i < 16; ++i) set_u64<<<1, 1, 0, s>>>(ws->counters + i, 0);
+  A.nflag = C + kCtrFlagged;
+  A.work = C + kCtrWork;
+  zero_u64<<<1, kNumCounters, 0, s>>>(C, kNumCounters);
   EngineReport& rep = ws->report;
   rep = EngineReport{};
-  auto mark = [&](int k) {
-    if (ws->profile) (void)hipEventRecord(ws->ev[k], s);
-  };
-  mark(0);
+  ws->busy_sum = ws->busy_den = 0;
+  ws->spans.clear();
+  ws->ev_used = 0;
+
   // 1. setup -> front[0]
+  size_t t = 0;
+  ENG_TRY(clock_begin(ws, s, t));
   A.front_out = ws->front[0];
-  A.n_out = ws->counters + 0;
+  A.n_out = C + kCtrSetupOut;
   engine_setup<<<grid_for(nq), kBlock, 0, s>>>(A);
   ENG_TRY(hipGetLastError());
+  clock_end(ws, s, t, kPhaseSetup);
   unsigned long long nfront = 0;
-  ENG_TRY(read_counter(ws, 0, nfront, s));
-  mark(1);
-  // 2. breadth-first expansion while the front is small
+  ENG_TRY(read_counter(ws, kCtrSetupOut, nfront, s));
   int cur = 0;
-  unsigned long long nleafc_total = 0;
-  for (int level = 0; level < tune.max_levels && nfront > 0 &&
-                      static_cast<long long>(nfront) < tune.front_target;
-       ++level) {
-    const int nxt = cur ^ 1;
-    ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], 2 * nfront));
-    ENG_TRY(grow_keep(ws->leafc, ws->leafc_cap, nleafc_total + nfront, nleafc_total, s));
-    A.front_in = ws->front[cur];
-    A.n_in = static_cast<long long>(nfront);
-    A.front_out = ws->front[nxt];
-    A.n_out = ws->counters + 1;
-    A.leafc = ws->leafc;
-    A.n_leafc = ws->counters + 2;
-    set_u64<<<1, 1, 0, s>>>(ws->counters + 1, 0);
-    engine_expand<<<grid_for(nfront), kBlock, 0, s>>>(A);
-    ENG_TRY(hipGetLastError());
-    ENG_TRY(read_counter(ws, 1, nfront, s));
-    ENG_TRY(read_counter(ws, 2, nleafc_total, s));
-    cur = nxt;
-    ++rep.levels;
-  }
-  mark(2);
-  // 3. leaf candidates of the breadth-first phase
-  if (nleafc_total > 0) {
-    A.leafc = ws->leafc;
-    A.n_in = static_cast<long long>(nleafc_total);
-    engine_leaves<<<grid_for(nleafc_total), kBlock, 0, s>>>(A);
-    ENG_TRY(hipGetLastError());
-  }
-  mark(3);
-  rep.front_size = static_cast<long long>(nfront);
-  // 4. persistent depth-first phase over the remaining front
-  size_t dfs_waves = 0;
-  if (nfront > 0) {
-    const long long waves_needed = (static_cast<long long>(nfront) + kWave - 1) / kWave;
-    const long long max_blocks = static_cast<long long>(cus) * tune.dfs_blocks_per_cu;
-    const unsigned blocks = static_cast<unsigned>(
-        std::max<long long>(1, std::min<long long>(max_blocks, (waves_needed + 3) / 4)));
-    const int stack_cap = 2 * kWave * (stack_need + 2) + 2 * kWave;
-    ENG_TRY(grow(ws->stacks, ws->stacks_cap,
-                 static_cast<size_t>(blocks) * kWavesPerBlock * stack_cap));
-    A.front_in = ws->front[cur];
-    A.n_in = static_cast<long long>(nfront);
-    A.head = ws->counters + 3;
-    A.stacks = ws->stacks;
-    A.stack_cap = stack_cap;
-    const size_t nwaves = static_cast<size_t>(blocks) * kWavesPerBlock;
-    A.wave_prof = nullptr;
-    if (ws->profile) {
-      ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * nwaves));
-      A.wave_prof = ws->wave_prof;
+
+  // Breadth-first expansion of front[cur] while it is small; leaf pairs of
+  // these levels are tested right after.
+  auto expand = [&]() -> hipError_t {
+    unsigned long long nleafc = 0;
+    ENG_TRY(clock_begin(ws, s, t));
+    for (int level = 0; level < tune.max_levels && nfront > 0 &&
+                        static_cast<long long>(nfront) < tune.front_target;
+         ++level) {
+      const int nxt = cur ^ 1;
+      ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], 2 * nfront));
+      ENG_TRY(grow_keep(ws->leafc, ws->leafc_cap, nleafc + nfront, nleafc, s));
+      EngineArgs X = A;
+      X.front_in = ws->front[cur];
+      X.n_in = static_cast<long long>(nfront);
+      X.front_out = ws->front[nxt];
+      X.n_out = C + kCtrExpandOut;
+      X.leafc = ws->leafc;
+      X.n_leafc = C + kCtrLeafc;
+      set_u64<<<1, 1, 0, s>>>(C + kCtrExpandOut, 0);
+      engine_expand<<<grid_for(nfront), kBlock, 0, s>>>(X);
+      ENG_TRY(hipGetLastError());
+      ENG_TRY(read_counter(ws, kCtrExpandOut, nfront, s));
+      ENG_TRY(read_counter(ws, kCtrLeafc, nleafc, s));
+      cur = nxt;
+      ++rep.levels;
     }
-    set_u64<<<1, 1, 0, s>>>(ws->counters + 3, 0);
-    engine_dfs<<<blocks, kBlock, 0, s>>>(A);
+    clock_end(ws, s, t, kPhaseExpand);
+    if (nleafc > 0) {
+      ENG_TRY(clock_begin(ws, s, t));
+      EngineArgs X = A;
+      X.leafc = ws->leafc;
+      X.n_in = static_cast<long long>(nleafc);
+      X.leaf_work = C + kCtrWork + kWorkLeafKernel;
+      engine_leaves<<<grid_for(nleafc), kBlock, 0, s>>>(X);
+      ENG_TRY(hipGetLastError());
+      clock_end(ws, s, t, kPhaseLeaves);
+      set_u64<<<1, 1, 0, s>>>(C + kCtrLeafc, 0);
+    }
+    return hipSuccess;
+  };
+
+  // 2. breadth-first expansion
+  ENG_TRY(expand());
+  rep.front_size = static_cast<long long>(nfront);
+
+  // 3. depth-first rounds over the remaining front
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, engine_dfs, kBlock, 0) != hipSuccess ||
+      occ < 1)
+    occ = 1;
+  const unsigned blocks =
+      static_cast<unsigned>(cus) * static_cast<unsigned>(std::min(occ, tune.dfs_blocks_per_cu));
+  const size_t nwaves = static_cast<size_t>(blocks) * kWavesPerBlock;
+  const int stack_cap = 2 * kWave * (stack_need + 2) + 2 * kWave;
+  const long long dcap[2] = {tune.defer_leaf, tune.defer_uncertain};
+  while (nfront > 0) {
+    if (rep.dfs_rounds >= 1000000) return hipErrorUnknown;  // no progress: never expected
+    const int nxt = cur ^ 1;
+    ENG_TRY(grow(ws->stacks, ws->stacks_cap, nwaves * stack_cap));
+    // next front: stack dumps, unconsumed input, children of uncertain tasks
+    ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], nwaves * stack_cap + nfront));
+    for (int w = 0; w < 2; ++w)
+      ENG_TRY(grow(ws->defer[w], ws->defer_cap[w], dcap[w] + nwaves * kLeafQ));
+    if (ws->profile) ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * nwaves));
+    zero_u64<<<1, kNumCounters, 0, s>>>(C + kCtrDefer0, kCtrNext + 1 - kCtrDefer0);
+    set_u64<<<1, 1, 0, s>>>(C + kCtrHead, 0);
+    EngineArgs D = A;
+    D.front_in = ws->front[cur];
+    D.n_in = static_cast<long long>(nfront);
+    D.front_out = ws->front[nxt];
+    D.n_out = C + kCtrNext;
+    D.head = C + kCtrHead;
+    D.stacks = ws->stacks;
+    D.stack_cap = stack_cap;
+    D.wave_prof = ws->profile ? ws->wave_prof : nullptr;
+    for (int w = 0; w < 2; ++w) {
+      D.defer[w] = ws->defer[w];
+      D.defer_cap[w] = dcap[w];
+    }
+    D.defer_ctl[0] = C + kCtrDefer0;
+    D.defer_ctl[1] = C + kCtrDefer1;
+    D.stop = C + kCtrStop;
+    D.idle = C + kCtrIdle;
+    // rebalance only when the front gives every wavefront a full step
+    D.idle_limit = static_cast<long long>(nfront) >= static_cast<long long>(kWave * nwaves)
+                       ? std::max<unsigned long long>(
+                             1, static_cast<unsigned long long>(tune.idle_stop * nwaves))
+                       : 0;
+    ENG_TRY(clock_begin(ws, s, t));
+    engine_dfs<<<blocks, kBlock, 0, s>>>(D);
     ENG_TRY(hipGetLastError());
-    dfs_waves = nwaves;
+    clock_end(ws, s, t, kPhaseDfs);
+    ++rep.dfs_rounds;
+    ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrHead, C + kCtrHead, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+    ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrDefer0, C + kCtrDefer0,
+                           (kCtrNext + 1 - kCtrDefer0) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+    ENG_TRY(hipStreamSynchronize(s));
+    const unsigned long long* H = ws->host_counters;
+    const unsigned long long head = H[kCtrHead];
+    const unsigned long long leaf_n = H[kCtrDefer0 + 1], leaf_d = H[kCtrDefer0 + 2];
+    const unsigned long long unc_n = H[kCtrDefer1 + 1], unc_d = H[kCtrDefer1 + 2];
+    unsigned long long nnext = H[kCtrNext];
+    if (ws->profile) ENG_TRY(summarize_waves(ws, nwaves, s));
+    // unconsumed input goes to the next front
+    const unsigned long long rem = nfront - std::min(head, nfront);
+    if (rem > 0) {
+      ENG_TRY(hipMemcpyAsync(ws->front[nxt] + nnext, ws->front[cur] + (nfront - rem),
+                             sizeof(unsigned long long) * rem, hipMemcpyDeviceToDevice, s));
+      nnext += rem;
+    }
+    // FP64 work deferred by the round
+    ENG_TRY(clock_begin(ws, s, t));
+    if (unc_n + unc_d > 0) {
+      ENG_TRY(grow_keep(ws->front[nxt], ws->front_cap[nxt], nnext + 2 * (unc_n + unc_d), nnext, s));
+      set_u64<<<1, 1, 0, s>>>(C + kCtrNext, nnext);
+      const unsigned long long* src[2] = {ws->defer[1], ws->defer[1] + dcap[1]};
+      const unsigned long long cnt[2] = {unc_n, unc_d};
+      for (int k = 0; k < 2; ++k) {
+        if (cnt[k] == 0) continue;
+        EngineArgs X = A;
+        X.leafc = const_cast<unsigned long long*>(src[k]);
+        X.n_in = static_cast<long long>(cnt[k]);
+        X.front_out = ws->front[nxt];
+        X.n_out = C + kCtrNext;
+        X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
+        engine_uncertain<<<grid_for(cnt[k]), kBlock, 0, s>>>(X);
+        ENG_TRY(hipGetLastError());
+      }
+    }
+    {
+      const unsigned long long* src[2] = {ws->defer[0], ws->defer[0] + dcap[0]};
+      const unsigned long long cnt[2] = {leaf_n, leaf_d};
+      for (int k = 0; k < 2; ++k) {
+        if (cnt[k] == 0) continue;
+        EngineArgs X = A;
+        X.leafc = const_cast<unsigned long long*>(src[k]);
+        X.n_in = static_cast<long long>(cnt[k]);
+        X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
+        engine_leaves<<<grid_for(cnt[k]), kBlock, 0, s>>>(X);
+        ENG_TRY(hipGetLastError());
+      }
+    }
+    clock_end(ws, s, t, kPhaseLeaves);
+    if (unc_n + unc_d > 0) ENG_TRY(read_counter(ws, kCtrNext, nnext, s));
+    cur = nxt;
+    nfront = nnext;
+    // widen a front too small to give every wavefront a full step
+    if (nfront > 0 && static_cast<long long>(nfront) < static_cast<long long>(kWave * nwaves))
+      ENG_TRY(expand());
   }
-  mark(4);
-  if (ws->profile && dfs_waves > 0) ENG_TRY(summarize_waves(ws, dfs_waves, s));
-  // 5. queries that reached the cap
+
+  // 4. queries that reached the cap
   unsigned long long nflag = 0;
-  ENG_TRY(read_counter(ws, 4, nflag, s));
+  ENG_TRY(read_counter(ws, kCtrFlagged, nflag, s));
   if (nflag > 0) {
+    ENG_TRY(clock_begin(ws, s, t));
     // per-query statistics of a truncated traversal need the depth-first
     // re-run; otherwise the recorded sets are truncated by depth-first keys
     // (which fit 64 bits while the summed tree depths do)
@@ -1240,25 +1525,19 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
       ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
     else
       ENG_TRY(fixup_keys(ws, L, nflag, s));
+    clock_end(ws, s, t, kPhaseFixup);
   }
-  mark(5);
   rep.flagged = static_cast<int>(nflag);
-  ENG_TRY(hipMemcpyAsync(ws->host_counters + 8, ws->counters + 8, 5 * sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, s));
+  ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrWork, C + kCtrWork,
+                         5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
-  rep.expand_bv_tests = static_cast<long long>(ws->host_counters[8 + kWorkExpandBV]);
-  rep.expand_leaf_tests = static_cast<long long>(ws->host_counters[8 + kWorkLeafKernel]);
-  rep.dfs_bv_tests = static_cast<long long>(ws->host_counters[8 + kWorkDfsBV]);
-  rep.dfs_leaf_tests = static_cast<long long>(ws->host_counters[8 + kWorkDfsLeaf]);
-  rep.fp64_fallbacks = static_cast<long long>(ws->host_counters[8 + kWorkFallback]);
-  if (ws->profile) {
-    float ms = 0;
-    double* out[5] = {&rep.setup_ms, &rep.expand_ms, &rep.leaves_ms, &rep.dfs_ms, &rep.fixup_ms};
-    for (int k = 0; k < 5; ++k) {
-      ENG_TRY(hipEventElapsedTime(&ms, ws->ev[k], ws->ev[k + 1]));
-      *out[k] = ms;
-    }
-  }
+  const unsigned long long* H = ws->host_counters + kCtrWork;
+  rep.expand_bv_tests = static_cast<long long>(H[kWorkExpandBV]);
+  rep.expand_leaf_tests = static_cast<long long>(H[kWorkLeafKernel]);
+  rep.dfs_bv_tests = static_cast<long long>(H[kWorkDfsBV]);
+  rep.dfs_leaf_tests = static_cast<long long>(H[kWorkDfsLeaf]);
+  rep.fp64_fallbacks = static_cast<long long>(H[kWorkFallback]);
+  ENG_TRY(clock_collect(ws));
   return hipSuccess;
 }
 

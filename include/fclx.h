@@ -154,11 +154,12 @@ typedef struct {
   int64_t front_size;        /* tasks handed to the depth-first kernel */
   int32_t levels;            /* breadth-first levels run */
   int32_t flagged;           /* queries whose contact count reached the cap */
-  /* depth-first load balance (profiling only): wavefronts that did work,
-   * their mean busy time over the kernel's span, and the times at which
-   * 50 % / 90 % / all of them had finished, from the first wave's start */
+  /* depth-first load balance (profiling only): wavefronts that did work in
+   * the first round, mean busy time over the span of every round, and the
+   * times at which 50 % / 90 % / all wavefronts of the first round had
+   * finished, from its first wavefront's start */
   int32_t dfs_waves;
-  int32_t reserved;
+  int32_t dfs_rounds;        /* depth-first rounds (work counts always) */
   double dfs_wave_busy;
   double dfs_end_p50_ms;
   double dfs_end_p90_ms;
@@ -166,6 +167,16 @@ typedef struct {
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
+/* Tunables of the FAST engine's exhaustive path, per context (no counterpart
+ * in the reference).  name / accepted range:
+ *   "max_levels"        1..64   breadth-first levels before a depth-first round
+ *   "front_target"      >= 64   stop expanding once the front holds this many tasks
+ *   "dfs_blocks_per_cu" 1..32   cap on resident depth-first blocks per CU
+ *   "defer_leaf"        >= 64   deferred leaf-pair buffer, entries
+ *   "defer_uncertain"   >= 64   deferred uncertain-task buffer, entries
+ *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
+ * Results do not depend on them; smaller buffers mean more rounds. */
+int fclx_set_engine_param(fclx_context* ctx, const char* name, double value);
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
 
 /* Wait for the stream, then report (and clear) the device status: FCLX_OK,

@@ -111,6 +111,31 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
     (cap=70 here) come back as the reference's depth-first prefix, by the
     exact re-run when statistics are requested and by depth-first keys
     otherwise."""
+    _fast_engine_case(ctx, cap, stats)
+
+
+STRESS = {"defer_leaf": 4096, "defer_uncertain": 64, "front_target": 100000,
+          "idle_stop": 0.25, "dfs_blocks_per_cu": 1}
+
+
+@pytest.mark.parametrize("cap,stats", [(100000, True), (70, False)])
+def test_fast_engine_round_boundaries(cap, stats):
+    """The same checks with tunables that end depth-first rounds constantly:
+    deferred buffers of a few thousand entries (rounds stop when they fill),
+    one resident block per CU, rebalancing once a quarter of the wavefronts
+    idle.  Stack hand-over, unconsumed input, uncertain-task
+    children and queue dumps all cross round boundaries many times."""
+    c = F.Context(0)
+    try:
+        for k, v in STRESS.items():
+            c.set_engine_param(k, v)
+        rep = _fast_engine_case(c, cap, stats)
+    finally:
+        c.close()
+    assert rep["dfs_rounds"] > 10
+
+
+def _fast_engine_case(ctx, cap, stats):
     (V1, T1), (V2, T2) = F.icosphere(9, 0.15, 21), F.heightfield(24, 0.05, 22)
     V2 = F.scale_to_radius(V2, 1.0)
     (V3, T3) = F.icosphere(6, 0.15, 23)
@@ -124,7 +149,7 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
     store = min(cap, 3000)
     oc, op, obv, olf = fclo.collide_batch(o, pairs, tf1, tf2, cap, store=store)
     gq = np.array(g, np.int32)[pairs]
-    print("case: fast exhaustive cap=%d n=%d" % (cap, n), flush=True)
+    print("case: fast exhaustive cap=%d n=%d stats=%s" % (cap, n, stats), flush=True)
     # before the fix-up a capped query has all its pairs recorded: room for all
     rc, gc, gp, gbv, glf, nrec = collide_gpu(ctx, gq, tf1, tf2, cap, engine=F.ENGINE_FAST,
                                              capacity=1 << 22, stats=stats)
@@ -145,6 +170,7 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
     if cap < 1000:
         assert flagged > 0
     assert oc.max() > 50
+    return ctx.engine_report()
 
 
 def test_env_rob_exhaustive(ctx):
