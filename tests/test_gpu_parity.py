@@ -114,14 +114,14 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
     _fast_engine_case(ctx, cap, stats)
 
 
-STRESS = {"defer_leaf": 4096, "defer_uncertain": 64, "front_target": 100000,
+STRESS = {"defer_leaf": 256, "defer_uncertain": 64, "front_target": 100000,
           "idle_stop": 0.25, "dfs_blocks_per_cu": 1}
 
 
 @pytest.mark.parametrize("cap,stats", [(100000, True), (70, False)])
 def test_fast_engine_round_boundaries(cap, stats):
     """The same checks with tunables that end depth-first rounds constantly:
-    deferred buffers of a few thousand entries (rounds stop when they fill),
+    deferred buffers of a few hundred entries (rounds stop when they fill),
     one resident block per CU, rebalancing once a quarter of the wavefronts
     idle.  Stack hand-over, unconsumed input, uncertain-task
     children and queue dumps all cross round boundaries many times."""
