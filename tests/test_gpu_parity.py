@@ -116,12 +116,12 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
 
 # "full": a tiny breadth-first front and deferred buffers of a few hundred
 # entries, so depth-first rounds stop constantly because a buffer is full;
-# "idle": a wide front and one resident block per CU, so rounds end once a
-# quarter of the wavefronts idle and the stacks are redistributed.
+# "idle": a wide front and one resident block per CU, and a round ends as
+# soon as one wavefront idles, so the other stacks are redistributed.
 STRESS = {
     "full": {"defer_leaf": 256, "defer_uncertain": 64, "front_target": 64, "max_levels": 2,
              "dfs_blocks_per_cu": 1},
-    "idle": {"front_target": 100000, "idle_stop": 0.25, "dfs_blocks_per_cu": 1},
+    "idle": {"front_target": 100000, "idle_stop": 0.0005, "dfs_blocks_per_cu": 1},
 }
 
 
