@@ -174,6 +174,29 @@ class Context:
             self._check(rc)
             return pairs[: n.value].copy(), stats
 
+    def collide_contact(self, m1, tf1, m2, tf2, cap=1):
+        """enable_contact form: (pairs (n,2), pos (n,3), normal (n,3), depth (n,), stats)."""
+        tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
+        tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
+        room = min(cap, 4096)
+        while True:
+            pairs = np.zeros((room, 2), np.int32)
+            pos = np.zeros((room, 3))
+            nrm = np.zeros((room, 3))
+            dep = np.zeros(room)
+            n = N.c_i32()
+            stats = np.zeros(2, np.int64)
+            rc = self._L.fclx_collide_contact(
+                self._h, m1, N.ptr(tf1, N.c_dbl), m2, N.ptr(tf2, N.c_dbl), cap,
+                N.ptr(pairs, N.c_i32), N.ptr(pos, N.c_dbl), N.ptr(nrm, N.c_dbl),
+                N.ptr(dep, N.c_dbl), room, ctypes.byref(n), N.ptr(stats, N.c_i64))
+            if rc == N.FCLX_ERR_CAPACITY:
+                room = n.value
+                continue
+            self._check(rc)
+            k = n.value
+            return pairs[:k].copy(), pos[:k].copy(), nrm[:k].copy(), dep[:k].copy(), stats
+
     def distance(self, m1, tf1, m2, tf2, nearest=True):
         tf1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
         tf2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
@@ -190,7 +213,9 @@ class Context:
     # batched calls on device tensors
     def collide_batch_async(self, query_meshes, tf1, tf2, num_max_contacts, counts, pairs,
                             num_pairs, engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,
-                            stream=None):
+                            stream=None, contact=None):
+        """contact: None, or (pos (K,3), normal (K,3), depth (K,)) float64 device
+        tensors for enable_contact (a record per contact)."""
         a = N.CollideBatchArgs()
         a.num_queries = int(query_meshes.shape[0])
         a.query_meshes = query_meshes.data_ptr()
@@ -204,6 +229,9 @@ class Context:
         a.num_pairs = num_pairs.data_ptr()
         a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
         a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
+        if contact is not None:
+            a.enable_contact = 1
+            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
         self._check(self._L.fclx_collide_batch(self._h, ctypes.byref(a), stream))
 
     def distance_batch_async(self, query_meshes, tf1, tf2, distance, points=None, ids=None,

@@ -47,7 +47,9 @@ class CollideBatchArgs(ctypes.Structure):
     _fields_ = [("num_queries", c_i32), ("query_meshes", c_vp), ("tf1", c_vp), ("tf2", c_vp),
                 ("num_max_contacts", c_i32), ("engine", c_i32), ("counts", c_vp),
                 ("pairs", c_vp), ("pair_capacity", c_i64), ("num_pairs", c_vp),
-                ("bv_tests", c_vp), ("leaf_tests", c_vp)]
+                ("bv_tests", c_vp), ("leaf_tests", c_vp), ("enable_contact", c_i32),
+                ("reserved", c_i32), ("contact_pos", c_vp), ("contact_normal", c_vp),
+                ("contact_depth", c_vp)]
 
 
 class DistanceBatchArgs(ctypes.Structure):
@@ -87,6 +89,8 @@ SIGNATURES = {
                               P_i64]),
     "fclx_distance": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_dbl, P_dbl, P_i32,
                               P_i64]),
+    "fclx_collide_contact": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, P_dbl,
+                                     P_dbl, P_dbl, c_i32, P_i32, P_i64]),
     "fclx_synth_icosphere_size": (None, [c_i32, P_i32, P_i32]),
     "fclx_synth_icosphere": (c_i32, [c_i32, c_dbl, c_u64, P_dbl, P_i32]),
     "fclx_synth_heightfield_size": (None, [c_i32, P_i32, P_i32]),

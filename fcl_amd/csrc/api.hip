@@ -63,6 +63,7 @@ struct fclx_context {
   long long* s_num = nullptr;    // 1
   long long* s_stats = nullptr;  // 2
   int32_t* s_pairs = nullptr;
+  double* s_geo = nullptr;       // contact geometry: 7 doubles per staged record
   long long s_pairs_cap = 0;
   double* s_dist = nullptr;      // 1 + 6 points
   int32_t* s_ids = nullptr;      // 2
@@ -181,6 +182,7 @@ void fclx_shutdown(fclx_context* ctx) {
   (void)hipFree(ctx->s_num);
   (void)hipFree(ctx->s_stats);
   (void)hipFree(ctx->s_pairs);
+  (void)hipFree(ctx->s_geo);
   (void)hipFree(ctx->s_dist);
   (void)hipFree(ctx->s_ids);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -398,7 +400,9 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
   if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
   if (a->num_queries < 0 || a->num_max_contacts < 1 || !a->counts || !a->num_pairs ||
       (a->num_queries > 0 && (!a->query_meshes || !a->tf1 || !a->tf2)) ||
-      (a->pair_capacity > 0 && !a->pairs))
+      (a->pair_capacity > 0 && !a->pairs) ||
+      (a->enable_contact && a->pairs &&
+       (!a->contact_pos || !a->contact_normal || !a->contact_depth)))
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid collide batch arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int depth, nmesh;
@@ -429,7 +433,16 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
   L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
   L.status = ctx->d_status;
   hipError_t e;
-  if (a->engine == FCLX_ENGINE_PLAIN) {
+  if (a->enable_contact) {
+    // contact geometry follows the reference's order and per-contact cap
+    if (a->engine != FCLX_ENGINE_PLAIN && a->engine != FCLX_ENGINE_FAST)
+      return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
+    L.contact = 1;
+    L.contact_pos = a->contact_pos;
+    L.contact_normal = a->contact_normal;
+    L.contact_depth = a->contact_depth;
+    e = launch_collide_plain(L, 2 * depth, s);
+  } else if (a->engine == FCLX_ENGINE_PLAIN) {
     e = launch_collide_plain(L, 2 * depth, s);
   } else if (a->engine == FCLX_ENGINE_FAST) {
     std::lock_guard<std::mutex> lock(ctx->ws_mu);
@@ -554,12 +567,14 @@ int fclx_sync(fclx_context* ctx, void* stream) {
   return FCLX_OK;
 }
 
-int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
-                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t pairs_capacity,
-                 int32_t* num_contacts, int64_t* stats) {
-  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1 || pairs_capacity < 0 ||
-      (pairs_capacity > 0 && !pairs))
-    return FCLX_ERR_INVALID_ARGUMENT;
+namespace {
+
+// One collide() query through the reference-order kernel with host staging;
+// contact mode also returns the geometry of each contact.
+int collide_one(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                const double tf2[12], int32_t cap, int32_t* pairs, double* pos, double* normal,
+                double* depth, int32_t capacity, int32_t* num_contacts, int64_t* stats,
+                bool contact) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (!valid_mesh(ctx, mesh1) || !valid_mesh(ctx, mesh2))
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
@@ -574,10 +589,10 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->s_tf, tf, sizeof(tf), hipMemcpyHostToDevice, s));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->s_qm, qm, sizeof(qm), hipMemcpyHostToDevice, s));
-  const int depth = ctx->meshes[mesh1].bvh.depth + ctx->meshes[mesh2].bvh.depth;
-  if (depth > kMaxStack)
+  const int depth_sum = ctx->meshes[mesh1].bvh.depth + ctx->meshes[mesh2].bvh.depth;
+  if (depth_sum > kMaxStack)
     return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
-  // device staging for the pairs starts small and grows to the count found
+  // device staging for the records starts small and grows to the count found
   long long want = std::min<long long>(cap, 4096);
   int32_t count = 0;
   long long st[2] = {0, 0};
@@ -585,9 +600,12 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
   for (int attempt = 0; attempt < 2; ++attempt) {
     if (want > ctx->s_pairs_cap) {
       (void)hipFree(ctx->s_pairs);
+      (void)hipFree(ctx->s_geo);
       ctx->s_pairs = nullptr;
+      ctx->s_geo = nullptr;
       ctx->s_pairs_cap = 0;
       HIP_TRY(ctx, dmalloc(&ctx->s_pairs, 3 * static_cast<size_t>(want)));
+      HIP_TRY(ctx, dmalloc(&ctx->s_geo, 7 * static_cast<size_t>(want)));
       ctx->s_pairs_cap = want;
     }
     zero_i64<<<1, 64, 0, s>>>(ctx->s_num, 1);
@@ -606,7 +624,13 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
     L.bv_tests = ctx->s_stats;
     L.leaf_tests = ctx->s_stats + 1;
     L.status = ctx->d_status;
-    HIP_TRY(ctx, launch_collide_plain(L, depth, s));
+    if (contact) {
+      L.contact = 1;
+      L.contact_pos = ctx->s_geo;
+      L.contact_normal = ctx->s_geo + 3 * ctx->s_pairs_cap;
+      L.contact_depth = ctx->s_geo + 6 * ctx->s_pairs_cap;
+    }
+    HIP_TRY(ctx, launch_collide_plain(L, depth_sum, s));
     HIP_TRY(ctx, hipMemcpyAsync(&count, ctx->s_count, sizeof(count), hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(&produced, ctx->s_num, sizeof(produced), hipMemcpyDeviceToHost,
@@ -625,7 +649,7 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
     stats[0] = st[0];
     stats[1] = st[1];
   }
-  const int32_t ncopy = std::min(count, pairs_capacity);
+  const int32_t ncopy = std::min(count, capacity);
   if (ncopy > 0) {
     std::vector<int32_t> rec(3 * static_cast<size_t>(ncopy));
     HIP_TRY(ctx, hipMemcpy(rec.data(), ctx->s_pairs, sizeof(int32_t) * rec.size(),
@@ -634,10 +658,41 @@ int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t
       pairs[2 * i] = rec[3 * i + 1];
       pairs[2 * i + 1] = rec[3 * i + 2];
     }
+    if (contact) {
+      const size_t cap_rec = static_cast<size_t>(ctx->s_pairs_cap);
+      HIP_TRY(ctx, hipMemcpy(pos, ctx->s_geo, sizeof(double) * 3 * ncopy, hipMemcpyDeviceToHost));
+      HIP_TRY(ctx, hipMemcpy(normal, ctx->s_geo + 3 * cap_rec, sizeof(double) * 3 * ncopy,
+                             hipMemcpyDeviceToHost));
+      HIP_TRY(ctx, hipMemcpy(depth, ctx->s_geo + 6 * cap_rec, sizeof(double) * ncopy,
+                             hipMemcpyDeviceToHost));
+    }
   }
-  if (count > pairs_capacity)
+  if (count > capacity)
     return fail(ctx, FCLX_ERR_CAPACITY, "pairs buffer smaller than the contact count");
   return FCLX_OK;
+}
+
+}  // namespace
+
+int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                 const double tf2[12], int32_t cap, int32_t* pairs, int32_t pairs_capacity,
+                 int32_t* num_contacts, int64_t* stats) {
+  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1 || pairs_capacity < 0 ||
+      (pairs_capacity > 0 && !pairs))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  return collide_one(ctx, mesh1, tf1, mesh2, tf2, cap, pairs, nullptr, nullptr, nullptr,
+                     pairs_capacity, num_contacts, stats, false);
+}
+
+int fclx_collide_contact(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                         const double tf2[12], int32_t cap, int32_t* pairs, double* pos,
+                         double* normal, double* depth, int32_t capacity,
+                         int32_t* num_contacts, int64_t* stats) {
+  if (!ctx || !tf1 || !tf2 || !num_contacts || cap < 1 || capacity < 0 ||
+      (capacity > 0 && (!pairs || !pos || !normal || !depth)))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  return collide_one(ctx, mesh1, tf1, mesh2, tf2, cap, pairs, pos, normal, depth, capacity,
+                     num_contacts, stats, true);
 }
 
 int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,

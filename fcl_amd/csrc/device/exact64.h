@@ -182,4 +182,84 @@ __device__ __forceinline__ bool tri_intersect64(const V3& P1, const V3& P2, cons
   return true;
 }
 
+// Plane of a triangle, n = normalised (v2-v1) x (v3-v1), t = n.v1
+// (intersect-inl.h buildTrianglePlane with geometry-inl.h normalize).  A
+// degenerate triangle has no plane; it gets n = 0, t = 0.
+__device__ __forceinline__ void triangle_plane64(const V3& a, const V3& b, const V3& c, V3& n,
+                                                 double& t) {
+  n = cross(b - a, c - a);
+  const double s = sq(n);
+  if (s > 0) {
+    const double l = sqrt(s);
+    n = v3(n.x / l, n.y / l, n.z / l);
+    t = dot(n, a);
+  } else {
+    n = v3(0, 0, 0);
+    t = 0;
+  }
+}
+
+// computeDeepestPoints (intersect-inl.h:848-885, epsilon 1e-5): the
+// vertices of a triangle deepest below the plane (n,t), within 1e-6 of the
+// deepest, none if all lie strictly on one side or all above -epsilon.
+__device__ __forceinline__ void deepest_points64(const V3 (&p)[3], const V3& n, double t,
+                                                 double& depth, V3 (&out)[3], unsigned& num) {
+  const double eps = 1e-5;
+  double max_depth = -1.7976931348623157e308;
+  unsigned k = 0, neg = 0, pos = 0, zero = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double dist = -(dot(n, p[i]) - t);
+    if (dist > eps) ++pos;
+    else if (dist < -eps) ++neg;
+    else ++zero;
+    if (dist > max_depth) {
+      max_depth = dist;
+      k = 1;
+      out[0] = p[i];
+    } else if (dist + 1e-6 >= max_depth) {
+      ++k;
+      out[k - 1] = p[i];
+    }
+  }
+  if (max_depth < -eps) k = 0;
+  if (zero == 0 && (neg == 0 || pos == 0)) k = 0;
+  depth = max_depth;
+  num = k;
+}
+
+// Contact output of intersect_Triangle (intersect-inl.h:802-842) for a pair
+// that passed the SAT: P in its own frame, Q already moved into it (R Q + T
+// as tri_intersect64 forms it).  Up to 2 points (P's frame), normal, depth.
+__device__ __forceinline__ unsigned tri_contact64(const V3& P1, const V3& P2, const V3& P3,
+                                                  const V3& Q1, const V3& Q2, const V3& Q3,
+                                                  V3 (&pts)[2], V3& normal, double& depth) {
+  V3 n1, n2;
+  double t1, t2;
+  triangle_plane64(P1, P2, P3, n1, t1);
+  triangle_plane64(Q1, Q2, Q3, n2, t2);
+  const V3 P[3] = {P1, P2, P3};
+  const V3 Q[3] = {Q1, Q2, Q3};
+  V3 dp1[3], dp2[3];
+  double pd1, pd2;
+  unsigned nd1, nd2;
+  deepest_points64(Q, n1, t1, pd2, dp2, nd2);
+  deepest_points64(P, n2, t2, pd1, dp1, nd1);
+  unsigned n;
+  if (pd1 > pd2) {
+    n = nd2 < 2 ? nd2 : 2;
+    pts[0] = dp2[0];
+    pts[1] = dp2[1];
+    normal = n1;
+    depth = pd2;
+  } else {
+    n = nd1 < 2 ? nd1 : 2;
+    pts[0] = dp1[0];
+    pts[1] = dp1[1];
+    normal = v3(-n2.x, -n2.y, -n2.z);
+    depth = pd1;
+  }
+  return n;
+}
+
 }  // namespace fclx

@@ -52,13 +52,18 @@ enum : int {
   kCtrFlagged = 4,    // queries that reached the cap
   // 5..7: cap fix-up
   kCtrWork = 8,       // 8 .. 8+kWork*: work counters
+  // the counters the depth-first kernel touches often sit on 128-B lines
+  // of their own, so atomics on one do not queue behind another
   kCtrDefer0 = 16,    // leaf pairs: reservation, written prefix end, dumps
-  kCtrDefer1 = 19,    // uncertain tasks: the same three
-  kCtrStop = 22,
-  kCtrIdle = 23,
-  kCtrNext = 24,      // next round's front
-  kNumCounters = 32
+  kCtrDefer1 = 32,    // uncertain tasks: the same three
+  kCtrStop = 48,      // polled by every wavefront
+  kCtrIdle = 64,
+  kCtrNext = 65,      // next round's front
+  kCtrRoundEnd = 66,  // one past the last counter a round resets
+  kNumCounters = 80
 };
+// depth-first steps between polls of the stop flag
+constexpr int kStopPoll = 16;
 
 struct QueryXf64 {
   double R[9];
@@ -444,13 +449,16 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
   bool stopped = false;
   unsigned long long n_bv = 0, n_unc = 0;  // wavefront-uniform
   const unsigned long long t_start = A.wave_prof ? wall_clock64() : 0;
+  unsigned step = 0;
 
   while (true) {
-    const int stop_seen =
-        __hip_atomic_load(A.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1 : 0;
-    if (__builtin_amdgcn_readfirstlane(stop_seen)) {  // one decision per wavefront
-      stopped = true;
-      break;
+    if ((++step % kStopPoll) == 0) {
+      const int stop_seen =
+          __hip_atomic_load(A.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1 : 0;
+      if (__builtin_amdgcn_readfirstlane(stop_seen)) {  // one decision per wavefront
+        stopped = true;
+        break;
+      }
     }
     // room for one full step in both queues
     const bool full = (nl >= kWave && !flush_deferred(A, 0, lq, nl, kWave)) ||
@@ -1424,7 +1432,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     for (int w = 0; w < 2; ++w)
       ENG_TRY(grow(ws->defer[w], ws->defer_cap[w], dcap[w] + nwaves * kLeafQ));
     if (ws->profile) ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * nwaves));
-    zero_u64<<<1, kNumCounters, 0, s>>>(C + kCtrDefer0, kCtrNext + 1 - kCtrDefer0);
+    zero_u64<<<1, kNumCounters, 0, s>>>(C + kCtrDefer0, kCtrRoundEnd - kCtrDefer0);
     set_u64<<<1, 1, 0, s>>>(C + kCtrHead, 0);
     EngineArgs D = A;
     D.front_in = ws->front[cur];
@@ -1456,7 +1464,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrHead, C + kCtrHead, sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
     ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrDefer0, C + kCtrDefer0,
-                           (kCtrNext + 1 - kCtrDefer0) * sizeof(unsigned long long),
+                           (kCtrRoundEnd - kCtrDefer0) * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
     ENG_TRY(hipStreamSynchronize(s));
     const unsigned long long* H = ws->host_counters;

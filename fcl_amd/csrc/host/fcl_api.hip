@@ -26,7 +26,7 @@ namespace detail {
 namespace {
 std::once_flag g_once;
 fclx_context* g_ctx = nullptr;
-std::atomic<bool> g_warned_contact{false}, g_warned_cost{false};
+std::atomic<bool> g_warned_cost{false};
 
 fclx_context* engine() {
   std::call_once(g_once, [] {
@@ -128,9 +128,6 @@ std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform
     return 0;
   }
   if (request.isSatisfied(result)) return result.numContacts();
-  if (request.enable_contact && !detail::g_warned_contact.exchange(true))
-    std::cerr << "fcl: contact points/normals are not computed on the GPU mesh path; "
-                 "contacts carry primitive ids only\n";
   if (request.enable_cost && !detail::g_warned_cost.exchange(true))
     std::cerr << "fcl: cost sources are not computed on the GPU mesh path\n";
   const std::size_t have = result.numContacts();
@@ -140,20 +137,41 @@ std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform
   detail::to_pose(tf1, p1);
   detail::to_pose(tf2, p2);
   fclx_context* ctx = detail::engine();
-  std::vector<int32_t> pairs(2 * std::min<int32_t>(cap, 4096));
+  int32_t room_n = std::min<int32_t>(cap, 4096);
+  std::vector<int32_t> pairs(2 * static_cast<size_t>(room_n));
+  std::vector<double> geo;  // enable_contact: pos (3), normal (3), depth per contact
+  auto run = [&](int32_t capacity, int32_t& n) {
+    if (!request.enable_contact)
+      return fclx_collide(ctx, id1, p1, id2, p2, cap, pairs.data(), capacity, &n, nullptr);
+    geo.resize(7 * static_cast<size_t>(capacity));
+    return fclx_collide_contact(ctx, id1, p1, id2, p2, cap, pairs.data(), geo.data(),
+                                geo.data() + 3 * static_cast<size_t>(capacity),
+                                geo.data() + 6 * static_cast<size_t>(capacity), capacity, &n,
+                                nullptr);
+  };
   int32_t n = 0;
-  int rc = fclx_collide(ctx, id1, p1, id2, p2, cap, pairs.data(),
-                        static_cast<int32_t>(pairs.size() / 2), &n, nullptr);
+  int rc = run(room_n, n);
   if (rc == FCLX_ERR_CAPACITY) {
+    room_n = n;
     pairs.resize(2 * static_cast<size_t>(n));
-    rc = fclx_collide(ctx, id1, p1, id2, p2, cap, pairs.data(), n, &n, nullptr);
+    rc = run(room_n, n);
   }
   if (rc != FCLX_OK) {
     std::cerr << "fcl: GPU collide failed: " << fclx_last_error(ctx) << "\n";
     return result.numContacts();
   }
-  for (int32_t i = 0; i < n; ++i)
-    result.addContact(Contact<double>(o1, o2, pairs[2 * i], pairs[2 * i + 1]));
+  for (int32_t i = 0; i < n; ++i) {
+    if (!request.enable_contact) {
+      result.addContact(Contact<double>(o1, o2, pairs[2 * i], pairs[2 * i + 1]));
+      continue;
+    }
+    const double* pos = geo.data() + 3 * static_cast<size_t>(i);
+    const double* nrm = geo.data() + 3 * static_cast<size_t>(room_n) + 3 * static_cast<size_t>(i);
+    const double depth = geo[6 * static_cast<size_t>(room_n) + i];
+    result.addContact(Contact<double>(o1, o2, pairs[2 * i], pairs[2 * i + 1],
+                                      Vector3d(pos[0], pos[1], pos[2]),
+                                      Vector3d(nrm[0], nrm[1], nrm[2]), depth));
+  }
   return result.numContacts();
 }
 

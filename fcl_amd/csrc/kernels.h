@@ -34,6 +34,12 @@ struct CollideLaunch {
   long long* bv_tests;         // optional, per query
   long long* leaf_tests;       // optional, per query
   int32_t* status;             // device status word (kStatus* bits)
+  // enable_contact (plain kernel only): a record per contact, with its
+  // world-frame point (3), normal (3) and penetration depth
+  int32_t contact;
+  double* contact_pos;
+  double* contact_normal;
+  double* contact_depth;
 };
 
 struct DistanceLaunch {

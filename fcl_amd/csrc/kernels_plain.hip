@@ -104,8 +104,47 @@ __global__ void __launch_bounds__(256) collide_plain_kernel(CollideLaunch L) {
         const int id1 = -(fc1 + 1), id2 = -(fc2 + 1);
         const int32_t* t1 = m1.tris + 3 * static_cast<size_t>(id1);
         const int32_t* t2 = m2.tris + 3 * static_cast<size_t>(id2);
-        if (tri_intersect64(vertex(m1, t1[0]), vertex(m1, t1[1]), vertex(m1, t1[2]),
-                            vertex(m2, t2[0]), vertex(m2, t2[1]), vertex(m2, t2[2]), R, T)) {
+        const V3 P1 = vertex(m1, t1[0]), P2 = vertex(m1, t1[1]), P3 = vertex(m1, t1[2]);
+        const V3 Q1 = vertex(m2, t2[0]), Q2 = vertex(m2, t2[1]), Q3 = vertex(m2, t2[2]);
+        if (L.contact) {
+          if (tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, T)) {
+            // mesh_collision_traversal_node-inl.h:578-598: the pair adds as
+            // many of its 0-2 contacts as the cap leaves room for
+            V3 pts[2], nrm;
+            double depth;
+            unsigned n = tri_contact64(P1, P2, P3, mul(R, Q1) + T, mul(R, Q2) + T,
+                                       mul(R, Q3) + T, pts, nrm, depth);
+            if (static_cast<unsigned>(cap) < static_cast<unsigned>(count) + n)
+              n = cap > count ? static_cast<unsigned>(cap - count) : 0u;
+            const double* w = L.tf1 + 12 * static_cast<size_t>(q);
+            for (unsigned i = 0; i < n; ++i) {
+              const long long slot = static_cast<long long>(
+                  atomicAdd(reinterpret_cast<unsigned long long*>(L.num_pairs), 1ULL));
+              if (slot < L.pair_capacity) {
+                L.pairs[3 * slot] = q;
+                L.pairs[3 * slot + 1] = id1;
+                L.pairs[3 * slot + 2] = id2;
+                const V3 pw = v3(dot(v3(w[0], w[1], w[2]), pts[i]) + w[9],
+                                 dot(v3(w[3], w[4], w[5]), pts[i]) + w[10],
+                                 dot(v3(w[6], w[7], w[8]), pts[i]) + w[11]);
+                const V3 nw = v3(dot(v3(w[0], w[1], w[2]), nrm), dot(v3(w[3], w[4], w[5]), nrm),
+                                 dot(v3(w[6], w[7], w[8]), nrm));
+                double* pp = L.contact_pos + 3 * slot;
+                double* np = L.contact_normal + 3 * slot;
+                pp[0] = pw.x;
+                pp[1] = pw.y;
+                pp[2] = pw.z;
+                np[0] = nw.x;
+                np[1] = nw.y;
+                np[2] = nw.z;
+                L.contact_depth[slot] = depth;
+              } else {
+                atomicOr(L.status, kStatusPairOverflow);
+              }
+            }
+            count += static_cast<int>(n);
+          }
+        } else if (tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, T)) {
           if (count < cap) {
             const long long slot = static_cast<long long>(
                 atomicAdd(reinterpret_cast<unsigned long long*>(L.num_pairs), 1ULL));

@@ -321,6 +321,11 @@ struct Contact {
   Contact() = default;
   Contact(const CollisionGeometry<S>* o1_, const CollisionGeometry<S>* o2_, int b1_, int b2_)
       : o1(o1_), o2(o2_), b1(b1_), b2(b2_) {}
+  // contact-inl.h:78-94: with the contact geometry (enable_contact)
+  Contact(const CollisionGeometry<S>* o1_, const CollisionGeometry<S>* o2_, int b1_, int b2_,
+          const Vector3<S>& pos_, const Vector3<S>& normal_, S depth_)
+      : o1(o1_), o2(o2_), b1(b1_), b2(b2_), normal(normal_), pos(pos_),
+        penetration_depth(depth_) {}
   bool operator<(const Contact& o) const {
     if (b1 == o.b1) return b2 < o.b2;
     return b1 < o.b1;

@@ -84,6 +84,16 @@ typedef struct {
   int64_t* num_pairs;          /* device: 1 int64, records produced (set by the call) */
   int64_t* bv_tests;           /* device, optional: per-query num_bv_tests */
   int64_t* leaf_tests;         /* device, optional: per-query num_leaf_tests */
+  /* CollisionRequest::enable_contact (mesh_collision_traversal_node-inl.h:
+   * 578-598): each record is one contact (a triangle pair gives 0-2), counts
+   * and num_max_contacts count contacts, and the geometry below is written
+   * per record in the world frame.  Runs the reference-order kernel whatever
+   * the engine.  The three arrays are required when pairs is set. */
+  int32_t enable_contact;
+  int32_t reserved;
+  double* contact_pos;         /* device: 3 per record */
+  double* contact_normal;      /* device: 3 per record */
+  double* contact_depth;       /* device: 1 per record */
 } fclx_collide_batch_args;
 
 typedef struct {
@@ -195,6 +205,14 @@ int fclx_sync(fclx_context* ctx, void* stream);
 int fclx_collide(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
                  const double tf2[12], int32_t cap, int32_t* pairs, int32_t pairs_capacity,
                  int32_t* num_contacts, int64_t* stats);
+/* fclx_collide with enable_contact (mesh_collision_traversal_node-inl.h:
+ * 578-598): up to cap contacts in the reference's order, each with its
+ * (b1, b2) in pairs, world-frame point (3 doubles) in pos, normal (3) in
+ * normal and penetration depth in depth; capacity counts contacts. */
+int fclx_collide_contact(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_t mesh2,
+                         const double tf2[12], int32_t cap, int32_t* pairs, double* pos,
+                         double* normal, double* depth, int32_t capacity,
+                         int32_t* num_contacts, int64_t* stats);
 /* One query: the drop-in path behind fcl::distance for two BVHModel<RSS> or
  * BVHModel<OBBRSS> (orientedMeshDistance, distance_func_matrix-inl.h:386-404).
  * points (optional) = nearest points on mesh1 and mesh2 in the world frame;

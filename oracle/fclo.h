@@ -85,6 +85,14 @@ int fclo_intersect_triangle(const double P1[3], const double P2[3],
                             const double P3[3], const double Q1[3],
                             const double Q2[3], const double Q3[3],
                             const double R[9], const double T[3]);
+/* The same test with the reference's contact output (enable_contact):
+ * up to 2 contact points (frame of P), the normal and the depth. */
+int fclo_intersect_triangle_contact(const double P1[3], const double P2[3],
+                                    const double P3[3], const double Q1[3],
+                                    const double Q2[3], const double Q3[3],
+                                    const double R[9], const double T[3],
+                                    double contacts[6], unsigned *num_contacts,
+                                    double *depth, double normal[3]);
 double fclo_rect_distance(const double Rab[9], const double Tab[3],
                           const double a[2], const double b[2], double P[3],
                           double Q[3]);
@@ -107,6 +115,14 @@ typedef struct {
 void fclo_collide(const fclo_model *m1, const double tf1[12],
                   const fclo_model *m2, const double tf2[12], int cap,
                   int *pairs, fclo_collide_stats *stats);
+
+/* collide() with enable_contact: cap counts contacts (a triangle pair adds
+ * 0-2); the first min(count, store) contacts are written as (b1,b2) pairs
+ * plus world-frame pos[3], normal[3] and depth. */
+void fclo_collide_contact(const fclo_model *m1, const double tf1[12],
+                          const fclo_model *m2, const double tf2[12], int cap,
+                          int store, int *pairs, double *pos, double *normal,
+                          double *depth, fclo_collide_stats *stats);
 
 /* ---- distance ------------------------------------------------------------ */
 typedef struct {

@@ -70,6 +70,10 @@ def lib():
             "fclo_node_size": (c_dbl, [PM, c_i32]),
             "fclo_obb_disjoint": (c_i32, [P_dbl, P_dbl, P_dbl, P_dbl]),
             "fclo_intersect_triangle": (c_i32, [P_dbl] * 8),
+            "fclo_intersect_triangle_contact": (c_i32, [P_dbl] * 9 + [ctypes.POINTER(ctypes.c_uint),
+                                                                      P_dbl, P_dbl]),
+            "fclo_collide_contact": (None, [PM, P_dbl, PM, P_dbl, c_i32, c_i32, P_i32, P_dbl,
+                                            P_dbl, P_dbl, ctypes.POINTER(CollideStats)]),
             "fclo_rect_distance": (c_dbl, [P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
             "fclo_rss_distance": (c_dbl, [P_dbl, P_dbl, ctypes.POINTER(Node),
                                           ctypes.POINTER(Node)]),
@@ -151,6 +155,26 @@ def collide(m1, tf1, m2, tf2, cap=1):
     lib().fclo_collide(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m), _p(tf2, c_dbl),
                        cap, _p(pairs, c_i32), ctypes.byref(st))
     return pairs[: st.num_contacts].copy(), (st.num_bv_tests, st.num_leaf_tests)
+
+
+def collide_contact(m1, tf1, m2, tf2, cap=1, store=None):
+    """collide() with enable_contact: (pairs (n,2), pos (n,3), normal (n,3),
+    depth (n,), (num_bv_tests, num_leaf_tests)); n = contacts."""
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    store = cap if store is None else store
+    k = max(store, 1)
+    pairs = np.zeros((k, 2), np.int32)
+    pos = np.zeros((k, 3))
+    nrm = np.zeros((k, 3))
+    dep = np.zeros(k)
+    st = CollideStats()
+    lib().fclo_collide_contact(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m),
+                               _p(tf2, c_dbl), cap, store, _p(pairs, c_i32), _p(pos, c_dbl),
+                               _p(nrm, c_dbl), _p(dep, c_dbl), ctypes.byref(st))
+    n = min(st.num_contacts, store)
+    return (pairs[:n].copy(), pos[:n].copy(), nrm[:n].copy(), dep[:n].copy(),
+            (st.num_bv_tests, st.num_leaf_tests), st.num_contacts)
 
 
 def distance(m1, tf1, m2, tf2, nearest=True):
@@ -245,6 +269,24 @@ def intersect_triangle(P, Q, R=None, T=None):
     qs = [np.ascontiguousarray(Q[i]) for i in range(3)]
     args += [_p(q, c_dbl) for q in qs] + [_p(R, c_dbl), _p(T, c_dbl)]
     return lib().fclo_intersect_triangle(*args)
+
+
+def intersect_triangle_contact(P, Q, R=None, T=None):
+    """Contact form: (intersects, points (n,3) in P's frame, normal, depth)."""
+    P = _d(P)
+    Q = _d(Q)
+    R = _d(np.eye(3) if R is None else R)
+    T = _d(np.zeros(3) if T is None else T)
+    args = [_p(np.ascontiguousarray(P[i]), c_dbl) for i in range(3)]
+    qs = [np.ascontiguousarray(Q[i]) for i in range(3)]
+    args += [_p(q, c_dbl) for q in qs] + [_p(R, c_dbl), _p(T, c_dbl)]
+    pts = np.zeros(6)
+    n = ctypes.c_uint(0)
+    depth = np.zeros(1)
+    normal = np.zeros(3)
+    hit = lib().fclo_intersect_triangle_contact(*args, _p(pts, c_dbl), ctypes.byref(n),
+                                                _p(depth, c_dbl), _p(normal, c_dbl))
+    return bool(hit), pts[: 3 * n.value].reshape(-1, 3), normal, float(depth[0])
 
 
 def rect_distance(Rab, Tab, a, b):

@@ -2,6 +2,7 @@
  * only): the 15-axis OBB separating-axis test, the 17-axis triangle SAT and
  * the recursive BVTT depth-first traversal with the reference's descent,
  * early-stop and contact-cap rules. */
+#include <float.h>
 #include <math.h>
 #include <string.h>
 #include "fclo.h"
@@ -178,6 +179,97 @@ int fclo_intersect_triangle(const double P1[3], const double P2[3],
   return 1;
 }
 
+/* intersect-inl.h:1054-1068 buildTrianglePlane with normalize
+ * (math/geometry-inl.h:413-426).  For a degenerate triangle the reference
+ * leaves the plane uninitialised; here it is n = 0, t = 0. */
+static void triangle_plane(const double v1[3], const double v2[3], const double v3[3],
+                           double n[3], double *t) {
+  double a[3], b[3];
+  sub3(v2, v1, a);
+  sub3(v3, v1, b);
+  fclo_cross(a, b, n);
+  double sq = fclo_dot(n, n);
+  if (sq > 0) {
+    double l = sqrt(sq);
+    n[0] /= l;
+    n[1] /= l;
+    n[2] /= l;
+    *t = fclo_dot(n, v1);
+  } else {
+    n[0] = n[1] = n[2] = 0;
+    *t = 0;
+  }
+}
+
+/* intersect-inl.h:848-885 computeDeepestPoints (epsilon 1e-5, :1116-1120). */
+static void deepest_points(const double pts[3][3], const double n[3], double t,
+                           double *depth, double out[3][3], unsigned *num) {
+  const double eps = 1e-5;
+  double max_depth = -DBL_MAX;
+  unsigned k = 0, neg = 0, pos = 0, zero = 0;
+  for (int i = 0; i < 3; ++i) {
+    double dist = -(fclo_dot(n, pts[i]) - t);
+    if (dist > eps) pos++;
+    else if (dist < -eps) neg++;
+    else zero++;
+    if (dist > max_depth) {
+      max_depth = dist;
+      k = 1;
+      memcpy(out[0], pts[i], sizeof(out[0]));
+    } else if (dist + 1e-6 >= max_depth) {
+      k++;
+      memcpy(out[k - 1], pts[i], sizeof(out[0]));
+    }
+  }
+  if (max_depth < -eps) k = 0;
+  if (zero == 0 && (neg == 0 || pos == 0)) k = 0;
+  *depth = max_depth;
+  *num = k;
+}
+
+/* intersect-inl.h:597-617 then 727-845 with contact output: the SAT of
+ * fclo_intersect_triangle, then planes of P and of Q' = R Q + T, the deepest
+ * points of each triangle below the other's plane, and the shallower
+ * penetration's points (at most 2), normal and depth.  Points are in the
+ * frame of P.  Returns 1 if the triangles intersect. */
+int fclo_intersect_triangle_contact(const double P1[3], const double P2[3],
+                                    const double P3[3], const double Q1[3],
+                                    const double Q2[3], const double Q3[3],
+                                    const double R[9], const double T[3],
+                                    double contacts[6], unsigned *num_contacts,
+                                    double *depth, double normal[3]) {
+  if (!fclo_intersect_triangle(P1, P2, P3, Q1, Q2, Q3, R, T)) return 0;
+  double Q[3][3];
+  const double *Qs[3] = {Q1, Q2, Q3};
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 3; ++i) Q[k][i] = fclo_dot(R + 3 * i, Qs[k]) + T[i];
+  double P[3][3];
+  memcpy(P[0], P1, sizeof(P[0]));
+  memcpy(P[1], P2, sizeof(P[0]));
+  memcpy(P[2], P3, sizeof(P[0]));
+  double n1[3], n2[3], t1, t2;
+  triangle_plane(P[0], P[1], P[2], n1, &t1);
+  triangle_plane(Q[0], Q[1], Q[2], n2, &t2);
+  double dp1[3][3], dp2[3][3], pd1, pd2;
+  unsigned nd1, nd2;
+  deepest_points(Q, n1, t1, &pd2, dp2, &nd2);
+  deepest_points(P, n2, t2, &pd1, dp1, &nd1);
+  if (pd1 > pd2) {
+    *num_contacts = nd2 < 2 ? nd2 : 2;
+    for (unsigned i = 0; i < *num_contacts; ++i) memcpy(contacts + 3 * i, dp2[i], 3 * sizeof(double));
+    memcpy(normal, n1, 3 * sizeof(double));
+    *depth = pd2;
+  } else {
+    *num_contacts = nd1 < 2 ? nd1 : 2;
+    for (unsigned i = 0; i < *num_contacts; ++i) memcpy(contacts + 3 * i, dp1[i], 3 * sizeof(double));
+    normal[0] = -n2[0];
+    normal[1] = -n2[1];
+    normal[2] = -n2[2];
+    *depth = pd1;
+  }
+  return 1;
+}
+
 typedef struct {
   const fclo_model *m1, *m2;
   double R[9], T[3];
@@ -186,6 +278,10 @@ typedef struct {
   int count;
   int *pairs;
   int64_t bv_tests, leaf_tests;
+  /* enable_contact: geometry per stored contact, world frame */
+  int contact;
+  const double *tf1;
+  double *pos, *nrm, *depth;
 } collide_ctx;
 
 /* isSatisfied (collision_request-inl.h:77-82) with enable_cost off: at least
@@ -199,13 +295,40 @@ static int bv_disjoint(collide_ctx *c, int b1, int b2) {
   return !fclo_obb_overlap(c->R, c->T, c->m1->bvs + b1, c->m2->bvs + b2);
 }
 
-/* mesh_collision_traversal_node-inl.h:526-620 (leaf, enable_contact off). */
+/* mesh_collision_traversal_node-inl.h:526-620 (leaf).  With enable_contact a
+ * pair adds its 0-2 contacts, as many as the cap leaves room for, with
+ * pos = tf1 * point, normal = tf1.linear() * normal. */
 static void leaf_test(collide_ctx *c, int b1, int b2) {
   c->leaf_tests++;
   int id1 = -(c->m1->bvs[b1].first_child + 1);
   int id2 = -(c->m2->bvs[b2].first_child + 1);
   const int *t1 = c->m1->tris + 3 * id1, *t2 = c->m2->tris + 3 * id2;
   const double *v1 = c->m1->vertices, *v2 = c->m2->vertices;
+  if (c->contact) {
+    double pts[6], nrm[3], depth;
+    unsigned n = 0;
+    if (!fclo_intersect_triangle_contact(v1 + 3 * t1[0], v1 + 3 * t1[1], v1 + 3 * t1[2],
+                                         v2 + 3 * t2[0], v2 + 3 * t2[1], v2 + 3 * t2[2],
+                                         c->R, c->T, pts, &n, &depth, nrm))
+      return;
+    if ((unsigned)c->cap < (unsigned)c->count + n)
+      n = c->cap > c->count ? (unsigned)(c->cap - c->count) : 0;
+    const double *tf = c->tf1;
+    for (unsigned i = 0; i < n; ++i) {
+      if (c->count < c->store) {
+        int k = c->count;
+        c->pairs[2 * k] = id1;
+        c->pairs[2 * k + 1] = id2;
+        for (int r = 0; r < 3; ++r) {
+          c->pos[3 * k + r] = fclo_dot(tf + 3 * r, pts + 3 * i) + tf[9 + r];
+          c->nrm[3 * k + r] = fclo_dot(tf + 3 * r, nrm);
+        }
+        c->depth[k] = depth;
+      }
+      c->count++;
+    }
+    return;
+  }
   if (fclo_intersect_triangle(v1 + 3 * t1[0], v1 + 3 * t1[1], v1 + 3 * t1[2],
                               v2 + 3 * t2[0], v2 + 3 * t2[1], v2 + 3 * t2[2],
                               c->R, c->T)) {
@@ -245,7 +368,8 @@ static void collide_rec(collide_ctx *c, int b1, int b2) {
 
 static void collide_store(const fclo_model *m1, const double tf1[12],
                           const fclo_model *m2, const double tf2[12], int cap,
-                          int store, int *pairs, fclo_collide_stats *stats) {
+                          int store, int *pairs, double *pos, double *nrm,
+                          double *depth, fclo_collide_stats *stats) {
   collide_ctx c;
   memset(&c, 0, sizeof(c));
   c.m1 = m1;
@@ -253,6 +377,11 @@ static void collide_store(const fclo_model *m1, const double tf1[12],
   c.cap = cap;
   c.store = store;
   c.pairs = pairs;
+  c.contact = pos != 0;
+  c.tf1 = tf1;
+  c.pos = pos;
+  c.nrm = nrm;
+  c.depth = depth;
   /* mesh_collision_traversal_node-inl.h:742 */
   fclo_relative_transform(tf1, tf2, c.R, c.T);
   if (cap > 0) collide_rec(&c, 0, 0);
@@ -264,7 +393,14 @@ static void collide_store(const fclo_model *m1, const double tf1[12],
 void fclo_collide(const fclo_model *m1, const double tf1[12],
                   const fclo_model *m2, const double tf2[12], int cap,
                   int *pairs, fclo_collide_stats *stats) {
-  collide_store(m1, tf1, m2, tf2, cap, cap, pairs, stats);
+  collide_store(m1, tf1, m2, tf2, cap, cap, pairs, 0, 0, 0, stats);
+}
+
+void fclo_collide_contact(const fclo_model *m1, const double tf1[12],
+                          const fclo_model *m2, const double tf2[12], int cap,
+                          int store, int *pairs, double *pos, double *normal,
+                          double *depth, fclo_collide_stats *stats) {
+  collide_store(m1, tf1, m2, tf2, cap, store, pairs, pos, normal, depth, stats);
 }
 
 void fclo_collide_batch(const fclo_model *const *models, int num_queries,
@@ -277,7 +413,7 @@ void fclo_collide_batch(const fclo_model *const *models, int num_queries,
     fclo_collide_stats st;
     collide_store(models[q_m1[q]], tf1s + 12 * (size_t)q, models[q_m2[q]],
                   tf2s + 12 * (size_t)q, cap, store,
-                  pairs ? pairs + 2 * (size_t)q * (size_t)store : 0, &st);
+                  pairs ? pairs + 2 * (size_t)q * (size_t)store : 0, 0, 0, 0, &st);
     counts[q] = st.num_contacts;
     if (bv_tests) bv_tests[q] = st.num_bv_tests;
     if (leaf_tests) leaf_tests[q] = st.num_leaf_tests;
