@@ -129,6 +129,16 @@ int max_depth(const fclx_context* ctx) {
   return d;
 }
 
+size_t max_nodes(const fclx_context* ctx) {
+  size_t n = 0;
+  for (const auto& m : ctx->meshes)
+    if (m.live) n = std::max(n, m.bvh.nodes.size());
+  return n;
+}
+
+// The FAST engine packs (query, node, node) into 64 bits: 21 bits per node.
+constexpr size_t kFastMaxNodes = size_t(1) << 21;
+
 // Largest L1 norm of an OBB centre over the mesh's nodes; the FP32 cull's
 // translation error term scales with it.
 double centre_bound(const std::vector<HostNode>& nodes) {
@@ -406,10 +416,12 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid collide batch arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int depth, nmesh;
+  size_t nodes;
   const MeshDev* table;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     depth = max_depth(ctx);
+    nodes = max_nodes(ctx);
     nmesh = static_cast<int>(ctx->meshes.size());
     table = ctx->d_table;
   }
@@ -442,7 +454,10 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     L.contact_normal = a->contact_normal;
     L.contact_depth = a->contact_depth;
     e = launch_collide_plain(L, 2 * depth, s);
-  } else if (a->engine == FCLX_ENGINE_PLAIN) {
+  } else if (a->engine == FCLX_ENGINE_PLAIN ||
+             (a->engine == FCLX_ENGINE_FAST && a->num_max_contacts > kOrderedCapMax &&
+              nodes >= kFastMaxNodes)) {
+    // meshes beyond the FAST engine's node-index width take the plain kernel
     e = launch_collide_plain(L, 2 * depth, s);
   } else if (a->engine == FCLX_ENGINE_FAST) {
     std::lock_guard<std::mutex> lock(ctx->ws_mu);

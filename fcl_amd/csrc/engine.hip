@@ -65,9 +65,15 @@ enum : int {
 // depth-first steps between polls of the stop flag
 constexpr int kStopPoll = 16;
 
-struct QueryXf64 {
+// FP64 relative pose of a query plus its meshes' triangle and vertex
+// arrays: everything a triangle test needs, in one 128-B line.
+struct alignas(128) QueryXf64 {
   double R[9];
   double T[3];
+  const double* v1;
+  const double* v2;
+  const int32_t* tri1;
+  const int32_t* tri2;
 };
 
 // task = (query:22 | b1:21 | b2:21)
@@ -206,6 +212,10 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
       QueryXf64 x;
       rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q), x.R,
                  x.T);
+      x.v1 = L.meshes[i1].verts;
+      x.v2 = L.meshes[i2].verts;
+      x.tri1 = L.meshes[i1].tris;
+      x.tri2 = L.meshes[i2].tris;
       A.qx64[q] = x;
       make_xf32(x.R, x.T, L.meshes[i1].cmax, L.meshes[i2].cmax, X);
       X.m1 = i1;
@@ -227,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
 // Outcome of testing one BVTT task.
 struct Visit {
   bool push = false;   // overlapping internal pair: children c0, c1
-  bool leafc = false;  // overlapping leaf pair: triangle test pending
+  bool leafc = false;  // overlapping leaf pair: triangle test pending, c0 = (q, tri1, tri2)
   bool fallback = false;
   unsigned long long c0 = 0, c1 = 0;
 };
@@ -262,6 +272,7 @@ __device__ __forceinline__ Visit visit_task(const EngineArgs& A, unsigned long l
     const int fc1 = n1.first_child, fc2 = n2.first_child;
     if (fc1 < 0 && fc2 < 0) {
       v.leafc = true;
+      v.c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));
     } else {
       v.push = true;
       if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
@@ -296,24 +307,25 @@ __device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q
   }
 }
 
+// Triangle test of a leaf candidate (q, tri1, tri2).
 __device__ __forceinline__ bool leaf_hit(const EngineArgs& A, unsigned long long task, int& q,
                                          int& id1, int& id2) {
-  int b1, b2;
-  unpack_task(task, q, b1, b2);
-  const QueryXf32& X = A.qx32[q];
-  const MeshDev& m1 = A.L.meshes[X.m1];
-  const MeshDev& m2 = A.L.meshes[X.m2];
-  id1 = -(m1.obb64[b1].first_child + 1);
-  id2 = -(m2.obb64[b2].first_child + 1);
-  const int32_t* t1 = m1.tris + 3 * static_cast<size_t>(id1);
-  const int32_t* t2 = m2.tris + 3 * static_cast<size_t>(id2);
+  unpack_task(task, q, id1, id2);
   const QueryXf64& x = A.qx64[q];
+  const int32_t* t1 = x.tri1 + 3 * static_cast<size_t>(id1);
+  const int32_t* t2 = x.tri2 + 3 * static_cast<size_t>(id2);
+  const int a0 = t1[0], a1 = t1[1], a2 = t1[2], c0 = t2[0], c1 = t2[1], c2 = t2[2];
+  auto vert = [](const double* v, int i) {
+    const double* p = v + 3 * static_cast<size_t>(i);
+    return v3(p[0], p[1], p[2]);
+  };
+  const V3 P1 = vert(x.v1, a0), P2 = vert(x.v1, a1), P3 = vert(x.v1, a2);
+  const V3 Q1 = vert(x.v2, c0), Q2 = vert(x.v2, c1), Q3 = vert(x.v2, c2);
   M3 R;
 #pragma unroll
   for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
   if (A.L.leaf_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.leaf_tests[q]), 1ULL);
-  return tri_intersect64(vtx(m1, t1[0]), vtx(m1, t1[1]), vtx(m1, t1[2]), vtx(m2, t2[0]),
-                         vtx(m2, t2[1]), vtx(m2, t2[2]), R, v3(x.T[0], x.T[1], x.T[2]));
+  return tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, v3(x.T[0], x.T[1], x.T[2]));
 }
 
 // ------------------------------------------------------- front expansion
@@ -334,7 +346,7 @@ __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
     A.front_out[so + 1] = v.c1;
   }
   const unsigned long long sl = wave_reserve(A.n_leafc, v.leafc, 1);
-  if (v.leafc) A.leafc[sl] = task;
+  if (v.leafc) A.leafc[sl] = v.c0;
 }
 
 __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
@@ -365,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) engine_uncertain(EngineArgs A) {
   }
   bool hit = false;
   int q = 0, id1 = 0, id2 = 0;
-  if (v.leafc) hit = leaf_hit(A, task, q, id1, id2);
+  if (v.leafc) hit = leaf_hit(A, v.c0, q, id1, id2);
   wave_count(A.leaf_work, v.leafc);
   record_hits(A, hit, q, id1, id2);
 }
@@ -389,7 +401,10 @@ __device__ __forceinline__ int step32(const EngineArgs& A, unsigned long long ta
   if (r == kCullDisjoint) return kStepNone;
   if (r == kCullUnsure) return kStepUncertain;
   const int fc1 = n1.first_child, fc2 = n2.first_child;
-  if (fc1 < 0 && fc2 < 0) return kStepLeaf;
+  if (fc1 < 0 && fc2 < 0) {
+    c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));  // leaf candidate: (q, tri1, tri2)
+    return kStepLeaf;
+  }
   if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
     c0 = pack_task(q, fc1, b2);
     c1 = pack_task(q, fc1 + 1, b2);
@@ -504,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     // leaf pairs and uncertain tasks into the LDS queues
     const unsigned long long lm = __ballot(r == kStepLeaf);
     const unsigned long long um = __ballot(r == kStepUncertain);
-    if (r == kStepLeaf) lq[nl + __popcll(lm & lanes_below())] = task;
+    if (r == kStepLeaf) lq[nl + __popcll(lm & lanes_below())] = c0;
     if (r == kStepUncertain) uq[nu + __popcll(um & lanes_below())] = task;
     __builtin_amdgcn_wave_barrier();
     nl += __popcll(lm);

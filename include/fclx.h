@@ -55,7 +55,9 @@ extern "C" {
  * a query come out in no particular order).  A query that reaches the cap
  * gets the reference's depth-first prefix, in order: its records are ordered
  * by depth-first path keys and cut, or, when per-query statistics are
- * requested, the query is re-run by PLAIN. */
+ * requested, the query is re-run by PLAIN.  Batches over a context holding
+ * a mesh of 2^20 or more triangles run PLAIN for caps above 64 (the FAST
+ * engine's task encoding has 21 bits per node index). */
 #define FCLX_ENGINE_PLAIN 0
 #define FCLX_ENGINE_FAST 1
 

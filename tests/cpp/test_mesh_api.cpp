@@ -200,6 +200,38 @@ int main(int argc, char** argv) {
     }
   }
 
+  // enable_contact: contact points, normals and depths as the oracle computes them
+  {
+    int checked = 0;
+    for (const auto& tf : tfs) {
+      for (std::size_t cap : {std::size_t(3), std::size_t(100000)}) {
+        CollisionRequest<double> req(cap, true);
+        CollisionResult<double> res;
+        const std::size_t n = collide(&m1, tf, &m2, I, req, res);
+        double p1[12], p2[12];
+        to_pose(tf, p1);
+        to_pose(I, p2);
+        const int store = 20000;
+        std::vector<int> pairs(2 * store);
+        std::vector<double> pos(3 * store), nrm(3 * store), dep(store);
+        fclo_collide_stats st;
+        fclo_collide_contact(&o1.m, p1, &o2.m, p2, static_cast<int>(cap), store, pairs.data(),
+                             pos.data(), nrm.data(), dep.data(), &st);
+        CHECK(n == static_cast<std::size_t>(st.num_contacts));
+        for (std::size_t i = 0; i < n && i < static_cast<std::size_t>(store); ++i) {
+          const Contact<double>& c = res.getContact(i);
+          CHECK(c.b1 == pairs[2 * i] && c.b2 == pairs[2 * i + 1]);
+          for (int k = 0; k < 3; ++k)
+            CHECK(c.pos[k] == pos[3 * i + k] && c.normal[k] == nrm[3 * i + k]);
+          CHECK(c.penetration_depth == dep[i]);
+          ++checked;
+        }
+      }
+    }
+    CHECK(checked > 0);
+    std::printf("enable_contact: %d contacts checked\n", checked);
+  }
+
   // error conventions of collide<S,Solver> / distance<Solver>
   {
     CollisionResult<double> res;

@@ -100,6 +100,56 @@ def test_exhaustive_and_capped_parity(ctx, cap):
     assert oc.max() == cap if cap < 10 else oc.max() > 10
 
 
+@pytest.mark.parametrize("engine", [F.ENGINE_PLAIN, F.ENGINE_FAST])
+@pytest.mark.parametrize("cap", [1, 5, 100000])
+def test_contact_mode_parity(ctx, cap, engine):
+    """enable_contact: per query the same contacts as the oracle, in order,
+    each with a bit-identical world-frame point, normal and depth; the cap
+    counts contacts (a triangle pair gives 0-2)."""
+    (V1, T1), (V2, T2) = F.icosphere(7, 0.15, 51), F.heightfield(30, 0.05, 52)
+    V2 = F.scale_to_radius(V2, 1.0)
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    n = 300
+    tf1, tf2 = F.poses(n, 0, 1.8, 0, 0xC7)
+    store = min(cap, 20000)
+    want = [fclo.collide_contact(o[0], tf1[q], o[1], tf2[q], cap=cap, store=store)
+            for q in range(n)]
+    total = sum(w[5] for w in want)
+    cap_rec = max(1, total + 16)
+    print("case: contact mode cap=%d engine=%d n=%d" % (cap, engine, n), flush=True)
+    d_qm = _t(np.tile(g, (n, 1)), torch.int32)
+    counts = torch.zeros(n, dtype=torch.int32, device=DEV)
+    pairs = torch.zeros((cap_rec, 3), dtype=torch.int32, device=DEV)
+    pos = torch.zeros((cap_rec, 3), dtype=torch.float64, device=DEV)
+    nrm = torch.zeros((cap_rec, 3), dtype=torch.float64, device=DEV)
+    dep = torch.zeros(cap_rec, dtype=torch.float64, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.collide_batch_async(d_qm, _t(tf1, torch.float64), _t(tf2, torch.float64), cap, counts,
+                            pairs, num, engine=engine, stream=s, contact=(pos, nrm, dep))
+    assert ctx.sync_status(s) == 0
+    nrec = int(num.item())
+    assert nrec == total
+    gc = counts.cpu().numpy()
+    rec, gpos, gnrm, gdep = (pairs[:nrec].cpu().numpy(), pos[:nrec].cpu().numpy(),
+                             nrm[:nrec].cpu().numpy(), dep[:nrec].cpu().numpy())
+    order = np.argsort(rec[:, 0], kind="stable")  # one lane per query: in order
+    rec, gpos, gnrm, gdep = rec[order], gpos[order], gnrm[order], gdep[order]
+    start = 0
+    for q in range(n):
+        wp, wpos, wn, wd, _, wcount = want[q]
+        assert gc[q] == wcount
+        k = wcount
+        assert (rec[start:start + k, 0] == q).all()
+        assert np.array_equal(rec[start:start + k, 1:], wp[:k])
+        assert np.array_equal(gpos[start:start + k], wpos[:k])
+        assert np.array_equal(gnrm[start:start + k], wn[:k])
+        assert np.array_equal(gdep[start:start + k], wd[:k])
+        start += k
+    assert total > 0 and (cap > 1 or (gc <= 1).all())
+
+
 def _set(p):
     return sorted(map(tuple, np.asarray(p).reshape(-1, 2).tolist()))
 

@@ -216,3 +216,103 @@ def test_eigen_old_diagonal_is_exact():
     d, V = fclo.eigen_old(np.diag([3.0, 1.0, 2.0]))
     assert list(d) == [3.0, 1.0, 2.0]
     assert np.array_equal(V, np.eye(3))
+
+
+# ------------------------------------------------ triangle contact output
+def _plane(T):
+    n = np.cross(T[1] - T[0], T[2] - T[0])
+    n = n / np.linalg.norm(n)
+    return n, n @ T[0]
+
+
+def test_triangle_contact_piercing_vertex_closed_form():
+    """A triangle Q dipping one vertex a depth d below triangle P's plane
+    (its other vertices far above, so P barely penetrates Q's plane): one
+    contact at that vertex, depth d, normal = P's unit normal.  The same
+    holds in any rigid frame (intersect-inl.h:802-842)."""
+    rng = np.random.default_rng(41)
+    P0 = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0.0]])
+    for _ in range(200):
+        d = rng.uniform(0.01, 0.4)
+        x, y = rng.uniform(0.2, 0.8, 2)
+        Q0 = np.array([[x, y, -d], [x + 0.1, y, 1.5], [x, y + 0.1, 1.5]])
+        Rm = util.random_rotation(rng)
+        t = rng.uniform(-3, 3, 3)
+        P, Q = P0 @ Rm.T + t, Q0 @ Rm.T + t
+        hit, pts, normal, depth = fclo.intersect_triangle_contact(P, Q)
+        assert hit and len(pts) == 1
+        assert abs(depth - d) < 1e-12
+        np.testing.assert_allclose(pts[0], Q[0], atol=1e-12)
+        np.testing.assert_allclose(normal, Rm @ np.array([0, 0, 1.0]), atol=1e-12)
+
+
+def test_triangle_contact_edge_gives_two_points():
+    """An edge of Q parallel to P's plane at depth d: two contacts (both
+    edge vertices) at that depth."""
+    P = np.array([[0, 0, 0], [3, 0, 0], [0, 3, 0.0]])
+    Q = np.array([[0.5, 0.5, -0.2], [1.0, 0.6, -0.2], [0.7, 0.5, 2.0]])
+    hit, pts, normal, depth = fclo.intersect_triangle_contact(P, Q)
+    assert hit and len(pts) == 2 and abs(depth - 0.2) < 1e-12
+    np.testing.assert_allclose(np.sort(pts[:, 0]), [0.5, 1.0])
+    np.testing.assert_allclose(normal, [0, 0, 1.0])
+
+
+def _intersecting_pairs(rng, n):
+    out = []
+    while len(out) < n:
+        P = rng.uniform(-1, 1, (3, 3))
+        Q = rng.uniform(-1, 1, (3, 3))
+        if fclo.intersect_triangle(P, Q):
+            out.append((P, Q))
+    return out
+
+
+def test_triangle_contact_points_are_deepest_vertices():
+    """On random intersecting pairs: every contact point is a vertex of one
+    triangle lying at the reported depth below the other's plane, no vertex
+    of that triangle lies deeper, and the normal is the other's unit normal
+    (sign: Q's points use +n_P, P's points use -n_Q).  Planes and depths are
+    recomputed here with numpy."""
+    rng = np.random.default_rng(43)
+    seen = 0
+    for P, Q in _intersecting_pairs(rng, 400):
+        hit, pts, normal, depth = fclo.intersect_triangle_contact(P, Q)
+        assert hit
+        if len(pts) == 0:
+            continue
+        seen += 1
+        nP, tP = _plane(P)
+        nQ, tQ = _plane(Q)
+        if np.allclose(normal, nP, atol=1e-9):
+            tri, n, t = Q, nP, tP
+        else:
+            np.testing.assert_allclose(normal, -nQ, atol=1e-9)
+            tri, n, t = P, nQ, tQ
+        d_all = -(tri @ n - t)
+        assert abs(d_all.max() - depth) < 1e-9
+        for p in pts:
+            assert min(np.abs(tri - p).sum(1)) < 1e-12
+            assert abs(-(p @ n - t) - depth) < 1e-6 + 1e-9
+        assert len(pts) <= 2
+    assert seen > 300
+
+
+def test_triangle_contact_swap_symmetry():
+    """Swapping the triangles keeps the points and depth and flips the
+    normal, whenever the two penetrations differ."""
+    rng = np.random.default_rng(47)
+    checked = 0
+    for P, Q in _intersecting_pairs(rng, 300):
+        _, pa, na, da = fclo.intersect_triangle_contact(P, Q)
+        _, pb, nb, db = fclo.intersect_triangle_contact(Q, P)
+        nP, tP = _plane(P)
+        nQ, tQ = _plane(Q)
+        pen_p = (-(P @ nQ - tQ)).max()
+        pen_q = (-(Q @ nP - tP)).max()
+        if abs(pen_p - pen_q) < 1e-9:
+            continue
+        checked += 1
+        assert da == db
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_allclose(na, -nb, atol=1e-15)
+    assert checked > 250

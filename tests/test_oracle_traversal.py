@@ -128,3 +128,42 @@ def test_distance_seed_and_overlap(spheres):
     m1, m2 = fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)
     d, _, _, _ = fclo.distance(m1, util.identity_pose(), m2, util.identity_pose())
     assert d == 0.0
+
+
+@pytest.mark.parametrize("cap", [1, 3, 8, 100000])
+def test_contact_mode_matches_pairwise_recomposition(spheres, cap):
+    """collide() with enable_contact (mesh_collision_traversal_node-inl.h:
+    578-598): rebuilt here from the exhaustive pair list in depth-first order
+    and the per-pair contact function, mapped to the world frame with numpy;
+    the traversal's output is the first `cap` contacts of that list (the cap
+    counts contacts, and a pair crossing it adds only what fits)."""
+    (V1, T1), (V2, T2) = spheres
+    m1, m2 = fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)
+    tf1, tf2 = F.poses(30, 0, 1.5, 0, 79)
+    R1s = tf1[:, :9].reshape(-1, 3, 3)
+    total = 0
+    for q in range(30):
+        pairs, _ = fclo.collide(m1, tf1[q], m2, tf2[q], cap=100000)
+        R1 = R1s[q]
+        R2 = tf2[q][:9].reshape(3, 3)
+        R = R1.T @ R2
+        T = R1.T @ (tf2[q][9:] - tf1[q][9:])
+        want = []
+        for a, b in pairs:
+            P = V1[T1[a]]
+            Q = V2[T2[b]]
+            hit, pts, nrm, dep = fclo.intersect_triangle_contact(P, Q, R, T)
+            assert hit
+            for p in pts:
+                want.append((a, b, R1 @ p + tf1[q][9:], R1 @ nrm, dep))
+        gp, gpos, gnrm, gdep, _, n = fclo.collide_contact(m1, tf1[q], m2, tf2[q], cap=cap,
+                                                          store=min(cap, 4096))
+        want = want[:cap]
+        assert n == len(want)
+        total += n
+        for i, (a, b, p, nv, d) in enumerate(want):
+            assert (gp[i] == (a, b)).all()
+            np.testing.assert_allclose(gpos[i], p, atol=1e-12)
+            np.testing.assert_allclose(gnrm[i], nv, atol=1e-12)
+            assert gdep[i] == d
+    assert total > 0
