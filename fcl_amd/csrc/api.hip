@@ -532,6 +532,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.defer_uncertain = static_cast<long long>(value);
   } else if (n == "idle_stop" && in(0, 1)) {
     t.idle_stop = value;
+  } else if (n == "stop_poll" && in(1, 1024)) {
+    t.stop_poll = static_cast<int>(value);
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter or value out of range");
   }

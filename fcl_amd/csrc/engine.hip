@@ -62,8 +62,7 @@ enum : int {
   kCtrRoundEnd = 66,  // one past the last counter a round resets
   kNumCounters = 80
 };
-// depth-first steps between polls of the stop flag
-constexpr int kStopPoll = 16;
+
 
 // FP64 relative pose of a query plus its meshes' triangle and vertex
 // arrays: everything a triangle test needs, in one 128-B line.
@@ -184,6 +183,7 @@ struct EngineArgs {
   unsigned long long* stop;   // nonzero: every wavefront dumps its state and exits
   unsigned long long* idle;   // wavefronts that ran out of work
   unsigned long long idle_limit;  // idle count that stops the round (0: never)
+  unsigned stop_poll;             // depth-first steps between polls of the stop flag
 };
 
 enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback };
@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
   unsigned step = 0;
 
   while (true) {
-    if ((++step % kStopPoll) == 0) {
+    if ((++step % A.stop_poll) == 0) {
       const int stop_seen =
           __hip_atomic_load(A.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1 : 0;
       if (__builtin_amdgcn_readfirstlane(stop_seen)) {  // one decision per wavefront
@@ -1465,6 +1465,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     D.defer_ctl[0] = C + kCtrDefer0;
     D.defer_ctl[1] = C + kCtrDefer1;
     D.stop = C + kCtrStop;
+    D.stop_poll = static_cast<unsigned>(tune.stop_poll);
     D.idle = C + kCtrIdle;
     // rebalance only when the front gives every wavefront a full step
     D.idle_limit = static_cast<long long>(nfront) >= static_cast<long long>(kWave * nwaves)

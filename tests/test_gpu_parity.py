@@ -171,7 +171,8 @@ def test_fast_engine_exhaustive(ctx, cap, stats):
 STRESS = {
     "full": {"defer_leaf": 256, "defer_uncertain": 64, "front_target": 64, "max_levels": 2,
              "dfs_blocks_per_cu": 1},
-    "idle": {"front_target": 100000, "idle_stop": 0.0005, "dfs_blocks_per_cu": 1},
+    "idle": {"front_target": 100000, "idle_stop": 0.0005, "dfs_blocks_per_cu": 1,
+             "stop_poll": 1},
 }
 
 
