@@ -301,9 +301,21 @@ __device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q
   } else {
     atomicOr(L.status, kStatusPairOverflow);
   }
-  const int old = atomicAdd(&L.counts[q], 1);
-  if (old + 1 >= L.cap) {
-    if (atomicExch(&A.qflag[q], 1) == 0) atomicAdd(A.nflag, 1ULL);
+  // counts: one atomic per distinct query among the wavefront's hits (the
+  // candidates of one flush nearly always share their query)
+  unsigned long long todo = __ballot(true);  // lanes still to count (all hit here)
+  while (todo != 0) {
+    const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+    const int ql = __shfl(q, leader);
+    const unsigned long long same = __ballot(q == ql);
+    if (lane_id() == leader) {
+      const int k = __popcll(same);
+      const int old = atomicAdd(&L.counts[ql], k);
+      if (old + k >= L.cap) {
+        if (atomicExch(&A.qflag[ql], 1) == 0) atomicAdd(A.nflag, 1ULL);
+      }
+    }
+    todo &= ~same;
   }
 }
 
