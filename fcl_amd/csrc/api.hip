@@ -29,6 +29,7 @@ struct MeshSlot {
   ObbNode32* obb32 = nullptr;
   double* verts = nullptr;
   int32_t* tris = nullptr;
+  double* tri_verts = nullptr;
   int32_t* span = nullptr;
   int32_t* prim_pos = nullptr;
   int64_t bytes = 0;
@@ -95,6 +96,7 @@ void free_slot(MeshSlot& s) {
   (void)hipFree(s.size64);
   (void)hipFree(s.obb32);
   (void)hipFree(s.verts);
+  (void)hipFree(s.tri_verts);
   (void)hipFree(s.tris);
   (void)hipFree(s.span);
   (void)hipFree(s.prim_pos);
@@ -258,12 +260,20 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
     span[2 * i + 1] = nodes[i].num_primitives;
   }
   for (int i = 0; i < num_tris; ++i) pos[s.bvh.primitive_indices[i]] = i;
+  // triangle-major copy of the vertices: one 128-B line per triangle
+  std::vector<double> tv(static_cast<size_t>(kTriVertsStride) * num_tris, 0.0);
+  for (int i = 0; i < num_tris; ++i)
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 3; ++c)
+        tv[kTriVertsStride * static_cast<size_t>(i) + 3 * k + c] =
+            vertices[3 * static_cast<size_t>(tris[3 * i + k]) + c];
   std::lock_guard<std::mutex> lock(ctx->mu);
   (void)hipSetDevice(ctx->device);
   bool ok = dmalloc(&s.obb64, nb) == hipSuccess && dmalloc(&s.rss64, nb) == hipSuccess &&
             dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32, nb) == hipSuccess &&
             dmalloc(&s.verts, 3 * static_cast<size_t>(num_vertices)) == hipSuccess &&
             dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess &&
+            dmalloc(&s.tri_verts, tv.size()) == hipSuccess &&
             dmalloc(&s.span, 2 * static_cast<size_t>(nb)) == hipSuccess &&
             dmalloc(&s.prim_pos, static_cast<size_t>(num_tris)) == hipSuccess;
   if (ok)
@@ -282,6 +292,8 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
          hipMemcpy(s.verts, vertices, sizeof(double) * 3 * num_vertices,
                    hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(s.tris, tris, sizeof(int32_t) * 3 * num_tris, hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(s.tri_verts, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice) ==
              hipSuccess;
   if (!ok) {
     free_slot(s);
@@ -289,7 +301,8 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   }
   s.bytes = static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
                                         sizeof(ObbNode32)) +
-            24LL * num_vertices + 16LL * num_tris + 8LL * nb;
+            24LL * num_vertices + 16LL * num_tris + 8LL * nb +
+            8LL * kTriVertsStride * num_tris;
   s.live = true;
   MeshDev& d = s.dev;
   d.span = s.span;
@@ -300,6 +313,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   d.obb32 = s.obb32;
   d.verts = s.verts;
   d.tris = s.tris;
+  d.tri_verts = s.tri_verts;
   d.num_bvs = nb;
   d.num_tris = num_tris;
   d.num_verts = num_vertices;

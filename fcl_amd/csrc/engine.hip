@@ -64,15 +64,13 @@ enum : int {
 };
 
 
-// FP64 relative pose of a query plus its meshes' triangle and vertex
+// FP64 relative pose of a query plus its meshes' triangle-major vertex
 // arrays: everything a triangle test needs, in one 128-B line.
 struct alignas(128) QueryXf64 {
   double R[9];
   double T[3];
-  const double* v1;
-  const double* v2;
-  const int32_t* tri1;
-  const int32_t* tri2;
+  const double* tv1;
+  const double* tv2;
 };
 
 // task = (query:22 | b1:21 | b2:21)
@@ -212,10 +210,8 @@ __global__ void __launch_bounds__(kBlock) engine_setup(EngineArgs A) {
       QueryXf64 x;
       rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q), x.R,
                  x.T);
-      x.v1 = L.meshes[i1].verts;
-      x.v2 = L.meshes[i2].verts;
-      x.tri1 = L.meshes[i1].tris;
-      x.tri2 = L.meshes[i2].tris;
+      x.tv1 = L.meshes[i1].tri_verts;
+      x.tv2 = L.meshes[i2].tri_verts;
       A.qx64[q] = x;
       make_xf32(x.R, x.T, L.meshes[i1].cmax, L.meshes[i2].cmax, X);
       X.m1 = i1;
@@ -324,15 +320,10 @@ __device__ __forceinline__ bool leaf_hit(const EngineArgs& A, unsigned long long
                                          int& id1, int& id2) {
   unpack_task(task, q, id1, id2);
   const QueryXf64& x = A.qx64[q];
-  const int32_t* t1 = x.tri1 + 3 * static_cast<size_t>(id1);
-  const int32_t* t2 = x.tri2 + 3 * static_cast<size_t>(id2);
-  const int a0 = t1[0], a1 = t1[1], a2 = t1[2], c0 = t2[0], c1 = t2[1], c2 = t2[2];
-  auto vert = [](const double* v, int i) {
-    const double* p = v + 3 * static_cast<size_t>(i);
-    return v3(p[0], p[1], p[2]);
-  };
-  const V3 P1 = vert(x.v1, a0), P2 = vert(x.v1, a1), P3 = vert(x.v1, a2);
-  const V3 Q1 = vert(x.v2, c0), Q2 = vert(x.v2, c1), Q3 = vert(x.v2, c2);
+  const double* p = x.tv1 + kTriVertsStride * static_cast<size_t>(id1);
+  const double* r = x.tv2 + kTriVertsStride * static_cast<size_t>(id2);
+  const V3 P1 = v3(p[0], p[1], p[2]), P2 = v3(p[3], p[4], p[5]), P3 = v3(p[6], p[7], p[8]);
+  const V3 Q1 = v3(r[0], r[1], r[2]), Q2 = v3(r[3], r[4], r[5]), Q3 = v3(r[6], r[7], r[8]);
   M3 R;
 #pragma unroll
   for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];

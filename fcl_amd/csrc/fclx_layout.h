@@ -11,6 +11,8 @@
 //                    the host so that no device sqrt enters a descent decision.
 //   obb32[n]   64 B: the OBB rounded to float, for conservative culling.
 //   verts      24 B each (x,y,z doubles); tris 12 B each (3 x int32).
+//   tri_verts 128 B per triangle: its three vertices (9 doubles) followed by
+//              padding, so a triangle test reads one line per triangle.
 // Node records are 128-B aligned so that one lane's node fetch is one L2
 // line; a wavefront of lanes on different nodes then issues 64 whole-line
 // requests instead of 16x as many partial ones.
@@ -48,6 +50,8 @@ static_assert(sizeof(ObbNode32) == 64, "ObbNode32 must be 64 B");
 
 // Per-mesh device descriptor (pointers into HBM).  Kernels receive an array
 // of these and index it with the query's mesh ids.
+constexpr int kTriVertsStride = 16;  // doubles per triangle in tri_verts
+
 struct MeshDev {
   const ObbNode64* obb64;
   const RssNode64* rss64;
@@ -64,6 +68,7 @@ struct MeshDev {
   double cmax;          // max over nodes of |OBB centre|_1 (bounds the FP32 cull error)
   const int32_t* span;      // 2 per node: first_primitive, num_primitives
   const int32_t* prim_pos;  // per triangle: its position in the build's primitive order
+  const double* tri_verts;  // kTriVertsStride per triangle: P1, P2, P3
 };
 
 }  // namespace fclx
