@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine-param", action="append", default=[], metavar="NAME=VALUE",
+                    help="FAST-engine tunable (fclx_set_engine_param); repeatable")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three device copies of MIB MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -103,6 +105,9 @@ def main():
     log("workload: %d meshes, %d queries, generated in %.1fs"
         % (len(W["meshes"]), args.queries, time.time() - t0))
     ctx = F.Context(0)
+    for kv in args.engine_param:
+        k, v = kv.split("=", 1)
+        ctx.set_engine_param(k, float(v))
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
         ids = list(ex.map(lambda m: ctx.mesh(m[0], m[1]), W["meshes"]))

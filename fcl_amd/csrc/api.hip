@@ -548,6 +548,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.idle_stop = value;
   } else if (n == "stop_poll" && in(1, 1024)) {
     t.stop_poll = static_cast<int>(value);
+  } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
+    t.dfs_waves_per_simd = static_cast<int>(value);
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter or value out of range");
   }

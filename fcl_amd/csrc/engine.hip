@@ -41,7 +41,16 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kLeafQ = 128;
+// Per-wavefront LDS queues of the depth-first kernel, flushed half at a
+// time to the deferred buffers.
+constexpr int kLeafQ = 256;       // leaf pairs
+constexpr int kUncQ = 128;        // uncertain tasks
+constexpr int kQueueCap[2] = {kLeafQ, kUncQ};
+// Each deferred buffer is split into kSub sub-buffers, one per XCD under the
+// hardware's round-robin block placement (blockIdx % kSub), each with its
+// reservation counter on a cache line of its own.
+constexpr int kSub = 8;
+constexpr unsigned long long kNoTask = ~0ULL;  // padding the consumers skip
 
 // Device counters of the exhaustive path (ws->counters[...]).
 enum : int {
@@ -53,14 +62,16 @@ enum : int {
   // 5..7: cap fix-up
   kCtrWork = 8,       // 8 .. 8+kWork*: work counters
   // the counters the depth-first kernel touches often sit on 128-B lines
-  // of their own, so atomics on one do not queue behind another
-  kCtrDefer0 = 16,    // leaf pairs: reservation, written prefix end, dumps
-  kCtrDefer1 = 32,    // uncertain tasks: the same three
-  kCtrStop = 48,      // polled by every wavefront
-  kCtrIdle = 64,
-  kCtrNext = 65,      // next round's front
-  kCtrRoundEnd = 66,  // one past the last counter a round resets
-  kNumCounters = 80
+  // of their own (16 counters apart), so atomics on one do not queue
+  // behind another
+  kCtrRoundBegin = 128,
+  kCtrResv = 128,     // reservation of sub-buffer x of buffer w: kCtrResv + 16 (w kSub + x)
+  kCtrDump = kCtrResv + 16 * 2 * kSub,  // dump counter of buffer w: kCtrDump + 16 w
+  kCtrStop = kCtrDump + 32,             // polled by every wavefront
+  kCtrIdle = kCtrStop + 16,
+  kCtrNext = kCtrIdle + 1,              // next round's front
+  kCtrRoundEnd = kCtrNext + 1,          // one past the last counter a round resets
+  kNumCounters = kCtrRoundEnd + 14
 };
 
 
@@ -150,6 +161,20 @@ struct QueryPtr {
   const double* s2;
 };
 
+// Up to kSub + 1 ranges of tasks read as one sequence (the sub-buffers of a
+// deferred buffer and its dump region); consumers skip kNoTask entries.
+struct TaskRanges {
+  const unsigned long long* p[kSub + 1];
+  long long start[kSub + 2];  // start[r] = entries before range r; start[count] = total
+  int count;
+};
+
+__device__ __forceinline__ unsigned long long range_task(const TaskRanges& R, long long i) {
+  int r = 0;
+  while (r + 1 < R.count && i >= R.start[r + 1]) ++r;
+  return R.p[r][i - R.start[r]];
+}
+
 struct EngineArgs {
   CollideLaunch L;
   QueryXf32* qx32;
@@ -168,16 +193,18 @@ struct EngineArgs {
   int stack_cap;
   unsigned long long* work;  // kWork* counters
   unsigned long long* leaf_work;  // work counter the leaf kernel adds to
+  TaskRanges in;                  // input of the leaf and uncertain kernels
   unsigned long long* wave_prof;  // profiling: (start, end, bv tests) per DFS wavefront, or null
   // depth-first rounds: tasks that need FP64 work are deferred to two
-  // buffers (0: overlapping leaf pairs, 1: tasks the FP32 cull left
-  // uncertain).  Flushes fill [0, defer_cap); a flush that does not fit
-  // fails and stops the round.  Per-wavefront leftovers are dumped after
-  // defer_cap.  defer_ctl[w]: reservation counter, end of the written
-  // prefix, dump counter.
+  // buffers (0: overlapping leaf pairs as (q, tri1, tri2), 1: tasks the
+  // FP32 cull left uncertain).  Buffer w holds kSub sub-buffers of
+  // defer_sub[w] entries; a flush reserves room in its block's sub-buffer
+  // and, when that does not fit, pads the part inside it with kNoTask,
+  // fails, and stops the round.  Per-wavefront leftovers go to the dump
+  // region after the sub-buffers.
   unsigned long long* defer[2];
-  long long defer_cap[2];
-  unsigned long long* defer_ctl[2];
+  long long defer_sub[2];
+  unsigned long long* ctr;  // the engine's counter array (kCtr* indices)
   unsigned long long* stop;   // nonzero: every wavefront dumps its state and exits
   unsigned long long* idle;   // wavefronts that ran out of work
   unsigned long long idle_limit;  // idle count that stops the round (0: never)
@@ -283,23 +310,13 @@ __device__ __forceinline__ Visit visit_task(const EngineArgs& A, unsigned long l
   return v;
 }
 
-// Record one intersecting pair; flags the query once its count reaches the cap.
-__device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q, int id1,
-                                            int id2) {
+// Per-query contact counts of the lanes that hit: one atomic per distinct
+// query among them (the candidates of one flush nearly always share their
+// query); flags a query once its count reaches the cap.  Call with only the
+// hitting lanes active.
+__device__ __forceinline__ void count_hits(const EngineArgs& A, int q) {
   const CollideLaunch& L = A.L;
-  const unsigned long long slot = wave_reserve(reinterpret_cast<unsigned long long*>(L.num_pairs),
-                                               hit, 1);
-  if (!hit) return;
-  if (static_cast<long long>(slot) < L.pair_capacity) {
-    L.pairs[3 * slot] = q;
-    L.pairs[3 * slot + 1] = id1;
-    L.pairs[3 * slot + 2] = id2;
-  } else {
-    atomicOr(L.status, kStatusPairOverflow);
-  }
-  // counts: one atomic per distinct query among the wavefront's hits (the
-  // candidates of one flush nearly always share their query)
-  unsigned long long todo = __ballot(true);  // lanes still to count (all hit here)
+  unsigned long long todo = __ballot(true);
   while (todo != 0) {
     const int leader = __ffsll(static_cast<long long>(todo)) - 1;
     const int ql = __shfl(q, leader);
@@ -313,6 +330,24 @@ __device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q
     }
     todo &= ~same;
   }
+}
+
+// Record intersecting pairs (one output reservation per wavefront) and
+// count them; for kernels that see few hits.
+__device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q, int id1,
+                                            int id2) {
+  const CollideLaunch& L = A.L;
+  const unsigned long long slot = wave_reserve(reinterpret_cast<unsigned long long*>(L.num_pairs),
+                                               hit, 1);
+  if (!hit) return;
+  if (static_cast<long long>(slot) < L.pair_capacity) {
+    L.pairs[3 * slot] = q;
+    L.pairs[3 * slot + 1] = id1;
+    L.pairs[3 * slot + 2] = id2;
+  } else {
+    atomicOr(L.status, kStatusPairOverflow);
+  }
+  count_hits(A, q);
 }
 
 // Triangle test of a leaf candidate (q, tri1, tri2).
@@ -352,13 +387,70 @@ __global__ void __launch_bounds__(kBlock) engine_expand(EngineArgs A) {
   if (v.leafc) A.leafc[sl] = v.c0;
 }
 
+// Triangle tests of leaf candidates leafc[0, n_in).  Persistent blocks walk
+// 256-candidate tiles; hits are staged in LDS and written out in runs of
+// up to kLeafStage records with one reservation on the global record
+// counter, which every block shares (same-address atomics serialise in L2,
+// so one per wavefront would throttle the whole kernel).
+constexpr int kLeafStage = 2048;
 __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool hit = false;
-  int q = 0, id1 = 0, id2 = 0;
-  if (i < A.n_in) hit = leaf_hit(A, A.leafc[i], q, id1, id2);
-  wave_count(A.leaf_work, i < A.n_in);
-  record_hits(A, hit, q, id1, id2);
+  __shared__ int32_t stage[3 * kLeafStage];
+  __shared__ unsigned n_stage;
+  __shared__ unsigned long long base;
+  const CollideLaunch& L = A.L;
+  if (threadIdx.x == 0) n_stage = 0;
+  __syncthreads();
+  auto flush = [&]() {  // block-uniform call
+    __syncthreads();
+    const unsigned n = n_stage;
+    if (threadIdx.x == 0 && n > 0)
+      base = atomicAdd(reinterpret_cast<unsigned long long*>(L.num_pairs),
+                       static_cast<unsigned long long>(n));
+    __syncthreads();
+    bool over = false;
+    for (unsigned r = threadIdx.x; r < n; r += kBlock) {
+      const long long o = static_cast<long long>(base + r);
+      if (o < L.pair_capacity) {
+        L.pairs[3 * o] = stage[3 * r];
+        L.pairs[3 * o + 1] = stage[3 * r + 1];
+        L.pairs[3 * o + 2] = stage[3 * r + 2];
+      } else {
+        over = true;
+      }
+    }
+    if (over) atomicOr(L.status, kStatusPairOverflow);
+    __syncthreads();
+    if (threadIdx.x == 0) n_stage = 0;
+    __syncthreads();
+  };
+  const long long total = A.in.start[A.in.count];
+  const long long tiles = (total + kBlock - 1) / kBlock;
+  unsigned long long tests = 0;  // this wavefront's triangle tests
+  for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const long long i = t * kBlock + threadIdx.x;
+    bool hit = false;
+    int q = 0, id1 = 0, id2 = 0;
+    const unsigned long long task = i < total ? range_task(A.in, i) : kNoTask;
+    if (task != kNoTask) hit = leaf_hit(A, task, q, id1, id2);
+    tests += __popcll(__ballot(task != kNoTask));
+    const unsigned long long m = __ballot(hit);
+    unsigned off = 0;
+    if (lane_id() == 0 && m != 0) off = atomicAdd(&n_stage, static_cast<unsigned>(__popcll(m)));
+    off = __shfl(off, 0);
+    if (hit) {
+      const unsigned r = off + static_cast<unsigned>(__popcll(m & lanes_below()));
+      stage[3 * r] = q;
+      stage[3 * r + 1] = id1;
+      stage[3 * r + 2] = id2;
+      count_hits(A, q);
+    }
+    __syncthreads();
+    const bool full = n_stage > kLeafStage - kBlock;
+    __syncthreads();  // every thread has read n_stage before the next tile adds to it
+    if (full) flush();
+  }
+  flush();
+  if (lane_id() == 0 && tests) atomicAdd(A.leaf_work, tests);
 }
 
 // Deferred uncertain tasks: the FP64 test decides (the FP32 pass already
@@ -366,13 +458,9 @@ __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
 // children of overlapping internal pairs go to the next round's front.
 __global__ void __launch_bounds__(kBlock) engine_uncertain(EngineArgs A) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool active = i < A.n_in;
-  unsigned long long task = 0;
+  unsigned long long task = i < A.in.start[A.in.count] ? range_task(A.in, i) : kNoTask;
   Visit v;
-  if (active) {
-    task = A.leafc[i];
-    v = visit_task<false>(A, task);
-  }
+  if (task != kNoTask) v = visit_task<false>(A, task);
   const unsigned long long so = wave_reserve(A.n_out, v.push, 2);
   if (v.push) {
     A.front_out[so] = v.c0;
@@ -418,30 +506,38 @@ __device__ __forceinline__ int step32(const EngineArgs& A, unsigned long long ta
   return kStepPush;
 }
 
-// Move the top k entries of a wavefront's LDS queue to deferred buffer w.
-// Returns false (and moves nothing) when the buffer has no room for them.
-__device__ __forceinline__ bool flush_deferred(const EngineArgs& A, int w,
+// Move the top k entries of a wavefront's LDS queue to sub-buffer x of
+// deferred buffer w.  Returns false (and moves nothing) when the
+// sub-buffer has no room for them.
+__device__ __forceinline__ bool flush_deferred(const EngineArgs& A, int w, int x,
                                                const unsigned long long* q, int& n, int k) {
   unsigned long long s = 0;
-  if (lane_id() == 0) s = atomicAdd(A.defer_ctl[w], static_cast<unsigned long long>(k));
+  if (lane_id() == 0)
+    s = atomicAdd(A.ctr + kCtrResv + 16 * (w * kSub + x), static_cast<unsigned long long>(k));
   s = __shfl(s, 0);
-  if (static_cast<long long>(s + k) > A.defer_cap[w]) return false;
-  if (lane_id() < k) A.defer[w][s + lane_id()] = q[n - k + lane_id()];
-  if (lane_id() == 0) atomicMax(A.defer_ctl[w] + 1, s + k);
+  const long long cap = A.defer_sub[w];
+  unsigned long long* dst = A.defer[w] + static_cast<size_t>(x) * cap;
+  if (static_cast<long long>(s + k) > cap) {
+    for (long long i = static_cast<long long>(s) + lane_id(); i < cap; i += kWave)
+      dst[i] = kNoTask;
+    return false;
+  }
+  for (int i = lane_id(); i < k; i += kWave) dst[s + i] = q[n - k + i];
   __builtin_amdgcn_wave_barrier();
   n -= k;
   return true;
 }
 
-// A stopped or finishing wavefront's queue leftovers, after defer_cap
-// (room for kLeafQ per wavefront is reserved there).
+// A stopped or finishing wavefront's queue leftovers, into the dump region
+// after the sub-buffers (room for a full queue per wavefront).
 __device__ __forceinline__ void dump_deferred(const EngineArgs& A, int w,
                                               const unsigned long long* q, int n) {
   if (n == 0) return;
   unsigned long long s = 0;
-  if (lane_id() == 0) s = atomicAdd(A.defer_ctl[w] + 2, static_cast<unsigned long long>(n));
+  if (lane_id() == 0) s = atomicAdd(A.ctr + kCtrDump + 16 * w, static_cast<unsigned long long>(n));
   s = __shfl(s, 0);
-  for (int i = lane_id(); i < n; i += kWave) A.defer[w][A.defer_cap[w] + s + i] = q[i];
+  unsigned long long* dst = A.defer[w] + static_cast<size_t>(kSub) * A.defer_sub[w];
+  for (int i = lane_id(); i < n; i += kWave) dst[s + i] = q[i];
 }
 
 // One depth-first round.  Every wavefront owns a LIFO stack in device
@@ -453,12 +549,15 @@ __device__ __forceinline__ void dump_deferred(const EngineArgs& A, int w,
 // full or when idle_limit wavefronts have run out of work; then every
 // wavefront appends its stack to front_out and its queue leftovers to the
 // dump regions, and the host runs the FP64 kernels and the next round.
-__global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
-  __shared__ unsigned long long q_all[2][kWavesPerBlock][kLeafQ];
+template <int kWavesPerSimd>
+__global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A) {
+  __shared__ unsigned long long lq_all[kWavesPerBlock][kLeafQ];
+  __shared__ unsigned long long uq_all[kWavesPerBlock][kUncQ];
   const int wave = threadIdx.x / kWave;
   const int lane = lane_id();
-  unsigned long long* lq = q_all[0][wave];
-  unsigned long long* uq = q_all[1][wave];
+  const int xcd = static_cast<int>(blockIdx.x % kSub);
+  unsigned long long* lq = lq_all[wave];
+  unsigned long long* uq = uq_all[wave];
   unsigned long long* stack =
       A.stacks + (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave) * A.stack_cap;
   int sp = 0;                 // wavefront-uniform
@@ -479,8 +578,8 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
       }
     }
     // room for one full step in both queues
-    const bool full = (nl >= kWave && !flush_deferred(A, 0, lq, nl, kWave)) ||
-                      (nu >= kWave && !flush_deferred(A, 1, uq, nu, kWave));
+    const bool full = (nl >= kLeafQ / 2 && !flush_deferred(A, 0, xcd, lq, nl, kLeafQ / 2)) ||
+                      (nu >= kUncQ / 2 && !flush_deferred(A, 1, xcd, uq, nu, kUncQ / 2));
     if (full) {
       if (lane == 0) __hip_atomic_store(A.stop, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       stopped = true;
@@ -541,8 +640,8 @@ __global__ void __launch_bounds__(kBlock) engine_dfs(EngineArgs A) {
     dump_deferred(A, 0, lq, nl);
     dump_deferred(A, 1, uq, nu);
   } else {
-    if (nl > 0 && !flush_deferred(A, 0, lq, nl, nl)) dump_deferred(A, 0, lq, nl);
-    if (nu > 0 && !flush_deferred(A, 1, uq, nu, nu)) dump_deferred(A, 1, uq, nu);
+    if (nl > 0 && !flush_deferred(A, 0, xcd, lq, nl, nl)) dump_deferred(A, 0, lq, nl);
+    if (nu > 0 && !flush_deferred(A, 1, xcd, uq, nu, nu)) dump_deferred(A, 1, uq, nu);
     if (lane == 0) {
       const unsigned long long idle = atomicAdd(A.idle, 1ULL) + 1;
       if (A.idle_limit != 0 && idle == A.idle_limit)
@@ -1090,6 +1189,19 @@ inline unsigned grid_for(long long n) {
   return static_cast<unsigned>((n + kBlock - 1) / kBlock);
 }
 
+// Persistent grid of the leaf kernel: every block resident, none idle.
+unsigned leaf_grid(long long n) {
+  static const unsigned resident = [] {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, engine_leaves, kBlock, 0) !=
+            hipSuccess ||
+        occ < 1)
+      occ = 1;
+    return static_cast<unsigned>(device_cus()) * static_cast<unsigned>(occ);
+  }();
+  return std::max(1u, std::min(grid_for(n), resident));
+}
+
 // Re-run the flagged queries with the plain depth-first kernel.
 hipError_t fixup_flagged(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                          unsigned long long nflag, hipStream_t s) {
@@ -1416,10 +1528,12 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     if (nleafc > 0) {
       ENG_TRY(clock_begin(ws, s, t));
       EngineArgs X = A;
-      X.leafc = ws->leafc;
-      X.n_in = static_cast<long long>(nleafc);
+      X.in.count = 1;
+      X.in.p[0] = ws->leafc;
+      X.in.start[0] = 0;
+      X.in.start[1] = static_cast<long long>(nleafc);
       X.leaf_work = C + kCtrWork + kWorkLeafKernel;
-      engine_leaves<<<grid_for(nleafc), kBlock, 0, s>>>(X);
+      engine_leaves<<<leaf_grid(nleafc), kBlock, 0, s>>>(X);
       ENG_TRY(hipGetLastError());
       clock_end(ws, s, t, kPhaseLeaves);
       set_u64<<<1, 1, 0, s>>>(C + kCtrLeafc, 0);
@@ -1432,15 +1546,22 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   rep.front_size = static_cast<long long>(nfront);
 
   // 3. depth-first rounds over the remaining front
+  // register budget of the depth-first kernel: 4, 5 or 6 wavefronts per SIMD
+  // (more waves hide more latency; fewer registers spill more)
+  auto dfs_kernel = tune.dfs_waves_per_simd >= 6   ? engine_dfs<6>
+                    : tune.dfs_waves_per_simd == 5 ? engine_dfs<5>
+                                                   : engine_dfs<4>;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, engine_dfs, kBlock, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dfs_kernel, kBlock, 0) != hipSuccess ||
       occ < 1)
     occ = 1;
   const unsigned blocks =
       static_cast<unsigned>(cus) * static_cast<unsigned>(std::min(occ, tune.dfs_blocks_per_cu));
   const size_t nwaves = static_cast<size_t>(blocks) * kWavesPerBlock;
   const int stack_cap = 2 * kWave * (stack_need + 2) + 2 * kWave;
-  const long long dcap[2] = {tune.defer_leaf, tune.defer_uncertain};
+  // entries per sub-buffer of each deferred buffer
+  const long long dsub[2] = {std::max<long long>(kLeafQ, tune.defer_leaf / kSub),
+                             std::max<long long>(kUncQ, tune.defer_uncertain / kSub)};
   while (nfront > 0) {
     if (rep.dfs_rounds >= 1000000) return hipErrorUnknown;  // no progress: never expected
     const int nxt = cur ^ 1;
@@ -1448,9 +1569,9 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     // next front: stack dumps, unconsumed input, children of uncertain tasks
     ENG_TRY(grow(ws->front[nxt], ws->front_cap[nxt], nwaves * stack_cap + nfront));
     for (int w = 0; w < 2; ++w)
-      ENG_TRY(grow(ws->defer[w], ws->defer_cap[w], dcap[w] + nwaves * kLeafQ));
+      ENG_TRY(grow(ws->defer[w], ws->defer_cap[w], kSub * dsub[w] + nwaves * kQueueCap[w]));
     if (ws->profile) ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * nwaves));
-    zero_u64<<<1, kNumCounters, 0, s>>>(C + kCtrDefer0, kCtrRoundEnd - kCtrDefer0);
+    zero_u64<<<1, kNumCounters, 0, s>>>(C + kCtrRoundBegin, kCtrRoundEnd - kCtrRoundBegin);
     set_u64<<<1, 1, 0, s>>>(C + kCtrHead, 0);
     EngineArgs D = A;
     D.front_in = ws->front[cur];
@@ -1463,10 +1584,9 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     D.wave_prof = ws->profile ? ws->wave_prof : nullptr;
     for (int w = 0; w < 2; ++w) {
       D.defer[w] = ws->defer[w];
-      D.defer_cap[w] = dcap[w];
+      D.defer_sub[w] = dsub[w];
     }
-    D.defer_ctl[0] = C + kCtrDefer0;
-    D.defer_ctl[1] = C + kCtrDefer1;
+    D.ctr = C;
     D.stop = C + kCtrStop;
     D.stop_poll = static_cast<unsigned>(tune.stop_poll);
     D.idle = C + kCtrIdle;
@@ -1476,21 +1596,38 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
                              1, static_cast<unsigned long long>(tune.idle_stop * nwaves))
                        : 0;
     ENG_TRY(clock_begin(ws, s, t));
-    engine_dfs<<<blocks, kBlock, 0, s>>>(D);
+    dfs_kernel<<<blocks, kBlock, 0, s>>>(D);
     ENG_TRY(hipGetLastError());
     clock_end(ws, s, t, kPhaseDfs);
     ++rep.dfs_rounds;
     ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrHead, C + kCtrHead, sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
-    ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrDefer0, C + kCtrDefer0,
-                           (kCtrRoundEnd - kCtrDefer0) * sizeof(unsigned long long),
+    ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrRoundBegin, C + kCtrRoundBegin,
+                           (kCtrRoundEnd - kCtrRoundBegin) * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
     ENG_TRY(hipStreamSynchronize(s));
     const unsigned long long* H = ws->host_counters;
     const unsigned long long head = H[kCtrHead];
-    const unsigned long long leaf_n = H[kCtrDefer0 + 1], leaf_d = H[kCtrDefer0 + 2];
-    const unsigned long long unc_n = H[kCtrDefer1 + 1], unc_d = H[kCtrDefer1 + 2];
     unsigned long long nnext = H[kCtrNext];
+    // what each deferred buffer holds: the filled part of every sub-buffer
+    // (padding included) and the dumps
+    TaskRanges in[2];
+    for (int w = 0; w < 2; ++w) {
+      in[w].count = 0;
+      in[w].start[0] = 0;
+      auto add = [&](const unsigned long long* p, long long n) {
+        if (n <= 0) return;
+        in[w].p[in[w].count] = p;
+        in[w].start[in[w].count + 1] = in[w].start[in[w].count] + n;
+        ++in[w].count;
+      };
+      for (int x = 0; x < kSub; ++x)
+        add(ws->defer[w] + x * dsub[w],
+            static_cast<long long>(
+                std::min<unsigned long long>(H[kCtrResv + 16 * (w * kSub + x)], dsub[w])));
+      add(ws->defer[w] + kSub * dsub[w], static_cast<long long>(H[kCtrDump + 16 * w]));
+    }
+    const long long n_leaf = in[0].start[in[0].count], n_unc = in[1].start[in[1].count];
     if (ws->profile) ENG_TRY(summarize_waves(ws, nwaves, s));
     // unconsumed input goes to the next front
     const unsigned long long rem = nfront - std::min(head, nfront);
@@ -1501,38 +1638,26 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     }
     // FP64 work deferred by the round
     ENG_TRY(clock_begin(ws, s, t));
-    if (unc_n + unc_d > 0) {
-      ENG_TRY(grow_keep(ws->front[nxt], ws->front_cap[nxt], nnext + 2 * (unc_n + unc_d), nnext, s));
+    if (n_unc > 0) {
+      ENG_TRY(grow_keep(ws->front[nxt], ws->front_cap[nxt], nnext + 2 * n_unc, nnext, s));
       set_u64<<<1, 1, 0, s>>>(C + kCtrNext, nnext);
-      const unsigned long long* src[2] = {ws->defer[1], ws->defer[1] + dcap[1]};
-      const unsigned long long cnt[2] = {unc_n, unc_d};
-      for (int k = 0; k < 2; ++k) {
-        if (cnt[k] == 0) continue;
-        EngineArgs X = A;
-        X.leafc = const_cast<unsigned long long*>(src[k]);
-        X.n_in = static_cast<long long>(cnt[k]);
-        X.front_out = ws->front[nxt];
-        X.n_out = C + kCtrNext;
-        X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
-        engine_uncertain<<<grid_for(cnt[k]), kBlock, 0, s>>>(X);
-        ENG_TRY(hipGetLastError());
-      }
+      EngineArgs X = A;
+      X.in = in[1];
+      X.front_out = ws->front[nxt];
+      X.n_out = C + kCtrNext;
+      X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
+      engine_uncertain<<<grid_for(n_unc), kBlock, 0, s>>>(X);
+      ENG_TRY(hipGetLastError());
     }
-    {
-      const unsigned long long* src[2] = {ws->defer[0], ws->defer[0] + dcap[0]};
-      const unsigned long long cnt[2] = {leaf_n, leaf_d};
-      for (int k = 0; k < 2; ++k) {
-        if (cnt[k] == 0) continue;
-        EngineArgs X = A;
-        X.leafc = const_cast<unsigned long long*>(src[k]);
-        X.n_in = static_cast<long long>(cnt[k]);
-        X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
-        engine_leaves<<<grid_for(cnt[k]), kBlock, 0, s>>>(X);
-        ENG_TRY(hipGetLastError());
-      }
+    if (n_leaf > 0) {
+      EngineArgs X = A;
+      X.in = in[0];
+      X.leaf_work = C + kCtrWork + kWorkDfsLeaf;
+      engine_leaves<<<leaf_grid(n_leaf), kBlock, 0, s>>>(X);
+      ENG_TRY(hipGetLastError());
     }
     clock_end(ws, s, t, kPhaseLeaves);
-    if (unc_n + unc_d > 0) ENG_TRY(read_counter(ws, kCtrNext, nnext, s));
+    if (n_unc > 0) ENG_TRY(read_counter(ws, kCtrNext, nnext, s));
     cur = nxt;
     nfront = nnext;
     // widen a front too small to give every wavefront a full step

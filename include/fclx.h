@@ -188,6 +188,7 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "defer_uncertain"   >= 64   deferred uncertain-task buffer, entries
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
+ *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
  * Results do not depend on them; smaller buffers mean more rounds. */
 int fclx_set_engine_param(fclx_context* ctx, const char* name, double value);
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
