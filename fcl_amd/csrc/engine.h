@@ -26,7 +26,7 @@ struct EngineTuning {
   long long defer_uncertain = 1LL << 24;  // deferred uncertain-task buffer (entries)
   double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
   int stop_poll = 16;           // depth-first steps between polls of the round's stop flag
-  int dfs_waves_per_simd = 5;   // register budget of the depth-first kernel (4..6)
+  int dfs_waves_per_simd = 4;   // register budget of the depth-first kernel (4..6)
 };
 EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 
