@@ -160,6 +160,42 @@ void fclo_distance_batch(const fclo_model *const *models, int num_queries,
                          double *dist, double *points, int *ids,
                          int64_t *bv_tests, int64_t *leaf_tests, int threads);
 
+/* ---- primitive shapes, GST_INDEP solver (fclo_shape.c) ------------------- */
+enum {
+  FCLO_SHAPE_BOX = 0,      /* dims = side lengths x, y, z */
+  FCLO_SHAPE_SPHERE = 1,   /* dims[0] = radius */
+  FCLO_SHAPE_CAPSULE = 2,  /* dims[0] = radius, dims[1] = lz */
+  FCLO_SHAPE_CYLINDER = 3, /* dims[0] = radius, dims[1] = lz */
+  FCLO_SHAPE_CONVEX = 4    /* vertices (local frame), num_vertices */
+};
+
+typedef struct {
+  int type;
+  int num_vertices;
+  double dims[3];
+  const double *vertices;
+} fclo_shape;
+
+typedef struct {
+  double normal[3];
+  double pos[3];
+  double depth;
+} fclo_contact;
+
+/* GJKSolver_indep::shapeIntersect with the contact vector (up to 4 for two
+ * boxes, else 1); out may be NULL for the boolean. */
+int fclo_shape_intersect(const fclo_shape *s1, const double tf1[12], const fclo_shape *s2,
+                         const double tf2[12], double tol, fclo_contact *out, int *n);
+/* collide() of two shapes: contacts added (<= cap) into out (room for 8). */
+int fclo_shape_collide(const fclo_shape *s1, const double tf1[12], const fclo_shape *s2,
+                       const double tf2[12], int cap, int enable_contact, double tol,
+                       fclo_contact *out);
+/* the same per query (OpenMP): counts[q]; out gets cap slots per query */
+void fclo_shape_collide_batch(const fclo_shape *shapes, int num_queries, const int *qs,
+                              const double *tf1s, const double *tf2s, int cap,
+                              int enable_contact, double tol, int *counts, fclo_contact *out,
+                              int threads);
+
 #ifdef __cplusplus
 }
 #endif
