@@ -160,7 +160,15 @@ void fclo_distance_batch(const fclo_model *const *models, int num_queries,
                          double *dist, double *points, int *ids,
                          int64_t *bv_tests, int64_t *leaf_tests, int threads);
 
-/* ---- primitive shapes, GST_INDEP solver (fclo_shape.c) ------------------- */
+/* ---- primitive shapes, GST_INDEP solver (fclo_shape.c) -------------------
+ * Pinned by the reference's shapeIntersectionGJK_* known answers (booleans,
+ * contact counts, normals at the reference's tolerances, closed-form points
+ * and depths), an LP intersection test on polytope pairs, and EPA's bound
+ * 0 <= depth <= true penetration.  Parity unpinned: EPA's depth and contact
+ * point beyond that bound (the reference's tests never check them for this
+ * solver, and its EPA can stop early on near-coplanar supports), the order
+ * of equally deep contacts when a cap cuts them (std::partial_sort), and
+ * box-box contact culling's atan2 (libm here). */
 enum {
   FCLO_SHAPE_BOX = 0,      /* dims = side lengths x, y, z */
   FCLO_SHAPE_SPHERE = 1,   /* dims[0] = radius */

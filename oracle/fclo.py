@@ -88,6 +88,14 @@ def lib():
             "fclo_distance_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
                                            P_dbl, c_i32, P_dbl, P_dbl, P_i32, P_i64, P_i64,
                                            c_i32]),
+            "fclo_shape_intersect": (c_i32, [ctypes.POINTER(Shape), P_dbl, ctypes.POINTER(Shape),
+                                             P_dbl, c_dbl, ctypes.POINTER(Contact),
+                                             ctypes.POINTER(c_i32)]),
+            "fclo_shape_collide": (c_i32, [ctypes.POINTER(Shape), P_dbl, ctypes.POINTER(Shape),
+                                           P_dbl, c_i32, c_i32, c_dbl, ctypes.POINTER(Contact)]),
+            "fclo_shape_collide_batch": (None, [ctypes.POINTER(Shape), c_i32, P_i32, P_dbl, P_dbl,
+                                                c_i32, c_i32, c_dbl, P_i32,
+                                                ctypes.POINTER(Contact), c_i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -95,6 +103,77 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+BOX, SPHERE, CAPSULE, CYLINDER, CONVEX = 0, 1, 2, 3, 4
+
+
+class Shape(ctypes.Structure):
+    _fields_ = [("type", c_i32), ("num_vertices", c_i32), ("dims", c_dbl * 3),
+                ("vertices", P_dbl)]
+
+
+class Contact(ctypes.Structure):
+    _fields_ = [("normal", c_dbl * 3), ("pos", c_dbl * 3), ("depth", c_dbl)]
+
+
+def make_shape(kind, dims=(0, 0, 0), vertices=None):
+    """A primitive for the shape functions; vertices (n,3) for CONVEX (kept
+    alive on the returned object)."""
+    s = Shape()
+    s.type = kind
+    d = list(dims) + [0] * (3 - len(dims))
+    for i in range(3):
+        s.dims[i] = float(d[i])
+    if vertices is not None:
+        v = np.ascontiguousarray(vertices, np.float64)
+        s._v = v
+        s.vertices = v.ctypes.data_as(ctypes.POINTER(c_dbl))
+        s.num_vertices = v.shape[0]
+    return s
+
+
+def shape_intersect(s1, tf1, s2, tf2, tol=1e-6, contacts=True):
+    """GJKSolver_indep::shapeIntersect: (hit, [(normal, pos, depth), ...])."""
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    out = (Contact * 8)()
+    n = c_i32(0)
+    hit = lib().fclo_shape_intersect(ctypes.byref(s1), _p(tf1, c_dbl), ctypes.byref(s2),
+                                     _p(tf2, c_dbl), tol, out if contacts else None,
+                                     ctypes.byref(n))
+    cs = [(np.array(out[i].normal[:]), np.array(out[i].pos[:]), out[i].depth)
+          for i in range(n.value)]
+    return bool(hit), cs
+
+
+def shape_collide(s1, tf1, s2, tf2, cap=1, enable_contact=True, tol=1e-6):
+    """collide() of two primitives: the contacts added (normal, pos, depth)."""
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    out = (Contact * 8)()
+    n = lib().fclo_shape_collide(ctypes.byref(s1), _p(tf1, c_dbl), ctypes.byref(s2),
+                                 _p(tf2, c_dbl), cap, 1 if enable_contact else 0, tol, out)
+    return [(np.array(out[i].normal[:]), np.array(out[i].pos[:]), out[i].depth)
+            for i in range(n)]
+
+
+def shape_collide_batch(shapes, qs, tf1, tf2, cap=1, enable_contact=True, tol=1e-6,
+                        threads=None):
+    """Per query q: shapes[qs[q,0]] vs shapes[qs[q,1]].  Returns counts (n,)
+    and contacts (n, cap, 7) = normal, pos, depth."""
+    n = qs.shape[0]
+    arr = (Shape * len(shapes))(*shapes)
+    qs = np.ascontiguousarray(qs, np.int32)
+    tf1 = _d(tf1)
+    tf2 = _d(tf2)
+    counts = np.zeros(n, np.int32)
+    out = (Contact * max(1, n * cap))()
+    lib().fclo_shape_collide_batch(arr, n, _p(qs, c_i32), _p(tf1, c_dbl), _p(tf2, c_dbl),
+                                   cap, 1 if enable_contact else 0, tol, _p(counts, c_i32),
+                                   out, threads or (os.cpu_count() or 1))
+    c = np.frombuffer(out, dtype=np.float64).reshape(-1, 7)[: n * cap].reshape(n, cap, 7).copy()
+    return counts, c
 
 
 def _p(a, t):
