@@ -16,6 +16,37 @@ OBBRSS = N.FCLX_BV_OBBRSS
 RSS = N.FCLX_BV_RSS
 ENGINE_PLAIN = N.FCLX_ENGINE_PLAIN
 ENGINE_FAST = N.FCLX_ENGINE_FAST
+BOX = N.FCLX_SHAPE_BOX
+SPHERE = N.FCLX_SHAPE_SPHERE
+CAPSULE = N.FCLX_SHAPE_CAPSULE
+CYLINDER = N.FCLX_SHAPE_CYLINDER
+CONVEX = N.FCLX_SHAPE_CONVEX
+
+# one fclx_shape record (include/fclx.h), 40 bytes
+SHAPE_DTYPE = np.dtype([("type", "<i4"), ("num_vertices", "<i4"), ("dims", "<f8", (3,)),
+                        ("vertex_offset", "<i8")])
+assert SHAPE_DTYPE.itemsize == ctypes.sizeof(N.ShapeRec) and all(
+    SHAPE_DTYPE.fields[f][1] == getattr(N.ShapeRec, f).offset for f, _ in N.ShapeRec._fields_)
+
+
+def shape_table(types, dims, convex_vertices=()):
+    """Records for fclx_shape_collide_batch.  types (n,), dims (n,3); the k-th
+    CONVEX entry takes convex_vertices[k] ((m,3) array).  Returns (records,
+    vertex pool (V,3) float64)."""
+    types = np.asarray(types, np.int32)
+    rec = np.zeros(len(types), SHAPE_DTYPE)
+    rec["type"] = types
+    rec["dims"] = np.asarray(dims, np.float64).reshape(len(types), 3)
+    conv = np.flatnonzero(types == CONVEX)
+    if len(conv) != len(convex_vertices):
+        raise ValueError("one vertex array per CONVEX shape is needed")
+    counts = np.array([len(v) for v in convex_vertices], np.int64)
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]]) if len(counts) else counts
+    rec["num_vertices"][conv] = counts
+    rec["vertex_offset"][conv] = offs
+    pool = (np.concatenate([np.asarray(v, np.float64).reshape(-1, 3) for v in convex_vertices])
+            if len(convex_vertices) else np.zeros((0, 3)))
+    return rec, np.ascontiguousarray(pool)
 
 
 class FclxError(RuntimeError):
@@ -250,6 +281,30 @@ class Context:
         a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
         a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
         self._check(self._L.fclx_distance_batch(self._h, ctypes.byref(a), stream))
+
+    def shape_collide_batch_async(self, shapes, vertices, pair_shapes, tf1, tf2,
+                                  num_max_contacts, counts, contact=None, tol=1e-6,
+                                  stream=None):
+        """fclx_shape_collide_batch.  shapes: device uint8 tensor of fclx_shape
+        records (shape_table's, viewed as bytes); vertices: device float64
+        (V,3) or None; contact: None, or (pos, normal, depth) device tensors of
+        (n*cap,3), (n*cap,3), (n*cap,) for enable_contact."""
+        a = N.ShapeCollideBatchArgs()
+        a.num_pairs = int(pair_shapes.shape[0])
+        a.num_shapes = int(shapes.numel() // SHAPE_DTYPE.itemsize)
+        a.shapes = shapes.data_ptr()
+        a.vertices = vertices.data_ptr() if vertices is not None and vertices.numel() else None
+        a.num_vertices = int(vertices.shape[0]) if a.vertices else 0
+        a.pair_shapes = pair_shapes.data_ptr()
+        a.tf1 = tf1.data_ptr()
+        a.tf2 = tf2.data_ptr()
+        a.num_max_contacts = int(num_max_contacts)
+        a.gjk_tolerance = float(tol)
+        a.counts = counts.data_ptr()
+        if contact is not None:
+            a.enable_contact = 1
+            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
+        self._check(self._L.fclx_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
     def set_profiling(self, enable=True):
         self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))

@@ -20,6 +20,11 @@ FCLX_ERR_STACK_OVERFLOW = -7
 
 FCLX_BV_OBBRSS = 0
 FCLX_BV_RSS = 1
+FCLX_SHAPE_BOX = 0
+FCLX_SHAPE_SPHERE = 1
+FCLX_SHAPE_CAPSULE = 2
+FCLX_SHAPE_CYLINDER = 3
+FCLX_SHAPE_CONVEX = 4
 FCLX_ENGINE_PLAIN = 0
 FCLX_ENGINE_FAST = 1
 
@@ -58,6 +63,19 @@ class DistanceBatchArgs(ctypes.Structure):
                 ("points", c_vp), ("ids", c_vp), ("bv_tests", c_vp), ("leaf_tests", c_vp)]
 
 
+class ShapeRec(ctypes.Structure):
+    _fields_ = [("type", c_i32), ("num_vertices", c_i32), ("dims", c_dbl * 3),
+                ("vertex_offset", c_i64)]
+
+
+class ShapeCollideBatchArgs(ctypes.Structure):
+    _fields_ = [("num_pairs", c_i32), ("num_shapes", c_i32), ("shapes", c_vp),
+                ("vertices", c_vp), ("num_vertices", c_i64), ("pair_shapes", c_vp),
+                ("tf1", c_vp), ("tf2", c_vp), ("num_max_contacts", c_i32),
+                ("enable_contact", c_i32), ("gjk_tolerance", c_dbl), ("counts", c_vp),
+                ("contact_pos", c_vp), ("contact_normal", c_vp), ("contact_depth", c_vp)]
+
+
 class EngineReport(ctypes.Structure):
     _fields_ = [("setup_ms", c_dbl), ("expand_ms", c_dbl), ("leaves_ms", c_dbl),
                 ("dfs_ms", c_dbl), ("fixup_ms", c_dbl), ("expand_bv_tests", c_i64),
@@ -85,6 +103,7 @@ SIGNATURES = {
     "fclx_collide_batch": (c_i32, [c_vp, ctypes.POINTER(CollideBatchArgs), c_vp]),
     "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
     "fclx_sync": (c_i32, [c_vp, c_vp]),
+    "fclx_shape_collide_batch": (c_i32, [c_vp, ctypes.POINTER(ShapeCollideBatchArgs), c_vp]),
     "fclx_collide": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, c_i32, P_i32,
                               P_i64]),
     "fclx_distance": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_dbl, P_dbl, P_i32,

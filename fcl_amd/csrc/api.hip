@@ -520,6 +520,39 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
   return FCLX_OK;
 }
 
+int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_args* a,
+                             void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->num_pairs < 0 || a->num_shapes < 0 || a->num_vertices < 0 ||
+      a->num_max_contacts < 1 || !(a->gjk_tolerance > 0) ||
+      (a->num_pairs > 0 &&
+       (!a->shapes || !a->pair_shapes || !a->tf1 || !a->tf2 || !a->counts)) ||
+      (a->num_pairs > 0 && a->enable_contact &&
+       (!a->contact_pos || !a->contact_normal || !a->contact_depth)))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid shape batch arguments");
+  (void)hipSetDevice(ctx->device);
+  ShapeLaunch L{};
+  L.num_pairs = a->num_pairs;
+  L.num_shapes = a->num_shapes;
+  L.shapes = a->shapes;
+  L.vertices = a->vertices;
+  L.num_vertices = a->vertices ? a->num_vertices : 0;
+  L.pair_shapes = a->pair_shapes;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.cap = a->num_max_contacts;
+  L.contact = a->enable_contact ? 1 : 0;
+  L.tol = a->gjk_tolerance;
+  L.counts = a->counts;
+  L.contact_pos = a->contact_pos;
+  L.contact_normal = a->contact_normal;
+  L.contact_depth = a->contact_depth;
+  L.status = ctx->d_status;
+  hipError_t e = launch_shape_collide(L, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(ctx, e, "shape collide launch");
+  return FCLX_OK;
+}
+
 int fclx_set_profiling(fclx_context* ctx, int32_t enable) {
   if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);
@@ -593,7 +626,7 @@ int fclx_sync(fclx_context* ctx, void* stream) {
       return fail(ctx, FCLX_ERR_STACK_OVERFLOW, "a traversal exceeded its stack bound");
     if (st & kStatusBadQuery)
       return fail(ctx, FCLX_ERR_INVALID_ARGUMENT,
-                  "a query named a missing mesh or an unsupported BV pair");
+                  "a query named a missing mesh or shape, or an unsupported BV pair");
     if (st & kStatusPairOverflow)
       return fail(ctx, FCLX_ERR_CAPACITY, "pair output exceeded pair_capacity");
   }

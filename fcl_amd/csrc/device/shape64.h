@@ -1,0 +1,1386 @@
+// FP64 primitive-shape collision with FCL's GST_INDEP solver, one shape pair
+// per lane: supports of box / sphere / capsule / cylinder / convex, GJK
+// with the reference's simplex bookkeeping, EPA with its face lists kept in
+// the reference's order, and the five closed forms the solver dispatches
+// to.  Evaluation order as in exact64.h (no fused multiply-add).
+//
+//   supports          convexity_based_algorithm/minkowski_diff-inl.h:64-273
+//   GJK               convexity_based_algorithm/gjk-inl.h:82-315
+//   EPA               convexity_based_algorithm/epa-inl.h:54-396
+//   projections       math/detail/project-inl.h:123-316
+//   closed forms      primitive_shape_algorithm/{sphere_sphere,sphere_box,
+//                     sphere_capsule,sphere_cylinder,box_box}-inl.h
+//   dispatch          gjk_solver_indep-inl.h:110-305
+//
+// EPA's recursive horizon walk (epa-inl.h:347-396) runs on an explicit
+// stack, and the recursive simplex completion (gjk-inl.h:230-290) as one
+// function per simplex rank; both visit in the reference's order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fclx {
+namespace shp {
+
+struct D3 {
+  double v[3];
+};
+
+__device__ __forceinline__ D3 mk(double a, double b, double c) { return D3{{a, b, c}}; }
+__device__ __forceinline__ D3 add(const D3& a, const D3& b) {
+  return mk(a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]);
+}
+__device__ __forceinline__ D3 sub(const D3& a, const D3& b) {
+  return mk(a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]);
+}
+__device__ __forceinline__ D3 neg(const D3& a) { return mk(-a.v[0], -a.v[1], -a.v[2]); }
+__device__ __forceinline__ D3 mul(const D3& a, double s) {
+  return mk(a.v[0] * s, a.v[1] * s, a.v[2] * s);
+}
+__device__ __forceinline__ D3 dvd(const D3& a, double s) {
+  return mk(a.v[0] / s, a.v[1] / s, a.v[2] / s);
+}
+__device__ __forceinline__ double dot(const D3& a, const D3& b) {
+  return (a.v[0] * b.v[0] + a.v[1] * b.v[1]) + a.v[2] * b.v[2];
+}
+__device__ __forceinline__ D3 cross(const D3& a, const D3& b) {
+  return mk(a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2],
+            a.v[0] * b.v[1] - a.v[1] * b.v[0]);
+}
+__device__ __forceinline__ double sqn(const D3& a) { return dot(a, a); }
+__device__ __forceinline__ double nrm(const D3& a) { return sqrt(sqn(a)); }
+__device__ __forceinline__ D3 normalized(const D3& a) {  // Eigen: unchanged at zero
+  const double z = sqn(a);
+  return z > 0 ? dvd(a, sqrt(z)) : a;
+}
+__device__ __forceinline__ double triple(const D3& a, const D3& b, const D3& c) {
+  return dot(a, cross(b, c));
+}
+
+struct Xf {
+  double R[9];  // row-major
+  D3 t;
+};
+
+__device__ __forceinline__ Xf xf_load(const double* tf) {
+  Xf r;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) r.R[i] = tf[i];
+  r.t = mk(tf[9], tf[10], tf[11]);
+  return r;
+}
+__device__ __forceinline__ D3 rot(const double* R, const D3& p) {
+  return mk(dot(mk(R[0], R[1], R[2]), p), dot(mk(R[3], R[4], R[5]), p),
+            dot(mk(R[6], R[7], R[8]), p));
+}
+__device__ __forceinline__ D3 rotT(const double* R, const D3& p) {
+  return mk(dot(mk(R[0], R[3], R[6]), p), dot(mk(R[1], R[4], R[7]), p),
+            dot(mk(R[2], R[5], R[8]), p));
+}
+__device__ __forceinline__ D3 apply(const Xf& T, const D3& p) { return add(rot(T.R, p), T.t); }
+__device__ __forceinline__ D3 col(const double* R, int j) { return mk(R[j], R[3 + j], R[6 + j]); }
+__device__ __forceinline__ void mulTN(const double* A, const double* B, double* out) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      out[3 * i + j] = (A[i] * B[j] + A[3 + i] * B[3 + j]) + A[6 + i] * B[6 + j];
+}
+// a^-1 b for isometries: (Ra^T Rb, Ra^T tb + (-(Ra^T ta)))
+__device__ __forceinline__ Xf inv_compose(const Xf& a, const Xf& b) {
+  Xf r;
+  mulTN(a.R, b.R, r.R);
+  r.t = add(rotT(a.R, b.t), neg(rotT(a.R, a.t)));
+  return r;
+}
+
+enum : int { kBox = 0, kSphere = 1, kCapsule = 2, kCylinder = 3, kConvex = 4 };
+
+// One primitive: type, dims (box sides / radius / radius+lz) and, for a
+// convex, its vertices (local frame).
+struct Shape {
+  int type;
+  int nv;
+  double dims[3];
+  const double* verts;
+};
+
+struct Contact {
+  D3 normal, pos;
+  double depth;
+};
+
+// ------------------------------------------------------------ supports
+__device__ inline D3 support(const Shape& s, const D3& d) {
+  switch (s.type) {
+    case kBox:
+      return mk(d.v[0] > 0 ? s.dims[0] / 2 : -s.dims[0] / 2,
+                d.v[1] > 0 ? s.dims[1] / 2 : -s.dims[1] / 2,
+                d.v[2] > 0 ? s.dims[2] / 2 : -s.dims[2] / 2);
+    case kSphere:
+      return mul(d, s.dims[0]);
+    case kCapsule: {
+      const double half_h = s.dims[1] * 0.5;
+      const D3 v = mul(d, s.dims[0]);
+      const D3 p1 = add(mk(0, 0, half_h), v), p2 = add(mk(0, 0, -half_h), v);
+      return dot(d, p1) > dot(d, p2) ? p1 : p2;
+    }
+    case kCylinder: {
+      const double zdist = sqrt(d.v[0] * d.v[0] + d.v[1] * d.v[1]);
+      const double half_h = s.dims[1] * 0.5;
+      if (zdist == 0.0) return mk(0, 0, d.v[2] > 0 ? half_h : -half_h);
+      const double k = s.dims[0] / zdist;
+      return mk(k * d.v[0], k * d.v[1], d.v[2] > 0 ? half_h : -half_h);
+    }
+    default: {  // convex
+      double maxdot = -1.7976931348623157e308;
+      D3 best = mk(0, 0, 0);
+      for (int i = 0; i < s.nv; ++i) {
+        const D3 p = mk(s.verts[3 * i], s.verts[3 * i + 1], s.verts[3 * i + 2]);
+        const double dt = dot(d, p);
+        if (dt > maxdot) {
+          best = p;
+          maxdot = dt;
+        }
+      }
+      return best;
+    }
+  }
+}
+
+struct MDiff {
+  Shape s0, s1;
+  double toshape1[9];  // R2^T R1
+  Xf toshape0;         // tf1^-1 tf2
+};
+
+__device__ __forceinline__ D3 support0(const MDiff& m, const D3& d) { return support(m.s0, d); }
+__device__ __forceinline__ D3 support1(const MDiff& m, const D3& d) {
+  return apply(m.toshape0, support(m.s1, rot(m.toshape1, d)));
+}
+__device__ __forceinline__ D3 msupport(const MDiff& m, const D3& d) {
+  return sub(support0(m, d), support1(m, neg(d)));
+}
+
+// -------------------------------------------------- simplex projection
+struct Proj {
+  double p[4];
+  double sqr;
+  int encode;
+};
+
+__device__ __forceinline__ Proj proj_init() {
+  Proj r;
+  r.p[0] = r.p[1] = r.p[2] = r.p[3] = 0;
+  r.sqr = -1;
+  r.encode = 0;
+  return r;
+}
+
+__device__ inline Proj project_line_origin(const D3& a, const D3& b) {
+  Proj r = proj_init();
+  const D3 d = sub(b, a);
+  const double l = sqn(d);
+  if (l > 0) {
+    const double t = -dot(a, d);
+    r.p[1] = (t >= l) ? 1 : ((t <= 0) ? 0 : (t / l));
+    r.p[0] = 1 - r.p[1];
+    if (t >= l) {
+      r.sqr = sqn(b);
+      r.encode = 2;
+    } else if (t <= 0) {
+      r.sqr = sqn(a);
+      r.encode = 1;
+    } else {
+      r.sqr = sqn(add(a, mul(d, r.p[1])));
+      r.encode = 3;
+    }
+  }
+  return r;
+}
+
+__device__ inline Proj project_triangle_origin(const D3& a, const D3& b, const D3& c) {
+  const int nexti[3] = {1, 2, 0};
+  Proj r = proj_init();
+  const D3 vt[3] = {a, b, c};
+  const D3 dl[3] = {sub(a, b), sub(b, c), sub(c, a)};
+  const D3 n = cross(dl[0], dl[1]);
+  const double l = sqn(n);
+  if (l > 0) {
+    double mindist = -1;
+    for (int i = 0; i < 3; ++i) {
+      if (dot(vt[i], cross(dl[i], n)) > 0) {
+        const int j = nexti[i];
+        const Proj rl = project_line_origin(vt[i], vt[j]);
+        if (mindist < 0 || rl.sqr < mindist) {
+          mindist = rl.sqr;
+          r.encode = ((rl.encode & 1) ? 1 << i : 0) + ((rl.encode & 2) ? 1 << j : 0);
+          r.p[i] = rl.p[0];
+          r.p[j] = rl.p[1];
+          r.p[nexti[j]] = 0;
+        }
+      }
+    }
+    if (mindist < 0) {
+      const double d = dot(a, n);
+      const double s = sqrt(l);
+      const D3 o = mul(n, d / l);
+      mindist = sqn(o);
+      r.encode = 7;
+      r.p[0] = nrm(cross(dl[1], sub(b, o))) / s;
+      r.p[1] = nrm(cross(dl[2], sub(c, o))) / s;
+      r.p[2] = 1 - r.p[0] - r.p[1];
+    }
+    r.sqr = mindist;
+  }
+  return r;
+}
+
+__device__ inline Proj project_tetra_origin(const D3& a, const D3& b, const D3& c, const D3& d) {
+  const int nexti[3] = {1, 2, 0};
+  Proj r = proj_init();
+  const D3 vt[4] = {a, b, c, d};
+  const D3 dl[3] = {sub(a, d), sub(b, d), sub(c, d)};
+  const double vl = triple(dl[0], dl[1], dl[2]);
+  const bool ng = (vl * dot(a, cross(sub(b, c), sub(a, b)))) <= 0;
+  if (ng && fabs(vl) > 0) {
+    double mindist = -1;
+    for (int i = 0; i < 3; ++i) {
+      const int j = nexti[i];
+      const double s = vl * dot(d, cross(dl[i], dl[j]));
+      if (s > 0) {
+        const Proj rt = project_triangle_origin(vt[i], vt[j], d);
+        if (mindist < 0 || rt.sqr < mindist) {
+          mindist = rt.sqr;
+          r.encode = ((rt.encode & 1) ? 1 << i : 0) + ((rt.encode & 2) ? 1 << j : 0) +
+                     ((rt.encode & 4) ? 8 : 0);
+          r.p[i] = rt.p[0];
+          r.p[j] = rt.p[1];
+          r.p[nexti[j]] = 0;
+          r.p[3] = rt.p[2];
+        }
+      }
+    }
+    if (mindist < 0) {
+      mindist = 0;
+      r.encode = 15;
+      r.p[0] = triple(c, b, d) / vl;
+      r.p[1] = triple(a, c, d) / vl;
+      r.p[2] = triple(b, a, d) / vl;
+      r.p[3] = 1 - (r.p[0] + r.p[1] + r.p[2]);
+    }
+    r.sqr = mindist;
+  } else if (!ng) {
+    r = project_triangle_origin(a, b, c);
+    r.p[3] = 0;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------- GJK
+enum : int { kGjkValid = 0, kGjkInside = 1, kGjkFailed = 2 };
+constexpr int kNvGjk = 4;
+constexpr int kNvEpa = 64;
+constexpr int kNfEpa = 128;
+
+struct SVert {
+  D3 d, w;
+};
+
+struct Simplex {
+  int c[4];
+  double p[4];
+  int rank;
+};
+
+// vertex store shared by GJK (0..3) and EPA (4..67), in private memory
+struct GjkState {
+  SVert vs[kNvGjk + kNvEpa];
+  int free_v[4];
+  int nfree;
+  Simplex sx[2];
+  int current;
+  D3 ray;
+};
+
+__device__ __forceinline__ void get_sv(const MDiff& m, const D3& d, SVert& sv) {
+  sv.d = normalized(d);
+  sv.w = msupport(m, sv.d);
+}
+__device__ __forceinline__ void sx_remove(GjkState& g, Simplex& s) {
+  g.free_v[g.nfree++] = s.c[--s.rank];
+}
+__device__ __forceinline__ void sx_append(GjkState& g, const MDiff& m, Simplex& s, const D3& v) {
+  s.p[s.rank] = 0;
+  s.c[s.rank] = g.free_v[--g.nfree];
+  get_sv(m, v, g.vs[s.c[s.rank++]]);
+}
+
+__device__ inline int gjk_evaluate(GjkState& g, const MDiff& m, const D3& guess, double tol) {
+  int iterations = 0;
+  double alpha = 0;
+  D3 lastw[4];
+  int clastw = 0;
+  for (int i = 0; i < 4; ++i) g.free_v[i] = i;
+  g.nfree = 4;
+  g.current = 0;
+  int status = kGjkValid;
+  g.sx[0].rank = 0;
+  g.ray = guess;
+  sx_append(g, m, g.sx[0], sqn(g.ray) > 0 ? neg(g.ray) : mk(1, 0, 0));
+  g.sx[0].p[0] = 1;
+  g.ray = g.vs[g.sx[0].c[0]].w;
+  lastw[0] = lastw[1] = lastw[2] = lastw[3] = g.ray;
+  do {
+    const int next = 1 - g.current;
+    Simplex& cs = g.sx[g.current];
+    Simplex& ns = g.sx[next];
+    const double rl = nrm(g.ray);
+    if (rl < tol) {
+      status = kGjkInside;
+      break;
+    }
+    sx_append(g, m, cs, neg(g.ray));
+    const D3 w = g.vs[cs.c[cs.rank - 1]].w;
+    bool found = false;
+    for (int i = 0; i < 4; ++i)
+      if (sqn(sub(w, lastw[i])) < tol) {
+        found = true;
+        break;
+      }
+    if (found) {
+      sx_remove(g, cs);
+      break;
+    }
+    lastw[clastw = (clastw + 1) & 3] = w;
+    const double omega = dot(g.ray, w) / rl;
+    alpha = (alpha < omega) ? omega : alpha;  // std::max
+    if ((rl - alpha) - tol * rl <= 0) {
+      sx_remove(g, cs);
+      break;
+    }
+    Proj pr = proj_init();
+    if (cs.rank == 2)
+      pr = project_line_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w);
+    else if (cs.rank == 3)
+      pr = project_triangle_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w, g.vs[cs.c[2]].w);
+    else if (cs.rank == 4)
+      pr = project_tetra_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w, g.vs[cs.c[2]].w,
+                                g.vs[cs.c[3]].w);
+    if (pr.sqr >= 0) {
+      ns.rank = 0;
+      g.ray = mk(0, 0, 0);
+      g.current = next;
+      for (int i = 0; i < cs.rank; ++i) {
+        if (pr.encode & (1 << i)) {
+          ns.c[ns.rank] = cs.c[i];
+          ns.p[ns.rank++] = pr.p[i];
+          g.ray = add(g.ray, mul(g.vs[cs.c[i]].w, pr.p[i]));
+        } else {
+          g.free_v[g.nfree++] = cs.c[i];
+        }
+      }
+      if (pr.encode == 15) status = kGjkInside;
+    } else {
+      sx_remove(g, cs);
+      break;
+    }
+    status = (++iterations < 128) ? status : kGjkFailed;
+  } while (status == kGjkValid);
+  return status;
+}
+
+// encloseOrigin (gjk-inl.h:230-290), one function per rank: each tries the
+// reference's candidate directions in order and recurses one rank up.
+__device__ inline bool enclose4(GjkState& g, Simplex& s) {
+  const D3 w3 = g.vs[s.c[3]].w;
+  return fabs(triple(sub(g.vs[s.c[0]].w, w3), sub(g.vs[s.c[1]].w, w3),
+                     sub(g.vs[s.c[2]].w, w3))) > 0;
+}
+__device__ inline bool enclose3(GjkState& g, const MDiff& m, Simplex& s) {
+  const D3 n = cross(sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w), sub(g.vs[s.c[2]].w, g.vs[s.c[0]].w));
+  if (sqn(n) > 0) {
+    sx_append(g, m, s, n);
+    if (enclose4(g, s)) return true;
+    sx_remove(g, s);
+    sx_append(g, m, s, neg(n));
+    if (enclose4(g, s)) return true;
+    sx_remove(g, s);
+  }
+  return false;
+}
+__device__ inline bool enclose2(GjkState& g, const MDiff& m, Simplex& s) {
+  const D3 d = sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w);
+  for (int i = 0; i < 3; ++i) {
+    D3 axis = mk(0, 0, 0);
+    axis.v[i] = 1;
+    const D3 p = cross(d, axis);
+    if (sqn(p) > 0) {
+      sx_append(g, m, s, p);
+      if (enclose3(g, m, s)) return true;
+      sx_remove(g, s);
+      sx_append(g, m, s, neg(p));
+      if (enclose3(g, m, s)) return true;
+      sx_remove(g, s);
+    }
+  }
+  return false;
+}
+__device__ inline bool enclose1(GjkState& g, const MDiff& m, Simplex& s) {
+  for (int i = 0; i < 3; ++i) {
+    D3 axis = mk(0, 0, 0);
+    axis.v[i] = 1;
+    sx_append(g, m, s, axis);
+    if (enclose2(g, m, s)) return true;
+    sx_remove(g, s);
+    sx_append(g, m, s, neg(axis));
+    if (enclose2(g, m, s)) return true;
+    sx_remove(g, s);
+  }
+  return false;
+}
+__device__ inline bool enclose_origin(GjkState& g, const MDiff& m, Simplex& s) {
+  switch (s.rank) {
+    case 1:
+      return enclose1(g, m, s);
+    case 2:
+      return enclose2(g, m, s);
+    case 3:
+      return enclose3(g, m, s);
+    case 4:
+      return enclose4(g, s);
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- EPA
+enum : int {
+  kEpaValid,
+  kEpaTouching,
+  kEpaDegenerated,
+  kEpaNonConvex,
+  kEpaInvalidHull,
+  kEpaOutOfFaces,
+  kEpaOutOfVertices,
+  kEpaAccuracyReached,
+  kEpaFallBack,
+  kEpaFailed
+};
+
+struct Face {
+  D3 n;
+  double d;
+  int8_t c[3];  // vertex indices
+  int8_t f[3];  // neighbour faces
+  int8_t e[3];  // neighbour edges
+  int8_t l[2];  // list links (-1: none)
+  int16_t pass;
+};
+
+struct Epa {
+  Face fc[kNfEpa];
+  int hull_root, hull_count, stock_root, stock_count;
+  int nextsv;
+  int status;
+  D3 normal;
+  double depth;
+  int rank;
+  int rc[3];
+  double rp[3];
+};
+
+__device__ __forceinline__ void list_append(Epa& E, bool hull, int f) {
+  int& root = hull ? E.hull_root : E.stock_root;
+  E.fc[f].l[0] = -1;
+  E.fc[f].l[1] = static_cast<int8_t>(root);
+  if (root >= 0) E.fc[root].l[0] = static_cast<int8_t>(f);
+  root = f;
+  ++(hull ? E.hull_count : E.stock_count);
+}
+__device__ __forceinline__ void list_remove(Epa& E, bool hull, int f) {
+  int& root = hull ? E.hull_root : E.stock_root;
+  const int l0 = E.fc[f].l[0], l1 = E.fc[f].l[1];
+  if (l1 >= 0) E.fc[l1].l[0] = static_cast<int8_t>(l0);
+  if (l0 >= 0) E.fc[l0].l[1] = static_cast<int8_t>(l1);
+  if (f == root) root = l1;
+  --(hull ? E.hull_count : E.stock_count);
+}
+__device__ __forceinline__ void bind(Epa& E, int fa, int ea, int fb, int eb) {
+  E.fc[fa].e[ea] = static_cast<int8_t>(eb);
+  E.fc[fa].f[ea] = static_cast<int8_t>(fb);
+  E.fc[fb].e[eb] = static_cast<int8_t>(ea);
+  E.fc[fb].f[eb] = static_cast<int8_t>(fa);
+}
+
+// epa-inl.h:109-141 getEdgeDist (with the reference's distance formula)
+__device__ inline bool edge_dist(const GjkState& g, const D3& n, int a, int b, double& dist) {
+  const D3 wa = g.vs[a].w, wb = g.vs[b].w;
+  const D3 ba = sub(wb, wa);
+  const D3 n_ab = cross(ba, n);
+  if (dot(wa, n_ab) < 0) {
+    const double a_dot_ba = dot(wa, ba), b_dot_ba = dot(wb, ba);
+    if (a_dot_ba > 0)
+      dist = nrm(wa);
+    else if (b_dot_ba < 0)
+      dist = nrm(wb);
+    else {
+      const double a_dot_b = dot(wa, wb);
+      const double v = sqn(wa) * sqn(wb) - a_dot_b * a_dot_b;
+      dist = sqrt(v > 0 ? v : 0.0);
+    }
+    return true;
+  }
+  return false;
+}
+
+__device__ inline int new_face(Epa& E, const GjkState& g, int a, int b, int c, bool forced,
+                               double tol) {
+  if (E.stock_root >= 0) {
+    const int f = E.stock_root;
+    list_remove(E, false, f);
+    list_append(E, true, f);
+    Face& F = E.fc[f];
+    F.pass = 0;
+    F.c[0] = static_cast<int8_t>(a);
+    F.c[1] = static_cast<int8_t>(b);
+    F.c[2] = static_cast<int8_t>(c);
+    const D3 wa = g.vs[a].w;
+    F.n = cross(sub(g.vs[b].w, wa), sub(g.vs[c].w, wa));
+    const double l = nrm(F.n);
+    if (l > tol) {
+      double d;
+      if (!(edge_dist(g, F.n, a, b, d) || edge_dist(g, F.n, b, c, d) ||
+            edge_dist(g, F.n, c, a, d)))
+        d = dot(wa, F.n) / l;
+      F.d = d;
+      F.n = dvd(F.n, l);
+      if (forced || F.d >= -tol) return f;
+      E.status = kEpaNonConvex;
+    } else {
+      E.status = kEpaDegenerated;
+    }
+    list_remove(E, true, f);
+    list_append(E, false, f);
+    return -1;
+  }
+  E.status = E.stock_root >= 0 ? kEpaOutOfVertices : kEpaOutOfFaces;
+  return -1;
+}
+
+__device__ inline int find_best(const Epa& E) {
+  int minf = E.hull_root;
+  double mind = E.fc[minf].d * E.fc[minf].d;
+  for (int f = E.fc[minf].l[1]; f >= 0; f = E.fc[f].l[1]) {
+    const double sqd = E.fc[f].d * E.fc[f].d;
+    if (sqd < mind) {
+      minf = f;
+      mind = sqd;
+    }
+  }
+  return minf;
+}
+
+struct Horizon {
+  int cf, ff, nf;
+};
+
+// epa-inl.h:347-396 expand(pass, w, f, e, horizon), recursion unrolled onto a
+// stack of frames; a frame in stage 1 waits for its first child, stage 2 for
+// its second.  Same visit order and results as the recursive form.
+__device__ inline bool expand(Epa& E, const GjkState& g, int pass, int w, int f0, int e0,
+                              Horizon& h, double tol) {
+  const int nexti[3] = {1, 2, 0};
+  const int previ[3] = {2, 0, 1};
+  int8_t sf[kNfEpa], se[kNfEpa], sst[kNfEpa];
+  int sp = 0;
+  int f = f0, e = e0;
+  bool result = false;
+  bool call = true;  // evaluate (f, e) as a fresh call
+  while (true) {
+    if (call) {
+      Face& F = E.fc[f];
+      result = false;
+      if (F.pass != pass) {
+        const int e1 = nexti[e];
+        if (dot(F.n, g.vs[w].w) - F.d < -tol) {
+          const int nf = new_face(E, g, F.c[e1], F.c[e], w, false, tol);
+          if (nf >= 0) {
+            bind(E, nf, 0, f, e);
+            if (h.cf >= 0)
+              bind(E, nf, 2, h.cf, 1);
+            else
+              h.ff = nf;
+            h.cf = nf;
+            ++h.nf;
+            result = true;
+          }
+        } else {
+          F.pass = static_cast<int16_t>(pass);
+          if (sp >= kNfEpa) return false;  // deeper than the face store: cannot happen
+          sf[sp] = static_cast<int8_t>(f);
+          se[sp] = static_cast<int8_t>(e);
+          sst[sp] = 1;
+          ++sp;
+          const int fe1 = F.f[e1], ee1 = F.e[e1];
+          f = fe1;
+          e = ee1;
+          continue;  // call the first child
+        }
+      }
+      call = false;
+    }
+    // return `result` to the parent frame
+    if (sp == 0) return result;
+    const int pf = sf[sp - 1], pe = se[sp - 1];
+    if (sst[sp - 1] == 1) {
+      if (!result) {
+        --sp;  // parent returns false
+        result = false;
+        continue;
+      }
+      sst[sp - 1] = 2;
+      const int e2 = previ[pe];
+      f = E.fc[pf].f[e2];
+      e = E.fc[pf].e[e2];
+      call = true;
+      continue;  // call the second child
+    }
+    // stage 2: both children done
+    --sp;
+    if (result) {
+      list_remove(E, true, pf);
+      list_append(E, false, pf);
+    }
+  }
+}
+
+__device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3& guess,
+                                   double tol) {
+  E.hull_root = -1;
+  E.hull_count = 0;
+  E.stock_root = -1;
+  E.stock_count = 0;
+  for (int i = 0; i < kNfEpa; ++i) list_append(E, false, kNfEpa - i - 1);
+  E.status = kEpaFailed;
+  E.normal = mk(0, 0, 0);
+  E.depth = 0;
+  E.nextsv = 0;
+  Simplex& s = g.sx[g.current];
+  if (s.rank > 1 && enclose_origin(g, m, s)) {
+    while (E.hull_root >= 0) {
+      const int f = E.hull_root;
+      list_remove(E, true, f);
+      list_append(E, false, f);
+    }
+    E.status = kEpaValid;
+    E.nextsv = 0;
+    const D3 w0 = g.vs[s.c[0]].w, w1 = g.vs[s.c[1]].w, w2 = g.vs[s.c[2]].w, w3 = g.vs[s.c[3]].w;
+    if (dot(sub(w0, w3), cross(sub(w1, w3), sub(w2, w3))) < 0) {
+      const int t = s.c[0];
+      s.c[0] = s.c[1];
+      s.c[1] = t;
+      const double tp = s.p[0];
+      s.p[0] = s.p[1];
+      s.p[1] = tp;
+    }
+    int tet[4];
+    tet[0] = new_face(E, g, s.c[0], s.c[1], s.c[2], true, tol);
+    tet[1] = new_face(E, g, s.c[1], s.c[0], s.c[3], true, tol);
+    tet[2] = new_face(E, g, s.c[2], s.c[1], s.c[3], true, tol);
+    tet[3] = new_face(E, g, s.c[0], s.c[2], s.c[3], true, tol);
+    if (E.hull_count == 4) {
+      int best = find_best(E);
+      Face outer = E.fc[best];
+      int pass = 0;
+      bind(E, tet[0], 0, tet[1], 0);
+      bind(E, tet[0], 1, tet[2], 0);
+      bind(E, tet[0], 2, tet[3], 0);
+      bind(E, tet[1], 1, tet[3], 2);
+      bind(E, tet[1], 2, tet[2], 1);
+      bind(E, tet[2], 2, tet[3], 1);
+      E.status = kEpaValid;
+      for (int it = 0; it < 255; ++it) {
+        if (E.nextsv < kNvEpa) {
+          Horizon h{-1, -1, 0};
+          const int w = kNvGjk + E.nextsv++;
+          bool valid = true;
+          E.fc[best].pass = static_cast<int16_t>(++pass);
+          get_sv(m, E.fc[best].n, g.vs[w]);
+          const double wdist = dot(E.fc[best].n, g.vs[w].w) - E.fc[best].d;
+          if (wdist > tol) {
+            for (int j = 0; j < 3 && valid; ++j)
+              valid &= expand(E, g, pass, w, E.fc[best].f[j], E.fc[best].e[j], h, tol);
+            if (valid && h.nf >= 3) {
+              bind(E, h.ff, 2, h.cf, 1);
+              list_remove(E, true, best);
+              list_append(E, false, best);
+              best = find_best(E);
+              outer = E.fc[best];
+            } else {
+              E.status = kEpaInvalidHull;
+              break;
+            }
+          } else {
+            E.status = kEpaAccuracyReached;
+            break;
+          }
+        } else {
+          E.status = kEpaOutOfVertices;
+          break;
+        }
+      }
+      const D3 projection = mul(outer.n, outer.d);
+      E.normal = outer.n;
+      E.depth = outer.d;
+      E.rank = 3;
+      for (int k = 0; k < 3; ++k) E.rc[k] = outer.c[k];
+      const D3 a = g.vs[outer.c[0]].w, b = g.vs[outer.c[1]].w, c = g.vs[outer.c[2]].w;
+      E.rp[0] = nrm(cross(sub(b, projection), sub(c, projection)));
+      E.rp[1] = nrm(cross(sub(c, projection), sub(a, projection)));
+      E.rp[2] = nrm(cross(sub(a, projection), sub(b, projection)));
+      const double sum = E.rp[0] + E.rp[1] + E.rp[2];
+      E.rp[0] /= sum;
+      E.rp[1] /= sum;
+      E.rp[2] /= sum;
+      return E.status;
+    }
+  }
+  E.status = kEpaFallBack;
+  E.normal = neg(guess);
+  const double nl = nrm(E.normal);
+  if (nl > 0)
+    E.normal = dvd(E.normal, nl);
+  else
+    E.normal = mk(1, 0, 0);
+  E.depth = 0;
+  E.rank = 1;
+  E.rc[0] = s.c[0];
+  E.rp[0] = 1;
+  return E.status;
+}
+
+// gjk_solver_indep-inl.h:110-165
+__device__ inline bool gjk_epa(const Shape& s1, const Xf& tf1, const Shape& s2, const Xf& tf2,
+                               double tol, bool want, Contact* out, int& n) {
+  const D3 guess = mk(1, 0, 0);
+  MDiff m;
+  m.s0 = s1;
+  m.s1 = s2;
+  mulTN(tf2.R, tf1.R, m.toshape1);
+  m.toshape0 = inv_compose(tf1, tf2);
+  GjkState g;
+  if (gjk_evaluate(g, m, neg(guess), tol) != kGjkInside) return false;
+  Epa E;
+  if (epa_evaluate(E, g, m, neg(guess), tol) == kEpaFailed) return false;
+  D3 w0 = mk(0, 0, 0);
+  for (int i = 0; i < E.rank; ++i) w0 = add(w0, mul(support0(m, g.vs[E.rc[i]].d), E.rp[i]));
+  if (want) {
+    out[0].normal = E.normal;
+    out[0].pos = apply(tf1, sub(w0, mul(E.normal, E.depth * 0.5)));
+    out[0].depth = -E.depth;
+    n = 1;
+  }
+  return true;
+}
+
+// -------------------------------------------------------- closed forms
+__device__ inline bool sphere_sphere(const Shape& s1, const Xf& tf1, const Shape& s2,
+                                     const Xf& tf2, bool want, Contact* out, int& n) {
+  const D3 diff = sub(tf2.t, tf1.t);
+  const double len = nrm(diff);
+  const double r1 = s1.dims[0], r2 = s2.dims[0];
+  if (len > r1 + r2) return false;
+  if (want) {
+    out[0].normal = len > 0 ? dvd(diff, len) : diff;
+    out[0].pos = add(tf1.t, dvd(mul(diff, r1), r1 + r2));
+    out[0].depth = r1 + r2 - len;
+    n = 1;
+  }
+  return true;
+}
+
+__device__ inline bool sphere_box(const Shape& sp, const Xf& X_FS, const Shape& bx,
+                                  const Xf& X_FB, bool want, Contact* out, int& n) {
+  const double r = sp.dims[0];
+  const Xf X_BS = inv_compose(X_FB, X_FS);
+  const D3 p_BC = X_BS.t;
+  D3 p_BN;
+  bool clamped = false;
+  for (int i = 0; i < 3; ++i) {
+    const double half = bx.dims[i] / 2;
+    p_BN.v[i] = p_BC.v[i];
+    if (p_BC.v[i] < -half) {
+      clamped = true;
+      p_BN.v[i] = -half;
+    }
+    if (p_BC.v[i] > half) {
+      clamped = true;
+      p_BN.v[i] = half;
+    }
+  }
+  const D3 p_CN = sub(p_BN, p_BC);
+  const double sq = sqn(p_CN);
+  if (sq > r * r) return false;
+  if (want) {
+    const double eps = 16 * 2.220446049250313e-16;
+    double depth;
+    D3 nv, p_BP;
+    if (clamped && sq > eps * eps) {
+      const double dist = sqrt(sq);
+      nv = dvd(p_CN, dist);
+      depth = r - dist;
+      p_BP = add(p_BN, mul(nv, depth * 0.5));
+    } else {
+      double min_distance = __builtin_inf();
+      int min_axis = -1;
+      for (int i = 0; i < 3; ++i) {
+        const double half = bx.dims[i] / 2;
+        const double dist = p_BC.v[i] >= 0 ? half - p_BC.v[i] : p_BC.v[i] + half;
+        if (dist + eps < min_distance) {
+          min_distance = dist;
+          min_axis = i;
+        }
+      }
+      nv = mk(0, 0, 0);
+      nv.v[min_axis] = p_BC.v[min_axis] >= 0 ? -1 : 1;
+      depth = min_distance + r;
+      p_BP = add(p_BC, mul(nv, (r - min_distance) / 2));
+    }
+    out[0].normal = rot(X_FB.R, nv);
+    out[0].pos = apply(X_FB, p_BP);
+    out[0].depth = depth;
+    n = 1;
+  }
+  return true;
+}
+
+__device__ inline bool sphere_capsule(const Shape& sp, const Xf& tf1, const Shape& cap,
+                                      const Xf& tf2, bool want, Contact* out, int& n) {
+  const D3 pos1 = mk(0., 0., 0.5 * cap.dims[1]);
+  const D3 pos2 = mk(0., 0., -0.5 * cap.dims[1]);
+  const D3 s_c = add(rotT(tf2.R, tf1.t), neg(rotT(tf2.R, tf2.t)));
+  const D3 v = sub(pos2, pos1), w = sub(s_c, pos1);
+  const double c1 = dot(w, v), c2 = dot(v, v);
+  D3 seg;
+  if (c1 <= 0)
+    seg = pos1;
+  else if (c2 <= c1)
+    seg = pos2;
+  else
+    seg = add(pos1, mul(v, c1 / c2));
+  const D3 diff = sub(s_c, seg);
+  const double distance = nrm(diff) - sp.dims[0] - cap.dims[0];
+  if (distance > 0) return false;
+  const D3 ln = neg(normalized(diff));
+  if (want) {
+    out[0].normal = rot(tf2.R, ln);
+    out[0].pos = apply(tf2, add(seg, mul(ln, distance)));
+    out[0].depth = -distance;
+    n = 1;
+  }
+  return true;
+}
+
+__device__ inline bool sphere_cylinder(const Shape& sp, const Xf& X_FS, const Shape& cy,
+                                       const Xf& X_FC, bool want, Contact* out, int& n) {
+  const double r_s = sp.dims[0], radius = cy.dims[0], height = cy.dims[1];
+  const Xf X_CS = inv_compose(X_FC, X_FS);
+  const D3 p_CS = X_CS.t;
+  D3 p_CN = p_CS;
+  bool outside = false;
+  const double half_h = height / 2;
+  if (p_CS.v[2] > half_h) {
+    outside = true;
+    p_CN.v[2] = half_h;
+  } else if (p_CS.v[2] < -half_h) {
+    outside = true;
+    p_CN.v[2] = -half_h;
+  }
+  double rx = p_CS.v[0], ry = p_CS.v[1];
+  const double sqd = rx * rx + ry * ry;
+  if (sqd > radius * radius) {
+    outside = true;
+    const double k = radius / sqrt(sqd);
+    rx *= k;
+    ry *= k;
+    p_CN.v[0] = rx;
+    p_CN.v[1] = ry;
+  }
+  const D3 p_SN = sub(p_CN, p_CS);
+  const double sq = sqn(p_SN);
+  if (sq > r_s * r_s) return false;
+  if (want) {
+    const double eps = 16 * 2.220446049250313e-16;
+    double depth;
+    D3 nv, p_CP;
+    if (outside && sq > eps * eps) {
+      const double d = sqrt(sq);
+      nv = dvd(p_SN, d);
+      depth = r_s - d;
+      p_CP = add(p_CN, mul(nv, depth * 0.5));
+    } else {
+      const double face = p_CS.v[2] >= 0 ? height / 2 - p_CS.v[2] : p_CS.v[2] + height / 2;
+      const double nx = p_CS.v[0], ny = p_CS.v[1];
+      const double d_xy = sqrt(nx * nx + ny * ny);
+      const double barrel = radius - d_xy;
+      if (barrel < face - eps) {
+        if (d_xy > eps) {
+          nv = mk(-nx / d_xy, -ny / d_xy, 0);
+          depth = r_s + barrel;
+          p_CP = add(p_CS, mul(nv, (r_s - barrel) / 2));
+        } else {
+          nv = mk(-1, 0, 0);
+          depth = r_s + radius;
+          p_CP = add(p_CS, mul(nv, (r_s - barrel) / 2));
+        }
+      } else {
+        nv = mk(0, 0, p_CS.v[2] >= 0 ? -1 : 1);
+        depth = face + r_s;
+        p_CP = add(p_CS, mul(nv, (r_s - face) / 2));
+      }
+    }
+    out[0].normal = rot(X_FC.R, nv);
+    out[0].pos = apply(X_FC, p_CP);
+    out[0].depth = depth;
+    n = 1;
+  }
+  return true;
+}
+
+__device__ inline void line_closest(const D3& pa, const D3& ua, const D3& pb, const D3& ub,
+                                    double& alpha, double& beta) {
+  const D3 p = sub(pb, pa);
+  const double uaub = dot(ua, ub);
+  const double q1 = dot(ua, p);
+  const double q2 = -dot(ub, p);
+  double d = 1 - uaub * uaub;
+  if (d <= static_cast<double>(0.0001f)) {
+    alpha = 0;
+    beta = 0;
+  } else {
+    d = 1 / d;
+    alpha = (q1 + uaub * q2) * d;
+    beta = (uaub * q1 + q2) * d;
+  }
+}
+
+// box_box-inl.h:110-170: clip the quad p against the rectangle [-h, h]
+__device__ inline int intersect_rect_quad(const double h[2], double p[8], double ret[16]) {
+  double buffer[16];
+  double src[16];
+  int nq = 4, nr = 0;
+  // q/r are ping-pong buffers: 0 = p (as src), 1 = ret, 2 = buffer
+  for (int i = 0; i < 8; ++i) src[i] = p[i];
+  double* q = src;
+  double* r = ret;
+  bool q_is_ret = false;
+  for (int dir = 0; dir <= 1; ++dir) {
+    for (int sign = -1; sign <= 1; sign += 2) {
+      double* pq = q;
+      double* pr = r;
+      nr = 0;
+      for (int i = nq; i > 0; --i) {
+        if (sign * pq[dir] < h[dir]) {
+          pr[0] = pq[0];
+          pr[1] = pq[1];
+          pr += 2;
+          nr++;
+          if (nr & 8) {
+            q = r;
+            q_is_ret = (r == ret);
+            goto done;
+          }
+        }
+        double* nextq = (i > 1) ? pq + 2 : q;
+        if ((sign * pq[dir] < h[dir]) ^ (sign * nextq[dir] < h[dir])) {
+          pr[1 - dir] = pq[1 - dir] + (nextq[1 - dir] - pq[1 - dir]) / (nextq[dir] - pq[dir]) *
+                                          (sign * h[dir] - pq[dir]);
+          pr[dir] = sign * h[dir];
+          pr += 2;
+          nr++;
+          if (nr & 8) {
+            q = r;
+            q_is_ret = (r == ret);
+            goto done;
+          }
+        }
+        pq += 2;
+      }
+      q = r;
+      q_is_ret = (r == ret);
+      r = q_is_ret ? buffer : ret;
+      nq = nr;
+    }
+  }
+done:
+  if (!q_is_ret)
+    for (int i = 0; i < 2 * nr; ++i) ret[i] = q[i];
+  return nr;
+}
+
+__device__ inline void cull_points(int n, const double p[], int m, int i0, int iret[]) {
+  double a, cx, cy, q;
+  switch (n) {
+    case 1:
+      cx = p[0];
+      cy = p[1];
+      break;
+    case 2:
+      cx = 0.5 * (p[0] + p[2]);
+      cy = 0.5 * (p[1] + p[3]);
+      break;
+    default:
+      a = 0;
+      cx = 0;
+      cy = 0;
+      for (int i = 0; i < n - 1; ++i) {
+        q = p[i * 2] * p[i * 2 + 3] - p[i * 2 + 2] * p[i * 2 + 1];
+        a += q;
+        cx += q * (p[i * 2] + p[i * 2 + 2]);
+        cy += q * (p[i * 2 + 1] + p[i * 2 + 3]);
+      }
+      q = p[n * 2 - 2] * p[1] - p[0] * p[n * 2 - 1];
+      if (fabs(a + q) > 2.220446049250313e-16)
+        a = 1 / (3 * (a + q));
+      else
+        a = 1e18f;
+      cx = a * (cx + q * (p[n * 2 - 2] + p[0]));
+      cy = a * (cy + q * (p[n * 2 - 1] + p[1]));
+  }
+  double A[8];
+  for (int i = 0; i < n; ++i) A[i] = atan2(p[i * 2 + 1] - cy, p[i * 2] - cx);
+  int avail[8];
+  for (int i = 0; i < n; ++i) avail[i] = 1;
+  avail[i0] = 0;
+  iret[0] = i0;
+  int k = 1;
+  const double pi = 3.14159265358979323846;
+  for (int j = 1; j < m; ++j) {
+    a = j * (2 * pi / m) + A[i0];
+    if (a > pi) a -= 2 * pi;
+    double maxdiff = 1e9, diff;
+    iret[k] = i0;
+    for (int i = 0; i < n; ++i) {
+      if (avail[i]) {
+        diff = fabs(A[i] - a);
+        if (diff > pi) diff = 2 * pi - diff;
+        if (diff < maxdiff) {
+          maxdiff = diff;
+          iret[k] = i;
+        }
+      }
+    }
+    avail[iret[k]] = 0;
+    ++k;
+  }
+}
+
+// box_box-inl.h:253-862 (boxBox2 in the world frame, up to 4 contacts)
+__device__ inline bool box_box(const Shape& b1, const Xf& tf1, const Shape& b2, const Xf& tf2,
+                               bool want, Contact* out, int& n) {
+  const double fudge = 1.05;
+  const double* R1 = tf1.R;
+  const double* R2 = tf2.R;
+  const D3 T1 = tf1.t, T2 = tf2.t;
+  D3 normalC = mk(0, 0, 0);
+  double s, s2, l;
+  int invert_normal, code;
+  const D3 p = sub(T2, T1);
+  const D3 pp = rotT(R1, p);
+  const D3 A = mul(mk(b1.dims[0], b1.dims[1], b1.dims[2]), 0.5);
+  const D3 B = mul(mk(b2.dims[0], b2.dims[1], b2.dims[2]), 0.5);
+  double R[9], Q[9];
+  mulTN(R1, R2, R);
+  for (int i = 0; i < 9; ++i) Q[i] = fabs(R[i]);
+  int best_col = -1;
+  bool normal_from_R1 = true;
+  double tmp;
+  s = -1.7976931348623157e308;
+  invert_normal = 0;
+  code = 0;
+  for (int k = 0; k < 3; ++k) {
+    tmp = pp.v[k];
+    s2 = fabs(tmp) - (dot(mk(Q[3 * k], Q[3 * k + 1], Q[3 * k + 2]), B) + A.v[k]);
+    if (s2 > 0) return false;
+    if (s2 > s) {
+      s = s2;
+      best_col = k;
+      normal_from_R1 = true;
+      invert_normal = tmp < 0;
+      code = 1 + k;
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    tmp = dot(col(R2, k), p);
+    s2 = fabs(tmp) - (dot(mk(Q[k], Q[3 + k], Q[6 + k]), A) + B.v[k]);
+    if (s2 > 0) return false;
+    if (s2 > s) {
+      s = s2;
+      best_col = k;
+      normal_from_R1 = false;
+      invert_normal = tmp < 0;
+      code = 4 + k;
+    }
+  }
+  const double eps = 2.220446049250313e-16;
+  {
+    const double ma = A.v[0] > A.v[1] ? (A.v[0] > A.v[2] ? A.v[0] : A.v[2])
+                                      : (A.v[1] > A.v[2] ? A.v[1] : A.v[2]);
+    const double mb = B.v[0] > B.v[1] ? (B.v[0] > B.v[2] ? B.v[0] : B.v[2])
+                                      : (B.v[1] > B.v[2] ? B.v[1] : B.v[2]);
+    double mm = ma > mb ? ma : mb;
+    if (mm < 1.0) mm = 1.0;
+    const double sf = mm * 10.0 * eps;
+    for (int i = 0; i < 9; ++i) Q[i] += sf;
+  }
+  const double* a = A.v;
+  const double* b = B.v;
+  // the nine edge-edge axes, in the reference's order
+  const double etmp[9] = {pp.v[2] * R[3] - pp.v[1] * R[6], pp.v[2] * R[4] - pp.v[1] * R[7],
+                          pp.v[2] * R[5] - pp.v[1] * R[8], pp.v[0] * R[6] - pp.v[2] * R[0],
+                          pp.v[0] * R[7] - pp.v[2] * R[1], pp.v[0] * R[8] - pp.v[2] * R[2],
+                          pp.v[1] * R[0] - pp.v[0] * R[3], pp.v[1] * R[1] - pp.v[0] * R[4],
+                          pp.v[1] * R[2] - pp.v[0] * R[5]};
+  const double erad[9] = {
+      ((a[1] * Q[6] + a[2] * Q[3]) + b[1] * Q[2]) + b[2] * Q[1],
+      ((a[1] * Q[7] + a[2] * Q[4]) + b[0] * Q[2]) + b[2] * Q[0],
+      ((a[1] * Q[8] + a[2] * Q[5]) + b[0] * Q[1]) + b[1] * Q[0],
+      ((a[0] * Q[6] + a[2] * Q[0]) + b[1] * Q[5]) + b[2] * Q[4],
+      ((a[0] * Q[7] + a[2] * Q[1]) + b[0] * Q[5]) + b[2] * Q[3],
+      ((a[0] * Q[8] + a[2] * Q[2]) + b[0] * Q[4]) + b[1] * Q[3],
+      ((a[0] * Q[3] + a[1] * Q[0]) + b[1] * Q[8]) + b[2] * Q[7],
+      ((a[0] * Q[4] + a[1] * Q[1]) + b[0] * Q[8]) + b[2] * Q[6],
+      ((a[0] * Q[5] + a[1] * Q[2]) + b[0] * Q[7]) + b[1] * Q[6]};
+  const D3 eax[9] = {mk(0, -R[6], R[3]), mk(0, -R[7], R[4]), mk(0, -R[8], R[5]),
+                     mk(R[6], 0, -R[0]), mk(R[7], 0, -R[1]), mk(R[8], 0, -R[2]),
+                     mk(-R[3], R[0], 0), mk(-R[4], R[1], 0), mk(-R[5], R[2], 0)};
+  for (int k = 0; k < 9; ++k) {
+    tmp = etmp[k];
+    s2 = fabs(tmp) - erad[k];
+    if (s2 > 0) return false;
+    const D3 nv = eax[k];
+    l = nrm(nv);
+    if (l > eps) {
+      s2 /= l;
+      if (s2 * fudge > s) {
+        s = s2;
+        best_col = -1;
+        normalC = dvd(nv, l);
+        invert_normal = tmp < 0;
+        code = 7 + k;
+      }
+    }
+  }
+  if (!code) return false;
+  D3 normal = best_col != -1 ? col(normal_from_R1 ? R1 : R2, best_col) : rot(R1, normalC);
+  if (invert_normal) normal = neg(normal);
+  const double depth = -s;
+  n = 0;
+  if (code > 6) {
+    D3 pa = T1;
+    for (int j = 0; j < 3; ++j) {
+      const double sign = dot(col(R1, j), normal) > 0 ? 1 : -1;
+      pa = add(pa, mul(col(R1, j), A.v[j] * sign));
+    }
+    D3 pb = T2;
+    for (int j = 0; j < 3; ++j) {
+      const double sign = dot(col(R2, j), normal) > 0 ? -1 : 1;
+      pb = add(pb, mul(col(R2, j), B.v[j] * sign));
+    }
+    double alpha, beta;
+    const D3 ua = col(R1, (code - 7) / 3), ub = col(R2, (code - 7) % 3);
+    line_closest(pa, ua, pb, ub, alpha, beta);
+    pa = add(pa, mul(ua, alpha));
+    pb = add(pb, mul(ub, beta));
+    if (want) {
+      out[0].normal = normal;
+      out[0].pos = mul(add(pa, pb), 0.5);
+      out[0].depth = depth;
+      n = 1;
+    }
+    return true;
+  }
+  const double* Ra = code <= 3 ? R1 : R2;
+  const double* Rb = code <= 3 ? R2 : R1;
+  const D3 Pa = code <= 3 ? T1 : T2, Pb = code <= 3 ? T2 : T1;
+  const D3 Sa = code <= 3 ? A : B, Sb = code <= 3 ? B : A;
+  const D3 normal2 = code <= 3 ? normal : neg(normal);
+  const D3 nr = rotT(Rb, normal2);
+  const D3 anr = mk(fabs(nr.v[0]), fabs(nr.v[1]), fabs(nr.v[2]));
+  int lanr, a1, a2;
+  if (anr.v[1] > anr.v[0]) {
+    if (anr.v[1] > anr.v[2]) {
+      a1 = 0;
+      lanr = 1;
+      a2 = 2;
+    } else {
+      a1 = 0;
+      a2 = 1;
+      lanr = 2;
+    }
+  } else {
+    if (anr.v[0] > anr.v[2]) {
+      lanr = 0;
+      a1 = 1;
+      a2 = 2;
+    } else {
+      a1 = 0;
+      a2 = 1;
+      lanr = 2;
+    }
+  }
+  D3 center;
+  if (nr.v[lanr] < 0)
+    center = add(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]));
+  else
+    center = sub(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]));
+  const int codeN = code <= 3 ? code - 1 : code - 4;
+  int code1, code2;
+  if (codeN == 0) {
+    code1 = 1;
+    code2 = 2;
+  } else if (codeN == 1) {
+    code1 = 0;
+    code2 = 2;
+  } else {
+    code1 = 0;
+    code2 = 1;
+  }
+  double quad[8];
+  const double c1 = dot(col(Ra, code1), center), c2 = dot(col(Ra, code2), center);
+  double m11 = dot(col(Rb, a1), col(Ra, code1)), m12 = dot(col(Rb, a2), col(Ra, code1));
+  double m21 = dot(col(Rb, a1), col(Ra, code2)), m22 = dot(col(Rb, a2), col(Ra, code2));
+  const double k1 = m11 * Sb.v[a1], k2 = m21 * Sb.v[a1], k3 = m12 * Sb.v[a2],
+               k4 = m22 * Sb.v[a2];
+  quad[0] = c1 - k1 - k3;
+  quad[1] = c2 - k2 - k4;
+  quad[2] = c1 - k1 + k3;
+  quad[3] = c2 - k2 + k4;
+  quad[4] = c1 + k1 + k3;
+  quad[5] = c2 + k2 + k4;
+  quad[6] = c1 + k1 - k3;
+  quad[7] = c2 + k2 - k4;
+  const double rect[2] = {Sa.v[code1], Sa.v[code2]};
+  double ret[16];
+  const int n_intersect = intersect_rect_quad(rect, quad, ret);
+  if (n_intersect < 1) return true;  // intersecting, no contact point
+  D3 points[8];
+  double dep[8];
+  const double det1 = 1.f / (m11 * m22 - m12 * m21);
+  m11 *= det1;
+  m12 *= det1;
+  m21 *= det1;
+  m22 *= det1;
+  int cnum = 0;
+  for (int j = 0; j < n_intersect; ++j) {
+    const double kk1 = m22 * (ret[j * 2] - c1) - m12 * (ret[j * 2 + 1] - c2);
+    const double kk2 = -m21 * (ret[j * 2] - c1) + m11 * (ret[j * 2 + 1] - c2);
+    points[cnum] = add(add(center, mul(col(Rb, a1), kk1)), mul(col(Rb, a2), kk2));
+    dep[cnum] = Sa.v[codeN] - dot(normal2, points[cnum]);
+    if (dep[cnum] >= 0) {
+      ret[cnum * 2] = ret[j * 2];
+      ret[cnum * 2 + 1] = ret[j * 2 + 1];
+      cnum++;
+    }
+  }
+  if (cnum < 1) return true;
+  int maxc = 4;
+  if (maxc > cnum) maxc = cnum;
+  if (maxc < 1) maxc = 1;
+  int iret[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  if (cnum > maxc) {
+    int i1 = 0;
+    double maxdepth = dep[0];
+    for (int i = 1; i < cnum; ++i)
+      if (dep[i] > maxdepth) {
+        maxdepth = dep[i];
+        i1 = i;
+      }
+    cull_points(cnum, ret, maxc, i1, iret);
+    cnum = maxc;
+  }
+  if (want) {
+    for (int j = 0; j < cnum; ++j) {
+      const int i = iret[j];
+      out[j].normal = normal;
+      out[j].pos = code < 4 ? add(add(points[i], Pa), mul(normal, dep[i] / 2))
+                            : sub(add(points[i], Pa), mul(normal, dep[i] / 2));
+      out[j].depth = dep[i];
+    }
+    n = cnum;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------ dispatch
+__device__ __forceinline__ void flip(Contact* c, int n) {
+  for (int i = 0; i < n; ++i) c[i].normal = neg(c[i].normal);
+}
+
+// GJKSolver_indep::shapeIntersect: up to 4 contacts (two boxes), else 1.
+__device__ inline bool shape_intersect(const Shape& s1, const Xf& a, const Shape& s2,
+                                       const Xf& b, double tol, bool want, Contact* out,
+                                       int& n) {
+  const int t1 = s1.type, t2 = s2.type;
+  n = 0;
+  bool r;
+  if (t1 == kSphere && t2 == kSphere)
+    r = sphere_sphere(s1, a, s2, b, want, out, n);
+  else if (t1 == kBox && t2 == kBox)
+    r = box_box(s1, a, s2, b, want, out, n);
+  else if (t1 == kSphere && t2 == kCapsule)
+    r = sphere_capsule(s1, a, s2, b, want, out, n);
+  else if (t1 == kCapsule && t2 == kSphere) {
+    r = sphere_capsule(s2, b, s1, a, want, out, n);
+    if (want) flip(out, n);
+  } else if (t1 == kSphere && t2 == kBox)
+    r = sphere_box(s1, a, s2, b, want, out, n);
+  else if (t1 == kBox && t2 == kSphere) {
+    r = sphere_box(s2, b, s1, a, want, out, n);
+    if (want) flip(out, n);
+  } else if (t1 == kSphere && t2 == kCylinder)
+    r = sphere_cylinder(s1, a, s2, b, want, out, n);
+  else if (t1 == kCylinder && t2 == kSphere) {
+    r = sphere_cylinder(s2, b, s1, a, want, out, n);
+    if (want) flip(out, n);
+  } else
+    r = gjk_epa(s1, a, s2, b, tol, want, out, n);
+  if (!want) n = 0;
+  return r;
+}
+
+// shape_collision_traversal_node-inl.h:69-134 with enable_cost off: the
+// contacts added for cap free slots (deepest first when they do not all
+// fit; one contact without geometry without enable_contact).
+__device__ inline int shape_collide(const Shape& s1, const Xf& a, const Shape& s2, const Xf& b,
+                                    int cap, bool enable_contact, double tol, Contact* out) {
+  if (cap <= 0) return 0;
+  if (enable_contact) {
+    Contact c[4];
+    int n = 0;
+    if (!shape_intersect(s1, a, s2, b, tol, true, c, n)) return 0;
+    int take = n;
+    if (cap < n) {
+      for (int i = 0; i < cap; ++i) {
+        int best = i;
+        for (int j = i + 1; j < n; ++j)
+          if (c[j].depth > c[best].depth) best = j;
+        const Contact t = c[i];
+        c[i] = c[best];
+        c[best] = t;
+      }
+      take = cap;
+    }
+    for (int i = 0; i < take; ++i) out[i] = c[i];
+    return take;
+  }
+  int n = 0;
+  if (!shape_intersect(s1, a, s2, b, tol, false, nullptr, n)) return 0;
+  out[0].normal = mk(0, 0, 0);
+  out[0].pos = mk(0, 0, 0);
+  out[0].depth = 0;
+  return 1;
+}
+
+}  // namespace shp
+}  // namespace fclx

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fclx.h"
 #include "fclx_layout.h"
 
 namespace fclx {
@@ -57,7 +58,28 @@ struct DistanceLaunch {
   int32_t* status;
 };
 
+// Primitive-shape pairs (records in the fclx_shape layout of fclx.h)
+struct ShapeLaunch {
+  int32_t num_pairs;
+  int32_t num_shapes;
+  const fclx_shape* shapes;    // device, num_shapes records
+  const double* vertices;      // device, convex vertex pool (3 per vertex)
+  long long num_vertices;
+  const int32_t* pair_shapes;  // 2 shape ids per pair
+  const double* tf1;           // 12 per pair
+  const double* tf2;
+  int32_t cap;                 // num_max_contacts
+  int32_t contact;             // enable_contact
+  double tol;                  // GJK and EPA tolerance
+  int32_t* counts;             // per pair
+  double* contact_pos;         // 3 per slot (slot = pair * cap + i), with contact
+  double* contact_normal;      // 3 per slot
+  double* contact_depth;       // 1 per slot
+  int32_t* status;
+};
+
 hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);
+hipError_t launch_shape_collide(const ShapeLaunch& L, hipStream_t s);
 hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s);
 
 }  // namespace fclx

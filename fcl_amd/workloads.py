@@ -13,10 +13,18 @@
   draw (normalised Exp(1), largest-remainder rounding); translation uniform
   in [-h, h]^3 with h = 1.1 * (R_A + R_B) = 2.2; exhaustive contacts
   (num_max_contacts = 100,000).
+- primitive_batch(): 4,000,000 primitive pairs, every shape drawn uniformly
+  from {Box, Sphere, Capsule, Cylinder, Convex}; box sides, radii and
+  lengths log-uniform in [0.05, 1]; a convex is 32 points from N(0, I)
+  scaled by a log-uniform factor in [0.05, 1] (its GJK support -- a scan of
+  the vertex list -- equals that of the points' hull, since points strictly
+  inside the hull never attain the maximum); both poses per pair with a
+  uniform random rotation (normalised Gaussian quaternion) and a translation
+  uniform in [-1, 1]^3; collide with enable_contact, GST_INDEP, FP64.
 """
 import numpy as np
 
-from . import heightfield, icosphere, poses, scale_to_radius
+from . import CONVEX, SHAPE_DTYPE, heightfield, icosphere, poses, scale_to_radius
 
 
 def synth_mesh(kind, n_tris, seed):
@@ -72,3 +80,52 @@ def uneven_batch(n_queries=1_000_000, n_pairs=64, seed=0x5EED, lo=500, hi=200_00
     tf1, tf2 = poses(n_queries, 0, 2.2, 0.0, int(rng.integers(1, 2**62)))
     return {"meshes": meshes, "pairs": pairs, "query_meshes": qm, "tf1": tf1, "tf2": tf2,
             "cap": 100000, "pair_counts": counts, "kinds": kinds, "tris_target": tris}
+
+
+def random_poses(n, rng, half_extent=1.0):
+    """(n, 12) poses: rotation from a normalised Gaussian quaternion
+    (row-major), translation uniform in [-half_extent, half_extent]^3."""
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    tf = np.empty((n, 12))
+    tf[:, 0] = 1 - 2 * (y * y + z * z)
+    tf[:, 1] = 2 * (x * y - w * z)
+    tf[:, 2] = 2 * (x * z + w * y)
+    tf[:, 3] = 2 * (x * y + w * z)
+    tf[:, 4] = 1 - 2 * (x * x + z * z)
+    tf[:, 5] = 2 * (y * z - w * x)
+    tf[:, 6] = 2 * (x * z - w * y)
+    tf[:, 7] = 2 * (y * z + w * x)
+    tf[:, 8] = 1 - 2 * (x * x + y * y)
+    tf[:, 9:] = rng.uniform(-half_extent, half_extent, (n, 3))
+    return tf
+
+
+def primitive_batch(n_pairs=4_000_000, seed=0x5A9E, lo=0.05, hi=1.0, convex_points=32):
+    """Returns dict with 'shapes' (2n fclx_shape records; pair q uses shapes
+    2q and 2q+1), 'vertices' (V,3) convex pool, 'pair_shapes' (n,2), 'tf1',
+    'tf2', 'cap' (1) and 'enable_contact' (True)."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    types = rng.integers(0, 5, n).astype(np.int32)
+    dims = np.exp(rng.uniform(np.log(lo), np.log(hi), (n, 3)))
+    rec = np.zeros(n, SHAPE_DTYPE)
+    rec["type"] = types
+    # unused dimensions are zero: sphere keeps the radius, capsule and
+    # cylinder (radius, lz), a convex none (its scale is baked into the points)
+    dims[types == 1, 1:] = 0
+    dims[(types == 2) | (types == 3), 2] = 0
+    conv = np.flatnonzero(types == CONVEX)
+    scale = dims[conv, 0].copy()
+    dims[conv] = 0
+    rec["dims"] = dims
+    rec["num_vertices"][conv] = convex_points
+    rec["vertex_offset"][conv] = np.arange(len(conv), dtype=np.int64) * convex_points
+    pts = rng.standard_normal((len(conv), convex_points, 3))
+    pts *= scale[:, None, None]
+    tf1 = random_poses(n_pairs, rng)
+    tf2 = random_poses(n_pairs, rng)
+    return {"shapes": rec, "vertices": pts.reshape(-1, 3), "tf1": tf1, "tf2": tf2,
+            "pair_shapes": np.arange(n, dtype=np.int32).reshape(n_pairs, 2), "cap": 1,
+            "enable_contact": True}

@@ -112,6 +112,50 @@ typedef struct {
   int64_t* leaf_tests;         /* device, optional */
 } fclx_distance_batch_args;
 
+/* Primitive shapes (geometry/shape/{box,sphere,capsule,cylinder,convex}.h),
+ * in their own local frames as the reference defines them: a box centred
+ * at the origin with the given side lengths, capsule and cylinder along z
+ * with total length lz (capsule: of its segment), a convex as its vertex
+ * list (the GJK support scans the vertices, minkowski_diff-inl.h:182-196,
+ * so faces are not needed). */
+#define FCLX_SHAPE_BOX 0
+#define FCLX_SHAPE_SPHERE 1
+#define FCLX_SHAPE_CAPSULE 2
+#define FCLX_SHAPE_CYLINDER 3
+#define FCLX_SHAPE_CONVEX 4
+
+typedef struct {
+  int32_t type;           /* FCLX_SHAPE_* */
+  int32_t num_vertices;   /* CONVEX: vertices in the pool */
+  double dims[3];         /* BOX: side x, y, z; SPHERE: radius; CAPSULE and
+                             CYLINDER: radius, lz */
+  int64_t vertex_offset;  /* CONVEX: index of its first vertex in the pool */
+} fclx_shape;
+
+typedef struct {
+  int32_t num_pairs;
+  int32_t num_shapes;
+  const fclx_shape* shapes;    /* device: num_shapes records */
+  const double* vertices;      /* device: convex vertex pool, 3 doubles each (NULL
+                                  when no shape is CONVEX) */
+  int64_t num_vertices;        /* vertices in the pool */
+  const int32_t* pair_shapes;  /* device: 2 shape ids per pair */
+  const double* tf1;           /* device: 12 per pair */
+  const double* tf2;           /* device: 12 per pair */
+  int32_t num_max_contacts;    /* CollisionRequest::num_max_contacts (>= 1) */
+  int32_t enable_contact;      /* CollisionRequest::enable_contact */
+  double gjk_tolerance;        /* CollisionRequest::gjk_tolerance (> 0; the
+                                  reference's default is 1e-6), used by GJK and
+                                  EPA alike (collision-inl.h:166-169) */
+  int32_t* counts;             /* device: contacts per pair */
+  /* device, required with enable_contact: slot = pair * num_max_contacts + i
+   * for i < counts[pair]; world-frame point (3), normal (3, pointing from
+   * the first shape to the second) and penetration depth (1) */
+  double* contact_pos;
+  double* contact_normal;
+  double* contact_depth;
+} fclx_shape_collide_batch_args;
+
 /* Context on one GPU (device ordinal).  No counterpart in the reference,
  * which has no device state. */
 int fclx_init(int32_t device, fclx_context** out);
@@ -147,6 +191,15 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* args,
                        void* stream);
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
                         void* stream);
+/* Batched fcl::collide between primitive shapes with the GST_INDEP solver:
+ * per pair what ShapeShapeCollide (collision_func_matrix-inl.h:81-112) and
+ * ShapeCollisionTraversalNode::leafTesting
+ * (shape_collision_traversal_node-inl.h:69-134, enable_cost off) give, via
+ * GJKSolver_indep::shapeIntersect (gjk_solver_indep-inl.h:110-305).
+ * Asynchronous; a pair naming a missing or malformed shape gets count 0 and
+ * makes fclx_sync return FCLX_ERR_INVALID_ARGUMENT. */
+int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_args* args,
+                             void* stream);
 /* Breakdown of the last FCLX_ENGINE_FAST collide batch with a cap above the
  * ordered limit (no counterpart in the reference, which only keeps
  * num_bv_tests/num_leaf_tests per traversal, bvh_collision_traversal_node.h:

@@ -133,6 +133,21 @@ def make_shape(kind, dims=(0, 0, 0), vertices=None):
     return s
 
 
+def shapes_from_table(rec, pool):
+    """Oracle shapes for fclx_shape records (fields type, num_vertices, dims,
+    vertex_offset) and their convex vertex pool (V,3)."""
+    pool = _d(pool).reshape(-1, 3)
+    out = []
+    for r in rec:
+        t = int(r["type"])
+        if t == CONVEX:
+            o = int(r["vertex_offset"])
+            out.append(make_shape(t, (0, 0, 0), pool[o:o + int(r["num_vertices"])]))
+        else:
+            out.append(make_shape(t, tuple(float(x) for x in r["dims"])))
+    return out
+
+
 def shape_intersect(s1, tf1, s2, tf2, tol=1e-6, contacts=True):
     """GJKSolver_indep::shapeIntersect: (hit, [(normal, pos, depth), ...])."""
     tf1 = _d(tf1).reshape(12)

@@ -37,7 +37,9 @@ def test_binding_structs_match_the_header(tmp_path):
     structs = {"fclx_mesh_info": N.MeshInfo,
                "fclx_collide_batch_args": N.CollideBatchArgs,
                "fclx_distance_batch_args": N.DistanceBatchArgs,
-               "fclx_engine_report": N.EngineReport}
+               "fclx_engine_report": N.EngineReport,
+               "fclx_shape": N.ShapeRec,
+               "fclx_shape_collide_batch_args": N.ShapeCollideBatchArgs}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "fclx.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
