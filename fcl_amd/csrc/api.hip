@@ -70,6 +70,10 @@ struct fclx_context {
   int32_t* s_ids = nullptr;      // 2
   EngineWorkspace* ws = nullptr;
   std::mutex ws_mu;       // the FAST engine's work buffers (one batch at a time)
+  ShapeWorkspace* sws = nullptr;
+  std::mutex sws_mu;      // the shape path's work buffers
+  int epa_blocks_per_cu = 0;
+  int shape_waves = 4;
 };
 
 namespace {
@@ -186,6 +190,7 @@ void fclx_shutdown(fclx_context* ctx) {
   for (auto& m : ctx->meshes)
     if (m.live) free_slot(m);
   engine_workspace_destroy(ctx->ws);
+  shape_workspace_destroy(ctx->sws);
   (void)hipFree(ctx->d_table);
   (void)hipFree(ctx->d_status);
   (void)hipFree(ctx->s_tf);
@@ -531,6 +536,10 @@ int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_a
        (!a->contact_pos || !a->contact_normal || !a->contact_depth)))
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid shape batch arguments");
   (void)hipSetDevice(ctx->device);
+  std::lock_guard<std::mutex> lock(ctx->sws_mu);
+  if (!ctx->sws) ctx->sws = shape_workspace_create(ctx->device);
+  shape_workspace_set_epa_blocks_per_cu(ctx->sws, ctx->epa_blocks_per_cu);
+  shape_workspace_set_waves(ctx->sws, ctx->shape_waves);
   ShapeLaunch L{};
   L.num_pairs = a->num_pairs;
   L.num_shapes = a->num_shapes;
@@ -548,7 +557,7 @@ int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_a
   L.contact_normal = a->contact_normal;
   L.contact_depth = a->contact_depth;
   L.status = ctx->d_status;
-  hipError_t e = launch_shape_collide(L, static_cast<hipStream_t>(stream));
+  hipError_t e = shape_collide_run(ctx->sws, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(ctx, e, "shape collide launch");
   return FCLX_OK;
 }
@@ -583,6 +592,12 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.stop_poll = static_cast<int>(value);
   } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
     t.dfs_waves_per_simd = static_cast<int>(value);
+  } else if (n == "epa_blocks_per_cu" && in(0, 32)) {
+    std::lock_guard<std::mutex> sl(ctx->sws_mu);
+    ctx->epa_blocks_per_cu = static_cast<int>(value);
+  } else if (n == "shape_waves_per_simd" && in(2, 4)) {
+    std::lock_guard<std::mutex> sl(ctx->sws_mu);
+    ctx->shape_waves = static_cast<int>(value);
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter or value out of range");
   }

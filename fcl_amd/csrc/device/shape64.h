@@ -293,9 +293,9 @@ struct Simplex {
   int rank;
 };
 
-// vertex store shared by GJK (0..3) and EPA (4..67), in private memory
+// GJK's four-vertex store and its two simplices (gjk.h:60-120)
 struct GjkState {
-  SVert vs[kNvGjk + kNvEpa];
+  SVert vs[kNvGjk];
   int free_v[4];
   int nfree;
   Simplex sx[2];
@@ -390,6 +390,17 @@ __device__ inline int gjk_evaluate(GjkState& g, const MDiff& m, const D3& guess,
   return status;
 }
 
+// gjk_solver_indep-inl.h:127-133: the Minkowski difference of the pair and
+// GJK from the guess (1, 0, 0) (a fresh solver per collide() call,
+// collision-inl.h:164-170, so no cached guess carries over)
+__device__ inline void make_mdiff(const Shape& s1, const Xf& tf1, const Shape& s2, const Xf& tf2,
+                                  MDiff& m) {
+  m.s0 = s1;
+  m.s1 = s2;
+  mulTN(tf2.R, tf1.R, m.toshape1);
+  m.toshape0 = inv_compose(tf1, tf2);
+}
+
 // encloseOrigin (gjk-inl.h:230-290), one function per rank: each tries the
 // reference's candidate directions in order and recurses one rank up.
 __device__ inline bool enclose4(GjkState& g, Simplex& s) {
@@ -397,7 +408,7 @@ __device__ inline bool enclose4(GjkState& g, Simplex& s) {
   return fabs(triple(sub(g.vs[s.c[0]].w, w3), sub(g.vs[s.c[1]].w, w3),
                      sub(g.vs[s.c[2]].w, w3))) > 0;
 }
-__device__ inline bool enclose3(GjkState& g, const MDiff& m, Simplex& s) {
+__device__ __noinline__ bool enclose3(GjkState& g, const MDiff& m, Simplex& s) {
   const D3 n = cross(sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w), sub(g.vs[s.c[2]].w, g.vs[s.c[0]].w));
   if (sqn(n) > 0) {
     sx_append(g, m, s, n);
@@ -409,7 +420,7 @@ __device__ inline bool enclose3(GjkState& g, const MDiff& m, Simplex& s) {
   }
   return false;
 }
-__device__ inline bool enclose2(GjkState& g, const MDiff& m, Simplex& s) {
+__device__ __noinline__ bool enclose2(GjkState& g, const MDiff& m, Simplex& s) {
   const D3 d = sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w);
   for (int i = 0; i < 3; ++i) {
     D3 axis = mk(0, 0, 0);
@@ -467,19 +478,45 @@ enum : int {
   kEpaFailed
 };
 
-struct Face {
+// One EPA face (epa.h SimplexF): normal, distance, vertices, neighbour
+// faces and the neighbour's edge index, and the pass stamp of the horizon
+// walk.  One cache line.
+struct alignas(64) EpaFace {
   D3 n;
   double d;
-  int8_t c[3];  // vertex indices
-  int8_t f[3];  // neighbour faces
-  int8_t e[3];  // neighbour edges
-  int8_t l[2];  // list links (-1: none)
-  int16_t pass;
+  int32_t pass;
+  int8_t c[3];
+  int8_t f[3];
+  int8_t e[3];
+};
+
+// EPA's working memory for one lane, a contiguous block in device memory.
+// The reference keeps the faces in two linked lists (hull and stock,
+// epa.h SimplexList); here:
+//  - the hull list is only ever prepended to (epa-inl.h:71-81), so its
+//    order from the root is the order of decreasing append time; stamp[f]
+//    records that time (-1: not on the hull), and the root is the hull face
+//    with the largest stamp;
+//  - the stock is a last-in-first-out store of free slots: the freed slots
+//    on a stack, then the never-used slots in index order, which is the
+//    order the reference's stock list hands them out;
+//  - d * d of every face sits in a dense array for the best-face scan.
+struct EpaMem {
+  double dsq[kNfEpa];
+  int32_t stamp[kNfEpa];
+  EpaFace fc[kNfEpa];
+  D3 w[kNvGjk + kNvEpa];    // support points (GJK's four first)
+  D3 dir[kNvGjk + kNvEpa];  // their search directions
+  int8_t freed[kNfEpa];     // stack of freed face slots
+  int16_t xs[kNfEpa];       // horizon walk stack: face | edge << 7 | stage << 9
 };
 
 struct Epa {
-  Face fc[kNfEpa];
-  int hull_root, hull_count, stock_root, stock_count;
+  EpaMem* M;
+  int nfreed;      // entries on the freed stack
+  int hi;          // slots handed out so far: [0, hi)
+  int hull_count;
+  int next_stamp;
   int nextsv;
   int status;
   D3 normal;
@@ -489,32 +526,25 @@ struct Epa {
   double rp[3];
 };
 
-__device__ __forceinline__ void list_append(Epa& E, bool hull, int f) {
-  int& root = hull ? E.hull_root : E.stock_root;
-  E.fc[f].l[0] = -1;
-  E.fc[f].l[1] = static_cast<int8_t>(root);
-  if (root >= 0) E.fc[root].l[0] = static_cast<int8_t>(f);
-  root = f;
-  ++(hull ? E.hull_count : E.stock_count);
+__device__ __forceinline__ void hull_add(Epa& E, int f) {
+  E.M->stamp[f] = E.next_stamp++;
+  ++E.hull_count;
 }
-__device__ __forceinline__ void list_remove(Epa& E, bool hull, int f) {
-  int& root = hull ? E.hull_root : E.stock_root;
-  const int l0 = E.fc[f].l[0], l1 = E.fc[f].l[1];
-  if (l1 >= 0) E.fc[l1].l[0] = static_cast<int8_t>(l0);
-  if (l0 >= 0) E.fc[l0].l[1] = static_cast<int8_t>(l1);
-  if (f == root) root = l1;
-  --(hull ? E.hull_count : E.stock_count);
+__device__ __forceinline__ void hull_drop(Epa& E, int f) {
+  E.M->stamp[f] = -1;
+  --E.hull_count;
 }
-__device__ __forceinline__ void bind(Epa& E, int fa, int ea, int fb, int eb) {
-  E.fc[fa].e[ea] = static_cast<int8_t>(eb);
-  E.fc[fa].f[ea] = static_cast<int8_t>(fb);
-  E.fc[fb].e[eb] = static_cast<int8_t>(ea);
-  E.fc[fb].f[eb] = static_cast<int8_t>(fa);
+__device__ __forceinline__ void stock_put(Epa& E, int f) { E.M->freed[E.nfreed++] = (int8_t)f; }
+__device__ __forceinline__ void bind(EpaMem& M, int fa, int ea, int fb, int eb) {
+  M.fc[fa].e[ea] = static_cast<int8_t>(eb);
+  M.fc[fa].f[ea] = static_cast<int8_t>(fb);
+  M.fc[fb].e[eb] = static_cast<int8_t>(ea);
+  M.fc[fb].f[eb] = static_cast<int8_t>(fa);
 }
 
 // epa-inl.h:109-141 getEdgeDist (with the reference's distance formula)
-__device__ inline bool edge_dist(const GjkState& g, const D3& n, int a, int b, double& dist) {
-  const D3 wa = g.vs[a].w, wb = g.vs[b].w;
+__device__ inline bool edge_dist(const EpaMem& M, const D3& n, int a, int b, double& dist) {
+  const D3 wa = M.w[a], wb = M.w[b];
   const D3 ba = sub(wb, wa);
   const D3 n_ab = cross(ba, n);
   if (dot(wa, n_ab) < 0) {
@@ -533,81 +563,99 @@ __device__ inline bool edge_dist(const GjkState& g, const D3& n, int a, int b, d
   return false;
 }
 
-__device__ inline int new_face(Epa& E, const GjkState& g, int a, int b, int c, bool forced,
-                               double tol) {
-  if (E.stock_root >= 0) {
-    const int f = E.stock_root;
-    list_remove(E, false, f);
-    list_append(E, true, f);
-    Face& F = E.fc[f];
-    F.pass = 0;
-    F.c[0] = static_cast<int8_t>(a);
-    F.c[1] = static_cast<int8_t>(b);
-    F.c[2] = static_cast<int8_t>(c);
-    const D3 wa = g.vs[a].w;
-    F.n = cross(sub(g.vs[b].w, wa), sub(g.vs[c].w, wa));
-    const double l = nrm(F.n);
-    if (l > tol) {
-      double d;
-      if (!(edge_dist(g, F.n, a, b, d) || edge_dist(g, F.n, b, c, d) ||
-            edge_dist(g, F.n, c, a, d)))
-        d = dot(wa, F.n) / l;
-      F.d = d;
-      F.n = dvd(F.n, l);
-      if (forced || F.d >= -tol) return f;
-      E.status = kEpaNonConvex;
-    } else {
-      E.status = kEpaDegenerated;
-    }
-    list_remove(E, true, f);
-    list_append(E, false, f);
+// epa-inl.h:145-189 newFace
+__device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double tol) {
+  EpaMem& M = *E.M;
+  int f;
+  if (E.nfreed > 0)
+    f = M.freed[--E.nfreed];
+  else if (E.hi < kNfEpa)
+    f = E.hi++;
+  else {
+    E.status = kEpaOutOfFaces;  // the reference's test of an empty stock
     return -1;
   }
-  E.status = E.stock_root >= 0 ? kEpaOutOfVertices : kEpaOutOfFaces;
+  hull_add(E, f);
+  EpaFace& F = M.fc[f];
+  F.pass = 0;
+  F.c[0] = static_cast<int8_t>(a);
+  F.c[1] = static_cast<int8_t>(b);
+  F.c[2] = static_cast<int8_t>(c);
+  const D3 wa = M.w[a];
+  const D3 n = cross(sub(M.w[b], wa), sub(M.w[c], wa));
+  const double l = nrm(n);
+  if (l > tol) {
+    double d;
+    if (!(edge_dist(M, n, a, b, d) || edge_dist(M, n, b, c, d) || edge_dist(M, n, c, a, d)))
+      d = dot(wa, n) / l;
+    F.d = d;
+    M.dsq[f] = d * d;
+    F.n = dvd(n, l);
+    if (forced || d >= -tol) return f;
+    E.status = kEpaNonConvex;
+  } else {
+    E.status = kEpaDegenerated;
+  }
+  hull_drop(E, f);
+  stock_put(E, f);
   return -1;
 }
 
+// epa-inl.h:193-209 findBest: the first face, in list order from the root,
+// whose d * d is below all before it -- the smallest d * d, ties to the
+// latest append; a NaN root is never displaced.
 __device__ inline int find_best(const Epa& E) {
-  int minf = E.hull_root;
-  double mind = E.fc[minf].d * E.fc[minf].d;
-  for (int f = E.fc[minf].l[1]; f >= 0; f = E.fc[f].l[1]) {
-    const double sqd = E.fc[f].d * E.fc[f].d;
-    if (sqd < mind) {
-      minf = f;
-      mind = sqd;
+  const EpaMem& M = *E.M;
+  int root = -1, root_stamp = -1;
+  double root_dsq = 0;
+  int best = -1, best_stamp = -1;
+  double best_dsq = 0;
+  for (int f = 0; f < E.hi; ++f) {
+    const int s = M.stamp[f];
+    if (s < 0) continue;
+    const double v = M.dsq[f];
+    if (s > root_stamp) {
+      root = f;
+      root_stamp = s;
+      root_dsq = v;
+    }
+    if (v == v && (best < 0 || v < best_dsq || (v == best_dsq && s > best_stamp))) {
+      best = f;
+      best_dsq = v;
+      best_stamp = s;
     }
   }
-  return minf;
+  return (root_dsq != root_dsq || best < 0) ? root : best;
 }
 
 struct Horizon {
   int cf, ff, nf;
 };
 
-// epa-inl.h:347-396 expand(pass, w, f, e, horizon), recursion unrolled onto a
-// stack of frames; a frame in stage 1 waits for its first child, stage 2 for
-// its second.  Same visit order and results as the recursive form.
-__device__ inline bool expand(Epa& E, const GjkState& g, int pass, int w, int f0, int e0,
-                              Horizon& h, double tol) {
+// epa-inl.h:347-396 expand(pass, w, f, e, horizon), recursion unrolled onto
+// a stack of frames; a frame in stage 1 waits for its first child, stage 2
+// for its second.  Same visit order and results as the recursive form.
+__device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& h,
+                              double tol) {
   const int nexti[3] = {1, 2, 0};
   const int previ[3] = {2, 0, 1};
-  int8_t sf[kNfEpa], se[kNfEpa], sst[kNfEpa];
+  EpaMem& M = *E.M;
   int sp = 0;
   int f = f0, e = e0;
   bool result = false;
   bool call = true;  // evaluate (f, e) as a fresh call
   while (true) {
     if (call) {
-      Face& F = E.fc[f];
+      EpaFace& F = M.fc[f];
       result = false;
       if (F.pass != pass) {
         const int e1 = nexti[e];
-        if (dot(F.n, g.vs[w].w) - F.d < -tol) {
-          const int nf = new_face(E, g, F.c[e1], F.c[e], w, false, tol);
+        if (dot(F.n, M.w[w]) - F.d < -tol) {
+          const int nf = new_face(E, F.c[e1], F.c[e], w, false, tol);
           if (nf >= 0) {
-            bind(E, nf, 0, f, e);
+            bind(M, nf, 0, f, e);
             if (h.cf >= 0)
-              bind(E, nf, 2, h.cf, 1);
+              bind(M, nf, 2, h.cf, 1);
             else
               h.ff = nf;
             h.cf = nf;
@@ -615,12 +663,10 @@ __device__ inline bool expand(Epa& E, const GjkState& g, int pass, int w, int f0
             result = true;
           }
         } else {
-          F.pass = static_cast<int16_t>(pass);
-          if (sp >= kNfEpa) return false;  // deeper than the face store: cannot happen
-          sf[sp] = static_cast<int8_t>(f);
-          se[sp] = static_cast<int8_t>(e);
-          sst[sp] = 1;
-          ++sp;
+          F.pass = pass;
+          // every frame marks a distinct face with this pass: at most kNfEpa
+          if (sp >= kNfEpa) return false;
+          M.xs[sp++] = static_cast<int16_t>(f | (e << 7) | (1 << 9));
           const int fe1 = F.f[e1], ee1 = F.e[e1];
           f = fe1;
           e = ee1;
@@ -631,50 +677,52 @@ __device__ inline bool expand(Epa& E, const GjkState& g, int pass, int w, int f0
     }
     // return `result` to the parent frame
     if (sp == 0) return result;
-    const int pf = sf[sp - 1], pe = se[sp - 1];
-    if (sst[sp - 1] == 1) {
+    const int top = M.xs[sp - 1];
+    const int pf = top & 127, pe = (top >> 7) & 3;
+    if (top >> 9 == 1) {
       if (!result) {
         --sp;  // parent returns false
-        result = false;
         continue;
       }
-      sst[sp - 1] = 2;
+      M.xs[sp - 1] = static_cast<int16_t>(pf | (pe << 7) | (2 << 9));
       const int e2 = previ[pe];
-      f = E.fc[pf].f[e2];
-      e = E.fc[pf].e[e2];
+      f = M.fc[pf].f[e2];
+      e = M.fc[pf].e[e2];
       call = true;
       continue;  // call the second child
     }
     // stage 2: both children done
     --sp;
     if (result) {
-      list_remove(E, true, pf);
-      list_append(E, false, pf);
+      hull_drop(E, pf);
+      stock_put(E, pf);
     }
   }
 }
 
+// epa-inl.h:225-343 evaluate.  GJK's vertex store is copied into the lane's
+// block after the enclosing step.
 __device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3& guess,
                                    double tol) {
-  E.hull_root = -1;
+  EpaMem& M = *E.M;
+  E.nfreed = 0;
+  E.hi = 0;
   E.hull_count = 0;
-  E.stock_root = -1;
-  E.stock_count = 0;
-  for (int i = 0; i < kNfEpa; ++i) list_append(E, false, kNfEpa - i - 1);
+  E.next_stamp = 0;
   E.status = kEpaFailed;
   E.normal = mk(0, 0, 0);
   E.depth = 0;
   E.nextsv = 0;
   Simplex& s = g.sx[g.current];
-  if (s.rank > 1 && enclose_origin(g, m, s)) {
-    while (E.hull_root >= 0) {
-      const int f = E.hull_root;
-      list_remove(E, true, f);
-      list_append(E, false, f);
-    }
+  const bool enclosed = s.rank > 1 && enclose_origin(g, m, s);
+  for (int i = 0; i < s.rank; ++i) {
+    M.w[s.c[i]] = g.vs[s.c[i]].w;
+    M.dir[s.c[i]] = g.vs[s.c[i]].d;
+  }
+  if (enclosed) {
     E.status = kEpaValid;
     E.nextsv = 0;
-    const D3 w0 = g.vs[s.c[0]].w, w1 = g.vs[s.c[1]].w, w2 = g.vs[s.c[2]].w, w3 = g.vs[s.c[3]].w;
+    const D3 w0 = M.w[s.c[0]], w1 = M.w[s.c[1]], w2 = M.w[s.c[2]], w3 = M.w[s.c[3]];
     if (dot(sub(w0, w3), cross(sub(w1, w3), sub(w2, w3))) < 0) {
       const int t = s.c[0];
       s.c[0] = s.c[1];
@@ -684,38 +732,42 @@ __device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3
       s.p[1] = tp;
     }
     int tet[4];
-    tet[0] = new_face(E, g, s.c[0], s.c[1], s.c[2], true, tol);
-    tet[1] = new_face(E, g, s.c[1], s.c[0], s.c[3], true, tol);
-    tet[2] = new_face(E, g, s.c[2], s.c[1], s.c[3], true, tol);
-    tet[3] = new_face(E, g, s.c[0], s.c[2], s.c[3], true, tol);
+    tet[0] = new_face(E, s.c[0], s.c[1], s.c[2], true, tol);
+    tet[1] = new_face(E, s.c[1], s.c[0], s.c[3], true, tol);
+    tet[2] = new_face(E, s.c[2], s.c[1], s.c[3], true, tol);
+    tet[3] = new_face(E, s.c[0], s.c[2], s.c[3], true, tol);
     if (E.hull_count == 4) {
       int best = find_best(E);
-      Face outer = E.fc[best];
+      EpaFace outer = M.fc[best];
       int pass = 0;
-      bind(E, tet[0], 0, tet[1], 0);
-      bind(E, tet[0], 1, tet[2], 0);
-      bind(E, tet[0], 2, tet[3], 0);
-      bind(E, tet[1], 1, tet[3], 2);
-      bind(E, tet[1], 2, tet[2], 1);
-      bind(E, tet[2], 2, tet[3], 1);
+      bind(M, tet[0], 0, tet[1], 0);
+      bind(M, tet[0], 1, tet[2], 0);
+      bind(M, tet[0], 2, tet[3], 0);
+      bind(M, tet[1], 1, tet[3], 2);
+      bind(M, tet[1], 2, tet[2], 1);
+      bind(M, tet[2], 2, tet[3], 1);
       E.status = kEpaValid;
       for (int it = 0; it < 255; ++it) {
         if (E.nextsv < kNvEpa) {
           Horizon h{-1, -1, 0};
           const int w = kNvGjk + E.nextsv++;
           bool valid = true;
-          E.fc[best].pass = static_cast<int16_t>(++pass);
-          get_sv(m, E.fc[best].n, g.vs[w]);
-          const double wdist = dot(E.fc[best].n, g.vs[w].w) - E.fc[best].d;
+          M.fc[best].pass = ++pass;
+          const D3 bn = M.fc[best].n;
+          const D3 wd = normalized(bn);
+          const D3 ww = msupport(m, wd);
+          M.dir[w] = wd;
+          M.w[w] = ww;
+          const double wdist = dot(bn, ww) - M.fc[best].d;
           if (wdist > tol) {
             for (int j = 0; j < 3 && valid; ++j)
-              valid &= expand(E, g, pass, w, E.fc[best].f[j], E.fc[best].e[j], h, tol);
+              valid &= expand(E, pass, w, M.fc[best].f[j], M.fc[best].e[j], h, tol);
             if (valid && h.nf >= 3) {
-              bind(E, h.ff, 2, h.cf, 1);
-              list_remove(E, true, best);
-              list_append(E, false, best);
+              bind(M, h.ff, 2, h.cf, 1);
+              hull_drop(E, best);
+              stock_put(E, best);
               best = find_best(E);
-              outer = E.fc[best];
+              outer = M.fc[best];
             } else {
               E.status = kEpaInvalidHull;
               break;
@@ -734,7 +786,7 @@ __device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3
       E.depth = outer.d;
       E.rank = 3;
       for (int k = 0; k < 3; ++k) E.rc[k] = outer.c[k];
-      const D3 a = g.vs[outer.c[0]].w, b = g.vs[outer.c[1]].w, c = g.vs[outer.c[2]].w;
+      const D3 a = M.w[outer.c[0]], b = M.w[outer.c[1]], c = M.w[outer.c[2]];
       E.rp[0] = nrm(cross(sub(b, projection), sub(c, projection)));
       E.rp[1] = nrm(cross(sub(c, projection), sub(a, projection)));
       E.rp[2] = nrm(cross(sub(a, projection), sub(b, projection)));
@@ -759,28 +811,23 @@ __device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3
   return E.status;
 }
 
-// gjk_solver_indep-inl.h:110-165
-__device__ inline bool gjk_epa(const Shape& s1, const Xf& tf1, const Shape& s2, const Xf& tf2,
-                               double tol, bool want, Contact* out, int& n) {
+// gjk_solver_indep-inl.h:136-160: the contact of a pair GJK found inside.
+// EPA's status is never Failed once GJK reports Inside (the enclosing path
+// ends in one of its loop states, every other path in FallBack), so the
+// pair always collides and EPA only supplies the contact.
+__device__ inline Contact epa_contact(EpaMem* mem, GjkState& g, const MDiff& m, const Xf& tf1,
+                                      double tol) {
   const D3 guess = mk(1, 0, 0);
-  MDiff m;
-  m.s0 = s1;
-  m.s1 = s2;
-  mulTN(tf2.R, tf1.R, m.toshape1);
-  m.toshape0 = inv_compose(tf1, tf2);
-  GjkState g;
-  if (gjk_evaluate(g, m, neg(guess), tol) != kGjkInside) return false;
   Epa E;
-  if (epa_evaluate(E, g, m, neg(guess), tol) == kEpaFailed) return false;
+  E.M = mem;
+  epa_evaluate(E, g, m, neg(guess), tol);
   D3 w0 = mk(0, 0, 0);
-  for (int i = 0; i < E.rank; ++i) w0 = add(w0, mul(support0(m, g.vs[E.rc[i]].d), E.rp[i]));
-  if (want) {
-    out[0].normal = E.normal;
-    out[0].pos = apply(tf1, sub(w0, mul(E.normal, E.depth * 0.5)));
-    out[0].depth = -E.depth;
-    n = 1;
-  }
-  return true;
+  for (int i = 0; i < E.rank; ++i) w0 = add(w0, mul(support0(m, mem->dir[E.rc[i]]), E.rp[i]));
+  Contact c;
+  c.normal = E.normal;
+  c.pos = apply(tf1, sub(w0, mul(E.normal, E.depth * 0.5)));
+  c.depth = -E.depth;
+  return c;
 }
 
 // -------------------------------------------------------- closed forms
@@ -1317,10 +1364,18 @@ __device__ __forceinline__ void flip(Contact* c, int n) {
   for (int i = 0; i < n; ++i) c[i].normal = neg(c[i].normal);
 }
 
-// GJKSolver_indep::shapeIntersect: up to 4 contacts (two boxes), else 1.
-__device__ inline bool shape_intersect(const Shape& s1, const Xf& a, const Shape& s2,
-                                       const Xf& b, double tol, bool want, Contact* out,
-                                       int& n) {
+// The pairs GJKSolver_indep::shapeIntersect answers in closed form
+// (gjk_solver_indep-inl.h:250-305): two spheres, two boxes, and a sphere
+// with a box, capsule or cylinder in either order.
+__device__ __forceinline__ bool is_closed_form(int t1, int t2) {
+  if (t1 == t2) return t1 == kSphere || t1 == kBox;
+  const int o = t1 == kSphere ? t2 : (t2 == kSphere ? t1 : -1);
+  return o == kBox || o == kCapsule || o == kCylinder;
+}
+
+// Closed-form shapeIntersect: up to 4 contacts (two boxes), else 1.
+__device__ inline bool closed_form(const Shape& s1, const Xf& a, const Shape& s2, const Xf& b,
+                                   bool want, Contact* out, int& n) {
   const int t1 = s1.type, t2 = s2.type;
   n = 0;
   bool r;
@@ -1340,46 +1395,36 @@ __device__ inline bool shape_intersect(const Shape& s1, const Xf& a, const Shape
     if (want) flip(out, n);
   } else if (t1 == kSphere && t2 == kCylinder)
     r = sphere_cylinder(s1, a, s2, b, want, out, n);
-  else if (t1 == kCylinder && t2 == kSphere) {
+  else {
     r = sphere_cylinder(s2, b, s1, a, want, out, n);
     if (want) flip(out, n);
-  } else
-    r = gjk_epa(s1, a, s2, b, tol, want, out, n);
+  }
   if (!want) n = 0;
   return r;
 }
 
-// shape_collision_traversal_node-inl.h:69-134 with enable_cost off: the
-// contacts added for cap free slots (deepest first when they do not all
-// fit; one contact without geometry without enable_contact).
-__device__ inline int shape_collide(const Shape& s1, const Xf& a, const Shape& s2, const Xf& b,
-                                    int cap, bool enable_contact, double tol, Contact* out) {
-  if (cap <= 0) return 0;
-  if (enable_contact) {
-    Contact c[4];
-    int n = 0;
-    if (!shape_intersect(s1, a, s2, b, tol, true, c, n)) return 0;
-    int take = n;
-    if (cap < n) {
-      for (int i = 0; i < cap; ++i) {
-        int best = i;
-        for (int j = i + 1; j < n; ++j)
-          if (c[j].depth > c[best].depth) best = j;
-        const Contact t = c[i];
-        c[i] = c[best];
-        c[best] = t;
-      }
-      take = cap;
-    }
-    for (int i = 0; i < take; ++i) out[i] = c[i];
-    return take;
+// shape_collision_traversal_node-inl.h:69-134 with enable_cost off, given
+// the pair's intersection result: the contacts added for cap free slots
+// (deepest first when they do not all fit; one contact without geometry
+// without enable_contact).
+__device__ inline int keep_contacts(bool hit, Contact* c, int n, int cap, bool enable_contact) {
+  if (!hit || cap <= 0) return 0;
+  if (!enable_contact) {
+    c[0].normal = mk(0, 0, 0);
+    c[0].pos = mk(0, 0, 0);
+    c[0].depth = 0;
+    return 1;
   }
-  int n = 0;
-  if (!shape_intersect(s1, a, s2, b, tol, false, nullptr, n)) return 0;
-  out[0].normal = mk(0, 0, 0);
-  out[0].pos = mk(0, 0, 0);
-  out[0].depth = 0;
-  return 1;
+  if (cap >= n) return n;
+  for (int i = 0; i < cap; ++i) {
+    int best = i;
+    for (int j = i + 1; j < n; ++j)
+      if (c[j].depth > c[best].depth) best = j;
+    const Contact t = c[i];
+    c[i] = c[best];
+    c[best] = t;
+  }
+  return cap;
 }
 
 }  // namespace shp

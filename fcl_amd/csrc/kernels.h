@@ -79,7 +79,16 @@ struct ShapeLaunch {
 };
 
 hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);
-hipError_t launch_shape_collide(const ShapeLaunch& L, hipStream_t s);
+// Sort buffers, the EPA queue and per-lane EPA memory of the shape path; one
+// per context, used by one batch at a time (the caller serialises calls;
+// batches on different streams are ordered through an event).
+struct ShapeWorkspace;
+ShapeWorkspace* shape_workspace_create(int device);
+void shape_workspace_destroy(ShapeWorkspace* w);
+void shape_workspace_set_epa_blocks_per_cu(ShapeWorkspace* w, int blocks);
+void shape_workspace_set_waves(ShapeWorkspace* w, int waves);
+size_t shape_epa_lane_bytes();
+hipError_t shape_collide_run(ShapeWorkspace* w, const ShapeLaunch& L, hipStream_t s);
 hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s);
 
 }  // namespace fclx

@@ -197,7 +197,9 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
  * (shape_collision_traversal_node-inl.h:69-134, enable_cost off) give, via
  * GJKSolver_indep::shapeIntersect (gjk_solver_indep-inl.h:110-305).
  * Asynchronous; a pair naming a missing or malformed shape gets count 0 and
- * makes fclx_sync return FCLX_ERR_INVALID_ARGUMENT. */
+ * makes fclx_sync return FCLX_ERR_INVALID_ARGUMENT.  The context's sort and
+ * EPA buffers serve one batch at a time: a batch on another stream waits
+ * on the previous one. */
 int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_args* args,
                              void* stream);
 /* Breakdown of the last FCLX_ENGINE_FAST collide batch with a cap above the
@@ -242,6 +244,11 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
  *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
+ *   "epa_blocks_per_cu" 0..32   resident blocks per CU of the shape path's EPA
+ *                               kernel (0: as many as fit); each lane holds
+ *                               a 13 KiB EPA block in device memory
+ *   "shape_waves_per_simd" 2..4 register budget of the shape path's GJK and
+ *                               EPA kernels
  * Results do not depend on them; smaller buffers mean more rounds. */
 int fclx_set_engine_param(fclx_context* ctx, const char* name, double value);
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
