@@ -73,7 +73,7 @@ struct fclx_context {
   ShapeWorkspace* sws = nullptr;
   std::mutex sws_mu;      // the shape path's work buffers
   int epa_blocks_per_cu = 0;
-  int shape_waves = 4;
+  int shape_waves = 2;
 };
 
 namespace {

@@ -148,15 +148,20 @@ __device__ inline D3 support(const Shape& s, const D3& d) {
   }
 }
 
+// The Minkowski difference of the pair in shape 0's frame
+// (minkowski_diff-inl.h:209-273): toshape1 = R2^T R1 turns a direction into
+// shape 1's frame, toshape0 = tf1^-1 tf2 maps shape 1's points back.
+// toshape0's rotation R1^T R2 is toshape1 transposed entry for entry (the
+// same products summed in the same order), so only toshape1 is kept.
 struct MDiff {
   Shape s0, s1;
-  double toshape1[9];  // R2^T R1
-  Xf toshape0;         // tf1^-1 tf2
+  double r[9];  // R2^T R1
+  D3 t;         // translation of tf1^-1 tf2
 };
 
 __device__ __forceinline__ D3 support0(const MDiff& m, const D3& d) { return support(m.s0, d); }
 __device__ __forceinline__ D3 support1(const MDiff& m, const D3& d) {
-  return apply(m.toshape0, support(m.s1, rot(m.toshape1, d)));
+  return add(rotT(m.r, support(m.s1, rot(m.r, d))), m.t);
 }
 __device__ __forceinline__ D3 msupport(const MDiff& m, const D3& d) {
   return sub(support0(m, d), support1(m, neg(d)));
@@ -208,6 +213,7 @@ __device__ inline Proj project_triangle_origin(const D3& a, const D3& b, const D
   const double l = sqn(n);
   if (l > 0) {
     double mindist = -1;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (dot(vt[i], cross(dl[i], n)) > 0) {
         const int j = nexti[i];
@@ -245,6 +251,7 @@ __device__ inline Proj project_tetra_origin(const D3& a, const D3& b, const D3& 
   const bool ng = (vl * dot(a, cross(sub(b, c), sub(a, b)))) <= 0;
   if (ng && fabs(vl) > 0) {
     double mindist = -1;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
       const int j = nexti[i];
       const double s = vl * dot(d, cross(dl[i], dl[j]));
@@ -283,106 +290,106 @@ constexpr int kNvGjk = 4;
 constexpr int kNvEpa = 64;
 constexpr int kNfEpa = 128;
 
-struct SVert {
-  D3 d, w;
-};
-
-struct Simplex {
-  int c[4];
-  double p[4];
+// GJK's state (gjk.h:60-120) by value: the current simplex's vertices in
+// order, with their search directions, and the ray.  The reference's
+// vertex store, free list and second simplex only decide which storage
+// slot holds a vertex, which changes no result; the barycentric weights it
+// stores are never read back.
+struct Gjk {
+  D3 w[4], d[4];
   int rank;
-};
-
-// GJK's four-vertex store and its two simplices (gjk.h:60-120)
-struct GjkState {
-  SVert vs[kNvGjk];
-  int free_v[4];
-  int nfree;
-  Simplex sx[2];
-  int current;
   D3 ray;
 };
 
-__device__ __forceinline__ void get_sv(const MDiff& m, const D3& d, SVert& sv) {
-  sv.d = normalized(d);
-  sv.w = msupport(m, sv.d);
-}
-__device__ __forceinline__ void sx_remove(GjkState& g, Simplex& s) {
-  g.free_v[g.nfree++] = s.c[--s.rank];
-}
-__device__ __forceinline__ void sx_append(GjkState& g, const MDiff& m, Simplex& s, const D3& v) {
-  s.p[s.rank] = 0;
-  s.c[s.rank] = g.free_v[--g.nfree];
-  get_sv(m, v, g.vs[s.c[s.rank++]]);
+// a[k] = v for a position k known only at run time, as a select per
+// component so that the array stays in registers
+__device__ __forceinline__ void put(D3 (&a)[4], int k, const D3& v) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bool hit = j == k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[j].v[c] = hit ? v.v[c] : a[j].v[c];
+  }
 }
 
-__device__ inline int gjk_evaluate(GjkState& g, const MDiff& m, const D3& guess, double tol) {
+// gjk-inl.h:323-335 appendVertex: direction and support point at the end
+__device__ __forceinline__ D3 gjk_append(Gjk& g, const MDiff& m, const D3& v) {
+  const D3 d = normalized(v);
+  const D3 w = msupport(m, d);
+  put(g.d, g.rank, d);
+  put(g.w, g.rank, w);
+  ++g.rank;
+  return w;
+}
+
+// gjk-inl.h:82-200 evaluate from the guess (1, 0, 0) (a fresh solver per
+// collide() call, collision-inl.h:164-170, so no cached guess carries over;
+// evaluate is called with -guess, gjk_solver_indep-inl.h:133)
+__device__ inline int gjk_evaluate(Gjk& g, const MDiff& m, double tol) {
   int iterations = 0;
   double alpha = 0;
   D3 lastw[4];
   int clastw = 0;
-  for (int i = 0; i < 4; ++i) g.free_v[i] = i;
-  g.nfree = 4;
-  g.current = 0;
   int status = kGjkValid;
-  g.sx[0].rank = 0;
-  g.ray = guess;
-  sx_append(g, m, g.sx[0], sqn(g.ray) > 0 ? neg(g.ray) : mk(1, 0, 0));
-  g.sx[0].p[0] = 1;
-  g.ray = g.vs[g.sx[0].c[0]].w;
+  g.rank = 0;
+  g.ray = mk(-1, 0, 0);
+  gjk_append(g, m, sqn(g.ray) > 0 ? neg(g.ray) : mk(1, 0, 0));
+  g.ray = g.w[0];
   lastw[0] = lastw[1] = lastw[2] = lastw[3] = g.ray;
   do {
-    const int next = 1 - g.current;
-    Simplex& cs = g.sx[g.current];
-    Simplex& ns = g.sx[next];
     const double rl = nrm(g.ray);
     if (rl < tol) {
       status = kGjkInside;
       break;
     }
-    sx_append(g, m, cs, neg(g.ray));
-    const D3 w = g.vs[cs.c[cs.rank - 1]].w;
+    const D3 w = gjk_append(g, m, neg(g.ray));
     bool found = false;
-    for (int i = 0; i < 4; ++i)
-      if (sqn(sub(w, lastw[i])) < tol) {
-        found = true;
-        break;
-      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) found |= sqn(sub(w, lastw[i])) < tol;
     if (found) {
-      sx_remove(g, cs);
+      --g.rank;
       break;
     }
-    lastw[clastw = (clastw + 1) & 3] = w;
+    clastw = (clastw + 1) & 3;
+    put(lastw, clastw, w);
     const double omega = dot(g.ray, w) / rl;
     alpha = (alpha < omega) ? omega : alpha;  // std::max
     if ((rl - alpha) - tol * rl <= 0) {
-      sx_remove(g, cs);
+      --g.rank;
       break;
     }
     Proj pr = proj_init();
-    if (cs.rank == 2)
-      pr = project_line_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w);
-    else if (cs.rank == 3)
-      pr = project_triangle_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w, g.vs[cs.c[2]].w);
-    else if (cs.rank == 4)
-      pr = project_tetra_origin(g.vs[cs.c[0]].w, g.vs[cs.c[1]].w, g.vs[cs.c[2]].w,
-                                g.vs[cs.c[3]].w);
+    if (g.rank == 2)
+      pr = project_line_origin(g.w[0], g.w[1]);
+    else if (g.rank == 3)
+      pr = project_triangle_origin(g.w[0], g.w[1], g.w[2]);
+    else if (g.rank == 4)
+      pr = project_tetra_origin(g.w[0], g.w[1], g.w[2], g.w[3]);
     if (pr.sqr >= 0) {
-      ns.rank = 0;
-      g.ray = mk(0, 0, 0);
-      g.current = next;
-      for (int i = 0; i < cs.rank; ++i) {
-        if (pr.encode & (1 << i)) {
-          ns.c[ns.rank] = cs.c[i];
-          ns.p[ns.rank++] = pr.p[i];
-          g.ray = add(g.ray, mul(g.vs[cs.c[i]].w, pr.p[i]));
-        } else {
-          g.free_v[g.nfree++] = cs.c[i];
+      // the next simplex: the vertices the projection kept, in order
+      D3 nw[4] = {g.w[0], g.w[1], g.w[2], g.w[3]};
+      D3 nd[4] = {g.d[0], g.d[1], g.d[2], g.d[3]};
+      int nr = 0;
+      D3 ray = mk(0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i < g.rank && (pr.encode & (1 << i))) {
+          put(nw, nr, g.w[i]);
+          put(nd, nr, g.d[i]);
+          ++nr;
+          ray = add(ray, mul(g.w[i], pr.p[i]));
         }
       }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        g.w[i] = nw[i];
+        g.d[i] = nd[i];
+      }
+      g.rank = nr;
+      g.ray = ray;
       if (pr.encode == 15) status = kGjkInside;
     } else {
-      sx_remove(g, cs);
+      --g.rank;
       break;
     }
     status = (++iterations < 128) ? status : kGjkFailed;
@@ -390,78 +397,94 @@ __device__ inline int gjk_evaluate(GjkState& g, const MDiff& m, const D3& guess,
   return status;
 }
 
-// gjk_solver_indep-inl.h:127-133: the Minkowski difference of the pair and
-// GJK from the guess (1, 0, 0) (a fresh solver per collide() call,
-// collision-inl.h:164-170, so no cached guess carries over)
+// gjk_solver_indep-inl.h:127-130: the pair's Minkowski difference
 __device__ inline void make_mdiff(const Shape& s1, const Xf& tf1, const Shape& s2, const Xf& tf2,
                                   MDiff& m) {
   m.s0 = s1;
   m.s1 = s2;
-  mulTN(tf2.R, tf1.R, m.toshape1);
-  m.toshape0 = inv_compose(tf1, tf2);
+  mulTN(tf2.R, tf1.R, m.r);
+  m.t = add(rotT(tf1.R, tf2.t), neg(rotT(tf1.R, tf1.t)));
 }
 
-// encloseOrigin (gjk-inl.h:230-290), one function per rank: each tries the
-// reference's candidate directions in order and recurses one rank up.
-__device__ inline bool enclose4(GjkState& g, Simplex& s) {
-  const D3 w3 = g.vs[s.c[3]].w;
-  return fabs(triple(sub(g.vs[s.c[0]].w, w3), sub(g.vs[s.c[1]].w, w3),
-                     sub(g.vs[s.c[2]].w, w3))) > 0;
+// encloseOrigin (gjk-inl.h:230-290) as a loop: a depth-first search over
+// the vertex appended at positions rank .. 3, trying the reference's
+// candidate directions in its order (rank 1: +-x, +-y, +-z; rank 2:
+// +-(w1 - w0) x axis for the axes where that is non-zero; rank 3: +-n of
+// the triangle if non-zero) and accepting the first tetrahedron of non-zero
+// volume.  On failure the simplex is back at its starting rank.
+__device__ __forceinline__ void set_vertex(Gjk& g, const MDiff& m, int k, const D3& v) {
+  const D3 d = normalized(v);
+  put(g.d, k, d);
+  put(g.w, k, msupport(m, d));
 }
-__device__ __noinline__ bool enclose3(GjkState& g, const MDiff& m, Simplex& s) {
-  const D3 n = cross(sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w), sub(g.vs[s.c[2]].w, g.vs[s.c[0]].w));
-  if (sqn(n) > 0) {
-    sx_append(g, m, s, n);
-    if (enclose4(g, s)) return true;
-    sx_remove(g, s);
-    sx_append(g, m, s, neg(n));
-    if (enclose4(g, s)) return true;
-    sx_remove(g, s);
-  }
-  return false;
+__device__ __forceinline__ bool enclose4(const Gjk& g) {
+  const D3 w3 = g.w[3];
+  return fabs(triple(sub(g.w[0], w3), sub(g.w[1], w3), sub(g.w[2], w3))) > 0;
 }
-__device__ __noinline__ bool enclose2(GjkState& g, const MDiff& m, Simplex& s) {
-  const D3 d = sub(g.vs[s.c[1]].w, g.vs[s.c[0]].w);
-  for (int i = 0; i < 3; ++i) {
-    D3 axis = mk(0, 0, 0);
-    axis.v[i] = 1;
-    const D3 p = cross(d, axis);
-    if (sqn(p) > 0) {
-      sx_append(g, m, s, p);
-      if (enclose3(g, m, s)) return true;
-      sx_remove(g, s);
-      sx_append(g, m, s, neg(p));
-      if (enclose3(g, m, s)) return true;
-      sx_remove(g, s);
+__device__ __forceinline__ D3 unit_axis(int i) {
+  return mk(i == 0 ? 1.0 : 0.0, i == 1 ? 1.0 : 0.0, i == 2 ? 1.0 : 0.0);
+}
+__device__ inline bool enclose_origin(Gjk& g, const MDiff& m) {
+  const int r0 = g.rank;
+  if (r0 == 4) return enclose4(g);
+  if (r0 < 1) return false;
+  int t1 = 0, t2 = 0, t3 = 0;  // candidates tried at positions 1, 2, 3
+  int k = r0;                  // position being filled
+  while (true) {
+    bool have = false;
+    D3 dir = mk(0, 0, 0);
+    if (k == 1) {
+      if (t1 < 6) {
+        const D3 ax = unit_axis(t1 >> 1);
+        dir = (t1 & 1) ? neg(ax) : ax;
+        have = true;
+      }
+    } else if (k == 2) {
+      while (t2 < 6) {
+        const D3 p = cross(sub(g.w[1], g.w[0]), unit_axis(t2 >> 1));
+        if (sqn(p) > 0) {
+          dir = (t2 & 1) ? neg(p) : p;
+          have = true;
+          break;
+        }
+        t2 += 2;  // neither sign of a zero candidate is tried
+      }
+    } else if (t3 < 2) {
+      const D3 n = cross(sub(g.w[1], g.w[0]), sub(g.w[2], g.w[0]));
+      if (sqn(n) > 0) {
+        dir = t3 ? neg(n) : n;
+        have = true;
+      } else {
+        t3 = 2;
+      }
     }
+    if (!have) {  // position k exhausted: the vertex at k - 1 goes
+      if (k == r0) {
+        g.rank = r0;
+        return false;
+      }
+      --k;
+      if (k == 1)
+        ++t1;
+      else
+        ++t2;
+      continue;
+    }
+    set_vertex(g, m, k, dir);
+    if (k == 3) {
+      if (enclose4(g)) {
+        g.rank = 4;
+        return true;
+      }
+      ++t3;
+      continue;
+    }
+    ++k;
+    if (k == 2)
+      t2 = 0;
+    else
+      t3 = 0;
   }
-  return false;
-}
-__device__ inline bool enclose1(GjkState& g, const MDiff& m, Simplex& s) {
-  for (int i = 0; i < 3; ++i) {
-    D3 axis = mk(0, 0, 0);
-    axis.v[i] = 1;
-    sx_append(g, m, s, axis);
-    if (enclose2(g, m, s)) return true;
-    sx_remove(g, s);
-    sx_append(g, m, s, neg(axis));
-    if (enclose2(g, m, s)) return true;
-    sx_remove(g, s);
-  }
-  return false;
-}
-__device__ inline bool enclose_origin(GjkState& g, const MDiff& m, Simplex& s) {
-  switch (s.rank) {
-    case 1:
-      return enclose1(g, m, s);
-    case 2:
-      return enclose2(g, m, s);
-    case 3:
-      return enclose3(g, m, s);
-    case 4:
-      return enclose4(g, s);
-  }
-  return false;
 }
 
 // ---------------------------------------------------------------- EPA
@@ -490,52 +513,57 @@ struct alignas(64) EpaFace {
   int8_t e[3];
 };
 
-// EPA's working memory for one lane, a contiguous block in device memory.
-// The reference keeps the faces in two linked lists (hull and stock,
+// The reference keeps EPA's faces in two linked lists (hull and stock,
 // epa.h SimplexList); here:
 //  - the hull list is only ever prepended to (epa-inl.h:71-81), so its
-//    order from the root is the order of decreasing append time; stamp[f]
-//    records that time (-1: not on the hull), and the root is the hull face
-//    with the largest stamp;
+//    order from the root is the order of decreasing append time; a stamp
+//    per face records that time (-1: not on the hull), and the root is the
+//    hull face with the largest stamp;
 //  - the stock is a last-in-first-out store of free slots: the freed slots
 //    on a stack, then the never-used slots in index order, which is the
 //    order the reference's stock list hands them out;
-//  - d * d of every face sits in a dense array for the best-face scan.
-struct EpaMem {
-  double dsq[kNfEpa];
-  int32_t stamp[kNfEpa];
+//  - d * d and the stamp of every face sit in arrays of their own, scanned
+//    by every lane of a wavefront in step.
+// Per-lane memory: one contiguous block (a face or vertex is one line).
+struct EpaLane {
   EpaFace fc[kNfEpa];
-  D3 w[kNvGjk + kNvEpa];    // support points (GJK's four first)
+  D3 w[kNvGjk + kNvEpa];    // support points (GJK's simplex first)
   D3 dir[kNvGjk + kNvEpa];  // their search directions
   int8_t freed[kNfEpa];     // stack of freed face slots
   int16_t xs[kNfEpa];       // horizon walk stack: face | edge << 7 | stage << 9
 };
+// Per-wavefront memory, lane-interleaved so that the scan's loads coalesce.
+struct EpaWave {
+  double dsq[kNfEpa][64];
+  int32_t stamp[kNfEpa][64];
+};
 
 struct Epa {
-  EpaMem* M;
+  EpaLane* M;
+  EpaWave* S;
+  int lane;
   int nfreed;      // entries on the freed stack
   int hi;          // slots handed out so far: [0, hi)
   int hull_count;
   int next_stamp;
   int nextsv;
   int status;
-  D3 normal;
-  double depth;
-  int rank;
-  int rc[3];
-  double rp[3];
+  int best;        // current best face (the reference's outer is its copy)
+  int pass;
+  int it;          // loop passes done
+  bool fallback;   // no tetrahedron: the FallBack result
 };
 
 __device__ __forceinline__ void hull_add(Epa& E, int f) {
-  E.M->stamp[f] = E.next_stamp++;
+  E.S->stamp[f][E.lane] = E.next_stamp++;
   ++E.hull_count;
 }
 __device__ __forceinline__ void hull_drop(Epa& E, int f) {
-  E.M->stamp[f] = -1;
+  E.S->stamp[f][E.lane] = -1;
   --E.hull_count;
 }
 __device__ __forceinline__ void stock_put(Epa& E, int f) { E.M->freed[E.nfreed++] = (int8_t)f; }
-__device__ __forceinline__ void bind(EpaMem& M, int fa, int ea, int fb, int eb) {
+__device__ __forceinline__ void bind(EpaLane& M, int fa, int ea, int fb, int eb) {
   M.fc[fa].e[ea] = static_cast<int8_t>(eb);
   M.fc[fa].f[ea] = static_cast<int8_t>(fb);
   M.fc[fb].e[eb] = static_cast<int8_t>(ea);
@@ -543,7 +571,7 @@ __device__ __forceinline__ void bind(EpaMem& M, int fa, int ea, int fb, int eb) 
 }
 
 // epa-inl.h:109-141 getEdgeDist (with the reference's distance formula)
-__device__ inline bool edge_dist(const EpaMem& M, const D3& n, int a, int b, double& dist) {
+__device__ inline bool edge_dist(const EpaLane& M, const D3& n, int a, int b, double& dist) {
   const D3 wa = M.w[a], wb = M.w[b];
   const D3 ba = sub(wb, wa);
   const D3 n_ab = cross(ba, n);
@@ -565,7 +593,7 @@ __device__ inline bool edge_dist(const EpaMem& M, const D3& n, int a, int b, dou
 
 // epa-inl.h:145-189 newFace
 __device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double tol) {
-  EpaMem& M = *E.M;
+  EpaLane& M = *E.M;
   int f;
   if (E.nfreed > 0)
     f = M.freed[--E.nfreed];
@@ -589,7 +617,7 @@ __device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double 
     if (!(edge_dist(M, n, a, b, d) || edge_dist(M, n, b, c, d) || edge_dist(M, n, c, a, d)))
       d = dot(wa, n) / l;
     F.d = d;
-    M.dsq[f] = d * d;
+    E.S->dsq[f][E.lane] = d * d;
     F.n = dvd(n, l);
     if (forced || d >= -tol) return f;
     E.status = kEpaNonConvex;
@@ -605,15 +633,15 @@ __device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double 
 // whose d * d is below all before it -- the smallest d * d, ties to the
 // latest append; a NaN root is never displaced.
 __device__ inline int find_best(const Epa& E) {
-  const EpaMem& M = *E.M;
+  const EpaWave& S = *E.S;
   int root = -1, root_stamp = -1;
   double root_dsq = 0;
   int best = -1, best_stamp = -1;
   double best_dsq = 0;
   for (int f = 0; f < E.hi; ++f) {
-    const int s = M.stamp[f];
+    const int s = S.stamp[f][E.lane];
     if (s < 0) continue;
-    const double v = M.dsq[f];
+    const double v = S.dsq[f][E.lane];
     if (s > root_stamp) {
       root = f;
       root_stamp = s;
@@ -635,11 +663,13 @@ struct Horizon {
 // epa-inl.h:347-396 expand(pass, w, f, e, horizon), recursion unrolled onto
 // a stack of frames; a frame in stage 1 waits for its first child, stage 2
 // for its second.  Same visit order and results as the recursive form.
+__device__ __forceinline__ int next3(int e) { return e == 2 ? 0 : e + 1; }
+__device__ __forceinline__ int prev3(int e) { return e == 0 ? 2 : e - 1; }
+
 __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& h,
                               double tol) {
-  const int nexti[3] = {1, 2, 0};
-  const int previ[3] = {2, 0, 1};
-  EpaMem& M = *E.M;
+  EpaLane& M = *E.M;
+  const D3 ww = M.w[w];
   int sp = 0;
   int f = f0, e = e0;
   bool result = false;
@@ -649,8 +679,8 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
       EpaFace& F = M.fc[f];
       result = false;
       if (F.pass != pass) {
-        const int e1 = nexti[e];
-        if (dot(F.n, M.w[w]) - F.d < -tol) {
+        const int e1 = next3(e);
+        if (dot(F.n, ww) - F.d < -tol) {
           const int nf = new_face(E, F.c[e1], F.c[e], w, false, tol);
           if (nf >= 0) {
             bind(M, nf, 0, f, e);
@@ -685,7 +715,7 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
         continue;
       }
       M.xs[sp - 1] = static_cast<int16_t>(pf | (pe << 7) | (2 << 9));
-      const int e2 = previ[pe];
+      const int e2 = prev3(pe);
       f = M.fc[pf].f[e2];
       e = M.fc[pf].e[e2];
       call = true;
@@ -700,46 +730,44 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
   }
 }
 
-// epa-inl.h:225-343 evaluate.  GJK's vertex store is copied into the lane's
-// block after the enclosing step.
-__device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3& guess,
-                                   double tol) {
-  EpaMem& M = *E.M;
+// epa-inl.h:225-343 evaluate, in three parts so that a lane can run one
+// loop pass at a time: epa_start (enclosing step and the tetrahedron; false
+// when the FallBack result applies), epa_step (one pass of the loop at
+// epa-inl.h:283-321; false once the loop has ended) and epa_finish (the
+// result of epa-inl.h:323-343 and its contact, gjk_solver_indep-inl.h:
+// 141-157).  The GJK simplex's vertices take slots 0-3 after the enclosing
+// step; c names them in the reference's order.
+__device__ inline bool epa_start(Epa& E, Gjk& g, const MDiff& m, double tol) {
+  EpaLane& M = *E.M;
   E.nfreed = 0;
   E.hi = 0;
   E.hull_count = 0;
   E.next_stamp = 0;
   E.status = kEpaFailed;
-  E.normal = mk(0, 0, 0);
-  E.depth = 0;
   E.nextsv = 0;
-  Simplex& s = g.sx[g.current];
-  const bool enclosed = s.rank > 1 && enclose_origin(g, m, s);
-  for (int i = 0; i < s.rank; ++i) {
-    M.w[s.c[i]] = g.vs[s.c[i]].w;
-    M.dir[s.c[i]] = g.vs[s.c[i]].d;
-  }
+  E.fallback = true;
+  const bool enclosed = g.rank > 1 && enclose_origin(g, m);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < g.rank) {
+      M.w[i] = g.w[i];
+      M.dir[i] = g.d[i];
+    }
   if (enclosed) {
     E.status = kEpaValid;
-    E.nextsv = 0;
-    const D3 w0 = M.w[s.c[0]], w1 = M.w[s.c[1]], w2 = M.w[s.c[2]], w3 = M.w[s.c[3]];
-    if (dot(sub(w0, w3), cross(sub(w1, w3), sub(w2, w3))) < 0) {
-      const int t = s.c[0];
-      s.c[0] = s.c[1];
-      s.c[1] = t;
-      const double tp = s.p[0];
-      s.p[0] = s.p[1];
-      s.p[1] = tp;
+    int c[4] = {0, 1, 2, 3};
+    if (dot(sub(g.w[0], g.w[3]), cross(sub(g.w[1], g.w[3]), sub(g.w[2], g.w[3]))) < 0) {
+      c[0] = 1;
+      c[1] = 0;
     }
     int tet[4];
-    tet[0] = new_face(E, s.c[0], s.c[1], s.c[2], true, tol);
-    tet[1] = new_face(E, s.c[1], s.c[0], s.c[3], true, tol);
-    tet[2] = new_face(E, s.c[2], s.c[1], s.c[3], true, tol);
-    tet[3] = new_face(E, s.c[0], s.c[2], s.c[3], true, tol);
+    tet[0] = new_face(E, c[0], c[1], c[2], true, tol);
+    tet[1] = new_face(E, c[1], c[0], c[3], true, tol);
+    tet[2] = new_face(E, c[2], c[1], c[3], true, tol);
+    tet[3] = new_face(E, c[0], c[2], c[3], true, tol);
     if (E.hull_count == 4) {
-      int best = find_best(E);
-      EpaFace outer = M.fc[best];
-      int pass = 0;
+      E.best = find_best(E);
+      E.pass = 0;
       bind(M, tet[0], 0, tet[1], 0);
       bind(M, tet[0], 1, tet[2], 0);
       bind(M, tet[0], 2, tet[3], 0);
@@ -747,88 +775,101 @@ __device__ inline int epa_evaluate(Epa& E, GjkState& g, const MDiff& m, const D3
       bind(M, tet[1], 2, tet[2], 1);
       bind(M, tet[2], 2, tet[3], 1);
       E.status = kEpaValid;
-      for (int it = 0; it < 255; ++it) {
-        if (E.nextsv < kNvEpa) {
-          Horizon h{-1, -1, 0};
-          const int w = kNvGjk + E.nextsv++;
-          bool valid = true;
-          M.fc[best].pass = ++pass;
-          const D3 bn = M.fc[best].n;
-          const D3 wd = normalized(bn);
-          const D3 ww = msupport(m, wd);
-          M.dir[w] = wd;
-          M.w[w] = ww;
-          const double wdist = dot(bn, ww) - M.fc[best].d;
-          if (wdist > tol) {
-            for (int j = 0; j < 3 && valid; ++j)
-              valid &= expand(E, pass, w, M.fc[best].f[j], M.fc[best].e[j], h, tol);
-            if (valid && h.nf >= 3) {
-              bind(M, h.ff, 2, h.cf, 1);
-              hull_drop(E, best);
-              stock_put(E, best);
-              best = find_best(E);
-              outer = M.fc[best];
-            } else {
-              E.status = kEpaInvalidHull;
-              break;
-            }
-          } else {
-            E.status = kEpaAccuracyReached;
-            break;
-          }
-        } else {
-          E.status = kEpaOutOfVertices;
-          break;
-        }
-      }
-      const D3 projection = mul(outer.n, outer.d);
-      E.normal = outer.n;
-      E.depth = outer.d;
-      E.rank = 3;
-      for (int k = 0; k < 3; ++k) E.rc[k] = outer.c[k];
-      const D3 a = M.w[outer.c[0]], b = M.w[outer.c[1]], c = M.w[outer.c[2]];
-      E.rp[0] = nrm(cross(sub(b, projection), sub(c, projection)));
-      E.rp[1] = nrm(cross(sub(c, projection), sub(a, projection)));
-      E.rp[2] = nrm(cross(sub(a, projection), sub(b, projection)));
-      const double sum = E.rp[0] + E.rp[1] + E.rp[2];
-      E.rp[0] /= sum;
-      E.rp[1] /= sum;
-      E.rp[2] /= sum;
-      return E.status;
+      E.it = 0;
+      E.fallback = false;
+      return true;
     }
   }
   E.status = kEpaFallBack;
-  E.normal = neg(guess);
-  const double nl = nrm(E.normal);
-  if (nl > 0)
-    E.normal = dvd(E.normal, nl);
-  else
-    E.normal = mk(1, 0, 0);
-  E.depth = 0;
-  E.rank = 1;
-  E.rc[0] = s.c[0];
-  E.rp[0] = 1;
-  return E.status;
+  return false;
 }
 
-// gjk_solver_indep-inl.h:136-160: the contact of a pair GJK found inside.
-// EPA's status is never Failed once GJK reports Inside (the enclosing path
-// ends in one of its loop states, every other path in FallBack), so the
-// pair always collides and EPA only supplies the contact.
-__device__ inline Contact epa_contact(EpaMem* mem, GjkState& g, const MDiff& m, const Xf& tf1,
-                                      double tol) {
-  const D3 guess = mk(1, 0, 0);
-  Epa E;
-  E.M = mem;
-  epa_evaluate(E, g, m, neg(guess), tol);
+__device__ inline bool epa_step(Epa& E, const MDiff& m, double tol) {
+  if (E.it >= 255) return false;
+  ++E.it;
+  EpaLane& M = *E.M;
+  if (E.nextsv >= kNvEpa) {
+    E.status = kEpaOutOfVertices;
+    return false;
+  }
+  Horizon h{-1, -1, 0};
+  const int w = kNvGjk + E.nextsv++;
+  bool valid = true;
+  const int best = E.best;
+  M.fc[best].pass = ++E.pass;
+  const D3 bn = M.fc[best].n;
+  const D3 wd = normalized(bn);
+  const D3 ww = msupport(m, wd);
+  M.dir[w] = wd;
+  M.w[w] = ww;
+  const double wdist = dot(bn, ww) - M.fc[best].d;
+  if (!(wdist > tol)) {
+    E.status = kEpaAccuracyReached;
+    return false;
+  }
+  for (int j = 0; j < 3 && valid; ++j)
+    valid &= expand(E, E.pass, w, M.fc[best].f[j], M.fc[best].e[j], h, tol);
+  if (!(valid && h.nf >= 3)) {
+    E.status = kEpaInvalidHull;
+    return false;
+  }
+  bind(M, h.ff, 2, h.cf, 1);
+  hull_drop(E, best);
+  stock_put(E, best);
+  E.best = find_best(E);
+  return true;
+}
+
+// The contact of a pair GJK found inside.  EPA's status is never Failed
+// once GJK reports Inside (the enclosing path ends in one of its loop
+// states, every other path in FallBack), so the pair always collides and
+// EPA only supplies the contact.
+__device__ inline Contact epa_finish(const Epa& E, const MDiff& m, const Xf& tf1) {
+  const EpaLane& M = *E.M;
+  D3 normal;
+  double depth;
+  int rank;
+  int rc[3] = {0, 0, 0};
+  double rp[3] = {1, 0, 0};
+  if (E.fallback) {
+    // -guess with guess = -(1, 0, 0) (evaluate is called with -guess),
+    // normalised
+    normal = mk(1, 0, 0);
+    depth = 0;
+    rank = 1;
+  } else {
+    const EpaFace& outer = M.fc[E.best];
+    const D3 projection = mul(outer.n, outer.d);
+    normal = outer.n;
+    depth = outer.d;
+    rank = 3;
+    for (int k = 0; k < 3; ++k) rc[k] = outer.c[k];
+    const D3 a = M.w[rc[0]], b = M.w[rc[1]], c2 = M.w[rc[2]];
+    rp[0] = nrm(cross(sub(b, projection), sub(c2, projection)));
+    rp[1] = nrm(cross(sub(c2, projection), sub(a, projection)));
+    rp[2] = nrm(cross(sub(a, projection), sub(b, projection)));
+    const double sum = rp[0] + rp[1] + rp[2];
+    rp[0] /= sum;
+    rp[1] /= sum;
+    rp[2] /= sum;
+  }
   D3 w0 = mk(0, 0, 0);
-  for (int i = 0; i < E.rank; ++i) w0 = add(w0, mul(support0(m, mem->dir[E.rc[i]]), E.rp[i]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < rank) w0 = add(w0, mul(support0(m, M.dir[rc[i]]), rp[i]));
   Contact c;
-  c.normal = E.normal;
-  c.pos = apply(tf1, sub(w0, mul(E.normal, E.depth * 0.5)));
-  c.depth = -E.depth;
+  c.normal = normal;
+  c.pos = apply(tf1, sub(w0, mul(normal, depth * 0.5)));
+  c.depth = -depth;
   return c;
 }
+
+// GJK's final simplex of a pair queued for EPA
+struct EpaSeed {
+  D3 w[4], d[4];
+  int32_t q;
+  int32_t rank;
+};
 
 // -------------------------------------------------------- closed forms
 __device__ inline bool sphere_sphere(const Shape& s1, const Xf& tf1, const Shape& s2,

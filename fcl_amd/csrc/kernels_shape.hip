@@ -13,10 +13,14 @@
 //      GJK finds inside is queued for EPA when contacts are wanted (without
 //      them it collides, and EPA -- whose status is never Failed after GJK's
 //      Inside -- would only have supplied the contact);
-//   3. EPA over the queue: a persistent grid whose wavefronts take 64 queued
-//      pairs at a time; each lane re-runs the pair's GJK (deterministic)
-//      and keeps EPA's faces and vertices in a contiguous block of device
-//      memory of its own.
+//   3. EPA over the queue (GJK's final simplex per pair): a persistent grid
+//      in which every lane runs its own pair one EPA loop pass at a time and
+//      takes the next pair of its wavefront's share when done, so lanes do
+//      not wait for the slowest pair of their wavefront; a lane keeps EPA's
+//      faces and vertices in a contiguous block of device memory of its own.
+// Same-address atomics are kept few: one per block per counter in the
+// sorting and GJK kernels (the GJK kernel's on per-XCD counters), one per 64
+// pairs in EPA.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,8 +46,8 @@ enum : int {
   kCtrStart = kCtrCursor + 32,   // per bucket: first position
   kCtrClosed = kCtrStart + 32,   // closed-form pairs (positions [0, closed))
   kCtrValid,                     // well-formed pairs (positions [0, valid))
-  kCtrQueue,                     // pairs queued for EPA
-  kCtrGrab,                      // EPA queue entries taken
+  kCtrQueue,                     // pairs queued for EPA, per XCD sub-queue (8)
+  kCtrGrab = kCtrQueue + 8,      // EPA queue entries handed to wavefronts
   kNumCtr
 };
 
@@ -205,12 +209,21 @@ __global__ void __launch_bounds__(kBlk) shape_closed(ShapeLaunch L, const int32_
   write_contacts(L, q, c, shp::keep_contacts(hit, c, n, L.cap, L.contact != 0));
 }
 
+// The EPA queue is split into 8 sub-queues, one per XCD (blocks are
+// dispatched to XCDs round-robin by index): a block reserves its entries
+// with one atomic on its XCD's counter.  Sub-queue x holds the pairs of the
+// blocks with index = x (mod 8), so sub_cap = ceil(grid / 8) * kBlk bounds
+// it.
 template <int kWaves>
 __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const int32_t* perm,
-                                                          int32_t* ctr, int32_t* queue) {
+                                                          int32_t* ctr, shp::EpaSeed* seeds,
+                                                          int sub_cap) {
+  __shared__ int wave_count[kBlk / 64];
+  __shared__ int block_base;
   const int i = ctr[kCtrClosed] + blockIdx.x * kBlk + threadIdx.x;
   bool push = false;
   int q = -1;
+  shp::Gjk g;
   if (i < ctr[kCtrValid]) {
     q = perm[i];
     shp::Shape s1, s2;
@@ -220,9 +233,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const i
     const shp::Xf b = shp::xf_load(L.tf2 + 12 * static_cast<size_t>(q));
     shp::MDiff m;
     shp::make_mdiff(s1, a, s2, b, m);
-    shp::GjkState g;
-    const bool inside = shp::gjk_evaluate(g, m, shp::neg(shp::mk(1, 0, 0)), L.tol) ==
-                        shp::kGjkInside;
+    const bool inside = shp::gjk_evaluate(g, m, L.tol) == shp::kGjkInside;
     if (!inside) {
       L.counts[q] = 0;
     } else if (!L.contact) {
@@ -231,43 +242,116 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const i
       push = true;
     }
   }
+  if (!L.contact) return;  // uniform over the grid
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long mask = __ballot(push);
-  if (mask) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[kCtrQueue], __popcll(mask));
-    base = __shfl(base, leader);
-    if (push) queue[base + __popcll(mask & ((1ull << lane) - 1))] = q;
+  if (lane == 0) wave_count[wave] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int k = 0; k < kBlk / 64; ++k) total += wave_count[k];
+    block_base = total ? atomicAdd(&ctr[kCtrQueue + (blockIdx.x & 7)], total) : 0;
+  }
+  __syncthreads();
+  if (push) {
+    int off = block_base;
+    for (int k = 0; k < wave; ++k) off += wave_count[k];
+    off += __popcll(mask & ((1ull << lane) - 1));
+    shp::EpaSeed& sd = seeds[static_cast<size_t>(blockIdx.x & 7) * sub_cap + off];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sd.w[k] = g.w[k];
+      sd.d[k] = g.d[k];
+    }
+    sd.q = q;
+    sd.rank = g.rank;
   }
 }
 
+// Idle lanes of a wavefront take new pairs once this many are idle (or all).
+constexpr int kRefill = 16;
+
+// EPA over the queued pairs (the sub-queues, concatenated).  A wavefront
+// takes 64 entries at a time with one atomic; every lane runs its own pair
+// one loop pass per trip and takes the wavefront's next entry when done, so
+// lanes do not wait for the slowest pair of their wavefront.
 template <int kWaves>
-__global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L, const int32_t* queue,
-                                                          int32_t* ctr, shp::EpaMem* mem) {
+__global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L,
+                                                          const shp::EpaSeed* seeds,
+                                                          int32_t* ctr, int sub_cap,
+                                                          shp::EpaLane* lanes,
+                                                          shp::EpaWave* waves) {
   const int lane = threadIdx.x & 63;
-  shp::EpaMem* my = mem + static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
-  const int total = ctr[kCtrQueue];
+  const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
+  int sub_end[8];  // running totals of the sub-queue sizes
+  int total = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    total += ctr[kCtrQueue + x];
+    sub_end[x] = total;
+  }
+  const int wid = static_cast<int>(gid / 64);
+  int cursor = 0, stop = 0;  // the wavefront's entries not yet started
+  bool exhausted = false;
+  shp::Epa E;
+  E.M = lanes + gid;
+  E.S = waves + wid;
+  E.lane = lane;
+  shp::MDiff m;
+  int q = -1;
+  bool active = false;
   while (true) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&ctr[kCtrGrab], 64);
-    base = __shfl(base, 0);
-    if (base >= total) break;
-    const int i = base + lane;
-    if (i < total) {
-      const int q = queue[i];
-      shp::Shape s1, s2;
-      load_shape(L, L.pair_shapes[2 * q], s1);
-      load_shape(L, L.pair_shapes[2 * q + 1], s2);
-      const shp::Xf a = shp::xf_load(L.tf1 + 12 * static_cast<size_t>(q));
-      const shp::Xf b = shp::xf_load(L.tf2 + 12 * static_cast<size_t>(q));
-      shp::MDiff m;
-      shp::make_mdiff(s1, a, s2, b, m);
-      shp::GjkState g;
-      shp::gjk_evaluate(g, m, shp::neg(shp::mk(1, 0, 0)), L.tol);
-      const shp::Contact c = shp::epa_contact(my, g, m, a, L.tol);
+    bool finish = false, started = false;
+    const unsigned long long idle = __ballot(!active);
+    const int nidle = __popcll(idle);
+    if (cursor >= stop && !exhausted && (nidle == 64 || nidle >= kRefill)) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&ctr[kCtrGrab], 64);
+      base = __shfl(base, 0);
+      cursor = base < total ? base : total;
+      stop = base + 64 < total ? base + 64 : total;
+      exhausted = base + 64 >= total;
+    }
+    if (cursor < stop && (nidle == 64 || nidle >= kRefill)) {
+      const int v = cursor + __popcll(idle & ((1ull << lane) - 1));
+      cursor = cursor + nidle < stop ? cursor + nidle : stop;
+      if (!active && v < stop) {
+        int x = 0, before = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+          if (v >= sub_end[k]) {
+            x = k + 1;
+            before = sub_end[k];
+          }
+        const shp::EpaSeed& sd = seeds[static_cast<size_t>(x) * sub_cap + (v - before)];
+        q = sd.q;
+        shp::Shape s1, s2;
+        load_shape(L, L.pair_shapes[2 * q], s1);
+        load_shape(L, L.pair_shapes[2 * q + 1], s2);
+        shp::make_mdiff(s1, shp::xf_load(L.tf1 + 12 * static_cast<size_t>(q)), s2,
+                        shp::xf_load(L.tf2 + 12 * static_cast<size_t>(q)), m);
+        shp::Gjk g;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          g.w[k] = sd.w[k];
+          g.d[k] = sd.d[k];
+        }
+        g.rank = sd.rank;
+        active = shp::epa_start(E, g, m, L.tol);
+        finish = !active;
+        started = true;
+      }
+    }
+    if (active && !started) {
+      active = shp::epa_step(E, m, L.tol);
+      finish = !active;
+    }
+    if (finish) {
+      const shp::Contact c =
+          shp::epa_finish(E, m, shp::xf_load(L.tf1 + 12 * static_cast<size_t>(q)));
       write_contacts(L, q, &c, 1);
     }
+    if (exhausted && cursor >= stop && !__ballot(active)) break;
   }
 }
 
@@ -277,13 +361,15 @@ struct ShapeWorkspace {
   int device = 0;
   long long cap = 0;
   int32_t* perm = nullptr;
-  int32_t* queue = nullptr;
   uint8_t* keys = nullptr;
+  shp::EpaSeed* seeds = nullptr;  // EPA queue (allocated with enable_contact)
+  long long seed_cap = 0;
   int32_t* ctr = nullptr;
-  shp::EpaMem* epa = nullptr;
+  shp::EpaLane* epa = nullptr;
+  shp::EpaWave* epa_waves = nullptr;
   int epa_blocks = 0;
   int epa_blocks_per_cu = 0;  // 0: as many as are resident
-  int waves = 4;              // register budget of the GJK and EPA kernels (waves per SIMD)
+  int waves = 2;              // register budget of the GJK and EPA kernels (waves per SIMD)
   hipEvent_t done = nullptr;
   bool pending = false;       // done marks the last batch's end
 };
@@ -296,11 +382,13 @@ ShapeWorkspace* shape_workspace_create(int device) {
 
 static void release_buffers(ShapeWorkspace* w) {
   (void)hipFree(w->perm);
-  (void)hipFree(w->queue);
   (void)hipFree(w->keys);
-  w->perm = w->queue = nullptr;
+  (void)hipFree(w->seeds);
+  w->perm = nullptr;
   w->keys = nullptr;
+  w->seeds = nullptr;
   w->cap = 0;
+  w->seed_cap = 0;
 }
 
 void shape_workspace_destroy(ShapeWorkspace* w) {
@@ -309,6 +397,7 @@ void shape_workspace_destroy(ShapeWorkspace* w) {
   release_buffers(w);
   (void)hipFree(w->ctr);
   (void)hipFree(w->epa);
+  (void)hipFree(w->epa_waves);
   if (w->done) (void)hipEventDestroy(w->done);
   delete w;
 }
@@ -324,34 +413,39 @@ namespace {
 // GJK and EPA kernels for a register budget of 2, 3 or 4 waves per SIMD
 struct ShapeKernels {
   const void* epa;
-  void (*launch_gjk)(int, hipStream_t, const ShapeLaunch&, const int32_t*, int32_t*, int32_t*);
-  void (*launch_epa)(int, hipStream_t, const ShapeLaunch&, const int32_t*, int32_t*,
-                     shp::EpaMem*);
+  void (*launch_gjk)(int, hipStream_t, const ShapeLaunch&, const int32_t*, int32_t*,
+                     shp::EpaSeed*, int);
+  void (*launch_epa)(int, hipStream_t, const ShapeLaunch&, const shp::EpaSeed*, int32_t*,
+                     int, shp::EpaLane*, shp::EpaWave*);
 };
 
 template <int W>
 ShapeKernels kernels_for() {
   return {reinterpret_cast<const void*>(&shape_epa<W>),
           [](int grid, hipStream_t s, const ShapeLaunch& L, const int32_t* perm, int32_t* ctr,
-             int32_t* queue) { shape_gjk<W><<<grid, kBlk, 0, s>>>(L, perm, ctr, queue); },
-          [](int grid, hipStream_t s, const ShapeLaunch& L, const int32_t* queue, int32_t* ctr,
-             shp::EpaMem* mem) { shape_epa<W><<<grid, kBlk, 0, s>>>(L, queue, ctr, mem); }};
+             shp::EpaSeed* seeds, int sub_cap) {
+            shape_gjk<W><<<grid, kBlk, 0, s>>>(L, perm, ctr, seeds, sub_cap);
+          },
+          [](int grid, hipStream_t s, const ShapeLaunch& L, const shp::EpaSeed* seeds,
+             int32_t* ctr, int sub_cap, shp::EpaLane* lanes, shp::EpaWave* waves) {
+            shape_epa<W><<<grid, kBlk, 0, s>>>(L, seeds, ctr, sub_cap, lanes, waves);
+          }};
 }
 
 ShapeKernels select_kernels(int waves) {
   switch (waves) {
-    case 2:
-      return kernels_for<2>();
     case 3:
       return kernels_for<3>();
-    default:
+    case 4:
       return kernels_for<4>();
+    default:
+      return kernels_for<2>();
   }
 }
 
 }  // namespace
 
-size_t shape_epa_lane_bytes() { return sizeof(shp::EpaMem); }
+size_t shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::EpaWave) / 64; }
 
 hipError_t shape_collide_run(ShapeWorkspace* w, const ShapeLaunch& L, hipStream_t s) {
   if (L.num_pairs <= 0) return hipSuccess;
@@ -371,44 +465,51 @@ hipError_t shape_collide_run(ShapeWorkspace* w, const ShapeLaunch& L, hipStream_
   resident = std::max(1, resident);
   if (w->epa_blocks_per_cu > 0) resident = std::min(resident, w->epa_blocks_per_cu);
   const int epa_blocks = cus * resident;
-  const bool grow = L.num_pairs > w->cap;
+  const int n = L.num_pairs;
+  const int grid = (n + kBlk - 1) / kBlk;
+  const int sub_cap = (grid + 7) / 8 * kBlk;
+  const bool grow = n > w->cap || (L.contact && 8LL * sub_cap > w->seed_cap);
   if ((grow || epa_blocks != w->epa_blocks) && w->pending) {
     if ((e = hipEventSynchronize(w->done)) != hipSuccess) return e;
     w->pending = false;
   }
   if (grow) {
     release_buffers(w);
-    const long long cap = std::max<long long>(L.num_pairs, 1 << 16);
+    const long long cap = std::max<long long>(n, 1 << 16);
+    const long long seed_cap = L.contact ? 8LL * sub_cap : 0;
     if ((e = hipMalloc(&w->perm, sizeof(int32_t) * cap)) != hipSuccess ||
-        (e = hipMalloc(&w->queue, sizeof(int32_t) * cap)) != hipSuccess ||
-        (e = hipMalloc(&w->keys, cap)) != hipSuccess) {
+        (e = hipMalloc(&w->keys, cap)) != hipSuccess ||
+        (seed_cap && (e = hipMalloc(&w->seeds, sizeof(shp::EpaSeed) * seed_cap)) != hipSuccess)) {
       release_buffers(w);
       return e;
     }
     w->cap = cap;
+    w->seed_cap = seed_cap;
   }
   if (epa_blocks != w->epa_blocks) {
     (void)hipFree(w->epa);
+    (void)hipFree(w->epa_waves);
     w->epa = nullptr;
+    w->epa_waves = nullptr;
     w->epa_blocks = 0;
-    if ((e = hipMalloc(&w->epa, sizeof(shp::EpaMem) * static_cast<size_t>(epa_blocks) * kBlk)) !=
-        hipSuccess)
+    const size_t lanes = static_cast<size_t>(epa_blocks) * kBlk;
+    if ((e = hipMalloc(&w->epa, sizeof(shp::EpaLane) * lanes)) != hipSuccess ||
+        (e = hipMalloc(&w->epa_waves, sizeof(shp::EpaWave) * (lanes / 64))) != hipSuccess)
       return e;
     w->epa_blocks = epa_blocks;
   }
   // a batch on another stream may still be using the buffers
   if (w->pending && (e = hipStreamWaitEvent(s, w->done, 0)) != hipSuccess) return e;
 
-  const int n = L.num_pairs;
   const int sort_grid = static_cast<int>((n + kBlk * kPerThread - 1) / (kBlk * kPerThread));
-  const int grid = (n + kBlk - 1) / kBlk;
   shape_zero<<<1, 64, 0, s>>>(w->ctr);
   shape_classify<<<sort_grid, kBlk, 0, s>>>(L, w->keys, w->ctr);
   shape_scan<<<1, 64, 0, s>>>(w->ctr);
   shape_scatter<<<sort_grid, kBlk, 0, s>>>(n, w->keys, w->ctr, w->perm);
   shape_closed<<<grid, kBlk, 0, s>>>(L, w->perm, w->ctr);
-  K.launch_gjk(grid, s, L, w->perm, w->ctr, w->queue);
-  if (L.contact) K.launch_epa(w->epa_blocks, s, L, w->queue, w->ctr, w->epa);
+  K.launch_gjk(grid, s, L, w->perm, w->ctr, w->seeds, sub_cap);
+  if (L.contact)
+    K.launch_epa(w->epa_blocks, s, L, w->seeds, w->ctr, sub_cap, w->epa, w->epa_waves);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipEventRecord(w->done, s)) != hipSuccess) return e;
   w->pending = true;

@@ -248,7 +248,7 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               kernel (0: as many as fit); each lane holds
  *                               a 13 KiB EPA block in device memory
  *   "shape_waves_per_simd" 2..4 register budget of the shape path's GJK and
- *                               EPA kernels
+ *                               EPA kernels (default 2)
  * Results do not depend on them; smaller buffers mean more rounds. */
 int fclx_set_engine_param(fclx_context* ctx, const char* name, double value);
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
