@@ -1,6 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark: batched uneven mesh collision (exhaustive contacts) on one GPU.
 
+(--workload primitives instead measures batched primitive-shape collide:
+4,000,000 pairs of box / sphere / capsule / cylinder / convex with
+enable_contact, GJK/EPA and the closed forms in FP64; see bench_primitives.)
+
 Workload: 64 mesh pairs (perturbed icospheres / heightfield terrains,
 500-200,000 triangles log-uniform), 1,000,000 pose queries split by a
 Dirichlet(1) draw, num_max_contacts = 100,000 (the full overlapping
@@ -47,6 +51,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when it is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def pair_checksum(torch, recs, nq):
     """Per-query sum of a hash of (b1, b2): equal sums with equal counts
     identify equal pair sets for the parity check between engines."""
@@ -59,9 +73,166 @@ def pair_checksum(torch, recs, nq):
     return out
 
 
+# FP64 operations of one loop pass, counted along the restatement's common
+# path (adds, multiplies, divisions and square roots one each; compares not
+# counted).  A support point is 39 + the two shapes' supports, a shape's
+# support on this workload's type mix 38 on average (box 3, sphere 3,
+# capsule 16, cylinder 9, convex of 32 points 160).  GJK pass: ray norm 6,
+# normalisation 9, support point 115, repeat test 36, progress test 11,
+# projection about 130, new ray 18.  EPA pass: normalisation 9, support
+# point 115, distance test 6, horizon walk 4.6 faces x 7, 3.5 new faces x 77
+# (edge distances, normal, offset).  Closed-form pair: about 120 on average.
+OPS_GJK_PASS = 330
+OPS_EPA_PASS = 460
+OPS_CLOSED_PAIR = 120
+# Algorithmic bytes of one pair: its two shape ids (8), two 40-B shape
+# records, two 96-B poses, the vertices of its convex shapes (32 x 24 B each;
+# 0.4 convexes per pair on this workload), and the results: count (4) and one
+# 56-B contact slot.
+BYTES_PRIM_PAIR = 8 + 80 + 192 + 0.4 * 768 + 4 + 56
+
+
+def bench_primitives(args):
+    """Batched primitive-shape collide: 4,000,000 pairs, enable_contact,
+    num_max_contacts 1; a step is one fclx_shape_collide_batch over all
+    pairs with inputs resident in HBM."""
+    import numpy as np
+    import torch
+
+    import fcl_amd as F
+    from fcl_amd import workloads
+
+    dev = torch.device("cuda:0")
+    n = args.prim_pairs
+    t0 = time.time()
+    b = workloads.primitive_batch(n)
+    log("workload: %d primitive pairs generated in %.1fs" % (n, time.time() - t0))
+    ctx = F.Context(0)
+    for kv in args.engine_param:
+        k, v = kv.split("=", 1)
+        ctx.set_engine_param(k, float(v))
+    rec = torch.from_numpy(b["shapes"].view(np.uint8)).to(dev)
+    pool = torch.from_numpy(b["vertices"]).to(dev)
+    ps = torch.from_numpy(b["pair_shapes"]).to(dev)
+    tf1 = torch.from_numpy(b["tf1"]).to(dev)
+    tf2 = torch.from_numpy(b["tf2"]).to(dev)
+    cap = b["cap"]
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    geo = (torch.empty((n * cap, 3), dtype=torch.float64, device=dev),
+           torch.empty((n * cap, 3), dtype=torch.float64, device=dev),
+           torch.empty(n * cap, dtype=torch.float64, device=dev))
+    passes = torch.zeros((n, 2), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def run(with_passes=False):
+        ctx.shape_collide_batch_async(rec, pool, ps, tf1, tf2, cap, counts, contact=geo,
+                                      stream=s, passes=passes if with_passes else None)
+
+    for _ in range(max(args.warmup, 1)):
+        run()
+    ctx.sync(s)
+    run(with_passes=True)
+    ctx.sync(s)
+    gjk_passes = int(passes[:, 0].sum().item())
+    epa_passes = int(passes[:, 1].sum().item())
+    types = b["shapes"]["type"]
+    t1 = types[b["pair_shapes"][:, 0]]
+    t2 = types[b["pair_shapes"][:, 1]]
+    closed = int((((t1 == 1) & (t2 == 1)) | ((t1 == 0) & (t2 == 0)) |
+                  ((t1 == 1) & np.isin(t2, [0, 2, 3])) |
+                  ((t2 == 1) & np.isin(t1, [0, 2, 3]))).sum())
+    hits = int((counts > 0).sum().item())
+    # timed steps, HIP events on the stream the batch runs on
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        run()
+    ev[1].record(stream)
+    ctx.sync(s)
+    elapsed = time.perf_counter() - t_start
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n / (elapsed / args.steps)
+    ops = gjk_passes * OPS_GJK_PASS + epa_passes * OPS_EPA_PASS + closed * OPS_CLOSED_PAIR
+    alg_bytes = n * BYTES_PRIM_PAIR
+    t_k = gpu_ms * 1e-3
+    roofline = {
+        "kernel": "shape pipeline (shape_gjk + shape_epa dominate)",
+        "bound": "valu",
+        "achieved": ops / t_k / 1e12,
+        "peak": PEAK_FP64 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": ops / t_k / PEAK_FP64,
+        "traffic": None,
+        "algorithmic_ops": ops,
+        "ops_per_pass": {"gjk": OPS_GJK_PASS, "epa": OPS_EPA_PASS,
+                         "closed_form_pair": OPS_CLOSED_PAIR},
+        "gjk_passes": gjk_passes, "epa_passes": epa_passes, "closed_form_pairs": closed,
+        "hbm_frac": alg_bytes / t_k / PEAK_HBM,
+        "algorithmic_bytes": alg_bytes,
+        "batch_ms_events": gpu_ms,
+    }
+    log("primitives: %.2f ms/step (events %.2f), %.3g pairs/s, %d colliding" %
+        (ms_per_step, gpu_ms, value, hits))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        threads = host_threads()
+        rng = np.random.default_rng(7)
+        probe = np.sort(rng.choice(n, size=min(n, 20000), replace=False))
+        qs = b["pair_shapes"]
+        a = time.perf_counter()
+        fclo.shape_collide_records(b["shapes"], b["vertices"], qs[probe], b["tf1"][probe],
+                                   b["tf2"][probe], cap, True, threads=threads, contacts=False)
+        probe_s = time.perf_counter() - a
+        n_s = int(min(n, max(len(probe), len(probe) * args.cpu_seconds / max(probe_s, 1e-3))))
+        sample = np.sort(rng.choice(n, size=n_s, replace=False))
+        a = time.perf_counter()
+        sc, _ = fclo.shape_collide_records(b["shapes"], b["vertices"], qs[sample],
+                                           b["tf1"][sample], b["tf2"][sample], cap, True,
+                                           threads=threads, contacts=False)
+        cpu_s = time.perf_counter() - a
+        match = bool(np.array_equal(sc, counts.cpu().numpy()[sample]))
+        cpu = {"value": n_s / cpu_s, "unit": "pairs/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d pairs, drawn uniformly (seed 7), OpenMP dynamic "
+                         "schedule; contact counts equal the GPU's: %s" % (n_s, n, match)}
+        log("cpu oracle: %d pairs in %.1fs on %d threads" % (n_s, cpu_s, threads))
+    out = {
+        "metric": "primitive collide pairs/sec (GJK/EPA + closed forms, enable_contact)",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded primitive shapes and poses)",
+        "config": {"workload": "batched primitive GJK/EPA: 4M pairs of box/sphere/capsule/"
+                               "cylinder/convex(32 pts), dims log-uniform [0.05,1], t in "
+                               "[-1,1]^3, enable_contact, GST_INDEP",
+                   "pairs": n, "num_max_contacts": cap, "colliding": hits},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["mesh", "primitives"], default="mesh",
+                    help="mesh: the batched uneven mesh collision (default); primitives: "
+                         "4M primitive-shape pairs")
+    ap.add_argument("--prim-pairs", type=int, default=4_000_000)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -83,6 +254,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(queries shard by index across GPUs; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "primitives":
+        bench_primitives(args)
+        return
 
     import numpy as np
     import torch
@@ -231,7 +406,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import fclo
 
-        threads = os.cpu_count() or 1
+        threads = host_threads()
         rng = np.random.default_rng(7)
         # size the sample from a probe so the run takes about cpu_seconds
         probe = np.sort(rng.choice(nq, size=min(nq, 2000), replace=False))

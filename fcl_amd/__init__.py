@@ -284,11 +284,12 @@ class Context:
 
     def shape_collide_batch_async(self, shapes, vertices, pair_shapes, tf1, tf2,
                                   num_max_contacts, counts, contact=None, tol=1e-6,
-                                  stream=None):
+                                  stream=None, passes=None):
         """fclx_shape_collide_batch.  shapes: device uint8 tensor of fclx_shape
         records (shape_table's, viewed as bytes); vertices: device float64
         (V,3) or None; contact: None, or (pos, normal, depth) device tensors of
-        (n*cap,3), (n*cap,3), (n*cap,) for enable_contact."""
+        (n*cap,3), (n*cap,3), (n*cap,) for enable_contact; passes: None or an
+        (n,2) int32 device tensor for the GJK / EPA loop passes per pair."""
         a = N.ShapeCollideBatchArgs()
         a.num_pairs = int(pair_shapes.shape[0])
         a.num_shapes = int(shapes.numel() // SHAPE_DTYPE.itemsize)
@@ -304,6 +305,7 @@ class Context:
         if contact is not None:
             a.enable_contact = 1
             a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
+        a.passes = passes.data_ptr() if passes is not None else None
         self._check(self._L.fclx_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
     def set_profiling(self, enable=True):

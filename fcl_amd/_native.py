@@ -73,7 +73,8 @@ class ShapeCollideBatchArgs(ctypes.Structure):
                 ("vertices", c_vp), ("num_vertices", c_i64), ("pair_shapes", c_vp),
                 ("tf1", c_vp), ("tf2", c_vp), ("num_max_contacts", c_i32),
                 ("enable_contact", c_i32), ("gjk_tolerance", c_dbl), ("counts", c_vp),
-                ("contact_pos", c_vp), ("contact_normal", c_vp), ("contact_depth", c_vp)]
+                ("contact_pos", c_vp), ("contact_normal", c_vp), ("contact_depth", c_vp),
+                ("passes", c_vp)]
 
 
 class EngineReport(ctypes.Structure):

@@ -556,6 +556,7 @@ int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_a
   L.contact_pos = a->contact_pos;
   L.contact_normal = a->contact_normal;
   L.contact_depth = a->contact_depth;
+  L.passes = a->passes;
   L.status = ctx->d_status;
   hipError_t e = shape_collide_run(ctx->sws, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(ctx, e, "shape collide launch");

@@ -325,7 +325,7 @@ __device__ __forceinline__ D3 gjk_append(Gjk& g, const MDiff& m, const D3& v) {
 // gjk-inl.h:82-200 evaluate from the guess (1, 0, 0) (a fresh solver per
 // collide() call, collision-inl.h:164-170, so no cached guess carries over;
 // evaluate is called with -guess, gjk_solver_indep-inl.h:133)
-__device__ inline int gjk_evaluate(Gjk& g, const MDiff& m, double tol) {
+__device__ inline int gjk_evaluate(Gjk& g, const MDiff& m, double tol, int* passes = nullptr) {
   int iterations = 0;
   double alpha = 0;
   D3 lastw[4];
@@ -394,6 +394,7 @@ __device__ inline int gjk_evaluate(Gjk& g, const MDiff& m, double tol) {
     }
     status = (++iterations < 128) ? status : kGjkFailed;
   } while (status == kGjkValid);
+  if (passes) *passes = iterations;
   return status;
 }
 
@@ -869,6 +870,7 @@ struct EpaSeed {
   D3 w[4], d[4];
   int32_t q;
   int32_t rank;
+  int32_t gjk_passes;
 };
 
 // -------------------------------------------------------- closed forms

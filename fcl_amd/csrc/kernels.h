@@ -75,6 +75,7 @@ struct ShapeLaunch {
   double* contact_pos;         // 3 per slot (slot = pair * cap + i), with contact
   double* contact_normal;      // 3 per slot
   double* contact_depth;       // 1 per slot
+  int32_t* passes;             // optional, 2 per pair: GJK and EPA loop passes
   int32_t* status;
 };
 

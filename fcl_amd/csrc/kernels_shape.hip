@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(kBlk) shape_classify(ShapeLaunch L, uint8_t* k
     } else {
       b = kBuckets - 1;
       L.counts[q] = 0;
+      if (L.passes) L.passes[2 * q] = L.passes[2 * q + 1] = 0;
       atomicOr(L.status, kStatusBadQuery);
     }
     keys[q] = static_cast<uint8_t>(b);
@@ -207,6 +208,7 @@ __global__ void __launch_bounds__(kBlk) shape_closed(ShapeLaunch L, const int32_
   int n = 0;
   const bool hit = shp::closed_form(s1, a, s2, b, L.contact != 0, c, n);
   write_contacts(L, q, c, shp::keep_contacts(hit, c, n, L.cap, L.contact != 0));
+  if (L.passes) L.passes[2 * q] = L.passes[2 * q + 1] = 0;
 }
 
 // The EPA queue is split into 8 sub-queues, one per XCD (blocks are
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const i
   __shared__ int block_base;
   const int i = ctr[kCtrClosed] + blockIdx.x * kBlk + threadIdx.x;
   bool push = false;
-  int q = -1;
+  int q = -1, passes = 0;
   shp::Gjk g;
   if (i < ctr[kCtrValid]) {
     q = perm[i];
@@ -233,13 +235,17 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const i
     const shp::Xf b = shp::xf_load(L.tf2 + 12 * static_cast<size_t>(q));
     shp::MDiff m;
     shp::make_mdiff(s1, a, s2, b, m);
-    const bool inside = shp::gjk_evaluate(g, m, L.tol) == shp::kGjkInside;
+    const bool inside = shp::gjk_evaluate(g, m, L.tol, &passes) == shp::kGjkInside;
     if (!inside) {
       L.counts[q] = 0;
     } else if (!L.contact) {
       L.counts[q] = 1;  // num_max_contacts >= 1
     } else {
       push = true;
+    }
+    if (L.passes && !push) {
+      L.passes[2 * q] = passes;
+      L.passes[2 * q + 1] = 0;
     }
   }
   if (!L.contact) return;  // uniform over the grid
@@ -265,6 +271,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_gjk(ShapeLaunch L, const i
     }
     sd.q = q;
     sd.rank = g.rank;
+    sd.gjk_passes = passes;
   }
 }
 
@@ -298,7 +305,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L,
   E.S = waves + wid;
   E.lane = lane;
   shp::MDiff m;
-  int q = -1;
+  int q = -1, gjk_passes = 0;
   bool active = false;
   while (true) {
     bool finish = false, started = false;
@@ -325,6 +332,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L,
           }
         const shp::EpaSeed& sd = seeds[static_cast<size_t>(x) * sub_cap + (v - before)];
         q = sd.q;
+        gjk_passes = sd.gjk_passes;
         shp::Shape s1, s2;
         load_shape(L, L.pair_shapes[2 * q], s1);
         load_shape(L, L.pair_shapes[2 * q + 1], s2);
@@ -350,6 +358,10 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L,
       const shp::Contact c =
           shp::epa_finish(E, m, shp::xf_load(L.tf1 + 12 * static_cast<size_t>(q)));
       write_contacts(L, q, &c, 1);
+      if (L.passes) {
+        L.passes[2 * q] = gjk_passes;
+        L.passes[2 * q + 1] = E.fallback ? 0 : E.it;
+      }
     }
     if (exhausted && cursor >= stop && !__ballot(active)) break;
   }

@@ -154,6 +154,10 @@ typedef struct {
   double* contact_pos;
   double* contact_normal;
   double* contact_depth;
+  /* device, optional: 2 per pair, the passes of GJK's loop and of EPA's
+   * loop (gjk-inl.h:101-196, epa-inl.h:283-321); 0 for closed-form pairs,
+   * and EPA's is 0 without enable_contact, where it is not needed */
+  int32_t* passes;
 } fclx_shape_collide_batch_args;
 
 /* Context on one GPU (device ordinal).  No counterpart in the reference,

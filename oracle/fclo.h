@@ -198,11 +198,25 @@ int fclo_shape_intersect(const fclo_shape *s1, const double tf1[12], const fclo_
 int fclo_shape_collide(const fclo_shape *s1, const double tf1[12], const fclo_shape *s2,
                        const double tf2[12], int cap, int enable_contact, double tol,
                        fclo_contact *out);
-/* the same per query (OpenMP): counts[q]; out gets cap slots per query */
+/* the same per query (OpenMP): counts[q]; out gets cap slots per query;
+ * passes (optional) gets the passes of GJK's loop and of EPA's loop per
+ * query (0 where a closed form answers or EPA falls back) */
 void fclo_shape_collide_batch(const fclo_shape *shapes, int num_queries, const int *qs,
                               const double *tf1s, const double *tf2s, int cap,
                               int enable_contact, double tol, int *counts, fclo_contact *out,
-                              int threads);
+                              int threads, int *passes);
+/* a shape as a 40-byte record (the engine's shape-table layout) */
+typedef struct {
+  int type;
+  int num_vertices;
+  double dims[3];
+  long long vertex_offset; /* CONVEX: first vertex in the pool */
+} fclo_shape_rec;
+/* fclo_shape_collide_batch over such records and their vertex pool */
+void fclo_shape_collide_records(const fclo_shape_rec *recs, const double *pool,
+                                int num_queries, const int *qs, const double *tf1s,
+                                const double *tf2s, int cap, int enable_contact, double tol,
+                                int *counts, fclo_contact *out, int threads, int *passes);
 
 #ifdef __cplusplus
 }

@@ -95,7 +95,10 @@ def lib():
                                            P_dbl, c_i32, c_i32, c_dbl, ctypes.POINTER(Contact)]),
             "fclo_shape_collide_batch": (None, [ctypes.POINTER(Shape), c_i32, P_i32, P_dbl, P_dbl,
                                                 c_i32, c_i32, c_dbl, P_i32,
-                                                ctypes.POINTER(Contact), c_i32]),
+                                                ctypes.POINTER(Contact), c_i32, P_i32]),
+            "fclo_shape_collide_records": (None, [ctypes.c_void_p, P_dbl, c_i32, P_i32, P_dbl,
+                                                  P_dbl, c_i32, c_i32, c_dbl, P_i32,
+                                                  ctypes.c_void_p, c_i32, P_i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -174,9 +177,10 @@ def shape_collide(s1, tf1, s2, tf2, cap=1, enable_contact=True, tol=1e-6):
 
 
 def shape_collide_batch(shapes, qs, tf1, tf2, cap=1, enable_contact=True, tol=1e-6,
-                        threads=None):
+                        threads=None, passes=False):
     """Per query q: shapes[qs[q,0]] vs shapes[qs[q,1]].  Returns counts (n,)
-    and contacts (n, cap, 7) = normal, pos, depth."""
+    and contacts (n, cap, 7) = normal, pos, depth; with passes=True also the
+    (n, 2) passes of GJK's and EPA's loops."""
     n = qs.shape[0]
     arr = (Shape * len(shapes))(*shapes)
     qs = np.ascontiguousarray(qs, np.int32)
@@ -184,11 +188,37 @@ def shape_collide_batch(shapes, qs, tf1, tf2, cap=1, enable_contact=True, tol=1e
     tf2 = _d(tf2)
     counts = np.zeros(n, np.int32)
     out = (Contact * max(1, n * cap))()
+    pp = np.zeros((n, 2), np.int32) if passes else None
     lib().fclo_shape_collide_batch(arr, n, _p(qs, c_i32), _p(tf1, c_dbl), _p(tf2, c_dbl),
                                    cap, 1 if enable_contact else 0, tol, _p(counts, c_i32),
-                                   out, threads or (os.cpu_count() or 1))
+                                   out, threads or (os.cpu_count() or 1),
+                                   _p(pp, c_i32) if passes else None)
     c = np.frombuffer(out, dtype=np.float64).reshape(-1, 7)[: n * cap].reshape(n, cap, 7).copy()
-    return counts, c
+    return (counts, c, pp) if passes else (counts, c)
+
+
+def shape_collide_records(rec, pool, qs, tf1, tf2, cap=1, enable_contact=True, tol=1e-6,
+                          threads=None, passes=False, contacts=True):
+    """shape_collide_batch over 40-byte shape records (fields type,
+    num_vertices, dims, vertex_offset) and their (V,3) vertex pool."""
+    rec = np.ascontiguousarray(rec)
+    assert rec.dtype.itemsize == 40
+    pool = _d(pool).reshape(-1, 3) if len(pool) else np.zeros((1, 3))
+    n = qs.shape[0]
+    qs = np.ascontiguousarray(qs, np.int32)
+    tf1 = _d(tf1)
+    tf2 = _d(tf2)
+    counts = np.zeros(n, np.int32)
+    out = np.zeros((max(1, n * cap), 7)) if contacts else None
+    pp = np.zeros((n, 2), np.int32) if passes else None
+    lib().fclo_shape_collide_records(rec.ctypes.data, _p(pool, c_dbl), n, _p(qs, c_i32),
+                                     _p(tf1, c_dbl), _p(tf2, c_dbl), cap,
+                                     1 if enable_contact else 0, tol, _p(counts, c_i32),
+                                     out.ctypes.data if contacts else None,
+                                     threads or (os.cpu_count() or 1),
+                                     _p(pp, c_i32) if passes else None)
+    c = out[: n * cap].reshape(n, cap, 7) if contacts else None
+    return (counts, c, pp) if passes else (counts, c)
 
 
 def _p(a, t):

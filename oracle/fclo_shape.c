@@ -265,6 +265,10 @@ typedef struct {
   int rank;
 } simplex;
 
+/* where gjk_epa_intersect reports the passes of GJK's and EPA's loops for
+ * the batch (per thread; NULL: not wanted) */
+static __thread int *t_passes;
+
 typedef struct {
   const mdiff *shape;
   svert vstore[NV_GJK + NV_EPA]; /* 0..3: GJK's store, 4..: EPA's */
@@ -366,6 +370,7 @@ static int gjk_evaluate(gjk_state *g, v3 guess) {
     }
     g->status = (++iterations < g->max_iter) ? g->status : GJK_FAILED;
   } while (g->status == GJK_VALID);
+  if (t_passes) t_passes[0] = iterations;
   return g->status;
 }
 
@@ -637,6 +642,7 @@ static int epa_evaluate(epa_state *E, gjk_state *g, v3 guess) {
       bind(E, tet[2], 2, tet[3], 1);
       E->status = EPA_VALID;
       for (int it = 0; it < E->max_iter; ++it) {
+        if (t_passes) t_passes[1] = it + 1;
         if (E->nextsv < NV_EPA) {
           horizon h = {-1, -1, 0};
           int w = NV_GJK + E->nextsv++;
@@ -1348,12 +1354,52 @@ int fclo_shape_collide(const fclo_shape *s1, const double tf1[12], const fclo_sh
 void fclo_shape_collide_batch(const fclo_shape *shapes, int num_queries, const int *qs,
                               const double *tf1s, const double *tf2s, int cap,
                               int enable_contact, double tol, int *counts, fclo_contact *out,
-                              int threads) {
+                              int threads, int *passes) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
   for (int q = 0; q < num_queries; ++q) {
     fclo_contact c[8];
+    int pp[2] = {0, 0};
+    t_passes = passes ? pp : 0;
     int n = fclo_shape_collide(shapes + qs[2 * q], tf1s + 12 * (size_t)q, shapes + qs[2 * q + 1],
                                tf2s + 12 * (size_t)q, cap, enable_contact, tol, c);
+    t_passes = 0;
+    if (passes) {
+      passes[2 * q] = pp[0];
+      passes[2 * q + 1] = pp[1];
+    }
+    counts[q] = n;
+    if (out)
+      for (int i = 0; i < n && i < cap; ++i) out[(size_t)q * cap + i] = c[i];
+  }
+}
+
+/* The batch over a table of 40-byte shape records (type, num_vertices,
+ * dims[3], vertex_offset -- the layout of the engine's shape records) and
+ * a vertex pool, so that large batches need no per-shape conversion. */
+void fclo_shape_collide_records(const fclo_shape_rec *recs, const double *pool,
+                                int num_queries, const int *qs, const double *tf1s,
+                                const double *tf2s, int cap, int enable_contact, double tol,
+                                int *counts, fclo_contact *out, int threads, int *passes) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+  for (int q = 0; q < num_queries; ++q) {
+    fclo_shape s[2];
+    for (int k = 0; k < 2; ++k) {
+      const fclo_shape_rec *r = recs + qs[2 * q + k];
+      s[k].type = r->type;
+      s[k].num_vertices = r->num_vertices;
+      for (int i = 0; i < 3; ++i) s[k].dims[i] = r->dims[i];
+      s[k].vertices = r->type == FCLO_SHAPE_CONVEX ? pool + 3 * r->vertex_offset : 0;
+    }
+    fclo_contact c[8];
+    int pp[2] = {0, 0};
+    t_passes = passes ? pp : 0;
+    int n = fclo_shape_collide(&s[0], tf1s + 12 * (size_t)q, &s[1], tf2s + 12 * (size_t)q, cap,
+                               enable_contact, tol, c);
+    t_passes = 0;
+    if (passes) {
+      passes[2 * q] = pp[0];
+      passes[2 * q + 1] = pp[1];
+    }
     counts[q] = n;
     if (out)
       for (int i = 0; i < n && i < cap; ++i) out[(size_t)q * cap + i] = c[i];

@@ -23,8 +23,9 @@ def _t(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV, dtype)
 
 
-def shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact, tol=1e-6):
-    """Returns (status, counts (n,), contacts (n, cap, 7) = normal, pos, depth)."""
+def shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact, tol=1e-6, passes=None):
+    """Returns (status, counts (n,), contacts (n, cap, 7) = normal, pos, depth);
+    passes: None, or a list that receives the (n, 2) GJK / EPA loop passes."""
     n = len(ps)
     d_rec = torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8)).to(DEV)
     d_pool = _t(np.asarray(pool, np.float64).reshape(-1, 3), torch.float64)
@@ -35,10 +36,13 @@ def shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact, tol=1e-6):
                torch.full((n * cap, 3), float("nan"), dtype=torch.float64, device=DEV),
                torch.full((n * cap,), float("nan"), dtype=torch.float64, device=DEV))
     s = torch.cuda.current_stream().cuda_stream
+    d_passes = torch.full((n, 2), -1, dtype=torch.int32, device=DEV) if passes is not None else None
     ctx.shape_collide_batch_async(d_rec, d_pool, _t(ps, torch.int32), _t(tf1, torch.float64),
                                   _t(tf2, torch.float64), cap, counts, contact=geo, tol=tol,
-                                  stream=s)
+                                  stream=s, passes=d_passes)
     rc = ctx.sync_status(s)
+    if passes is not None:
+        passes.append(d_passes.cpu().numpy())
     out = np.zeros((n, cap, 7))
     if contact:
         pos, nrm, dep = (t.cpu().numpy() for t in geo)
@@ -51,9 +55,14 @@ def shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact, tol=1e-6):
 def check_parity(ctx, rec, pool, ps, tf1, tf2, cap, contact, label):
     print("case: %s n=%d cap=%d contact=%d" % (label, len(ps), cap, contact), flush=True)
     shapes = fclo.shapes_from_table(rec, pool)
-    oc, ocon = fclo.shape_collide_batch(shapes, ps, tf1, tf2, cap, contact)
-    rc, gc, gcon = shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact)
+    oc, ocon, opass = fclo.shape_collide_batch(shapes, ps, tf1, tf2, cap, contact, passes=True)
+    gp = []
+    rc, gc, gcon = shapes_gpu(ctx, rec, pool, ps, tf1, tf2, cap, contact, passes=gp)
     assert rc == 0
+    # loop passes: GJK's always; EPA's where it ran for the contact
+    assert np.array_equal(gp[0][:, 0], opass[:, 0])
+    if contact:
+        assert np.array_equal(gp[0][:, 1], opass[:, 1])
     bad = np.flatnonzero(gc != oc)
     assert len(bad) == 0, "counts differ at %s (gpu %s, oracle %s)" % (
         bad[:8], gc[bad[:8]], oc[bad[:8]])
