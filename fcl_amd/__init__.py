@@ -308,6 +308,35 @@ class Context:
         a.passes = passes.data_ptr() if passes is not None else None
         self._check(self._L.fclx_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
+    def shape_collide(self, s1, v1, tf1, s2, v2, tf2, cap=1, contact=True, tol=1e-6):
+        """fclx_shape_collide: one primitive pair with host arrays.  s1/s2 are
+        shape_table records (one row each), v1/v2 a convex's (m,3) vertices
+        or None.  Returns ([(normal, pos, depth), ...], (gjk_passes, epa_passes))."""
+        recs = []
+        for rec in (s1, s2):
+            r = N.ShapeRec()
+            r.type = int(rec["type"])
+            r.num_vertices = int(rec["num_vertices"])
+            for k in range(3):
+                r.dims[k] = float(rec["dims"][k])
+            recs.append(r)
+        vs = [None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+              for v in (v1, v2)]
+        t1 = np.ascontiguousarray(tf1, np.float64).reshape(12)
+        t2 = np.ascontiguousarray(tf2, np.float64).reshape(12)
+        pos = np.zeros((4, 3))
+        nrm = np.zeros((4, 3))
+        dep = np.zeros(4)
+        n = ctypes.c_int32(0)
+        stats = np.zeros(2, np.int32)
+        self._check(self._L.fclx_shape_collide(
+            self._h, ctypes.byref(recs[0]), None if vs[0] is None else N.ptr(vs[0], N.c_dbl),
+            N.ptr(t1, N.c_dbl), ctypes.byref(recs[1]),
+            None if vs[1] is None else N.ptr(vs[1], N.c_dbl), N.ptr(t2, N.c_dbl), int(cap),
+            1 if contact else 0, float(tol), N.ptr(pos, N.c_dbl), N.ptr(nrm, N.c_dbl),
+            N.ptr(dep, N.c_dbl), ctypes.byref(n), N.ptr(stats, N.c_i32)))
+        return [(nrm[i].copy(), pos[i].copy(), float(dep[i])) for i in range(n.value)], tuple(stats)
+
     def set_profiling(self, enable=True):
         self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))
 

@@ -105,6 +105,9 @@ SIGNATURES = {
     "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
     "fclx_sync": (c_i32, [c_vp, c_vp]),
     "fclx_shape_collide_batch": (c_i32, [c_vp, ctypes.POINTER(ShapeCollideBatchArgs), c_vp]),
+    "fclx_shape_collide": (c_i32, [c_vp, ctypes.POINTER(ShapeRec), P_dbl, P_dbl,
+                                   ctypes.POINTER(ShapeRec), P_dbl, P_dbl, c_i32, c_i32, c_dbl,
+                                   P_dbl, P_dbl, P_dbl, P_i32, P_i32]),
     "fclx_collide": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, c_i32, P_i32,
                               P_i64]),
     "fclx_distance": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_dbl, P_dbl, P_i32,

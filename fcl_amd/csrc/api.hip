@@ -72,6 +72,8 @@ struct fclx_context {
   std::mutex ws_mu;       // the FAST engine's work buffers (one batch at a time)
   ShapeWorkspace* sws = nullptr;
   std::mutex sws_mu;      // the shape path's work buffers
+  char* s_shape = nullptr;        // single shape-pair staging (under mu)
+  size_t s_shape_bytes = 0;
   int epa_blocks_per_cu = 0;
   int shape_waves = 2;
 };
@@ -202,6 +204,7 @@ void fclx_shutdown(fclx_context* ctx) {
   (void)hipFree(ctx->s_geo);
   (void)hipFree(ctx->s_dist);
   (void)hipFree(ctx->s_ids);
+  (void)hipFree(ctx->s_shape);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -560,6 +563,104 @@ int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_a
   L.status = ctx->d_status;
   hipError_t e = shape_collide_run(ctx->sws, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(ctx, e, "shape collide launch");
+  return FCLX_OK;
+}
+
+int fclx_shape_collide(fclx_context* ctx, const fclx_shape* s1, const double* verts1,
+                       const double tf1[12], const fclx_shape* s2, const double* verts2,
+                       const double tf2[12], int32_t cap, int32_t enable_contact,
+                       double gjk_tolerance, double* pos, double* normal, double* depth,
+                       int32_t* num_contacts, int32_t* stats) {
+  if (!ctx || !s1 || !s2 || !tf1 || !tf2 || !num_contacts || !(gjk_tolerance > 0))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  *num_contacts = 0;
+  if (cap <= 0) return FCLX_OK;
+  if (enable_contact && (!pos || !normal || !depth))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "contact outputs are required");
+  const fclx_shape* in[2] = {s1, s2};
+  const double* vin[2] = {verts1, verts2};
+  fclx_shape rec[2];
+  int64_t nv = 0;
+  for (int k = 0; k < 2; ++k) {
+    rec[k] = *in[k];
+    rec[k].vertex_offset = 0;
+    if (rec[k].type == FCLX_SHAPE_CONVEX) {
+      if (!vin[k] || rec[k].num_vertices < 1)
+        return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "a convex shape needs its vertices");
+      rec[k].vertex_offset = nv;
+      nv += rec[k].num_vertices;
+    }
+  }
+  // staging: records 80 B, pair ids 8, poses 192, count 4, passes 8, contacts
+  // 4 x 56 B, then the vertices
+  const size_t off_ids = 80, off_tf = 88, off_cnt = 280, off_pass = 288, off_geo = 296,
+               off_v = 296 + 4 * 56;
+  const size_t bytes = off_v + 24 * static_cast<size_t>(nv);
+  std::vector<char> host(bytes, 0);
+  std::memcpy(host.data(), rec, 80);
+  const int32_t ids[2] = {0, 1};
+  std::memcpy(host.data() + off_ids, ids, 8);
+  std::memcpy(host.data() + off_tf, tf1, 96);
+  std::memcpy(host.data() + off_tf + 96, tf2, 96);
+  for (int k = 0; k < 2; ++k)
+    if (rec[k].type == FCLX_SHAPE_CONVEX)
+      std::memcpy(host.data() + off_v + 24 * rec[k].vertex_offset, vin[k],
+                  24 * static_cast<size_t>(rec[k].num_vertices));
+  const int32_t kcap = std::min<int32_t>(cap, 4);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (bytes > ctx->s_shape_bytes) {
+    (void)hipFree(ctx->s_shape);
+    ctx->s_shape = nullptr;
+    ctx->s_shape_bytes = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->s_shape, bytes));
+    ctx->s_shape_bytes = bytes;
+  }
+  char* d = ctx->s_shape;
+  HIP_TRY(ctx, hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+  ShapeLaunch L{};
+  L.num_pairs = 1;
+  L.num_shapes = 2;
+  L.shapes = reinterpret_cast<const fclx_shape*>(d);
+  L.vertices = nv ? reinterpret_cast<const double*>(d + off_v) : nullptr;
+  L.num_vertices = nv;
+  L.pair_shapes = reinterpret_cast<const int32_t*>(d + off_ids);
+  L.tf1 = reinterpret_cast<const double*>(d + off_tf);
+  L.tf2 = reinterpret_cast<const double*>(d + off_tf + 96);
+  L.cap = kcap;
+  L.contact = enable_contact ? 1 : 0;
+  L.tol = gjk_tolerance;
+  L.counts = reinterpret_cast<int32_t*>(d + off_cnt);
+  L.contact_pos = reinterpret_cast<double*>(d + off_geo);
+  L.contact_normal = reinterpret_cast<double*>(d + off_geo + 96);
+  L.contact_depth = reinterpret_cast<double*>(d + off_geo + 192);
+  L.passes = reinterpret_cast<int32_t*>(d + off_pass);
+  L.status = ctx->d_status;
+  {
+    std::lock_guard<std::mutex> sl(ctx->sws_mu);
+    if (!ctx->sws) ctx->sws = shape_workspace_create(ctx->device);
+    shape_workspace_set_epa_blocks_per_cu(ctx->sws, ctx->epa_blocks_per_cu);
+    shape_workspace_set_waves(ctx->sws, ctx->shape_waves);
+    hipError_t e = shape_collide_run(ctx->sws, L, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "shape collide launch");
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(host.data(), d, off_v, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  int32_t st = 0;
+  HIP_TRY(ctx, hipMemcpy(&st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st & kStatusBadQuery) {
+    HIP_TRY(ctx, hipMemset(ctx->d_status, 0, sizeof(int32_t)));
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "malformed shape");
+  }
+  int32_t n = 0;
+  std::memcpy(&n, host.data() + off_cnt, 4);
+  *num_contacts = n;
+  if (stats) std::memcpy(stats, host.data() + off_pass, 8);
+  if (enable_contact) {
+    std::memcpy(pos, host.data() + off_geo, 24 * static_cast<size_t>(n));
+    std::memcpy(normal, host.data() + off_geo + 96, 24 * static_cast<size_t>(n));
+    std::memcpy(depth, host.data() + off_geo + 192, 8 * static_cast<size_t>(n));
+  }
   return FCLX_OK;
 }
 

@@ -6,8 +6,9 @@
 // pre-satisfied result short-circuits (collision_func_matrix-inl.h:580,
 // distance_func_matrix-inl.h:396), and a geometry pair outside the mesh
 // path prints the reference's "not supported" warning with the neutral
-// return.  The mesh-mesh cells run on the GPU through one process-wide
-// engine context (device 0, or FCLX_DEVICE).
+// return.  The mesh-mesh cells and the primitive-primitive cells of the
+// five supported shapes run on the GPU through one process-wide engine
+// context (device 0, or FCLX_DEVICE).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -97,6 +98,103 @@ int mesh_id_of(const CollisionGeometry<double>* g, NODE_TYPE want) {
   return -1;
 }
 
+// One of the five primitives as an engine shape record (with the vertex
+// list of a convex); false for any other geometry.
+bool to_shape(const CollisionGeometry<double>* g, fclx_shape& r, std::vector<double>& verts) {
+  if (!g || g->getObjectType() != OT_GEOM) return false;
+  r = fclx_shape{};
+  switch (g->getNodeType()) {
+    case GEOM_BOX: {
+      const auto* b = static_cast<const Boxd*>(g);
+      r.type = FCLX_SHAPE_BOX;
+      for (int k = 0; k < 3; ++k) r.dims[k] = b->side[k];
+      return true;
+    }
+    case GEOM_SPHERE:
+      r.type = FCLX_SHAPE_SPHERE;
+      r.dims[0] = static_cast<const Sphered*>(g)->radius;
+      return true;
+    case GEOM_CAPSULE: {
+      const auto* c = static_cast<const Capsuled*>(g);
+      r.type = FCLX_SHAPE_CAPSULE;
+      r.dims[0] = c->radius;
+      r.dims[1] = c->lz;
+      return true;
+    }
+    case GEOM_CYLINDER: {
+      const auto* c = static_cast<const Cylinderd*>(g);
+      r.type = FCLX_SHAPE_CYLINDER;
+      r.dims[0] = c->radius;
+      r.dims[1] = c->lz;
+      return true;
+    }
+    case GEOM_CONVEX: {
+      const auto& v = static_cast<const Convexd*>(g)->getVertices();
+      r.type = FCLX_SHAPE_CONVEX;
+      r.num_vertices = static_cast<int32_t>(v.size());
+      verts.resize(3 * v.size());
+      for (size_t i = 0; i < v.size(); ++i)
+        for (int k = 0; k < 3; ++k) verts[3 * i + k] = v[i][k];
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+// the pairs both solvers answer in closed form (gjk_solver_libccd-inl.h:
+// 244-252, gjk_solver_indep-inl.h:248-256)
+bool closed_form_pair(int t1, int t2) {
+  if (t1 == t2) return t1 == FCLX_SHAPE_SPHERE || t1 == FCLX_SHAPE_BOX;
+  const int o = t1 == FCLX_SHAPE_SPHERE ? t2 : (t2 == FCLX_SHAPE_SPHERE ? t1 : -1);
+  return o == FCLX_SHAPE_BOX || o == FCLX_SHAPE_CAPSULE || o == FCLX_SHAPE_CYLINDER;
+}
+
+std::atomic<bool> g_warned_libccd{false};
+std::atomic<bool> g_warned_guess{false};
+
+// ShapeShapeCollide (collision_func_matrix-inl.h:278-311) through the GPU
+std::size_t shape_collide(const CollisionGeometry<double>* o1, const fclx_shape& r1,
+                          const std::vector<double>& v1, const Transform3d& tf1,
+                          const CollisionGeometry<double>* o2, const fclx_shape& r2,
+                          const std::vector<double>& v2, const Transform3d& tf2,
+                          const CollisionRequest<double>& request,
+                          CollisionResult<double>& result) {
+  if (request.isSatisfied(result)) return result.numContacts();
+  if (request.gjk_solver_type == GST_LIBCCD && !closed_form_pair(r1.type, r2.type) &&
+      !g_warned_libccd.exchange(true))
+    std::cerr << "fcl: the libccd solver is not available; GJK/EPA pairs use GST_INDEP\n";
+  if (request.enable_cached_gjk_guess && !g_warned_guess.exchange(true))
+    std::cerr << "fcl: cached GJK guesses are not supported; GJK starts from (1, 0, 0)\n";
+  fclx_context* ctx = engine();
+  if (!ctx) return result.numContacts();
+  const std::size_t room = request.num_max_contacts - result.numContacts();
+  const int32_t cap = room > 4 ? 4 : static_cast<int32_t>(room);
+  double p1[12], p2[12];
+  to_pose(tf1, p1);
+  to_pose(tf2, p2);
+  double pos[12], nrm[12], dep[4];
+  int32_t n = 0;
+  const int rc = fclx_shape_collide(ctx, &r1, v1.empty() ? nullptr : v1.data(), p1, &r2,
+                                    v2.empty() ? nullptr : v2.data(), p2, cap,
+                                    request.enable_contact ? 1 : 0, request.gjk_tolerance, pos,
+                                    nrm, dep, &n, nullptr);
+  if (rc != FCLX_OK) {
+    std::cerr << "fcl: shape collide failed: " << fclx_last_error(ctx) << "\n";
+    return result.numContacts();
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    if (request.enable_contact)
+      result.addContact(Contact<double>(
+          o1, o2, Contact<double>::NONE, Contact<double>::NONE,
+          Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+          Vector3d(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), dep[i]));
+    else
+      result.addContact(Contact<double>(o1, o2, Contact<double>::NONE, Contact<double>::NONE));
+  }
+  return result.numContacts();
+}
+
 void unsupported(const char* what, const CollisionGeometry<double>* o1,
                  const CollisionGeometry<double>* o2) {
   std::cerr << "Warning: " << what << " function between node type "
@@ -120,6 +218,12 @@ std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform
     std::cerr << "Warning: should stop early as num_max_contact is " << request.num_max_contacts
               << " !\n";
     return 0;
+  }
+  {
+    fclx_shape r1, r2;
+    std::vector<double> v1, v2;
+    if (detail::to_shape(o1, r1, v1) && detail::to_shape(o2, r2, v2))
+      return detail::shape_collide(o1, r1, v1, tf1, o2, r2, v2, tf2, request, result);
   }
   const int id1 = detail::mesh_id_of(o1, BV_OBBRSS);
   const int id2 = detail::mesh_id_of(o2, BV_OBBRSS);

@@ -12,8 +12,10 @@
 //   CollisionObject  include/fcl/narrowphase/collision_object.h
 // Eigen is not required: Vector3/Matrix3/Transform3 are small value types
 // with the subset of Eigen's interface that callers of this path use.
-// Mesh-mesh queries on BVHModel<OBBRSS<double>> (collide, distance) and
-// BVHModel<RSS<double>> (distance) run on the GPU; any other pair of
+//   Box / Sphere / Capsule / Cylinder / Convex  include/fcl/geometry/shape/
+// Mesh-mesh queries on BVHModel<OBBRSS<double>> (collide, distance),
+// BVHModel<RSS<double>> (distance) and collide between two of the five
+// primitives (the GST_INDEP solver) run on the GPU; any other pair of
 // geometry types is reported unsupported with the reference's neutral
 // return (0 contacts / max distance).  Only S = double is instantiated.
 #ifndef FCL_AMD_FCL_H
@@ -308,6 +310,85 @@ class BVHModel : public CollisionGeometry<typename BV::S> {
 };
 
 // ------------------------------------------------------ requests/results
+// ------------------------------------------------------------ shapes
+// The primitives of collide()'s shape path (geometry/shape/shape_base.h,
+// box.h:58-67, sphere.h:57-60, capsule.h:58-64, cylinder.h:58-64,
+// convex.h:110-152), in the reference's local frames: centred at the
+// origin, capsule and cylinder along z with total length lz (the capsule's
+// segment length).  A convex keeps its vertex and face lists; its support
+// function scans the vertices (minkowski_diff-inl.h:182-196).
+template <typename S>
+class ShapeBase : public CollisionGeometry<S> {
+ public:
+  OBJECT_TYPE getObjectType() const override { return OT_GEOM; }
+};
+
+template <typename S_>
+class Box : public ShapeBase<S_> {
+ public:
+  using S = S_;
+  Box(S x, S y, S z) : side(x, y, z) {}
+  explicit Box(const Vector3<S>& side_) : side(side_) {}
+  Box() = default;
+  NODE_TYPE getNodeType() const override { return GEOM_BOX; }
+  Vector3<S> side;
+};
+
+template <typename S_>
+class Sphere : public ShapeBase<S_> {
+ public:
+  using S = S_;
+  explicit Sphere(S radius_) : radius(radius_) {}
+  NODE_TYPE getNodeType() const override { return GEOM_SPHERE; }
+  S radius;
+};
+
+template <typename S_>
+class Capsule : public ShapeBase<S_> {
+ public:
+  using S = S_;
+  Capsule(S radius_, S lz_) : radius(radius_), lz(lz_) {}
+  NODE_TYPE getNodeType() const override { return GEOM_CAPSULE; }
+  S radius;
+  S lz;
+};
+
+template <typename S_>
+class Cylinder : public ShapeBase<S_> {
+ public:
+  using S = S_;
+  Cylinder(S radius_, S lz_) : radius(radius_), lz(lz_) {}
+  NODE_TYPE getNodeType() const override { return GEOM_CYLINDER; }
+  S radius;
+  S lz;
+};
+
+template <typename S_>
+class Convex : public ShapeBase<S_> {
+ public:
+  using S = S_;
+  Convex(const std::shared_ptr<const std::vector<Vector3<S>>>& vertices, int num_faces,
+         const std::shared_ptr<const std::vector<int>>& faces, bool throw_if_invalid = false)
+      : vertices_(vertices), num_faces_(num_faces), faces_(faces) {
+    (void)throw_if_invalid;
+  }
+  NODE_TYPE getNodeType() const override { return GEOM_CONVEX; }
+  const std::vector<Vector3<S>>& getVertices() const { return *vertices_; }
+  int getFaceCount() const { return num_faces_; }
+  const std::vector<int>& getFaces() const { return *faces_; }
+
+ private:
+  std::shared_ptr<const std::vector<Vector3<S>>> vertices_;
+  int num_faces_;
+  std::shared_ptr<const std::vector<int>> faces_;
+};
+
+using Boxd = Box<double>;
+using Sphered = Sphere<double>;
+using Capsuled = Capsule<double>;
+using Cylinderd = Cylinder<double>;
+using Convexd = Convex<double>;
+
 template <typename S>
 struct Contact {
   const CollisionGeometry<S>* o1 = nullptr;

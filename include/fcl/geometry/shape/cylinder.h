@@ -1,0 +1,6 @@
+// Forwarding header at FCL's include path (include/fcl/geometry/shape/cylinder.h):
+// everything this path needs is declared in fcl/fcl.h.
+#ifndef FCL_AMD_FWD_GEOMETRY_SHAPE_CYLINDER_H
+#define FCL_AMD_FWD_GEOMETRY_SHAPE_CYLINDER_H
+#include "fcl/fcl.h"
+#endif

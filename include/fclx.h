@@ -281,6 +281,21 @@ int fclx_collide_contact(fclx_context* ctx, int32_t mesh1, const double tf1[12],
                          const double tf2[12], int32_t cap, int32_t* pairs, double* pos,
                          double* normal, double* depth, int32_t capacity,
                          int32_t* num_contacts, int64_t* stats);
+/* One primitive pair with host buffers: the drop-in path behind
+ * fcl::collide for two shapes (ShapeShapeCollide,
+ * collision_func_matrix-inl.h:278-311, with GJKSolver_indep).  verts1 /
+ * verts2: a CONVEX shape's vertex list (3 doubles each; NULL otherwise; the
+ * records' vertex_offset is not used).  cap = num_max_contacts minus the
+ * contacts already in the caller's result.  *num_contacts receives the
+ * contacts added (<= cap, at most 4); with enable_contact their world-frame
+ * point, normal and depth go to pos (3 each), normal (3) and depth (1),
+ * which need room for min(cap, 4) contacts.  stats (optional) = {GJK loop
+ * passes, EPA loop passes}. */
+int fclx_shape_collide(fclx_context* ctx, const fclx_shape* s1, const double* verts1,
+                       const double tf1[12], const fclx_shape* s2, const double* verts2,
+                       const double tf2[12], int32_t cap, int32_t enable_contact,
+                       double gjk_tolerance, double* pos, double* normal, double* depth,
+                       int32_t* num_contacts, int32_t* stats);
 /* One query: the drop-in path behind fcl::distance for two BVHModel<RSS> or
  * BVHModel<OBBRSS> (orientedMeshDistance, distance_func_matrix-inl.h:386-404).
  * points (optional) = nearest points on mesh1 and mesh2 in the world frame;

@@ -19,3 +19,13 @@ def test_cpp_mesh_api():
                        stderr=subprocess.STDOUT, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0 and "PASS" in r.stdout
+
+
+def test_cpp_shape_api():
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "test_shape_api")
+    assert os.path.exists(exe), "build the C++ tests first (__graft_entry__.build())"
+    print("case: C++ drop-in API on primitive pairs", flush=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0 and "PASS" in r.stdout

@@ -182,3 +182,26 @@ def test_empty_batch(ctx):
     rc, gc, _ = shapes_gpu(ctx, rec, pool, np.zeros((0, 2), np.int32), np.zeros((0, 12)),
                            np.zeros((0, 12)), 1, True)
     assert rc == 0 and len(gc) == 0
+
+
+def test_single_pair_entry(ctx):
+    """fclx_shape_collide (host buffers, one pair: the path behind
+    fcl::collide for two primitives) gives the batch's contacts and passes."""
+    b = W.primitive_batch(400, seed=0x51)
+    rec, pool, ps = b["shapes"], b["vertices"], b["pair_shapes"]
+    print("case: single-pair entry, 400 pairs", flush=True)
+    oc, ocon, opass = fclo.shape_collide_records(rec, pool, ps, b["tf1"], b["tf2"], 4, True,
+                                                 passes=True)
+    for q in range(len(ps)):
+        sv = []
+        for k in range(2):
+            r = rec[ps[q, k]]
+            o = int(r["vertex_offset"])
+            sv.append(pool[o:o + int(r["num_vertices"])] if r["type"] == F.CONVEX else None)
+        cs, passes = ctx.shape_collide(rec[ps[q, 0]], sv[0], b["tf1"][q], rec[ps[q, 1]], sv[1],
+                                       b["tf2"][q], cap=4)
+        assert len(cs) == oc[q]
+        assert passes == tuple(opass[q])
+        for i, (nrm, pos, dep) in enumerate(cs):
+            assert np.array_equal(nrm, ocon[q, i, 0:3]) and np.array_equal(pos, ocon[q, i, 3:6])
+            assert dep == ocon[q, i, 6]
