@@ -355,3 +355,98 @@ class Context:
     def sync_status(self, stream=None):
         """Like sync() but returns the status code instead of raising."""
         return self._L.fclx_sync(self._h, stream)
+
+
+class Server:
+    """fclx_server_*: the request loop over one resident scene mesh.  A
+    request is a robot's link meshes and its link poses along a trajectory
+    ((waypoints, links, 12) float64); its result is a (waypoints, links)
+    uint8 array, 1 where the link collides with the scene."""
+
+    def __init__(self, ctx, scene_mesh, scene_pose=None, num_streams=2, hbm_budget=0,
+                 max_batch_queries=0, engine=ENGINE_FAST):
+        self._ctx = ctx
+        self._L = ctx._L
+        cfg = N.ServerConfig()
+        cfg.scene_mesh = int(scene_mesh)
+        cfg.num_streams = int(num_streams)
+        pose = (np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float64) if scene_pose is None
+                else np.ascontiguousarray(scene_pose, np.float64).reshape(12))
+        for i in range(12):
+            cfg.scene_pose[i] = float(pose[i])
+        cfg.hbm_budget = int(hbm_budget)
+        cfg.max_batch_queries = int(max_batch_queries)
+        cfg.engine = int(engine)
+        h = ctypes.c_void_p()
+        ctx._check(self._L.fclx_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._live = {}
+
+    def _request(self, link_meshes, poses, tag):
+        lm = np.ascontiguousarray(link_meshes, np.int32)
+        P = np.ascontiguousarray(poses, np.float64)
+        assert P.ndim == 3 and P.shape[1] == len(lm) and P.shape[2] == 12
+        res = np.zeros(P.shape[:2], np.uint8)
+        r = N.Request()
+        r.num_links = len(lm)
+        r.num_waypoints = P.shape[0]
+        r.link_meshes = lm.ctypes.data
+        r.link_poses = P.ctypes.data
+        r.result = res.ctypes.data
+        r.tag = int(tag)
+        return r, (lm, P, res)
+
+    def submit(self, link_meshes, poses, tag):
+        """Queues a request; its result array is returned by poll()."""
+        r, keep = self._request(link_meshes, poses, tag)
+        self._live[int(tag)] = keep
+        self._ctx._check(self._L.fclx_server_submit(self._h, ctypes.byref(r)))
+
+    def poll(self, max_n=256, wait_ms=0):
+        """Completions: a list of (tag, status, num_queries, queue_ms,
+        latency_ms, result array)."""
+        buf = (N.Completion * max_n)()
+        n = ctypes.c_int32(0)
+        self._ctx._check(self._L.fclx_server_poll(self._h, buf, max_n, int(wait_ms),
+                                                  ctypes.byref(n)))
+        out = []
+        for c in buf[:n.value]:
+            keep = self._live.pop(int(c.tag), None)
+            out.append((c.tag, c.status, c.num_queries, c.queue_ms, c.latency_ms,
+                        keep[2] if keep else None))
+        return out
+
+    def replay(self, requests, arrival_ms):
+        """requests: list of (link_meshes, poses); submits request i at
+        arrival_ms[i] (ms after the call) and waits for all.  Returns (results,
+        latencies_ms, elapsed_ms)."""
+        n = len(requests)
+        arr = (N.Request * max(1, n))()
+        keep = []
+        for i, (lm, P) in enumerate(requests):
+            r, k = self._request(lm, P, i)
+            arr[i] = r
+            keep.append(k)
+        t = np.ascontiguousarray(arrival_ms, np.float64)
+        lat = np.zeros(n)
+        el = ctypes.c_double(0)
+        self._ctx._check(self._L.fclx_server_replay(self._h, arr, N.ptr(t, N.c_dbl), n,
+                                                    N.ptr(lat, N.c_dbl), ctypes.byref(el)))
+        return [k[2] for k in keep], lat, el.value
+
+    def stats(self):
+        b = ctypes.c_int64(0)
+        q = ctypes.c_int64(0)
+        self._ctx._check(self._L.fclx_server_stats(self._h, ctypes.byref(b), ctypes.byref(q)))
+        return b.value, q.value
+
+    def close(self):
+        if self._h:
+            self._L.fclx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -77,6 +77,21 @@ class ShapeCollideBatchArgs(ctypes.Structure):
                 ("passes", c_vp)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("scene_mesh", c_i32), ("num_streams", c_i32), ("scene_pose", c_dbl * 12),
+                ("hbm_budget", c_i64), ("max_batch_queries", c_i32), ("engine", c_i32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("num_links", c_i32), ("num_waypoints", c_i32), ("link_meshes", c_vp),
+                ("link_poses", c_vp), ("result", c_vp), ("tag", c_i64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("tag", c_i64), ("status", c_i32), ("num_queries", c_i32), ("queue_ms", c_dbl),
+                ("latency_ms", c_dbl)]
+
+
 class EngineReport(ctypes.Structure):
     _fields_ = [("setup_ms", c_dbl), ("expand_ms", c_dbl), ("leaves_ms", c_dbl),
                 ("dfs_ms", c_dbl), ("fixup_ms", c_dbl), ("expand_bv_tests", c_i64),
@@ -114,6 +129,12 @@ SIGNATURES = {
                               P_i64]),
     "fclx_collide_contact": (c_i32, [c_vp, c_i32, P_dbl, c_i32, P_dbl, c_i32, P_i32, P_dbl,
                                      P_dbl, P_dbl, c_i32, P_i32, P_i64]),
+    "fclx_server_create": (c_i32, [c_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(c_vp)]),
+    "fclx_server_submit": (c_i32, [c_vp, ctypes.POINTER(Request)]),
+    "fclx_server_poll": (c_i32, [c_vp, ctypes.POINTER(Completion), c_i32, c_i32, P_i32]),
+    "fclx_server_replay": (c_i32, [c_vp, ctypes.POINTER(Request), P_dbl, c_i32, P_dbl, P_dbl]),
+    "fclx_server_stats": (c_i32, [c_vp, P_i64, P_i64]),
+    "fclx_server_destroy": (c_i32, [c_vp]),
     "fclx_synth_icosphere_size": (None, [c_i32, P_i32, P_i32]),
     "fclx_synth_icosphere": (c_i32, [c_i32, c_dbl, c_u64, P_dbl, P_i32]),
     "fclx_synth_heightfield_size": (None, [c_i32, P_i32, P_i32]),

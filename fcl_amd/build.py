@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfclx.so")
 
 HIP_SOURCES = ["api.hip", "kernels_plain.hip", "kernels_shape.hip", "engine.hip",
-               "host/fcl_api.hip"]
+               "host/fcl_api.hip", "host/serve.hip"]
 HOST_SOURCES = ["host/bvh_build.cpp", "host/synth.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

@@ -158,6 +158,10 @@ double centre_bound(const std::vector<HostNode>& nodes) {
 
 }  // namespace
 
+namespace fclx {
+int context_device(const fclx_context* ctx) { return ctx ? ctx->device : 0; }
+}  // namespace fclx
+
 extern "C" {
 
 int fclx_init(int32_t device, fclx_context** out) {

@@ -344,7 +344,7 @@ __device__ __forceinline__ void record_hits(const EngineArgs& A, bool hit, int q
     L.pairs[3 * slot] = q;
     L.pairs[3 * slot + 1] = id1;
     L.pairs[3 * slot + 2] = id2;
-  } else {
+  } else if (L.pairs) {  // no buffer: counts only
     atomicOr(L.status, kStatusPairOverflow);
   }
   count_hits(A, q);
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) engine_leaves(EngineArgs A) {
         over = true;
       }
     }
-    if (over) atomicOr(L.status, kStatusPairOverflow);
+    if (over && L.pairs) atomicOr(L.status, kStatusPairOverflow);
     __syncthreads();
     if (threadIdx.x == 0) n_stage = 0;
     __syncthreads();
@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(kBlock) ordered_collide(CollideLaunch L,
               L.pairs[3 * slot] = q;
               L.pairs[3 * slot + 1] = id1;
               L.pairs[3 * slot + 2] = id2;
-            } else {
+            } else if (L.pairs) {  // no buffer: counts only
               atomicOr(L.status, kStatusPairOverflow);
             }
             ++count;
@@ -1053,16 +1053,30 @@ struct EngineWorkspace {
   double busy_sum = 0, busy_den = 0;
   EngineReport report{};
   EngineTuning tune{};
+  // work counters of the small-cap kernel: batches on different streams may
+  // run at once, so each launch takes the next slot of a ring (one cache
+  // line each) and waits for the slot's previous launch to end
+  static constexpr int kOrdSlots = 16;
+  unsigned long long* ord_ctr = nullptr;  // kOrdSlots x 16
+  hipEvent_t ord_ev[kOrdSlots] = {};
+  bool ord_used[kOrdSlots] = {};
+  int ord_next = 0;
 };
 
 EngineTuning& engine_tuning(EngineWorkspace* ws) { return ws->tune; }
 
 EngineWorkspace* engine_workspace_create() {
   auto* ws = new EngineWorkspace();
-  if (hipMalloc(reinterpret_cast<void**>(&ws->counters),
-                kNumCounters * sizeof(unsigned long long)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&ws->host_counters),
-                    kNumCounters * sizeof(unsigned long long)) != hipSuccess) {
+  bool ok = hipMalloc(reinterpret_cast<void**>(&ws->counters),
+                      kNumCounters * sizeof(unsigned long long)) == hipSuccess &&
+            hipHostMalloc(reinterpret_cast<void**>(&ws->host_counters),
+                          kNumCounters * sizeof(unsigned long long)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->ord_ctr),
+                      EngineWorkspace::kOrdSlots * 16 * sizeof(unsigned long long)) ==
+                hipSuccess;
+  for (int i = 0; ok && i < EngineWorkspace::kOrdSlots; ++i)
+    ok = hipEventCreateWithFlags(&ws->ord_ev[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
     engine_workspace_destroy(ws);
     return nullptr;
   }
@@ -1084,6 +1098,11 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
   (void)hipFree(ws->defer[0]);
   (void)hipFree(ws->defer[1]);
   if (ws->host_counters) (void)hipHostFree(ws->host_counters);
+  for (int i = 0; i < EngineWorkspace::kOrdSlots; ++i) {
+    if (ws->ord_used[i]) (void)hipEventSynchronize(ws->ord_ev[i]);
+    if (ws->ord_ev[i]) (void)hipEventDestroy(ws->ord_ev[i]);
+  }
+  (void)hipFree(ws->ord_ctr);
   for (auto& e : ws->ev)
     if (e) (void)hipEventDestroy(e);
   delete ws;
@@ -1451,16 +1470,23 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   if (nq == 0) return hipSuccess;
   const int cus = device_cus();
   if (L.cap <= kOrderedCapMax) {
-    set_u64<<<1, 1, 0, s>>>(ws->counters + 0, 0);
+    const int slot = ws->ord_next;
+    ws->ord_next = (slot + 1) % EngineWorkspace::kOrdSlots;
+    if (ws->ord_used[slot]) ENG_TRY(hipStreamWaitEvent(s, ws->ord_ev[slot], 0));
+    unsigned long long* ctr = ws->ord_ctr + 16 * slot;
+    set_u64<<<1, 1, 0, s>>>(ctr, 0);
     const unsigned grid = static_cast<unsigned>(
         std::min<long long>(grid_for(nq), static_cast<long long>(cus) * 8));
     if (stack_need <= 64)
-      ordered_collide<64><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
+      ordered_collide<64><<<grid, kBlock, 0, s>>>(L, ctr);
     else if (stack_need <= 128)
-      ordered_collide<128><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
+      ordered_collide<128><<<grid, kBlock, 0, s>>>(L, ctr);
     else
-      ordered_collide<256><<<grid, kBlock, 0, s>>>(L, ws->counters + 0);
-    return hipGetLastError();
+      ordered_collide<256><<<grid, kBlock, 0, s>>>(L, ctr);
+    ENG_TRY(hipGetLastError());
+    ENG_TRY(hipEventRecord(ws->ord_ev[slot], s));
+    ws->ord_used[slot] = true;
+    return hipSuccess;
   }
   if (nq >= (1 << 22)) return hipErrorInvalidValue;  // task packing limit
   const EngineTuning& tune = ws->tune;

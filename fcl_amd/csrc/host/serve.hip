@@ -1,0 +1,407 @@
+// Request loop over one resident scene (fclx_server_*, include/fclx.h).
+//
+// Requests queue on the host.  A worker thread owns num_streams slots, each
+// with its own HIP stream, device buffers (query mesh ids, the scene pose
+// filled once per query slot, link poses, counts) and pinned staging.
+// Whenever a slot is free it takes, in arrival order, every waiting request
+// that fits the slot's query capacity and the device-memory budget, copies
+// their poses in one transfer, launches one first-contact collide batch
+// (CollisionRequest(1, false) per link and waypoint), copies the counts back
+// and records an event; when the event has fired it writes each request's
+// results and reports its completion.  Slots run concurrently on the
+// device, so later requests overlap earlier ones.  Every query of a batch is
+// independent of the others, so a request's result is the same alone and in
+// any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/fclx.h"
+
+namespace fclx {
+int context_device(const fclx_context* ctx);
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
+
+__global__ void fill_pose(double* out, int n, fclx_server_config cfg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) out[12 * static_cast<size_t>(i) + k] = cfg.scene_pose[k];
+}
+
+struct Pending {
+  fclx_request req;
+  Clock::time_point t_submit;
+  int64_t nq;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  int64_t cap = 0;  // queries
+  int32_t* d_qm = nullptr;
+  double* d_tf1 = nullptr;
+  double* d_tf2 = nullptr;
+  int32_t* d_counts = nullptr;
+  int64_t* d_num = nullptr;
+  int32_t* h_qm = nullptr;
+  double* h_tf2 = nullptr;
+  int32_t* h_counts = nullptr;
+  bool busy = false;
+  int status = FCLX_OK;
+  std::vector<Pending> batch;
+  std::vector<int64_t> first;  // first query of each request of the batch
+  int64_t queries = 0;
+  Clock::time_point t_launch;
+};
+
+void free_slot_buffers(Slot& s) {
+  (void)hipFree(s.d_qm);
+  (void)hipFree(s.d_tf1);
+  (void)hipFree(s.d_tf2);
+  (void)hipFree(s.d_counts);
+  (void)hipFree(s.d_num);
+  if (s.h_qm) (void)hipHostFree(s.h_qm);
+  if (s.h_tf2) (void)hipHostFree(s.h_tf2);
+  if (s.h_counts) (void)hipHostFree(s.h_counts);
+  s.d_qm = s.h_qm = s.d_counts = s.h_counts = nullptr;
+  s.d_tf1 = s.d_tf2 = s.h_tf2 = nullptr;
+  s.d_num = nullptr;
+  s.cap = 0;
+}
+
+}  // namespace
+
+struct fclx_server {
+  fclx_context* ctx = nullptr;
+  fclx_server_config cfg{};
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;       // submissions and stop
+  std::condition_variable cv_done;  // completions
+  std::deque<Pending> queue;
+  std::deque<fclx_completion> done;
+  std::vector<Slot> slots;
+  std::thread worker;
+  bool stop = false;
+  int64_t in_flight_bytes = 0;
+  int64_t batches = 0, queries = 0;
+};
+
+namespace {
+
+// Slot buffers for at least n queries (the slot is idle).
+hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
+  if (n <= s.cap) return hipSuccess;
+  free_slot_buffers(s);
+  const int64_t cap = std::max<int64_t>(n, 1 << 16);
+  hipError_t e;
+  if ((e = hipMalloc(&s.d_qm, 8 * cap)) != hipSuccess ||
+      (e = hipMalloc(&s.d_tf1, 96 * cap)) != hipSuccess ||
+      (e = hipMalloc(&s.d_tf2, 96 * cap)) != hipSuccess ||
+      (e = hipMalloc(&s.d_counts, 4 * cap)) != hipSuccess ||
+      (e = hipMalloc(&s.d_num, 8)) != hipSuccess ||
+      (e = hipHostMalloc(&s.h_qm, 8 * cap)) != hipSuccess ||
+      (e = hipHostMalloc(&s.h_tf2, 96 * cap)) != hipSuccess ||
+      (e = hipHostMalloc(&s.h_counts, 4 * cap)) != hipSuccess) {
+    free_slot_buffers(s);
+    return e;
+  }
+  s.cap = cap;
+  fill_pose<<<static_cast<unsigned>((cap + 255) / 256), 256, 0, s.stream>>>(
+      s.d_tf1, static_cast<int>(cap), S->cfg);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipStreamSynchronize(s.stream);
+}
+
+// Copies, launch and read-back of the slot's batch (filled by the caller).
+int slot_launch(fclx_server* S, Slot& s) {
+  int64_t k = 0;
+  s.first.clear();
+  for (const Pending& p : s.batch) {
+    s.first.push_back(k);
+    const int L = p.req.num_links;
+    std::memcpy(s.h_tf2 + 12 * k, p.req.link_poses, 96 * static_cast<size_t>(p.nq));
+    for (int64_t j = 0; j < p.nq; ++j) {
+      s.h_qm[2 * (k + j)] = S->cfg.scene_mesh;
+      s.h_qm[2 * (k + j) + 1] = p.req.link_meshes[j % L];
+    }
+    k += p.nq;
+  }
+  s.queries = k;
+  if (hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+      hipMemcpyAsync(s.d_qm, s.h_qm, 8 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+    return FCLX_ERR_DEVICE;
+  fclx_collide_batch_args a{};
+  a.num_queries = static_cast<int32_t>(k);
+  a.query_meshes = s.d_qm;
+  a.tf1 = s.d_tf1;
+  a.tf2 = s.d_tf2;
+  a.num_max_contacts = 1;
+  a.engine = S->cfg.engine;
+  a.counts = s.d_counts;
+  a.pairs = nullptr;
+  a.pair_capacity = 0;
+  a.num_pairs = s.d_num;
+  int rc = fclx_collide_batch(S->ctx, &a, s.stream);
+  if (rc != FCLX_OK) return rc;
+  if (hipMemcpyAsync(s.h_counts, s.d_counts, 4 * k, hipMemcpyDeviceToHost, s.stream) !=
+          hipSuccess ||
+      hipEventRecord(s.done, s.stream) != hipSuccess)
+    return FCLX_ERR_DEVICE;
+  return FCLX_OK;
+}
+
+// Results of a finished slot to its requests (called with mu not held).
+void slot_retire(fclx_server* S, Slot& s, std::vector<fclx_completion>& out) {
+  for (size_t i = 0; i < s.batch.size(); ++i) {
+    const Pending& p = s.batch[i];
+    if (s.status == FCLX_OK) {
+      const int32_t* c = s.h_counts + s.first[i];
+      for (int64_t j = 0; j < p.nq; ++j) p.req.result[j] = c[j] > 0 ? 1 : 0;
+    }
+    fclx_completion cp{};
+    cp.tag = p.req.tag;
+    cp.status = s.status;
+    cp.num_queries = static_cast<int32_t>(p.nq);
+    cp.queue_ms = std::chrono::duration<double, std::milli>(s.t_launch - p.t_submit).count();
+    cp.latency_ms = ms_since(p.t_submit);
+    out.push_back(cp);
+  }
+}
+
+void worker_main(fclx_server* S) {
+  (void)hipSetDevice(S->device);
+  std::vector<fclx_completion> finished;
+  std::unique_lock<std::mutex> lock(S->mu);
+  while (true) {
+    bool progress = false;
+    // retire finished slots
+    for (Slot& s : S->slots) {
+      if (!s.busy) continue;
+      lock.unlock();
+      const bool ready = s.status != FCLX_OK || hipEventQuery(s.done) == hipSuccess;
+      if (ready) {
+        finished.clear();
+        slot_retire(S, s, finished);
+      }
+      lock.lock();
+      if (!ready) continue;
+      for (const Pending& p : s.batch) S->in_flight_bytes -= p.nq * kQueryBytes;
+      s.batch.clear();
+      s.busy = false;
+      for (const auto& c : finished) S->done.push_back(c);
+      S->cv_done.notify_all();
+      progress = true;
+    }
+    // launch into free slots
+    for (Slot& s : S->slots) {
+      if (s.busy || S->queue.empty()) continue;
+      int64_t nq = 0, bytes = 0;
+      s.batch.clear();
+      while (!S->queue.empty()) {
+        const Pending& p = S->queue.front();
+        const int64_t b = p.nq * kQueryBytes;
+        const bool first = s.batch.empty();
+        // a request larger than a batch or than the budget runs alone once
+        // nothing else is in flight
+        if (!first && nq + p.nq > S->cfg.max_batch_queries) break;
+        if (S->in_flight_bytes + bytes + b > S->cfg.hbm_budget &&
+            !(first && S->in_flight_bytes == 0))
+          break;
+        nq += p.nq;
+        bytes += b;
+        s.batch.push_back(p);
+        S->queue.pop_front();
+      }
+      if (s.batch.empty()) break;  // waits for memory to free up
+      S->in_flight_bytes += bytes;
+      s.busy = true;
+      s.t_launch = Clock::now();
+      ++S->batches;
+      S->queries += nq;
+      lock.unlock();
+      int rc = slot_reserve(S, s, nq) == hipSuccess ? FCLX_OK : FCLX_ERR_OUT_OF_MEMORY;
+      if (rc == FCLX_OK) rc = slot_launch(S, s);
+      if (rc != FCLX_OK) (void)hipStreamSynchronize(s.stream);  // nothing left in flight
+      lock.lock();
+      s.status = rc;
+      progress = true;
+    }
+    bool any_busy = false;
+    for (const Slot& s : S->slots) any_busy |= s.busy;
+    if (S->stop && S->queue.empty() && !any_busy) break;
+    if (!progress) {
+      if (any_busy)
+        S->cv.wait_for(lock, std::chrono::microseconds(20));
+      else
+        S->cv.wait_for(lock, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_server** out) {
+  if (!ctx || !cfg || !out) return FCLX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (cfg->num_streams < 1 || cfg->num_streams > 8 || cfg->hbm_budget < 0 ||
+      cfg->max_batch_queries < 0 ||
+      (cfg->engine != FCLX_ENGINE_FAST && cfg->engine != FCLX_ENGINE_PLAIN))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  fclx_mesh_info info{};
+  if (fclx_mesh_get_info(ctx, cfg->scene_mesh, &info) != FCLX_OK ||
+      info.bv_type != FCLX_BV_OBBRSS)
+    return FCLX_ERR_INVALID_ARGUMENT;
+  auto* S = new fclx_server();
+  S->ctx = ctx;
+  S->cfg = *cfg;
+  if (S->cfg.hbm_budget == 0) S->cfg.hbm_budget = int64_t(2) << 30;
+  if (S->cfg.max_batch_queries == 0) S->cfg.max_batch_queries = 1 << 20;
+  S->device = fclx::context_device(ctx);
+  (void)hipSetDevice(S->device);
+  S->slots.resize(cfg->num_streams);
+  for (Slot& s : S->slots) {
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+      for (Slot& t : S->slots) {
+        if (t.stream) (void)hipStreamDestroy(t.stream);
+        if (t.done) (void)hipEventDestroy(t.done);
+      }
+      delete S;
+      return FCLX_ERR_DEVICE;
+    }
+  }
+  S->worker = std::thread(worker_main, S);
+  *out = S;
+  return FCLX_OK;
+}
+
+int fclx_server_submit(fclx_server* S, const fclx_request* req) {
+  if (!S || !req || req->num_links < 1 || req->num_waypoints < 1 || !req->link_meshes ||
+      !req->link_poses || !req->result)
+    return FCLX_ERR_INVALID_ARGUMENT;
+  const int64_t nq = static_cast<int64_t>(req->num_links) * req->num_waypoints;
+  if (nq > (int64_t(1) << 30)) return FCLX_ERR_INVALID_ARGUMENT;
+  for (int32_t l = 0; l < req->num_links; ++l) {
+    fclx_mesh_info info{};
+    if (fclx_mesh_get_info(S->ctx, req->link_meshes[l], &info) != FCLX_OK ||
+        info.bv_type != FCLX_BV_OBBRSS)
+      return FCLX_ERR_INVALID_ARGUMENT;
+  }
+  Pending p;
+  p.req = *req;
+  p.t_submit = Clock::now();
+  p.nq = nq;
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    if (S->stop) return FCLX_ERR_INVALID_ARGUMENT;
+    S->queue.push_back(p);
+  }
+  S->cv.notify_one();
+  return FCLX_OK;
+}
+
+int fclx_server_poll(fclx_server* S, fclx_completion* out, int32_t max, int32_t wait_ms,
+                     int32_t* n) {
+  if (!S || !n || max < 0 || (max > 0 && !out)) return FCLX_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lock(S->mu);
+  if (S->done.empty() && wait_ms > 0)
+    S->cv_done.wait_for(lock, std::chrono::milliseconds(wait_ms),
+                        [S] { return !S->done.empty(); });
+  int32_t k = 0;
+  while (k < max && !S->done.empty()) {
+    out[k++] = S->done.front();
+    S->done.pop_front();
+  }
+  *n = k;
+  return FCLX_OK;
+}
+
+int fclx_server_replay(fclx_server* S, const fclx_request* requests, const double* arrival_ms,
+                       int32_t n, double* latency_ms, double* elapsed_ms) {
+  if (!S || n < 0 || (n > 0 && (!requests || !arrival_ms || !latency_ms)))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  const Clock::time_point t0 = Clock::now();
+  std::unordered_map<int64_t, int32_t> index;  // tag -> request
+  std::vector<fclx_completion> buf(256);
+  int32_t got = 0, next = 0;
+  int status = FCLX_OK;
+  auto collect = [&](int32_t wait) {
+    int32_t k = 0;
+    fclx_server_poll(S, buf.data(), static_cast<int32_t>(buf.size()), wait, &k);
+    for (int32_t i = 0; i < k; ++i) {
+      auto it = index.find(buf[i].tag);
+      if (it == index.end()) continue;
+      latency_ms[it->second] = buf[i].latency_ms;
+      if (buf[i].status != FCLX_OK) status = buf[i].status;
+      ++got;
+    }
+  };
+  while (next < n) {
+    const auto due = t0 + std::chrono::duration_cast<Clock::duration>(
+                              std::chrono::duration<double, std::milli>(arrival_ms[next]));
+    if (Clock::now() < due) {
+      collect(0);
+      std::this_thread::sleep_until(due);
+      continue;
+    }
+    fclx_request r = requests[next];
+    r.tag = next;
+    index[next] = next;
+    const int rc = fclx_server_submit(S, &r);
+    if (rc != FCLX_OK) return rc;
+    ++next;
+  }
+  while (got < n) collect(100);
+  if (elapsed_ms) *elapsed_ms = ms_since(t0);
+  return status;
+}
+
+int fclx_server_stats(fclx_server* S, int64_t* batches, int64_t* queries) {
+  if (!S) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(S->mu);
+  if (batches) *batches = S->batches;
+  if (queries) *queries = S->queries;
+  return FCLX_OK;
+}
+
+int fclx_server_destroy(fclx_server* S) {
+  if (!S) return FCLX_ERR_INVALID_ARGUMENT;
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  (void)hipSetDevice(S->device);
+  for (Slot& s : S->slots) {
+    free_slot_buffers(s);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  delete S;
+  return FCLX_OK;
+}
+
+}  // extern "C"

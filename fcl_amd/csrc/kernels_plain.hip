@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) collide_plain_kernel(CollideLaunch L) {
                 np[1] = nw.y;
                 np[2] = nw.z;
                 L.contact_depth[slot] = depth;
-              } else {
+              } else if (L.pairs) {  // no buffer: counts only
                 atomicOr(L.status, kStatusPairOverflow);
               }
             }
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) collide_plain_kernel(CollideLaunch L) {
               L.pairs[3 * slot] = q;
               L.pairs[3 * slot + 1] = id1;
               L.pairs[3 * slot + 2] = id2;
-            } else {
+            } else if (L.pairs) {  // no buffer: counts only
               atomicOr(L.status, kStatusPairOverflow);
             }
             ++count;

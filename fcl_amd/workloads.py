@@ -129,3 +129,136 @@ def primitive_batch(n_pairs=4_000_000, seed=0x5A9E, lo=0.05, hi=1.0, convex_poin
     return {"shapes": rec, "vertices": pts.reshape(-1, 3), "tf1": tf1, "tf2": tf2,
             "pair_shapes": np.arange(n, dtype=np.int32).reshape(n_pairs, 2), "cap": 1,
             "enable_contact": True}
+
+
+# ------------------------------------------------------------------ serving
+def box_mesh(center, size, yaw):
+    """A box as 8 vertices and 12 triangles (outward), turned by yaw about z."""
+    c, s = np.cos(yaw), np.sin(yaw)
+    sx, sy, sz = np.asarray(size, float) / 2
+    corners = np.array([[x, y, z] for x in (-sx, sx) for y in (-sy, sy) for z in (-sz, sz)])
+    R = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+    V = corners @ R.T + np.asarray(center, float)
+    T = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                  [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.int32)
+    return V, T
+
+
+def serving_scene(grid=400, n_boxes=15133, extent=4.0, seed=0x5CE7E):
+    """One mesh: a grid x grid heightfield over [-extent, extent]^2 (heights
+    U(-0.04, 0.04) * extent) and n_boxes boxes (sides U[0.05, 0.25], on the
+    ground plane, random yaw).  grid 400 and 15,133 boxes give 318,402 +
+    181,596 = 499,998 triangles."""
+    rng = np.random.default_rng(seed)
+    V, T = heightfield(grid, 0.02, int(rng.integers(1, 2**62)))
+    V = V * extent
+    Vs, Ts, off = [V], [T], len(V)
+    sizes = rng.uniform(0.05, 0.25, (n_boxes, 3))
+    xy = rng.uniform(-0.95 * extent, 0.95 * extent, (n_boxes, 2))
+    yaw = rng.uniform(0, 2 * np.pi, n_boxes)
+    for i in range(n_boxes):
+        bv, bt = box_mesh((xy[i, 0], xy[i, 1], sizes[i, 2] / 2), sizes[i], yaw[i])
+        Vs.append(bv)
+        Ts.append(bt + off)
+        off += 8
+    return np.ascontiguousarray(np.concatenate(Vs)), np.ascontiguousarray(np.concatenate(Ts))
+
+
+def capsule_mesh(radius, length, n_tris, noise=0.1, seed=0):
+    """A capsule along z (segment `length`, radius `radius`) tessellated into
+    about n_tris triangles: rings along the stadium profile between two pole
+    vertices; every vertex's distance from the axis is scaled by
+    1 + U(-noise, noise)."""
+    rng = np.random.default_rng(seed)
+    m = max(8, int(round(np.sqrt(n_tris))))          # vertices around
+    rings = max(3, int(round(n_tris / (2.0 * m))))   # interior rings
+    half_arc = np.pi * radius / 2
+    total = 2 * half_arc + length
+    s = np.arange(1, rings + 1) / (rings + 1) * total
+    rho = np.empty(rings)
+    z = np.empty(rings)
+    for i, si in enumerate(s):
+        if si < half_arc:
+            a = si / radius
+            rho[i], z[i] = radius * np.sin(a), -length / 2 - radius * np.cos(a)
+        elif si <= half_arc + length:
+            rho[i], z[i] = radius, -length / 2 + (si - half_arc)
+        else:
+            a = (si - half_arc - length) / radius
+            rho[i], z[i] = radius * np.cos(a), length / 2 + radius * np.sin(a)
+    th = np.arange(m) * (2 * np.pi / m)
+    scale = 1 + rng.uniform(-noise, noise, (rings, m))
+    X = (rho[:, None] * scale) * np.cos(th)[None, :]
+    Y = (rho[:, None] * scale) * np.sin(th)[None, :]
+    Z = np.repeat(z[:, None], m, 1)
+    ring_v = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    V = np.concatenate([[[0, 0, -length / 2 - radius]], ring_v, [[0, 0, length / 2 + radius]]])
+    top = len(V) - 1
+    tris = []
+    j = np.arange(m)
+    jn = (j + 1) % m
+    tris.append(np.stack([np.zeros(m, int), 1 + jn, 1 + j], 1))       # bottom fan
+    for r in range(rings - 1):
+        a0, a1 = 1 + r * m, 1 + (r + 1) * m
+        tris.append(np.stack([a0 + j, a0 + jn, a1 + jn], 1))
+        tris.append(np.stack([a0 + j, a1 + jn, a1 + j], 1))
+    last = 1 + (rings - 1) * m
+    tris.append(np.stack([last + j, last + jn, np.full(m, top)], 1))  # top fan
+    return np.ascontiguousarray(V, np.float64), np.ascontiguousarray(np.concatenate(tris), np.int32)
+
+
+def serving_robot(n_links=20, lo=1000, hi=50000, seed=0x40B07):
+    """20 capsule-like link meshes, triangle counts log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    links = []
+    for i in range(n_links):
+        nt = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        links.append(capsule_mesh(rng.uniform(0.03, 0.08), rng.uniform(0.15, 0.4), nt, 0.1,
+                                  int(rng.integers(1, 2**62))))
+    return links
+
+
+def _small_rotations(rng, n, sigma):
+    """n rotation matrices about uniform random axes by angles N(0, sigma)."""
+    ax = rng.standard_normal((n, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    ang = rng.normal(0, sigma, n)
+    K = np.zeros((n, 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -ax[:, 2], ax[:, 1], -ax[:, 0]
+    K[:, 1, 0], K[:, 2, 0], K[:, 2, 1] = ax[:, 2], -ax[:, 1], ax[:, 0]
+    s, c = np.sin(ang)[:, None, None], np.cos(ang)[:, None, None]
+    return np.eye(3)[None] + s * K + (1 - c) * (K @ K)
+
+
+def serving_request(n_links, n_waypoints, rng, extent=4.0, sigma_t=0.01, sigma_r=0.02):
+    """Link poses (n_waypoints, n_links, 12), waypoint-major: each link
+    starts near a common base (x, y in [-0.85, 0.85] * extent, z in
+    [0.05, 0.45]) with a uniform random orientation and moves by a random
+    walk: translation steps N(0, sigma_t) per axis, rotation steps about a
+    random axis by N(0, sigma_r) rad."""
+    base = np.array([*rng.uniform(-0.85 * extent, 0.85 * extent, 2), rng.uniform(0.05, 0.45)])
+    t = base + rng.uniform(-0.4, 0.4, (n_links, 3))
+    t[:, 2] = np.clip(t[:, 2], -0.05, None)
+    q = rng.standard_normal((n_links, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+                 1).reshape(n_links, 3, 3)
+    out = np.empty((n_waypoints, n_links, 12))
+    steps_t = rng.normal(0, sigma_t, (n_waypoints, n_links, 3))
+    for k in range(n_waypoints):
+        if k:
+            t = t + steps_t[k]
+            R = _small_rotations(rng, n_links, sigma_r) @ R
+        out[k, :, :9] = R.reshape(n_links, 9)
+        out[k, :, 9:] = t
+    return out
+
+
+def serving_requests(n, n_links=20, lo=64, hi=4096, seed=0x5E4E):
+    """n requests: waypoint counts log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    lens = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(int)
+    return [np.ascontiguousarray(serving_request(n_links, int(L), rng)) for L in lens]

@@ -304,6 +304,67 @@ int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_
                   const double tf2[12], int32_t enable_nearest_points, double* min_distance,
                   double* points, int32_t* ids, int64_t* stats);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop over one resident scene (no counterpart in the reference,
+ * whose collide() is one synchronous call): planner requests -- a robot's
+ * link meshes and a trajectory of link poses -- are queued as they arrive
+ * and run as first-contact collide() queries (CollisionRequest(1, false),
+ * the scene as the first object), each link against the scene at each
+ * waypoint.  A worker thread forms a batch from all waiting requests that
+ * fit a free stream slot and the device-memory budget, so requests that
+ * arrive while others are on the GPU join the next batch and batches on
+ * different streams overlap on the device.  A request's result does not
+ * depend on what it is batched with. */
+typedef struct fclx_server fclx_server;
+
+typedef struct {
+  int32_t scene_mesh;          /* OBBRSS mesh id of the scene */
+  int32_t num_streams;         /* batches in flight at once, 1..8 */
+  double scene_pose[12];       /* the scene's pose */
+  int64_t hbm_budget;          /* device bytes of requests in flight at once; a
+                                  query takes 204 B (0: 2 GiB) */
+  int32_t max_batch_queries;   /* queries per batch at most (0: 1<<20); a
+                                  request larger than this runs alone */
+  int32_t engine;              /* FCLX_ENGINE_FAST or FCLX_ENGINE_PLAIN */
+} fclx_server_config;
+
+typedef struct {
+  int32_t num_links;
+  int32_t num_waypoints;
+  const int32_t* link_meshes;  /* host: a mesh id (OBBRSS) per link */
+  const double* link_poses;    /* host: num_waypoints x num_links poses (12 each),
+                                  waypoint-major, world frame */
+  uint8_t* result;             /* host: num_waypoints x num_links, written before
+                                  the completion is reported: 1 = collides */
+  int64_t tag;                 /* returned with the completion */
+} fclx_request;
+/* The request's arrays must stay valid until its completion is reported. */
+
+typedef struct {
+  int64_t tag;
+  int32_t status;              /* FCLX_OK or an FCLX_ERR_* code */
+  int32_t num_queries;
+  double queue_ms;             /* submission to launch */
+  double latency_ms;           /* submission to results in host memory */
+} fclx_completion;
+
+int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_server** out);
+/* Queues a request; returns at once. */
+int fclx_server_submit(fclx_server* srv, const fclx_request* req);
+/* Up to max completions; waits up to wait_ms (0: no wait) for the first. */
+int fclx_server_poll(fclx_server* srv, fclx_completion* out, int32_t max, int32_t wait_ms,
+                     int32_t* n);
+/* Load replay for measurement: submits requests[i] at arrival_ms[i] after
+ * the call (ascending), waits for all of them, and gives each request's
+ * latency (submission to results) and the wall time of the whole replay. */
+int fclx_server_replay(fclx_server* srv, const fclx_request* requests,
+                       const double* arrival_ms, int32_t n, double* latency_ms,
+                       double* elapsed_ms);
+/* Totals since creation: batches launched, queries run. */
+int fclx_server_stats(fclx_server* srv, int64_t* batches, int64_t* queries);
+/* Finishes every queued request, then stops the worker and frees it. */
+int fclx_server_destroy(fclx_server* srv);
+
 #ifdef __cplusplus
 }
 #endif
