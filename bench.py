@@ -226,12 +226,121 @@ def bench_primitives(args):
     ctx.close()
 
 
+def bench_serving(args):
+    """Robot-trajectory serving: one resident ~500k-triangle scene, a
+    20-link robot, requests of 64-4096 waypoints arriving as Poisson
+    streams; per load the p50 / p99 request latency (arrival to results in
+    host memory) and the queries/s the loop sustained."""
+    import numpy as np
+
+    import fcl_amd as F
+    from fcl_amd import workloads
+
+    t0 = time.time()
+    sV, sT = workloads.serving_scene()
+    links = workloads.serving_robot()
+    pool = workloads.serving_requests(args.serve_pool)
+    log("serving workload: scene %d tris, %d links (%d-%d tris), pool of %d requests "
+        "(mean %.0f waypoints), generated in %.1fs"
+        % (len(sT), len(links), min(len(l[1]) for l in links), max(len(l[1]) for l in links),
+           len(pool), np.mean([P.shape[0] for P in pool]), time.time() - t0))
+    ctx = F.Context(0)
+    for kv in args.engine_param:
+        k, v = kv.split("=", 1)
+        ctx.set_engine_param(k, float(v))
+    t0 = time.time()
+    scene = ctx.mesh(sV, sT)
+    link_ids = np.array([ctx.mesh(V, T) for V, T in links], np.int32)
+    log("uploaded BVHs in %.1fs" % (time.time() - t0))
+    engine = {"bool": F.ENGINE_BOOL, "fast": F.ENGINE_FAST, "plain": F.ENGINE_PLAIN}[
+        args.serve_engine]
+    srv = F.Server(ctx, scene, num_streams=args.serve_streams, engine=engine)
+    nlink = len(links)
+    srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
+    rng = np.random.default_rng(11)
+    loads = []
+    for rate in args.serve_rates:
+        gaps = rng.exponential(1000.0 / rate, int(rate * args.serve_seconds * 1.5) + 16)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals < args.serve_seconds * 1000.0]
+        n = len(arrivals)
+        idx = rng.integers(0, len(pool), n)
+        reqs = [(link_ids, pool[i]) for i in idx]
+        nq = np.array([pool[i].shape[0] * nlink for i in idx])
+        b0, q0 = srv.stats()
+        _, lat, elapsed = srv.replay(reqs, arrivals)
+        b1, q1 = srv.stats()
+        p50, p99 = np.percentile(lat, [50, 99])
+        loads.append({"rate_per_s": rate, "requests": n, "queries": int(nq.sum()),
+                      "p50_ms": float(p50), "p99_ms": float(p99), "max_ms": float(lat.max()),
+                      "queries_per_s": float(nq.sum() / (elapsed / 1e3)),
+                      "offered_queries_per_s": float(rate * nq.mean()),
+                      "batches": b1 - b0, "elapsed_ms": elapsed})
+        log("rate %5d/s: %5d requests, p50 %.2f ms, p99 %.2f ms, %.3g queries/s, %d batches"
+            % (rate, n, p50, p99, nq.sum() / (elapsed / 1e3), b1 - b0))
+    # one request alone, the largest of the pool
+    big = max(pool, key=lambda P: P.shape[0])
+    _, lat1, _ = srv.replay([(link_ids, big)], [0.0])
+    srv.close()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        threads = host_threads()
+        om = [fclo.OracleMesh(sV, sT)] + [fclo.OracleMesh(V, T) for V, T in links]
+        P = np.concatenate([p.reshape(-1, 12) for p in pool[:4]])
+        nq = len(P)
+        qm = np.stack([np.zeros(nq, np.int32),
+                       1 + np.tile(np.arange(nlink, dtype=np.int32), nq // nlink)], 1)
+        tf1 = np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float), (nq, 1))
+        a = time.perf_counter()
+        fclo.collide_batch(om, qm, tf1, P, 1, store=0, threads=threads)
+        cpu_s = time.perf_counter() - a
+        cpu = {"value": nq / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "the %d link-waypoint queries of 4 pool requests (collide, cap 1), "
+                         "OpenMP dynamic schedule" % nq}
+        log("cpu oracle: %d queries in %.1fs on %d threads" % (nq, cpu_s, threads))
+    top = loads[-1]
+    out = {
+        "metric": "robot-trajectory serving: p99 request latency at Poisson load",
+        "value": top["p99_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f32 cull + f64 leaves",
+        "data": "synthetic (seeded heightfield + boxes scene, capsule-like links, random-walk "
+                "trajectories)",
+        "config": {"workload": "serving: 500k-tri scene, 20 links of 1k-50k tris, 64-4096 "
+                               "waypoints log-uniform, CollisionRequest(1, false) per link and "
+                               "waypoint", "streams": args.serve_streams,
+                   "engine": args.serve_engine, "engine_params": args.engine_param,
+                   "rate_per_s": top["rate_per_s"], "pool": len(pool)},
+        "loads": loads,
+        "largest_request_alone_ms": float(lat1[0]),
+        "largest_request_queries": int(big.shape[0] * nlink),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["mesh", "primitives"], default="mesh",
+    ap.add_argument("--workload", choices=["mesh", "primitives", "serving"], default="mesh",
                     help="mesh: the batched uneven mesh collision (default); primitives: "
-                         "4M primitive-shape pairs")
+                         "4M primitive-shape pairs; serving: robot-trajectory requests at "
+                         "Poisson loads")
+    ap.add_argument("--serve-rates", type=float, nargs="+",
+                    default=[100, 500, 1000, 2000, 5000])
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrival window per load")
+    ap.add_argument("--serve-streams", type=int, default=3)
+    ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--serve-engine", choices=["bool", "fast", "plain"], default="bool",
+                    help="engine behind the request loop (bool: booleans only)")
     ap.add_argument("--prim-pairs", type=int, default=4_000_000)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -257,6 +366,9 @@ def main():
 
     if args.workload == "primitives":
         bench_primitives(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     import numpy as np

@@ -16,6 +16,7 @@ OBBRSS = N.FCLX_BV_OBBRSS
 RSS = N.FCLX_BV_RSS
 ENGINE_PLAIN = N.FCLX_ENGINE_PLAIN
 ENGINE_FAST = N.FCLX_ENGINE_FAST
+ENGINE_BOOL = N.FCLX_ENGINE_BOOL
 BOX = N.FCLX_SHAPE_BOX
 SPHERE = N.FCLX_SHAPE_SPHERE
 CAPSULE = N.FCLX_SHAPE_CAPSULE
@@ -347,7 +348,11 @@ class Context:
     def engine_report(self):
         r = N.EngineReport()
         self._check(self._L.fclx_engine_last_report(self._h, ctypes.byref(r)))
-        return {f: getattr(r, f) for f, _ in N.EngineReport._fields_}
+        out = {}
+        for f, _ in N.EngineReport._fields_:
+            v = getattr(r, f)
+            out[f] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
 
     def sync(self, stream=None):
         return self._check(self._L.fclx_sync(self._h, stream))
@@ -364,7 +369,7 @@ class Server:
     uint8 array, 1 where the link collides with the scene."""
 
     def __init__(self, ctx, scene_mesh, scene_pose=None, num_streams=2, hbm_budget=0,
-                 max_batch_queries=0, engine=ENGINE_FAST):
+                 max_batch_queries=0, engine=ENGINE_BOOL):
         self._ctx = ctx
         self._L = ctx._L
         cfg = N.ServerConfig()

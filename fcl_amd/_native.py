@@ -27,6 +27,7 @@ FCLX_SHAPE_CYLINDER = 3
 FCLX_SHAPE_CONVEX = 4
 FCLX_ENGINE_PLAIN = 0
 FCLX_ENGINE_FAST = 1
+FCLX_ENGINE_BOOL = 2
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
@@ -99,7 +100,8 @@ class EngineReport(ctypes.Structure):
                 ("fp64_fallbacks", c_i64), ("front_size", c_i64), ("levels", c_i32),
                 ("flagged", c_i32), ("dfs_waves", c_i32), ("dfs_rounds", c_i32),
                 ("dfs_wave_busy", c_dbl), ("dfs_end_p50_ms", c_dbl), ("dfs_end_p90_ms", c_dbl),
-                ("dfs_end_max_ms", c_dbl)]
+                ("dfs_end_max_ms", c_dbl), ("bool_steps", c_i64 * 3), ("bool_lanes", c_i64 * 3),
+                ("bool_skipped", c_i64), ("bool_rounds", c_i64)]
 
 
 # name -> (restype, argtypes)

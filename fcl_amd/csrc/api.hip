@@ -480,6 +480,14 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     L.contact_normal = a->contact_normal;
     L.contact_depth = a->contact_depth;
     e = launch_collide_plain(L, 2 * depth, s);
+  } else if (a->engine == FCLX_ENGINE_BOOL) {
+    if (nodes >= kFastMaxNodes || a->num_queries >= (1 << 22)) {
+      L.cap = 1;  // min(count, 1)
+      e = launch_collide_plain(L, 2 * depth, s);
+    } else {
+      std::lock_guard<std::mutex> lock(ctx->ws_mu);
+      e = engine_anyhit_collide(ctx->ws, L, 2 * depth, s);
+    }
   } else if (a->engine == FCLX_ENGINE_PLAIN ||
              (a->engine == FCLX_ENGINE_FAST && a->num_max_contacts > kOrderedCapMax &&
               nodes >= kFastMaxNodes)) {
@@ -698,6 +706,12 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.stop_poll = static_cast<int>(value);
   } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
     t.dfs_waves_per_simd = static_cast<int>(value);
+  } else if (n == "bool_queries_per_wave" && (value == 8 || value == 16)) {
+    t.bool_queries_per_wave = static_cast<int>(value);
+  } else if (n == "bool_waves_per_simd" && (value == 3 || value == 4)) {
+    t.bool_waves_per_simd = static_cast<int>(value);
+  } else if (n == "bool_leaf_batch" && in(1, 64)) {
+    t.bool_leaf_batch = static_cast<int>(value);
   } else if (n == "epa_blocks_per_cu" && in(0, 32)) {
     std::lock_guard<std::mutex> sl(ctx->sws_mu);
     ctx->epa_blocks_per_cu = static_cast<int>(value);
@@ -732,6 +746,12 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   out->dfs_end_p50_ms = r.dfs_end_p50_ms;
   out->dfs_end_p90_ms = r.dfs_end_p90_ms;
   out->dfs_end_max_ms = r.dfs_end_max_ms;
+  for (int k = 0; k < 3; ++k) {
+    out->bool_steps[k] = r.bool_steps[k];
+    out->bool_lanes[k] = r.bool_lanes[k];
+  }
+  out->bool_skipped = r.bool_skipped;
+  out->bool_rounds = r.bool_rounds;
   return FCLX_OK;
 }
 

@@ -14,6 +14,10 @@ void engine_workspace_destroy(EngineWorkspace* ws);
 // breadth-first front expansion + persistent wavefront DFS, and re-run any
 // query whose contact count reaches the cap with the plain kernel so that
 // its truncated list is the reference's depth-first prefix.
+// Booleans only: counts[q] = 1 when the query's pair collides, else 0; no
+// records or statistics.  Batches on different streams may overlap.
+hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
+                                 hipStream_t s);
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                           hipStream_t s);
 
@@ -27,6 +31,9 @@ struct EngineTuning {
   double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
   int stop_poll = 16;           // depth-first steps between polls of the round's stop flag
   int dfs_waves_per_simd = 4;   // register budget of the depth-first kernel (4..6)
+  int bool_queries_per_wave = 16;  // queries a boolean-engine wavefront works on at once (8, 16)
+  int bool_waves_per_simd = 4;    // register budget of the boolean kernel (3, 4)
+  int bool_leaf_batch = 64;       // leaf pairs queued before the boolean kernel tests them
 };
 EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 
@@ -39,6 +46,10 @@ struct EngineReport {
   int levels, flagged;
   int dfs_waves, dfs_rounds;
   double dfs_wave_busy, dfs_end_p50_ms, dfs_end_p90_ms, dfs_end_max_ms;
+  // BOOL engine (profiling): wavefront steps and the lanes they used, per
+  // kind (FP32 BV, FP64 triangle, FP64 BV); tasks dropped because their
+  // query was answered; query rounds
+  long long bool_steps[3], bool_lanes[3], bool_skipped, bool_rounds;
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
 const EngineReport& engine_report(const EngineWorkspace* ws);

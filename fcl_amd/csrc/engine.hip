@@ -855,6 +855,256 @@ __global__ void zero_u64(unsigned long long* p, int n) {
   if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
 
+// ------------------------------------------------- booleans (any hit)
+// Whether each query's pair collides (count 1 or 0, no records).  That
+// answer does not depend on the order in which the BVTT is visited, so a
+// wavefront works on kAnyQ queries at once: their records sit in LDS, their
+// tasks share one LIFO stack in device memory, 64 tasks are tested per step
+// (FP32 cull, FP64 when unsure, the reference's descent rule), and a
+// query's remaining tasks are dropped once one of its leaf pairs
+// intersects.  Queries go to wavefronts by a static interleave, so no
+// counter is shared between wavefronts.
+constexpr int kAnyWaves = kBlock / kWave;
+
+struct AnyQuery {
+  QueryXf32 X;
+  QueryPtr P;
+  QueryXf64 x;
+};
+
+__device__ __forceinline__ Visit visit_any(const CollideLaunch& L, const AnyQuery& Q, int q,
+                                           int b1, int b2) {
+  const ObbNode32& n1 = Q.P.n1[b1];
+  const ObbNode32& n2 = Q.P.n2[b2];
+  Visit v;
+  int r = obb_cull32(Q.X, n1, n2);
+  if (r == kCullUnsure) {
+    v.fallback = true;
+    M3 R;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R.m[i] = Q.x.R[i];
+    r = obb_overlap64(R, v3(Q.x.T[0], Q.x.T[1], Q.x.T[2]), L.meshes[Q.X.m1].obb64[b1],
+                      L.meshes[Q.X.m2].obb64[b2])
+            ? kCullOverlap
+            : kCullDisjoint;
+  }
+  if (r == kCullOverlap) {
+    const int fc1 = n1.first_child, fc2 = n2.first_child;
+    if (fc1 < 0 && fc2 < 0) {
+      v.leafc = true;
+      v.c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));
+    } else {
+      v.push = true;
+      if ((fc2 < 0) || (fc1 >= 0 && (Q.P.s1[b1] > Q.P.s2[b2]))) {
+        v.c0 = pack_task(q, fc1, b2);
+        v.c1 = pack_task(q, fc1 + 1, b2);
+      } else {
+        v.c0 = pack_task(q, b1, fc2);
+        v.c1 = pack_task(q, b1, fc2 + 1);
+      }
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool leaf_any(const AnyQuery& Q, int id1, int id2) {
+  const double* p = Q.x.tv1 + kTriVertsStride * static_cast<size_t>(id1);
+  const double* r = Q.x.tv2 + kTriVertsStride * static_cast<size_t>(id2);
+  const V3 P1 = v3(p[0], p[1], p[2]), P2 = v3(p[3], p[4], p[5]), P3 = v3(p[6], p[7], p[8]);
+  const V3 Q1 = v3(r[0], r[1], r[2]), Q2 = v3(r[3], r[4], r[5]), Q3 = v3(r[6], r[7], r[8]);
+  M3 R;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R.m[i] = Q.x.R[i];
+  return tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, v3(Q.x.T[0], Q.x.T[1], Q.x.T[2]));
+}
+
+// Children or deferred work of one BV task after the FP32 cull.
+enum : int { kAnyNone = 0, kAnyPush, kAnyLeaf, kAnyUnsure };
+
+__device__ __forceinline__ int cull_any(const AnyQuery& Q, int q, int b1, int b2,
+                                        unsigned long long& c0, unsigned long long& c1) {
+  const ObbNode32& n1 = Q.P.n1[b1];
+  const ObbNode32& n2 = Q.P.n2[b2];
+  const int r = obb_cull32(Q.X, n1, n2);
+  if (r == kCullUnsure) return kAnyUnsure;
+  if (r == kCullDisjoint) return kAnyNone;
+  const int fc1 = n1.first_child, fc2 = n2.first_child;
+  if (fc1 < 0 && fc2 < 0) {
+    c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));
+    return kAnyLeaf;
+  }
+  if ((fc2 < 0) || (fc1 >= 0 && (Q.P.s1[b1] > Q.P.s2[b2]))) {
+    c0 = pack_task(q, fc1, b2);
+    c1 = pack_task(q, fc1 + 1, b2);
+  } else {
+    c0 = pack_task(q, b1, fc2);
+    c1 = pack_task(q, b1, fc2 + 1);
+  }
+  return kAnyPush;
+}
+
+// A wavefront's steps come in three kinds, one kind per step for all its
+// lanes: up to 64 FP32 BV tests from the stack; up to 64 FP64 triangle
+// tests from the leaf queue; up to 64 FP64 BV tests of the pairs the FP32
+// cull left unsure.  The queues (LDS, 128 entries) are worked off once they
+// hold leaf_batch (<= 64) leaf pairs / 64 unsure pairs or the stack is
+// empty, the leaf queue first, so that no lane waits on another's FP64
+// work; a smaller leaf batch finds a query's first hit sooner.
+constexpr int kAnyQueue = 128;
+
+template <int kAnyQ, int kWavesPerSimd>
+__global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLaunch L,
+                                                                       unsigned long long* stacks,
+                                                                       int stack_cap,
+                                                                       int leaf_batch,
+                                                                       unsigned long long* prof) {
+  __shared__ AnyQuery qs[kAnyWaves][kAnyQ];
+  __shared__ int found_s[kAnyWaves][kAnyQ];
+  __shared__ unsigned long long lq_s[kAnyWaves][kAnyQueue];
+  __shared__ unsigned long long uq_s[kAnyWaves][kAnyQueue];
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const long long gw = static_cast<long long>(blockIdx.x) * kAnyWaves + wave;
+  const long long nw = static_cast<long long>(gridDim.x) * kAnyWaves;
+  unsigned long long* stack = stacks + gw * stack_cap;
+  AnyQuery* Qs = qs[wave];
+  int* found = found_s[wave];
+  unsigned long long* lq = lq_s[wave];
+  unsigned long long* uq = uq_s[wave];
+  const long long nq = L.num_queries;
+  for (long long q0 = gw * kAnyQ; q0 < nq; q0 += nw * kAnyQ) {
+    const int nk = static_cast<int>(nq - q0 < kAnyQ ? nq - q0 : kAnyQ);
+    bool root = false;
+    if (lane < nk) {
+      const int q = static_cast<int>(q0) + lane;
+      const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
+      found[lane] = 0;
+      if (query_ok(L.meshes, L.num_meshes, i1, i2)) {
+        AnyQuery& Q = Qs[lane];
+        rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q),
+                   Q.x.R, Q.x.T);
+        const MeshDev& m1 = L.meshes[i1];
+        const MeshDev& m2 = L.meshes[i2];
+        Q.x.tv1 = m1.tri_verts;
+        Q.x.tv2 = m2.tri_verts;
+        make_xf32(Q.x.R, Q.x.T, m1.cmax, m2.cmax, Q.X);
+        Q.X.m1 = i1;
+        Q.X.m2 = i2;
+        Q.X.valid = 1;
+        Q.P = QueryPtr{m1.obb32, m2.obb32, m1.size64, m2.size64};
+        root = true;
+      } else {
+        atomicOr(L.status, kStatusBadQuery);
+      }
+      if (L.bv_tests) L.bv_tests[q] = 0;
+      if (L.leaf_tests) L.leaf_tests[q] = 0;
+    }
+    const unsigned long long rm = __ballot(root);
+    if (root) stack[__popcll(rm & lanes_below())] = pack_task(static_cast<int>(q0) + lane, 0, 0);
+    int sp = __popcll(rm), nl = 0, nu = 0;
+    unsigned long long st[3] = {0, 0, 0}, ln[3] = {0, 0, 0}, skipped = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    while (sp > 0 || nl > 0 || nu > 0) {
+      // all queries answered: what is left is moot
+      if (__ballot(lane < nk && !found[lane] && ((rm >> lane) & 1)) == 0) break;
+      if (nl >= leaf_batch || (sp == 0 && nl > 0)) {
+        // FP64 triangle tests
+        const int n = nl < kWave ? nl : kWave;
+        nl -= n;
+        bool skip = false;
+        if (lane < n) {
+          int q, t1, t2;
+          unpack_task(lq[nl + lane], q, t1, t2);
+          const int slot = q - static_cast<int>(q0);
+          skip = found[slot] != 0;
+          if (!skip && leaf_any(Qs[slot], t1, t2)) found[slot] = 1;
+        }
+        ++st[1];
+        ln[1] += n;
+        skipped += __popcll(__ballot(skip));
+      } else if (nu >= kWave || (sp == 0 && nu > 0)) {
+        // FP64 BV tests of unsure pairs: children to the stack, leaves queued
+        const int n = nu < kWave ? nu : kWave;
+        nu -= n;
+        Visit v;
+        bool skip = false;
+        if (lane < n) {
+          int q, b1, b2;
+          unpack_task(uq[nu + lane], q, b1, b2);
+          const int slot = q - static_cast<int>(q0);
+          skip = found[slot] != 0;
+          if (!skip) v = visit_any(L, Qs[slot], q, b1, b2);
+        }
+        ++st[2];
+        ln[2] += n;
+        skipped += __popcll(__ballot(skip));
+        const unsigned long long pm = __ballot(v.push);
+        const int npush = 2 * __popcll(pm);
+        if (sp + npush > stack_cap) {
+          if (lane == 0) atomicOr(L.status, kStatusStackOverflow);
+          break;  // never write past the stack; the overflow is reported
+        }
+        if (v.push) {
+          const int o = sp + 2 * __popcll(pm & lanes_below());
+          stack[o] = v.c0;
+          stack[o + 1] = v.c1;
+        }
+        sp += npush;
+        const unsigned long long lm = __ballot(v.leafc);
+        if (v.leafc) lq[nl + __popcll(lm & lanes_below())] = v.c0;
+        nl += __popcll(lm);
+      } else {
+        // FP32 BV tests from the stack
+        const int n = sp < kWave ? sp : kWave;
+        sp -= n;
+        int kind = kAnyNone;
+        unsigned long long task = 0, c0 = 0, c1 = 0;
+        bool skip = false;
+        if (lane < n) {
+          int q, b1, b2;
+          task = stack[sp + lane];
+          unpack_task(task, q, b1, b2);
+          const int slot = q - static_cast<int>(q0);
+          skip = found[slot] != 0;
+          if (!skip) kind = cull_any(Qs[slot], q, b1, b2, c0, c1);
+        }
+        ++st[0];
+        ln[0] += n;
+        skipped += __popcll(__ballot(skip));
+        const unsigned long long pm = __ballot(kind == kAnyPush);
+        const int npush = 2 * __popcll(pm);
+        if (sp + npush > stack_cap) {
+          if (lane == 0) atomicOr(L.status, kStatusStackOverflow);
+          break;
+        }
+        if (kind == kAnyPush) {
+          const int o = sp + 2 * __popcll(pm & lanes_below());
+          stack[o] = c0;
+          stack[o + 1] = c1;
+        }
+        sp += npush;
+        const unsigned long long lm = __ballot(kind == kAnyLeaf);
+        const unsigned long long um = __ballot(kind == kAnyUnsure);
+        if (kind == kAnyLeaf) lq[nl + __popcll(lm & lanes_below())] = c0;
+        if (kind == kAnyUnsure) uq[nu + __popcll(um & lanes_below())] = task;
+        nl += __popcll(lm);
+        nu += __popcll(um);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < nk) L.counts[q0 + lane] = found[lane];
+    if (prof && lane == 0) {
+      for (int k = 0; k < 3; ++k) {
+        atomicAdd(prof + k, st[k]);
+        atomicAdd(prof + 3 + k, ln[k]);
+      }
+      atomicAdd(prof + 6, skipped);
+      atomicAdd(prof + 7, 1ULL);
+    }
+  }
+}
+
 // ---------------------------------------------- ordered, small caps
 template <int STACK>
 __global__ void __launch_bounds__(kBlock) ordered_collide(CollideLaunch L,
@@ -1057,6 +1307,13 @@ struct EngineWorkspace {
   // run at once, so each launch takes the next slot of a ring (one cache
   // line each) and waits for the slot's previous launch to end
   static constexpr int kOrdSlots = 16;
+  // stack areas of the boolean kernel, a ring guarded like the counters
+  static constexpr int kAnySlots = 4;
+  unsigned long long* any_stk[kAnySlots] = {};
+  size_t any_cap[kAnySlots] = {};
+  hipEvent_t any_ev[kAnySlots] = {};
+  bool any_used[kAnySlots] = {};
+  int any_next = 0;
   unsigned long long* ord_ctr = nullptr;  // kOrdSlots x 16
   hipEvent_t ord_ev[kOrdSlots] = {};
   bool ord_used[kOrdSlots] = {};
@@ -1076,6 +1333,8 @@ EngineWorkspace* engine_workspace_create() {
                 hipSuccess;
   for (int i = 0; ok && i < EngineWorkspace::kOrdSlots; ++i)
     ok = hipEventCreateWithFlags(&ws->ord_ev[i], hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; ok && i < EngineWorkspace::kAnySlots; ++i)
+    ok = hipEventCreateWithFlags(&ws->any_ev[i], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     engine_workspace_destroy(ws);
     return nullptr;
@@ -1103,6 +1362,11 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
     if (ws->ord_ev[i]) (void)hipEventDestroy(ws->ord_ev[i]);
   }
   (void)hipFree(ws->ord_ctr);
+  for (int i = 0; i < EngineWorkspace::kAnySlots; ++i) {
+    if (ws->any_used[i]) (void)hipEventSynchronize(ws->any_ev[i]);
+    if (ws->any_ev[i]) (void)hipEventDestroy(ws->any_ev[i]);
+    (void)hipFree(ws->any_stk[i]);
+  }
   for (auto& e : ws->ev)
     if (e) (void)hipEventDestroy(e);
   delete ws;
@@ -1462,6 +1726,69 @@ hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long
 }
 
 }  // namespace
+
+hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
+                                 hipStream_t s) {
+  if (!ws) return hipErrorInvalidValue;
+  const long long nq = L.num_queries;
+  if (nq == 0) return hipSuccess;
+  const int cus = device_cus();
+  const int qpw = ws->tune.bool_queries_per_wave;
+  const unsigned grid = static_cast<unsigned>(std::max<long long>(
+      1, std::min<long long>((nq + qpw * kAnyWaves - 1) / (qpw * kAnyWaves),
+                             static_cast<long long>(cus) * 8)));
+  // 64-wide LIFO: a generous bound of 128 entries per BVTT level
+  const int stack_cap = 128 * (stack_need + 2);
+  const size_t need = static_cast<size_t>(grid) * kAnyWaves * stack_cap;
+  const int slot = ws->any_next;
+  ws->any_next = (slot + 1) % EngineWorkspace::kAnySlots;
+  if (need > ws->any_cap[slot]) {
+    if (ws->any_used[slot]) ENG_TRY(hipEventSynchronize(ws->any_ev[slot]));
+    (void)hipFree(ws->any_stk[slot]);
+    ws->any_stk[slot] = nullptr;
+    ws->any_cap[slot] = 0;
+    const size_t cap = std::max<size_t>(need, static_cast<size_t>(cus) * 8 * kAnyWaves * 128 * 66);
+    ENG_TRY(hipMalloc(reinterpret_cast<void**>(&ws->any_stk[slot]),
+                      cap * sizeof(unsigned long long)));
+    ws->any_cap[slot] = cap;
+  } else if (ws->any_used[slot]) {
+    ENG_TRY(hipStreamWaitEvent(s, ws->any_ev[slot], 0));
+  }
+  const int w = ws->tune.bool_waves_per_simd;
+  const int lb = ws->tune.bool_leaf_batch;
+  unsigned long long* prof = nullptr;
+  if (ws->profile) {  // counters of this batch (the batch is waited for)
+    prof = ws->counters + kCtrWork;
+    set_u64<<<1, 1, 0, s>>>(prof, 0);
+    for (int k = 1; k < 8; ++k) set_u64<<<1, 1, 0, s>>>(prof + k, 0);
+  }
+  if (qpw == 16)
+    w == 4 ? engine_anyhit<16, 4><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
+                                                                lb, prof)
+           : engine_anyhit<16, 3><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
+                                                                lb, prof);
+  else
+    w == 4 ? engine_anyhit<8, 4><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
+                                                                lb, prof)
+           : engine_anyhit<8, 3><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
+                                                                lb, prof);
+  ENG_TRY(hipGetLastError());
+  ENG_TRY(hipEventRecord(ws->any_ev[slot], s));
+  ws->any_used[slot] = true;
+  if (prof) {
+    unsigned long long h[8];
+    ENG_TRY(hipMemcpyAsync(h, prof, sizeof(h), hipMemcpyDeviceToHost, s));
+    ENG_TRY(hipStreamSynchronize(s));
+    EngineReport& r = ws->report;
+    for (int k = 0; k < 3; ++k) {
+      r.bool_steps[k] = static_cast<long long>(h[k]);
+      r.bool_lanes[k] = static_cast<long long>(h[3 + k]);
+    }
+    r.bool_skipped = static_cast<long long>(h[6]);
+    r.bool_rounds = static_cast<long long>(h[7]);
+  }
+  return hipSuccess;
+}
 
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                           hipStream_t s) {

@@ -267,7 +267,8 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
   *out = nullptr;
   if (cfg->num_streams < 1 || cfg->num_streams > 8 || cfg->hbm_budget < 0 ||
       cfg->max_batch_queries < 0 ||
-      (cfg->engine != FCLX_ENGINE_FAST && cfg->engine != FCLX_ENGINE_PLAIN))
+      (cfg->engine != FCLX_ENGINE_FAST && cfg->engine != FCLX_ENGINE_PLAIN &&
+       cfg->engine != FCLX_ENGINE_BOOL))
     return FCLX_ERR_INVALID_ARGUMENT;
   fclx_mesh_info info{};
   if (fclx_mesh_get_info(ctx, cfg->scene_mesh, &info) != FCLX_OK ||
@@ -290,6 +291,22 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
       }
       delete S;
       return FCLX_ERR_DEVICE;
+    }
+  }
+  // every slot at its full size now, so that no allocation (pinned host
+  // memory above all) falls inside a request's latency
+  const int64_t share = S->cfg.hbm_budget / (kQueryBytes * cfg->num_streams);
+  const int64_t cap = std::max<int64_t>(1 << 16, std::min<int64_t>(S->cfg.max_batch_queries,
+                                                                  share));
+  for (Slot& s : S->slots) {
+    if (slot_reserve(S, s, cap) != hipSuccess) {
+      for (Slot& t : S->slots) {
+        free_slot_buffers(t);
+        if (t.stream) (void)hipStreamDestroy(t.stream);
+        if (t.done) (void)hipEventDestroy(t.done);
+      }
+      delete S;
+      return FCLX_ERR_OUT_OF_MEMORY;
     }
   }
   S->worker = std::thread(worker_main, S);

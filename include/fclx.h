@@ -60,6 +60,14 @@ extern "C" {
  * engine's task encoding has 21 bits per node index). */
 #define FCLX_ENGINE_PLAIN 0
 #define FCLX_ENGINE_FAST 1
+/* BOOL: only whether each query's pair collides -- counts[q] = 1 or 0, the
+ * same as min(collide() count, 1) for any num_max_contacts -- with no
+ * records and no statistics (bv_tests / leaf_tests are zeroed); a wavefront
+ * works on 8 queries at once and stops a query at its first intersecting
+ * triangle pair.  Batches on different streams may run at once.  Batches
+ * beyond 2^22 queries or over meshes of 2^20 or more triangles run the
+ * PLAIN kernel with a cap of 1 (same answers). */
+#define FCLX_ENGINE_BOOL 2
 
 typedef struct fclx_context fclx_context;
 
@@ -235,6 +243,14 @@ typedef struct {
   double dfs_end_p50_ms;
   double dfs_end_p90_ms;
   double dfs_end_max_ms;
+  /* last BOOL-engine batch (profiling only): wavefront steps and the lanes
+   * they used, per kind (0: FP32 BV tests, 1: FP64 triangle tests, 2: FP64
+   * BV tests); tasks dropped because their query was already answered;
+   * rounds of queries taken by wavefronts */
+  int64_t bool_steps[3];
+  int64_t bool_lanes[3];
+  int64_t bool_skipped;
+  int64_t bool_rounds;
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
@@ -248,6 +264,10 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
  *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
+ *   "bool_queries_per_wave" 8, 16  queries a BOOL-engine wavefront works on
+ *   "bool_waves_per_simd" 3, 4  register budget of the BOOL-engine kernel
+ *   "bool_leaf_batch"   1..64   leaf pairs a BOOL-engine wavefront queues
+ *                               before testing them
  *   "epa_blocks_per_cu" 0..32   resident blocks per CU of the shape path's EPA
  *                               kernel (0: as many as fit); each lane holds
  *                               a 13 KiB EPA block in device memory
@@ -325,7 +345,8 @@ typedef struct {
                                   query takes 204 B (0: 2 GiB) */
   int32_t max_batch_queries;   /* queries per batch at most (0: 1<<20); a
                                   request larger than this runs alone */
-  int32_t engine;              /* FCLX_ENGINE_FAST or FCLX_ENGINE_PLAIN */
+  int32_t engine;              /* FCLX_ENGINE_BOOL (the natural choice: only
+                                  booleans are returned), _FAST or _PLAIN */
 } fclx_server_config;
 
 typedef struct {

@@ -353,3 +353,69 @@ def test_empty_model_rejected(ctx):
     with pytest.raises(F.FclxError) as e:
         ctx.mesh(np.zeros((0, 3)), np.zeros((0, 3), np.int32))
     assert e.value.code == N.FCLX_ERR_EMPTY_MODEL
+
+
+def _bool_gpu(ctx, qm, tf1, tf2, stream=None):
+    n = qm.shape[0]
+    counts = torch.full((n,), -1, dtype=torch.int32, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    d = (_t(qm, torch.int32), _t(tf1, torch.float64), _t(tf2, torch.float64))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(s):
+        ctx.collide_batch_async(d[0], d[1], d[2], 1, counts, None, num, engine=F.ENGINE_BOOL,
+                                stream=s.cuda_stream)
+    return counts, num, d
+
+
+@pytest.mark.parametrize("case", ["c0", "uneven"])
+def test_bool_engine_matches_oracle(ctx, c0_meshes, case):
+    """FCLX_ENGINE_BOOL: counts are exactly (oracle count > 0), on the plain
+    mesh-vs-mesh workload and on uneven pairs (sphere, terrain, small
+    sphere; self pairs included)."""
+    if case == "c0":
+        o, g = c0_meshes
+        n = 3000
+        pairs = np.tile(np.array([0, 1], np.int32), (n, 1))
+        tf1, tf2 = F.poses(n, 0, 2.2, 0, 0xC0)
+    else:
+        (V1, T1), (V2, T2) = F.icosphere(9, 0.15, 21), F.heightfield(24, 0.05, 22)
+        V2 = F.scale_to_radius(V2, 1.0)
+        (V3, T3) = F.icosphere(6, 0.15, 23)
+        meshes = [(V1, T1), (V2, T2), (V3, T3)]
+        o = [fclo.OracleMesh(V, T) for V, T in meshes]
+        g = [ctx.mesh(V, T) for V, T in meshes]
+        n = 4000
+        rng = np.random.default_rng(41)
+        pairs = np.array([(0, 1), (1, 2), (0, 2), (2, 2), (1, 1)], np.int32)[rng.integers(0, 5, n)]
+        tf1, tf2 = F.poses(n, 0, 1.6, 0, 0xC4)
+    oc = fclo.collide_batch(o, pairs, tf1, tf2, 1, store=0)[0]
+    print("case: bool engine %s n=%d" % (case, n), flush=True)
+    counts, num, _ = _bool_gpu(ctx, np.array(g, np.int32)[pairs], tf1, tf2)
+    assert ctx.sync_status(torch.cuda.current_stream().cuda_stream) == 0
+    assert np.array_equal(counts.cpu().numpy(), (oc > 0).astype(np.int32))
+    assert int(num.item()) == 0
+    assert 0.1 < (oc > 0).mean() < 0.9
+
+
+def test_bool_engine_concurrent_streams(ctx, c0_meshes):
+    """Boolean batches on several streams at once give what they give one
+    after the other."""
+    o, g = c0_meshes
+    n = 2000
+    qm = np.tile(np.array(g, np.int32), (n, 1))
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    poses = [F.poses(n, 0, 2.2, 0, 0xD0 + k) for k in range(4)]
+    print("case: bool engine, 4 streams x %d queries" % n, flush=True)
+    alone = []
+    for k in range(4):
+        c, _, _ = _bool_gpu(ctx, qm, poses[k][0], poses[k][1], streams[k])
+        streams[k].synchronize()
+        alone.append(c.cpu().numpy())
+    outs = [_bool_gpu(ctx, qm, poses[k][0], poses[k][1], streams[k]) for k in range(4)]
+    for s in streams:
+        s.synchronize()
+    assert ctx.sync_status(None) == 0
+    for k in range(4):
+        assert np.array_equal(outs[k][0].cpu().numpy(), alone[k])
+        oc = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), poses[k][0], poses[k][1], 1)[0]
+        assert np.array_equal(alone[k], (oc > 0).astype(np.int32))
