@@ -706,10 +706,14 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.stop_poll = static_cast<int>(value);
   } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
     t.dfs_waves_per_simd = static_cast<int>(value);
-  } else if (n == "bool_queries_per_wave" && (value == 8 || value == 16)) {
+  } else if (n == "bool_queries_per_wave" && (value == 8 || value == 16 || value == 32)) {
     t.bool_queries_per_wave = static_cast<int>(value);
-  } else if (n == "bool_waves_per_simd" && (value == 3 || value == 4)) {
+  } else if (n == "bool_waves_per_simd" && in(2, 4)) {
     t.bool_waves_per_simd = static_cast<int>(value);
+  } else if (n == "bool_width" && in(1, 64)) {
+    t.bool_width = static_cast<int>(value);
+  } else if (n == "bool_refill" && in(1, 32)) {
+    t.bool_refill = static_cast<int>(value);
   } else if (n == "bool_leaf_batch" && in(1, 64)) {
     t.bool_leaf_batch = static_cast<int>(value);
   } else if (n == "epa_blocks_per_cu" && in(0, 32)) {
@@ -752,6 +756,11 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   }
   out->bool_skipped = r.bool_skipped;
   out->bool_rounds = r.bool_rounds;
+  out->bool_stack_max = r.bool_stack_max;
+  out->bool_wave_ms = r.bool_wave_ms;
+  out->bool_end_p50_ms = r.bool_end_p50_ms;
+  out->bool_end_p90_ms = r.bool_end_p90_ms;
+  out->bool_span_ms = r.bool_span_ms;
   return FCLX_OK;
 }
 

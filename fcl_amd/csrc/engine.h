@@ -31,9 +31,11 @@ struct EngineTuning {
   double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
   int stop_poll = 16;           // depth-first steps between polls of the round's stop flag
   int dfs_waves_per_simd = 4;   // register budget of the depth-first kernel (4..6)
-  int bool_queries_per_wave = 16;  // queries a boolean-engine wavefront works on at once (8, 16)
-  int bool_waves_per_simd = 4;    // register budget of the boolean kernel (3, 4)
+  int bool_queries_per_wave = 16;  // queries a boolean-engine wavefront keeps in flight (8, 16, 32)
+  int bool_waves_per_simd = 4;    // register budget of the boolean kernel (2, 3, 4; 32 slots: 2, 3)
   int bool_leaf_batch = 64;       // leaf pairs queued before the boolean kernel tests them
+  int bool_width = 32;            // most tasks of one query a boolean-kernel step tests
+  int bool_refill = 4;            // free slots that make a boolean wavefront take new queries
 };
 EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 
@@ -49,7 +51,9 @@ struct EngineReport {
   // BOOL engine (profiling): wavefront steps and the lanes they used, per
   // kind (FP32 BV, FP64 triangle, FP64 BV); tasks dropped because their
   // query was answered; query rounds
-  long long bool_steps[3], bool_lanes[3], bool_skipped, bool_rounds;
+  long long bool_steps[3], bool_lanes[3], bool_skipped, bool_rounds, bool_stack_max;
+  double bool_wave_ms;  // mean wavefront lifetime
+  double bool_end_p50_ms, bool_end_p90_ms, bool_span_ms;  // wavefront ends after the first start
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
 const EngineReport& engine_report(const EngineWorkspace* ws);

@@ -857,251 +857,444 @@ __global__ void zero_u64(unsigned long long* p, int n) {
 
 // ------------------------------------------------- booleans (any hit)
 // Whether each query's pair collides (count 1 or 0, no records).  That
-// answer does not depend on the order in which the BVTT is visited, so a
-// wavefront works on kAnyQ queries at once: their records sit in LDS, their
-// tasks share one LIFO stack in device memory, 64 tasks are tested per step
-// (FP32 cull, FP64 when unsure, the reference's descent rule), and a
-// query's remaining tasks are dropped once one of its leaf pairs
-// intersects.  Queries go to wavefronts by a static interleave, so no
-// counter is shared between wavefronts.
+// answer does not depend on the order in which the BVTT is visited, only on
+// whether some leaf pair intersects, but the work it takes does: a pair
+// that collides is answered at its first intersecting leaf pair, which a
+// narrow depth-first descent reaches soonest, while a wavefront needs 64
+// tasks per step.  So a wavefront keeps kSlots queries in flight, each with
+// a LIFO stack of its own in device memory, and a step tests up to
+// 64 / kSlots tasks from the top of each stack (up to `width` when other
+// stacks run short): every query descends nearly depth first while the
+// lanes stay busy.  A slot whose query is answered takes the next one from
+// the launch's query counter, kSlots queries per atomic.
 constexpr int kAnyWaves = kBlock / kWave;
 
-struct AnyQuery {
-  QueryXf32 X;
-  QueryPtr P;
-  QueryXf64 x;
+// A slot's query in LDS as 16-byte chunks, chunk-major: lanes that read the
+// same field of different slots read neighbouring words (no bank conflicts).
+template <typename T, int kSlots>
+struct SlotRecords {
+  static constexpr int kChunks = sizeof(T) / 16;
+  static_assert(sizeof(T) % 16 == 0, "records are whole 16-byte chunks");
+  uint4 c[kChunks][kSlots];
+  __device__ __forceinline__ void put(int slot, const T& v) {
+    const uint4* p = reinterpret_cast<const uint4*>(&v);
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) c[k][slot] = p[k];
+  }
+  __device__ __forceinline__ T get(int slot) const {
+    union {
+      uint4 u[kChunks];
+      T v;
+    } r;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) r.u[k] = c[k][slot];
+    return r.v;
+  }
 };
 
-__device__ __forceinline__ Visit visit_any(const CollideLaunch& L, const AnyQuery& Q, int q,
+// FP64 test of a BV pair the FP32 cull left unsure; c0/c1 carry `tag` in
+// the query field.
+__device__ __forceinline__ Visit visit_any(const CollideLaunch& L, const QueryXf32& X,
+                                           const QueryPtr& P, const QueryXf64& x, int tag,
                                            int b1, int b2) {
-  const ObbNode32& n1 = Q.P.n1[b1];
-  const ObbNode32& n2 = Q.P.n2[b2];
+  const ObbNode32& n1 = P.n1[b1];
+  const ObbNode32& n2 = P.n2[b2];
   Visit v;
-  int r = obb_cull32(Q.X, n1, n2);
-  if (r == kCullUnsure) {
-    v.fallback = true;
-    M3 R;
+  M3 R;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) R.m[i] = Q.x.R[i];
-    r = obb_overlap64(R, v3(Q.x.T[0], Q.x.T[1], Q.x.T[2]), L.meshes[Q.X.m1].obb64[b1],
-                      L.meshes[Q.X.m2].obb64[b2])
-            ? kCullOverlap
-            : kCullDisjoint;
-  }
-  if (r == kCullOverlap) {
+  for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
+  if (obb_overlap64(R, v3(x.T[0], x.T[1], x.T[2]), L.meshes[X.m1].obb64[b1],
+                    L.meshes[X.m2].obb64[b2])) {
     const int fc1 = n1.first_child, fc2 = n2.first_child;
     if (fc1 < 0 && fc2 < 0) {
       v.leafc = true;
-      v.c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));
+      v.c0 = pack_task(tag, -(fc1 + 1), -(fc2 + 1));
     } else {
       v.push = true;
-      if ((fc2 < 0) || (fc1 >= 0 && (Q.P.s1[b1] > Q.P.s2[b2]))) {
-        v.c0 = pack_task(q, fc1, b2);
-        v.c1 = pack_task(q, fc1 + 1, b2);
+      if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+        v.c0 = pack_task(tag, fc1, b2);
+        v.c1 = pack_task(tag, fc1 + 1, b2);
       } else {
-        v.c0 = pack_task(q, b1, fc2);
-        v.c1 = pack_task(q, b1, fc2 + 1);
+        v.c0 = pack_task(tag, b1, fc2);
+        v.c1 = pack_task(tag, b1, fc2 + 1);
       }
     }
   }
   return v;
 }
 
-__device__ __forceinline__ bool leaf_any(const AnyQuery& Q, int id1, int id2) {
-  const double* p = Q.x.tv1 + kTriVertsStride * static_cast<size_t>(id1);
-  const double* r = Q.x.tv2 + kTriVertsStride * static_cast<size_t>(id2);
+__device__ __forceinline__ bool leaf_any(const QueryXf64& x, int id1, int id2) {
+  const double* p = x.tv1 + kTriVertsStride * static_cast<size_t>(id1);
+  const double* r = x.tv2 + kTriVertsStride * static_cast<size_t>(id2);
   const V3 P1 = v3(p[0], p[1], p[2]), P2 = v3(p[3], p[4], p[5]), P3 = v3(p[6], p[7], p[8]);
   const V3 Q1 = v3(r[0], r[1], r[2]), Q2 = v3(r[3], r[4], r[5]), Q3 = v3(r[6], r[7], r[8]);
   M3 R;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) R.m[i] = Q.x.R[i];
-  return tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, v3(Q.x.T[0], Q.x.T[1], Q.x.T[2]));
+  for (int i = 0; i < 9; ++i) R.m[i] = x.R[i];
+  return tri_intersect64(P1, P2, P3, Q1, Q2, Q3, R, v3(x.T[0], x.T[1], x.T[2]));
 }
 
-// Children or deferred work of one BV task after the FP32 cull.
+// Outcome of one BV task after the FP32 cull: nothing, two children, a leaf
+// pair for the triangle test, or a pair for the FP64 BV test.
 enum : int { kAnyNone = 0, kAnyPush, kAnyLeaf, kAnyUnsure };
 
-__device__ __forceinline__ int cull_any(const AnyQuery& Q, int q, int b1, int b2,
-                                        unsigned long long& c0, unsigned long long& c1) {
-  const ObbNode32& n1 = Q.P.n1[b1];
-  const ObbNode32& n2 = Q.P.n2[b2];
-  const int r = obb_cull32(Q.X, n1, n2);
+__device__ __forceinline__ int cull_any(const QueryXf32& X, const QueryPtr& P, int tag, int b1,
+                                        int b2, unsigned long long& c0, unsigned long long& c1) {
+  const ObbNode32& n1 = P.n1[b1];
+  const ObbNode32& n2 = P.n2[b2];
+  const int r = obb_cull32(X, n1, n2);
   if (r == kCullUnsure) return kAnyUnsure;
   if (r == kCullDisjoint) return kAnyNone;
   const int fc1 = n1.first_child, fc2 = n2.first_child;
   if (fc1 < 0 && fc2 < 0) {
-    c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));
+    c0 = pack_task(tag, -(fc1 + 1), -(fc2 + 1));
     return kAnyLeaf;
   }
-  if ((fc2 < 0) || (fc1 >= 0 && (Q.P.s1[b1] > Q.P.s2[b2]))) {
-    c0 = pack_task(q, fc1, b2);
-    c1 = pack_task(q, fc1 + 1, b2);
+  if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+    c0 = pack_task(tag, fc1, b2);
+    c1 = pack_task(tag, fc1 + 1, b2);
   } else {
-    c0 = pack_task(q, b1, fc2);
-    c1 = pack_task(q, b1, fc2 + 1);
+    c0 = pack_task(tag, b1, fc2);
+    c1 = pack_task(tag, b1, fc2 + 1);
   }
   return kAnyPush;
 }
 
 // A wavefront's steps come in three kinds, one kind per step for all its
-// lanes: up to 64 FP32 BV tests from the stack; up to 64 FP64 triangle
-// tests from the leaf queue; up to 64 FP64 BV tests of the pairs the FP32
-// cull left unsure.  The queues (LDS, 128 entries) are worked off once they
-// hold leaf_batch (<= 64) leaf pairs / 64 unsure pairs or the stack is
-// empty, the leaf queue first, so that no lane waits on another's FP64
-// work; a smaller leaf batch finds a query's first hit sooner.
+// lanes: FP32 BV tests from the slots' stacks; up to 64 FP64 triangle tests
+// from the leaf queue; up to 64 FP64 BV tests of the pairs the FP32 cull
+// left unsure.  The queues (LDS, 128 entries) are worked off once they hold
+// leaf_batch (<= 64) leaf pairs / 64 unsure pairs, or once the stacks
+// would fill less than half the lanes, so that no lane waits on another's
+// FP64 work and no slot waits long on its queued pairs.  Queue entries
+// carry (generation, slot): a slot whose query has a hit is reused at once
+// and the entries its query left behind are dropped when they come up; a
+// slot whose query has none is reused once its stack is empty and none of
+// its entries is queued.
 constexpr int kAnyQueue = 128;
+constexpr int kAnySlotBits = 5;  // slots per wavefront <= 32
+constexpr int kAnyXcds = 8;      // query ranges (one per XCD)
+constexpr int kAnyAreaHead = 16 * kAnyXcds;  // area: range counters, 128 B apart; then the stacks
+constexpr int kAnySlotCap = 1024;            // stack entries per slot (at least)
+constexpr int kAnyGenMask = (1 << (22 - kAnySlotBits)) - 1;
 
-template <int kAnyQ, int kWavesPerSimd>
+template <int kSlots>
+struct AnyWave {
+  SlotRecords<QueryXf32, kSlots> x32;
+  SlotRecords<QueryPtr, kSlots> ptr;
+  SlotRecords<QueryXf64, kSlots> x64;
+  int sp[kSlots];     // stack depth per slot
+  int pend[kSlots];   // queued leaf and unsure entries of the slot's query
+  int found[kSlots];  // 0 open, 1 hit, 2 abandoned (stack full; reported)
+  int gen[kSlots];    // generation of the slot's query
+  int offs[kSlots];   // first lane of each slot in a BV step
+  int base[kSlots];   // stack position of that lane's task
+  unsigned long long lq[kAnyQueue];
+  unsigned long long uq[kAnyQueue];
+  unsigned char lane_slot[kWave];
+};
+
+// Inclusive sum over lanes 0..n-1 (n <= 32; valid in those lanes) with DPP
+// row shifts, and lane n-1's value on the scalar unit.
+template <int n>
+__device__ __forceinline__ int scan_lanes(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  if (n > 2) v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  if (n > 4) v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  if (n > 8) v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  if (n > 16) {
+    const int row0 = __builtin_amdgcn_readlane(v, 15);
+    if (lane_id() >= 16) v += row0;
+  }
+  return v;
+}
+template <int n>
+__device__ __forceinline__ int lane_total(int incl) {
+  return __builtin_amdgcn_readlane(incl, n - 1);
+}
+
+template <int kSlots, int kWavesPerSimd>
 __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLaunch L,
-                                                                       unsigned long long* stacks,
-                                                                       int stack_cap,
-                                                                       int leaf_batch,
-                                                                       unsigned long long* prof) {
-  __shared__ AnyQuery qs[kAnyWaves][kAnyQ];
-  __shared__ int found_s[kAnyWaves][kAnyQ];
-  __shared__ unsigned long long lq_s[kAnyWaves][kAnyQueue];
-  __shared__ unsigned long long uq_s[kAnyWaves][kAnyQueue];
+                                                                       unsigned long long* area,
+                                                                       unsigned long long* stk,
+                                                                       int slot_cap, int wide_cap,
+                                                                       int width,
+                                                                       int leaf_batch, int refill,
+                                                                       unsigned long long* prof,
+                                                                       unsigned long long* wprof) {
+  static_assert(kSlots <= (1 << kAnySlotBits) && kSlots <= 32, "slot index must fit its field");
+  __shared__ AnyWave<kSlots> waves_s[kAnyWaves];
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
+  AnyWave<kSlots>& W = waves_s[wave];
   const long long gw = static_cast<long long>(blockIdx.x) * kAnyWaves + wave;
-  const long long nw = static_cast<long long>(gridDim.x) * kAnyWaves;
-  unsigned long long* stack = stacks + gw * stack_cap;
-  AnyQuery* Qs = qs[wave];
-  int* found = found_s[wave];
-  unsigned long long* lq = lq_s[wave];
-  unsigned long long* uq = uq_s[wave];
-  const long long nq = L.num_queries;
-  for (long long q0 = gw * kAnyQ; q0 < nq; q0 += nw * kAnyQ) {
-    const int nk = static_cast<int>(nq - q0 < kAnyQ ? nq - q0 : kAnyQ);
-    bool root = false;
-    if (lane < nk) {
-      const int q = static_cast<int>(q0) + lane;
-      const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
-      found[lane] = 0;
-      if (query_ok(L.meshes, L.num_meshes, i1, i2)) {
-        AnyQuery& Q = Qs[lane];
-        rel_pose64(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q),
-                   Q.x.R, Q.x.T);
-        const MeshDev& m1 = L.meshes[i1];
-        const MeshDev& m2 = L.meshes[i2];
-        Q.x.tv1 = m1.tri_verts;
-        Q.x.tv2 = m2.tri_verts;
-        make_xf32(Q.x.R, Q.x.T, m1.cmax, m2.cmax, Q.X);
-        Q.X.m1 = i1;
-        Q.X.m2 = i2;
-        Q.X.valid = 1;
-        Q.P = QueryPtr{m1.obb32, m2.obb32, m1.size64, m2.size64};
-        root = true;
-      } else {
-        atomicOr(L.status, kStatusBadQuery);
-      }
-      if (L.bv_tests) L.bv_tests[q] = 0;
-      if (L.leaf_tests) L.leaf_tests[q] = 0;
+  // Queries are split into one contiguous range per XCD (workgroups go to
+  // the XCDs round-robin), each with its counter on a line of its own:
+  // neighbouring queries (consecutive waypoints of a trajectory) touch the
+  // same part of the scene, so an XCD's L2 holds fewer BVH nodes.  A
+  // wavefront whose range is used up takes from the next XCD's.
+  int xcd = static_cast<int>(blockIdx.x & (kAnyXcds - 1));
+  int xcd_left = kAnyXcds;
+  unsigned long long* const stacks = stk + gw * kSlots * static_cast<long long>(slot_cap);
+  const bool slot_lane = lane < kSlots;
+  unsigned long long* const mine = stacks + static_cast<long long>(slot_lane ? lane : 0) * slot_cap;
+  const int nq = L.num_queries;
+  int q = -1;          // slot lanes: the query of this lane's slot
+  int spv = 0;         // slot lanes: its stack depth (W.sp mirrors it)
+  int nx = 0, ne = 0;  // queries taken from the counter, not yet in a slot
+  bool exhausted = false;  // every query of the batch is taken
+  bool check = true;       // some slot may be answered, or free
+  int nl = 0, nu = 0;
+  unsigned long long st[3] = {0, 0, 0}, ln[3] = {0, 0, 0}, skipped = 0, rounds = 0;
+  int sp_max = 0;
+  const unsigned long long t_start = prof ? wall_clock64() : 0;
+  if (slot_lane) W.gen[lane] = 0;
+  // Starts query qq in this lane's slot (false: a bad query).
+  auto start = [&](int qq) {
+    const int i1 = L.query_meshes[2 * qq], i2 = L.query_meshes[2 * qq + 1];
+    if (L.bv_tests) L.bv_tests[qq] = 0;
+    if (L.leaf_tests) L.leaf_tests[qq] = 0;
+    if (!query_ok(L.meshes, L.num_meshes, i1, i2)) {
+      atomicOr(L.status, kStatusBadQuery);
+      return false;
     }
-    const unsigned long long rm = __ballot(root);
-    if (root) stack[__popcll(rm & lanes_below())] = pack_task(static_cast<int>(q0) + lane, 0, 0);
-    int sp = __popcll(rm), nl = 0, nu = 0;
-    unsigned long long st[3] = {0, 0, 0}, ln[3] = {0, 0, 0}, skipped = 0;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    while (sp > 0 || nl > 0 || nu > 0) {
-      // all queries answered: what is left is moot
-      if (__ballot(lane < nk && !found[lane] && ((rm >> lane) & 1)) == 0) break;
-      if (nl >= leaf_batch || (sp == 0 && nl > 0)) {
-        // FP64 triangle tests
-        const int n = nl < kWave ? nl : kWave;
-        nl -= n;
-        bool skip = false;
-        if (lane < n) {
-          int q, t1, t2;
-          unpack_task(lq[nl + lane], q, t1, t2);
-          const int slot = q - static_cast<int>(q0);
-          skip = found[slot] != 0;
-          if (!skip && leaf_any(Qs[slot], t1, t2)) found[slot] = 1;
+    QueryXf64 x;
+    rel_pose64(L.tf1 + 12 * static_cast<size_t>(qq), L.tf2 + 12 * static_cast<size_t>(qq), x.R,
+               x.T);
+    const MeshDev& m1 = L.meshes[i1];
+    const MeshDev& m2 = L.meshes[i2];
+    x.tv1 = m1.tri_verts;
+    x.tv2 = m2.tri_verts;
+    QueryXf32 X;
+    make_xf32(x.R, x.T, m1.cmax, m2.cmax, X);
+    X.m1 = i1;
+    X.m2 = i2;
+    X.valid = 1;
+    W.x32.put(lane, X);
+    W.ptr.put(lane, QueryPtr{m1.obb32, m2.obb32, m1.size64, m2.size64});
+    W.x64.put(lane, x);
+    mine[0] = pack_task(0, 0, 0);
+    W.sp[lane] = 1;
+    W.found[lane] = 0;
+    W.pend[lane] = 0;
+    W.gen[lane] = (W.gen[lane] + 1) & kAnyGenMask;
+    return true;
+  };
+  for (;;) {
+    if (check) {
+      check = false;
+      // retire answered queries: a hit, or nothing left on the stack or queued
+      if (slot_lane && q >= 0) {
+        spv = W.sp[lane];
+        const int f = W.found[lane];
+        if (f != 0 || (spv == 0 && W.pend[lane] == 0)) {
+          L.counts[q] = f == 1 ? 1 : 0;
+          q = -1;
+          spv = 0;
         }
-        ++st[1];
-        ln[1] += n;
-        skipped += __popcll(__ballot(skip));
-      } else if (nu >= kWave || (sp == 0 && nu > 0)) {
-        // FP64 BV tests of unsure pairs: children to the stack, leaves queued
-        const int n = nu < kWave ? nu : kWave;
-        nu -= n;
-        Visit v;
-        bool skip = false;
-        if (lane < n) {
-          int q, b1, b2;
-          unpack_task(uq[nu + lane], q, b1, b2);
-          const int slot = q - static_cast<int>(q0);
-          skip = found[slot] != 0;
-          if (!skip) v = visit_any(L, Qs[slot], q, b1, b2);
+      }
+      const int tasks = lane_total<kSlots>(scan_lanes<kSlots>(spv));
+      unsigned long long freem = __ballot(slot_lane && q < 0);
+      if (!exhausted && freem != 0 && (__popcll(freem) >= refill || tasks < kWave)) {
+        ++rounds;
+        while (freem != 0) {
+          if (nx == ne) {
+            const long long lo = static_cast<long long>(nq) * xcd / kAnyXcds;
+            const long long hi = static_cast<long long>(nq) * (xcd + 1) / kAnyXcds;
+            unsigned long long b = 0;
+            if (lane == 0) b = atomicAdd(area + 16 * xcd, static_cast<unsigned long long>(kSlots));
+            b = __shfl(b, 0);
+            if (static_cast<long long>(b) >= hi - lo) {
+              xcd = (xcd + 1) & (kAnyXcds - 1);
+              if (--xcd_left == 0) {
+                exhausted = true;
+                break;
+              }
+              continue;
+            }
+            nx = static_cast<int>(lo + static_cast<long long>(b));
+            ne = hi - nx < kSlots ? static_cast<int>(hi) : nx + kSlots;
+          }
+          const int r = __popcll(freem & lanes_below());
+          const int n = __popcll(freem) < ne - nx ? __popcll(freem) : ne - nx;
+          if (slot_lane && q < 0 && r < n) {
+            q = nx + r;
+            if (start(q)) {
+              spv = 1;
+            } else {
+              L.counts[q] = 0;
+              q = -1;
+            }
+          }
+          nx += n;
+          freem = __ballot(slot_lane && q < 0);
         }
-        ++st[2];
-        ln[2] += n;
-        skipped += __popcll(__ballot(skip));
-        const unsigned long long pm = __ballot(v.push);
-        const int npush = 2 * __popcll(pm);
-        if (sp + npush > stack_cap) {
-          if (lane == 0) atomicOr(L.status, kStatusStackOverflow);
-          break;  // never write past the stack; the overflow is reported
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      if (exhausted && __ballot(slot_lane && q >= 0) == 0) break;
+    }
+
+    // tasks per slot for a BV step: an even share of the lanes first, then
+    // what is left to the slots with deeper stacks, up to `width`.  A step
+    // that takes t tasks grows the stack by at most t, so a slot keeps its
+    // depth under wide_cap with wide steps and goes one task at a time above
+    // it: the stack then holds at most two entries per level above that
+    // point, which the margin between wide_cap and slot_cap covers.
+    // (once every query is taken, lanes would idle: up to all 64 per query)
+    const int lim = wide_cap - spv > 1 ? wide_cap - spv : 1;
+    const int wmax = exhausted ? kWave : width;
+    const int cap_w = wmax < lim ? wmax : lim;
+    int take = spv < kWave / kSlots ? spv : kWave / kSlots;
+    take = take < cap_w ? take : cap_w;
+    int incl = scan_lanes<kSlots>(take);
+    int total = lane_total<kSlots>(incl);
+    if (total < kWave) {
+      const int e = (spv < cap_w ? spv : cap_w) - take;
+      const int ei = scan_lanes<kSlots>(e);
+      const int room = kWave - total - (ei - e);
+      take += room <= 0 ? 0 : (room < e ? room : e);
+      incl = scan_lanes<kSlots>(take);
+      total = lane_total<kSlots>(incl);
+    }
+    const int excl = incl - take;
+
+    if (nl >= leaf_batch || (total == 0 && nl > 0)) {
+      // FP64 triangle tests
+      const int n = nl < kWave ? nl : kWave;
+      nl -= n;
+      bool skip = false;
+      if (lane < n) {
+        int tag, t1, t2;
+        unpack_task(W.lq[nl + lane], tag, t1, t2);
+        const int slot = tag & ((1 << kAnySlotBits) - 1);
+        skip = (tag >> kAnySlotBits) != W.gen[slot] || W.found[slot] != 0;
+        if (!skip) {
+          if (leaf_any(W.x64.get(slot), t1, t2)) W.found[slot] = 1;
+          atomicSub(&W.pend[slot], 1);
         }
-        if (v.push) {
-          const int o = sp + 2 * __popcll(pm & lanes_below());
-          stack[o] = v.c0;
-          stack[o + 1] = v.c1;
+      }
+      ++st[1];
+      ln[1] += n;
+      skipped += __popcll(__ballot(skip));
+      check = true;
+    } else if (nu >= kWave || (total == 0 && nu > 0)) {
+      // FP64 BV tests of unsure pairs: children to their slot's stack
+      const int n = nu < kWave ? nu : kWave;
+      nu -= n;
+      Visit v;
+      int slot = 0;
+      bool skip = false;
+      if (lane < n) {
+        int tag, b1, b2;
+        unpack_task(W.uq[nu + lane], tag, b1, b2);
+        slot = tag & ((1 << kAnySlotBits) - 1);
+        skip = (tag >> kAnySlotBits) != W.gen[slot] || W.found[slot] != 0;
+        if (!skip) {
+          v = visit_any(L, W.x32.get(slot), W.ptr.get(slot), W.x64.get(slot), tag, b1, b2);
+          atomicSub(&W.pend[slot], 1);
         }
-        sp += npush;
-        const unsigned long long lm = __ballot(v.leafc);
-        if (v.leafc) lq[nl + __popcll(lm & lanes_below())] = v.c0;
-        nl += __popcll(lm);
-      } else {
-        // FP32 BV tests from the stack
-        const int n = sp < kWave ? sp : kWave;
-        sp -= n;
-        int kind = kAnyNone;
-        unsigned long long task = 0, c0 = 0, c1 = 0;
-        bool skip = false;
-        if (lane < n) {
-          int q, b1, b2;
-          task = stack[sp + lane];
-          unpack_task(task, q, b1, b2);
-          const int slot = q - static_cast<int>(q0);
-          skip = found[slot] != 0;
-          if (!skip) kind = cull_any(Qs[slot], q, b1, b2, c0, c1);
+      }
+      if (v.push) {
+        const int o = atomicAdd(&W.sp[slot], 2);
+        if (o + 2 <= slot_cap) {
+          unsigned long long* stk = stacks + static_cast<long long>(slot) * slot_cap;
+          stk[o] = v.c0;
+          stk[o + 1] = v.c1;
+        } else {
+          W.found[slot] = 2;
+          atomicOr(L.status, kStatusStackOverflow);
         }
-        ++st[0];
-        ln[0] += n;
-        skipped += __popcll(__ballot(skip));
-        const unsigned long long pm = __ballot(kind == kAnyPush);
-        const int npush = 2 * __popcll(pm);
-        if (sp + npush > stack_cap) {
-          if (lane == 0) atomicOr(L.status, kStatusStackOverflow);
-          break;
-        }
-        if (kind == kAnyPush) {
-          const int o = sp + 2 * __popcll(pm & lanes_below());
-          stack[o] = c0;
-          stack[o + 1] = c1;
-        }
-        sp += npush;
-        const unsigned long long lm = __ballot(kind == kAnyLeaf);
-        const unsigned long long um = __ballot(kind == kAnyUnsure);
-        if (kind == kAnyLeaf) lq[nl + __popcll(lm & lanes_below())] = c0;
-        if (kind == kAnyUnsure) uq[nu + __popcll(um & lanes_below())] = task;
-        nl += __popcll(lm);
-        nu += __popcll(um);
+      }
+      const unsigned long long lm = __ballot(v.leafc);
+      if (v.leafc) {
+        W.lq[nl + __popcll(lm & lanes_below())] = v.c0;
+        atomicAdd(&W.pend[slot], 1);
+      }
+      nl += __popcll(lm);
+      ++st[2];
+      ln[2] += n;
+      skipped += __popcll(__ballot(skip));
+      check = true;  // stacks grew through W.sp
+    } else if (total > 0) {
+      // FP32 BV tests from the tops of the stacks
+      if (slot_lane) {
+        W.offs[lane] = excl;
+        W.base[lane] = spv - take;
+        for (int j = 0; j < take; ++j) W.lane_slot[excl + j] = static_cast<unsigned char>(lane);
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      int kind = kAnyNone, slot = 0, first = 0, b1 = 0, b2 = 0, tag = 0, at = 0;
+      unsigned long long c0 = 0, c1 = 0;
+      if (lane < total) {
+        slot = W.lane_slot[lane];
+        first = W.offs[slot];
+        at = W.base[slot];
+        tag = (W.gen[slot] << kAnySlotBits) | slot;
+        int dummy;
+        unpack_task(stacks[static_cast<long long>(slot) * slot_cap + at + lane - first], dummy, b1,
+                    b2);
+        kind = cull_any(W.x32.get(slot), W.ptr.get(slot), tag, b1, b2, c0, c1);
+      }
+      const unsigned long long pm = __ballot(kind == kAnyPush);
+      const unsigned long long lm = __ballot(kind == kAnyLeaf);
+      const unsigned long long um = __ballot(kind == kAnyUnsure);
+      if (kind == kAnyPush) {
+        const unsigned long long same = ~((1ULL << first) - 1ULL);
+        const int o = at + 2 * __popcll(pm & lanes_below() & same);
+        if (o + 2 <= slot_cap) {
+          unsigned long long* stk = stacks + static_cast<long long>(slot) * slot_cap;
+          stk[o] = c0;
+          stk[o + 1] = c1;
+        }
+      }
+      if (kind == kAnyLeaf) W.lq[nl + __popcll(lm & lanes_below())] = c0;
+      if (kind == kAnyUnsure) W.uq[nu + __popcll(um & lanes_below())] = pack_task(tag, b1, b2);
+      nl += __popcll(lm);
+      nu += __popcll(um);
+      bool emptied = false;
+      if (slot_lane && take > 0) {
+        const unsigned long long own = (take >= kWave ? ~0ULL : ((1ULL << take) - 1ULL)) << excl;
+        const int nsp = spv - take + 2 * __popcll(pm & own);
+        if (nsp > slot_cap) {
+          W.found[lane] = 2;
+          atomicOr(L.status, kStatusStackOverflow);
+          emptied = true;
+        } else {
+          spv = nsp;
+          W.sp[lane] = nsp;
+          emptied = nsp == 0;
+          sp_max = nsp > sp_max ? nsp : sp_max;
+        }
+        W.pend[lane] += __popcll((lm | um) & own);
+      }
+      if (__ballot(emptied) != 0) check = true;
+      ++st[0];
+      ln[0] += total;
+    } else {
+      check = true;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < nk) L.counts[q0 + lane] = found[lane];
-    if (prof && lane == 0) {
-      for (int k = 0; k < 3; ++k) {
-        atomicAdd(prof + k, st[k]);
-        atomicAdd(prof + 3 + k, ln[k]);
-      }
-      atomicAdd(prof + 6, skipped);
-      atomicAdd(prof + 7, 1ULL);
+  }
+  if (prof && lane == 0) {
+    for (int k = 0; k < 3; ++k) {
+      atomicAdd(prof + k, st[k]);
+      atomicAdd(prof + 3 + k, ln[k]);
     }
+    atomicAdd(prof + 6, skipped);
+    atomicAdd(prof + 7, rounds);
+  }
+  if (prof) {
+    for (int d = 1; d < kSlots; d <<= 1) {
+      const int o = __shfl_xor(sp_max, d);
+      sp_max = o > sp_max ? o : sp_max;
+    }
+  }
+  if (prof && lane == 0) {
+    atomicMax(prof + 8, static_cast<unsigned long long>(sp_max));
+    unsigned long long* w = wprof + 3 * gw;
+    w[0] = t_start;
+    w[1] = wall_clock64();
+    w[2] = rounds;
   }
 }
 
@@ -1733,13 +1926,21 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   const long long nq = L.num_queries;
   if (nq == 0) return hipSuccess;
   const int cus = device_cus();
-  const int qpw = ws->tune.bool_queries_per_wave;
-  const unsigned grid = static_cast<unsigned>(std::max<long long>(
-      1, std::min<long long>((nq + qpw * kAnyWaves - 1) / (qpw * kAnyWaves),
-                             static_cast<long long>(cus) * 8)));
-  // 64-wide LIFO: a generous bound of 128 entries per BVTT level
-  const int stack_cap = 128 * (stack_need + 2);
-  const size_t need = static_cast<size_t>(grid) * kAnyWaves * stack_cap;
+  const int K = ws->tune.bool_queries_per_wave;
+  const int wps = ws->tune.bool_waves_per_simd;
+  const int width = std::max(ws->tune.bool_width, kWave / K);
+  // as many wavefronts as fit at once (slots take queries from a counter),
+  // fewer for small batches
+  const long long nw = std::max<long long>(
+      1, std::min<long long>((nq + K - 1) / K, static_cast<long long>(cus) * 4 * wps));
+  const unsigned grid = static_cast<unsigned>((nw + kAnyWaves - 1) / kAnyWaves);
+  // A slot's stack stays sorted by BVTT level (the children of a step's
+  // tasks go on top, in order), and a step takes at most `width` tasks from
+  // its top, so it never holds more than 2 width entries of one level;
+  // children of unsure pairs come back later, hence the queue-sized margin.
+  const int slot_cap = std::max(kAnySlotCap, 4 * (stack_need + 2) + 2 * kAnyQueue);
+  const int wide_cap = slot_cap - (stack_need + 2) - 2 * kAnyQueue;
+  const size_t need = kAnyAreaHead + static_cast<size_t>(grid) * kAnyWaves * K * slot_cap;
   const int slot = ws->any_next;
   ws->any_next = (slot + 1) % EngineWorkspace::kAnySlots;
   if (need > ws->any_cap[slot]) {
@@ -1747,45 +1948,71 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
     (void)hipFree(ws->any_stk[slot]);
     ws->any_stk[slot] = nullptr;
     ws->any_cap[slot] = 0;
-    const size_t cap = std::max<size_t>(need, static_cast<size_t>(cus) * 8 * kAnyWaves * 128 * 66);
     ENG_TRY(hipMalloc(reinterpret_cast<void**>(&ws->any_stk[slot]),
-                      cap * sizeof(unsigned long long)));
-    ws->any_cap[slot] = cap;
+                      need * sizeof(unsigned long long)));
+    ws->any_cap[slot] = need;
   } else if (ws->any_used[slot]) {
     ENG_TRY(hipStreamWaitEvent(s, ws->any_ev[slot], 0));
   }
-  const int w = ws->tune.bool_waves_per_simd;
+  unsigned long long* area = ws->any_stk[slot];
+  zero_u64<<<1, kAnyAreaHead, 0, s>>>(area, kAnyAreaHead);  // the range counters
   const int lb = ws->tune.bool_leaf_batch;
+  const int rf = ws->tune.bool_refill;
   unsigned long long* prof = nullptr;
   if (ws->profile) {  // counters of this batch (the batch is waited for)
     prof = ws->counters + kCtrWork;
-    set_u64<<<1, 1, 0, s>>>(prof, 0);
-    for (int k = 1; k < 8; ++k) set_u64<<<1, 1, 0, s>>>(prof + k, 0);
+    zero_u64<<<1, 64, 0, s>>>(prof, 9);
+    ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * static_cast<size_t>(grid) * kAnyWaves));
   }
-  if (qpw == 16)
-    w == 4 ? engine_anyhit<16, 4><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
-                                                                lb, prof)
-           : engine_anyhit<16, 3><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
-                                                                lb, prof);
+#define FCLX_ANYHIT(k, w) \
+  engine_anyhit<k, w><<<grid, kBlock, 0, s>>>(L, area, area + kAnyAreaHead, slot_cap, wide_cap, \
+                                              width, lb, rf, prof, ws->wave_prof)
+  if (K == 8)
+    wps >= 4 ? FCLX_ANYHIT(8, 4) : wps == 3 ? FCLX_ANYHIT(8, 3) : FCLX_ANYHIT(8, 2);
+  else if (K == 16)
+    wps >= 4 ? FCLX_ANYHIT(16, 4) : wps == 3 ? FCLX_ANYHIT(16, 3) : FCLX_ANYHIT(16, 2);
   else
-    w == 4 ? engine_anyhit<8, 4><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
-                                                                lb, prof)
-           : engine_anyhit<8, 3><<<grid, kBlock, 0, s>>>(L, ws->any_stk[slot], stack_cap,
-                                                                lb, prof);
+    wps >= 3 ? FCLX_ANYHIT(32, 3) : FCLX_ANYHIT(32, 2);
+#undef FCLX_ANYHIT
   ENG_TRY(hipGetLastError());
   ENG_TRY(hipEventRecord(ws->any_ev[slot], s));
   ws->any_used[slot] = true;
   if (prof) {
-    unsigned long long h[8];
+    unsigned long long h[9];
+    const size_t nwv = static_cast<size_t>(grid) * kAnyWaves;
+    std::vector<unsigned long long> wv(3 * nwv);
     ENG_TRY(hipMemcpyAsync(h, prof, sizeof(h), hipMemcpyDeviceToHost, s));
+    ENG_TRY(hipMemcpyAsync(wv.data(), ws->wave_prof, sizeof(unsigned long long) * wv.size(),
+                           hipMemcpyDeviceToHost, s));
     ENG_TRY(hipStreamSynchronize(s));
+    int dev = 0, khz = 1;
+    ENG_TRY(hipGetDevice(&dev));
+    ENG_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     EngineReport& r = ws->report;
+    unsigned long long t0 = ~0ULL;
+    double life = 0;
+    std::vector<unsigned long long> ends(nwv);
+    for (size_t w = 0; w < nwv; ++w) {
+      t0 = std::min(t0, wv[3 * w]);
+      ends[w] = wv[3 * w + 1];
+      life += static_cast<double>(wv[3 * w + 1] - wv[3 * w]);
+    }
+    std::sort(ends.begin(), ends.end());
+    auto at = [&](double f) {
+      return static_cast<double>(ends[std::min(nwv - 1, static_cast<size_t>(f * (nwv - 1)))] - t0) /
+             khz;
+    };
+    r.bool_wave_ms = life / static_cast<double>(nwv) / khz;
+    r.bool_end_p50_ms = at(0.5);
+    r.bool_end_p90_ms = at(0.9);
+    r.bool_span_ms = at(1.0);
     for (int k = 0; k < 3; ++k) {
       r.bool_steps[k] = static_cast<long long>(h[k]);
       r.bool_lanes[k] = static_cast<long long>(h[3 + k]);
     }
     r.bool_skipped = static_cast<long long>(h[6]);
     r.bool_rounds = static_cast<long long>(h[7]);
+    r.bool_stack_max = static_cast<long long>(h[8]);
   }
   return hipSuccess;
 }

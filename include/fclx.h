@@ -245,12 +245,17 @@ typedef struct {
   double dfs_end_max_ms;
   /* last BOOL-engine batch (profiling only): wavefront steps and the lanes
    * they used, per kind (0: FP32 BV tests, 1: FP64 triangle tests, 2: FP64
-   * BV tests); tasks dropped because their query was already answered;
-   * rounds of queries taken by wavefronts */
+   * BV tests); queued pairs dropped because their query was already
+   * answered; times a wavefront took new queries into free slots */
   int64_t bool_steps[3];
   int64_t bool_lanes[3];
   int64_t bool_skipped;
   int64_t bool_rounds;
+  int64_t bool_stack_max; /* deepest query stack, entries */
+  double bool_wave_ms; /* mean wavefront lifetime, ms */
+  double bool_end_p50_ms; /* wavefront ends after the first start: median, */
+  double bool_end_p90_ms; /* 90th percentile */
+  double bool_span_ms;    /* and last, ms */
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
@@ -264,10 +269,17 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
  *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
- *   "bool_queries_per_wave" 8, 16  queries a BOOL-engine wavefront works on
- *   "bool_waves_per_simd" 3, 4  register budget of the BOOL-engine kernel
+ *   "bool_queries_per_wave" 8, 16, 32  queries a BOOL-engine wavefront keeps
+ *                               in flight (default 16)
+ *   "bool_waves_per_simd" 2..4  register budget of the BOOL-engine kernel
+ *                               (32 queries per wavefront: at most 3)
  *   "bool_leaf_batch"   1..64   leaf pairs a BOOL-engine wavefront queues
  *                               before testing them
+ *   "bool_width"        1..64   most tasks of one query a BOOL-engine step
+ *                               tests (at least 64 / queries per wavefront;
+ *                               default 32)
+ *   "bool_refill"       1..32   free query slots that make a BOOL-engine
+ *                               wavefront take new queries (default 4)
  *   "epa_blocks_per_cu" 0..32   resident blocks per CU of the shape path's EPA
  *                               kernel (0: as many as fit); each lane holds
  *                               a 13 KiB EPA block in device memory

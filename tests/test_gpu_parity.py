@@ -397,6 +397,57 @@ def test_bool_engine_matches_oracle(ctx, c0_meshes, case):
     assert 0.1 < (oc > 0).mean() < 0.9
 
 
+BOOL_DEFAULTS = {"bool_queries_per_wave": 16, "bool_waves_per_simd": 4, "bool_leaf_batch": 64,
+                 "bool_width": 32, "bool_refill": 4}
+
+
+@pytest.mark.parametrize("params", [
+    {"bool_queries_per_wave": 8, "bool_waves_per_simd": 3},
+    {"bool_queries_per_wave": 32, "bool_width": 2, "bool_waves_per_simd": 3, "bool_refill": 1},
+    {"bool_queries_per_wave": 32, "bool_width": 64, "bool_waves_per_simd": 2, "bool_leaf_batch": 7},
+    {"bool_width": 64, "bool_leaf_batch": 1, "bool_refill": 16},
+    {"bool_width": 1, "bool_waves_per_simd": 2, "bool_leaf_batch": 17, "bool_refill": 32},
+], ids=["q8w3", "q32narrow", "q32wide", "wide_lb1", "narrow_lb17"])
+def test_bool_engine_tunables(ctx, c0_meshes, params):
+    """Every slot count, width, leaf batch and refill threshold gives the
+    oracle's booleans, on 6000 and 1000 queries of the plain pair and on a
+    batch smaller than one wavefront's slots."""
+    o, g = c0_meshes
+    print("case: bool engine tunables %s" % params, flush=True)
+    try:
+        for k, v in params.items():
+            ctx.set_engine_param(k, v)
+        for n, seed in ((6000, 0xE0), (1000, 0xE1), (5, 0xE2)):
+            tf1, tf2 = F.poses(n, 0, 2.2, 0, seed)
+            oc = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 1, store=0)[0]
+            counts, _, _ = _bool_gpu(ctx, np.tile(np.array(g, np.int32), (n, 1)), tf1, tf2)
+            assert ctx.sync_status(torch.cuda.current_stream().cuda_stream) == 0
+            assert np.array_equal(counts.cpu().numpy(), (oc > 0).astype(np.int32))
+    finally:
+        for k, v in BOOL_DEFAULTS.items():
+            ctx.set_engine_param(k, v)
+
+
+def test_bool_engine_bad_queries(ctx, c0_meshes):
+    """Bad mesh ids in a BOOL batch: the batch reports an invalid argument,
+    those queries count 0 and the others are still answered."""
+    o, g = c0_meshes
+    n = 300
+    tf1, tf2 = F.poses(n, 0, 2.2, 0, 0xE3)
+    qm = np.tile(np.array(g, np.int32), (n, 1))
+    bad = np.arange(3, n, 37)
+    qm[bad, 1] = 999
+    print("case: bool engine, %d bad queries of %d" % (len(bad), n), flush=True)
+    counts, _, _ = _bool_gpu(ctx, qm, tf1, tf2)
+    assert ctx.sync_status(torch.cuda.current_stream().cuda_stream) == N.FCLX_ERR_INVALID_ARGUMENT
+    oc = fclo.collide_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2, 1, store=0)[0]
+    want = (oc > 0).astype(np.int32)
+    want[bad] = 0
+    assert np.array_equal(counts.cpu().numpy(), want)
+    counts, _, _ = _bool_gpu(ctx, np.tile(np.array(g, np.int32), (n, 1)), tf1, tf2)
+    assert ctx.sync_status(torch.cuda.current_stream().cuda_stream) == 0
+
+
 def test_bool_engine_concurrent_streams(ctx, c0_meshes):
     """Boolean batches on several streams at once give what they give one
     after the other."""
