@@ -706,7 +706,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.stop_poll = static_cast<int>(value);
   } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
     t.dfs_waves_per_simd = static_cast<int>(value);
-  } else if (n == "bool_queries_per_wave" && (value == 8 || value == 16 || value == 32)) {
+  } else if (n == "bool_queries_per_wave" &&
+             (value == 0 || value == 4 || value == 8 || value == 16 || value == 32)) {
     t.bool_queries_per_wave = static_cast<int>(value);
   } else if (n == "bool_waves_per_simd" && in(2, 4)) {
     t.bool_waves_per_simd = static_cast<int>(value);

@@ -31,7 +31,8 @@ struct EngineTuning {
   double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
   int stop_poll = 16;           // depth-first steps between polls of the round's stop flag
   int dfs_waves_per_simd = 4;   // register budget of the depth-first kernel (4..6)
-  int bool_queries_per_wave = 16;  // queries a boolean-engine wavefront keeps in flight (8, 16, 32)
+  int bool_queries_per_wave = 0;  // queries a boolean-engine wavefront keeps in flight (4, 8, 16,
+                                  // 32; 0: 4 below 65536 queries, else 8)
   int bool_waves_per_simd = 4;    // register budget of the boolean kernel (2, 3, 4; 32 slots: 2, 3)
   int bool_leaf_batch = 64;       // leaf pairs queued before the boolean kernel tests them
   int bool_width = 32;            // most tasks of one query a boolean-kernel step tests

@@ -975,6 +975,7 @@ constexpr int kAnyQueue = 128;
 constexpr int kAnySlotBits = 5;  // slots per wavefront <= 32
 constexpr int kAnyXcds = 8;      // query ranges (one per XCD)
 constexpr int kAnyAreaHead = 16 * kAnyXcds;  // area: range counters, 128 B apart; then the stacks
+constexpr long long kAnySmallBatch = 65536;  // below: 4 slots per wavefront by default, else 8
 constexpr int kAnySlotCap = 1024;            // stack entries per slot (at least)
 constexpr int kAnyGenMask = (1 << (22 - kAnySlotBits)) - 1;
 
@@ -1926,7 +1927,11 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   const long long nq = L.num_queries;
   if (nq == 0) return hipSuccess;
   const int cus = device_cus();
-  const int K = ws->tune.bool_queries_per_wave;
+  // slots per wavefront: fewer for small batches, so that they still spread
+  // over many wavefronts
+  const int K = ws->tune.bool_queries_per_wave > 0 ? ws->tune.bool_queries_per_wave
+                : nq < kAnySmallBatch                ? 4
+                                                     : 8;
   const int wps = ws->tune.bool_waves_per_simd;
   const int width = std::max(ws->tune.bool_width, kWave / K);
   // as many wavefronts as fit at once (slots take queries from a counter),
@@ -1944,16 +1949,25 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   const int slot = ws->any_next;
   ws->any_next = (slot + 1) % EngineWorkspace::kAnySlots;
   if (need > ws->any_cap[slot]) {
-    if (ws->any_used[slot]) ENG_TRY(hipEventSynchronize(ws->any_ev[slot]));
-    (void)hipFree(ws->any_stk[slot]);
-    ws->any_stk[slot] = nullptr;
-    ws->any_cap[slot] = 0;
-    ENG_TRY(hipMalloc(reinterpret_cast<void**>(&ws->any_stk[slot]),
-                      need * sizeof(unsigned long long)));
-    ws->any_cap[slot] = need;
-  } else if (ws->any_used[slot]) {
-    ENG_TRY(hipStreamWaitEvent(s, ws->any_ev[slot], 0));
+    // Every area of the ring grows at once, to what any batch with these
+    // meshes and settings can need (all the wavefronts that fit, 8 slots
+    // each at least): freeing device memory waits for the whole device, so
+    // it should not happen again while batches of other streams run.
+    const size_t full = std::max(
+        need, kAnyAreaHead + static_cast<size_t>(cus) * 4 * std::max(wps, 4) * std::max(K, 8) *
+                                 static_cast<size_t>(slot_cap));
+    for (int i = 0; i < EngineWorkspace::kAnySlots; ++i) {
+      if (ws->any_cap[i] >= full) continue;
+      if (ws->any_used[i]) ENG_TRY(hipEventSynchronize(ws->any_ev[i]));
+      (void)hipFree(ws->any_stk[i]);
+      ws->any_stk[i] = nullptr;
+      ws->any_cap[i] = 0;
+      ENG_TRY(hipMalloc(reinterpret_cast<void**>(&ws->any_stk[i]),
+                        full * sizeof(unsigned long long)));
+      ws->any_cap[i] = full;
+    }
   }
+  if (ws->any_used[slot]) ENG_TRY(hipStreamWaitEvent(s, ws->any_ev[slot], 0));
   unsigned long long* area = ws->any_stk[slot];
   zero_u64<<<1, kAnyAreaHead, 0, s>>>(area, kAnyAreaHead);  // the range counters
   const int lb = ws->tune.bool_leaf_batch;
@@ -1967,7 +1981,9 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
 #define FCLX_ANYHIT(k, w) \
   engine_anyhit<k, w><<<grid, kBlock, 0, s>>>(L, area, area + kAnyAreaHead, slot_cap, wide_cap, \
                                               width, lb, rf, prof, ws->wave_prof)
-  if (K == 8)
+  if (K == 4)
+    wps >= 4 ? FCLX_ANYHIT(4, 4) : wps == 3 ? FCLX_ANYHIT(4, 3) : FCLX_ANYHIT(4, 2);
+  else if (K == 8)
     wps >= 4 ? FCLX_ANYHIT(8, 4) : wps == 3 ? FCLX_ANYHIT(8, 3) : FCLX_ANYHIT(8, 2);
   else if (K == 16)
     wps >= 4 ? FCLX_ANYHIT(16, 4) : wps == 3 ? FCLX_ANYHIT(16, 3) : FCLX_ANYHIT(16, 2);

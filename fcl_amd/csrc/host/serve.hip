@@ -3,8 +3,9 @@
 // Requests queue on the host.  A worker thread owns num_streams slots, each
 // with its own HIP stream, device buffers (query mesh ids, the scene pose
 // filled once per query slot, link poses, counts) and pinned staging.
-// Whenever a slot is free it takes, in arrival order, every waiting request
-// that fits the slot's query capacity and the device-memory budget, copies
+// Whenever a slot is free it takes waiting requests, smallest first (oldest
+// first once one has waited 4 ms), as long as they fit the batch size and
+// the device-memory budget, copies
 // their poses in one transfer, launches one first-contact collide batch
 // (CollisionRequest(1, false) per link and waypoint), copies the counts back
 // and records an event; when the event has fired it writes each request's
@@ -39,6 +40,9 @@ double ms_since(Clock::time_point t0) {
 }
 
 constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
+constexpr auto kAgeLimit = std::chrono::milliseconds(4);  // then a request goes first
+constexpr int32_t kBatchDefault = 1 << 16;                 // queries per batch at most
+constexpr int64_t kSlotMin = 1 << 18;  // queries a slot holds without growing
 
 __global__ void fill_pose(double* out, int n, fclx_server_config cfg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -103,6 +107,7 @@ struct fclx_server {
   std::thread worker;
   bool stop = false;
   int64_t in_flight_bytes = 0;
+  int64_t queued_queries = 0;  // of the waiting requests
   int64_t batches = 0, queries = 0;
 };
 
@@ -217,20 +222,35 @@ void worker_main(fclx_server* S) {
       if (s.busy || S->queue.empty()) continue;
       int64_t nq = 0, bytes = 0;
       s.batch.clear();
+      // a backlog goes out in larger batches (a share of it per stream, as
+      // far as the slot holds without growing), which fill the device better
+      // than many small ones
+      const int64_t batch_cap = std::max<int64_t>(
+          S->cfg.max_batch_queries,
+          std::min<int64_t>(s.cap, S->queued_queries / static_cast<int64_t>(S->slots.size())));
       while (!S->queue.empty()) {
-        const Pending& p = S->queue.front();
+        // the smallest waiting request first, so that short requests do not
+        // wait behind long ones; but the oldest first once it has waited
+        // kAgeLimit, so that none waits for ever
+        size_t pick = 0;
+        if (Clock::now() - S->queue.front().t_submit < kAgeLimit) {
+          for (size_t i = 1; i < S->queue.size(); ++i)
+            if (S->queue[i].nq < S->queue[pick].nq) pick = i;
+        }
+        const Pending& p = S->queue[pick];
         const int64_t b = p.nq * kQueryBytes;
         const bool first = s.batch.empty();
         // a request larger than a batch or than the budget runs alone once
         // nothing else is in flight
-        if (!first && nq + p.nq > S->cfg.max_batch_queries) break;
+        if (!first && nq + p.nq > batch_cap) break;
         if (S->in_flight_bytes + bytes + b > S->cfg.hbm_budget &&
             !(first && S->in_flight_bytes == 0))
           break;
         nq += p.nq;
         bytes += b;
         s.batch.push_back(p);
-        S->queue.pop_front();
+        S->queued_queries -= p.nq;
+        S->queue.erase(S->queue.begin() + static_cast<std::ptrdiff_t>(pick));
       }
       if (s.batch.empty()) break;  // waits for memory to free up
       S->in_flight_bytes += bytes;
@@ -278,7 +298,7 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
   S->ctx = ctx;
   S->cfg = *cfg;
   if (S->cfg.hbm_budget == 0) S->cfg.hbm_budget = int64_t(2) << 30;
-  if (S->cfg.max_batch_queries == 0) S->cfg.max_batch_queries = 1 << 20;
+  if (S->cfg.max_batch_queries == 0) S->cfg.max_batch_queries = kBatchDefault;
   S->device = fclx::context_device(ctx);
   (void)hipSetDevice(S->device);
   S->slots.resize(cfg->num_streams);
@@ -296,8 +316,9 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
   // every slot at its full size now, so that no allocation (pinned host
   // memory above all) falls inside a request's latency
   const int64_t share = S->cfg.hbm_budget / (kQueryBytes * cfg->num_streams);
-  const int64_t cap = std::max<int64_t>(1 << 16, std::min<int64_t>(S->cfg.max_batch_queries,
-                                                                  share));
+  const int64_t cap =
+      std::max<int64_t>(1 << 16, std::min<int64_t>(
+                                     std::max<int64_t>(S->cfg.max_batch_queries, kSlotMin), share));
   for (Slot& s : S->slots) {
     if (slot_reserve(S, s, cap) != hipSuccess) {
       for (Slot& t : S->slots) {
@@ -334,6 +355,7 @@ int fclx_server_submit(fclx_server* S, const fclx_request* req) {
     std::lock_guard<std::mutex> lock(S->mu);
     if (S->stop) return FCLX_ERR_INVALID_ARGUMENT;
     S->queue.push_back(p);
+    S->queued_queries += p.nq;
   }
   S->cv.notify_one();
   return FCLX_OK;

@@ -269,8 +269,9 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
  *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
- *   "bool_queries_per_wave" 8, 16, 32  queries a BOOL-engine wavefront keeps
- *                               in flight (default 16)
+ *   "bool_queries_per_wave" 0, 4, 8, 16, 32  queries a BOOL-engine wavefront
+ *                               keeps in flight (default 0: 4 for batches
+ *                               below 65536 queries, else 8)
  *   "bool_waves_per_simd" 2..4  register budget of the BOOL-engine kernel
  *                               (32 queries per wavefront: at most 3)
  *   "bool_leaf_batch"   1..64   leaf pairs a BOOL-engine wavefront queues
@@ -342,11 +343,12 @@ int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_
  * link meshes and a trajectory of link poses -- are queued as they arrive
  * and run as first-contact collide() queries (CollisionRequest(1, false),
  * the scene as the first object), each link against the scene at each
- * waypoint.  A worker thread forms a batch from all waiting requests that
- * fit a free stream slot and the device-memory budget, so requests that
- * arrive while others are on the GPU join the next batch and batches on
- * different streams overlap on the device.  A request's result does not
- * depend on what it is batched with. */
+ * waypoint.  A worker thread forms a batch for a free stream slot from the
+ * waiting requests -- the smallest first, the oldest first once one has
+ * waited 4 ms -- that fit the batch size and the device-memory budget, so
+ * requests that arrive while others are on the GPU join the next batch and
+ * batches on different streams overlap on the device.  A request's result
+ * does not depend on what it is batched with. */
 typedef struct fclx_server fclx_server;
 
 typedef struct {
@@ -355,8 +357,10 @@ typedef struct {
   double scene_pose[12];       /* the scene's pose */
   int64_t hbm_budget;          /* device bytes of requests in flight at once; a
                                   query takes 204 B (0: 2 GiB) */
-  int32_t max_batch_queries;   /* queries per batch at most (0: 1<<20); a
-                                  request larger than this runs alone */
+  int32_t max_batch_queries;   /* queries per batch at most (0: 65536), or a
+                                  stream's share of the waiting queries when
+                                  that is more; a request larger than this
+                                  runs alone */
   int32_t engine;              /* FCLX_ENGINE_BOOL (the natural choice: only
                                   booleans are returned), _FAST or _PLAIN */
 } fclx_server_config;
