@@ -282,6 +282,51 @@ def bench_serving(args):
     big = max(pool, key=lambda P: P.shape[0])
     _, lat1, _ = srv.replay([(link_ids, big)], [0.0])
     srv.close()
+    # the BOOL engine on one batch of 16 pool requests, inputs resident: time
+    # and, in a profiled run, its work (fclx_engine_last_report)
+    engine = None
+    if args.serve_engine == "bool":
+        import torch
+
+        P = np.ascontiguousarray(np.concatenate([p.reshape(-1, 12) for p in pool[:16]]))
+        nq = len(P)
+        dev = "cuda:0"
+        dqm = torch.from_numpy(np.stack([np.full(nq, scene, np.int32),
+                                         np.tile(link_ids, nq // nlink)], 1)).to(dev)
+        dtf1 = torch.from_numpy(np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float),
+                                        (nq, 1))).to(dev)
+        dtf2 = torch.from_numpy(P).to(dev)
+        cnt = torch.zeros(nq, dtype=torch.int32, device=dev)
+        num = torch.zeros(1, dtype=torch.int64, device=dev)
+
+        def one():
+            ctx.collide_batch_async(dqm, dtf1, dtf2, 1, cnt, None, num, engine=F.ENGINE_BOOL)
+            ctx.sync()
+
+        for _ in range(3):
+            one()
+        ts = []
+        for _ in range(5):
+            a = time.perf_counter()
+            one()
+            ts.append(time.perf_counter() - a)
+        ctx.set_profiling(True)
+        one()
+        r = ctx.engine_report()
+        ctx.set_profiling(False)
+        engine = {"queries": nq, "ms": 1e3 * min(ts), "queries_per_s": nq / min(ts),
+                  "hit_fraction": float((cnt > 0).float().mean().item()),
+                  "fp32_bv_tests_per_query": r["bool_lanes"][0] / nq,
+                  "fp64_triangle_tests_per_query": r["bool_lanes"][1] / nq,
+                  "fp64_bv_tests_per_query": r["bool_lanes"][2] / nq,
+                  "lanes_per_bv_step": r["bool_lanes"][0] / max(1, r["bool_steps"][0]),
+                  "wavefront_ms": r["bool_wave_ms"], "wavefront_end_p50_ms": r["bool_end_p50_ms"],
+                  "wavefront_end_p90_ms": r["bool_end_p90_ms"], "last_end_ms": r["bool_span_ms"],
+                  "deepest_stack": r["bool_stack_max"]}
+        log("bool engine, %d queries resident: %.2f ms (%.3g queries/s), %.0f FP32 BV tests and "
+            "%.0f triangle tests per query" % (nq, engine["ms"], engine["queries_per_s"],
+                                              engine["fp32_bv_tests_per_query"],
+                                              engine["fp64_triangle_tests_per_query"]))
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -320,6 +365,7 @@ def bench_serving(args):
         "loads": loads,
         "largest_request_alone_ms": float(lat1[0]),
         "largest_request_queries": int(big.shape[0] * nlink),
+        "bool_engine_batch": engine,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
