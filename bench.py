@@ -252,10 +252,65 @@ def bench_serving(args):
     scene = ctx.mesh(sV, sT)
     link_ids = np.array([ctx.mesh(V, T) for V, T in links], np.int32)
     log("uploaded BVHs in %.1fs" % (time.time() - t0))
+    nlink = len(links)
+    # the BOOL engine on one batch of 16 pool requests, inputs resident: time
+    # and, in a profiled run, its work (fclx_engine_last_report)
+    engine_batch = None
+    tuned = None
+    if args.serve_engine == "bool":
+        import torch
+
+        P = np.ascontiguousarray(np.concatenate([p.reshape(-1, 12) for p in pool[:16]]))
+        nq = len(P)
+        dev = "cuda:0"
+        dqm = torch.from_numpy(np.stack([np.full(nq, scene, np.int32),
+                                         np.tile(link_ids, nq // nlink)], 1)).to(dev)
+        dtf1 = torch.from_numpy(np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float),
+                                        (nq, 1))).to(dev)
+        dtf2 = torch.from_numpy(P).to(dev)
+        cnt = torch.zeros(nq, dtype=torch.int32, device=dev)
+        num = torch.zeros(1, dtype=torch.int64, device=dev)
+
+        def one():
+            ctx.collide_batch_async(dqm, dtf1, dtf2, 1, cnt, None, num, engine=F.ENGINE_BOOL)
+            ctx.sync()
+
+        for _ in range(3):
+            one()
+        if args.autotune:
+            from fcl_amd import autotune
+
+            def checked():
+                one()
+                return int(cnt.sum().item())
+
+            tuned, _, _ = autotune.tune(ctx, checked, autotune.BOOL_SPACE, log=log)
+        ts = []
+        for _ in range(5):
+            a = time.perf_counter()
+            one()
+            ts.append(time.perf_counter() - a)
+        ctx.set_profiling(True)
+        one()
+        r = ctx.engine_report()
+        ctx.set_profiling(False)
+        engine_batch = {"queries": nq, "ms": 1e3 * min(ts), "queries_per_s": nq / min(ts),
+                  "hit_fraction": float((cnt > 0).float().mean().item()),
+                  "fp32_bv_tests_per_query": r["bool_lanes"][0] / nq,
+                  "fp64_triangle_tests_per_query": r["bool_lanes"][1] / nq,
+                  "fp64_bv_tests_per_query": r["bool_lanes"][2] / nq,
+                  "lanes_per_bv_step": r["bool_lanes"][0] / max(1, r["bool_steps"][0]),
+                  "wavefront_ms": r["bool_wave_ms"], "wavefront_end_p50_ms": r["bool_end_p50_ms"],
+                  "wavefront_end_p90_ms": r["bool_end_p90_ms"], "last_end_ms": r["bool_span_ms"],
+                  "deepest_stack": r["bool_stack_max"]}
+        e = engine_batch
+        log("bool engine, %d queries resident: %.2f ms (%.3g queries/s), %.0f FP32 BV tests and "
+            "%.0f triangle tests per query" % (nq, e["ms"], e["queries_per_s"],
+                                              e["fp32_bv_tests_per_query"],
+                                              e["fp64_triangle_tests_per_query"]))
     engine = {"bool": F.ENGINE_BOOL, "fast": F.ENGINE_FAST, "plain": F.ENGINE_PLAIN}[
         args.serve_engine]
     srv = F.Server(ctx, scene, num_streams=args.serve_streams, engine=engine)
-    nlink = len(links)
     srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
     rng = np.random.default_rng(11)
     loads = []
@@ -282,51 +337,6 @@ def bench_serving(args):
     big = max(pool, key=lambda P: P.shape[0])
     _, lat1, _ = srv.replay([(link_ids, big)], [0.0])
     srv.close()
-    # the BOOL engine on one batch of 16 pool requests, inputs resident: time
-    # and, in a profiled run, its work (fclx_engine_last_report)
-    engine = None
-    if args.serve_engine == "bool":
-        import torch
-
-        P = np.ascontiguousarray(np.concatenate([p.reshape(-1, 12) for p in pool[:16]]))
-        nq = len(P)
-        dev = "cuda:0"
-        dqm = torch.from_numpy(np.stack([np.full(nq, scene, np.int32),
-                                         np.tile(link_ids, nq // nlink)], 1)).to(dev)
-        dtf1 = torch.from_numpy(np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float),
-                                        (nq, 1))).to(dev)
-        dtf2 = torch.from_numpy(P).to(dev)
-        cnt = torch.zeros(nq, dtype=torch.int32, device=dev)
-        num = torch.zeros(1, dtype=torch.int64, device=dev)
-
-        def one():
-            ctx.collide_batch_async(dqm, dtf1, dtf2, 1, cnt, None, num, engine=F.ENGINE_BOOL)
-            ctx.sync()
-
-        for _ in range(3):
-            one()
-        ts = []
-        for _ in range(5):
-            a = time.perf_counter()
-            one()
-            ts.append(time.perf_counter() - a)
-        ctx.set_profiling(True)
-        one()
-        r = ctx.engine_report()
-        ctx.set_profiling(False)
-        engine = {"queries": nq, "ms": 1e3 * min(ts), "queries_per_s": nq / min(ts),
-                  "hit_fraction": float((cnt > 0).float().mean().item()),
-                  "fp32_bv_tests_per_query": r["bool_lanes"][0] / nq,
-                  "fp64_triangle_tests_per_query": r["bool_lanes"][1] / nq,
-                  "fp64_bv_tests_per_query": r["bool_lanes"][2] / nq,
-                  "lanes_per_bv_step": r["bool_lanes"][0] / max(1, r["bool_steps"][0]),
-                  "wavefront_ms": r["bool_wave_ms"], "wavefront_end_p50_ms": r["bool_end_p50_ms"],
-                  "wavefront_end_p90_ms": r["bool_end_p90_ms"], "last_end_ms": r["bool_span_ms"],
-                  "deepest_stack": r["bool_stack_max"]}
-        log("bool engine, %d queries resident: %.2f ms (%.3g queries/s), %.0f FP32 BV tests and "
-            "%.0f triangle tests per query" % (nq, engine["ms"], engine["queries_per_s"],
-                                              engine["fp32_bv_tests_per_query"],
-                                              engine["fp64_triangle_tests_per_query"]))
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -365,7 +375,8 @@ def bench_serving(args):
         "loads": loads,
         "largest_request_alone_ms": float(lat1[0]),
         "largest_request_queries": int(big.shape[0] * nlink),
-        "bool_engine_batch": engine,
+        "bool_engine_batch": engine_batch,
+        "autotuned": tuned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -399,6 +410,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine-param", action="append", default=[], metavar="NAME=VALUE",
                     help="FAST-engine tunable (fclx_set_engine_param); repeatable")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the engine's tunables on the workload before timing "
+                         "(fcl_amd.autotune; the setting found goes into the JSON config)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three device copies of MIB MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -481,6 +495,16 @@ def main():
     # ---------------------------------------------------------------- FAST
     for _ in range(max(args.warmup, 1)):
         run(F.ENGINE_FAST)
+    tuned = None
+    if args.autotune:
+        from fcl_amd import autotune
+
+        def one():
+            run(F.ENGINE_FAST)
+            return int(num.item()), int(counts.sum().item())
+
+        tuned, tuned_ms, trials = autotune.tune(ctx, one, autotune.FAST_SPACE, repeats=1, log=log)
+        log("autotune: %d trials, best %s at %.1f ms" % (len(trials), tuned, tuned_ms))
     ctx.set_profiling(True)
     run(F.ENGINE_FAST)
     rep = ctx.engine_report()
@@ -608,7 +632,8 @@ def main():
         "config": {"workload": "batched uneven mesh collision: 64 pairs, 500-200k tris "
                                "log-uniform, Dirichlet(1) query split, exhaustive contacts",
                    "queries": nq, "pairs": args.pairs, "num_max_contacts": cap,
-                   "contact_pairs": total_pairs},
+                   "contact_pairs": total_pairs, "engine_params": args.engine_param,
+                   "autotuned": tuned},
         "p50_ms": p[0],
         "p99_ms": p[1],
         "plain_gpu": plain,

@@ -342,8 +342,13 @@ class Context:
         self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))
 
     def set_engine_param(self, name, value):
-        """Tunables of the FAST engine's exhaustive path (fclx_set_engine_param)."""
+        """Engine tunables (fclx_set_engine_param); results do not depend on them."""
         self._check(self._L.fclx_set_engine_param(self._h, name.encode(), float(value)))
+
+    def get_engine_param(self, name):
+        v = N.c_dbl(0)
+        self._check(self._L.fclx_get_engine_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def engine_report(self):
         r = N.EngineReport()

@@ -111,6 +111,7 @@ SIGNATURES = {
     "fclx_set_profiling": (c_i32, [c_vp, c_i32]),
     "fclx_engine_last_report": (c_i32, [c_vp, ctypes.POINTER(EngineReport)]),
     "fclx_set_engine_param": (c_i32, [c_vp, ctypes.c_char_p, c_dbl]),
+    "fclx_get_engine_param": (c_i32, [c_vp, ctypes.c_char_p, ctypes.POINTER(c_dbl)]),
     "fclx_init": (c_i32, [c_i32, ctypes.POINTER(c_vp)]),
     "fclx_shutdown": (None, [c_vp]),
     "fclx_last_error": (ctypes.c_char_p, [c_vp]),

@@ -684,6 +684,36 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable) {
   return FCLX_OK;
 }
 
+int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
+  if (!ctx || !name || !value || !ctx->ws) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->ws_mu);
+  const EngineTuning& t = engine_tuning(ctx->ws);
+  const std::string n(name);
+  if (n == "max_levels") *value = t.max_levels;
+  else if (n == "front_target") *value = static_cast<double>(t.front_target);
+  else if (n == "dfs_blocks_per_cu") *value = t.dfs_blocks_per_cu;
+  else if (n == "defer_leaf") *value = static_cast<double>(t.defer_leaf);
+  else if (n == "defer_uncertain") *value = static_cast<double>(t.defer_uncertain);
+  else if (n == "idle_stop") *value = t.idle_stop;
+  else if (n == "stop_poll") *value = t.stop_poll;
+  else if (n == "dfs_waves_per_simd") *value = t.dfs_waves_per_simd;
+  else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
+  else if (n == "bool_waves_per_simd") *value = t.bool_waves_per_simd;
+  else if (n == "bool_width") *value = t.bool_width;
+  else if (n == "bool_refill") *value = t.bool_refill;
+  else if (n == "bool_leaf_batch") *value = t.bool_leaf_batch;
+  else if (n == "epa_blocks_per_cu") {
+    std::lock_guard<std::mutex> sl(ctx->sws_mu);
+    *value = ctx->epa_blocks_per_cu;
+  } else if (n == "shape_waves_per_simd") {
+    std::lock_guard<std::mutex> sl(ctx->sws_mu);
+    *value = ctx->shape_waves;
+  } else {
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter");
+  }
+  return FCLX_OK;
+}
+
 int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
   if (!ctx || !name || !ctx->ws) return FCLX_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->ws_mu);

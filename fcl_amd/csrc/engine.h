@@ -24,7 +24,8 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
 // Tunables of the exhaustive path, per context (defaults chosen by measurement).
 struct EngineTuning {
   int max_levels = 24;          // breadth-first levels before a depth-first round
-  long long front_target = 1 << 22;  // stop expanding once the front is this large
+  long long front_target = 1 << 18;  // stop expanding once the front is this large (a full
+                                     // step for every resident wavefront)
   int dfs_blocks_per_cu = 8;    // persistent grid cap: blocks of 4 wavefronts per CU
   long long defer_leaf = 1LL << 27;       // deferred leaf-pair buffer (entries)
   long long defer_uncertain = 1LL << 24;  // deferred uncertain-task buffer (entries)

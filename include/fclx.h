@@ -263,6 +263,7 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  * in the reference).  name / accepted range:
  *   "max_levels"        1..64   breadth-first levels before a depth-first round
  *   "front_target"      >= 64   stop expanding once the front holds this many tasks
+ *                               (default 1 << 18)
  *   "dfs_blocks_per_cu" 1..32   cap on resident depth-first blocks per CU
  *   "defer_leaf"        >= 64   deferred leaf-pair buffer, entries
  *   "defer_uncertain"   >= 64   deferred uncertain-task buffer, entries
@@ -288,6 +289,9 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               EPA kernels (default 2)
  * Results do not depend on them; smaller buffers mean more rounds. */
 int fclx_set_engine_param(fclx_context* ctx, const char* name, double value);
+/* The current value of a tunable above (for tuners); no reference
+ * counterpart. */
+int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value);
 int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out);
 
 /* Wait for the stream, then report (and clear) the device status: FCLX_OK,
