@@ -41,7 +41,7 @@ BYTES_LEAF = 144       # six FP64 vertices
 # copy of known size (python bench.py --copy-calibration); keyed by
 # (queries, pairs).  profiles/engine_dfs_traffic.md holds the runs.
 TRAFFIC = {
-    (1_000_000, 64): {"bytes": 7.300e12,
+    (1_000_000, 64): {"bytes": 6.596e12,
                       "source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), "
                                 "engine_dfs launches of one step"},
 }
