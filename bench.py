@@ -34,6 +34,8 @@ OPS_BV = 315           # FP32 ops of one OBB test (126 transform + 189 SAT)
 OPS_LEAF = 923         # FP64 ops of one triangle SAT
 BYTES_BV = 128         # two 64-B FP32 node records
 BYTES_LEAF = 144       # six FP64 vertices
+OPS_RSS = 380          # ops of one RSS lower bound (126 transform + ~250 rectDistance)
+OPS_TRI_DIST = 900     # FP64 ops of one triangle distance
 
 # HBM bytes the depth-first kernel moves per step (summed over its launches,
 # one per round), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over
@@ -226,6 +228,144 @@ def bench_primitives(args):
     ctx.close()
 
 
+def bench_distance(args):
+    """Mesh distance: two perturbed icospheres of 20,480 triangles in
+    BVHModel<RSS>, 10,000 separated poses (|t| uniform in [2.1, 4] along a
+    uniform direction), DistanceRequest with nearest points; one step = one
+    distance batch over all poses, inputs resident."""
+    import numpy as np
+    import torch
+
+    import fcl_amd as F
+
+    n = args.dist_queries
+    t0 = time.time()
+    (V1, T1), (V2, T2) = F.icosphere(32, 0.15, 0xD0), F.icosphere(32, 0.15, 0xD1)
+    tf1, tf2 = F.poses(n, 1, 2.1, 4.0, 0xD2)
+    ctx = F.Context(0)
+    for kv in args.engine_param:
+        k, v = kv.split("=", 1)
+        ctx.set_engine_param(k, float(v))
+    g = np.array([ctx.mesh(V1, T1, F.RSS), ctx.mesh(V2, T2, F.RSS)], np.int32)
+    log("distance workload: 2 x %d triangles (RSS), %d poses, built in %.1fs"
+        % (len(T1), n, time.time() - t0))
+    dev = "cuda:0"
+    qm = torch.from_numpy(np.tile(g, (n, 1))).to(dev)
+    d1, d2 = torch.from_numpy(tf1).to(dev), torch.from_numpy(tf2).to(dev)
+    dist = torch.zeros(n, dtype=torch.float64, device=dev)
+    pts = torch.zeros((n, 6), dtype=torch.float64, device=dev)
+    ids = torch.zeros((n, 2), dtype=torch.int32, device=dev)
+    bvt = torch.zeros(n, dtype=torch.int64, device=dev)
+    lft = torch.zeros(n, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def run(stats=False, engine=F.ENGINE_FAST):
+        ctx.distance_batch_async(qm, d1, d2, dist, pts, ids, engine=engine,
+                                 bv_tests=bvt if stats else None,
+                                 leaf_tests=lft if stats else None, stream=s)
+
+    # the plain kernel: the reference's visit order; its test counts are the
+    # reference's (the roofline's work), its distances the comparison
+    a = time.perf_counter()
+    run(True, F.ENGINE_PLAIN)
+    ctx.sync(s)
+    plain_s = time.perf_counter() - a
+    plain_d = dist.clone()
+    w_bv, w_leaf = int(bvt.sum().item()), int(lft.sum().item())
+    run(True)
+    ctx.sync(s)
+    fast_bv, fast_leaf = int(bvt.sum().item()), int(lft.sum().item())
+    exact = bool(torch.equal(dist, plain_d))
+    for _ in range(args.warmup):
+        run()
+    ctx.sync(s)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t_start = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        run()
+    ev[1].record(stream)
+    ctx.sync(s)
+    elapsed = time.perf_counter() - t_start
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n / (elapsed / args.steps)
+    t_k = gpu_ms * 1e-3
+    ops = w_bv * OPS_RSS + w_leaf * OPS_TRI_DIST
+    alg_bytes = w_bv * BYTES_BV + w_leaf * BYTES_LEAF
+    roofline = {
+        "kernel": "split distance engine (distance_subtrees dominates), FP64",
+        "bound": "valu",
+        "achieved": ops / t_k / 1e12,
+        "peak": PEAK_FP64 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": ops / t_k / PEAK_FP64,
+        "traffic": None,
+        "algorithmic_ops": ops,
+        "ops_per_test": {"rss_bound": OPS_RSS, "triangle_distance": OPS_TRI_DIST},
+        "bv_tests": w_bv, "leaf_tests": w_leaf,
+        "executed_bv_tests": fast_bv, "executed_leaf_tests": fast_leaf,
+        "hbm_frac": alg_bytes / t_k / PEAK_HBM,
+        "algorithmic_bytes": alg_bytes,
+        "batch_ms_events": gpu_ms,
+    }
+    d = dist.cpu().numpy()
+    log("distance: %.2f ms/step (events %.2f), %.3g queries/s; %.0f RSS and %.0f triangle "
+        "tests per query (the reference's: %.0f and %.0f); %d of %d touching" %
+        (ms_per_step, gpu_ms, value, fast_bv / n, fast_leaf / n, w_bv / n, w_leaf / n,
+         int((d == 0).sum()), n))
+    log("plain: %.1f ms, %.3g queries/s, distances equal %s" % (plain_s * 1e3, n / plain_s, exact))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        threads = host_threads()
+        om = [fclo.OracleMesh(V1, T1, fclo.RSS), fclo.OracleMesh(V2, T2, fclo.RSS)]
+        rng = np.random.default_rng(7)
+        qpairs = np.tile(np.array([0, 1], np.int32), (n, 1))
+        probe = np.sort(rng.choice(n, size=min(n, 200), replace=False))
+        a = time.perf_counter()
+        fclo.distance_batch(om, qpairs[probe], tf1[probe], tf2[probe], threads=threads)
+        probe_s = time.perf_counter() - a
+        n_s = int(min(n, max(len(probe), len(probe) * args.cpu_seconds / max(probe_s, 1e-3))))
+        sample = np.sort(rng.choice(n, size=n_s, replace=False))
+        a = time.perf_counter()
+        od = fclo.distance_batch(om, qpairs[sample], tf1[sample], tf2[sample], threads=threads)[0]
+        cpu_s = time.perf_counter() - a
+        match = bool(np.array_equal(od, d[sample]))
+        cpu = {"value": n_s / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d poses, drawn uniformly (seed 7), OpenMP dynamic "
+                         "schedule; distances bit-equal to the GPU's: %s" % (n_s, n, match)}
+        log("cpu oracle: %d queries in %.1fs on %d threads, equal %s" % (n_s, cpu_s, threads,
+                                                                         match))
+    out = {
+        "metric": "mesh distance queries/sec (DistanceRequest with nearest points)",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded perturbed icospheres, separated poses)",
+        "config": {"workload": "mesh distance: 2 icospheres of 20,480 tris, BVHModel<RSS>, "
+                               "|t| in [2.1, 4], enable_nearest_points",
+                   "queries": n},
+        "plain_gpu": {"value": n / plain_s, "ms": plain_s * 1e3},
+        "speedup_vs_plain": value / (n / plain_s),
+        "exact_match_vs_plain": exact,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def bench_serving(args):
     """Robot-trajectory serving: one resident ~500k-triangle scene, a
     20-link robot, requests of 64-4096 waypoints arriving as Poisson
@@ -386,10 +526,11 @@ def bench_serving(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["mesh", "primitives", "serving"], default="mesh",
+    ap.add_argument("--workload", choices=["mesh", "primitives", "serving", "distance"],
+                    default="mesh",
                     help="mesh: the batched uneven mesh collision (default); primitives: "
                          "4M primitive-shape pairs; serving: robot-trajectory requests at "
-                         "Poisson loads")
+                         "Poisson loads; distance: 10k mesh distance queries (20k-tri RSS)")
     ap.add_argument("--serve-rates", type=float, nargs="+",
                     default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--serve-seconds", type=float, default=1.0,
@@ -399,6 +540,7 @@ def main():
     ap.add_argument("--serve-engine", choices=["bool", "fast", "plain"], default="bool",
                     help="engine behind the request loop (bool: booleans only)")
     ap.add_argument("--prim-pairs", type=int, default=4_000_000)
+    ap.add_argument("--dist-queries", type=int, default=10_000)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -429,6 +571,9 @@ def main():
         return
     if args.workload == "serving":
         bench_serving(args)
+        return
+    if args.workload == "distance":
+        bench_distance(args)
         return
 
     import numpy as np

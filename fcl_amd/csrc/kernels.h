@@ -91,5 +91,8 @@ void shape_workspace_set_waves(ShapeWorkspace* w, int waves);
 size_t shape_epa_lane_bytes();
 hipError_t shape_collide_run(ShapeWorkspace* w, const ShapeLaunch& L, hipStream_t s);
 hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s);
+// FAST engine: each query split into up to 32 subtrees run on lanes of their
+// own (work buffers from the stream-ordered allocator)
+hipError_t launch_distance_split(const DistanceLaunch& L, int stack_need, hipStream_t s);
 
 }  // namespace fclx

@@ -112,7 +112,15 @@ typedef struct {
   const double* tf1;           /* device */
   const double* tf2;           /* device */
   int32_t enable_nearest_points;
-  int32_t engine;
+  int32_t engine;              /* FCLX_ENGINE_PLAIN: one query per lane, the
+                                  reference's visit order, every output and
+                                  statistic as the reference's.
+                                  FCLX_ENGINE_FAST: each query split into up
+                                  to 32 subtrees searched in parallel with a
+                                  shared best distance: the same min_distance;
+                                  ids and points can name another pair of
+                                  equal distance; the statistics count this
+                                  engine's own tests */
   double* distance;            /* device: min_distance per query */
   double* points;              /* device, optional: 6 per query, world frame */
   int32_t* ids;                /* device, optional: (b1, b2) per query */

@@ -327,6 +327,45 @@ def test_distance_parity(ctx, bv_type):
         assert d == od[q] and tuple(i) == tuple(oids[q]) and tuple(st) == (obv[q], olf[q])
 
 
+@pytest.mark.parametrize("bv_type", [F.RSS, F.OBBRSS])
+def test_distance_fast_engine(ctx, bv_type):
+    """FCLX_ENGINE_FAST distance (queries split into subtrees searched in
+    parallel): min_distance, the reported pair and the nearest points
+    bit-identical to the oracle's, on separated and on overlapping poses
+    (ties between neighbouring triangles are common; the engine reports the
+    pair the reference finds first)."""
+    (V1, T1), (V2, T2) = F.icosphere(14, 0.15, 0xD0), F.icosphere(9, 0.15, 0xD3)
+    o = [fclo.OracleMesh(V1, T1, bv_type), fclo.OracleMesh(V2, T2, bv_type)]
+    g = [ctx.mesh(V1, T1, bv_type), ctx.mesh(V2, T2, bv_type)]
+    n = 600
+    a1, a2 = F.poses(n // 2, 1, 2.1, 4.0, 0xD4)
+    b1, b2 = F.poses(n - n // 2, 1, 0.3, 2.2, 0xD5)
+    tf1, tf2 = np.concatenate([a1, b1]), np.concatenate([a2, b2])
+    od, opts, oids, _, _ = fclo.distance_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2)
+    print("case: fast distance bv=%d n=%d" % (bv_type, n), flush=True)
+    qm = np.tile(g, (n, 1))
+    dist = torch.zeros(n, dtype=torch.float64, device=DEV)
+    pts = torch.zeros((n, 6), dtype=torch.float64, device=DEV)
+    ids = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
+    bvt = torch.zeros(n, dtype=torch.int64, device=DEV)
+    lft = torch.zeros(n, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.distance_batch_async(_t(qm, torch.int32), _t(tf1, torch.float64), _t(tf2, torch.float64),
+                             dist, pts, ids, engine=F.ENGINE_FAST, bv_tests=bvt, leaf_tests=lft,
+                             stream=s)
+    assert ctx.sync_status(s) == 0
+    gd = dist.cpu().numpy()
+    gi = ids.cpu().numpy()
+    gp = pts.cpu().numpy()
+    assert np.array_equal(gd.view(np.uint64), od.view(np.uint64))
+    pos = od > 0
+    assert 0.2 < pos.mean() < 0.9, "both separated and touching poses: %g" % pos.mean()
+    bad = np.flatnonzero(np.any(gi != oids, axis=1))
+    assert len(bad) == 0, "queries %s report another pair than the oracle" % bad[:10]
+    assert np.array_equal(gp.view(np.uint64), opts.view(np.uint64))
+    assert np.all(bvt.cpu().numpy() > 0) and np.all(lft.cpu().numpy() >= 0)
+
+
 def test_bad_queries_are_reported(ctx, c0_meshes):
     o, g = c0_meshes
     tf1, tf2 = F.poses(4, 0, 2.2, 0, 1)
