@@ -295,7 +295,7 @@ def bench_distance(args):
     ops = w_bv * OPS_RSS + w_leaf * OPS_TRI_DIST
     alg_bytes = w_bv * BYTES_BV + w_leaf * BYTES_LEAF
     roofline = {
-        "kernel": "split distance engine (distance_subtrees dominates), FP64",
+        "kernel": "engine_distance_wave (wavefront-wide, FP64)",
         "bound": "valu",
         "achieved": ops / t_k / 1e12,
         "peak": PEAK_FP64 / 1e12,

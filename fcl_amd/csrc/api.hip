@@ -535,7 +535,7 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
   L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
   L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
   L.status = ctx->d_status;
-  hipError_t e = a->engine == FCLX_ENGINE_FAST ? launch_distance_split(L, 2 * depth, s)
+  hipError_t e = a->engine == FCLX_ENGINE_FAST ? launch_distance_wave(L, 2 * depth, s)
                                                : launch_distance_plain(L, 2 * depth, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "distance launch");
   return FCLX_OK;

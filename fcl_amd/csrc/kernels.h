@@ -94,5 +94,8 @@ hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStr
 // FAST engine: each query split into up to 32 subtrees run on lanes of their
 // own (work buffers from the stream-ordered allocator)
 hipError_t launch_distance_split(const DistanceLaunch& L, int stack_need, hipStream_t s);
+// FAST engine: wavefront-wide search with the reference's visit order kept
+// as path keys (engine_dist.hip); the split engine for BVTTs deeper than 62
+hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStream_t s);
 
 }  // namespace fclx
