@@ -421,7 +421,7 @@ __global__ void dist_zero(unsigned long long* p) { *p = 0; }
 hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStream_t s) {
   if (L.num_queries == 0) return hipSuccess;
   // path keys hold 63 levels of descent
-  if (stack_need > 62) return launch_distance_split(L, stack_need, s);
+  if (stack_need > 62) return launch_distance_plain(L, stack_need, s);
   int dev = 0, cus = 0;
   hipError_t e;
   if ((e = hipGetDevice(&dev)) != hipSuccess ||

@@ -91,11 +91,9 @@ void shape_workspace_set_waves(ShapeWorkspace* w, int waves);
 size_t shape_epa_lane_bytes();
 hipError_t shape_collide_run(ShapeWorkspace* w, const ShapeLaunch& L, hipStream_t s);
 hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s);
-// FAST engine: each query split into up to 32 subtrees run on lanes of their
-// own (work buffers from the stream-ordered allocator)
-hipError_t launch_distance_split(const DistanceLaunch& L, int stack_need, hipStream_t s);
 // FAST engine: wavefront-wide search with the reference's visit order kept
-// as path keys (engine_dist.hip); the split engine for BVTTs deeper than 62
+// as path keys (engine_dist.hip); BVTTs deeper than 62 levels go to the
+// plain kernel
 hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStream_t s);
 
 }  // namespace fclx

@@ -13,8 +13,6 @@
 // re-checked against the current minimum when it is popped.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
-
 #include "device/dist64.h"
 #include "device/pose64.h"
 #include "kernels.h"
@@ -282,323 +280,6 @@ hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStrea
   else
     collide_plain_kernel<256><<<grid, block, 0, s>>>(L);
   return hipGetLastError();
-}
-
-// ------------------------------------------------ split distance (FAST)
-// The plain kernel runs one query per lane, so a batch of 10,000 queries
-// fills 157 wavefronts of a device that holds 4,096 at once.  Here each
-// query's BVTT is first split kDistSplitLevels levels down by the descent
-// rule (both children kept, their lower bounds computed), giving up to
-// kDistSplit subtrees.  Every subtree then runs the plain kernel's
-// depth-first search on a lane of its own, pruned by the smaller of its own
-// best distance and its query's best so far, which the lanes of a query
-// share through an atomic minimum on the distance's bits (distances are
-// non-negative, so their bits order as their values).  A last kernel takes
-// each query's minimum: the seed pair (triangle 0 with triangle 0) first,
-// then the subtrees in the reference's visit order (at every split the
-// child of smaller bound first), the first of equal distances kept.
-//
-// Every pruned subtree's bound is at least a distance some pair reaches, so
-// the minimum equals the reference's.  Pairs of equal distance are common
-// (neighbouring triangles share the nearest vertex); the query's shared
-// best prunes only bounds above it, so the first pair of the minimum
-// distance in the reference's order is still found and reported.
-// num_bv_tests and num_leaf_tests count this engine's own tests.
-constexpr int kDistSplitLevels = 5;
-constexpr int kDistSplit = 1 << kDistSplitLevels;
-constexpr int kDistChunk = 1 << 17;  // queries per launch group (bounds the work buffers)
-
-struct DistTask {
-  int b1, b2;
-  double d;  // lower bound of the pair (negative: none, the root)
-};
-struct DistBest {
-  double d;
-  int b1, b2;
-  double P[3], Q[3];  // model-1 frame
-};
-
-__device__ __forceinline__ double leaf_distance(const MeshDev& m1, const MeshDev& m2, const M3& R,
-                                                const V3& T, int id1, int id2, V3& P, V3& Q) {
-  const int32_t* t1 = m1.tris + 3 * static_cast<size_t>(id1);
-  const int32_t* t2 = m2.tris + 3 * static_cast<size_t>(id2);
-  const V3 S[3] = {vertex(m1, t1[0]), vertex(m1, t1[1]), vertex(m1, t1[2])};
-  const V3 Tt[3] = {mul(R, vertex(m2, t2[0])) + T, mul(R, vertex(m2, t2[1])) + T,
-                    mul(R, vertex(m2, t2[2])) + T};
-  return tri_distance64(S, Tt, P, Q);
-}
-
-__device__ __forceinline__ unsigned long long dist_bits(double d) {
-  return static_cast<unsigned long long>(__double_as_longlong(d));
-}
-
-__global__ void __launch_bounds__(256) distance_split(DistanceLaunch L, DistTask* tasks,
-                                                      int* ntask, DistBest* seed,
-                                                      unsigned long long* gbest) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= L.num_queries) return;
-  const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
-  ntask[q] = 0;
-  if (!query_ok(L.meshes, L.num_meshes, i1, i2, /*need_obbrss=*/false)) {
-    L.distance[q] = 1.7976931348623157e308;
-    if (L.ids) {
-      L.ids[2 * q] = -1;
-      L.ids[2 * q + 1] = -1;
-    }
-    if (L.bv_tests) L.bv_tests[q] = 0;
-    if (L.leaf_tests) L.leaf_tests[q] = 0;
-    atomicOr(L.status, kStatusBadQuery);
-    return;
-  }
-  const MeshDev m1 = L.meshes[i1];
-  const MeshDev m2 = L.meshes[i2];
-  M3 R;
-  V3 T;
-  inverse_compose(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q), R, T);
-  V3 P, Q;
-  const double d0 = leaf_distance(m1, m2, R, T, 0, 0, P, Q);  // distancePreprocessOrientedNode
-  DistBest sd;
-  sd.d = d0;
-  sd.b1 = 0;
-  sd.b2 = 0;
-  sd.P[0] = P.x; sd.P[1] = P.y; sd.P[2] = P.z;
-  sd.Q[0] = Q.x; sd.Q[1] = Q.y; sd.Q[2] = Q.z;
-  seed[q] = sd;
-  gbest[q] = dist_bits(d0);
-  DistTask* out = tasks + static_cast<size_t>(q) * kDistSplit;
-  DistTask cur[kDistSplit];
-  int n = 1;
-  cur[0] = DistTask{0, 0, -1.0};
-  long long bv = 0;
-  for (int level = 0; level < kDistSplitLevels; ++level) {
-    DistTask nxt[kDistSplit];
-    int m = 0;
-    for (int i = 0; i < n; ++i) {
-      const DistTask t = cur[i];
-      const int fc1 = m1.rss64[t.b1].first_child, fc2 = m2.rss64[t.b2].first_child;
-      if (fc1 < 0 && fc2 < 0) {
-        nxt[m++] = t;
-        continue;
-      }
-      int a1, a2, c1, c2;
-      if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[t.b1] > m2.size64[t.b2]))) {
-        a1 = fc1; a2 = t.b2; c1 = fc1 + 1; c2 = t.b2;
-      } else {
-        a1 = t.b1; a2 = fc2; c1 = t.b1; c2 = fc2 + 1;
-      }
-      const double d1 = rss_distance64(R, T, m1.rss64[a1], m2.rss64[a2]);
-      const double d2 = rss_distance64(R, T, m1.rss64[c1], m2.rss64[c2]);
-      bv += 2;
-      // in the reference's visit order: the nearer child first, a on ties
-      if (d2 < d1) {
-        nxt[m++] = DistTask{c1, c2, d2};
-        nxt[m++] = DistTask{a1, a2, d1};
-      } else {
-        nxt[m++] = DistTask{a1, a2, d1};
-        nxt[m++] = DistTask{c1, c2, d2};
-      }
-    }
-    for (int i = 0; i < m; ++i) cur[i] = nxt[i];
-    n = m;
-  }
-  for (int i = 0; i < n; ++i) out[i] = cur[i];
-  ntask[q] = n;
-  if (L.bv_tests) L.bv_tests[q] = bv;
-  if (L.leaf_tests) L.leaf_tests[q] = 0;
-}
-
-template <int STACK>
-__global__ void __launch_bounds__(256) distance_subtrees(DistanceLaunch L, const DistTask* tasks,
-                                                         const int* ntask,
-                                                         unsigned long long* gbest,
-                                                         DistBest* best_out) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int q = static_cast<int>(i / kDistSplit), k = static_cast<int>(i % kDistSplit);
-  if (q >= L.num_queries || k >= ntask[q]) return;
-  const DistTask t0 = tasks[i];
-  const MeshDev m1 = L.meshes[L.query_meshes[2 * q]];
-  const MeshDev m2 = L.meshes[L.query_meshes[2 * q + 1]];
-  M3 R;
-  V3 T;
-  inverse_compose(L.tf1 + 12 * static_cast<size_t>(q), L.tf2 + 12 * static_cast<size_t>(q), R, T);
-  long long bv_tests = 0, leaf_tests = 0;
-  double best = 1.7976931348623157e308;
-  int best1 = -1, best2 = -1;
-  V3 bestP = v3(0, 0, 0), bestQ = v3(0, 0, 0);
-  // The query's best so far (refreshed every kRefresh visits) prunes only
-  // pairs whose bound exceeds it, the subtree's own best as the reference
-  // prunes (bound >= best): every pair the reference could report first is
-  // then still reached, in the reference's order within the subtree.
-  constexpr int kRefresh = 16;
-  double shared = __longlong_as_double(static_cast<long long>(
-      __hip_atomic_load(gbest + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-  auto keep = [&](double d) { return !(d >= best) && !(d > shared); };
-  bool overflow = false;
-  if (t0.d < 0 || keep(t0.d)) {
-    DistEntry stack[STACK];
-    int sp = 0;
-    int b1 = t0.b1, b2 = t0.b2;
-    int visits = 0;
-    while (true) {
-      if (++visits % kRefresh == 0)
-        shared = __longlong_as_double(static_cast<long long>(
-            __hip_atomic_load(gbest + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-      const RssNode64& n1 = m1.rss64[b1];
-      const RssNode64& n2 = m2.rss64[b2];
-      const int fc1 = n1.first_child, fc2 = n2.first_child;
-      bool have_next = false;
-      if (fc1 < 0 && fc2 < 0) {
-        ++leaf_tests;
-        V3 P, Q;
-        const int id1 = -(fc1 + 1), id2 = -(fc2 + 1);
-        const double d = leaf_distance(m1, m2, R, T, id1, id2, P, Q);
-        if (best > d) {
-          best = d;
-          best1 = id1;
-          best2 = id2;
-          bestP = P;
-          bestQ = Q;
-          if (d < shared) {
-            atomicMin(gbest + q, dist_bits(d));
-            shared = d;
-          }
-        }
-      } else {
-        int a1, a2, c1, c2;
-        if ((fc2 < 0) || (fc1 >= 0 && (m1.size64[b1] > m2.size64[b2]))) {
-          a1 = fc1; a2 = b2; c1 = fc1 + 1; c2 = b2;
-        } else {
-          a1 = b1; a2 = fc2; c1 = b1; c2 = fc2 + 1;
-        }
-        const double d1 = rss_distance64(R, T, m1.rss64[a1], m2.rss64[a2]);
-        const double d2 = rss_distance64(R, T, m1.rss64[c1], m2.rss64[c2]);
-        bv_tests += 2;
-        int n1a, n1b, f1a, f1b;
-        double dn, df;
-        if (d2 < d1) {
-          n1a = c1; n1b = c2; dn = d2; f1a = a1; f1b = a2; df = d1;
-        } else {
-          n1a = a1; n1b = a2; dn = d1; f1a = c1; f1b = c2; df = d2;
-        }
-        if (sp >= STACK) {
-          overflow = true;
-          break;
-        }
-        stack[sp++] = DistEntry{f1a, f1b, df};
-        if (keep(dn)) {
-          b1 = n1a;
-          b2 = n1b;
-          have_next = true;
-        }
-      }
-      if (have_next) continue;
-      bool found = false;
-      while (sp > 0) {
-        const DistEntry e = stack[--sp];
-        if (keep(e.d)) {
-          b1 = e.b1;
-          b2 = e.b2;
-          found = true;
-          break;
-        }
-      }
-      if (!found) break;
-    }
-  }
-  DistBest r;
-  r.d = best;
-  r.b1 = best1;
-  r.b2 = best2;
-  r.P[0] = bestP.x; r.P[1] = bestP.y; r.P[2] = bestP.z;
-  r.Q[0] = bestQ.x; r.Q[1] = bestQ.y; r.Q[2] = bestQ.z;
-  best_out[i] = r;
-  if (L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(L.bv_tests + q),
-                            static_cast<unsigned long long>(bv_tests));
-  if (L.leaf_tests) atomicAdd(reinterpret_cast<unsigned long long*>(L.leaf_tests + q),
-                              static_cast<unsigned long long>(leaf_tests));
-  if (overflow) atomicOr(L.status, kStatusStackOverflow);
-}
-
-__global__ void __launch_bounds__(256) distance_reduce(DistanceLaunch L, const int* ntask,
-                                                       const DistBest* seed,
-                                                       const DistBest* best) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= L.num_queries) return;
-  const int i1 = L.query_meshes[2 * q], i2 = L.query_meshes[2 * q + 1];
-  if (!query_ok(L.meshes, L.num_meshes, i1, i2, /*need_obbrss=*/false)) return;  // written
-  DistBest r = seed[q];
-  const DistBest* b = best + static_cast<size_t>(q) * kDistSplit;
-  for (int k = 0; k < ntask[q]; ++k)
-    if (b[k].d < r.d) r = b[k];
-  L.distance[q] = r.d;
-  if (L.ids) {
-    L.ids[2 * q] = r.b1;
-    L.ids[2 * q + 1] = r.b2;
-  }
-  if (L.points) {
-    M3 R1;
-    V3 t1;
-    load_pose(L.tf1 + 12 * static_cast<size_t>(q), R1, t1);
-    const V3 wp = mul(R1, v3(r.P[0], r.P[1], r.P[2])) + t1;
-    const V3 wq = mul(R1, v3(r.Q[0], r.Q[1], r.Q[2])) + t1;
-    double* o = L.points + 6 * static_cast<size_t>(q);
-    o[0] = wp.x; o[1] = wp.y; o[2] = wp.z;
-    o[3] = wq.x; o[4] = wq.y; o[5] = wq.z;
-  }
-}
-
-hipError_t launch_distance_split(const DistanceLaunch& L, int stack_need, hipStream_t s) {
-  if (L.num_queries == 0) return hipSuccess;
-  const long long chunk = std::min<long long>(L.num_queries, kDistChunk);
-  DistTask* tasks = nullptr;
-  DistBest *seed = nullptr, *best = nullptr;
-  int* ntask = nullptr;
-  unsigned long long* gbest = nullptr;
-  hipError_t e;
-  if ((e = hipMallocAsync(reinterpret_cast<void**>(&tasks),
-                          sizeof(DistTask) * kDistSplit * chunk, s)) != hipSuccess ||
-      (e = hipMallocAsync(reinterpret_cast<void**>(&best),
-                          sizeof(DistBest) * kDistSplit * chunk, s)) != hipSuccess ||
-      (e = hipMallocAsync(reinterpret_cast<void**>(&seed), sizeof(DistBest) * chunk, s)) !=
-          hipSuccess ||
-      (e = hipMallocAsync(reinterpret_cast<void**>(&ntask), sizeof(int) * chunk, s)) !=
-          hipSuccess ||
-      (e = hipMallocAsync(reinterpret_cast<void**>(&gbest), sizeof(unsigned long long) * chunk,
-                          s)) != hipSuccess) {
-    for (void* p : {static_cast<void*>(tasks), static_cast<void*>(best),
-                    static_cast<void*>(seed), static_cast<void*>(ntask),
-                    static_cast<void*>(gbest)})
-      if (p) (void)hipFreeAsync(p, s);
-    return e;
-  }
-  for (long long q0 = 0; q0 < L.num_queries && e == hipSuccess; q0 += chunk) {
-    DistanceLaunch C = L;
-    C.num_queries = static_cast<int32_t>(std::min<long long>(chunk, L.num_queries - q0));
-    C.query_meshes = L.query_meshes + 2 * q0;
-    C.tf1 = L.tf1 + 12 * q0;
-    C.tf2 = L.tf2 + 12 * q0;
-    C.distance = L.distance + q0;
-    C.points = L.points ? L.points + 6 * q0 : nullptr;
-    C.ids = L.ids ? L.ids + 2 * q0 : nullptr;
-    C.bv_tests = L.bv_tests ? L.bv_tests + q0 : nullptr;
-    C.leaf_tests = L.leaf_tests ? L.leaf_tests + q0 : nullptr;
-    const unsigned gq = static_cast<unsigned>((C.num_queries + 255) / 256);
-    const unsigned gt =
-        static_cast<unsigned>((static_cast<long long>(C.num_queries) * kDistSplit + 255) / 256);
-    distance_split<<<gq, 256, 0, s>>>(C, tasks, ntask, seed, gbest);
-    if (stack_need <= 64)
-      distance_subtrees<64><<<gt, 256, 0, s>>>(C, tasks, ntask, gbest, best);
-    else if (stack_need <= 128)
-      distance_subtrees<128><<<gt, 256, 0, s>>>(C, tasks, ntask, gbest, best);
-    else
-      distance_subtrees<256><<<gt, 256, 0, s>>>(C, tasks, ntask, gbest, best);
-    distance_reduce<<<gq, 256, 0, s>>>(C, ntask, seed, best);
-    e = hipGetLastError();
-  }
-  for (void* p : {static_cast<void*>(tasks), static_cast<void*>(best), static_cast<void*>(seed),
-                  static_cast<void*>(ntask), static_cast<void*>(gbest)})
-    (void)hipFreeAsync(p, s);
-  return e;
 }
 
 hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStream_t s) {

@@ -115,11 +115,11 @@ typedef struct {
   int32_t engine;              /* FCLX_ENGINE_PLAIN: one query per lane, the
                                   reference's visit order, every output and
                                   statistic as the reference's.
-                                  FCLX_ENGINE_FAST: each query split into up
-                                  to 32 subtrees searched in parallel with a
-                                  shared best distance: the same min_distance;
-                                  ids and points can name another pair of
-                                  equal distance; the statistics count this
+                                  FCLX_ENGINE_FAST: a wavefront searches
+                                  several queries 64 bounds at a time; the
+                                  same min_distance, ids and points as the
+                                  reference (its visit order is kept as path
+                                  keys for ties); the statistics count this
                                   engine's own tests */
   double* distance;            /* device: min_distance per query */
   double* points;              /* device, optional: 6 per query, world frame */

@@ -329,8 +329,8 @@ def test_distance_parity(ctx, bv_type):
 
 @pytest.mark.parametrize("bv_type", [F.RSS, F.OBBRSS])
 def test_distance_fast_engine(ctx, bv_type):
-    """FCLX_ENGINE_FAST distance (queries split into subtrees searched in
-    parallel): min_distance, the reported pair and the nearest points
+    """FCLX_ENGINE_FAST distance (wavefront-wide search of several queries
+    at once): min_distance, the reported pair and the nearest points
     bit-identical to the oracle's, on separated and on overlapping poses
     (ties between neighbouring triangles are common; the engine reports the
     pair the reference finds first)."""
