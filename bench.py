@@ -277,6 +277,16 @@ def bench_distance(args):
     ctx.sync(s)
     fast_bv, fast_leaf = int(bvt.sum().item()), int(lft.sum().item())
     exact = bool(torch.equal(dist, plain_d))
+    tuned = None
+    if args.autotune:
+        from fcl_amd import autotune
+
+        def checked():
+            run()
+            ctx.sync(s)
+            return dist.cpu().numpy().tobytes()
+
+        tuned, _, _ = autotune.tune(ctx, checked, autotune.DIST_SPACE, log=log)
     for _ in range(args.warmup):
         run()
     ctx.sync(s)
@@ -355,7 +365,7 @@ def bench_distance(args):
         "data": "synthetic (seeded perturbed icospheres, separated poses)",
         "config": {"workload": "mesh distance: 2 icospheres of 20,480 tris, BVHModel<RSS>, "
                                "|t| in [2.1, 4], enable_nearest_points",
-                   "queries": n},
+                   "queries": n, "engine_params": args.engine_param, "autotuned": tuned},
         "plain_gpu": {"value": n / plain_s, "ms": plain_s * 1e3},
         "speedup_vs_plain": value / (n / plain_s),
         "exact_match_vs_plain": exact,

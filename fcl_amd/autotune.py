@@ -23,6 +23,13 @@ FAST_SPACE = (
     ("stop_poll", (8, 16, 32)),
 )
 
+DIST_SPACE = (
+    ("dist_queries_per_wave", (2, 4, 8)),
+    ("dist_waves_per_simd", (2, 3)),
+    ("dist_width", (16, 32, 64)),
+    ("dist_leaf_batch", (8, 16, 32, 64)),
+)
+
 BOOL_SPACE = (
     ("bool_queries_per_wave", (0, 4, 8, 16)),
     ("bool_width", (16, 32, 64)),

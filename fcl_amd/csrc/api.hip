@@ -535,7 +535,12 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
   L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
   L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
   L.status = ctx->d_status;
-  hipError_t e = a->engine == FCLX_ENGINE_FAST ? launch_distance_wave(L, 2 * depth, s)
+  DistanceTuning tu;
+  {
+    std::lock_guard<std::mutex> lock(ctx->ws_mu);
+    tu = engine_tuning(ctx->ws).dist;
+  }
+  hipError_t e = a->engine == FCLX_ENGINE_FAST ? launch_distance_wave(L, 2 * depth, tu, s)
                                                : launch_distance_plain(L, 2 * depth, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "distance launch");
   return FCLX_OK;
@@ -698,6 +703,10 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   else if (n == "idle_stop") *value = t.idle_stop;
   else if (n == "stop_poll") *value = t.stop_poll;
   else if (n == "dfs_waves_per_simd") *value = t.dfs_waves_per_simd;
+  else if (n == "dist_queries_per_wave") *value = t.dist.queries_per_wave;
+  else if (n == "dist_waves_per_simd") *value = t.dist.waves_per_simd;
+  else if (n == "dist_width") *value = t.dist.width;
+  else if (n == "dist_leaf_batch") *value = t.dist.leaf_batch;
   else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
   else if (n == "bool_waves_per_simd") *value = t.bool_waves_per_simd;
   else if (n == "bool_width") *value = t.bool_width;
@@ -737,6 +746,14 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.stop_poll = static_cast<int>(value);
   } else if (n == "dfs_waves_per_simd" && in(4, 6)) {
     t.dfs_waves_per_simd = static_cast<int>(value);
+  } else if (n == "dist_queries_per_wave" && (value == 2 || value == 4 || value == 8)) {
+    t.dist.queries_per_wave = static_cast<int>(value);
+  } else if (n == "dist_waves_per_simd" && in(2, 3)) {
+    t.dist.waves_per_simd = static_cast<int>(value);
+  } else if (n == "dist_width" && in(1, 64)) {
+    t.dist.width = static_cast<int>(value);
+  } else if (n == "dist_leaf_batch" && in(1, 64)) {
+    t.dist.leaf_batch = static_cast<int>(value);
   } else if (n == "bool_queries_per_wave" &&
              (value == 0 || value == 4 || value == 8 || value == 16 || value == 32)) {
     t.bool_queries_per_wave = static_cast<int>(value);

@@ -111,8 +111,8 @@ __device__ __forceinline__ double tri_pair(const MeshDev& m1, const MeshDev& m2,
   return tri_distance64(S, Tt, P, Q);
 }
 
-template <int kSlots>
-__global__ void __launch_bounds__(kDBlock, 2) engine_distance_wave(DistanceLaunch L,
+template <int kSlots, int kWavesPerSimd>
+__global__ void __launch_bounds__(kDBlock, kWavesPerSimd) engine_distance_wave(DistanceLaunch L,
                                                                    unsigned long long* next_q,
                                                                    DTask* stk, int slot_cap,
                                                                    int wide_cap, int width,
@@ -418,7 +418,8 @@ __global__ void dist_zero(unsigned long long* p) { *p = 0; }
 
 }  // namespace
 
-hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStream_t s) {
+hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, const DistanceTuning& tu,
+                                hipStream_t s) {
   if (L.num_queries == 0) return hipSuccess;
   // path keys hold 63 levels of descent
   if (stack_need > 62) return launch_distance_plain(L, stack_need, s);
@@ -428,11 +429,12 @@ hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStre
       (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) !=
           hipSuccess)
     return e;
-  constexpr int K = 4;
-  const int width = 32;
-  const int leaf_batch = 16;
+  const int K = tu.queries_per_wave;
+  const int wps = tu.waves_per_simd;
+  const int width = std::max(tu.width, kDWave / K);
+  const int leaf_batch = tu.leaf_batch;
   const long long nw = std::max<long long>(
-      1, std::min<long long>((L.num_queries + K - 1) / K, static_cast<long long>(cus) * 4 * 2));
+      1, std::min<long long>((L.num_queries + K - 1) / K, static_cast<long long>(cus) * 4 * wps));
   const unsigned grid = static_cast<unsigned>((nw + kDWaves - 1) / kDWaves);
   // as the BOOL engine: sorted by level, at most 2 entries per level above
   // the point where steps go one task at a time
@@ -450,8 +452,16 @@ hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStre
     return e;
   }
   dist_zero<<<1, 1, 0, s>>>(ctr);
-  engine_distance_wave<K><<<grid, kDBlock, 0, s>>>(L, ctr, stk, slot_cap, wide_cap, width,
-                                                   leaf_batch);
+#define FCLX_DIST(k, w) \
+  engine_distance_wave<k, w><<<grid, kDBlock, 0, s>>>(L, ctr, stk, slot_cap, wide_cap, width, \
+                                                      leaf_batch)
+  if (K == 2)
+    wps >= 3 ? FCLX_DIST(2, 3) : FCLX_DIST(2, 2);
+  else if (K == 8)
+    wps >= 3 ? FCLX_DIST(8, 3) : FCLX_DIST(8, 2);
+  else
+    wps >= 3 ? FCLX_DIST(4, 3) : FCLX_DIST(4, 2);
+#undef FCLX_DIST
   e = hipGetLastError();
   (void)hipFreeAsync(ctr, s);
   (void)hipFreeAsync(stk, s);

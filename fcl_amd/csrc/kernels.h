@@ -94,6 +94,13 @@ hipError_t launch_distance_plain(const DistanceLaunch& L, int stack_need, hipStr
 // FAST engine: wavefront-wide search with the reference's visit order kept
 // as path keys (engine_dist.hip); BVTTs deeper than 62 levels go to the
 // plain kernel
-hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, hipStream_t s);
+struct DistanceTuning {
+  int queries_per_wave = 4;  // 2, 4, 8
+  int waves_per_simd = 2;    // 2, 3
+  int width = 32;            // most bounds of one query per step
+  int leaf_batch = 16;       // leaf pairs queued before their distances are taken
+};
+hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, const DistanceTuning& tu,
+                                hipStream_t s);
 
 }  // namespace fclx

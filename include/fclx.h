@@ -290,6 +290,13 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               default 32)
  *   "bool_refill"       1..32   free query slots that make a BOOL-engine
  *                               wavefront take new queries (default 4)
+ *   "dist_queries_per_wave" 2, 4, 8  queries a FAST distance wavefront keeps
+ *                               in flight (default 4)
+ *   "dist_waves_per_simd" 2, 3  register budget of the FAST distance kernel
+ *   "dist_width"        1..64   most bounds of one query per distance step
+ *                               (default 32)
+ *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are
+ *                               taken (default 16)
  *   "epa_blocks_per_cu" 0..32   resident blocks per CU of the shape path's EPA
  *                               kernel (0: as many as fit); each lane holds
  *                               a 13 KiB EPA block in device memory

@@ -72,6 +72,7 @@ def test_a_setting_that_changes_the_result_is_an_error(clock):
 
 
 def test_spaces_name_engine_tunables():
-    names = [n for n, _ in autotune.FAST_SPACE + autotune.BOOL_SPACE]
+    spaces = autotune.FAST_SPACE + autotune.BOOL_SPACE + autotune.DIST_SPACE
+    names = [n for n, _ in spaces]
     assert len(names) == len(set(names))
-    assert all(len(v) >= 2 for _, v in autotune.FAST_SPACE + autotune.BOOL_SPACE)
+    assert all(len(v) >= 2 for _, v in spaces)

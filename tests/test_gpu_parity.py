@@ -327,6 +327,45 @@ def test_distance_parity(ctx, bv_type):
         assert d == od[q] and tuple(i) == tuple(oids[q]) and tuple(st) == (obv[q], olf[q])
 
 
+DIST_DEFAULTS = {"dist_queries_per_wave": 4, "dist_waves_per_simd": 2, "dist_width": 32,
+                 "dist_leaf_batch": 16}
+
+
+@pytest.mark.parametrize("params", [
+    {},
+    {"dist_queries_per_wave": 2, "dist_waves_per_simd": 3, "dist_width": 64},
+    {"dist_queries_per_wave": 8, "dist_width": 8, "dist_leaf_batch": 1},
+], ids=["default", "q2w3", "q8narrow"])
+def test_distance_fast_engine_tunables(ctx, params):
+    """The FAST distance engine under other slot counts, widths and leaf
+    batches: the same outputs as the oracle's (OBBRSS meshes, 300 poses, a
+    third of them touching)."""
+    (V1, T1), (V2, T2) = F.icosphere(10, 0.15, 0xD6), F.icosphere(8, 0.15, 0xD7)
+    o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    n = 300
+    tf1, tf2 = F.poses(n, 1, 1.2, 3.5, 0xD8)
+    od, opts, oids, _, _ = fclo.distance_batch(o, np.tile([0, 1], (n, 1)), tf1, tf2)
+    print("case: fast distance tunables %s" % params, flush=True)
+    try:
+        for k, v in params.items():
+            ctx.set_engine_param(k, v)
+        dist = torch.zeros(n, dtype=torch.float64, device=DEV)
+        pts = torch.zeros((n, 6), dtype=torch.float64, device=DEV)
+        ids = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
+        s = torch.cuda.current_stream().cuda_stream
+        ctx.distance_batch_async(_t(np.tile(g, (n, 1)), torch.int32), _t(tf1, torch.float64),
+                                 _t(tf2, torch.float64), dist, pts, ids, engine=F.ENGINE_FAST,
+                                 stream=s)
+        assert ctx.sync_status(s) == 0
+        assert np.array_equal(dist.cpu().numpy().view(np.uint64), od.view(np.uint64))
+        assert np.array_equal(ids.cpu().numpy(), oids)
+        assert np.array_equal(pts.cpu().numpy().view(np.uint64), opts.view(np.uint64))
+    finally:
+        for k, v in DIST_DEFAULTS.items():
+            ctx.set_engine_param(k, v)
+
+
 @pytest.mark.parametrize("bv_type", [F.RSS, F.OBBRSS])
 def test_distance_fast_engine(ctx, bv_type):
     """FCLX_ENGINE_FAST distance (wavefront-wide search of several queries
