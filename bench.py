@@ -573,7 +573,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(queries shard by index across GPUs; see DESIGN.md).", file=sys.stderr)
+              "(queries would shard by index across GPUs).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "primitives":
