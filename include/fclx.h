@@ -63,8 +63,10 @@ extern "C" {
 /* BOOL: only whether each query's pair collides -- counts[q] = 1 or 0, the
  * same as min(collide() count, 1) for any num_max_contacts -- with no
  * records and no statistics (bv_tests / leaf_tests are zeroed); a wavefront
- * works on 8 queries at once and stops a query at its first intersecting
- * triangle pair.  Batches on different streams may run at once.  Batches
+ * keeps 4 or 8 queries in flight, each descending nearly depth first on a
+ * stack of its own, and stops a query at its first intersecting triangle
+ * pair.  Batches on different streams may run at once (up to four without
+ * waiting for each other's work areas).  Batches
  * beyond 2^22 queries or over meshes of 2^20 or more triangles run the
  * PLAIN kernel with a cap of 1 (same answers). */
 #define FCLX_ENGINE_BOOL 2
