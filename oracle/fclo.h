@@ -174,7 +174,9 @@ enum {
   FCLO_SHAPE_SPHERE = 1,   /* dims[0] = radius */
   FCLO_SHAPE_CAPSULE = 2,  /* dims[0] = radius, dims[1] = lz */
   FCLO_SHAPE_CYLINDER = 3, /* dims[0] = radius, dims[1] = lz */
-  FCLO_SHAPE_CONVEX = 4    /* vertices (local frame), num_vertices */
+  FCLO_SHAPE_CONVEX = 4,   /* vertices (local frame), num_vertices */
+  FCLO_SHAPE_TRIANGLE = 5  /* TriangleP: vertices = a, b, c (only as the mesh
+                              side of the mesh-shape leaf test) */
 };
 
 typedef struct {
@@ -217,6 +219,38 @@ void fclo_shape_collide_records(const fclo_shape_rec *recs, const double *pool,
                                 int num_queries, const int *qs, const double *tf1s,
                                 const double *tf2s, int cap, int enable_contact, double tol,
                                 int *counts, fclo_contact *out, int threads, int *passes);
+
+/* ---- mesh vs primitive shape, BVHModel<OBBRSS> (fclo_shape.c) ------------
+ * fcl::collide(mesh, tf1, shape, tf2) for an OBBRSS mesh and a Box, Sphere,
+ * Capsule, Cylinder or Convex (>= 4 vertices) with the GST_INDEP solver:
+ * orientedBVHShapeCollide (collision_func_matrix-inl.h:375-419, 500-521),
+ * the shape's OBBRSS from its bound vertices (utility-inl.h:124-133 with
+ * BV_fitter fitn), descent of the mesh only (traversal_recurse-inl.h:82-129,
+ * BVTesting = OBB overlap of the shape's box with the node's box posed by
+ * tf1), leaf test meshShapeCollisionOrientedNodeLeafTesting
+ * (mesh_shape_collision_traversal_node-inl.h:205-283): sphereTriangleIntersect
+ * for a sphere, GJK + EPA on (shape, TriangleP) for the rest.
+ * Contacts are (b1 = triangle id, b2 = NONE) in the reference's order, at
+ * most cap; with enable_contact each has the reference's point, normal and
+ * penetration depth.  Returns the count; stats optional. */
+void fclo_shape_obb(const fclo_shape *s, const double tf[12], double axis[9], double to[3],
+                    double extent[3]);
+int fclo_mesh_shape_collide(const fclo_model *m, const double tf1[12], const fclo_shape *s,
+                            const double tf2[12], int cap, int enable_contact, double tol,
+                            int *prims, fclo_contact *out, fclo_collide_stats *st);
+/* per query q: mesh models[mesh_ids[q]], shape recs[shape_ids[q]]; prims and
+ * out get cap slots per query (either may be NULL), stats 2 per query */
+void fclo_mesh_shape_collide_batch(const fclo_model *const *models, const int *mesh_ids,
+                                   const fclo_shape_rec *recs, const double *pool,
+                                   int num_queries, const int *shape_ids, const double *tf1s,
+                                   const double *tf2s, int cap, int enable_contact, double tol,
+                                   int *counts, int *prims, fclo_contact *out, int64_t *stats,
+                                   int threads);
+/* sphereTriangleIntersect (sphere_triangle-inl.h:138-230) on its own */
+int fclo_sphere_triangle(double radius, const double center[3], const double P1[3],
+                         const double P2[3], const double P3[3], double point[3],
+                         double *depth, double normal[3]);
+void fclo_axis_from_eigen(const double E[9], const double s[3], double axis[9]);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,13 @@ def lib():
             "fclo_shape_collide_records": (None, [ctypes.c_void_p, P_dbl, c_i32, P_i32, P_dbl,
                                                   P_dbl, c_i32, c_i32, c_dbl, P_i32,
                                                   ctypes.c_void_p, c_i32, P_i32]),
+            "fclo_shape_obb": (None, [ctypes.POINTER(Shape), P_dbl, P_dbl, P_dbl, P_dbl]),
+            "fclo_sphere_triangle": (c_i32, [c_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl,
+                                             P_dbl]),
+            "fclo_mesh_shape_collide_batch": (None, [ctypes.POINTER(PM), P_i32, ctypes.c_void_p,
+                                                     P_dbl, c_i32, P_i32, P_dbl, P_dbl, c_i32,
+                                                     c_i32, c_dbl, P_i32, P_i32,
+                                                     ctypes.c_void_p, P_i64, c_i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -108,7 +115,7 @@ def lib():
     return _lib
 
 
-BOX, SPHERE, CAPSULE, CYLINDER, CONVEX = 0, 1, 2, 3, 4
+BOX, SPHERE, CAPSULE, CYLINDER, CONVEX, TRIANGLE = 0, 1, 2, 3, 4, 5
 
 
 class Shape(ctypes.Structure):
@@ -450,3 +457,55 @@ def eigen_old(M):
     v = np.zeros(9)
     lib().fclo_eigen_old(_p(M, c_dbl), _p(d, c_dbl), _p(v, c_dbl))
     return d, v.reshape(3, 3)
+
+
+def shape_obb(shape, tf):
+    """(axis (3,3), centre, extent) of the OBB computeBV<OBBRSS> gives a posed
+    shape (the generic fit over its bound vertices)."""
+    axis = np.zeros(9)
+    to = np.zeros(3)
+    ext = np.zeros(3)
+    lib().fclo_shape_obb(ctypes.byref(shape), _p(_d(tf), c_dbl), _p(axis, c_dbl),
+                         _p(to, c_dbl), _p(ext, c_dbl))
+    return axis.reshape(3, 3), to, ext
+
+
+def sphere_triangle(radius, center, P1, P2, P3):
+    """sphereTriangleIntersect: (hit, point, depth, normal) as the solver
+    returns them."""
+    point = np.zeros(3)
+    normal = np.zeros(3)
+    depth = np.zeros(1)
+    hit = lib().fclo_sphere_triangle(float(radius), _p(_d(center), c_dbl), _p(_d(P1), c_dbl),
+                                     _p(_d(P2), c_dbl), _p(_d(P3), c_dbl), _p(point, c_dbl),
+                                     _p(depth, c_dbl), _p(normal, c_dbl))
+    return bool(hit), point, float(depth[0]), normal
+
+
+def mesh_shape_collide_batch(meshes, mesh_ids, rec, pool, shape_ids, tf1, tf2, cap=1,
+                             enable_contact=False, tol=1e-6, threads=None):
+    """collide(mesh, tf1, shape, tf2) per query (BVHModel<OBBRSS> vs a
+    primitive, GST_INDEP).  Returns counts (n,), prims (n, cap) -- the
+    contacts' b1 --, contacts (n, cap, 7) = normal, pos, depth, and stats
+    (n, 2) = num_bv_tests, num_leaf_tests."""
+    rec = np.ascontiguousarray(rec)
+    assert rec.dtype.itemsize == 40
+    pool = _d(pool).reshape(-1, 3) if len(pool) else np.zeros((1, 3))
+    mesh_ids = np.ascontiguousarray(mesh_ids, np.int32)
+    shape_ids = np.ascontiguousarray(shape_ids, np.int32)
+    n = shape_ids.shape[0]
+    tf1 = _d(tf1)
+    tf2 = _d(tf2)
+    counts = np.zeros(n, np.int32)
+    prims = np.full((max(1, n * cap),), -1, np.int32)
+    out = np.zeros((max(1, n * cap), 7))
+    stats = np.zeros((max(1, n), 2), np.int64)
+    arr = _model_array(meshes)
+    lib().fclo_mesh_shape_collide_batch(arr, _p(mesh_ids, c_i32), rec.ctypes.data,
+                                        _p(pool, c_dbl), n, _p(shape_ids, c_i32),
+                                        _p(tf1, c_dbl), _p(tf2, c_dbl), cap,
+                                        1 if enable_contact else 0, tol, _p(counts, c_i32),
+                                        _p(prims, c_i32), out.ctypes.data, _p(stats, c_i64),
+                                        threads or (os.cpu_count() or 1))
+    return (counts, prims[: n * cap].reshape(n, cap), out[: n * cap].reshape(n, cap, 7),
+            stats[:n])

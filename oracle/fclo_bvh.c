@@ -50,8 +50,7 @@ static void covariance(const mesh_ref *m, const unsigned *idx, int n,
 
 /* math/geometry-inl.h:561-598 (axisFromEigen): columns 0,1 are the
  * eigenvectors of the largest and middle eigenvalue, column 2 their cross. */
-static void axis_from_eigen(const double E[9], const double s[3],
-                            double axis[9]) {
+void fclo_axis_from_eigen(const double E[9], const double s[3], double axis[9]) {
   int mn, mid, mx;
   if (s[0] > s[1]) { mx = 0; mn = 1; } else { mn = 0; mx = 1; }
   if (s[2] < s[mn]) { mid = mn; mn = 2; }
@@ -226,7 +225,7 @@ static void fit_node(const mesh_ref *m, const unsigned *idx, int n,
   double M[9], E[9], s[3];
   covariance(m, idx, n, M);
   fclo_eigen_old(M, s, E);
-  axis_from_eigen(E, s, node->obb_axis);
+  fclo_axis_from_eigen(E, s, node->obb_axis);
   memcpy(node->rss_axis, node->obb_axis, sizeof(node->obb_axis));
   extent_and_center(m, idx, n, node->obb_axis, node->obb_to, node->obb_extent);
   rss_fit(m, idx, n, node->rss_axis, node->rss_to, node->rss_l, &node->rss_r);

@@ -105,6 +105,16 @@ static v3 get_support(const fclo_shape *s, v3 d) {
       double k = s->dims[0] / zdist;
       return V(k * d.x[0], k * d.x[1], d.x[2] > 0 ? half_h : -half_h);
     }
+    case FCLO_SHAPE_TRIANGLE: {
+      /* minkowski_diff-inl.h:79-100 (TriangleP): the first of the larger
+       * dot products, c only when strictly above */
+      v3 a = V(s->vertices[0], s->vertices[1], s->vertices[2]);
+      v3 b = V(s->vertices[3], s->vertices[4], s->vertices[5]);
+      v3 c = V(s->vertices[6], s->vertices[7], s->vertices[8]);
+      double da = vdot(d, a), db = vdot(d, b), dc = vdot(d, c);
+      if (da > db) return dc > da ? c : a;
+      return dc > db ? c : b;
+    }
     case FCLO_SHAPE_CONVEX: {
       double maxdot = -DBL_MAX;
       v3 best = V(0, 0, 0);
@@ -1403,5 +1413,342 @@ void fclo_shape_collide_records(const fclo_shape_rec *recs, const double *pool,
     counts[q] = n;
     if (out)
       for (int i = 0; i < n && i < cap; ++i) out[(size_t)q * cap + i] = c[i];
+  }
+}
+
+/* ============================================ mesh vs shape (BVHModel<OBBRSS>) */
+
+/* getBoundVertices in the shape's frame (box-inl.h:122-138,
+ * sphere-inl.h:96-120, capsule-inl.h:108-162, cylinder-inl.h:102-126,
+ * convex-inl.h:252-262): the points whose fit bounds the shape.  Returns
+ * their count (<= 36 for the closed shapes). */
+static int bound_points(const fclo_shape *s, v3 *p) {
+  switch (s->type) {
+    case FCLO_SHAPE_BOX: {
+      double a = s->dims[0] / 2, b = s->dims[1] / 2, c = s->dims[2] / 2;
+      int k = 0;
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          for (int l = 0; l < 2; ++l)
+            p[k++] = V(i ? -a : a, j ? -b : b, l ? -c : c);
+      return 8;
+    }
+    case FCLO_SHAPE_SPHERE: {
+      double m = (1 + sqrt(5.0)) / 2.0;
+      double e = s->dims[0] * 6 / (sqrt(27.0) + sqrt(15.0));
+      double a = e, b = m * e;
+      p[0] = V(0, a, b);   p[1] = V(0, -a, b);  p[2] = V(0, a, -b);  p[3] = V(0, -a, -b);
+      p[4] = V(a, b, 0);   p[5] = V(-a, b, 0);  p[6] = V(a, -b, 0);  p[7] = V(-a, -b, 0);
+      p[8] = V(b, 0, a);   p[9] = V(b, 0, -a);  p[10] = V(-b, 0, a); p[11] = V(-b, 0, -a);
+      return 12;
+    }
+    case FCLO_SHAPE_CAPSULE: {
+      double r = s->dims[0], hl = s->dims[1] * 0.5;
+      double m = (1 + sqrt(5.0)) / 2.0;
+      double e = r * 6 / (sqrt(27.0) + sqrt(15.0));
+      double a = e, b = m * e, r2 = r * 2 / sqrt(3.0);
+      for (int h = 0; h < 2; ++h) {
+        double z = h ? -hl : hl;
+        v3 *q = p + 12 * h;
+        q[0] = V(0, a, b + z);   q[1] = V(0, -a, b + z);
+        q[2] = V(0, a, -b + z);  q[3] = V(0, -a, -b + z);
+        q[4] = V(a, b, z);       q[5] = V(-a, b, z);
+        q[6] = V(a, -b, z);      q[7] = V(-a, -b, z);
+        q[8] = V(b, 0, a + z);   q[9] = V(b, 0, -a + z);
+        q[10] = V(-b, 0, a + z); q[11] = V(-b, 0, -a + z);
+      }
+      /* the capsule writes b - hl and -b - hl: x + (-hl) == x - hl exactly */
+      double c = 0.5 * r2, d = r;
+      for (int h = 0; h < 2; ++h) {
+        double z = h ? -hl : hl;
+        v3 *q = p + 24 + 6 * h;
+        q[0] = V(r2, 0, z);  q[1] = V(c, d, z);   q[2] = V(-c, d, z);
+        q[3] = V(-r2, 0, z); q[4] = V(-c, -d, z); q[5] = V(c, -d, z);
+      }
+      return 36;
+    }
+    case FCLO_SHAPE_CYLINDER: {
+      double r = s->dims[0], hl = s->dims[1] * 0.5;
+      double r2 = r * 2 / sqrt(3.0), a = 0.5 * r2, b = r;
+      for (int h = 0; h < 2; ++h) {
+        double z = h ? hl : -hl;
+        v3 *q = p + 6 * h;
+        q[0] = V(r2, 0, z);  q[1] = V(a, b, z);   q[2] = V(-a, b, z);
+        q[3] = V(-r2, 0, z); q[4] = V(-a, -b, z); q[5] = V(a, -b, z);
+      }
+      return 12;
+    }
+  }
+  return 0;
+}
+
+/* The OBB part of computeBV<OBBRSS>(shape, tf) (utility-inl.h:124-133: the
+ * generic path, there being no OBBRSS specialisation for these shapes):
+ * OBB fitn (BV_fitter / bv/utility-inl.h:131-145) over the posed bound
+ * vertices -- point covariance (geometry-inl.h:1335-1421, point branch),
+ * eigen_old, axisFromEigen, getExtentAndCenter_pointcloud
+ * (geometry-inl.h:228-287).  Only the OBB enters the BV test. */
+void fclo_shape_obb(const fclo_shape *s, const double tf[12], double axis[9], double to[3],
+                    double extent[3]) {
+  v3 local[36];
+  xf T = xf_from(tf);
+  int n = s->type == FCLO_SHAPE_CONVEX ? s->num_vertices : bound_points(s, local);
+  double S1[3] = {0, 0, 0};
+  double s00 = 0, s11 = 0, s22 = 0, s01 = 0, s02 = 0, s12 = 0;
+  for (int i = 0; i < n; ++i) {
+    v3 q = s->type == FCLO_SHAPE_CONVEX
+               ? V(s->vertices[3 * i], s->vertices[3 * i + 1], s->vertices[3 * i + 2])
+               : local[i];
+    v3 p = xf_apply(&T, q);
+    for (int k = 0; k < 3; ++k) S1[k] += p.x[k];
+    s00 += p.x[0] * p.x[0];
+    s11 += p.x[1] * p.x[1];
+    s22 += p.x[2] * p.x[2];
+    s01 += p.x[0] * p.x[1];
+    s02 += p.x[0] * p.x[2];
+    s12 += p.x[1] * p.x[2];
+  }
+  double M[9], E[9], ev[3], np = (double)n;
+  M[0] = s00 - S1[0] * S1[0] / np;
+  M[4] = s11 - S1[1] * S1[1] / np;
+  M[8] = s22 - S1[2] * S1[2] / np;
+  M[1] = s01 - S1[0] * S1[1] / np;
+  M[5] = s12 - S1[1] * S1[2] / np;
+  M[2] = s02 - S1[0] * S1[2] / np;
+  M[3] = M[1];
+  M[6] = M[2];
+  M[7] = M[5];
+  fclo_eigen_old(M, ev, E);
+  fclo_axis_from_eigen(E, ev, axis);
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = 0; i < n; ++i) {
+    v3 q = s->type == FCLO_SHAPE_CONVEX
+               ? V(s->vertices[3 * i], s->vertices[3 * i + 1], s->vertices[3 * i + 2])
+               : local[i];
+    v3 p = xf_apply(&T, q);
+    for (int k = 0; k < 3; ++k) {
+      double pr = vdot(col(axis, k), p);
+      if (pr > mx[k]) mx[k] = pr;
+      if (pr < mn[k]) mn[k] = pr;
+    }
+  }
+  v3 o = V((mx[0] + mn[0]) / 2, (mx[1] + mn[1]) / 2, (mx[2] + mn[2]) / 2);
+  v3 c = rot(axis, o);
+  for (int k = 0; k < 3; ++k) {
+    to[k] = c.x[k];
+    extent[k] = (mx[k] - mn[k]) * 0.5;
+  }
+}
+
+/* segmentSqrDistance (sphere_triangle-inl.h:84-108) */
+static double segment_sqr_distance(v3 from, v3 to, v3 p, v3 *nearest) {
+  v3 diff = vsub(p, from);
+  v3 v = vsub(to, from);
+  double t = vdot(v, diff);
+  if (t > 0) {
+    double dotVV = vdot(v, v);
+    if (t < dotVV) {
+      t /= dotVV;
+      diff = vsub(diff, vmul(v, t));
+    } else {
+      t = 1;
+      diff = vsub(diff, v);
+    }
+  } else {
+    t = 0;
+  }
+  *nearest = vadd(from, vmul(v, t));
+  return vdot(diff, diff);
+}
+
+/* projectInTriangle (sphere_triangle-inl.h:112-135) */
+static int project_in_triangle(v3 p1, v3 p2, v3 p3, v3 normal, v3 p) {
+  double r1 = vdot(vcross(vsub(p2, p1), normal), vsub(p, p1));
+  double r2 = vdot(vcross(vsub(p3, p2), normal), vsub(p, p2));
+  double r3 = vdot(vcross(vsub(p1, p3), normal), vsub(p, p3));
+  return (r1 > 0 && r2 > 0 && r3 > 0) || (r1 <= 0 && r2 <= 0 && r3 <= 0);
+}
+
+/* sphereTriangleIntersect (sphere_triangle-inl.h:138-230); the outputs are
+ * the solver's (normal from the sphere's centre towards the contact). */
+static int sphere_triangle(double radius, v3 center, v3 P1, v3 P2, v3 P3, v3 *point,
+                           double *depth, v3 *normal_out) {
+  v3 normal = vnormalized(vcross(vsub(P2, P1), vsub(P3, P1)));
+  double rt = radius + DBL_EPSILON;
+  double dist = vdot(vsub(center, P1), normal);
+  if (dist < 0) {
+    dist *= -1;
+    normal = vmul(normal, -1);
+  }
+  if (!(dist < rt)) return 0;
+  int has = 0;
+  v3 cp = V(NAN, NAN, NAN);
+  if (project_in_triangle(P1, P2, P3, normal, center)) {
+    has = 1;
+    cp = vsub(center, vmul(normal, dist));
+  } else {
+    double r2 = rt * rt;
+    v3 ne;
+    if (segment_sqr_distance(P1, P2, center, &ne) < r2) { has = 1; cp = ne; }
+    if (segment_sqr_distance(P2, P3, center, &ne) < r2) { has = 1; cp = ne; }
+    if (segment_sqr_distance(P3, P1, center, &ne) < r2) { has = 1; cp = ne; }
+  }
+  if (!has) return 0;
+  v3 cc = vsub(cp, center);
+  double dsq = vsq(cc);
+  if (!(dsq < rt * rt)) return 0;
+  if (dsq > 0) {
+    *normal_out = vnormalized(cc);
+    *depth = -(radius - sqrt(dsq));
+  } else {
+    *normal_out = vneg(normal);
+    *depth = -radius;
+  }
+  *point = cp;
+  return 1;
+}
+
+int fclo_sphere_triangle(double radius, const double center[3], const double P1[3],
+                         const double P2[3], const double P3[3], double point[3],
+                         double *depth, double normal[3]) {
+  v3 p, n;
+  double d = 0;
+  int hit = sphere_triangle(radius, V(center[0], center[1], center[2]), V(P1[0], P1[1], P1[2]),
+                            V(P2[0], P2[1], P2[2]), V(P3[0], P3[1], P3[2]), &p, &d, &n);
+  if (hit) {
+    for (int k = 0; k < 3; ++k) {
+      point[k] = p.x[k];
+      normal[k] = n.x[k];
+    }
+    *depth = d;
+  }
+  return hit;
+}
+
+typedef struct {
+  const fclo_model *m;
+  const fclo_shape *s;
+  const double *tf1, *tf2;
+  xf T1, T2;
+  fclo_node sbv;  /* the shape's OBB (world frame) */
+  int cap, enable_contact, count;
+  double tol;
+  int *prims;
+  fclo_contact *out;
+  fclo_collide_stats *st;
+} ms_state;
+
+/* meshShapeCollisionOrientedNodeLeafTesting
+ * (mesh_shape_collision_traversal_node-inl.h:205-283, enable_cost off) */
+static void ms_leaf(ms_state *S, int prim) {
+  S->st->num_leaf_tests++;
+  const int *t = S->m->tris + 3 * prim;
+  double tv[9];
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j) tv[3 * k + j] = S->m->vertices[3 * t[k] + j];
+  fclo_contact c;
+  int hit;
+  if (S->s->type == FCLO_SHAPE_SPHERE) {
+    /* gjk_solver_indep-inl.h:527-545: the triangle posed by tf1 */
+    v3 P[3];
+    for (int k = 0; k < 3; ++k) P[k] = xf_apply(&S->T1, V(tv[3 * k], tv[3 * k + 1], tv[3 * k + 2]));
+    v3 p, n;
+    double d;
+    hit = sphere_triangle(S->s->dims[0], S->T2.t, P[0], P[1], P[2], &p, &d, &n);
+    if (hit) {
+      /* the leaf stores -normal and the solver's depth */
+      for (int k = 0; k < 3; ++k) {
+        c.pos[k] = p.x[k];
+        c.normal[k] = -n.x[k];
+      }
+      c.depth = d;
+    }
+  } else {
+    /* gjk_solver_indep-inl.h:448-505: GJK/EPA on (shape at tf2, TriangleP at
+     * tf1); the leaf's -normal undoes the solver's, leaving EPA's normal */
+    fclo_shape tri;
+    tri.type = FCLO_SHAPE_TRIANGLE;
+    tri.num_vertices = 3;
+    tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
+    tri.vertices = tv;
+    int n = 0;
+    hit = gjk_epa_intersect(S->s, &S->T2, &tri, &S->T1, S->tol, &c, &n);
+  }
+  if (!hit || S->count >= S->cap) return;
+  int i = S->count++;
+  if (S->prims) S->prims[i] = prim;
+  if (S->out) {
+    if (S->enable_contact)
+      S->out[i] = c;
+    else
+      memset(&S->out[i], 0, sizeof(S->out[i]));
+  }
+}
+
+/* collisionRecurse (traversal_recurse-inl.h:82-129) with the second node
+ * always a leaf and firstOverSecond always true: BV test, then the leaf
+ * test or both children, stopping once isSatisfied (count == cap). */
+static void ms_recurse(ms_state *S, int b) {
+  const fclo_node *nd = &S->m->bvs[b];
+  S->st->num_bv_tests++;
+  if (!fclo_obb_overlap(S->T1.R, S->T1.t.x, &S->sbv, nd)) return;
+  if (nd->first_child < 0) {
+    ms_leaf(S, -(nd->first_child + 1));
+    return;
+  }
+  ms_recurse(S, nd->first_child);
+  if (S->count >= S->cap) return;
+  ms_recurse(S, nd->first_child + 1);
+}
+
+int fclo_mesh_shape_collide(const fclo_model *m, const double tf1[12], const fclo_shape *s,
+                            const double tf2[12], int cap, int enable_contact, double tol,
+                            int *prims, fclo_contact *out, fclo_collide_stats *st) {
+  fclo_collide_stats dummy;
+  ms_state S;
+  memset(&S, 0, sizeof(S));
+  S.m = m;
+  S.s = s;
+  S.T1 = xf_from(tf1);
+  S.T2 = xf_from(tf2);
+  S.cap = cap;
+  S.enable_contact = enable_contact;
+  S.tol = tol;
+  S.prims = prims;
+  S.out = out;
+  S.st = st ? st : &dummy;
+  S.st->num_contacts = 0;
+  S.st->num_bv_tests = 0;
+  S.st->num_leaf_tests = 0;
+  if (cap <= 0 || m->num_bvs <= 0) return 0;
+  fclo_shape_obb(s, tf2, S.sbv.obb_axis, S.sbv.obb_to, S.sbv.obb_extent);
+  ms_recurse(&S, 0);
+  S.st->num_contacts = S.count;
+  return S.count;
+}
+
+void fclo_mesh_shape_collide_batch(const fclo_model *const *models, const int *mesh_ids,
+                                   const fclo_shape_rec *recs, const double *pool,
+                                   int num_queries, const int *shape_ids, const double *tf1s,
+                                   const double *tf2s, int cap, int enable_contact, double tol,
+                                   int *counts, int *prims, fclo_contact *out, int64_t *stats,
+                                   int threads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  for (int q = 0; q < num_queries; ++q) {
+    const fclo_shape_rec *r = recs + shape_ids[q];
+    fclo_shape s;
+    s.type = r->type;
+    s.num_vertices = r->num_vertices;
+    for (int i = 0; i < 3; ++i) s.dims[i] = r->dims[i];
+    s.vertices = r->type == FCLO_SHAPE_CONVEX ? pool + 3 * r->vertex_offset : 0;
+    fclo_collide_stats st;
+    counts[q] = fclo_mesh_shape_collide(
+        models[mesh_ids[q]], tf1s + 12 * (size_t)q, &s, tf2s + 12 * (size_t)q, cap,
+        enable_contact, tol, prims ? prims + (size_t)q * cap : 0,
+        out ? out + (size_t)q * cap : 0, &st);
+    if (stats) {
+      stats[2 * q] = st.num_bv_tests;
+      stats[2 * q + 1] = st.num_leaf_tests;
+    }
   }
 }
