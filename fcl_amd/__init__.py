@@ -309,6 +309,37 @@ class Context:
         a.passes = passes.data_ptr() if passes is not None else None
         self._check(self._L.fclx_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
+    def mesh_shape_collide_batch_async(self, query_meshes, shapes, vertices, query_shapes, tf1,
+                                       tf2, num_max_contacts, counts, primitives=None,
+                                       contact=None, tol=1e-6, stats=None, stream=None):
+        """fclx_mesh_shape_collide_batch: collide(mesh, tf1, shape, tf2) per
+        query for BVHModel<OBBRSS> meshes and primitive shapes.
+        query_meshes, query_shapes: (n,) int32 device tensors; shapes: device
+        uint8 tensor of fclx_shape records; vertices: device float64 (V,3) or
+        None; primitives: None or an (n*cap,) int32 device tensor for the
+        contacts' triangles; contact: None, or (pos, normal, depth) device
+        tensors for enable_contact; stats: None or (n,2) int64 device tensor
+        (BV tests, leaf tests)."""
+        a = N.MeshShapeCollideBatchArgs()
+        a.num_queries = int(query_meshes.shape[0])
+        a.query_meshes = query_meshes.data_ptr()
+        a.query_shapes = query_shapes.data_ptr()
+        a.num_shapes = int(shapes.numel() // SHAPE_DTYPE.itemsize)
+        a.shapes = shapes.data_ptr()
+        a.vertices = vertices.data_ptr() if vertices is not None and vertices.numel() else None
+        a.num_vertices = int(vertices.shape[0]) if a.vertices else 0
+        a.tf1 = tf1.data_ptr()
+        a.tf2 = tf2.data_ptr()
+        a.num_max_contacts = int(num_max_contacts)
+        a.gjk_tolerance = float(tol)
+        a.counts = counts.data_ptr()
+        a.primitives = primitives.data_ptr() if primitives is not None else None
+        if contact is not None:
+            a.enable_contact = 1
+            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
+        a.stats = stats.data_ptr() if stats is not None else None
+        self._check(self._L.fclx_mesh_shape_collide_batch(self._h, ctypes.byref(a), stream))
+
     def shape_collide(self, s1, v1, tf1, s2, v2, tf2, cap=1, contact=True, tol=1e-6):
         """fclx_shape_collide: one primitive pair with host arrays.  s1/s2 are
         shape_table records (one row each), v1/v2 a convex's (m,3) vertices

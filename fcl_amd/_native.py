@@ -78,6 +78,16 @@ class ShapeCollideBatchArgs(ctypes.Structure):
                 ("passes", c_vp)]
 
 
+class MeshShapeCollideBatchArgs(ctypes.Structure):
+    _fields_ = [("num_queries", c_i32), ("query_meshes", c_vp), ("query_shapes", c_vp),
+                ("num_shapes", c_i32), ("shapes", c_vp), ("vertices", c_vp),
+                ("num_vertices", c_i64), ("tf1", c_vp), ("tf2", c_vp),
+                ("num_max_contacts", c_i32), ("enable_contact", c_i32),
+                ("gjk_tolerance", c_dbl), ("counts", c_vp), ("primitives", c_vp),
+                ("contact_pos", c_vp), ("contact_normal", c_vp), ("contact_depth", c_vp),
+                ("stats", c_vp)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("scene_mesh", c_i32), ("num_streams", c_i32), ("scene_pose", c_dbl * 12),
                 ("hbm_budget", c_i64), ("max_batch_queries", c_i32), ("engine", c_i32)]
@@ -125,6 +135,8 @@ SIGNATURES = {
     "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
     "fclx_sync": (c_i32, [c_vp, c_vp]),
     "fclx_shape_collide_batch": (c_i32, [c_vp, ctypes.POINTER(ShapeCollideBatchArgs), c_vp]),
+    "fclx_mesh_shape_collide_batch": (c_i32, [c_vp, ctypes.POINTER(MeshShapeCollideBatchArgs),
+                                              c_vp]),
     "fclx_shape_collide": (c_i32, [c_vp, ctypes.POINTER(ShapeRec), P_dbl, P_dbl,
                                    ctypes.POINTER(ShapeRec), P_dbl, P_dbl, c_i32, c_i32, c_dbl,
                                    P_dbl, P_dbl, P_dbl, P_i32, P_i32]),

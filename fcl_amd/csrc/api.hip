@@ -546,6 +546,47 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
   return FCLX_OK;
 }
 
+int fclx_mesh_shape_collide_batch(fclx_context* ctx, const fclx_mesh_shape_collide_batch_args* a,
+                                  void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->num_queries < 0 || a->num_shapes < 0 || a->num_vertices < 0 ||
+      a->num_max_contacts < 1 || !(a->gjk_tolerance > 0) ||
+      (a->num_queries > 0 && (!a->query_meshes || !a->query_shapes || !a->shapes || !a->tf1 ||
+                              !a->tf2 || !a->counts)) ||
+      (a->num_queries > 0 && a->enable_contact &&
+       (!a->contact_pos || !a->contact_normal || !a->contact_depth)))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid mesh-shape batch arguments");
+  MeshShapeLaunch L{};
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    L.meshes = ctx->d_table;
+    L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+  }
+  (void)hipSetDevice(ctx->device);
+  L.num_queries = a->num_queries;
+  L.query_meshes = a->query_meshes;
+  L.query_shapes = a->query_shapes;
+  L.num_shapes = a->num_shapes;
+  L.shapes = a->shapes;
+  L.vertices = a->vertices;
+  L.num_vertices = a->vertices ? a->num_vertices : 0;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.cap = a->num_max_contacts;
+  L.contact = a->enable_contact ? 1 : 0;
+  L.tol = a->gjk_tolerance;
+  L.counts = a->counts;
+  L.prims = a->primitives;
+  L.contact_pos = a->contact_pos;
+  L.contact_normal = a->contact_normal;
+  L.contact_depth = a->contact_depth;
+  L.stats = reinterpret_cast<long long*>(a->stats);
+  L.status = ctx->d_status;
+  hipError_t e = launch_mesh_shape_collide(L, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(ctx, e, "mesh-shape collide launch");
+  return FCLX_OK;
+}
+
 int fclx_shape_collide_batch(fclx_context* ctx, const fclx_shape_collide_batch_args* a,
                              void* stream) {
   if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;

@@ -94,7 +94,7 @@ __device__ __forceinline__ Xf inv_compose(const Xf& a, const Xf& b) {
   return r;
 }
 
-enum : int { kBox = 0, kSphere = 1, kCapsule = 2, kCylinder = 3, kConvex = 4 };
+enum : int { kBox = 0, kSphere = 1, kCapsule = 2, kCylinder = 3, kConvex = 4, kTriangle = 5 };
 
 // One primitive: type, dims (box sides / radius / radius+lz) and, for a
 // convex, its vertices (local frame).
@@ -131,6 +131,16 @@ __device__ inline D3 support(const Shape& s, const D3& d) {
       if (zdist == 0.0) return mk(0, 0, d.v[2] > 0 ? half_h : -half_h);
       const double k = s.dims[0] / zdist;
       return mk(k * d.v[0], k * d.v[1], d.v[2] > 0 ? half_h : -half_h);
+    }
+    case kTriangle: {
+      // TriangleP (minkowski_diff-inl.h:79-100): verts = a, b, c; the
+      // first of the larger dot products, c only when strictly above
+      const D3 a = mk(s.verts[0], s.verts[1], s.verts[2]);
+      const D3 b = mk(s.verts[3], s.verts[4], s.verts[5]);
+      const D3 c = mk(s.verts[6], s.verts[7], s.verts[8]);
+      const double da = dot(d, a), db = dot(d, b), dc = dot(d, c);
+      if (da > db) return dc > da ? c : a;
+      return dc > db ? c : b;
     }
     default: {  // convex
       double maxdot = -1.7976931348623157e308;
@@ -553,6 +563,7 @@ struct Epa {
   int pass;
   int it;          // loop passes done
   bool fallback;   // no tetrahedron: the FallBack result
+  int fb_vertex;   // the simplex's first vertex then (1 once the winding swap moved it)
 };
 
 __device__ __forceinline__ void hull_add(Epa& E, int f) {
@@ -747,6 +758,7 @@ __device__ inline bool epa_start(Epa& E, Gjk& g, const MDiff& m, double tol) {
   E.status = kEpaFailed;
   E.nextsv = 0;
   E.fallback = true;
+  E.fb_vertex = 0;
   const bool enclosed = g.rank > 1 && enclose_origin(g, m);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -760,6 +772,9 @@ __device__ inline bool epa_start(Epa& E, Gjk& g, const MDiff& m, double tol) {
     if (dot(sub(g.w[0], g.w[3]), cross(sub(g.w[1], g.w[3]), sub(g.w[2], g.w[3]))) < 0) {
       c[0] = 1;
       c[1] = 0;
+      // the reference swaps the simplex's first two vertices in place, so a
+      // FallBack after a failed tetrahedron reports the swapped first one
+      E.fb_vertex = 1;
     }
     int tet[4];
     tet[0] = new_face(E, c[0], c[1], c[2], true, tol);
@@ -838,6 +853,7 @@ __device__ inline Contact epa_finish(const Epa& E, const MDiff& m, const Xf& tf1
     normal = mk(1, 0, 0);
     depth = 0;
     rank = 1;
+    rc[0] = E.fb_vertex;
   } else {
     const EpaFace& outer = M.fc[E.best];
     const D3 projection = mul(outer.n, outer.d);

@@ -79,6 +79,33 @@ struct ShapeLaunch {
   int32_t* status;
 };
 
+// Mesh (BVHModel<OBBRSS>) against a primitive shape, one query per lane.
+struct MeshShapeLaunch {
+  const MeshDev* meshes;        // device table indexed by mesh id
+  int32_t num_meshes;
+  int32_t num_queries;
+  const int32_t* query_meshes;  // per query: the mesh
+  const int32_t* query_shapes;  // per query: the shape record
+  int32_t num_shapes;
+  const fclx_shape* shapes;     // device, num_shapes records
+  const double* vertices;       // device, convex vertex pool (3 per vertex)
+  long long num_vertices;
+  const double* tf1;            // 12 per query: the mesh's pose
+  const double* tf2;            // 12 per query: the shape's pose
+  int32_t cap;                  // num_max_contacts
+  int32_t contact;              // enable_contact
+  double tol;                   // GJK and EPA tolerance
+  int32_t* counts;              // per query
+  int32_t* prims;               // optional, cap per query: the contacts' triangle (b1)
+  double* contact_pos;          // 3 per slot (slot = query * cap + i), with contact
+  double* contact_normal;       // 3 per slot
+  double* contact_depth;        // 1 per slot
+  long long* stats;             // optional, 2 per query: BV tests, leaf tests
+  int32_t* status;
+};
+hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, hipStream_t s);
+size_t mesh_shape_epa_lane_bytes();
+
 hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);
 // Sort buffers, the EPA queue and per-lane EPA memory of the shape path; one
 // per context, used by one batch at a time (the caller serialises calls;

@@ -1,0 +1,578 @@
+// Batched fcl::collide between a BVHModel<OBBRSS> and a primitive shape (Box,
+// Sphere, Capsule, Cylinder, Convex) with FCL's GST_INDEP solver, in FP64:
+//   orientedBVHShapeCollide (collision_func_matrix-inl.h:375-419, 500-521);
+//   the shape's bounding box: computeBV<OBBRSS> by the generic fit over the
+//   shape's bound vertices (geometry/shape/utility-inl.h:124-133, OBB fitn
+//   bv/utility-inl.h:131-145), of which the BV test uses the OBB;
+//   collisionRecurse (traversal_recurse-inl.h:82-129) over the mesh alone --
+//   the shape is one leaf -- with BVTesting = OBB overlap of the shape's box
+//   and the node's box posed by the mesh's transform
+//   (mesh_shape_collision_traversal_node.h, OBBRSS node);
+//   leaf test meshShapeCollisionOrientedNodeLeafTesting
+//   (mesh_shape_collision_traversal_node-inl.h:205-283): sphereTriangleIntersect
+//   (sphere_triangle-inl.h:138-230) for a sphere, GJK + EPA on the shape and
+//   the posed TriangleP for the others (gjk_solver_indep-inl.h:448-505).
+//
+// One query per lane.  A lane fits the shape's box (Jacobi eigenvectors of
+// the bound vertices' covariance), descends the mesh depth first with the
+// reference's child order and stops at num_max_contacts.  Without
+// enable_contact a GJK "inside" is the answer (EPA's status is never Failed
+// after it); with it the lane runs EPA in a block of device memory of its
+// own, so that grid is persistent and sized to the EPA memory it holds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device/exact64.h"
+#include "device/shape64.h"
+#include "kernels.h"
+
+namespace fclx {
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kMsStack = 64;  // pending right children: the mesh depth bound
+constexpr int kEpaBlocksPerCu = 2;
+
+using shp::D3;
+using shp::mk;
+
+// eigen_old (math/geometry-inl.h:475-558): cyclic Jacobi, at most 50 sweeps;
+// vout(r, k) = v[k][r] as the reference lays it out.  All index loops are
+// unrolled so the matrices stay in registers.
+__device__ void eigen_old(const double m[9], double dout[3], double vout[9]) {
+  double R[3][3];
+  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double b[3], z[3], d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) R[i][j] = m[3 * i + j];
+#pragma unroll
+  for (int ip = 0; ip < 3; ++ip) {
+    b[ip] = d[ip] = R[ip][ip];
+    z[ip] = 0;
+  }
+  for (int it = 0; it < 50; ++it) {
+    double sm = 0;
+#pragma unroll
+    for (int ip = 0; ip < 3; ++ip)
+#pragma unroll
+      for (int iq = ip + 1; iq < 3; ++iq) sm += fabs(R[ip][iq]);
+    if (sm == 0.0) break;
+    const double tresh = it < 3 ? 0.2 * sm / (3 * 3) : 0.0;
+#pragma unroll
+    for (int ip = 0; ip < 3; ++ip) {
+#pragma unroll
+      for (int iq = ip + 1; iq < 3; ++iq) {
+        double g = 100.0 * fabs(R[ip][iq]);
+        if (it > 3 && fabs(d[ip]) + g == fabs(d[ip]) && fabs(d[iq]) + g == fabs(d[iq])) {
+          R[ip][iq] = 0.0;
+        } else if (fabs(R[ip][iq]) > tresh) {
+          double h = d[iq] - d[ip];
+          double t;
+          if (fabs(h) + g == fabs(h)) {
+            t = (R[ip][iq]) / h;
+          } else {
+            const double theta = 0.5 * h / (R[ip][iq]);
+            t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+            if (theta < 0.0) t = -t;
+          }
+          const double c = 1.0 / sqrt(1 + t * t);
+          const double s = t * c;
+          const double tau = s / (1.0 + c);
+          h = t * R[ip][iq];
+          z[ip] -= h;
+          z[iq] += h;
+          d[ip] -= h;
+          d[iq] += h;
+          R[ip][iq] = 0.0;
+#pragma unroll
+          for (int j = 0; j < ip; ++j) {
+            g = R[j][ip];
+            h = R[j][iq];
+            R[j][ip] = g - s * (h + g * tau);
+            R[j][iq] = h + s * (g - h * tau);
+          }
+#pragma unroll
+          for (int j = ip + 1; j < iq; ++j) {
+            g = R[ip][j];
+            h = R[j][iq];
+            R[ip][j] = g - s * (h + g * tau);
+            R[j][iq] = h + s * (g - h * tau);
+          }
+#pragma unroll
+          for (int j = iq + 1; j < 3; ++j) {
+            g = R[ip][j];
+            h = R[iq][j];
+            R[ip][j] = g - s * (h + g * tau);
+            R[iq][j] = h + s * (g - h * tau);
+          }
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            g = v[j][ip];
+            h = v[j][iq];
+            v[j][ip] = g - s * (h + g * tau);
+            v[j][iq] = h + s * (g - h * tau);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int ip = 0; ip < 3; ++ip) {
+      b[ip] += z[ip];
+      d[ip] = b[ip];
+      z[ip] = 0.0;
+    }
+  }
+  // converged, or 50 sweeps done (the accumulator as it stands)
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vout[3 * r + k] = v[k][r];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dout[k] = d[k];
+}
+
+__device__ __forceinline__ double pick3(const double* a, int i) {
+  return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
+// axisFromEigen (math/geometry-inl.h:561-598): columns = the eigenvectors of
+// the largest and middle eigenvalues and their cross product.
+__device__ void axis_from_eigen(const double E[9], const double s[3], double axis[9]) {
+  int mn, mid, mx;
+  if (s[0] > s[1]) {
+    mx = 0;
+    mn = 1;
+  } else {
+    mn = 0;
+    mx = 1;
+  }
+  if (s[2] < pick3(s, mn)) {
+    mid = mn;
+    mn = 2;
+  } else if (s[2] > pick3(s, mx)) {
+    mid = mx;
+    mx = 2;
+  } else {
+    mid = 2;
+  }
+  D3 c0, c1;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    c0.v[r] = mx == 0 ? E[r] : (mx == 1 ? E[3 + r] : E[6 + r]);
+    c1.v[r] = mid == 0 ? E[r] : (mid == 1 ? E[3 + r] : E[6 + r]);
+  }
+  const D3 c2 = shp::cross(c0, c1);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    axis[3 * r + 0] = c0.v[r];
+    axis[3 * r + 1] = c1.v[r];
+    axis[3 * r + 2] = c2.v[r];
+  }
+}
+
+// getBoundVertices in the shape's frame (box-inl.h:122-138,
+// sphere-inl.h:96-120, capsule-inl.h:108-162, cylinder-inl.h:102-126); the
+// same points in the same order.
+__device__ int bound_points(const shp::Shape& s, D3* p) {
+  switch (s.type) {
+    case shp::kBox: {
+      const double a = s.dims[0] / 2, b = s.dims[1] / 2, c = s.dims[2] / 2;
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int l = 0; l < 2; ++l) p[k++] = mk(i ? -a : a, j ? -b : b, l ? -c : c);
+      return 8;
+    }
+    case shp::kSphere: {
+      const double m = (1 + sqrt(5.0)) / 2.0;
+      const double e = s.dims[0] * 6 / (sqrt(27.0) + sqrt(15.0));
+      const double a = e, b = m * e;
+      p[0] = mk(0, a, b);
+      p[1] = mk(0, -a, b);
+      p[2] = mk(0, a, -b);
+      p[3] = mk(0, -a, -b);
+      p[4] = mk(a, b, 0);
+      p[5] = mk(-a, b, 0);
+      p[6] = mk(a, -b, 0);
+      p[7] = mk(-a, -b, 0);
+      p[8] = mk(b, 0, a);
+      p[9] = mk(b, 0, -a);
+      p[10] = mk(-b, 0, a);
+      p[11] = mk(-b, 0, -a);
+      return 12;
+    }
+    case shp::kCapsule: {
+      const double r = s.dims[0], hl = s.dims[1] * 0.5;
+      const double m = (1 + sqrt(5.0)) / 2.0;
+      const double e = r * 6 / (sqrt(27.0) + sqrt(15.0));
+      const double a = e, b = m * e, r2 = r * 2 / sqrt(3.0);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // b - hl written as b + (-hl): the same double
+        const double z = h ? -hl : hl;
+        D3* q = p + 12 * h;
+        q[0] = mk(0, a, b + z);
+        q[1] = mk(0, -a, b + z);
+        q[2] = mk(0, a, -b + z);
+        q[3] = mk(0, -a, -b + z);
+        q[4] = mk(a, b, z);
+        q[5] = mk(-a, b, z);
+        q[6] = mk(a, -b, z);
+        q[7] = mk(-a, -b, z);
+        q[8] = mk(b, 0, a + z);
+        q[9] = mk(b, 0, -a + z);
+        q[10] = mk(-b, 0, a + z);
+        q[11] = mk(-b, 0, -a + z);
+      }
+      const double c = 0.5 * r2, d = r;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double z = h ? -hl : hl;
+        D3* q = p + 24 + 6 * h;
+        q[0] = mk(r2, 0, z);
+        q[1] = mk(c, d, z);
+        q[2] = mk(-c, d, z);
+        q[3] = mk(-r2, 0, z);
+        q[4] = mk(-c, -d, z);
+        q[5] = mk(c, -d, z);
+      }
+      return 36;
+    }
+    case shp::kCylinder: {
+      const double r = s.dims[0], hl = s.dims[1] * 0.5;
+      const double r2 = r * 2 / sqrt(3.0), a = 0.5 * r2, b = r;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double z = h ? hl : -hl;
+        D3* q = p + 6 * h;
+        q[0] = mk(r2, 0, z);
+        q[1] = mk(a, b, z);
+        q[2] = mk(-a, b, z);
+        q[3] = mk(-r2, 0, z);
+        q[4] = mk(-a, -b, z);
+        q[5] = mk(a, -b, z);
+      }
+      return 12;
+    }
+  }
+  return 0;
+}
+
+// The shape's box in the world frame: OBB fitn over the posed bound
+// vertices (a convex's own vertices): point covariance
+// (geometry-inl.h:1335-1421), eigen_old, axisFromEigen,
+// getExtentAndCenter_pointcloud (geometry-inl.h:228-287).
+__device__ void shape_obb(const shp::Shape& s, const shp::Xf& T, double axis[9], double to[3],
+                          double extent[3]) {
+  D3 local[36];
+  const bool convex = s.type == shp::kConvex;
+  const int n = convex ? s.nv : bound_points(s, local);
+  double S1[3] = {0, 0, 0};
+  double s00 = 0, s11 = 0, s22 = 0, s01 = 0, s02 = 0, s12 = 0;
+  for (int i = 0; i < n; ++i) {
+    const D3 q = convex ? mk(s.verts[3 * i], s.verts[3 * i + 1], s.verts[3 * i + 2]) : local[i];
+    const D3 p = shp::apply(T, q);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S1[k] += p.v[k];
+    s00 += p.v[0] * p.v[0];
+    s11 += p.v[1] * p.v[1];
+    s22 += p.v[2] * p.v[2];
+    s01 += p.v[0] * p.v[1];
+    s02 += p.v[0] * p.v[2];
+    s12 += p.v[1] * p.v[2];
+  }
+  const double np = static_cast<double>(n);
+  double M[9], E[9], ev[3];
+  M[0] = s00 - S1[0] * S1[0] / np;
+  M[4] = s11 - S1[1] * S1[1] / np;
+  M[8] = s22 - S1[2] * S1[2] / np;
+  M[1] = s01 - S1[0] * S1[1] / np;
+  M[5] = s12 - S1[1] * S1[2] / np;
+  M[2] = s02 - S1[0] * S1[2] / np;
+  M[3] = M[1];
+  M[6] = M[2];
+  M[7] = M[5];
+  eigen_old(M, ev, E);
+  axis_from_eigen(E, ev, axis);
+  double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+  double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+  for (int i = 0; i < n; ++i) {
+    const D3 q = convex ? mk(s.verts[3 * i], s.verts[3 * i + 1], s.verts[3 * i + 2]) : local[i];
+    const D3 p = shp::apply(T, q);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double pr = shp::dot(shp::col(axis, k), p);
+      if (pr > mx[k]) mx[k] = pr;
+      if (pr < mn[k]) mn[k] = pr;
+    }
+  }
+  const D3 o = mk((mx[0] + mn[0]) / 2, (mx[1] + mn[1]) / 2, (mx[2] + mn[2]) / 2);
+  const D3 c = shp::rot(axis, o);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    to[k] = c.v[k];
+    extent[k] = (mx[k] - mn[k]) * 0.5;
+  }
+}
+
+// segmentSqrDistance (sphere_triangle-inl.h:84-108)
+__device__ __forceinline__ double segment_sqr_distance(const D3& from, const D3& to, const D3& p,
+                                                       D3& nearest) {
+  D3 diff = shp::sub(p, from);
+  const D3 v = shp::sub(to, from);
+  double t = shp::dot(v, diff);
+  if (t > 0) {
+    const double dotVV = shp::dot(v, v);
+    if (t < dotVV) {
+      t /= dotVV;
+      diff = shp::sub(diff, shp::mul(v, t));
+    } else {
+      t = 1;
+      diff = shp::sub(diff, v);
+    }
+  } else {
+    t = 0;
+  }
+  nearest = shp::add(from, shp::mul(v, t));
+  return shp::dot(diff, diff);
+}
+
+// projectInTriangle (sphere_triangle-inl.h:112-135)
+__device__ __forceinline__ bool project_in_triangle(const D3& p1, const D3& p2, const D3& p3,
+                                                    const D3& n, const D3& p) {
+  const double r1 = shp::dot(shp::cross(shp::sub(p2, p1), n), shp::sub(p, p1));
+  const double r2 = shp::dot(shp::cross(shp::sub(p3, p2), n), shp::sub(p, p2));
+  const double r3 = shp::dot(shp::cross(shp::sub(p1, p3), n), shp::sub(p, p3));
+  return (r1 > 0 && r2 > 0 && r3 > 0) || (r1 <= 0 && r2 <= 0 && r3 <= 0);
+}
+
+// sphereTriangleIntersect (sphere_triangle-inl.h:138-230): the solver's
+// outputs (normal from the centre towards the contact, depth <= 0).
+__device__ bool sphere_triangle(double radius, const D3& center, const D3& P1, const D3& P2,
+                                const D3& P3, D3& point, double& depth, D3& normal_out) {
+  D3 normal = shp::normalized(shp::cross(shp::sub(P2, P1), shp::sub(P3, P1)));
+  const double rt = radius + 2.220446049250313e-16;
+  double dist = shp::dot(shp::sub(center, P1), normal);
+  if (dist < 0) {
+    dist *= -1;
+    normal = shp::mul(normal, -1.0);
+  }
+  if (!(dist < rt)) return false;
+  bool has = false;
+  D3 cp = mk(0, 0, 0);
+  if (project_in_triangle(P1, P2, P3, normal, center)) {
+    has = true;
+    cp = shp::sub(center, shp::mul(normal, dist));
+  } else {
+    const double r2 = rt * rt;
+    D3 ne;
+    if (segment_sqr_distance(P1, P2, center, ne) < r2) {
+      has = true;
+      cp = ne;
+    }
+    if (segment_sqr_distance(P2, P3, center, ne) < r2) {
+      has = true;
+      cp = ne;
+    }
+    if (segment_sqr_distance(P3, P1, center, ne) < r2) {
+      has = true;
+      cp = ne;
+    }
+  }
+  if (!has) return false;
+  const D3 cc = shp::sub(cp, center);
+  const double dsq = shp::sqn(cc);
+  if (!(dsq < rt * rt)) return false;
+  if (dsq > 0) {
+    normal_out = shp::normalized(cc);
+    depth = -(radius - sqrt(dsq));
+  } else {
+    normal_out = shp::neg(normal);
+    depth = -radius;
+  }
+  point = cp;
+  return true;
+}
+
+// A usable query: a mesh of OBBRSS nodes no deeper than the stack, and a
+// shape of a known type (a convex with at least 4 vertices inside the pool).
+__device__ __forceinline__ bool load_query(const MeshShapeLaunch& L, int q, shp::Shape& s) {
+  const int mi = L.query_meshes[q], si = L.query_shapes[q];
+  if (mi < 0 || mi >= L.num_meshes || si < 0 || si >= L.num_shapes) return false;
+  const MeshDev& M = L.meshes[mi];
+  if (M.num_bvs <= 0 || M.bv_type != 0 || M.depth > kMsStack) return false;
+  const fclx_shape& r = L.shapes[si];
+  s.type = r.type;
+  s.nv = r.num_vertices;
+  s.dims[0] = r.dims[0];
+  s.dims[1] = r.dims[1];
+  s.dims[2] = r.dims[2];
+  s.verts = nullptr;
+  switch (r.type) {
+    case shp::kBox:
+    case shp::kSphere:
+    case shp::kCapsule:
+    case shp::kCylinder:
+      return true;
+    case shp::kConvex:
+      if (r.num_vertices < 4 || r.vertex_offset < 0 ||
+          r.vertex_offset + r.num_vertices > L.num_vertices || L.vertices == nullptr)
+        return false;
+      s.verts = L.vertices + 3 * r.vertex_offset;
+      return true;
+  }
+  return false;
+}
+
+template <bool kContact>
+__global__ void __launch_bounds__(kBlk) meshshape_collide(MeshShapeLaunch L, shp::EpaLane* lanes,
+                                                          shp::EpaWave* waves) {
+  const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kBlk;
+  shp::Epa E;
+  if constexpr (kContact) {
+    E.M = lanes + gid;
+    E.S = waves + gid / 64;
+    E.lane = threadIdx.x & 63;
+  }
+  for (size_t qq = gid; qq < static_cast<size_t>(L.num_queries); qq += stride) {
+    const int q = static_cast<int>(qq);
+    shp::Shape s;
+    if (!load_query(L, q, s)) {
+      L.counts[q] = 0;
+      if (L.stats) {
+        L.stats[2 * qq] = 0;
+        L.stats[2 * qq + 1] = 0;
+      }
+      atomicOr(L.status, kStatusBadQuery);
+      continue;
+    }
+    const MeshDev& M = L.meshes[L.query_meshes[q]];
+    const shp::Xf T1 = shp::xf_load(L.tf1 + 12 * qq);
+    const shp::Xf T2 = shp::xf_load(L.tf2 + 12 * qq);
+    double sa[9], sto[3], sext[3];
+    shape_obb(s, T2, sa, sto, sext);
+    M3 R0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R0.m[i] = T1.R[i];
+    const V3 T0 = v3(T1.t.v[0], T1.t.v[1], T1.t.v[2]);
+    int stk[kMsStack];
+    int sp = 0, b = 0, count = 0;
+    long long nbv = 0, nleaf = 0;
+    while (true) {
+      const ObbNode64& nd = M.obb64[b];
+      ++nbv;
+      double B[9], Tr[3];
+      box_relative(R0, T0, sa, sto, nd.axis, nd.to, B, Tr);
+      if (!obb_disjoint64(B, Tr, sext, nd.extent)) {
+        const int fc = nd.first_child;
+        if (fc >= 0) {
+          stk[sp++] = fc + 1;
+          b = fc;
+          continue;
+        }
+        const int prim = -(fc + 1);
+        ++nleaf;
+        const double* tv = M.tri_verts + static_cast<size_t>(prim) * kTriVertsStride;
+        bool hit;
+        shp::Contact c;
+        if (s.type == shp::kSphere) {
+          // gjk_solver_indep-inl.h:527-545: the triangle posed by the mesh's tf
+          D3 P[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            P[k] = shp::apply(T1, mk(tv[3 * k], tv[3 * k + 1], tv[3 * k + 2]));
+          D3 p, n;
+          double d;
+          hit = sphere_triangle(s.dims[0], T2.t, P[0], P[1], P[2], p, d, n);
+          if (hit) {
+            c.pos = p;
+            c.normal = shp::neg(n);  // the leaf stores minus the solver's normal
+            c.depth = d;
+          }
+        } else {
+          shp::Shape tri;
+          tri.type = shp::kTriangle;
+          tri.nv = 3;
+          tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
+          tri.verts = tv;
+          shp::MDiff m;
+          shp::make_mdiff(s, T2, tri, T1, m);
+          shp::Gjk g;
+          hit = shp::gjk_evaluate(g, m, L.tol) == shp::kGjkInside;
+          if constexpr (kContact) {
+            if (hit) {
+              // the leaf's -normal undoes the solver's: EPA's normal stays
+              bool more = shp::epa_start(E, g, m, L.tol);
+              while (more) more = shp::epa_step(E, m, L.tol);
+              c = shp::epa_finish(E, m, T2);
+            }
+          }
+        }
+        if (hit) {
+          const size_t k = qq * L.cap + count;
+          if (L.prims) L.prims[k] = prim;
+          if constexpr (kContact) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              L.contact_pos[3 * k + j] = c.pos.v[j];
+              L.contact_normal[3 * k + j] = c.normal.v[j];
+            }
+            L.contact_depth[k] = c.depth;
+          }
+          if (++count >= L.cap) break;
+        }
+      }
+      if (sp == 0) break;
+      b = stk[--sp];
+    }
+    L.counts[q] = count;
+    if (L.stats) {
+      L.stats[2 * qq] = nbv;
+      L.stats[2 * qq + 1] = nleaf;
+    }
+  }
+}
+
+}  // namespace
+
+size_t mesh_shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::EpaWave) / 64; }
+
+hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, hipStream_t s) {
+  if (L.num_queries <= 0) return hipSuccess;
+  const long long blocks = (static_cast<long long>(L.num_queries) + kBlk - 1) / kBlk;
+  if (!L.contact) {
+    meshshape_collide<false><<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, nullptr, nullptr);
+    return hipGetLastError();
+  }
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const long long grid = std::min<long long>(blocks, static_cast<long long>(cus) * kEpaBlocksPerCu);
+  const size_t nl = static_cast<size_t>(grid) * kBlk;
+  shp::EpaLane* lanes = nullptr;
+  shp::EpaWave* waves = nullptr;
+  e = hipMallocAsync(reinterpret_cast<void**>(&lanes), nl * sizeof(shp::EpaLane), s);
+  if (e != hipSuccess) return e;
+  e = hipMallocAsync(reinterpret_cast<void**>(&waves), nl / 64 * sizeof(shp::EpaWave), s);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(lanes, s);
+    return e;
+  }
+  meshshape_collide<true><<<static_cast<unsigned>(grid), kBlk, 0, s>>>(L, lanes, waves);
+  e = hipGetLastError();
+  (void)hipFreeAsync(lanes, s);
+  (void)hipFreeAsync(waves, s);
+  return e;
+}
+
+}  // namespace fclx

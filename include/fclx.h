@@ -213,6 +213,48 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* args,
                        void* stream);
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
                         void* stream);
+/* Batched fcl::collide(mesh, tf1, shape, tf2) for a BVHModel<OBBRSS> and a
+ * primitive (BOX, SPHERE, CAPSULE, CYLINDER, or CONVEX with >= 4 vertices)
+ * with the GST_INDEP solver, enable_cost off: per query what
+ * BVHShapeCollider<OBBRSS> (orientedBVHShapeCollide,
+ * collision_func_matrix-inl.h:375-419, 500-521) adds to an empty result --
+ * contacts (b1 = triangle id, b2 = Contact::NONE) in the reference's order,
+ * at most num_max_contacts, from meshShapeCollisionOrientedNodeLeafTesting
+ * (mesh_shape_collision_traversal_node-inl.h:205-283).  Asynchronous; a
+ * query naming a missing mesh or shape, an RSS mesh, a mesh deeper than 64
+ * levels or a convex with fewer than 4 vertices gets count 0 and makes
+ * fclx_sync return FCLX_ERR_INVALID_ARGUMENT.  With enable_contact the call
+ * takes stream-ordered device memory for EPA (about 12 KB per resident lane,
+ * freed when the batch ends). */
+typedef struct {
+  int32_t num_queries;
+  const int32_t* query_meshes; /* device: mesh id per query */
+  const int32_t* query_shapes; /* device: shape id per query (into shapes) */
+  int32_t num_shapes;
+  const fclx_shape* shapes;    /* device: num_shapes records */
+  const double* vertices;      /* device: convex vertex pool (NULL if none) */
+  int64_t num_vertices;
+  const double* tf1;           /* device: the mesh's pose, 12 per query */
+  const double* tf2;           /* device: the shape's pose, 12 per query */
+  int32_t num_max_contacts;    /* >= 1 */
+  int32_t enable_contact;
+  double gjk_tolerance;        /* > 0 (the reference's default is 1e-6) */
+  int32_t* counts;             /* device: contacts per query */
+  int32_t* primitives;         /* device, optional: slot = query *
+                                  num_max_contacts + i: the triangle (b1) */
+  /* device, required with enable_contact, per slot: the reference's
+   * Contact::pos (3), normal (3) and penetration_depth (1).  As in the
+   * reference, a sphere's normal points from the triangle to the sphere and
+   * the other shapes' from the shape to the triangle, and depths are <= 0. */
+  double* contact_pos;
+  double* contact_normal;
+  double* contact_depth;
+  int64_t* stats;              /* device, optional: 2 per query, num_bv_tests
+                                  and num_leaf_tests */
+} fclx_mesh_shape_collide_batch_args;
+int fclx_mesh_shape_collide_batch(fclx_context* ctx,
+                                  const fclx_mesh_shape_collide_batch_args* args, void* stream);
+
 /* Batched fcl::collide between primitive shapes with the GST_INDEP solver:
  * per pair what ShapeShapeCollide (collision_func_matrix-inl.h:81-112) and
  * ShapeCollisionTraversalNode::leafTesting
