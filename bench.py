@@ -34,6 +34,8 @@ OPS_BV = 315           # FP32 ops of one OBB test (126 transform + 189 SAT)
 OPS_LEAF = 923         # FP64 ops of one triangle SAT
 BYTES_BV = 128         # two 64-B FP32 node records
 BYTES_LEAF = 144       # six FP64 vertices
+BYTES_BV64 = 128       # one FP64 OBB node record (mesh-vs-primitive descent)
+BYTES_LEAF_TRI = 128   # one triangle's three FP64 vertices (128-B record)
 OPS_RSS = 380          # ops of one RSS lower bound (126 transform + ~250 rectDistance)
 OPS_TRI_DIST = 900     # FP64 ops of one triangle distance
 
@@ -376,6 +378,143 @@ def bench_distance(args):
     ctx.close()
 
 
+def bench_meshshape(args):
+    """Mesh against primitives: a perturbed icosphere of 20,480 triangles in
+    BVHModel<OBBRSS> (an environment) against robot links modelled as boxes,
+    spheres, capsules, cylinders and a 7-vertex convex, posed with |t|
+    uniform in [0.8, 1.2] along a uniform direction so that most of them
+    straddle the surface; CollisionRequest defaults (num_max_contacts 1, no
+    contact geometry, GST_INDEP).  One step = one batch over all queries,
+    inputs resident."""
+    import numpy as np
+    import torch
+
+    import fcl_amd as F
+
+    n = args.ms_queries
+    t0 = time.time()
+    V, T = F.icosphere(32, 0.15, 0xE0)
+    hull = np.array([[1, 0, 0], [-1, 0, 0], [0, 1.5, 0], [0, -1, 0], [0, 0, 0.7],
+                     [0.2, 0.3, -1.1], [0.5, 0.5, 0.5]], np.float64) * 0.12
+    kinds = [F.BOX, F.SPHERE, F.CAPSULE, F.CYLINDER, F.CONVEX]
+    dims = [(0.3, 0.2, 0.4), (0.15, 0, 0), (0.08, 0.3, 0), (0.1, 0.3, 0), (0, 0, 0)]
+    rec, pool = F.shape_table(kinds, dims, [hull])
+    tf1, tf2 = F.poses(n, 1, 0.8, 1.2, 0xE1)
+    names = ["box", "sphere", "capsule", "cylinder", "convex"]
+    use = [names.index(k) for k in args.ms_kinds.split(",")]
+    qs = np.array(use, np.int32)[np.arange(n) % len(use)]
+    ctx = F.Context(0)
+    for kv in args.engine_param:
+        k, v = kv.split("=", 1)
+        ctx.set_engine_param(k, float(v))
+    mid = ctx.mesh(V, T)
+    log("mesh-shape workload: %d triangles (OBBRSS) vs %d primitives, built in %.1fs"
+        % (len(T), n, time.time() - t0))
+    dev = "cuda:0"
+    d_qm = torch.full((n,), mid, dtype=torch.int32, device=dev)
+    d_qs = torch.from_numpy(qs).to(dev)
+    d_rec = torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8)).to(dev)
+    d_pool = torch.from_numpy(pool).to(dev)
+    d1, d2 = torch.from_numpy(tf1).to(dev), torch.from_numpy(tf2).to(dev)
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    stats = torch.zeros((n, 2), dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def run(with_stats=False):
+        ctx.mesh_shape_collide_batch_async(d_qm, d_rec, d_pool, d_qs, d1, d2, 1, counts,
+                                           stats=stats if with_stats else None, stream=s)
+
+    run(True)
+    ctx.sync(s)
+    w_bv, w_leaf = (int(x) for x in stats.sum(0).tolist())
+    for _ in range(args.warmup):
+        run()
+    ctx.sync(s)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t_start = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        run()
+    ev[1].record(stream)
+    ctx.sync(s)
+    elapsed = time.perf_counter() - t_start
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n / (elapsed / args.steps)
+    t_k = gpu_ms * 1e-3
+    c = counts.cpu().numpy()
+    # FP64 work: an OBB test per BV test, and per leaf test one GJK loop
+    # pass as a floor (a leaf runs one to a few tens of passes)
+    ops = w_bv * OPS_BV + w_leaf * OPS_GJK_PASS
+    alg_bytes = w_bv * BYTES_BV64 + w_leaf * BYTES_LEAF_TRI
+    roofline = {
+        "kernel": "meshshape_collide<false> (one query per lane, FP64)",
+        "bound": "valu",
+        "achieved": ops / t_k / 1e12,
+        "peak": PEAK_FP64 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": ops / t_k / PEAK_FP64,
+        "traffic": None,
+        "algorithmic_ops": ops,
+        "ops_per_test": {"obb": OPS_BV, "leaf_gjk_pass_floor": OPS_GJK_PASS},
+        "bv_tests": w_bv, "leaf_tests": w_leaf,
+        "hbm_frac": alg_bytes / t_k / PEAK_HBM,
+        "algorithmic_bytes": alg_bytes,
+        "batch_ms_events": gpu_ms,
+    }
+    log("mesh-shape: %.2f ms/step (events %.2f), %.3g queries/s; %.1f BV and %.1f leaf tests "
+        "per query; %d of %d collide" % (ms_per_step, gpu_ms, value, w_bv / n, w_leaf / n,
+                                         int((c > 0).sum()), n))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        threads = host_threads()
+        om = fclo.OracleMesh(V, T)
+        rng = np.random.default_rng(7)
+        zeros = np.zeros(n, np.int32)
+        probe = np.sort(rng.choice(n, size=min(n, 2000), replace=False))
+        a = time.perf_counter()
+        fclo.mesh_shape_collide_batch([om], zeros[probe], rec, pool, qs[probe], tf1[probe],
+                                      tf2[probe], threads=threads)
+        probe_s = time.perf_counter() - a
+        n_s = int(min(n, max(len(probe), len(probe) * args.cpu_seconds / max(probe_s, 1e-3))))
+        sample = np.sort(rng.choice(n, size=n_s, replace=False))
+        a = time.perf_counter()
+        oc = fclo.mesh_shape_collide_batch([om], zeros[sample], rec, pool, qs[sample],
+                                           tf1[sample], tf2[sample], threads=threads)[0]
+        cpu_s = time.perf_counter() - a
+        match = bool(np.array_equal(oc, c[sample]))
+        cpu = {"value": n_s / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d queries, drawn uniformly (seed 7), OpenMP dynamic "
+                         "schedule; counts equal to the GPU's: %s" % (n_s, n, match)}
+        log("cpu oracle: %d queries in %.1fs on %d threads, equal %s" % (n_s, cpu_s, threads,
+                                                                         match))
+    out = {
+        "metric": "mesh-vs-primitive collide queries/sec (CollisionRequest defaults)",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded perturbed icosphere, primitives at |t| in [0.8, 1.2])",
+        "config": {"workload": "mesh-shape collide: icosphere of 20,480 tris (OBBRSS) vs "
+                               "box/sphere/capsule/cylinder/convex links, num_max_contacts 1",
+                   "queries": n, "kinds": args.ms_kinds, "engine_params": args.engine_param},
+        "colliding": int((c > 0).sum()),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_serving(args):
     """Robot-trajectory serving: one resident ~500k-triangle scene, a
     20-link robot, requests of 64-4096 waypoints arriving as Poisson
@@ -536,11 +675,13 @@ def bench_serving(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["mesh", "primitives", "serving", "distance"],
+    ap.add_argument("--workload",
+                    choices=["mesh", "primitives", "serving", "distance", "meshshape"],
                     default="mesh",
                     help="mesh: the batched uneven mesh collision (default); primitives: "
                          "4M primitive-shape pairs; serving: robot-trajectory requests at "
-                         "Poisson loads; distance: 10k mesh distance queries (20k-tri RSS)")
+                         "Poisson loads; distance: 10k mesh distance queries (20k-tri RSS); "
+                         "meshshape: 200k primitives against a 20k-tri mesh")
     ap.add_argument("--serve-rates", type=float, nargs="+",
                     default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--serve-seconds", type=float, default=1.0,
@@ -551,6 +692,9 @@ def main():
                     help="engine behind the request loop (bool: booleans only)")
     ap.add_argument("--prim-pairs", type=int, default=4_000_000)
     ap.add_argument("--dist-queries", type=int, default=10_000)
+    ap.add_argument("--ms-queries", type=int, default=200_000)
+    ap.add_argument("--ms-kinds", default="box,sphere,capsule,cylinder,convex",
+                    help="primitive kinds of the meshshape workload, used in turn")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -584,6 +728,9 @@ def main():
         return
     if args.workload == "distance":
         bench_distance(args)
+        return
+    if args.workload == "meshshape":
+        bench_meshshape(args)
         return
 
     import numpy as np

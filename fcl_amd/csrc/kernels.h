@@ -103,7 +103,8 @@ struct MeshShapeLaunch {
   long long* stats;             // optional, 2 per query: BV tests, leaf tests
   int32_t* status;
 };
-hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, hipStream_t s);
+// waves: register budget, 1..3 waves per SIMD
+hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s);
 size_t mesh_shape_epa_lane_bytes();
 
 hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);

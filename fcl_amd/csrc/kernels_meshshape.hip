@@ -431,9 +431,59 @@ __device__ __forceinline__ bool load_query(const MeshShapeLaunch& L, int q, shp:
   return false;
 }
 
+// The leaf test of one (shape, triangle) pair, kept out of line so that the
+// descent loop's registers are not sized for GJK and EPA.
 template <bool kContact>
-__global__ void __launch_bounds__(kBlk) meshshape_collide(MeshShapeLaunch L, shp::EpaLane* lanes,
-                                                          shp::EpaWave* waves) {
+__device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, const shp::Xf& T2,
+                                       const double* tv, double tol, shp::Epa& E,
+                                       shp::Contact& c) {
+  bool hit;
+  if (s.type == shp::kSphere) {
+    // gjk_solver_indep-inl.h:527-545: the triangle posed by the mesh's tf
+    D3 P[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P[k] = shp::apply(T1, mk(tv[3 * k], tv[3 * k + 1], tv[3 * k + 2]));
+    D3 p, n;
+    double d;
+    hit = sphere_triangle(s.dims[0], T2.t, P[0], P[1], P[2], p, d, n);
+    if (hit) {
+      c.pos = p;
+      c.normal = shp::neg(n);  // the leaf stores minus the solver's normal
+      c.depth = d;
+    }
+  } else {
+    shp::Shape tri;
+    tri.type = shp::kTriangle;
+    tri.nv = 3;
+    tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
+    tri.verts = tv;
+    shp::MDiff m;
+    shp::make_mdiff(s, T2, tri, T1, m);
+    shp::Gjk g;
+    hit = shp::gjk_evaluate(g, m, tol) == shp::kGjkInside;
+    if constexpr (kContact) {
+      if (hit) {
+        // the leaf's -normal undoes the solver's: EPA's normal stays
+        bool more = shp::epa_start(E, g, m, tol);
+        while (more) more = shp::epa_step(E, m, tol);
+        c = shp::epa_finish(E, m, T2);
+      }
+    }
+  }
+  return hit;
+}
+
+// Lanes of a wavefront step together: a lane whose descent reached a leaf
+// that passed its BV test waits there, the others keep descending, and the
+// waiting lanes test their leaves together once they are at least half of
+// the live lanes (or nobody can descend).  The costly leaf test (GJK) then
+// runs once for many lanes instead of once per descent step.  Each lane's
+// own order of tests is the reference's, so counts, triangles and test
+// statistics are unchanged.
+template <bool kContact, int kWaves>
+__global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunch L,
+                                                                  shp::EpaLane* lanes,
+                                                                  shp::EpaWave* waves) {
   const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * kBlk;
   shp::Epa E;
@@ -442,101 +492,108 @@ __global__ void __launch_bounds__(kBlk) meshshape_collide(MeshShapeLaunch L, shp
     E.S = waves + gid / 64;
     E.lane = threadIdx.x & 63;
   }
-  for (size_t qq = gid; qq < static_cast<size_t>(L.num_queries); qq += stride) {
-    const int q = static_cast<int>(qq);
-    shp::Shape s;
-    if (!load_query(L, q, s)) {
+  size_t qq = gid;
+  shp::Shape s;
+  shp::Xf T1, T2;
+  double sa[9], sto[3], sext[3];
+  const ObbNode64* nodes = nullptr;
+  const double* tris = nullptr;
+  int stk[kMsStack];
+  int sp = 0, b = 0, count = 0, pend = -1;
+  long long nbv = 0, nleaf = 0;
+  bool live = false;  // a query in progress
+  // take queries until one is usable (the others are answered at once)
+  auto start = [&]() {
+    while (qq < static_cast<size_t>(L.num_queries)) {
+      const int q = static_cast<int>(qq);
+      if (load_query(L, q, s)) {
+        const MeshDev& M = L.meshes[L.query_meshes[q]];
+        nodes = M.obb64;
+        tris = M.tri_verts;
+        T1 = shp::xf_load(L.tf1 + 12 * qq);
+        T2 = shp::xf_load(L.tf2 + 12 * qq);
+        shape_obb(s, T2, sa, sto, sext);
+        sp = 0;
+        b = 0;
+        count = 0;
+        pend = -1;
+        nbv = 0;
+        nleaf = 0;
+        live = true;
+        return;
+      }
       L.counts[q] = 0;
       if (L.stats) {
         L.stats[2 * qq] = 0;
         L.stats[2 * qq + 1] = 0;
       }
       atomicOr(L.status, kStatusBadQuery);
-      continue;
+      qq += stride;
     }
-    const MeshDev& M = L.meshes[L.query_meshes[q]];
-    const shp::Xf T1 = shp::xf_load(L.tf1 + 12 * qq);
-    const shp::Xf T2 = shp::xf_load(L.tf2 + 12 * qq);
-    double sa[9], sto[3], sext[3];
-    shape_obb(s, T2, sa, sto, sext);
-    M3 R0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) R0.m[i] = T1.R[i];
-    const V3 T0 = v3(T1.t.v[0], T1.t.v[1], T1.t.v[2]);
-    int stk[kMsStack];
-    int sp = 0, b = 0, count = 0;
-    long long nbv = 0, nleaf = 0;
-    while (true) {
-      const ObbNode64& nd = M.obb64[b];
-      ++nbv;
-      double B[9], Tr[3];
-      box_relative(R0, T0, sa, sto, nd.axis, nd.to, B, Tr);
-      if (!obb_disjoint64(B, Tr, sext, nd.extent)) {
-        const int fc = nd.first_child;
-        if (fc >= 0) {
-          stk[sp++] = fc + 1;
-          b = fc;
-          continue;
-        }
-        const int prim = -(fc + 1);
-        ++nleaf;
-        const double* tv = M.tri_verts + static_cast<size_t>(prim) * kTriVertsStride;
-        bool hit;
-        shp::Contact c;
-        if (s.type == shp::kSphere) {
-          // gjk_solver_indep-inl.h:527-545: the triangle posed by the mesh's tf
-          D3 P[3];
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            P[k] = shp::apply(T1, mk(tv[3 * k], tv[3 * k + 1], tv[3 * k + 2]));
-          D3 p, n;
-          double d;
-          hit = sphere_triangle(s.dims[0], T2.t, P[0], P[1], P[2], p, d, n);
-          if (hit) {
-            c.pos = p;
-            c.normal = shp::neg(n);  // the leaf stores minus the solver's normal
-            c.depth = d;
-          }
-        } else {
-          shp::Shape tri;
-          tri.type = shp::kTriangle;
-          tri.nv = 3;
-          tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
-          tri.verts = tv;
-          shp::MDiff m;
-          shp::make_mdiff(s, T2, tri, T1, m);
-          shp::Gjk g;
-          hit = shp::gjk_evaluate(g, m, L.tol) == shp::kGjkInside;
-          if constexpr (kContact) {
-            if (hit) {
-              // the leaf's -normal undoes the solver's: EPA's normal stays
-              bool more = shp::epa_start(E, g, m, L.tol);
-              while (more) more = shp::epa_step(E, m, L.tol);
-              c = shp::epa_finish(E, m, T2);
-            }
-          }
-        }
-        if (hit) {
-          const size_t k = qq * L.cap + count;
-          if (L.prims) L.prims[k] = prim;
-          if constexpr (kContact) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              L.contact_pos[3 * k + j] = c.pos.v[j];
-              L.contact_normal[3 * k + j] = c.normal.v[j];
-            }
-            L.contact_depth[k] = c.depth;
-          }
-          if (++count >= L.cap) break;
-        }
-      }
-      if (sp == 0) break;
-      b = stk[--sp];
-    }
-    L.counts[q] = count;
+    live = false;
+  };
+  auto finish = [&]() {
+    L.counts[qq] = count;
     if (L.stats) {
       L.stats[2 * qq] = nbv;
       L.stats[2 * qq + 1] = nleaf;
+    }
+    qq += stride;
+    start();
+  };
+  auto next = [&]() {  // the next node of the descent, or the query's end
+    if (sp == 0)
+      finish();
+    else
+      b = stk[--sp];
+  };
+  start();
+  while (true) {
+    const bool walk = live && pend < 0;
+    const unsigned long long mw = __ballot(walk);
+    const unsigned long long ml = __ballot(live && pend >= 0);
+    if (!mw && !ml) break;
+    if (ml && (!mw || 2 * __popcll(ml) >= __popcll(mw | ml))) {
+      if (pend < 0) continue;
+      const int prim = pend;
+      pend = -1;
+      ++nleaf;
+      shp::Contact c;
+      const bool hit = leaf_test<kContact>(s, T1, T2, tris + static_cast<size_t>(prim) * kTriVertsStride,
+                                           L.tol, E, c);
+      if (hit) {
+        const size_t k = qq * L.cap + count;
+        if (L.prims) L.prims[k] = prim;
+        if constexpr (kContact) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            L.contact_pos[3 * k + j] = c.pos.v[j];
+            L.contact_normal[3 * k + j] = c.normal.v[j];
+          }
+          L.contact_depth[k] = c.depth;
+        }
+        if (++count >= L.cap) {
+          finish();
+          continue;
+        }
+      }
+      next();
+    } else if (walk) {
+      const ObbNode64& nd = nodes[b];
+      ++nbv;
+      M3 R0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) R0.m[i] = T1.R[i];
+      double B[9], Tr[3];
+      box_relative(R0, v3(T1.t.v[0], T1.t.v[1], T1.t.v[2]), sa, sto, nd.axis, nd.to, B, Tr);
+      if (obb_disjoint64(B, Tr, sext, nd.extent)) {
+        next();
+      } else if (nd.first_child >= 0) {
+        stk[sp++] = nd.first_child + 1;
+        b = nd.first_child;
+      } else {
+        pend = -(nd.first_child + 1);  // tested in the next leaf round
+      }
     }
   }
 }
@@ -545,11 +602,26 @@ __global__ void __launch_bounds__(kBlk) meshshape_collide(MeshShapeLaunch L, shp
 
 size_t mesh_shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::EpaWave) / 64; }
 
-hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, hipStream_t s) {
+template <bool kContact>
+void launch_variant(int waves, unsigned grid, hipStream_t s, const MeshShapeLaunch& L,
+                    shp::EpaLane* lanes, shp::EpaWave* wv) {
+  switch (waves) {
+    case 1:
+      meshshape_collide<kContact, 1><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+      break;
+    case 3:
+      meshshape_collide<kContact, 3><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+      break;
+    default:
+      meshshape_collide<kContact, 2><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+  }
+}
+
+hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s) {
   if (L.num_queries <= 0) return hipSuccess;
   const long long blocks = (static_cast<long long>(L.num_queries) + kBlk - 1) / kBlk;
   if (!L.contact) {
-    meshshape_collide<false><<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, nullptr, nullptr);
+    launch_variant<false>(waves, static_cast<unsigned>(blocks), s, L, nullptr, nullptr);
     return hipGetLastError();
   }
   int dev = 0, cus = 0;
@@ -560,18 +632,18 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, hipStream_t s) {
   const long long grid = std::min<long long>(blocks, static_cast<long long>(cus) * kEpaBlocksPerCu);
   const size_t nl = static_cast<size_t>(grid) * kBlk;
   shp::EpaLane* lanes = nullptr;
-  shp::EpaWave* waves = nullptr;
+  shp::EpaWave* wv = nullptr;
   e = hipMallocAsync(reinterpret_cast<void**>(&lanes), nl * sizeof(shp::EpaLane), s);
   if (e != hipSuccess) return e;
-  e = hipMallocAsync(reinterpret_cast<void**>(&waves), nl / 64 * sizeof(shp::EpaWave), s);
+  e = hipMallocAsync(reinterpret_cast<void**>(&wv), nl / 64 * sizeof(shp::EpaWave), s);
   if (e != hipSuccess) {
     (void)hipFreeAsync(lanes, s);
     return e;
   }
-  meshshape_collide<true><<<static_cast<unsigned>(grid), kBlk, 0, s>>>(L, lanes, waves);
+  launch_variant<true>(waves, static_cast<unsigned>(grid), s, L, lanes, wv);
   e = hipGetLastError();
   (void)hipFreeAsync(lanes, s);
-  (void)hipFreeAsync(waves, s);
+  (void)hipFreeAsync(wv, s);
   return e;
 }
 

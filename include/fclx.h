@@ -337,6 +337,8 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "dist_queries_per_wave" 2, 4, 8  queries a FAST distance wavefront keeps
  *                               in flight (default 4)
  *   "dist_waves_per_simd" 2, 3  register budget of the FAST distance kernel
+ *   "meshshape_waves_per_simd" 1..3  register budget of the mesh-vs-primitive
+ *                               kernel (default 1)
  *   "dist_width"        1..64   most bounds of one query per distance step
  *                               (default 32)
  *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are
