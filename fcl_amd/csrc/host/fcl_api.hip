@@ -11,6 +11,7 @@
 // context (device 0, or FCLX_DEVICE).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 #include <cstdio>
@@ -195,6 +196,110 @@ std::size_t shape_collide(const CollisionGeometry<double>* o1, const fclx_shape&
   return result.numContacts();
 }
 
+// BVHShapeCollider<OBBRSS> (collision_func_matrix-inl.h:500-521,
+// orientedBVHShapeCollide 375-419) through the batch kernel with one query:
+// contacts (triangle, NONE) in the reference's order, appended to result.
+std::size_t mesh_shape_collide(const CollisionGeometry<double>* o1, int mesh,
+                               const Transform3d& tf1, const CollisionGeometry<double>* o2,
+                               const fclx_shape& r2, const std::vector<double>& v2,
+                               const Transform3d& tf2, const CollisionRequest<double>& request,
+                               CollisionResult<double>& result) {
+  if (request.isSatisfied(result)) return result.numContacts();
+  if (request.gjk_solver_type == GST_LIBCCD && !g_warned_libccd.exchange(true))
+    std::cerr << "fcl: the libccd solver is not available; GJK/EPA pairs use GST_INDEP\n";
+  if (request.enable_cached_gjk_guess && !g_warned_guess.exchange(true))
+    std::cerr << "fcl: cached GJK guesses are not supported; GJK starts from (1, 0, 0)\n";
+  if (request.enable_cost && !g_warned_cost.exchange(true))
+    std::cerr << "fcl: cost sources are not computed on the GPU mesh path\n";
+  fclx_context* ctx = engine();
+  if (!ctx) return result.numContacts();
+  fclx_mesh_info info{};
+  if (fclx_mesh_get_info(ctx, mesh, &info) != FCLX_OK) return result.numContacts();
+  // a mesh-shape query has at most one contact per triangle
+  const std::size_t room = request.num_max_contacts - result.numContacts();
+  const int32_t cap = static_cast<int32_t>(
+      std::max<std::size_t>(1, std::min<std::size_t>(room, static_cast<std::size_t>(info.num_tris))));
+  fclx_shape rec = r2;
+  rec.vertex_offset = 0;
+  // one device block: ids, record, vertices, poses, counts, triangles, geometry
+  const size_t nv = v2.size() / 3;
+  const size_t doubles = 24 + 3 * nv + 7 * static_cast<size_t>(cap);
+  const size_t ints = 3 + static_cast<size_t>(cap);
+  const size_t bytes = 8 * doubles + sizeof(fclx_shape) + 4 * ints + 64;
+  char* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), bytes) != hipSuccess) {
+    std::cerr << "fcl: mesh-shape collide: device allocation failed\n";
+    return result.numContacts();
+  }
+  double* d_tf = reinterpret_cast<double*>(d);
+  double* d_verts = d_tf + 24;
+  double* d_pos = d_verts + 3 * nv;
+  double* d_nrm = d_pos + 3 * static_cast<size_t>(cap);
+  double* d_dep = d_nrm + 3 * static_cast<size_t>(cap);
+  fclx_shape* d_rec = reinterpret_cast<fclx_shape*>(d_dep + cap);
+  int32_t* d_int = reinterpret_cast<int32_t*>(d_rec + 1);
+  int32_t* d_prims = d_int + 3;
+  int32_t hin[3] = {mesh, 0, -1};
+  double tfs[24];
+  to_pose(tf1, tfs);
+  to_pose(tf2, tfs + 12);
+  bool ok = hipMemcpy(d_tf, tfs, sizeof(tfs), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(d_rec, &rec, sizeof(rec), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(d_int, hin, sizeof(hin), hipMemcpyHostToDevice) == hipSuccess &&
+            (nv == 0 || hipMemcpy(d_verts, v2.data(), 8 * v2.size(), hipMemcpyHostToDevice) ==
+                            hipSuccess);
+  int32_t n = 0;
+  std::vector<int32_t> prims(cap);
+  std::vector<double> geo(7 * static_cast<size_t>(cap));
+  if (ok) {
+    fclx_mesh_shape_collide_batch_args a{};
+    a.num_queries = 1;
+    a.query_meshes = d_int;
+    a.query_shapes = d_int + 1;
+    a.num_shapes = 1;
+    a.shapes = d_rec;
+    a.vertices = nv ? d_verts : nullptr;
+    a.num_vertices = static_cast<int64_t>(nv);
+    a.tf1 = d_tf;
+    a.tf2 = d_tf + 12;
+    a.num_max_contacts = cap;
+    a.enable_contact = request.enable_contact ? 1 : 0;
+    a.gjk_tolerance = request.gjk_tolerance;
+    a.counts = d_int + 2;
+    a.primitives = d_prims;
+    a.contact_pos = d_pos;
+    a.contact_normal = d_nrm;
+    a.contact_depth = d_dep;
+    ok = fclx_mesh_shape_collide_batch(ctx, &a, nullptr) == FCLX_OK &&
+         fclx_sync(ctx, nullptr) == FCLX_OK &&
+         hipMemcpy(&n, d_int + 2, 4, hipMemcpyDeviceToHost) == hipSuccess && n >= 0 &&
+         n <= cap &&
+         hipMemcpy(prims.data(), d_prims, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost) ==
+             hipSuccess &&
+         (!request.enable_contact ||
+          hipMemcpy(geo.data(), d_pos, 8 * 7 * static_cast<size_t>(cap),
+                    hipMemcpyDeviceToHost) == hipSuccess);
+  }
+  (void)hipFree(d);
+  if (!ok) {
+    std::cerr << "fcl: mesh-shape collide failed: " << fclx_last_error(ctx) << "\n";
+    return result.numContacts();
+  }
+  const double* pos = geo.data();
+  const double* nrm = pos + 3 * static_cast<size_t>(cap);
+  const double* dep = nrm + 3 * static_cast<size_t>(cap);
+  for (int32_t i = 0; i < n; ++i) {
+    if (request.enable_contact)
+      result.addContact(Contact<double>(
+          o1, o2, prims[i], Contact<double>::NONE,
+          Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+          Vector3d(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), dep[i]));
+    else
+      result.addContact(Contact<double>(o1, o2, prims[i], Contact<double>::NONE));
+  }
+  return result.numContacts();
+}
+
 void unsupported(const char* what, const CollisionGeometry<double>* o1,
                  const CollisionGeometry<double>* o2) {
   std::cerr << "Warning: " << what << " function between node type "
@@ -226,6 +331,12 @@ std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform
       return detail::shape_collide(o1, r1, v1, tf1, o2, r2, v2, tf2, request, result);
   }
   const int id1 = detail::mesh_id_of(o1, BV_OBBRSS);
+  {
+    fclx_shape r2;
+    std::vector<double> v2;
+    if (id1 >= 0 && detail::to_shape(o2, r2, v2))
+      return detail::mesh_shape_collide(o1, id1, tf1, o2, r2, v2, tf2, request, result);
+  }
   const int id2 = detail::mesh_id_of(o2, BV_OBBRSS);
   if (id1 < 0 || id2 < 0) {
     detail::unsupported("collision", o1, o2);

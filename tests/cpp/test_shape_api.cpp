@@ -17,8 +17,10 @@
 #include "fcl/geometry/shape/convex.h"
 #include "fcl/geometry/shape/cylinder.h"
 #include "fcl/geometry/shape/sphere.h"
+#include "fcl/geometry/bvh/BVH_model.h"
 #include "fcl/narrowphase/collision.h"
 #include "fclo.h"
+#include "fclx_synth.h"
 
 using namespace fcl;
 
@@ -177,6 +179,64 @@ int main() {
     CHECK(collide(&s1, Transform3d::Identity(), &s2, tf, req, res) == 1);
     CHECK(res.getContact(0).penetration_depth == 0.25);
     CHECK(res.getContact(0).normal[0] == 1 && res.getContact(0).pos[0] == 0.375);
+  }
+  // a mesh (BVHModel<OBBRSS>) against each primitive kind: contacts, their
+  // triangles (b1) and geometry equal to the oracle's mesh-shape collide
+  {
+    int32_t nv = 0, nt = 0;
+    fclx_synth_icosphere_size(8, &nv, &nt);
+    std::vector<double> V(3 * static_cast<size_t>(nv));
+    std::vector<int32_t> T(3 * static_cast<size_t>(nt));
+    CHECK(fclx_synth_icosphere(8, 0.15, 0xF0, V.data(), T.data()) == 0);
+    std::vector<Vector3d> ps;
+    std::vector<Triangle> ts;
+    for (int32_t i = 0; i < nv; ++i) ps.emplace_back(V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+    for (int32_t i = 0; i < nt; ++i) ts.emplace_back(T[3 * i], T[3 * i + 1], T[3 * i + 2]);
+    auto mesh = std::make_shared<BVHModel<OBBRSSd>>();
+    CHECK(mesh->beginModel() == BVH_OK);
+    CHECK(mesh->addSubModel(ps, ts) == BVH_OK);
+    CHECK(mesh->endModel() == BVH_OK);
+    fclo_model om;
+    CHECK(fclo_model_build(&om, FCLO_BV_OBBRSS, V.data(), nv, T.data(), nt) == 0);
+    long ms_queries = 0, ms_hits = 0;
+    for (int type = 0; type < 5; ++type)
+      for (int rep = 0; rep < 30; ++rep) {
+        Shape b = make_shape(type, rng);
+        const Transform3d tf1 = random_pose(rng, 0.2), tf2 = random_pose(rng, 1.0);
+        double p1[12], p2[12];
+        to_pose(tf1, p1);
+        to_pose(tf2, p2);
+        for (int cap : {1, 5}) {
+          for (bool geometry : {false, true}) {
+            CollisionRequest<double> req(cap, geometry);
+            req.gjk_solver_type = GST_INDEP;
+            CollisionResult<double> res;
+            const std::size_t n = collide(mesh.get(), tf1, b.geom.get(), tf2, req, res);
+            int prims[8];
+            fclo_contact oc[8];
+            fclo_collide_stats st;
+            const int on = fclo_mesh_shape_collide(&om, p1, &b.o, p2, cap, geometry ? 1 : 0,
+                                                   1e-6, prims, oc, &st);
+            ++ms_queries;
+            CHECK(n == static_cast<std::size_t>(on) && res.numContacts() == n);
+            for (std::size_t i = 0; i < n && static_cast<int>(i) < on; ++i) {
+              const Contact<double>& c = res.getContact(i);
+              CHECK(c.o1 == mesh.get() && c.o2 == b.geom.get());
+              CHECK(c.b1 == prims[i] && c.b2 == Contact<double>::NONE);
+              if (!geometry) continue;
+              for (int k = 0; k < 3; ++k) {
+                CHECK(c.pos[k] == oc[i].pos[k]);
+                CHECK(c.normal[k] == oc[i].normal[k]);
+              }
+              CHECK(c.penetration_depth == oc[i].depth);
+            }
+            ms_hits += n > 0;
+          }
+        }
+      }
+    fclo_model_free(&om);
+    CHECK(ms_hits > 40);
+    std::printf("%ld mesh-shape queries, %ld colliding\n", ms_queries, ms_hits);
   }
   if (failures == 0) std::printf("PASS\n");
   return failures == 0 ? 0 : 1;
