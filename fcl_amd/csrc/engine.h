@@ -24,7 +24,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
 // Tunables of the exhaustive path, per context (defaults chosen by measurement).
 struct EngineTuning {
   DistanceTuning dist;          // FAST distance engine
-  int meshshape_waves_per_simd = 1;  // register budget of the mesh-vs-primitive kernel
+  int meshshape_waves_per_simd = 2;  // register budget of the mesh-vs-primitive kernel
   int max_levels = 24;          // breadth-first levels before a depth-first round
   long long front_target = 1 << 18;  // stop expanding once the front is this large (a full
                                      // step for every resident wavefront)

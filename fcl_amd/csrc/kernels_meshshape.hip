@@ -431,6 +431,57 @@ __device__ __forceinline__ bool load_query(const MeshShapeLaunch& L, int q, shp:
   return false;
 }
 
+// Queries grouped by shape kind, so that a wavefront's lanes run the same
+// leaf test (the order of queries does not change any query's answer).
+// Keys: the five kinds, then 5 for a query whose shape id is out of range
+// (answered as a bad query by the main kernel).  ctr[0..5]: counts,
+// ctr[8..13]: placement cursors.
+constexpr int kKinds = 6;
+
+__device__ __forceinline__ int kind_of(const MeshShapeLaunch& L, int q) {
+  const int si = L.query_shapes[q];
+  if (si < 0 || si >= L.num_shapes) return kKinds - 1;
+  const int t = L.shapes[si].type;
+  return t >= 0 && t < kKinds - 1 ? t : kKinds - 1;
+}
+
+__global__ void ms_zero(int32_t* ctr) {
+  if (threadIdx.x < 16) ctr[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(kBlk) ms_count(MeshShapeLaunch L, int32_t* ctr) {
+  __shared__ int cnt[kKinds];
+  if (threadIdx.x < kKinds) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < L.num_queries) atomicAdd(&cnt[kind_of(L, i)], 1);
+  __syncthreads();
+  if (threadIdx.x < kKinds && cnt[threadIdx.x]) atomicAdd(&ctr[threadIdx.x], cnt[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(kBlk) ms_place(MeshShapeLaunch L, int32_t* ctr,
+                                                 int32_t* order) {
+  __shared__ int cnt[kKinds];
+  __shared__ int base[kKinds];
+  if (threadIdx.x < kKinds) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  int key = 0, off = 0;
+  if (i < L.num_queries) {
+    key = kind_of(L, i);
+    off = atomicAdd(&cnt[key], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kKinds) {
+    int start = 0;
+    for (int k = 0; k < static_cast<int>(threadIdx.x); ++k) start += ctr[k];
+    base[threadIdx.x] =
+        start + (cnt[threadIdx.x] ? atomicAdd(&ctr[8 + threadIdx.x], cnt[threadIdx.x]) : 0);
+  }
+  __syncthreads();
+  if (i < L.num_queries) order[base[key] + off] = i;
+}
+
 // The leaf test of one (shape, triangle) pair, kept out of line so that the
 // descent loop's registers are not sized for GJK and EPA.
 template <bool kContact>
@@ -482,6 +533,7 @@ __device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, c
 // statistics are unchanged.
 template <bool kContact, int kWaves>
 __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunch L,
+                                                                  const int32_t* order,
                                                                   shp::EpaLane* lanes,
                                                                   shp::EpaWave* waves) {
   const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
@@ -492,7 +544,8 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
     E.S = waves + gid / 64;
     E.lane = threadIdx.x & 63;
   }
-  size_t qq = gid;
+  size_t qq = gid;  // position in the (sorted) order
+  size_t q = 0;     // the query at that position
   shp::Shape s;
   shp::Xf T1, T2;
   double sa[9], sto[3], sext[3];
@@ -505,13 +558,13 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
   // take queries until one is usable (the others are answered at once)
   auto start = [&]() {
     while (qq < static_cast<size_t>(L.num_queries)) {
-      const int q = static_cast<int>(qq);
-      if (load_query(L, q, s)) {
+      q = order ? static_cast<size_t>(order[qq]) : qq;
+      if (load_query(L, static_cast<int>(q), s)) {
         const MeshDev& M = L.meshes[L.query_meshes[q]];
         nodes = M.obb64;
         tris = M.tri_verts;
-        T1 = shp::xf_load(L.tf1 + 12 * qq);
-        T2 = shp::xf_load(L.tf2 + 12 * qq);
+        T1 = shp::xf_load(L.tf1 + 12 * q);
+        T2 = shp::xf_load(L.tf2 + 12 * q);
         shape_obb(s, T2, sa, sto, sext);
         sp = 0;
         b = 0;
@@ -524,8 +577,8 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
       }
       L.counts[q] = 0;
       if (L.stats) {
-        L.stats[2 * qq] = 0;
-        L.stats[2 * qq + 1] = 0;
+        L.stats[2 * q] = 0;
+        L.stats[2 * q + 1] = 0;
       }
       atomicOr(L.status, kStatusBadQuery);
       qq += stride;
@@ -533,10 +586,10 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
     live = false;
   };
   auto finish = [&]() {
-    L.counts[qq] = count;
+    L.counts[q] = count;
     if (L.stats) {
-      L.stats[2 * qq] = nbv;
-      L.stats[2 * qq + 1] = nleaf;
+      L.stats[2 * q] = nbv;
+      L.stats[2 * q + 1] = nleaf;
     }
     qq += stride;
     start();
@@ -562,7 +615,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
       const bool hit = leaf_test<kContact>(s, T1, T2, tris + static_cast<size_t>(prim) * kTriVertsStride,
                                            L.tol, E, c);
       if (hit) {
-        const size_t k = qq * L.cap + count;
+        const size_t k = q * L.cap + count;
         if (L.prims) L.prims[k] = prim;
         if constexpr (kContact) {
 #pragma unroll
@@ -604,46 +657,68 @@ size_t mesh_shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::E
 
 template <bool kContact>
 void launch_variant(int waves, unsigned grid, hipStream_t s, const MeshShapeLaunch& L,
-                    shp::EpaLane* lanes, shp::EpaWave* wv) {
+                    const int32_t* order, shp::EpaLane* lanes, shp::EpaWave* wv) {
   switch (waves) {
     case 1:
-      meshshape_collide<kContact, 1><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+      meshshape_collide<kContact, 1><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
       break;
     case 3:
-      meshshape_collide<kContact, 3><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+      meshshape_collide<kContact, 3><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
       break;
     default:
-      meshshape_collide<kContact, 2><<<grid, kBlk, 0, s>>>(L, lanes, wv);
+      meshshape_collide<kContact, 2><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
   }
 }
+
+// Batches at least this large are grouped by shape kind first.
+constexpr int kSortMin = 4096;
 
 hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s) {
   if (L.num_queries <= 0) return hipSuccess;
   const long long blocks = (static_cast<long long>(L.num_queries) + kBlk - 1) / kBlk;
+  // the order of queries grouped by shape kind (stream-ordered memory)
+  int32_t* order = nullptr;
+  hipError_t e = hipSuccess;
+  if (L.num_queries >= kSortMin) {
+    e = hipMallocAsync(reinterpret_cast<void**>(&order),
+                       (static_cast<size_t>(L.num_queries) + 16) * sizeof(int32_t), s);
+    if (e != hipSuccess) return e;
+    int32_t* ctr = order + L.num_queries;
+    ms_zero<<<1, 64, 0, s>>>(ctr);
+    ms_count<<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, ctr);
+    ms_place<<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, ctr, order);
+  }
   if (!L.contact) {
-    launch_variant<false>(waves, static_cast<unsigned>(blocks), s, L, nullptr, nullptr);
-    return hipGetLastError();
+    launch_variant<false>(waves, static_cast<unsigned>(blocks), s, L, order, nullptr, nullptr);
+    e = hipGetLastError();
+    if (order) (void)hipFreeAsync(order, s);
+    return e;
   }
   int dev = 0, cus = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
+  e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) {
+    if (order) (void)hipFreeAsync(order, s);
+    return e;
+  }
   const long long grid = std::min<long long>(blocks, static_cast<long long>(cus) * kEpaBlocksPerCu);
   const size_t nl = static_cast<size_t>(grid) * kBlk;
   shp::EpaLane* lanes = nullptr;
   shp::EpaWave* wv = nullptr;
   e = hipMallocAsync(reinterpret_cast<void**>(&lanes), nl * sizeof(shp::EpaLane), s);
-  if (e != hipSuccess) return e;
-  e = hipMallocAsync(reinterpret_cast<void**>(&wv), nl / 64 * sizeof(shp::EpaWave), s);
+  if (e == hipSuccess) {
+    e = hipMallocAsync(reinterpret_cast<void**>(&wv), nl / 64 * sizeof(shp::EpaWave), s);
+    if (e != hipSuccess) (void)hipFreeAsync(lanes, s);
+  }
   if (e != hipSuccess) {
-    (void)hipFreeAsync(lanes, s);
+    if (order) (void)hipFreeAsync(order, s);
     return e;
   }
-  launch_variant<true>(waves, static_cast<unsigned>(grid), s, L, lanes, wv);
+  launch_variant<true>(waves, static_cast<unsigned>(grid), s, L, order, lanes, wv);
   e = hipGetLastError();
   (void)hipFreeAsync(lanes, s);
   (void)hipFreeAsync(wv, s);
+  if (order) (void)hipFreeAsync(order, s);
   return e;
 }
 

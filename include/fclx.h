@@ -338,7 +338,7 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               in flight (default 4)
  *   "dist_waves_per_simd" 2, 3  register budget of the FAST distance kernel
  *   "meshshape_waves_per_simd" 1..3  register budget of the mesh-vs-primitive
- *                               kernel (default 1)
+ *                               kernel (default 2)
  *   "dist_width"        1..64   most bounds of one query per distance step
  *                               (default 32)
  *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are

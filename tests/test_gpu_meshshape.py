@@ -89,7 +89,7 @@ def test_mesh_shape_parity(ctx, kind, cap):
     m = fclo.OracleMesh(V, T)
     k = KINDS.index(kind)
     rec, pool = shape_table([kind], [DIMS[k]], [HULL] if kind == CONVEX else [])
-    n = 1500
+    n = 4500  # above the size from which batches are grouped by shape kind
     tf1, tf2 = _case(n, 100 + k)
     counts = check(ctx, [m], [mid], np.zeros(n, np.int32), rec, pool, np.zeros(n, np.int32),
                    tf1, tf2, cap, False, "kind %d" % kind)
@@ -102,7 +102,7 @@ def test_mesh_shape_contact_parity(ctx):
     mids = [ctx.mesh(V, T) for V, T in meshes]
     ms = [fclo.OracleMesh(V, T) for V, T in meshes]
     rec, pool = shape_table(KINDS, DIMS, [HULL])
-    n = 1200
+    n = 4200  # mixed kinds, grouped by kind on the device
     rng = np.random.default_rng(9)
     qm = rng.integers(0, 2, n).astype(np.int32)
     qs = rng.integers(0, 5, n).astype(np.int32)
@@ -110,6 +110,9 @@ def test_mesh_shape_contact_parity(ctx):
     for cap in (1, 6, 64):
         counts = check(ctx, ms, mids, qm, rec, pool, qs, tf1, tf2, cap, True, "contact")
         assert counts.sum() > n // 4
+    # small batches are not grouped: the same answers
+    check(ctx, ms, mids, qm[:900], rec, pool, qs[:900], tf1[:900], tf2[:900], 6, True,
+          "contact, ungrouped")
     for mid in mids:
         ctx.mesh_destroy(mid)
 
