@@ -338,6 +338,15 @@ std::size_t collide<double>(const CollisionGeometry<double>* o1, const Transform
       return detail::mesh_shape_collide(o1, id1, tf1, o2, r2, v2, tf2, request, result);
   }
   const int id2 = detail::mesh_id_of(o2, BV_OBBRSS);
+  {
+    // a shape first and a mesh second: the reference calls the mesh-shape
+    // cell with the two swapped (collision-inl.h:120-129), so the contacts
+    // name the mesh first
+    fclx_shape r1;
+    std::vector<double> v1;
+    if (id2 >= 0 && detail::to_shape(o1, r1, v1))
+      return detail::mesh_shape_collide(o2, id2, tf2, o1, r1, v1, tf1, request, result);
+  }
   if (id1 < 0 || id2 < 0) {
     detail::unsupported("collision", o1, o2);
     return 0;

@@ -231,6 +231,14 @@ int main() {
               CHECK(c.penetration_depth == oc[i].depth);
             }
             ms_hits += n > 0;
+            // shape first: the same contacts, the mesh still named first
+            CollisionResult<double> sw;
+            CHECK(collide(b.geom.get(), tf2, mesh.get(), tf1, req, sw) == n);
+            for (std::size_t i = 0; i < n && i < sw.numContacts(); ++i) {
+              const Contact<double>& c = sw.getContact(i);
+              CHECK(c.o1 == mesh.get() && c.b1 == prims[i]);
+              if (geometry) CHECK(c.penetration_depth == oc[i].depth);
+            }
           }
         }
       }
