@@ -428,6 +428,16 @@ def bench_meshshape(args):
     run(True)
     ctx.sync(s)
     w_bv, w_leaf = (int(x) for x in stats.sum(0).tolist())
+    tuned = None
+    if args.autotune:
+        from fcl_amd import autotune
+
+        def checked():
+            run()
+            ctx.sync(s)
+            return counts.cpu().numpy().tobytes()
+
+        tuned, _, _ = autotune.tune(ctx, checked, autotune.MESHSHAPE_SPACE, log=log)
     for _ in range(args.warmup):
         run()
     ctx.sync(s)
@@ -507,7 +517,8 @@ def bench_meshshape(args):
         "data": "synthetic (seeded perturbed icosphere, primitives at |t| in [0.8, 1.2])",
         "config": {"workload": "mesh-shape collide: icosphere of 20,480 tris (OBBRSS) vs "
                                "box/sphere/capsule/cylinder/convex links, num_max_contacts 1",
-                   "queries": n, "kinds": args.ms_kinds, "engine_params": args.engine_param},
+                   "queries": n, "kinds": args.ms_kinds, "engine_params": args.engine_param,
+                   "autotuned": tuned},
         "colliding": int((c > 0).sum()),
         "roofline": roofline,
         "cpu_baseline": cpu,

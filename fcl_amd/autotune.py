@@ -30,6 +30,10 @@ DIST_SPACE = (
     ("dist_leaf_batch", (8, 16, 32, 64)),
 )
 
+MESHSHAPE_SPACE = (
+    ("meshshape_waves_per_simd", (1, 2, 3)),
+)
+
 BOOL_SPACE = (
     ("bool_queries_per_wave", (0, 4, 8, 16)),
     ("bool_width", (16, 32, 64)),
