@@ -454,6 +454,27 @@ def bench_meshshape(args):
     value = n / (elapsed / args.steps)
     t_k = gpu_ms * 1e-3
     c = counts.cpu().numpy()
+    # the same batch with contact geometry (enable_contact, up to 4 contacts
+    # per query): EPA after GJK at every hit, timed once after a warm-up
+    cap_c = 4
+    geo = (torch.empty((n * cap_c, 3), dtype=torch.float64, device=dev),
+           torch.empty((n * cap_c, 3), dtype=torch.float64, device=dev),
+           torch.empty(n * cap_c, dtype=torch.float64, device=dev))
+    counts_c = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    def run_contact():
+        ctx.mesh_shape_collide_batch_async(d_qm, d_rec, d_pool, d_qs, d1, d2, cap_c, counts_c,
+                                           contact=geo, stream=s)
+
+    run_contact()
+    ctx.sync(s)
+    ev[0].record(stream)
+    run_contact()
+    ev[1].record(stream)
+    ctx.sync(s)
+    contact_ms = ev[0].elapsed_time(ev[1])
+    log("with contact geometry (cap %d): %.2f ms, %d contacts" %
+        (cap_c, contact_ms, int(counts_c.sum().item())))
     # FP64 work: an OBB test per BV test, and per leaf test one GJK loop
     # pass as a floor (a leaf runs one to a few tens of passes)
     ops = w_bv * OPS_BV + w_leaf * OPS_GJK_PASS
@@ -520,6 +541,8 @@ def bench_meshshape(args):
                    "queries": n, "kinds": args.ms_kinds, "engine_params": args.engine_param,
                    "autotuned": tuned},
         "colliding": int((c > 0).sum()),
+        "contact_mode": {"num_max_contacts": cap_c, "ms": contact_ms,
+                         "queries_per_s": n / (contact_ms * 1e-3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
