@@ -32,6 +32,7 @@ DIST_SPACE = (
 
 MESHSHAPE_SPACE = (
     ("meshshape_waves_per_simd", (1, 2, 3)),
+    ("meshshape_leaf_share", (0, 25, 50, 75, 100)),
 )
 
 BOOL_SPACE = (

@@ -586,6 +586,7 @@ int fclx_mesh_shape_collide_batch(fclx_context* ctx, const fclx_mesh_shape_colli
   {
     std::lock_guard<std::mutex> lock(ctx->ws_mu);
     waves = engine_tuning(ctx->ws).meshshape_waves_per_simd;
+    L.leaf_share = engine_tuning(ctx->ws).meshshape_leaf_share;
   }
   hipError_t e = launch_mesh_shape_collide(L, waves, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(ctx, e, "mesh-shape collide launch");
@@ -752,6 +753,7 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   else if (n == "dist_queries_per_wave") *value = t.dist.queries_per_wave;
   else if (n == "dist_waves_per_simd") *value = t.dist.waves_per_simd;
   else if (n == "meshshape_waves_per_simd") *value = t.meshshape_waves_per_simd;
+  else if (n == "meshshape_leaf_share") *value = t.meshshape_leaf_share;
   else if (n == "dist_width") *value = t.dist.width;
   else if (n == "dist_leaf_batch") *value = t.dist.leaf_batch;
   else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
@@ -799,6 +801,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.dist.waves_per_simd = static_cast<int>(value);
   } else if (n == "meshshape_waves_per_simd" && in(1, 3)) {
     t.meshshape_waves_per_simd = static_cast<int>(value);
+  } else if (n == "meshshape_leaf_share" && in(0, 100)) {
+    t.meshshape_leaf_share = static_cast<int>(value);
   } else if (n == "dist_width" && in(1, 64)) {
     t.dist.width = static_cast<int>(value);
   } else if (n == "dist_leaf_batch" && in(1, 64)) {

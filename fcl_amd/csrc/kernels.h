@@ -102,6 +102,8 @@ struct MeshShapeLaunch {
   double* contact_depth;        // 1 per slot
   long long* stats;             // optional, 2 per query: BV tests, leaf tests
   int32_t* status;
+  int32_t leaf_share;           // percent of the live lanes waiting at leaves that starts a
+                                // leaf round (1..100; 0: 50, or 100 with contact)
 };
 // waves: register budget, 1..3 waves per SIMD
 hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s);

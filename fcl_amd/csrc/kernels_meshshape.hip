@@ -526,8 +526,8 @@ __device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, c
 
 // Lanes of a wavefront step together: a lane whose descent reached a leaf
 // that passed its BV test waits there, the others keep descending, and the
-// waiting lanes test their leaves together once they are at least half of
-// the live lanes (or nobody can descend).  The costly leaf test (GJK) then
+// waiting lanes test their leaves together once they are a set share of the
+// live lanes (or nobody can descend).  The costly leaf test (GJK) then
 // runs once for many lanes instead of once per descent step.  Each lane's
 // own order of tests is the reference's, so counts, triangles and test
 // statistics are unchanged.
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
     const unsigned long long mw = __ballot(walk);
     const unsigned long long ml = __ballot(live && pend >= 0);
     if (!mw && !ml) break;
-    if (ml && (!mw || 2 * __popcll(ml) >= __popcll(mw | ml))) {
+    if (ml && (!mw || 100 * __popcll(ml) >= L.leaf_share * __popcll(mw | ml))) {
       if (pend < 0) continue;
       const int prim = pend;
       pend = -1;
@@ -673,8 +673,11 @@ void launch_variant(int waves, unsigned grid, hipStream_t s, const MeshShapeLaun
 // Batches at least this large are grouped by shape kind first.
 constexpr int kSortMin = 4096;
 
-hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s) {
-  if (L.num_queries <= 0) return hipSuccess;
+hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipStream_t s) {
+  if (in.num_queries <= 0) return hipSuccess;
+  MeshShapeLaunch L = in;
+  // measured: half the live lanes for booleans, all of them before EPA
+  if (L.leaf_share <= 0) L.leaf_share = L.contact ? 100 : 50;
   const long long blocks = (static_cast<long long>(L.num_queries) + kBlk - 1) / kBlk;
   // the order of queries grouped by shape kind (stream-ordered memory)
   int32_t* order = nullptr;
