@@ -551,7 +551,10 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
   double sa[9], sto[3], sext[3];
   const ObbNode64* nodes = nullptr;
   const double* tris = nullptr;
-  int stk[kMsStack];
+  // pending right children in LDS, one column per lane (lane-interleaved:
+  // conflict-free whatever depth each lane is at)
+  __shared__ int stk[kMsStack][kBlk];
+  const int col = threadIdx.x;
   int sp = 0, b = 0, count = 0, pend = -1;
   long long nbv = 0, nleaf = 0;
   bool live = false;  // a query in progress
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
     if (sp == 0)
       finish();
     else
-      b = stk[--sp];
+      b = stk[--sp][col];
   };
   start();
   while (true) {
@@ -642,7 +645,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
       if (obb_disjoint64(B, Tr, sext, nd.extent)) {
         next();
       } else if (nd.first_child >= 0) {
-        stk[sp++] = nd.first_child + 1;
+        stk[sp++][col] = nd.first_child + 1;
         b = nd.first_child;
       } else {
         pend = -(nd.first_child + 1);  // tested in the next leaf round
