@@ -127,8 +127,34 @@ def bvh_build(V, T, bv_type=OBBRSS):
 
 
 # ------------------------------------------------------------------- context
-def _dptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+_DTYPES = {"i32": "int32", "i64": "int64", "f64": "float64", "u8": "uint8"}
+
+
+def _req(t, name, dtype, min_numel, optional=False):
+    """Checks a tensor handed to a batched entry point: contiguous, of the
+    given dtype, holding at least min_numel elements, so that the device code
+    never reads or writes outside it.  Returns its address (None for an
+    absent optional tensor).  _on_device checks the device afterwards."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError("%s is required" % name)
+    want = _DTYPES[dtype]
+    if str(t.dtype) != "torch." + want:
+        raise ValueError("%s must be %s, got %s" % (name, want, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if t.numel() < min_numel:
+        raise ValueError("%s holds %d elements, at least %d are needed"
+                         % (name, t.numel(), min_numel))
+    return t.data_ptr()
+
+
+def _on_device(*tensors):
+    """Every tensor given (None skipped) must live in device memory."""
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise ValueError("batched calls take device tensors, got one on %s" % t.device)
 
 
 class Context:
@@ -249,38 +275,46 @@ class Context:
         """contact: None, or (pos (K,3), normal (K,3), depth (K,)) float64 device
         tensors for enable_contact (a record per contact)."""
         a = N.CollideBatchArgs()
-        a.num_queries = int(query_meshes.shape[0])
-        a.query_meshes = query_meshes.data_ptr()
-        a.tf1 = tf1.data_ptr()
-        a.tf2 = tf2.data_ptr()
+        n = int(query_meshes.shape[0])
+        a.num_queries = n
+        a.query_meshes = _req(query_meshes, "query_meshes", "i32", 2 * n)
+        a.tf1 = _req(tf1, "tf1", "f64", 12 * n)
+        a.tf2 = _req(tf2, "tf2", "f64", 12 * n)
         a.num_max_contacts = int(num_max_contacts)
         a.engine = int(engine)
-        a.counts = counts.data_ptr()
-        a.pairs = pairs.data_ptr() if pairs is not None else None
-        a.pair_capacity = int(pairs.shape[0]) if pairs is not None else 0
-        a.num_pairs = num_pairs.data_ptr()
-        a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
-        a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
+        a.counts = _req(counts, "counts", "i32", n)
+        a.pairs = _req(pairs, "pairs", "i32", 0, optional=True)
+        a.pair_capacity = int(pairs.numel() // 3) if pairs is not None else 0
+        a.num_pairs = _req(num_pairs, "num_pairs", "i64", 1)
+        a.bv_tests = _req(bv_tests, "bv_tests", "i64", n, optional=True)
+        a.leaf_tests = _req(leaf_tests, "leaf_tests", "i64", n, optional=True)
         if contact is not None:
             a.enable_contact = 1
-            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
+            cap = a.pair_capacity
+            a.contact_pos = _req(contact[0], "contact pos", "f64", 3 * cap)
+            a.contact_normal = _req(contact[1], "contact normal", "f64", 3 * cap)
+            a.contact_depth = _req(contact[2], "contact depth", "f64", cap)
+        _on_device(query_meshes, tf1, tf2, counts, pairs, num_pairs, bv_tests, leaf_tests,
+                   *(contact or ()))
         self._check(self._L.fclx_collide_batch(self._h, ctypes.byref(a), stream))
 
     def distance_batch_async(self, query_meshes, tf1, tf2, distance, points=None, ids=None,
                              engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,
                              stream=None, nearest=True):
         a = N.DistanceBatchArgs()
-        a.num_queries = int(query_meshes.shape[0])
-        a.query_meshes = query_meshes.data_ptr()
-        a.tf1 = tf1.data_ptr()
-        a.tf2 = tf2.data_ptr()
+        n = int(query_meshes.shape[0])
+        a.num_queries = n
+        a.query_meshes = _req(query_meshes, "query_meshes", "i32", 2 * n)
+        a.tf1 = _req(tf1, "tf1", "f64", 12 * n)
+        a.tf2 = _req(tf2, "tf2", "f64", 12 * n)
         a.enable_nearest_points = 1 if nearest else 0
         a.engine = int(engine)
-        a.distance = distance.data_ptr()
-        a.points = points.data_ptr() if points is not None else None
-        a.ids = ids.data_ptr() if ids is not None else None
-        a.bv_tests = bv_tests.data_ptr() if bv_tests is not None else None
-        a.leaf_tests = leaf_tests.data_ptr() if leaf_tests is not None else None
+        a.distance = _req(distance, "distance", "f64", n)
+        a.points = _req(points, "points", "f64", 6 * n, optional=True)
+        a.ids = _req(ids, "ids", "i32", 2 * n, optional=True)
+        a.bv_tests = _req(bv_tests, "bv_tests", "i64", n, optional=True)
+        a.leaf_tests = _req(leaf_tests, "leaf_tests", "i64", n, optional=True)
+        _on_device(query_meshes, tf1, tf2, distance, points, ids, bv_tests, leaf_tests)
         self._check(self._L.fclx_distance_batch(self._h, ctypes.byref(a), stream))
 
     def shape_collide_batch_async(self, shapes, vertices, pair_shapes, tf1, tf2,
@@ -292,21 +326,27 @@ class Context:
         (n*cap,3), (n*cap,3), (n*cap,) for enable_contact; passes: None or an
         (n,2) int32 device tensor for the GJK / EPA loop passes per pair."""
         a = N.ShapeCollideBatchArgs()
-        a.num_pairs = int(pair_shapes.shape[0])
+        n = int(pair_shapes.shape[0])
+        cap = int(num_max_contacts)
+        a.num_pairs = n
         a.num_shapes = int(shapes.numel() // SHAPE_DTYPE.itemsize)
-        a.shapes = shapes.data_ptr()
-        a.vertices = vertices.data_ptr() if vertices is not None and vertices.numel() else None
-        a.num_vertices = int(vertices.shape[0]) if a.vertices else 0
-        a.pair_shapes = pair_shapes.data_ptr()
-        a.tf1 = tf1.data_ptr()
-        a.tf2 = tf2.data_ptr()
-        a.num_max_contacts = int(num_max_contacts)
+        a.shapes = _req(shapes, "shapes", "u8", 0)
+        has_v = vertices is not None and vertices.numel() > 0
+        a.vertices = _req(vertices, "vertices", "f64", 0) if has_v else None
+        a.num_vertices = int(vertices.numel() // 3) if has_v else 0
+        a.pair_shapes = _req(pair_shapes, "pair_shapes", "i32", 2 * n)
+        a.tf1 = _req(tf1, "tf1", "f64", 12 * n)
+        a.tf2 = _req(tf2, "tf2", "f64", 12 * n)
+        a.num_max_contacts = cap
         a.gjk_tolerance = float(tol)
-        a.counts = counts.data_ptr()
+        a.counts = _req(counts, "counts", "i32", n)
         if contact is not None:
             a.enable_contact = 1
-            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
-        a.passes = passes.data_ptr() if passes is not None else None
+            a.contact_pos = _req(contact[0], "contact pos", "f64", 3 * n * cap)
+            a.contact_normal = _req(contact[1], "contact normal", "f64", 3 * n * cap)
+            a.contact_depth = _req(contact[2], "contact depth", "f64", n * cap)
+        a.passes = _req(passes, "passes", "i32", 2 * n, optional=True)
+        _on_device(shapes, vertices, pair_shapes, tf1, tf2, counts, passes, *(contact or ()))
         self._check(self._L.fclx_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
     def mesh_shape_collide_batch_async(self, query_meshes, shapes, vertices, query_shapes, tf1,
@@ -321,23 +361,30 @@ class Context:
         tensors for enable_contact; stats: None or (n,2) int64 device tensor
         (BV tests, leaf tests)."""
         a = N.MeshShapeCollideBatchArgs()
-        a.num_queries = int(query_meshes.shape[0])
-        a.query_meshes = query_meshes.data_ptr()
-        a.query_shapes = query_shapes.data_ptr()
+        n = int(query_meshes.shape[0])
+        cap = int(num_max_contacts)
+        a.num_queries = n
+        a.query_meshes = _req(query_meshes, "query_meshes", "i32", n)
+        a.query_shapes = _req(query_shapes, "query_shapes", "i32", n)
         a.num_shapes = int(shapes.numel() // SHAPE_DTYPE.itemsize)
-        a.shapes = shapes.data_ptr()
-        a.vertices = vertices.data_ptr() if vertices is not None and vertices.numel() else None
-        a.num_vertices = int(vertices.shape[0]) if a.vertices else 0
-        a.tf1 = tf1.data_ptr()
-        a.tf2 = tf2.data_ptr()
-        a.num_max_contacts = int(num_max_contacts)
+        a.shapes = _req(shapes, "shapes", "u8", 0)
+        has_v = vertices is not None and vertices.numel() > 0
+        a.vertices = _req(vertices, "vertices", "f64", 0) if has_v else None
+        a.num_vertices = int(vertices.numel() // 3) if has_v else 0
+        a.tf1 = _req(tf1, "tf1", "f64", 12 * n)
+        a.tf2 = _req(tf2, "tf2", "f64", 12 * n)
+        a.num_max_contacts = cap
         a.gjk_tolerance = float(tol)
-        a.counts = counts.data_ptr()
-        a.primitives = primitives.data_ptr() if primitives is not None else None
+        a.counts = _req(counts, "counts", "i32", n)
+        a.primitives = _req(primitives, "primitives", "i32", n * cap, optional=True)
         if contact is not None:
             a.enable_contact = 1
-            a.contact_pos, a.contact_normal, a.contact_depth = (t.data_ptr() for t in contact)
-        a.stats = stats.data_ptr() if stats is not None else None
+            a.contact_pos = _req(contact[0], "contact pos", "f64", 3 * n * cap)
+            a.contact_normal = _req(contact[1], "contact normal", "f64", 3 * n * cap)
+            a.contact_depth = _req(contact[2], "contact depth", "f64", n * cap)
+        a.stats = _req(stats, "stats", "i64", 2 * n, optional=True)
+        _on_device(query_meshes, shapes, vertices, query_shapes, tf1, tf2, counts, primitives,
+                   stats, *(contact or ()))
         self._check(self._L.fclx_mesh_shape_collide_batch(self._h, ctypes.byref(a), stream))
 
     def shape_collide(self, s1, v1, tf1, s2, v2, tf2, cap=1, contact=True, tol=1e-6):

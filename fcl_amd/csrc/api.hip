@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -76,6 +77,7 @@ struct fclx_context {
   size_t s_shape_bytes = 0;
   int epa_blocks_per_cu = 0;
   int shape_waves = 2;
+  std::atomic<int> serve_fail_launches{0};  // failure injection for the serving loop
 };
 
 namespace {
@@ -146,6 +148,8 @@ size_t max_nodes(const fclx_context* ctx) {
 
 // The FAST engine packs (query, node, node) into 64 bits: 21 bits per node.
 constexpr size_t kFastMaxNodes = size_t(1) << 21;
+// ... and 22 bits per query.
+constexpr int32_t kFastMaxQueries = int32_t(1) << 22;
 
 // Largest L1 norm of an OBB centre over the mesh's nodes; the FP32 cull's
 // translation error term scales with it.
@@ -432,8 +436,16 @@ int fclx_bvh_build(int32_t bv_type, const double* vertices, int32_t num_vertices
   return FCLX_OK;
 }
 
-int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream) {
-  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace fclx {
+
+// fclx_collide_batch with the device status word given by the caller (the
+// serving loop keeps one per stream slot, so that a batch's flags are its
+// own).
+int collide_batch_status(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream,
+                         int32_t* d_status) {
+  if (!ctx || !a || !d_status) return FCLX_ERR_INVALID_ARGUMENT;
   if (a->num_queries < 0 || a->num_max_contacts < 1 || !a->counts || !a->num_pairs ||
       (a->num_queries > 0 && (!a->query_meshes || !a->tf1 || !a->tf2)) ||
       (a->pair_capacity > 0 && !a->pairs) ||
@@ -469,7 +481,7 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
   L.num_pairs = reinterpret_cast<long long*>(a->num_pairs);
   L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
   L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
-  L.status = ctx->d_status;
+  L.status = d_status;
   hipError_t e;
   if (a->enable_contact) {
     // contact geometry follows the reference's order and per-contact cap
@@ -481,7 +493,7 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     L.contact_depth = a->contact_depth;
     e = launch_collide_plain(L, 2 * depth, s);
   } else if (a->engine == FCLX_ENGINE_BOOL) {
-    if (nodes >= kFastMaxNodes || a->num_queries >= (1 << 22)) {
+    if (nodes >= kFastMaxNodes || a->num_queries >= kFastMaxQueries) {
       L.cap = 1;  // min(count, 1)
       e = launch_collide_plain(L, 2 * depth, s);
     } else {
@@ -490,8 +502,9 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
     }
   } else if (a->engine == FCLX_ENGINE_PLAIN ||
              (a->engine == FCLX_ENGINE_FAST && a->num_max_contacts > kOrderedCapMax &&
-              nodes >= kFastMaxNodes)) {
-    // meshes beyond the FAST engine's node-index width take the plain kernel
+              (nodes >= kFastMaxNodes || a->num_queries >= kFastMaxQueries))) {
+    // meshes and batches beyond the FAST engine's task encoding (node and
+    // query index widths) take the plain kernel
     e = launch_collide_plain(L, 2 * depth, s);
   } else if (a->engine == FCLX_ENGINE_FAST) {
     std::lock_guard<std::mutex> lock(ctx->ws_mu);
@@ -501,6 +514,31 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "collide launch");
   return FCLX_OK;
+}
+
+// The FCLX_* code of a set of device status flags (0 when none is set).
+int status_code(int32_t flags) {
+  if (flags & (kStatusStackOverflow | kStatusQueueOverflow)) return FCLX_ERR_STACK_OVERFLOW;
+  if (flags & kStatusBadQuery) return FCLX_ERR_INVALID_ARGUMENT;
+  if (flags & kStatusPairOverflow) return FCLX_ERR_CAPACITY;
+  return FCLX_OK;
+}
+
+// Server batch launches still to fail on purpose (see "serve_fail_launches").
+bool take_injected_failure(fclx_context* ctx) {
+  int n = ctx->serve_fail_launches.load();
+  while (n > 0)
+    if (ctx->serve_fail_launches.compare_exchange_weak(n, n - 1)) return true;
+  return false;
+}
+
+}  // namespace fclx
+
+extern "C" {
+
+int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  return fclx::collide_batch_status(ctx, a, stream, ctx->d_status);
 }
 
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, void* stream) {
@@ -561,6 +599,7 @@ int fclx_mesh_shape_collide_batch(fclx_context* ctx, const fclx_mesh_shape_colli
     std::lock_guard<std::mutex> lock(ctx->mu);
     L.meshes = ctx->d_table;
     L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+    L.max_depth = max_depth(ctx);
   }
   (void)hipSetDevice(ctx->device);
   L.num_queries = a->num_queries;
@@ -767,6 +806,8 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   } else if (n == "shape_waves_per_simd") {
     std::lock_guard<std::mutex> sl(ctx->sws_mu);
     *value = ctx->shape_waves;
+  } else if (n == "serve_fail_launches") {
+    *value = ctx->serve_fail_launches.load();
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter");
   }
@@ -824,6 +865,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
   } else if (n == "shape_waves_per_simd" && in(2, 4)) {
     std::lock_guard<std::mutex> sl(ctx->sws_mu);
     ctx->shape_waves = static_cast<int>(value);
+  } else if (n == "serve_fail_launches" && in(0, 1 << 20)) {
+    ctx->serve_fail_launches.store(static_cast<int>(value));
   } else {
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine parameter or value out of range");
   }

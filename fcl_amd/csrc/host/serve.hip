@@ -29,6 +29,10 @@
 
 namespace fclx {
 int context_device(const fclx_context* ctx);
+int collide_batch_status(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream,
+                         int32_t* d_status);
+int status_code(int32_t flags);
+bool take_injected_failure(fclx_context* ctx);
 }
 
 namespace {
@@ -43,6 +47,8 @@ constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
 constexpr auto kAgeLimit = std::chrono::milliseconds(4);  // then a request goes first
 constexpr int32_t kBatchDefault = 1 << 16;                 // queries per batch at most
 constexpr int64_t kSlotMin = 1 << 18;  // queries a slot holds without growing
+
+__global__ void clear_status(int32_t* status) { *status = 0; }
 
 __global__ void fill_pose(double* out, int n, fclx_server_config cfg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,9 +72,11 @@ struct Slot {
   double* d_tf2 = nullptr;
   int32_t* d_counts = nullptr;
   int64_t* d_num = nullptr;
+  int32_t* d_status = nullptr;  // the batch's device status flags
   int32_t* h_qm = nullptr;
   double* h_tf2 = nullptr;
   int32_t* h_counts = nullptr;
+  int32_t* h_status = nullptr;
   bool busy = false;
   int status = FCLX_OK;
   std::vector<Pending> batch;
@@ -83,10 +91,12 @@ void free_slot_buffers(Slot& s) {
   (void)hipFree(s.d_tf2);
   (void)hipFree(s.d_counts);
   (void)hipFree(s.d_num);
+  (void)hipFree(s.d_status);
   if (s.h_qm) (void)hipHostFree(s.h_qm);
   if (s.h_tf2) (void)hipHostFree(s.h_tf2);
   if (s.h_counts) (void)hipHostFree(s.h_counts);
-  s.d_qm = s.h_qm = s.d_counts = s.h_counts = nullptr;
+  if (s.h_status) (void)hipHostFree(s.h_status);
+  s.d_qm = s.h_qm = s.d_counts = s.h_counts = s.d_status = s.h_status = nullptr;
   s.d_tf1 = s.d_tf2 = s.h_tf2 = nullptr;
   s.d_num = nullptr;
   s.cap = 0;
@@ -124,6 +134,8 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
       (e = hipMalloc(&s.d_tf2, 96 * cap)) != hipSuccess ||
       (e = hipMalloc(&s.d_counts, 4 * cap)) != hipSuccess ||
       (e = hipMalloc(&s.d_num, 8)) != hipSuccess ||
+      (e = hipMalloc(&s.d_status, 4)) != hipSuccess ||
+      (e = hipHostMalloc(&s.h_status, 4)) != hipSuccess ||
       (e = hipHostMalloc(&s.h_qm, 8 * cap)) != hipSuccess ||
       (e = hipHostMalloc(&s.h_tf2, 96 * cap)) != hipSuccess ||
       (e = hipHostMalloc(&s.h_counts, 4 * cap)) != hipSuccess) {
@@ -152,7 +164,10 @@ int slot_launch(fclx_server* S, Slot& s) {
     k += p.nq;
   }
   s.queries = k;
-  if (hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+  if (fclx::take_injected_failure(S->ctx)) return FCLX_ERR_DEVICE;
+  clear_status<<<1, 1, 0, s.stream>>>(s.d_status);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
       hipMemcpyAsync(s.d_qm, s.h_qm, 8 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess)
     return FCLX_ERR_DEVICE;
   fclx_collide_batch_args a{};
@@ -166,9 +181,11 @@ int slot_launch(fclx_server* S, Slot& s) {
   a.pairs = nullptr;
   a.pair_capacity = 0;
   a.num_pairs = s.d_num;
-  int rc = fclx_collide_batch(S->ctx, &a, s.stream);
+  int rc = fclx::collide_batch_status(S->ctx, &a, s.stream, s.d_status);
   if (rc != FCLX_OK) return rc;
   if (hipMemcpyAsync(s.h_counts, s.d_counts, 4 * k, hipMemcpyDeviceToHost, s.stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(s.h_status, s.d_status, 4, hipMemcpyDeviceToHost, s.stream) !=
           hipSuccess ||
       hipEventRecord(s.done, s.stream) != hipSuccess)
     return FCLX_ERR_DEVICE;
@@ -203,7 +220,19 @@ void worker_main(fclx_server* S) {
     for (Slot& s : S->slots) {
       if (!s.busy) continue;
       lock.unlock();
-      const bool ready = s.status != FCLX_OK || hipEventQuery(s.done) == hipSuccess;
+      // any answer but "not ready" ends the batch: after a device error the
+      // query keeps returning that error, and the slot must not wait for ever
+      bool ready = s.status != FCLX_OK;
+      if (!ready) {
+        const hipError_t q = hipEventQuery(s.done);
+        ready = q != hipErrorNotReady;
+        if (q != hipSuccess && ready) {
+          (void)hipGetLastError();
+          s.status = FCLX_ERR_DEVICE;
+        } else if (ready) {
+          s.status = fclx::status_code(*s.h_status);
+        }
+      }
       if (ready) {
         finished.clear();
         slot_retire(S, s, finished);

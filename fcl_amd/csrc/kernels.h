@@ -104,6 +104,7 @@ struct MeshShapeLaunch {
   int32_t* status;
   int32_t leaf_share;           // percent of the live lanes waiting at leaves that starts a
                                 // leaf round (1..100; 0: 50, or 100 with contact)
+  int32_t max_depth;            // deepest mesh of the context (levels)
 };
 // waves: register budget, 1..3 waves per SIMD
 hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStream_t s);

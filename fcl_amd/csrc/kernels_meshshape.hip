@@ -32,7 +32,10 @@ namespace fclx {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kMsStack = 64;  // pending right children: the mesh depth bound
+constexpr int kMsStack = 64;  // pending right children in LDS: meshes up to this depth
+// deeper meshes (up to kMsDeepMax levels) keep the stack in device memory
+constexpr int kMsDeepMax = 4096;
+constexpr int kMsDeepBlocksPerCu = 2;
 constexpr int kEpaBlocksPerCu = 2;
 
 using shp::D3;
@@ -401,13 +404,15 @@ __device__ bool sphere_triangle(double radius, const D3& center, const D3& P1, c
   return true;
 }
 
-// A usable query: a mesh of OBBRSS nodes no deeper than the stack, and a
-// shape of a known type (a convex with at least 4 vertices inside the pool).
-__device__ __forceinline__ bool load_query(const MeshShapeLaunch& L, int q, shp::Shape& s) {
+// A usable query: a mesh of OBBRSS nodes no deeper than the stack (depth_max
+// levels), and a shape of a known type (a convex with at least 4 vertices
+// inside the pool).
+__device__ __forceinline__ bool load_query(const MeshShapeLaunch& L, int q, shp::Shape& s,
+                                           int depth_max) {
   const int mi = L.query_meshes[q], si = L.query_shapes[q];
   if (mi < 0 || mi >= L.num_meshes || si < 0 || si >= L.num_shapes) return false;
   const MeshDev& M = L.meshes[mi];
-  if (M.num_bvs <= 0 || M.bv_type != 0 || M.depth > kMsStack) return false;
+  if (M.num_bvs <= 0 || M.bv_type != 0 || M.depth > depth_max) return false;
   const fclx_shape& r = L.shapes[si];
   s.type = r.type;
   s.nv = r.num_vertices;
@@ -531,11 +536,16 @@ __device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, c
 // runs once for many lanes instead of once per descent step.  Each lane's
 // own order of tests is the reference's, so counts, triangles and test
 // statistics are unchanged.
-template <bool kContact, int kWaves>
+//
+// kDeep: the pending right children live in device memory (deep_stack, one
+// column of L.max_depth entries per lane, lane-interleaved) instead of LDS,
+// for contexts holding a mesh deeper than kMsStack levels.
+template <bool kContact, int kWaves, bool kDeep>
 __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunch L,
                                                                   const int32_t* order,
                                                                   shp::EpaLane* lanes,
-                                                                  shp::EpaWave* waves) {
+                                                                  shp::EpaWave* waves,
+                                                                  int* deep_stack) {
   const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * kBlk;
   shp::Epa E;
@@ -552,17 +562,33 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
   const ObbNode64* nodes = nullptr;
   const double* tris = nullptr;
   // pending right children in LDS, one column per lane (lane-interleaved:
-  // conflict-free whatever depth each lane is at)
-  __shared__ int stk[kMsStack][kBlk];
+  // conflict-free whatever depth each lane is at); a deep stack holds at
+  // most L.max_depth - 1 entries
+  __shared__ int stk[kDeep ? 1 : kMsStack][kBlk];
   const int col = threadIdx.x;
+  const int depth_max = kDeep ? L.max_depth : kMsStack;
   int sp = 0, b = 0, count = 0, pend = -1;
   long long nbv = 0, nleaf = 0;
   bool live = false;  // a query in progress
+  auto push = [&](int v) {
+    if constexpr (kDeep)
+      deep_stack[static_cast<size_t>(sp) * stride + gid] = v;
+    else
+      stk[sp][col] = v;
+    ++sp;
+  };
+  auto pop = [&]() -> int {
+    --sp;
+    if constexpr (kDeep)
+      return deep_stack[static_cast<size_t>(sp) * stride + gid];
+    else
+      return stk[sp][col];
+  };
   // take queries until one is usable (the others are answered at once)
   auto start = [&]() {
     while (qq < static_cast<size_t>(L.num_queries)) {
       q = order ? static_cast<size_t>(order[qq]) : qq;
-      if (load_query(L, static_cast<int>(q), s)) {
+      if (load_query(L, static_cast<int>(q), s, depth_max)) {
         const MeshDev& M = L.meshes[L.query_meshes[q]];
         nodes = M.obb64;
         tris = M.tri_verts;
@@ -601,7 +627,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
     if (sp == 0)
       finish();
     else
-      b = stk[--sp][col];
+      b = pop();
   };
   start();
   while (true) {
@@ -645,7 +671,7 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
       if (obb_disjoint64(B, Tr, sext, nd.extent)) {
         next();
       } else if (nd.first_child >= 0) {
-        stk[sp++][col] = nd.first_child + 1;
+        push(nd.first_child + 1);
         b = nd.first_child;
       } else {
         pend = -(nd.first_child + 1);  // tested in the next leaf round
@@ -660,16 +686,21 @@ size_t mesh_shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::E
 
 template <bool kContact>
 void launch_variant(int waves, unsigned grid, hipStream_t s, const MeshShapeLaunch& L,
-                    const int32_t* order, shp::EpaLane* lanes, shp::EpaWave* wv) {
+                    const int32_t* order, shp::EpaLane* lanes, shp::EpaWave* wv,
+                    int* deep_stack) {
+  if (deep_stack) {
+    meshshape_collide<kContact, 1, true><<<grid, kBlk, 0, s>>>(L, order, lanes, wv, deep_stack);
+    return;
+  }
   switch (waves) {
     case 1:
-      meshshape_collide<kContact, 1><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
+      meshshape_collide<kContact, 1, false><<<grid, kBlk, 0, s>>>(L, order, lanes, wv, nullptr);
       break;
     case 3:
-      meshshape_collide<kContact, 3><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
+      meshshape_collide<kContact, 3, false><<<grid, kBlk, 0, s>>>(L, order, lanes, wv, nullptr);
       break;
     default:
-      meshshape_collide<kContact, 2><<<grid, kBlk, 0, s>>>(L, order, lanes, wv);
+      meshshape_collide<kContact, 2, false><<<grid, kBlk, 0, s>>>(L, order, lanes, wv, nullptr);
   }
 }
 
@@ -694,12 +725,6 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipSt
     ms_count<<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, ctr);
     ms_place<<<static_cast<unsigned>(blocks), kBlk, 0, s>>>(L, ctr, order);
   }
-  if (!L.contact) {
-    launch_variant<false>(waves, static_cast<unsigned>(blocks), s, L, order, nullptr, nullptr);
-    e = hipGetLastError();
-    if (order) (void)hipFreeAsync(order, s);
-    return e;
-  }
   int dev = 0, cus = 0;
   e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -707,7 +732,31 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipSt
     if (order) (void)hipFreeAsync(order, s);
     return e;
   }
-  const long long grid = std::min<long long>(blocks, static_cast<long long>(cus) * kEpaBlocksPerCu);
+  // a context with a mesh deeper than the LDS stack: a persistent grid whose
+  // lanes keep their stacks in device memory (queries on deeper meshes
+  // than kMsDeepMax are refused by load_query)
+  const bool deep = L.max_depth > kMsStack;
+  if (deep) L.max_depth = std::min(L.max_depth, kMsDeepMax);
+  long long grid = blocks;
+  if (deep) grid = std::min<long long>(grid, static_cast<long long>(cus) * kMsDeepBlocksPerCu);
+  if (L.contact) grid = std::min<long long>(grid, static_cast<long long>(cus) * kEpaBlocksPerCu);
+  int* deep_stack = nullptr;
+  if (deep) {
+    e = hipMallocAsync(reinterpret_cast<void**>(&deep_stack),
+                       static_cast<size_t>(grid) * kBlk * L.max_depth * sizeof(int), s);
+    if (e != hipSuccess) {
+      if (order) (void)hipFreeAsync(order, s);
+      return e;
+    }
+  }
+  if (!L.contact) {
+    launch_variant<false>(waves, static_cast<unsigned>(grid), s, L, order, nullptr, nullptr,
+                          deep_stack);
+    e = hipGetLastError();
+    if (deep_stack) (void)hipFreeAsync(deep_stack, s);
+    if (order) (void)hipFreeAsync(order, s);
+    return e;
+  }
   const size_t nl = static_cast<size_t>(grid) * kBlk;
   shp::EpaLane* lanes = nullptr;
   shp::EpaWave* wv = nullptr;
@@ -717,13 +766,15 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipSt
     if (e != hipSuccess) (void)hipFreeAsync(lanes, s);
   }
   if (e != hipSuccess) {
+    if (deep_stack) (void)hipFreeAsync(deep_stack, s);
     if (order) (void)hipFreeAsync(order, s);
     return e;
   }
-  launch_variant<true>(waves, static_cast<unsigned>(grid), s, L, order, lanes, wv);
+  launch_variant<true>(waves, static_cast<unsigned>(grid), s, L, order, lanes, wv, deep_stack);
   e = hipGetLastError();
   (void)hipFreeAsync(lanes, s);
   (void)hipFreeAsync(wv, s);
+  if (deep_stack) (void)hipFreeAsync(deep_stack, s);
   if (order) (void)hipFreeAsync(order, s);
   return e;
 }

@@ -56,8 +56,9 @@ extern "C" {
  * gets the reference's depth-first prefix, in order: its records are ordered
  * by depth-first path keys and cut, or, when per-query statistics are
  * requested, the query is re-run by PLAIN.  Batches over a context holding
- * a mesh of 2^20 or more triangles run PLAIN for caps above 64 (the FAST
- * engine's task encoding has 21 bits per node index). */
+ * a mesh of 2^20 or more triangles, and batches of 2^22 or more queries, run
+ * PLAIN for caps above 64 (the FAST engine's task encoding has 21 bits per
+ * node index and 22 per query). */
 #define FCLX_ENGINE_PLAIN 0
 #define FCLX_ENGINE_FAST 1
 /* BOOL: only whether each query's pair collides -- counts[q] = 1 or 0, the
@@ -221,9 +222,12 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
  * contacts (b1 = triangle id, b2 = Contact::NONE) in the reference's order,
  * at most num_max_contacts, from meshShapeCollisionOrientedNodeLeafTesting
  * (mesh_shape_collision_traversal_node-inl.h:205-283).  Asynchronous; a
- * query naming a missing mesh or shape, an RSS mesh, a mesh deeper than 64
+ * query naming a missing mesh or shape, an RSS mesh, a mesh deeper than 4096
  * levels or a convex with fewer than 4 vertices gets count 0 and makes
- * fclx_sync return FCLX_ERR_INVALID_ARGUMENT.  With enable_contact the call
+ * fclx_sync return FCLX_ERR_INVALID_ARGUMENT.  While the context holds a
+ * mesh deeper than 64 levels, batches keep their descent stacks in
+ * stream-ordered device memory (4 bytes per level per resident lane) instead
+ * of LDS.  With enable_contact the call
  * takes stream-ordered device memory for EPA (about 12 KB per resident lane,
  * freed when the batch ends). */
 typedef struct {

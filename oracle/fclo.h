@@ -154,6 +154,10 @@ void fclo_collide_batch(const fclo_model *const *models, int num_queries,
                         const double *tf2s, int cap, int store, int *counts,
                         int *pairs, int64_t *bv_tests, int64_t *leaf_tests,
                         int threads);
+void fclo_collide_batch_flat(const fclo_model *const *models, int num_queries,
+                             const int *q_m1, const int *q_m2, const double *tf1s,
+                             const double *tf2s, int cap, const int64_t *offsets,
+                             int *counts, int *pairs, int threads);
 void fclo_distance_batch(const fclo_model *const *models, int num_queries,
                          const int *q_m1, const int *q_m2, const double *tf1s,
                          const double *tf2s, int enable_nearest_points,

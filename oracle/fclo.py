@@ -85,6 +85,8 @@ def lib():
             "fclo_distance_all_pairs": (c_dbl, [PM, P_dbl, PM, P_dbl, P_i32]),
             "fclo_collide_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl, P_dbl,
                                           c_i32, c_i32, P_i32, P_i32, P_i64, P_i64, c_i32]),
+            "fclo_collide_batch_flat": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
+                                               P_dbl, c_i32, P_i64, P_i32, P_i32, c_i32]),
             "fclo_distance_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
                                            P_dbl, c_i32, P_dbl, P_dbl, P_i32, P_i64, P_i64,
                                            c_i32]),
@@ -362,6 +364,27 @@ def collide_batch(meshes, qm, tf1, tf2, cap, store=0, threads=None):
                              _p(tf2, c_dbl), cap, store, _p(counts, c_i32), _p(pairs, c_i32),
                              _p(bvt, c_i64), _p(lft, c_i64), threads or os.cpu_count())
     return counts, pairs[:, :store], bvt, lft
+
+
+def collide_batch_flat(meshes, qm, tf1, tf2, cap, threads=None):
+    """Batch collide with every contact kept: a counting pass, then a pass
+    that writes each query's pairs (reference order) at its offset.
+    Returns counts (n,), offsets (n+1,) int64 and records (total, 2)."""
+    qm = np.ascontiguousarray(qm, np.int32)
+    n = qm.shape[0]
+    counts = collide_batch(meshes, qm, tf1, tf2, cap, store=0, threads=threads)[0]
+    offsets = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    recs = np.zeros((max(int(offsets[-1]), 1), 2), np.int32)
+    again = np.zeros(n, np.int32)
+    q1 = np.ascontiguousarray(qm[:, 0])
+    q2 = np.ascontiguousarray(qm[:, 1])
+    lib().fclo_collide_batch_flat(_model_array(meshes), n, _p(q1, c_i32), _p(q2, c_i32),
+                                  _p(_d(tf1), c_dbl), _p(_d(tf2), c_dbl), cap,
+                                  _p(offsets, c_i64), _p(again, c_i32), _p(recs, c_i32),
+                                  threads or os.cpu_count())
+    assert np.array_equal(again, counts)
+    return counts, offsets, recs[: offsets[-1]]
 
 
 def distance_batch(meshes, qm, tf1, tf2, nearest=True, threads=None):

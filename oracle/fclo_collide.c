@@ -420,6 +420,25 @@ void fclo_collide_batch(const fclo_model *const *models, int num_queries,
   }
 }
 
+/* fclo_collide_batch with every query's records in one flat array: query
+ * q's first min(count, offsets[q+1] - offsets[q]) pairs, in the reference's
+ * order, go to pairs + 2 * offsets[q].  Callers size the room per query
+ * from a counting pass (store = 0). */
+void fclo_collide_batch_flat(const fclo_model *const *models, int num_queries,
+                             const int *q_m1, const int *q_m2, const double *tf1s,
+                             const double *tf2s, int cap, const int64_t *offsets,
+                             int *counts, int *pairs, int threads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  for (int q = 0; q < num_queries; ++q) {
+    fclo_collide_stats st;
+    const int64_t room = offsets[q + 1] - offsets[q];
+    collide_store(models[q_m1[q]], tf1s + 12 * (size_t)q, models[q_m2[q]],
+                  tf2s + 12 * (size_t)q, cap, (int)room, pairs + 2 * offsets[q], 0, 0, 0,
+                  &st);
+    counts[q] = st.num_contacts;
+  }
+}
+
 /* All triangle pairs with no hierarchy (test of the traversal's culling):
  * writes every intersecting (id1, id2) in row-major order, returns the count
  * (pairs beyond store are counted but not written). */

@@ -148,3 +148,56 @@ def test_mesh_shape_edge_cases(ctx):
     assert rc == 0 and gc[0] == oc[0]
     ctx.mesh_destroy(mid)
     ctx.mesh_destroy(rss)
+
+
+def chain_mesh(n, s=0.05):
+    """Small triangles along the x axis whose positions grow so fast that
+    every mean split peels off the farthest one: a hierarchy n levels deep
+    (x_k = 2 ((k + 1) x_{k-1} - sum of the earlier x))."""
+    xs = [0.0, 1.0]
+    total = 1.0
+    while len(xs) < n:
+        x = 2.0 * ((len(xs) + 1) * xs[-1] - total)
+        xs.append(x)
+        total += x
+    V = np.array([[x + dx, dy, dz] for x in xs for dx, dy, dz in
+                  ((0, 0, 0), (0, s, 0), (0, 0, s))], np.float64)
+    T = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    return V, T, xs
+
+
+def test_mesh_shape_deep_hierarchy():
+    """A mesh 72 levels deep (a chain of triangles that the mean split
+    peels off one at a time): the descent stack leaves LDS for device
+    memory and every count, triangle, statistic and contact equals the
+    oracle's.  Own context: a deep mesh switches every mesh-shape batch of
+    its context to the device-memory stack."""
+    V, T, xs = chain_mesh(72)
+    depth = F.bvh_build(V, T)["depth"]
+    assert depth == 72, depth
+    c2 = F.Context(0)
+    try:
+        mid = c2.mesh(V, T)
+        assert c2.mesh_info(mid)["depth"] == depth
+        m = fclo.OracleMesh(V, T)
+        rec, pool = shape_table(KINDS, DIMS, [HULL])
+        n = 600
+        rng = np.random.default_rng(21)
+        qs = rng.integers(0, 5, n).astype(np.int32)
+        tf1 = np.tile(util.identity_pose(), (n, 1))
+        tf2 = util.random_transforms(n, [-0.3, -0.3, -0.3, 0.3, 0.3, 0.3], seed=22)
+        # each shape next to one of the chain's ten nearest triangles, the
+        # deepest leaves of the hierarchy
+        tf2[:, 9] += np.array(xs)[rng.integers(0, 10, n)]
+        # the FP64 box tests of the upper levels (extents near 1e125) let few
+        # queries through to the leaves, but those descend past 64 levels
+        _, _, _, os_ = fclo.mesh_shape_collide_batch([m], np.zeros(n, np.int32), rec, pool, qs,
+                                                     tf1, tf2, cap=1)
+        assert (os_[:, 0] >= 66).sum() >= 5  # 65 or more right children pending
+        for cap, contact in ((1, False), (8, False), (4, True)):
+            counts = check(c2, [m], [mid], np.zeros(n, np.int32), rec, pool, qs, tf1, tf2, cap,
+                           contact, "deep mesh")
+            assert (counts > 0).sum() >= 5
+        c2.mesh_destroy(mid)
+    finally:
+        c2.close()

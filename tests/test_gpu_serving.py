@@ -123,3 +123,28 @@ def test_rejects_bad_requests(ctx, setup):
         srv.close()
     with pytest.raises(F.FclxError):
         F.Server(ctx, 9999)
+
+
+def test_failed_launch_is_reported_and_destroy_returns(ctx, setup):
+    """A batch whose launch fails (forced from the host through the
+    "serve_fail_launches" parameter, before any device work) completes with
+    an error status instead of holding its slot; the loop goes on serving
+    the next requests exactly and destroy() returns."""
+    scene, link_ids, reqs, expect = setup
+    print("case: serving with one injected launch failure", flush=True)
+    srv = F.Server(ctx, scene, num_streams=2)
+    try:
+        ctx.set_engine_param("serve_fail_launches", 1)
+        srv.submit(link_ids, reqs[0], tag=1)
+        got = []
+        for _ in range(100):
+            got += srv.poll(8, wait_ms=50)
+            if got:
+                break
+        assert len(got) == 1 and got[0][0] == 1
+        assert got[0][1] == F.N.FCLX_ERR_DEVICE, "the failed batch must carry its error"
+        assert ctx.get_engine_param("serve_fail_launches") == 0
+        res, _, _ = srv.replay([(link_ids, P) for P in reqs[:4]], np.zeros(4))
+        _check(res, expect[:4])
+    finally:
+        srv.close()

@@ -1,0 +1,202 @@
+"""The benchmark workloads themselves, at their full mesh sizes, against the
+oracle on the GPU.
+
+- Batched uneven mesh collision: the 64 mesh pairs of the benchmark (500 to
+  200,000 triangles, icospheres and heightfields) with 10,000 queries split
+  over the pairs by the same Dirichlet draw.  Exhaustive contacts
+  (num_max_contacts = 100,000): per query the same count and the same
+  multiset of (b1, b2) pairs as the oracle; queries that reach the cap get
+  the reference's depth-first prefix, in order.  Capped at 70: the prefix
+  of every query that reaches the cap, in order, the full set otherwise.
+- Mesh distance between the two 20,480-triangle icospheres.
+- Serving: the 500,000-triangle scene against the 20-link robot for a few
+  trajectories of the request pool, booleans equal to the oracle's.
+Integer outputs are compared exactly, distances and points bit for bit.
+"""
+import concurrent.futures as cf
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import fcl_amd as F
+import fclo
+from fcl_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV, dtype)
+
+
+def _oracle_meshes(meshes, bv_type=fclo.OBBRSS):
+    with cf.ThreadPoolExecutor(THREADS) as ex:
+        return list(ex.map(lambda m: fclo.OracleMesh(m[0], m[1], bv_type), meshes))
+
+
+@pytest.fixture(scope="module")
+def uneven(ctx):
+    w = W.uneven_batch(n_queries=10000, n_pairs=64)
+    ids = np.array([ctx.mesh(V, T) for V, T in w["meshes"]], np.int32)
+    om = _oracle_meshes(w["meshes"])
+    sizes = [len(T) for _, T in w["meshes"]]
+    assert min(sizes) < 1000 and max(sizes) > 100000, sizes
+    yield w, ids, om
+    for m in ids:
+        ctx.mesh_destroy(int(m))
+
+
+def _gpu_records(ctx, w, ids, cap, engine, capacity):
+    qm = ids[w["query_meshes"]]
+    n = len(qm)
+    counts = torch.zeros(n, dtype=torch.int32, device=DEV)
+    pairs = torch.zeros((capacity, 3), dtype=torch.int32, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.collide_batch_async(_t(qm, torch.int32), _t(w["tf1"], torch.float64),
+                            _t(w["tf2"], torch.float64), cap, counts, pairs, num,
+                            engine=engine, stream=s)
+    rc = ctx.sync_status(s)
+    nrec = int(num.item())
+    rec = pairs[:min(nrec, capacity)].cpu().numpy()
+    return rc, counts.cpu().numpy(), rec, nrec
+
+
+def _check_records(gc, rec, oc, off, orec, cap):
+    """Per query: equal counts; equal pair multisets; and for queries at the
+    cap the reference's order (records of one query keep production order
+    under a stable sort by query)."""
+    assert np.array_equal(gc, oc)
+    assert len(rec) == off[-1]
+    g = rec[np.argsort(rec[:, 0], kind="stable")]
+    qo = np.repeat(np.arange(len(oc)), oc)
+    assert np.array_equal(g[:, 0], qo)
+    flagged = np.flatnonzero(oc >= cap)
+    for q in flagged:
+        a, b = off[q], off[q + 1]
+        assert np.array_equal(g[a:b, 1:], orec[a:b]), "query %d: not the depth-first prefix" % q
+    # multisets: sort (query, b1, b2) on both sides
+    ks = lambda q, p: np.lexsort((p[:, 1], p[:, 0], q))
+    go = g[ks(g[:, 0], g[:, 1:])]
+    oo = orec[ks(qo, orec)]
+    assert np.array_equal(go[:, 1:], oo)
+    return len(flagged)
+
+
+@pytest.mark.parametrize("cap", [100000, 70])
+def test_uneven_batch_fast_engine(ctx, uneven, cap):
+    w, ids, om = uneven
+    print("case: uneven batch, 64 pairs, 10000 queries, FAST cap=%d" % cap, flush=True)
+    oc, off, orec = fclo.collide_batch_flat(om, w["query_meshes"], w["tf1"], w["tf2"], cap,
+                                            threads=THREADS)
+    # before the fix-up a query at the cap holds all its pairs: room for them
+    rc, gc, rec, nrec = _gpu_records(ctx, w, ids, cap, F.ENGINE_FAST, int(2 * oc.sum()) + 4096)
+    assert rc == 0 and nrec == oc.sum()
+    flagged = _check_records(gc, rec, oc, off, orec, cap)
+    assert (oc > 0).mean() > 0.1
+    if cap == 70:
+        assert flagged > 100
+
+
+def test_uneven_batch_plain_engine(ctx, uneven):
+    """The plain FP64 traversal (the speed-up's baseline) on the same batch,
+    capped at 70: every query's records in the reference's order."""
+    w, ids, om = uneven
+    cap = 70
+    print("case: uneven batch, PLAIN cap=%d" % cap, flush=True)
+    oc, off, orec = fclo.collide_batch_flat(om, w["query_meshes"], w["tf1"], w["tf2"], cap,
+                                            threads=THREADS)
+    rc, gc, rec, nrec = _gpu_records(ctx, w, ids, cap, F.ENGINE_PLAIN, int(oc.sum()) + 16)
+    assert rc == 0 and nrec == oc.sum()
+    _check_records(gc, rec, oc, off, orec, 1)  # every query in order
+
+
+def test_uneven_batch_bool_engine(ctx, uneven):
+    w, ids, om = uneven
+    print("case: uneven batch, BOOL", flush=True)
+    oc = fclo.collide_batch(om, w["query_meshes"], w["tf1"], w["tf2"], 1, threads=THREADS)[0]
+    qm = ids[w["query_meshes"]]
+    n = len(qm)
+    counts = torch.zeros(n, dtype=torch.int32, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.collide_batch_async(_t(qm, torch.int32), _t(w["tf1"], torch.float64),
+                            _t(w["tf2"], torch.float64), 1, counts, None, num,
+                            engine=F.ENGINE_BOOL, stream=s)
+    assert ctx.sync_status(s) == 0
+    assert np.array_equal(counts.cpu().numpy(), oc)
+
+
+@pytest.mark.parametrize("engine", [F.ENGINE_FAST, F.ENGINE_PLAIN])
+def test_distance_workload_full_size(ctx, engine):
+    """Mesh distance between the 20,480-triangle RSS icospheres, 1,000 of
+    the workload's poses: distances, nearest points and ids bit-identical
+    (statistics too for the PLAIN engine, which keeps the reference's
+    order)."""
+    w = W.mesh_distance_pair(1000)
+    (V1, T1), (V2, T2) = w["meshes"]
+    assert len(T1) == 20480
+    g = [ctx.mesh(V1, T1, F.RSS), ctx.mesh(V2, T2, F.RSS)]
+    o = _oracle_meshes(w["meshes"], fclo.RSS)
+    n = len(w["tf1"])
+    od, opts, oids, obv, olf = fclo.distance_batch(o, w["query_meshes"], w["tf1"], w["tf2"],
+                                                   threads=THREADS)
+    print("case: distance workload, 2x20480 tris, n=%d engine=%d" % (n, engine), flush=True)
+    dist = torch.zeros(n, dtype=torch.float64, device=DEV)
+    pts = torch.zeros((n, 6), dtype=torch.float64, device=DEV)
+    ids = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
+    bvt = torch.zeros(n, dtype=torch.int64, device=DEV)
+    lft = torch.zeros(n, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+    ctx.distance_batch_async(_t(np.tile(g, (n, 1)), torch.int32), _t(w["tf1"], torch.float64),
+                             _t(w["tf2"], torch.float64), dist, pts, ids, engine=engine,
+                             bv_tests=bvt, leaf_tests=lft, stream=s)
+    assert ctx.sync_status(s) == 0
+    assert np.array_equal(dist.cpu().numpy().view(np.uint64), od.view(np.uint64))
+    assert np.array_equal(ids.cpu().numpy(), oids)
+    assert np.array_equal(pts.cpu().numpy().view(np.uint64), opts.view(np.uint64))
+    if engine == F.ENGINE_PLAIN:
+        assert np.array_equal(bvt.cpu().numpy(), obv) and np.array_equal(lft.cpu().numpy(), olf)
+    for m in g:
+        ctx.mesh_destroy(m)
+
+
+def test_serving_workload_full_scene(ctx):
+    """The 500,000-triangle scene and the 20-link robot (1,000-50,000
+    triangles per link) with four trajectories of the request pool, through
+    the request loop: every boolean equals the oracle's."""
+    sV, sT = W.serving_scene()
+    assert len(sT) == 499998
+    links = W.serving_robot()
+    scene = ctx.mesh(sV, sT)
+    link_ids = np.array([ctx.mesh(V, T) for V, T in links], np.int32)
+    om = _oracle_meshes([(sV, sT)] + links)
+    rng = np.random.default_rng(0x5E4E)
+    reqs = [np.ascontiguousarray(W.serving_request(20, L, rng)) for L in (64, 150, 400, 1000)]
+    expect = []
+    for P in reqs:
+        nq = P.shape[0] * 20
+        qm = np.stack([np.zeros(nq, np.int32),
+                       1 + np.tile(np.arange(20, dtype=np.int32), P.shape[0])], 1)
+        tf1 = np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float), (nq, 1))
+        c = fclo.collide_batch(om, qm, tf1, P.reshape(nq, 12), 1, threads=THREADS)[0]
+        expect.append((c > 0).astype(np.uint8).reshape(P.shape[0], 20))
+    frac = np.mean(np.concatenate([e.ravel() for e in expect]))
+    assert 0.05 < frac < 0.95, frac
+    print("case: serving, full scene, 4 requests (%.2f colliding)" % frac, flush=True)
+    srv = F.Server(ctx, scene, num_streams=3)
+    try:
+        res, lat, _ = srv.replay([(link_ids, P) for P in reqs], np.array([0, 0, 1.0, 2.0]))
+        for i, (r, e) in enumerate(zip(res, expect)):
+            assert np.array_equal(r, e), "request %d differs from the oracle" % i
+        assert np.all(lat > 0)
+    finally:
+        srv.close()
+    ctx.mesh_destroy(scene)
+    for m in link_ids:
+        ctx.mesh_destroy(int(m))
