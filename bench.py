@@ -40,15 +40,44 @@ OPS_RSS = 380          # ops of one RSS lower bound (126 transform + ~250 rectDi
 OPS_TRI_DIST = 900     # FP64 ops of one triangle distance
 
 # HBM bytes the depth-first kernel moves per step (summed over its launches,
-# one per round), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over
-# this workload, each scaled by the reading of the same counter for a device
-# copy of known size (python bench.py --copy-calibration); keyed by
-# (queries, pairs).  profiles/engine_dfs_traffic.md holds the runs.
-TRAFFIC = {
-    (1_000_000, 64): {"bytes": 6.596e12,
-                      "source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), "
-                                "engine_dfs launches of one step"},
-}
+# one per round) come from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
+# over this workload, each scaled by the reading of the same counter for a
+# device copy of known size (python bench.py --copy-calibration).  The
+# summary with the commands is profiles/engine_dfs_traffic.md; its
+# machine-readable line, in profiles/engine_dfs_traffic.txt, is parsed by
+# measured_traffic().
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "engine_dfs_traffic.txt")
+
+
+def measured_traffic(queries, pairs):
+    """{"bytes", "kernel_ms", "source"} of the profiled engine_dfs traffic for
+    this workload size, or None: from the line
+    "traffic: queries=Q pairs=P bytes_per_step=B kernel_ms_per_step=T" of the
+    profile summary (T: the kernel's summed time per step in the profiled
+    run)."""
+    import re
+
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    for m in re.finditer(r"^traffic: queries=(\d+) pairs=(\d+) bytes_per_step=(\S+) "
+                         r"kernel_ms_per_step=(\S+)\s*$", text, re.M):
+        if int(m.group(1)) == queries and int(m.group(2)) == pairs:
+            return {"bytes": float(m.group(3)), "kernel_ms": float(m.group(4)),
+                    "source": "profiles/engine_dfs_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                              "WRITE_SIZE, scaled by a copy of known size"}
+    return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -549,6 +578,190 @@ def bench_meshshape(args):
     print(json.dumps(out), flush=True)
 
 
+def first_contact_measure(ctx, steps, no_cpu, log_prefix="first contact"):
+    """Plain mesh-vs-mesh collide, CollisionRequest(1, false): two perturbed
+    icospheres of 2,420 triangles, 10,000 poses.  The FAST engine runs this
+    cap through its ordered kernel (one query per lane, FP32 cull, FP64
+    fallback and leaves, the reference's order); beside it the plain FP64
+    kernel and the CPU oracle.  Returns a dict for the JSON line."""
+    import numpy as np
+    import torch
+
+    import fcl_amd as F
+    from fcl_amd import workloads
+
+    W = workloads.mesh_collide_pair(10000)
+    g = [ctx.mesh(V, T) for V, T in W["meshes"]]
+    n = len(W["tf1"])
+    dev = torch.device("cuda:0")
+    d_qm = torch.from_numpy(np.tile(np.array(g, np.int32), (n, 1))).to(dev)
+    d1 = torch.from_numpy(W["tf1"]).to(dev)
+    d2 = torch.from_numpy(W["tf2"]).to(dev)
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    pairs = torch.zeros((n, 3), dtype=torch.int32, device=dev)
+    num = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(engine, reps):
+        for _ in range(2):
+            ctx.collide_batch_async(d_qm, d1, d2, 1, counts, pairs, num, engine=engine,
+                                    stream=stream.cuda_stream)
+        ctx.sync(stream.cuda_stream)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            ctx.collide_batch_async(d_qm, d1, d2, 1, counts, pairs, num, engine=engine,
+                                    stream=stream.cuda_stream)
+        b.record(stream)
+        ctx.sync(stream.cuda_stream)
+        ms = a.elapsed_time(b) / reps
+        rec = pairs[:int(num.item())].cpu().numpy()
+        return ms, counts.cpu().numpy(), rec[np.argsort(rec[:, 0])]
+
+    fast_ms, fc, fp = timed(F.ENGINE_FAST, max(steps, 5) * 4)
+    plain_ms, pc, pp = timed(F.ENGINE_PLAIN, max(steps, 5))
+    out = {"workload": "2 x 2420-tri icospheres, 10000 poses, num_max_contacts 1",
+           "queries": n, "colliding": int((fc > 0).sum()),
+           "ordered_ms": fast_ms, "ordered_queries_per_s": n / (fast_ms * 1e-3),
+           "plain_ms": plain_ms, "plain_queries_per_s": n / (plain_ms * 1e-3),
+           "exact_match_vs_plain": bool(np.array_equal(fc, pc) and np.array_equal(fp, pp))}
+    log("%s: ordered %.3f ms, plain %.3f ms for %d queries" % (log_prefix, fast_ms, plain_ms, n))
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        om = [fclo.OracleMesh(V, T) for V, T in W["meshes"]]
+        threads = host_threads()
+        a = time.perf_counter()
+        oc, op, obv, olf = fclo.collide_batch(om, W["query_meshes"], W["tf1"], W["tf2"], 1,
+                                              store=1, threads=threads)
+        cpu_s = time.perf_counter() - a
+        a = time.perf_counter()
+        fclo.collide_batch(om, W["query_meshes"][:2000], W["tf1"][:2000], W["tf2"][:2000], 1,
+                           threads=1)
+        one_s = time.perf_counter() - a
+        hit = oc > 0
+        out["exact_match_vs_oracle"] = bool(
+            np.array_equal(fc, oc) and np.array_equal(fp[:, 1:], op[hit, 0]))
+        out["cpu_baseline"] = {"value": n / cpu_s, "unit": "queries/s", "cores": threads,
+                               "kind": "port", "sample": "all 10000 poses",
+                               "one_thread": {"value": 2000 / one_s, "queries": 2000}}
+        # roofline of the ordered kernel on the oracle's work counts: FP32
+        # OBB tests (315 ops, 128 B) and FP64 triangle tests (923 ops, 144 B)
+        t = fast_ms * 1e-3
+        bv, lf = float(obv.sum()), float(olf.sum())
+        valu = (bv * OPS_BV / PEAK_FP32 + lf * OPS_LEAF / PEAK_FP64) / t
+        hbm = (bv * BYTES_BV + lf * BYTES_LEAF) / t / PEAK_HBM
+        out["roofline"] = {"kernel": "ordered_collide", "bound": "valu", "valu_frac": valu,
+                           "hbm_frac": hbm, "bv_tests_per_query": bv / n,
+                           "leaf_tests_per_query": lf / n,
+                           "note": "2 x 4839 nodes stay in cache: bound by VALU and latency"}
+    for m in g:
+        ctx.mesh_destroy(m)
+    return out
+
+
+def bench_first_contact(args):
+    import fcl_amd as F
+
+    ctx = F.Context(0)
+    fc = first_contact_measure(ctx, args.steps, args.no_cpu)
+    out = {
+        "metric": "mesh collide queries/sec, first contact (CollisionRequest(1, false))",
+        "value": fc["ordered_queries_per_s"], "unit": "queries/s", "n_gpus": 1,
+        "steps": max(args.steps, 5) * 4, "warmup": 2, "ms_per_step": fc["ordered_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 cull + f64 leaves", "data": "synthetic (seeded perturbed icospheres)",
+        "config": {"workload": fc["workload"]},
+        "plain_gpu": {"value": fc["plain_queries_per_s"], "ms": fc["plain_ms"]},
+        "speedup_vs_plain": fc["ordered_queries_per_s"] / fc["plain_queries_per_s"],
+        "exact_match_vs_plain": fc["exact_match_vs_plain"],
+        "exact_match_vs_oracle": fc.get("exact_match_vs_oracle"),
+        "roofline": fc.get("roofline"), "cpu_baseline": fc.get("cpu_baseline"),
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine, seed=11):
+    """Replays Poisson request streams (one per rate, `seconds` long) from
+    the pool through a fresh request loop; per rate the p50 / p99 request
+    latency (arrival to results in host memory) and the queries/s."""
+    import numpy as np
+
+    import fcl_amd as F
+
+    nlink = len(link_ids)
+    srv = F.Server(ctx, scene, num_streams=streams, engine=engine)
+    srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
+    rng = np.random.default_rng(seed)
+    loads = []
+    for rate in rates:
+        gaps = rng.exponential(1000.0 / rate, int(rate * seconds * 1.5) + 16)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals < seconds * 1000.0]
+        n = len(arrivals)
+        idx = rng.integers(0, len(pool), n)
+        reqs = [(link_ids, pool[i]) for i in idx]
+        nq = np.array([pool[i].shape[0] * nlink for i in idx])
+        b0, q0 = srv.stats()
+        _, lat, elapsed = srv.replay(reqs, arrivals)
+        b1, q1 = srv.stats()
+        p50, p99 = np.percentile(lat, [50, 99])
+        loads.append({"rate_per_s": rate, "requests": n, "queries": int(nq.sum()),
+                      "p50_ms": float(p50), "p99_ms": float(p99), "max_ms": float(lat.max()),
+                      "queries_per_s": float(nq.sum() / (elapsed / 1e3)),
+                      "offered_queries_per_s": float(rate * nq.mean()),
+                      "batches": b1 - b0, "elapsed_ms": elapsed})
+        log("rate %5d/s: %5d requests, p50 %.2f ms, p99 %.2f ms, %.3g queries/s, %d batches"
+            % (rate, n, p50, p99, nq.sum() / (elapsed / 1e3), b1 - b0))
+    return srv, loads
+
+
+def serving_summary(rates, seconds, streams, no_cpu):
+    """The serving workload inside the default benchmark line: the 500k-tri
+    scene and 20-link robot, p50 / p99 at the given loads, and the booleans
+    of four pool requests against the oracle's."""
+    import numpy as np
+
+    import fcl_amd as F
+    from fcl_amd import workloads
+
+    sV, sT = workloads.serving_scene()
+    links = workloads.serving_robot()
+    pool = workloads.serving_requests(64)
+    ctx = F.Context(0)
+    scene = ctx.mesh(sV, sT)
+    link_ids = np.array([ctx.mesh(V, T) for V, T in links], np.int32)
+    srv, loads = serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams,
+                                 F.ENGINE_BOOL)
+    out = {"workload": "500k-tri scene, 20 links of 1k-50k tris, pool of 64 trajectories of "
+                       "64-4096 waypoints, Poisson arrivals, %d streams, BOOL engine" % streams,
+           "seconds_per_rate": seconds, "loads": loads}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fclo
+
+        with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            om = list(ex.map(lambda m: fclo.OracleMesh(m[0], m[1]), [(sV, sT)] + links))
+        reqs = pool[:4]
+        got, _, _ = srv.replay([(link_ids, P) for P in reqs], np.zeros(len(reqs)))
+        ok = True
+        for P, r in zip(reqs, got):
+            nq = P.shape[0] * len(links)
+            qm = np.stack([np.zeros(nq, np.int32),
+                           1 + np.tile(np.arange(len(links), dtype=np.int32), P.shape[0])], 1)
+            tf1 = np.tile(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float), (nq, 1))
+            c = fclo.collide_batch(om, qm, tf1, P.reshape(nq, 12), 1,
+                                   threads=host_threads())[0]
+            ok &= bool(np.array_equal((c > 0).astype(np.uint8).reshape(r.shape), r))
+        out["booleans_equal_oracle"] = ok
+        out["checked_queries"] = int(sum(P.shape[0] for P in reqs) * len(links))
+    srv.close()
+    ctx.close()
+    return out
+
+
 def bench_serving(args):
     """Robot-trajectory serving: one resident ~500k-triangle scene, a
     20-link robot, requests of 64-4096 waypoints arriving as Poisson
@@ -633,29 +846,8 @@ def bench_serving(args):
                                               e["fp64_triangle_tests_per_query"]))
     engine = {"bool": F.ENGINE_BOOL, "fast": F.ENGINE_FAST, "plain": F.ENGINE_PLAIN}[
         args.serve_engine]
-    srv = F.Server(ctx, scene, num_streams=args.serve_streams, engine=engine)
-    srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
-    rng = np.random.default_rng(11)
-    loads = []
-    for rate in args.serve_rates:
-        gaps = rng.exponential(1000.0 / rate, int(rate * args.serve_seconds * 1.5) + 16)
-        arrivals = np.cumsum(gaps)
-        arrivals = arrivals[arrivals < args.serve_seconds * 1000.0]
-        n = len(arrivals)
-        idx = rng.integers(0, len(pool), n)
-        reqs = [(link_ids, pool[i]) for i in idx]
-        nq = np.array([pool[i].shape[0] * nlink for i in idx])
-        b0, q0 = srv.stats()
-        _, lat, elapsed = srv.replay(reqs, arrivals)
-        b1, q1 = srv.stats()
-        p50, p99 = np.percentile(lat, [50, 99])
-        loads.append({"rate_per_s": rate, "requests": n, "queries": int(nq.sum()),
-                      "p50_ms": float(p50), "p99_ms": float(p99), "max_ms": float(lat.max()),
-                      "queries_per_s": float(nq.sum() / (elapsed / 1e3)),
-                      "offered_queries_per_s": float(rate * nq.mean()),
-                      "batches": b1 - b0, "elapsed_ms": elapsed})
-        log("rate %5d/s: %5d requests, p50 %.2f ms, p99 %.2f ms, %.3g queries/s, %d batches"
-            % (rate, n, p50, p99, nq.sum() / (elapsed / 1e3), b1 - b0))
+    srv, loads = serving_measure(ctx, scene, link_ids, pool, args.serve_rates,
+                                 args.serve_seconds, args.serve_streams, engine)
     # one request alone, the largest of the pool
     big = max(pool, key=lambda P: P.shape[0])
     _, lat1, _ = srv.replay([(link_ids, big)], [0.0])
@@ -710,12 +902,16 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload",
-                    choices=["mesh", "primitives", "serving", "distance", "meshshape"],
+                    choices=["mesh", "primitives", "serving", "distance", "meshshape",
+                             "first-contact"],
                     default="mesh",
                     help="mesh: the batched uneven mesh collision (default); primitives: "
                          "4M primitive-shape pairs; serving: robot-trajectory requests at "
                          "Poisson loads; distance: 10k mesh distance queries (20k-tri RSS); "
-                         "meshshape: 200k primitives against a 20k-tri mesh")
+                         "meshshape: 200k primitives against a 20k-tri mesh; first-contact: "
+                         "10k first-contact queries between 2420-tri icospheres")
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="mesh workload: leave out the first-contact and serving sub-objects")
     ap.add_argument("--serve-rates", type=float, nargs="+",
                     default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--serve-seconds", type=float, default=1.0,
@@ -765,6 +961,9 @@ def main():
         return
     if args.workload == "meshshape":
         bench_meshshape(args)
+        return
+    if args.workload == "first-contact":
+        bench_first_contact(args)
         return
 
     import numpy as np
@@ -877,39 +1076,45 @@ def main():
         log("plain: %.1f ms, %.3g queries/s, exact match %s" % (plain_s * 1e3, nq / plain_s, exact))
 
     # ------------------------------------------------------------- roofline
-    # The depth-first kernel's algorithmic work is its share of the exact
-    # traversal (the oracle's num_bv_tests / num_leaf_tests minus what the
-    # breadth-first phase did).  VALU fraction prices it at the full-test op
-    # counts, HBM fraction at the bytes of the records one test reads.  The
-    # reported bound is the HBM one (the larger fraction on this workload);
-    # there is no matrix-core work, and the VALU fraction is given beside it.
-    # Node records are re-used from L2, so the algorithmic-byte rate can
-    # exceed the HBM peak; traffic_frac is the rate of the bytes measured.
+    # The depth-first kernel (FP32 only) does the BV tests of the exact
+    # traversal that the breadth-first phase left; its triangle tests are
+    # queued for engine_leaves and not charged here.  Its bound is HBM: the
+    # achieved rate is the bytes it moves (rocprofv3 counters of the same
+    # workload, profiles/engine_dfs_traffic.md) over its time in this run
+    # (HIP events, summed over the step's launches).  The algorithmic bytes
+    # (128 B per BV test) are given beside it: node records are re-used from
+    # L2, so their rate is not bounded by the HBM peak.  The VALU fraction
+    # prices the BV tests at 315 FP32 operations each.
     t_dfs = rep["dfs_ms"] * 1e-3
-    alg_bytes = rep["dfs_bv_tests"] * BYTES_BV + rep["dfs_leaf_tests"] * BYTES_LEAF
-    valu_frac = hbm_frac = None
+    alg_bytes = rep["dfs_bv_tests"] * BYTES_BV
+    valu_frac = None
     if t_dfs > 0:
-        valu_frac = (rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 +
-                     rep["dfs_leaf_tests"] * OPS_LEAF / PEAK_FP64) / t_dfs
-        hbm_frac = alg_bytes / t_dfs / PEAK_HBM
-    traffic = TRAFFIC.get((nq, args.pairs))
+        valu_frac = rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 / t_dfs
+    traffic = measured_traffic(nq, args.pairs)
+    achieved = traffic["bytes"] / t_dfs if traffic and t_dfs > 0 else None
+    frac = achieved / PEAK_HBM if achieved else None
+    if frac is not None and frac > 1:
+        log("measured traffic exceeds the HBM peak: profile out of date, roofline left null")
+        achieved = frac = None
     roofline = {
         "kernel": "engine_dfs",
         "bound": "hbm",
-        "achieved": alg_bytes / t_dfs / 1e9 if t_dfs > 0 else None,
+        "achieved": achieved / 1e9 if achieved else None,
         "peak": PEAK_HBM / 1e9,
         "unit": "GB/s",
-        "frac": hbm_frac,
+        "frac": frac,
         "traffic": traffic["bytes"] if traffic else None,
         "traffic_source": traffic["source"] if traffic else None,
-        # measured bytes over the kernel's time: the fraction of HBM peak it uses
-        "traffic_frac": traffic["bytes"] / t_dfs / PEAK_HBM if traffic and t_dfs > 0 else None,
+        # the same bytes over the kernel's time inside the profiled run
+        "frac_in_profiled_run": (traffic["bytes"] / (traffic["kernel_ms"] * 1e-3) / PEAK_HBM
+                                 if traffic else None),
         "algorithmic_bytes": alg_bytes,
+        "algorithmic_rate_gbs": alg_bytes / t_dfs / 1e9 if t_dfs > 0 else None,
         "valu_frac": valu_frac,
         "valu_peak_tflops": PEAK_FP32 / 1e12,
         "kernel_ms": rep["dfs_ms"],
         "bv_tests": rep["dfs_bv_tests"],
-        "leaf_tests": rep["dfs_leaf_tests"],
+        "leaf_tests_in_engine_leaves": rep["dfs_leaf_tests"],
         "fp64_fallback_rate": rep["fp64_fallbacks"] / max(1, rep["dfs_bv_tests"] +
                                                            rep["expand_bv_tests"]),
         "phase_ms": {k: rep[k] for k in ("setup_ms", "expand_ms", "leaves_ms", "dfs_ms",
@@ -943,13 +1148,30 @@ def main():
         sc, _, _, _ = fclo.collide_batch(omeshes, Q[sample], W["tf1"][sample], W["tf2"][sample],
                                          cap, threads=threads)
         cpu_s = time.perf_counter() - a
+        # one thread, on the first queries of the same sample (about a third
+        # of the multi-threaded run's CPU time)
+        n1 = int(min(n_s, max(200, n_s * args.cpu_seconds / 3 / max(cpu_s * threads, 1e-3))))
+        s1 = sample[:n1]
+        a = time.perf_counter()
+        c1, _, _, _ = fclo.collide_batch(omeshes, Q[s1], W["tf1"][s1], W["tf2"][s1], cap,
+                                         threads=1)
+        one_s = time.perf_counter() - a
         gpu_c = fast_counts.cpu().numpy()
         cpu_match = bool(np.array_equal(sc, gpu_c[sample])) and \
-            bool(np.array_equal(pc, gpu_c[probe]))
+            bool(np.array_equal(pc, gpu_c[probe])) and bool(np.array_equal(c1, gpu_c[s1]))
         cpu = {"value": n_s / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": "%d of the %d queries, drawn uniformly (seed 7), OpenMP dynamic "
-                         "schedule; contact counts equal the GPU's: %s" % (n_s, nq, cpu_match)}
-        log("cpu oracle: %d queries in %.1fs on %d threads" % (n_s, cpu_s, threads))
+                         "schedule; contact counts equal the GPU's: %s" % (n_s, nq, cpu_match),
+               "one_thread": {"value": n1 / one_s, "unit": "queries/s", "queries": n1}}
+        log("cpu oracle: %d queries in %.1fs on %d threads; %d in %.1fs on one"
+            % (n_s, cpu_s, threads, n1, one_s))
+
+    # ------------------------------------------- first contact and serving
+    first_contact = serving = None
+    if not args.skip_extras:
+        first_contact = first_contact_measure(ctx, args.steps, args.no_cpu)
+        serving = serving_summary([1000, 2000], 1.0, args.serve_streams, args.no_cpu)
 
     p = np.percentile(step_ms, [50, 99]) if step_ms else [None, None]
     out = {
@@ -977,6 +1199,12 @@ def main():
         "exact_match_vs_plain": exact,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        # request latency of the serving workload (arrival to booleans in
+        # host memory) at 1000 and 2000 requests/s
+        "request_latency_ms": ({str(int(l["rate_per_s"])): {"p50": l["p50_ms"], "p99": l["p99_ms"]}
+                                for l in serving["loads"]} if serving else None),
+        "first_contact": first_contact,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

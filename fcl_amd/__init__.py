@@ -445,6 +445,67 @@ class Context:
         return self._L.fclx_sync(self._h, stream)
 
 
+class FrontSet:
+    """fclx_front_set_*: device-resident BVTT front lists, one per sequence
+    of temporally coherent queries (e.g. a link along a trajectory)."""
+
+    def __init__(self, ctx, num_sequences, capacity=4096):
+        self._ctx = ctx
+        self._L = ctx._L
+        h = ctypes.c_void_p()
+        ctx._check(self._L.fclx_front_set_create(ctx._h, int(num_sequences), int(capacity),
+                                                 ctypes.byref(h)))
+        self._h = h
+        self.num_sequences = int(num_sequences)
+        self.capacity = int(capacity)
+
+    def clear(self, stream=None):
+        self._ctx._check(self._L.fclx_front_set_clear(self._h, stream))
+
+    def read(self, sequence):
+        """The sequence's front as an (n, 2) int32 array (b1, b2)."""
+        n = ctypes.c_int32(0)
+        buf = np.zeros((self.capacity, 2), np.int32)
+        self._ctx._check(self._L.fclx_front_set_read(self._h, int(sequence), N.ptr(buf, N.c_i32),
+                                                     self.capacity, ctypes.byref(n)))
+        return buf[:min(n.value, self.capacity)].copy()
+
+    def collide_async(self, seq_offsets, query_meshes, tf1, tf2, num_max_contacts, counts,
+                      pairs, num_pairs, bv_tests=None, leaf_tests=None, stream=None):
+        """fclx_collide_front_batch: sequence s is queries
+        [seq_offsets[s], seq_offsets[s+1]), run in order from front s."""
+        a = N.FrontBatchArgs()
+        ns = int(seq_offsets.shape[0]) - 1
+        n = int(query_meshes.shape[0])
+        a.num_sequences = ns
+        a.seq_offsets = _req(seq_offsets, "seq_offsets", "i32", ns + 1)
+        a.query_meshes = _req(query_meshes, "query_meshes", "i32", 2 * n)
+        a.tf1 = _req(tf1, "tf1", "f64", 12 * n)
+        a.tf2 = _req(tf2, "tf2", "f64", 12 * n)
+        a.num_max_contacts = int(num_max_contacts)
+        a.counts = _req(counts, "counts", "i32", n)
+        a.pairs = _req(pairs, "pairs", "i32", 0, optional=True)
+        a.pair_capacity = int(pairs.numel() // 3) if pairs is not None else 0
+        a.num_pairs = _req(num_pairs, "num_pairs", "i64", 1)
+        a.bv_tests = _req(bv_tests, "bv_tests", "i64", n, optional=True)
+        a.leaf_tests = _req(leaf_tests, "leaf_tests", "i64", n, optional=True)
+        _on_device(seq_offsets, query_meshes, tf1, tf2, counts, pairs, num_pairs, bv_tests,
+                   leaf_tests)
+        self._ctx._check(self._L.fclx_collide_front_batch(self._ctx._h, ctypes.byref(a), self._h,
+                                                          stream))
+
+    def close(self):
+        if self._h:
+            self._L.fclx_front_set_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """fclx_server_*: the request loop over one resident scene mesh.  A
     request is a robot's link meshes and its link poses along a trajectory

@@ -117,6 +117,13 @@ class EngineReport(ctypes.Structure):
 
 
 # name -> (restype, argtypes)
+class FrontBatchArgs(ctypes.Structure):
+    _fields_ = [("num_sequences", c_i32), ("seq_offsets", c_vp), ("query_meshes", c_vp),
+                ("tf1", c_vp), ("tf2", c_vp), ("num_max_contacts", c_i32), ("counts", c_vp),
+                ("pairs", c_vp), ("pair_capacity", c_i64), ("num_pairs", c_vp),
+                ("bv_tests", c_vp), ("leaf_tests", c_vp)]
+
+
 SIGNATURES = {
     "fclx_set_profiling": (c_i32, [c_vp, c_i32]),
     "fclx_engine_last_report": (c_i32, [c_vp, ctypes.POINTER(EngineReport)]),
@@ -152,6 +159,11 @@ SIGNATURES = {
     "fclx_server_replay": (c_i32, [c_vp, ctypes.POINTER(Request), P_dbl, c_i32, P_dbl, P_dbl]),
     "fclx_server_stats": (c_i32, [c_vp, P_i64, P_i64]),
     "fclx_server_destroy": (c_i32, [c_vp]),
+    "fclx_front_set_create": (c_i32, [c_vp, c_i32, c_i32, ctypes.POINTER(c_vp)]),
+    "fclx_front_set_destroy": (c_i32, [c_vp]),
+    "fclx_front_set_clear": (c_i32, [c_vp, c_vp]),
+    "fclx_front_set_read": (c_i32, [c_vp, c_i32, P_i32, c_i32, P_i32]),
+    "fclx_collide_front_batch": (c_i32, [c_vp, ctypes.POINTER(FrontBatchArgs), c_vp, c_vp]),
     "fclx_synth_icosphere_size": (None, [c_i32, P_i32, P_i32]),
     "fclx_synth_icosphere": (c_i32, [c_i32, c_dbl, c_u64, P_dbl, P_i32]),
     "fclx_synth_heightfield_size": (None, [c_i32, P_i32, P_i32]),

@@ -42,6 +42,14 @@ hipError_t dmalloc(T** p, size_t n) {
   return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * std::max<size_t>(n, 1));
 }
 
+__global__ void clear_fronts(int32_t* sizes, int32_t* meshes, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sizes[i] = 0;
+  meshes[2 * i] = -1;
+  meshes[2 * i + 1] = -1;
+}
+
 __global__ void zero_i64(long long* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -520,7 +528,7 @@ int collide_batch_status(fclx_context* ctx, const fclx_collide_batch_args* a, vo
 int status_code(int32_t flags) {
   if (flags & (kStatusStackOverflow | kStatusQueueOverflow)) return FCLX_ERR_STACK_OVERFLOW;
   if (flags & kStatusBadQuery) return FCLX_ERR_INVALID_ARGUMENT;
-  if (flags & kStatusPairOverflow) return FCLX_ERR_CAPACITY;
+  if (flags & (kStatusPairOverflow | kStatusFrontOverflow)) return FCLX_ERR_CAPACITY;
   return FCLX_OK;
 }
 
@@ -909,6 +917,132 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   return FCLX_OK;
 }
 
+struct fclx_front_set {
+  fclx_context* ctx = nullptr;
+  int32_t num_sequences = 0;
+  int32_t capacity = 0;
+  int2* fronts = nullptr;
+  int2* second = nullptr;
+  int32_t* sizes = nullptr;
+  int32_t* meshes = nullptr;
+};
+
+namespace {
+void free_front_set(fclx_front_set* f) {
+  (void)hipFree(f->fronts);
+  (void)hipFree(f->second);
+  (void)hipFree(f->sizes);
+  (void)hipFree(f->meshes);
+  delete f;
+}
+}  // namespace
+
+int fclx_front_set_create(fclx_context* ctx, int32_t num_sequences, int32_t capacity,
+                          fclx_front_set** out) {
+  if (!ctx || !out || num_sequences < 1 || capacity < 1) return FCLX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  (void)hipSetDevice(ctx->device);
+  auto* f = new (std::nothrow) fclx_front_set();
+  if (!f) return FCLX_ERR_OUT_OF_MEMORY;
+  f->ctx = ctx;
+  f->num_sequences = num_sequences;
+  f->capacity = capacity;
+  const size_t entries = static_cast<size_t>(num_sequences) * capacity;
+  hipError_t e;
+  if ((e = hipMalloc(&f->fronts, entries * sizeof(int2))) != hipSuccess ||
+      (e = hipMalloc(&f->second, entries * sizeof(int2))) != hipSuccess ||
+      (e = hipMalloc(&f->sizes, num_sequences * sizeof(int32_t))) != hipSuccess ||
+      (e = hipMalloc(&f->meshes, 2 * num_sequences * sizeof(int32_t))) != hipSuccess) {
+    free_front_set(f);
+    return hip_fail(ctx, e, "front set allocation");
+  }
+  clear_fronts<<<(num_sequences + 255) / 256, 256, 0, ctx->stream>>>(f->sizes, f->meshes,
+                                                                    num_sequences);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+    free_front_set(f);
+    return hip_fail(ctx, e, "front set clear");
+  }
+  *out = f;
+  return FCLX_OK;
+}
+
+int fclx_front_set_destroy(fclx_front_set* f) {
+  if (!f) return FCLX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(f->ctx->device);
+  free_front_set(f);
+  return FCLX_OK;
+}
+
+int fclx_front_set_clear(fclx_front_set* f, void* stream) {
+  if (!f) return FCLX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(f->ctx->device);
+  clear_fronts<<<(f->num_sequences + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
+      f->sizes, f->meshes, f->num_sequences);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? FCLX_OK : hip_fail(f->ctx, e, "front set clear");
+}
+
+int fclx_front_set_read(fclx_front_set* f, int32_t sequence, int32_t* pairs, int32_t room,
+                        int32_t* n) {
+  if (!f || !n || sequence < 0 || sequence >= f->num_sequences || room < 0 ||
+      (room > 0 && !pairs))
+    return FCLX_ERR_INVALID_ARGUMENT;
+  fclx_context* ctx = f->ctx;
+  (void)hipSetDevice(ctx->device);
+  int32_t size = 0;
+  HIP_TRY(ctx, hipMemcpy(&size, f->sizes + sequence, sizeof(size), hipMemcpyDeviceToHost));
+  *n = size;
+  const int32_t k = std::min(size, room);
+  if (k > 0)
+    HIP_TRY(ctx, hipMemcpy(pairs, f->fronts + static_cast<size_t>(sequence) * f->capacity,
+                           static_cast<size_t>(k) * sizeof(int2), hipMemcpyDeviceToHost));
+  return FCLX_OK;
+}
+
+int fclx_collide_front_batch(fclx_context* ctx, const fclx_collide_front_batch_args* a,
+                             fclx_front_set* f, void* stream) {
+  if (!ctx || !a || !f || f->ctx != ctx) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->num_sequences < 0 || a->num_sequences > f->num_sequences || a->num_max_contacts < 1 ||
+      !a->counts || !a->num_pairs || (a->pair_capacity > 0 && !a->pairs) ||
+      (a->num_sequences > 0 &&
+       (!a->seq_offsets || !a->query_meshes || !a->tf1 || !a->tf2)))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid front batch arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FrontLaunch L{};
+  int depth;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    depth = max_depth(ctx);
+    L.meshes = ctx->d_table;
+    L.num_meshes = static_cast<int32_t>(ctx->meshes.size());
+  }
+  if (2 * depth > kMaxStack)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
+  (void)hipSetDevice(ctx->device);
+  zero_i64<<<1, 64, 0, s>>>(reinterpret_cast<long long*>(a->num_pairs), 1);
+  L.num_sequences = a->num_sequences;
+  L.seq_offsets = a->seq_offsets;
+  L.query_meshes = a->query_meshes;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.cap = a->num_max_contacts;
+  L.counts = a->counts;
+  L.pairs = a->pairs;
+  L.pair_capacity = a->pairs ? a->pair_capacity : 0;
+  L.num_pairs = reinterpret_cast<long long*>(a->num_pairs);
+  L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
+  L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
+  L.fronts = f->fronts;
+  L.second = f->second;
+  L.front_sizes = f->sizes;
+  L.front_meshes = f->meshes;
+  L.front_cap = f->capacity;
+  L.status = ctx->d_status;
+  const hipError_t e = launch_collide_front(L, 2 * depth, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "front collide launch");
+  return FCLX_OK;
+}
+
 int fclx_sync(fclx_context* ctx, void* stream) {
   if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);
@@ -924,6 +1058,8 @@ int fclx_sync(fclx_context* ctx, void* stream) {
                   "a query named a missing mesh or shape, or an unsupported BV pair");
     if (st & kStatusPairOverflow)
       return fail(ctx, FCLX_ERR_CAPACITY, "pair output exceeded pair_capacity");
+    if (st & kStatusFrontOverflow)
+      return fail(ctx, FCLX_ERR_CAPACITY, "a BVTT front outgrew its front-set capacity");
   }
   return FCLX_OK;
 }

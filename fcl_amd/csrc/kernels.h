@@ -14,6 +14,7 @@ enum : int32_t {
   kStatusPairOverflow = 2,
   kStatusQueueOverflow = 4,
   kStatusBadQuery = 8,  // a query named a missing mesh or an unsupported BV pair
+  kStatusFrontOverflow = 16,  // a BVTT front outgrew its room
 };
 
 // Largest per-lane stack the traversal kernels are instantiated for; the
@@ -111,6 +112,31 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& L, int waves, hipStr
 size_t mesh_shape_epa_lane_bytes();
 
 hipError_t launch_collide_plain(const CollideLaunch& L, int stack_need, hipStream_t s);
+
+// collide() with BVTT front lists: one lane per sequence of queries
+struct FrontLaunch {
+  const MeshDev* meshes;
+  int32_t num_meshes;
+  int32_t num_sequences;
+  const int32_t* seq_offsets;  // num_sequences + 1: sequence s = queries [off[s], off[s+1])
+  const int32_t* query_meshes; // 2 per query
+  const double* tf1;
+  const double* tf2;
+  int32_t cap;                 // num_max_contacts
+  int32_t* counts;
+  int32_t* pairs;              // 3 per record: query, b1, b2
+  long long pair_capacity;
+  long long* num_pairs;
+  long long* bv_tests;
+  long long* leaf_tests;
+  int2* fronts;                // front_cap entries per sequence
+  int2* second;                // front_cap entries per sequence (scratch)
+  int32_t* front_sizes;        // per sequence
+  int32_t* front_meshes;       // 2 per sequence: the mesh pair of its front
+  int32_t front_cap;
+  int32_t* status;
+};
+hipError_t launch_collide_front(const FrontLaunch& L, int stack_need, hipStream_t s);
 // Sort buffers, the EPA queue and per-lane EPA memory of the shape path; one
 // per context, used by one batch at a time (the caller serialises calls;
 // batches on different streams are ordered through an event).

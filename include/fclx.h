@@ -410,6 +410,58 @@ int fclx_distance(fclx_context* ctx, int32_t mesh1, const double tf1[12], int32_
                   const double tf2[12], int32_t enable_nearest_points, double* min_distance,
                   double* points, int32_t* ids, int64_t* stats);
 
+/* ------------------------------------------------------ BVTT front lists
+ * collide() with a front list (detail::collide(node, front_list),
+ * collision_node-inl.h:71-82; BVHFrontList, geometry/bvh/detail/
+ * BVH_front.h:49-72; propagateBVHFrontListCollisionRecurse,
+ * traversal_recurse-inl.h:463-521), for temporally coherent queries such as
+ * the waypoints of a trajectory.  A front set holds, per sequence, the BVTT
+ * nodes where the sequence's last query stopped (its front, up to
+ * `capacity` (b1, b2) entries, 16 B of device memory per entry).  A batch
+ * runs each sequence's queries in order on one lane: a query whose front is
+ * empty traverses from the roots recording the front (no early stop: the
+ * reference skips the stop rule whenever a front list is given), one whose
+ * front holds entries propagates it.  Per query: counts (contacts, at most
+ * num_max_contacts -- a pair reached by opening a front entry is tested
+ * twice and may count twice, as in the reference), records (query, b1, b2)
+ * in the reference's order per query, and the reference's num_bv_tests /
+ * num_leaf_tests.  Only whether the count is nonzero is the same as a fresh
+ * collide() (test_fcl_frontlist.cpp checks exactly this).  A front
+ * belongs to one mesh pair: a query naming another pair than its
+ * sequence's front starts from the roots.  A front that outgrows the
+ * capacity is dropped (the next query starts from the roots) and fclx_sync
+ * reports FCLX_ERR_CAPACITY.  enable_contact is not supported here. */
+typedef struct fclx_front_set fclx_front_set;
+
+int fclx_front_set_create(fclx_context* ctx, int32_t num_sequences, int32_t capacity,
+                          fclx_front_set** out);
+int fclx_front_set_destroy(fclx_front_set* fronts);
+/* Empties every front (asynchronous on the stream). */
+int fclx_front_set_clear(fclx_front_set* fronts, void* stream);
+/* Host copy of one sequence's front: *n = its size, the first min(n, room)
+ * entries (b1, b2) to pairs.  Synchronous. */
+int fclx_front_set_read(fclx_front_set* fronts, int32_t sequence, int32_t* pairs, int32_t room,
+                        int32_t* n);
+
+typedef struct {
+  int32_t num_sequences;        /* <= the set's num_sequences; sequence s uses front s */
+  const int32_t* seq_offsets;   /* device: num_sequences + 1; sequence s is queries
+                                   [seq_offsets[s], seq_offsets[s+1]) in order */
+  const int32_t* query_meshes;  /* device: 2 mesh ids (OBBRSS) per query */
+  const double* tf1;            /* device: 12 per query */
+  const double* tf2;            /* device: 12 per query */
+  int32_t num_max_contacts;     /* >= 1 */
+  int32_t* counts;              /* device: per query */
+  int32_t* pairs;               /* device: 3 int32 per record (query, b1, b2); may be NULL */
+  int64_t pair_capacity;
+  int64_t* num_pairs;           /* device: records produced */
+  int64_t* bv_tests;            /* device, optional: per query */
+  int64_t* leaf_tests;          /* device, optional: per query */
+} fclx_collide_front_batch_args;
+
+int fclx_collide_front_batch(fclx_context* ctx, const fclx_collide_front_batch_args* args,
+                             fclx_front_set* fronts, void* stream);
+
 /* ---------------------------------------------------------------- serving
  * A request loop over one resident scene (no counterpart in the reference,
  * whose collide() is one synchronous call): planner requests -- a robot's

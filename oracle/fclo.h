@@ -116,6 +116,15 @@ void fclo_collide(const fclo_model *m1, const double tf1[12],
                   const fclo_model *m2, const double tf2[12], int cap,
                   int *pairs, fclo_collide_stats *stats);
 
+/* collide() with a BVTT front list (detail::collide(node, front_list),
+ * collision_node-inl.h:71-82; traversal_recurse-inl.h:82-129, 463-521): a
+ * front of n_front entries in (2 ints each; 0 entries: traverse from the
+ * roots), the new front out (front_room entries, *n_out its size). */
+void fclo_collide_front(const fclo_model *m1, const double tf1[12], const fclo_model *m2,
+                        const double tf2[12], int cap, int store, int *pairs, const int *front,
+                        int n_front, int *front_out, int front_room, int *n_out,
+                        fclo_collide_stats *stats);
+
 /* collide() with enable_contact: cap counts contacts (a triangle pair adds
  * 0-2); the first min(count, store) contacts are written as (b1,b2) pairs
  * plus world-frame pos[3], normal[3] and depth. */

@@ -85,6 +85,8 @@ def lib():
             "fclo_distance_all_pairs": (c_dbl, [PM, P_dbl, PM, P_dbl, P_i32]),
             "fclo_collide_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl, P_dbl,
                                           c_i32, c_i32, P_i32, P_i32, P_i64, P_i64, c_i32]),
+            "fclo_collide_front": (None, [PM, P_dbl, PM, P_dbl, c_i32, c_i32, P_i32, P_i32, c_i32,
+                                          P_i32, c_i32, P_i32, ctypes.POINTER(CollideStats)]),
             "fclo_collide_batch_flat": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
                                                P_dbl, c_i32, P_i64, P_i32, P_i32, c_i32]),
             "fclo_distance_batch": (None, [ctypes.POINTER(PM), c_i32, P_i32, P_i32, P_dbl,
@@ -364,6 +366,31 @@ def collide_batch(meshes, qm, tf1, tf2, cap, store=0, threads=None):
                              _p(tf2, c_dbl), cap, store, _p(counts, c_i32), _p(pairs, c_i32),
                              _p(bvt, c_i64), _p(lft, c_i64), threads or os.cpu_count())
     return counts, pairs[:, :store], bvt, lft
+
+
+def collide_front(m1, tf1, m2, tf2, front, cap=1 << 30):
+    """One collide() with a BVTT front list: front (k,2) int32 from the
+    previous call (k = 0: traverse from the roots).  Returns (pairs (n,2) in
+    order, new front (k',2), (num_contacts, num_bv_tests, num_leaf_tests))."""
+    tf1 = _d(tf1).reshape(12)
+    tf2 = _d(tf2).reshape(12)
+    fin = np.ascontiguousarray(front, np.int32).reshape(-1, 2)
+    store = 4096
+    room = max(2 * len(fin), 1024)
+    while True:
+        pairs = np.zeros((min(cap, store), 2), np.int32)
+        fout = np.zeros((room, 2), np.int32)
+        n = c_i32(0)
+        st = CollideStats()
+        lib().fclo_collide_front(ctypes.byref(m1.m), _p(tf1, c_dbl), ctypes.byref(m2.m),
+                                 _p(tf2, c_dbl), cap, len(pairs), _p(pairs, c_i32), _p(fin, c_i32),
+                                 len(fin), _p(fout, c_i32), room, ctypes.byref(n),
+                                 ctypes.byref(st))
+        if n.value <= room and min(st.num_contacts, cap) <= len(pairs):
+            return (pairs[:min(st.num_contacts, cap)].copy(), fout[:n.value].copy(),
+                    (st.num_contacts, st.num_bv_tests, st.num_leaf_tests))
+        room = max(room, n.value)
+        store = max(store, min(st.num_contacts, cap))
 
 
 def collide_batch_flat(meshes, qm, tf1, tf2, cap, threads=None):

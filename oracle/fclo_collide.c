@@ -4,6 +4,7 @@
  * early-stop and contact-cap rules. */
 #include <float.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include "fclo.h"
 
@@ -364,6 +365,130 @@ static void collide_rec(collide_ctx *c, int b1, int b2) {
     if (can_stop(c)) return;
     collide_rec(c, b1, n2->first_child + 1);
   }
+}
+
+/* ---- BVTT front lists (geometry/bvh/detail/BVH_front.h:49-72) ------------
+ * A front is the list of (b1, b2) BVTT nodes where the last traversal
+ * stopped: leaf pairs (recorded before their BV test) and internal pairs
+ * found disjoint.  An entry is invalidated by storing b1 as -(b1 + 1). */
+typedef struct {
+  int *p;  /* 2 ints per entry */
+  int n, cap;
+} front_list;
+
+static void front_add(front_list *f, int b1, int b2) {
+  if (f->n == f->cap) {
+    f->cap = f->cap ? 2 * f->cap : 64;
+    f->p = (int *)realloc(f->p, 2 * (size_t)f->cap * sizeof(int));
+  }
+  f->p[2 * f->n] = b1;
+  f->p[2 * f->n + 1] = b2;
+  f->n++;
+}
+
+static int first_over_second(const collide_ctx *c, int b1, int b2) {
+  const int l1 = c->m1->bvs[b1].first_child < 0, l2 = c->m2->bvs[b2].first_child < 0;
+  return l2 || (!l1 && (fclo_node_size(c->m1, b1) > fclo_node_size(c->m2, b2)));
+}
+
+/* traversal_recurse-inl.h:82-129 (collisionRecurse) with a front list:
+ * every leaf pair is added to the front before its BV test, an internal
+ * pair when it is found disjoint, and there is no early stop (the stop rule
+ * is skipped whenever a front list is given). */
+static void collide_rec_front(collide_ctx *c, int b1, int b2, front_list *f) {
+  const fclo_node *n1 = c->m1->bvs + b1, *n2 = c->m2->bvs + b2;
+  const int l1 = n1->first_child < 0, l2 = n2->first_child < 0;
+  if (l1 && l2) {
+    front_add(f, b1, b2);
+    if (bv_disjoint(c, b1, b2)) return;
+    leaf_test(c, b1, b2);
+    return;
+  }
+  if (bv_disjoint(c, b1, b2)) {
+    front_add(f, b1, b2);
+    return;
+  }
+  if (first_over_second(c, b1, b2)) {
+    collide_rec_front(c, n1->first_child, b2, f);
+    collide_rec_front(c, n1->first_child + 1, b2, f);
+  } else {
+    collide_rec_front(c, b1, n2->first_child, f);
+    collide_rec_front(c, b1, n2->first_child + 1, f);
+  }
+}
+
+/* traversal_recurse-inl.h:463-521 (propagateBVHFrontListCollisionRecurse).
+ * The loop also visits the entries that descents append to the list while
+ * it runs (std::list iteration reaches them): a leaf pair found that way is
+ * tested once more, and its contact counted again.  Afterwards the invalid
+ * entries are erased (the rest keep their order) and the re-recorded leaf
+ * pairs are appended. */
+static void propagate_front(collide_ctx *c, front_list *f) {
+  front_list append = {0, 0, 0};
+  for (int i = 0; i < f->n; ++i) {
+    const int b1 = f->p[2 * i], b2 = f->p[2 * i + 1];
+    const int l1 = c->m1->bvs[b1].first_child < 0, l2 = c->m2->bvs[b2].first_child < 0;
+    if (l1 && l2) {
+      f->p[2 * i] = -(b1 + 1);
+      collide_rec_front(c, b1, b2, &append);
+    } else if (!bv_disjoint(c, b1, b2)) {
+      f->p[2 * i] = -(b1 + 1);
+      const fclo_node *n1 = c->m1->bvs + b1, *n2 = c->m2->bvs + b2;
+      if (first_over_second(c, b1, b2)) {
+        collide_rec_front(c, n1->first_child, b2, f);
+        collide_rec_front(c, n1->first_child + 1, b2, f);
+      } else {
+        collide_rec_front(c, b1, n2->first_child, f);
+        collide_rec_front(c, b1, n2->first_child + 1, f);
+      }
+    }
+  }
+  int k = 0;
+  for (int i = 0; i < f->n; ++i)
+    if (f->p[2 * i] >= 0) {
+      f->p[2 * k] = f->p[2 * i];
+      f->p[2 * k + 1] = f->p[2 * i + 1];
+      ++k;
+    }
+  f->n = k;
+  for (int i = 0; i < append.n; ++i) front_add(f, append.p[2 * i], append.p[2 * i + 1]);
+  free(append.p);
+}
+
+/* detail::collide(node, front_list) (collision_node-inl.h:71-82) with a
+ * non-null front list, for a mesh pair: propagate the front when it holds
+ * entries, else traverse from the roots recording it.  front/n_front: the
+ * front in (2 ints per entry); front_out (room front_room entries) gets the
+ * new front and *n_out its full size.  Contacts are capped at cap (the
+ * traversal goes on: no early stop), their first min(count, store) pairs go
+ * to pairs. */
+void fclo_collide_front(const fclo_model *m1, const double tf1[12], const fclo_model *m2,
+                        const double tf2[12], int cap, int store, int *pairs, const int *front,
+                        int n_front, int *front_out, int front_room, int *n_out,
+                        fclo_collide_stats *stats) {
+  collide_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.m1 = m1;
+  c.m2 = m2;
+  c.cap = cap;
+  c.store = store;
+  c.pairs = pairs;
+  fclo_relative_transform(tf1, tf2, c.R, c.T);
+  front_list f = {0, 0, 0};
+  for (int i = 0; i < n_front; ++i) front_add(&f, front[2 * i], front[2 * i + 1]);
+  if (f.n > 0)
+    propagate_front(&c, &f);
+  else
+    collide_rec_front(&c, 0, 0, &f);
+  *n_out = f.n;
+  for (int i = 0; i < f.n && i < front_room; ++i) {
+    front_out[2 * i] = f.p[2 * i];
+    front_out[2 * i + 1] = f.p[2 * i + 1];
+  }
+  free(f.p);
+  stats->num_contacts = c.count;
+  stats->num_bv_tests = c.bv_tests;
+  stats->num_leaf_tests = c.leaf_tests;
 }
 
 static void collide_store(const fclo_model *m1, const double tf1[12],

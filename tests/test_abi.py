@@ -39,7 +39,8 @@ def test_binding_structs_match_the_header(tmp_path):
                "fclx_distance_batch_args": N.DistanceBatchArgs,
                "fclx_engine_report": N.EngineReport,
                "fclx_shape": N.ShapeRec,
-               "fclx_shape_collide_batch_args": N.ShapeCollideBatchArgs}
+               "fclx_shape_collide_batch_args": N.ShapeCollideBatchArgs,
+               "fclx_collide_front_batch_args": N.FrontBatchArgs}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "fclx.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)

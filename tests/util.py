@@ -171,3 +171,24 @@ def random_rotation(rng):
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def random_transform_pairs(n, extents, delta_trans, delta_rot, seed):
+    """Pose pairs like test::generateRandomTransforms with deltas
+    (test_fcl_utility.h:375-417) from an explicit seed: the first pose as
+    random_transforms draws it, the second with translation offsets uniform
+    in [-delta_trans, delta_trans] and Euler angles offset uniformly in
+    [-delta_rot, delta_rot]."""
+    rng = np.random.default_rng(seed)
+    a_out = np.zeros((n, 12))
+    b_out = np.zeros((n, 12))
+    for i in range(n):
+        t = np.array([rng.uniform(extents[k], extents[k + 3]) for k in range(3)])
+        ang = rng.uniform(0, 2 * np.pi, 3)
+        dt = rng.uniform(-np.asarray(delta_trans, float), np.asarray(delta_trans, float))
+        da = rng.uniform(-delta_rot, delta_rot, 3)
+        a_out[i, :9] = euler_to_matrix(*ang).reshape(9)
+        a_out[i, 9:] = t
+        b_out[i, :9] = euler_to_matrix(*(ang + da)).reshape(9)
+        b_out[i, 9:] = t + dt
+    return a_out, b_out
