@@ -195,6 +195,16 @@ class Context:
                                              ctypes.byref(mid)))
         return mid.value
 
+    def mesh_refit(self, mid, vertices, bottomup=True, stream=None):
+        """endReplaceModel(refit = true, bottomup) on the device with the
+        mesh's vertices replaced by `vertices` ((n,3) float64 device tensor,
+        the mesh's vertex count); returns once the refit is done."""
+        info = self.mesh_info(mid)
+        ptr = _req(vertices, "vertices", "f64", 3 * info["num_vertices"])
+        _on_device(vertices)
+        self._check(self._L.fclx_mesh_refit(self._h, int(mid), ctypes.c_void_p(ptr),
+                                            1 if bottomup else 0, stream))
+
     def mesh_destroy(self, mid):
         self._check(self._L.fclx_mesh_destroy(self._h, mid))
 

@@ -159,6 +159,7 @@ SIGNATURES = {
     "fclx_server_replay": (c_i32, [c_vp, ctypes.POINTER(Request), P_dbl, c_i32, P_dbl, P_dbl]),
     "fclx_server_stats": (c_i32, [c_vp, P_i64, P_i64]),
     "fclx_server_destroy": (c_i32, [c_vp]),
+    "fclx_mesh_refit": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_vp]),
     "fclx_front_set_create": (c_i32, [c_vp, c_i32, c_i32, ctypes.POINTER(c_vp)]),
     "fclx_front_set_destroy": (c_i32, [c_vp]),
     "fclx_front_set_clear": (c_i32, [c_vp, c_vp]),

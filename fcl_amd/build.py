@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfclx.so")
 
-HIP_SOURCES = ["api.hip", "kernels_plain.hip", "kernels_front.hip", "kernels_shape.hip",
+HIP_SOURCES = ["api.hip", "kernels_plain.hip", "kernels_front.hip", "kernels_refit.hip",
+               "kernels_shape.hip",
                "kernels_meshshape.hip",
                "engine.hip", "engine_dist.hip", "host/fcl_api.hip", "host/serve.hip"]
 HOST_SOURCES = ["host/bvh_build.cpp", "host/synth.cpp"]

@@ -35,6 +35,13 @@ struct MeshSlot {
   int32_t* prim_pos = nullptr;
   int64_t bytes = 0;
   MeshDev dev{};
+  // device refit: scratch, the nodes level by level (bottom-up refit) and
+  // whether the device BVs differ from the host build (export reads them)
+  int32_t* refit_order = nullptr;
+  double* refit_cmax = nullptr;
+  int32_t* level_nodes = nullptr;
+  std::vector<int32_t> level_offsets;
+  bool refitted = false;
 };
 
 template <typename T>
@@ -116,6 +123,9 @@ void free_slot(MeshSlot& s) {
   (void)hipFree(s.tris);
   (void)hipFree(s.span);
   (void)hipFree(s.prim_pos);
+  (void)hipFree(s.refit_order);
+  (void)hipFree(s.refit_cmax);
+  (void)hipFree(s.level_nodes);
   s = MeshSlot{};
 }
 
@@ -425,8 +435,95 @@ int fclx_mesh_export_nodes(fclx_context* ctx, int32_t mesh_id, double* bv_data, 
   if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (!valid_mesh(ctx, mesh_id)) return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
-  export_nodes(ctx->meshes[mesh_id].bvh, bv_data, links, sizes);
+  const MeshSlot& m = ctx->meshes[mesh_id];
+  export_nodes(m.bvh, bv_data, links, sizes);
+  if (!m.refitted || (!bv_data && !sizes)) return FCLX_OK;
+  // refit on the device: the BVs and sizes come from device memory
+  (void)hipSetDevice(ctx->device);
+  const size_t nb = m.bvh.nodes.size();
+  std::vector<ObbNode64> o(nb);
+  std::vector<RssNode64> r(nb);
+  HIP_TRY(ctx, hipMemcpy(o.data(), m.obb64, nb * sizeof(ObbNode64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(r.data(), m.rss64, nb * sizeof(RssNode64), hipMemcpyDeviceToHost));
+  if (sizes)
+    HIP_TRY(ctx, hipMemcpy(sizes, m.size64, nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (bv_data)
+    for (size_t i = 0; i < nb; ++i) {
+      double* d = bv_data + 31 * i;
+      std::memcpy(d, o[i].axis, 9 * sizeof(double));
+      std::memcpy(d + 9, o[i].to, 3 * sizeof(double));
+      std::memcpy(d + 12, o[i].extent, 3 * sizeof(double));
+      std::memcpy(d + 15, r[i].axis, 9 * sizeof(double));
+      std::memcpy(d + 24, r[i].to, 3 * sizeof(double));
+      d[27] = r[i].l[0];
+      d[28] = r[i].l[1];
+      d[29] = r[i].r;
+      d[30] = 0;
+    }
   return FCLX_OK;
+}
+
+int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
+                    int32_t bottomup, void* stream) {
+  if (!ctx || !vertices) return FCLX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!valid_mesh(ctx, mesh_id)) return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "bad mesh id");
+  (void)hipSetDevice(ctx->device);
+  MeshSlot& m = ctx->meshes[mesh_id];
+  const auto& nodes = m.bvh.nodes;
+  const int nb = static_cast<int>(nodes.size());
+  const int nt = static_cast<int>(m.bvh.tris.size() / 3);
+  const int nv = static_cast<int>(m.bvh.vertices.size() / 3);
+  if (!m.refit_order) {
+    HIP_TRY(ctx, dmalloc(&m.refit_order, static_cast<size_t>(nt)));
+    HIP_TRY(ctx, dmalloc(&m.refit_cmax, 1));
+  }
+  if (bottomup && !m.level_nodes) {
+    // the nodes level by level from the root
+    std::vector<int32_t> order{0}, offsets{0, 1};
+    for (size_t a = 0; a < order.size();) {
+      const size_t b = order.size();
+      for (size_t i = a; i < b; ++i) {
+        const int fc = nodes[order[i]].first_child;
+        if (fc >= 0) {
+          order.push_back(fc);
+          order.push_back(fc + 1);
+        }
+      }
+      if (order.size() > b) offsets.push_back(static_cast<int32_t>(order.size()));
+      a = b;
+    }
+    HIP_TRY(ctx, dmalloc(&m.level_nodes, order.size()));
+    HIP_TRY(ctx, hipMemcpy(m.level_nodes, order.data(), order.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+    m.level_offsets = offsets;
+  }
+  RefitLaunch L{};
+  L.new_verts = vertices;
+  L.verts = m.verts;
+  L.tri_verts = m.tri_verts;
+  L.tris = m.tris;
+  L.span = m.span;
+  L.prim_pos = m.prim_pos;
+  L.order = m.refit_order;
+  L.obb64 = m.obb64;
+  L.rss64 = m.rss64;
+  L.size64 = m.size64;
+  L.obb32 = m.obb32;
+  L.cmax = m.refit_cmax;
+  L.num_verts = nv;
+  L.num_tris = nt;
+  L.num_bvs = nb;
+  L.bv_type = m.bvh.bv_type;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(ctx, launch_refit(L, bottomup != 0, m.level_nodes, m.level_offsets.data(),
+                            static_cast<int>(m.level_offsets.size()) - 1, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  double cmax = 0;
+  HIP_TRY(ctx, hipMemcpy(&cmax, m.refit_cmax, sizeof(double), hipMemcpyDeviceToHost));
+  m.dev.cmax = cmax;
+  m.refitted = true;
+  return table_sync(ctx);
 }
 
 int fclx_bvh_build(int32_t bv_type, const double* vertices, int32_t num_vertices,

@@ -137,6 +137,30 @@ struct FrontLaunch {
   int32_t* status;
 };
 hipError_t launch_collide_front(const FrontLaunch& L, int stack_need, hipStream_t s);
+
+// Device refit of one mesh (kernels_refit.hip)
+struct RefitLaunch {
+  const double* new_verts;     // 3 per vertex
+  double* verts;               // the mesh's vertex array (replaced)
+  double* tri_verts;           // the mesh's per-triangle vertex records (rebuilt)
+  const int32_t* tris;
+  const int32_t* span;         // 2 per node
+  const int32_t* prim_pos;     // per triangle: position in the build order
+  int32_t* order;              // scratch, per position: the triangle
+  ObbNode64* obb64;
+  RssNode64* rss64;
+  double* size64;
+  ObbNode32* obb32;
+  double* cmax;                // device scalar: largest |OBB centre|_1
+  int32_t num_verts;
+  int32_t num_tris;
+  int32_t num_bvs;
+  int32_t bv_type;
+};
+// bottomup: level_nodes holds the node ids level by level (root level 0),
+// level_offsets_host[l] .. [l+1] (host array of levels + 1) bounding each.
+hipError_t launch_refit(const RefitLaunch& L, bool bottomup, const int32_t* level_nodes,
+                        const int32_t* level_offsets_host, int levels, hipStream_t s);
 // Sort buffers, the EPA queue and per-lane EPA memory of the shape path; one
 // per context, used by one batch at a time (the caller serialises calls;
 // batches on different streams are ordered through an event).

@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include "device/exact64.h"
+#include "device/fit64.h"
 #include "device/shape64.h"
 #include "kernels.h"
 
@@ -41,102 +42,6 @@ constexpr int kEpaBlocksPerCu = 2;
 using shp::D3;
 using shp::mk;
 
-// eigen_old (math/geometry-inl.h:475-558): cyclic Jacobi, at most 50 sweeps;
-// vout(r, k) = v[k][r] as the reference lays it out.  All index loops are
-// unrolled so the matrices stay in registers.
-__device__ void eigen_old(const double m[9], double dout[3], double vout[9]) {
-  double R[3][3];
-  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  double b[3], z[3], d[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) R[i][j] = m[3 * i + j];
-#pragma unroll
-  for (int ip = 0; ip < 3; ++ip) {
-    b[ip] = d[ip] = R[ip][ip];
-    z[ip] = 0;
-  }
-  for (int it = 0; it < 50; ++it) {
-    double sm = 0;
-#pragma unroll
-    for (int ip = 0; ip < 3; ++ip)
-#pragma unroll
-      for (int iq = ip + 1; iq < 3; ++iq) sm += fabs(R[ip][iq]);
-    if (sm == 0.0) break;
-    const double tresh = it < 3 ? 0.2 * sm / (3 * 3) : 0.0;
-#pragma unroll
-    for (int ip = 0; ip < 3; ++ip) {
-#pragma unroll
-      for (int iq = ip + 1; iq < 3; ++iq) {
-        double g = 100.0 * fabs(R[ip][iq]);
-        if (it > 3 && fabs(d[ip]) + g == fabs(d[ip]) && fabs(d[iq]) + g == fabs(d[iq])) {
-          R[ip][iq] = 0.0;
-        } else if (fabs(R[ip][iq]) > tresh) {
-          double h = d[iq] - d[ip];
-          double t;
-          if (fabs(h) + g == fabs(h)) {
-            t = (R[ip][iq]) / h;
-          } else {
-            const double theta = 0.5 * h / (R[ip][iq]);
-            t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
-            if (theta < 0.0) t = -t;
-          }
-          const double c = 1.0 / sqrt(1 + t * t);
-          const double s = t * c;
-          const double tau = s / (1.0 + c);
-          h = t * R[ip][iq];
-          z[ip] -= h;
-          z[iq] += h;
-          d[ip] -= h;
-          d[iq] += h;
-          R[ip][iq] = 0.0;
-#pragma unroll
-          for (int j = 0; j < ip; ++j) {
-            g = R[j][ip];
-            h = R[j][iq];
-            R[j][ip] = g - s * (h + g * tau);
-            R[j][iq] = h + s * (g - h * tau);
-          }
-#pragma unroll
-          for (int j = ip + 1; j < iq; ++j) {
-            g = R[ip][j];
-            h = R[j][iq];
-            R[ip][j] = g - s * (h + g * tau);
-            R[j][iq] = h + s * (g - h * tau);
-          }
-#pragma unroll
-          for (int j = iq + 1; j < 3; ++j) {
-            g = R[ip][j];
-            h = R[iq][j];
-            R[ip][j] = g - s * (h + g * tau);
-            R[iq][j] = h + s * (g - h * tau);
-          }
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            g = v[j][ip];
-            h = v[j][iq];
-            v[j][ip] = g - s * (h + g * tau);
-            v[j][iq] = h + s * (g - h * tau);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int ip = 0; ip < 3; ++ip) {
-      b[ip] += z[ip];
-      d[ip] = b[ip];
-      z[ip] = 0.0;
-    }
-  }
-  // converged, or 50 sweeps done (the accumulator as it stands)
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) vout[3 * r + k] = v[k][r];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) dout[k] = d[k];
-}
 
 __device__ __forceinline__ double pick3(const double* a, int i) {
   return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);

@@ -201,6 +201,19 @@ int fclx_mesh_get_info(fclx_context* ctx, int32_t mesh_id, fclx_mesh_info* info)
 int fclx_mesh_export_nodes(fclx_context* ctx, int32_t mesh_id, double* bv_data,
                            int32_t* links, double* sizes);
 
+/* BVHModel::beginReplaceModel / replaceSubModel(vertices) /
+ * endReplaceModel(refit = true, bottomup) (BVH_model-inl.h:521-620) on the
+ * device: the mesh's vertices are replaced by `vertices` (device memory, 3
+ * doubles per vertex, the mesh's vertex count) and its hierarchy refit in
+ * place, keeping the topology (refitTree, :941-1076).  bottomup = 1 (the
+ * reference's default): leaves fitted to their triangles and parents merged
+ * from their children, level by level; 0: every node fitted to its
+ * triangles as the build does.  Runs on `stream` and returns once done (the
+ * mesh's cull bound is updated on the host); queries on the mesh must not
+ * run meanwhile.  fclx_mesh_export_nodes then reports the refit BVs. */
+int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
+                    int32_t bottomup, void* stream);
+
 /* Host-only BVH build with no device and no context: the same hierarchy
  * fclx_mesh_create uploads, written to caller arrays sized for 2*num_tris-1
  * nodes in the fclx_mesh_export_nodes layout.  *depth may be NULL. */

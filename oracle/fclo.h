@@ -73,6 +73,13 @@ void fclo_inverse_compose(const double tf1[12], const double tf2[12],
 int fclo_model_build(fclo_model *m, int bv_type, const double *vertices,
                      int num_vertices, const int *tris, int num_tris);
 void fclo_model_free(fclo_model *m);
+/* BVHModel::beginReplaceModel / replaceSubModel(vertices) /
+ * endReplaceModel(refit = true, bottomup) (BVH_model-inl.h:521-620): the
+ * vertices replaced (same count), the hierarchy refit keeping its topology
+ * -- top-down refits every node as the build fits it, bottom-up fits each
+ * leaf's triangle (fit3) and merges children (operator+) up to the root
+ * (refitTree, :941-1076). */
+int fclo_model_refit(fclo_model *m, const double *vertices, int bottomup);
 int fclo_model_depth(const fclo_model *m);
 double fclo_node_size(const fclo_model *m, int b);
 

@@ -66,6 +66,7 @@ def lib():
             "fclo_inverse_compose": (None, [P_dbl, P_dbl, P_dbl]),
             "fclo_model_build": (c_i32, [PM, c_i32, P_dbl, c_i32, P_i32, c_i32]),
             "fclo_model_free": (None, [PM]),
+            "fclo_model_refit": (c_i32, [PM, P_dbl, c_i32]),
             "fclo_model_depth": (c_i32, [PM]),
             "fclo_node_size": (c_dbl, [PM, c_i32]),
             "fclo_obb_disjoint": (c_i32, [P_dbl, P_dbl, P_dbl, P_dbl]),
@@ -264,6 +265,15 @@ class OracleMesh:
 
     def depth(self):
         return lib().fclo_model_depth(ctypes.byref(self.m))
+
+    def refit(self, V, bottomup=True):
+        """endReplaceModel(refit = true, bottomup) after replacing the
+        vertices by V (same count)."""
+        V = _d(V)
+        if V.shape != self.V.shape:
+            raise ValueError("a replaced model keeps its vertex count")
+        self.V = V
+        lib().fclo_model_refit(ctypes.byref(self.m), _p(V, c_dbl), 1 if bottomup else 0)
 
     def nodes(self):
         """(bv[nodes,31], links[nodes,3], sizes[nodes]) in the layout of

@@ -35,11 +35,13 @@ def _order(links):
     return np.array(order)
 
 
-def violations(V, T, bv, links, gap=1e-6, tie=1e-9):
+def violations(V, T, bv, links, gap=1e-6, tie=1e-9, any_sign=False):
     """Rule violations of a built hierarchy (bv: nodes x >= 15, OBB axis
     row-major, centre, extent; links: first_child, first_primitive,
     num_primitives).  Returns a list of strings (empty when every rule
-    holds)."""
+    holds).  any_sign: the left child may hold the centroids above the mean
+    instead (a refit's eigenvector can point the other way than the
+    build's, which fixed the order of the children)."""
     out = []
     order = _order(links)
     scale = max(np.abs(V).max(), 1.0)
@@ -74,7 +76,7 @@ def violations(V, T, bv, links, gap=1e-6, tie=1e-9):
                 out.append("node %d: one-sided split not halved" % i)
             continue
         want = set(prims[below].tolist())
-        if left != want:
+        if left != want and not (any_sign and left == set(prims[~below].tolist())):
             out.append("node %d: children do not split at the mean" % i)
     return out
 
