@@ -426,6 +426,20 @@ class Context:
             N.ptr(dep, N.c_dbl), ctypes.byref(n), N.ptr(stats, N.c_i32)))
         return [(nrm[i].copy(), pos[i].copy(), float(dep[i])) for i in range(n.value)], tuple(stats)
 
+    def broadphase_pairs_async(self, aabbs, pairs, num_pairs, groups=None, stream=None):
+        """fclx_broadphase_pairs: aabbs (n,6) float64 device tensor (lower
+        xyz, upper xyz); pairs (cap,2) int32; num_pairs (1,) int64."""
+        n = int(aabbs.shape[0])
+        a = _req(aabbs, "aabbs", "f64", 6 * n)
+        g = _req(groups, "groups", "i32", n, optional=True)
+        p = _req(pairs, "pairs", "i32", 0)
+        cnt = _req(num_pairs, "num_pairs", "i64", 1)
+        _on_device(aabbs, groups, pairs, num_pairs)
+        self._check(self._L.fclx_broadphase_pairs(self._h, n, ctypes.c_void_p(a),
+                                                  ctypes.c_void_p(g) if g else None,
+                                                  ctypes.c_void_p(p), int(pairs.numel() // 2),
+                                                  ctypes.c_void_p(cnt), stream))
+
     def set_profiling(self, enable=True):
         self._check(self._L.fclx_set_profiling(self._h, 1 if enable else 0))
 

@@ -160,6 +160,7 @@ SIGNATURES = {
     "fclx_server_stats": (c_i32, [c_vp, P_i64, P_i64]),
     "fclx_server_destroy": (c_i32, [c_vp]),
     "fclx_mesh_refit": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "fclx_broadphase_pairs": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "fclx_front_set_create": (c_i32, [c_vp, c_i32, c_i32, ctypes.POINTER(c_vp)]),
     "fclx_front_set_destroy": (c_i32, [c_vp]),
     "fclx_front_set_clear": (c_i32, [c_vp, c_vp]),

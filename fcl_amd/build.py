@@ -19,6 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfclx.so")
 
 HIP_SOURCES = ["api.hip", "kernels_plain.hip", "kernels_front.hip", "kernels_refit.hip",
+               "kernels_broadphase.hip",
                "kernels_shape.hip",
                "kernels_meshshape.hip",
                "engine.hip", "engine_dist.hip", "host/fcl_api.hip", "host/serve.hip"]

@@ -1140,6 +1140,21 @@ int fclx_collide_front_batch(fclx_context* ctx, const fclx_collide_front_batch_a
   return FCLX_OK;
 }
 
+int fclx_broadphase_pairs(fclx_context* ctx, int32_t num_objects, const double* aabbs,
+                          const int32_t* groups, int32_t* pairs, int64_t pair_capacity,
+                          int64_t* num_pairs, void* stream) {
+  if (!ctx || num_objects < 0 || !num_pairs || pair_capacity < 0 ||
+      (num_objects > 0 && !aabbs) || (pair_capacity > 0 && !pairs) ||
+      pair_capacity > (int64_t(1) << 31) - 1)
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid broadphase arguments");
+  (void)hipSetDevice(ctx->device);
+  const hipError_t e = launch_broadphase(num_objects, aabbs, groups, pairs, pair_capacity,
+                                         reinterpret_cast<long long*>(num_pairs),
+                                         static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(ctx, e, "broadphase launch");
+  return FCLX_OK;
+}
+
 int fclx_sync(fclx_context* ctx, void* stream) {
   if (!ctx) return FCLX_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);

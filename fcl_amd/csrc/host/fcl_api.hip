@@ -516,4 +516,100 @@ std::size_t collideBatch(const std::vector<CollideQuery>& queries,
   return ret;
 }
 
+namespace detail {
+
+bool broadphase_pairs(const std::vector<AABB<double>>& boxes, const std::vector<int32_t>* groups,
+                      std::vector<std::pair<int32_t, int32_t>>& out) {
+  out.clear();
+  const size_t n = boxes.size();
+  if (n < 2) return true;
+  fclx_context* ctx = engine();
+  if (!ctx) return false;
+  std::vector<double> b(6 * n);
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      b[6 * i + k] = boxes[i].min_[k];
+      b[6 * i + 3 + k] = boxes[i].max_[k];
+    }
+  double* d_b = nullptr;
+  int32_t *d_g = nullptr, *d_p = nullptr;
+  int64_t* d_n = nullptr;
+  int64_t cap = std::max<int64_t>(1024, 8 * static_cast<int64_t>(n)), found = 0;
+  bool ok = hipMalloc(&d_b, b.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc(&d_n, sizeof(int64_t)) == hipSuccess &&
+            hipMemcpy(d_b, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice) ==
+                hipSuccess;
+  if (ok && groups)
+    ok = hipMalloc(&d_g, n * sizeof(int32_t)) == hipSuccess &&
+         hipMemcpy(d_g, groups->data(), n * sizeof(int32_t), hipMemcpyHostToDevice) ==
+             hipSuccess;
+  for (int attempt = 0; ok && attempt < 2; ++attempt) {
+    (void)hipFree(d_p);
+    d_p = nullptr;
+    ok = hipMalloc(&d_p, 2 * cap * sizeof(int32_t)) == hipSuccess &&
+         fclx_broadphase_pairs(ctx, static_cast<int32_t>(n), d_b, d_g, d_p, cap, d_n, nullptr) ==
+             FCLX_OK &&
+         hipStreamSynchronize(nullptr) == hipSuccess &&
+         hipMemcpy(&found, d_n, sizeof(int64_t), hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok || found <= cap) break;
+    cap = found;  // room for every pair, then once more
+  }
+  if (ok && found > cap) ok = false;
+  if (ok && found > 0) {
+    std::vector<int32_t> p(2 * static_cast<size_t>(found));
+    ok = hipMemcpy(p.data(), d_p, p.size() * sizeof(int32_t), hipMemcpyDeviceToHost) ==
+         hipSuccess;
+    for (int64_t k = 0; ok && k < found; ++k) out.emplace_back(p[2 * k], p[2 * k + 1]);
+  }
+  (void)hipFree(d_b);
+  (void)hipFree(d_g);
+  (void)hipFree(d_p);
+  (void)hipFree(d_n);
+  if (!ok) std::cerr << "fcl: device broadphase failed: " << fclx_last_error(ctx) << "\n";
+  return ok;
+}
+
+void default_collide_pairs(const std::vector<ObjectPair>& pairs, DefaultCollisionData<double>* data) {
+  if (data->done || pairs.empty()) return;
+  const CollisionRequest<double>& req = data->request;
+  // the mesh pairs' narrow phase in one batch (each pair as if its result
+  // were empty, capped at num_max_contacts)
+  std::vector<CollideQuery> qs;
+  std::vector<int> batched(pairs.size(), -1);
+  if (!req.enable_contact && !req.enable_cost && req.num_max_contacts > 0) {
+    for (size_t k = 0; k < pairs.size(); ++k) {
+      const auto* g1 = pairs[k].first->collisionGeometry().get();
+      const auto* g2 = pairs[k].second->collisionGeometry().get();
+      if (mesh_id_of(g1, BV_OBBRSS) >= 0 && mesh_id_of(g2, BV_OBBRSS) >= 0) {
+        batched[k] = static_cast<int>(qs.size());
+        qs.push_back({g1, pairs[k].first->getTransform(), g2, pairs[k].second->getTransform()});
+      }
+    }
+  }
+  std::vector<CollisionResult<double>> res;
+  if (!qs.empty() && collideBatch(qs, req, res) != qs.size()) {
+    res.clear();
+    std::fill(batched.begin(), batched.end(), -1);
+  }
+  // fold in pair order, exactly as DefaultCollisionFunction pair by pair:
+  // a pair adds its contacts until the shared result holds num_max_contacts
+  for (size_t k = 0; k < pairs.size() && !data->done; ++k) {
+    const int q = batched[k];
+    const std::size_t have = data->result.numContacts();
+    if (q >= 0 && have + res[q].numContacts() < req.num_max_contacts) {
+      // the whole list fits: the same contacts as collide() would add
+      for (const auto& c : res[q].contacts) data->result.addContact(c);
+    } else if (q < 0 || res[q].numContacts() > 0) {
+      // a list the cap cuts (or geometry outside the batch): collide()
+      // itself, which adds the reference's depth-first prefix
+      DefaultCollisionFunction(pairs[k].first, pairs[k].second, data);
+      continue;
+    }
+    if (data->result.isCollision() && data->result.numContacts() >= req.num_max_contacts)
+      data->done = true;
+  }
+}
+
+}  // namespace detail
+
 }  // namespace fcl

@@ -138,6 +138,13 @@ struct FrontLaunch {
 };
 hipError_t launch_collide_front(const FrontLaunch& L, int stack_need, hipStream_t s);
 
+// Broadphase pairs of overlapping world AABBs (kernels_broadphase.hip):
+// aabbs 6 doubles per object (lower x y z, upper x y z); groups optional
+// (pairs only across groups); pairs 2 per pair, ascending (lower, higher);
+// *num_pairs = all pairs found (those beyond capacity are not stored).
+hipError_t launch_broadphase(int n, const double* aabbs, const int32_t* groups, int32_t* pairs,
+                             long long capacity, long long* num_pairs, hipStream_t s);
+
 // Device refit of one mesh (kernels_refit.hip)
 struct RefitLaunch {
   const double* new_verts;     // 3 per vertex

@@ -21,11 +21,13 @@
 #ifndef FCL_AMD_FCL_H
 #define FCL_AMD_FCL_H
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
 #include <memory>
+#include <utility>
 #include <vector>
 
 namespace fcl {
@@ -131,6 +133,41 @@ using Vector3d = Vector3<double>;
 using Matrix3d = Matrix3<double>;
 using Transform3d = Transform3<double>;
 
+// Axis-aligned box (math/bv/AABB.h): the broadphase's bounding volume.
+// An empty box (the default) has min_ above max_, so adding a point sets it.
+template <typename S>
+struct AABB {
+  Vector3<S> min_, max_;
+  AABB()
+      : min_(std::numeric_limits<S>::max(), std::numeric_limits<S>::max(),
+             std::numeric_limits<S>::max()),
+        max_(-std::numeric_limits<S>::max(), -std::numeric_limits<S>::max(),
+             -std::numeric_limits<S>::max()) {}
+  AABB(const Vector3<S>& a, const Vector3<S>& b) : min_(a), max_(b) {}
+  // AABB-inl.h: closed intervals, touching boxes overlap
+  bool overlap(const AABB& o) const {
+    for (int k = 0; k < 3; ++k)
+      if (min_[k] > o.max_[k] || max_[k] < o.min_[k]) return false;
+    return true;
+  }
+  AABB& operator+=(const Vector3<S>& p) {
+    for (int k = 0; k < 3; ++k) {
+      if (p[k] < min_[k]) min_[k] = p[k];
+      if (p[k] > max_[k]) max_[k] = p[k];
+    }
+    return *this;
+  }
+  Vector3<S> center() const { return (min_ + max_) * S(0.5); }
+  S width() const { return max_[0] - min_[0]; }
+  S height() const { return max_[1] - min_[1]; }
+  S depth() const { return max_[2] - min_[2]; }
+};
+template <typename S>
+AABB<S> translate(const AABB<S>& a, const Vector3<S>& t) {
+  return AABB<S>(a.min_ + t, a.max_ + t);
+}
+using AABBd = AABB<double>;
+
 // -------------------------------------------------------------- geometry
 enum OBJECT_TYPE { OT_UNKNOWN, OT_BVH, OT_GEOM, OT_OCTREE, OT_COUNT };
 enum NODE_TYPE {
@@ -157,9 +194,23 @@ class CollisionGeometry {
   virtual ~CollisionGeometry() = default;
   virtual OBJECT_TYPE getObjectType() const { return OT_UNKNOWN; }
   virtual NODE_TYPE getNodeType() const { return BV_UNKNOWN; }
+  // collision_geometry.h: the local AABB, its centre and the radius of the
+  // sphere about that centre holding the geometry
+  virtual void computeLocalAABB() {}
   void* getUserData() const { return user_data; }
   void setUserData(void* d) { user_data = d; }
   void* user_data = nullptr;
+  Vector3<S> aabb_center;
+  S aabb_radius = 1;
+  AABB<S> aabb_local;
+
+ protected:
+  // centre and radius from aabb_local (shape_base-inl.h / computeLocalAABB
+  // of the shapes)
+  void setCenterAndRadius() {
+    aabb_center = aabb_local.center();
+    aabb_radius = (aabb_local.min_ - aabb_center).norm();
+  }
 };
 
 // Bounding-volume tags of the two supported hierarchies.
@@ -287,6 +338,21 @@ class BVHModel : public CollisionGeometry<typename BV::S> {
     return BVH_OK;
   }
 
+  // BVH_model-inl.h computeLocalAABB: the vertices' box, its centre, and
+  // the largest distance of a vertex from that centre
+  void computeLocalAABB() override {
+    AABB<S> box;
+    for (const auto& v : verts_) box += v;
+    this->aabb_center = box.center();
+    S r = 0;
+    for (const auto& v : verts_) {
+      const S d = (this->aabb_center - v).squaredNorm();
+      if (d > r) r = d;
+    }
+    this->aabb_radius = std::sqrt(r);
+    this->aabb_local = box;
+  }
+
   // device handle of the built model (-1 before endModel)
   int deviceMeshId() const { return dev_ ? detail::device_mesh_id(dev_.get()) : -1; }
 
@@ -331,6 +397,11 @@ class Box : public ShapeBase<S_> {
   explicit Box(const Vector3<S>& side_) : side(side_) {}
   Box() = default;
   NODE_TYPE getNodeType() const override { return GEOM_BOX; }
+  void computeLocalAABB() override {
+    const Vector3<S> h = side * S(0.5);
+    this->aabb_local = AABB<S>(-h, h);
+    this->setCenterAndRadius();
+  }
   Vector3<S> side;
 };
 
@@ -340,6 +411,12 @@ class Sphere : public ShapeBase<S_> {
   using S = S_;
   explicit Sphere(S radius_) : radius(radius_) {}
   NODE_TYPE getNodeType() const override { return GEOM_SPHERE; }
+  void computeLocalAABB() override {
+    this->aabb_local = AABB<S>(Vector3<S>(-radius, -radius, -radius),
+                               Vector3<S>(radius, radius, radius));
+    this->aabb_center = Vector3<S>();
+    this->aabb_radius = radius;
+  }
   S radius;
 };
 
@@ -349,6 +426,11 @@ class Capsule : public ShapeBase<S_> {
   using S = S_;
   Capsule(S radius_, S lz_) : radius(radius_), lz(lz_) {}
   NODE_TYPE getNodeType() const override { return GEOM_CAPSULE; }
+  void computeLocalAABB() override {
+    const Vector3<S> h(radius, radius, S(0.5) * lz + radius);
+    this->aabb_local = AABB<S>(-h, h);
+    this->setCenterAndRadius();
+  }
   S radius;
   S lz;
 };
@@ -359,6 +441,11 @@ class Cylinder : public ShapeBase<S_> {
   using S = S_;
   Cylinder(S radius_, S lz_) : radius(radius_), lz(lz_) {}
   NODE_TYPE getNodeType() const override { return GEOM_CYLINDER; }
+  void computeLocalAABB() override {
+    const Vector3<S> h(radius, radius, S(0.5) * lz);
+    this->aabb_local = AABB<S>(-h, h);
+    this->setCenterAndRadius();
+  }
   S radius;
   S lz;
 };
@@ -373,6 +460,12 @@ class Convex : public ShapeBase<S_> {
     (void)throw_if_invalid;
   }
   NODE_TYPE getNodeType() const override { return GEOM_CONVEX; }
+  void computeLocalAABB() override {
+    AABB<S> box;
+    for (const auto& v : *vertices_) box += v;
+    this->aabb_local = box;
+    this->setCenterAndRadius();
+  }
   const std::vector<Vector3<S>>& getVertices() const { return *vertices_; }
   int getFaceCount() const { return num_faces_; }
   const std::vector<int>& getFaces() const { return *faces_; }
@@ -519,18 +612,43 @@ class DistanceResult {
 template <typename S>
 class CollisionObject {
  public:
+  // collision_object-inl.h:63-86: the geometry's local AABB, then the world one
   explicit CollisionObject(const std::shared_ptr<CollisionGeometry<S>>& geom,
                            const Transform3<S>& tf = Transform3<S>::Identity())
-      : geom_(geom), tf_(tf) {}
+      : geom_(geom), tf_(tf) {
+    geom_->computeLocalAABB();
+    computeAABB();
+  }
   const std::shared_ptr<CollisionGeometry<S>>& collisionGeometry() const { return geom_; }
   const Transform3<S>& getTransform() const { return tf_; }
+  const Matrix3<S>& getRotation() const { return tf_.linear(); }
+  const Vector3<S>& getTranslation() const { return tf_.translation(); }
+  // as in the reference, the world AABB follows a new pose only after
+  // computeAABB()
   void setTransform(const Transform3<S>& tf) { tf_ = tf; }
   OBJECT_TYPE getObjectType() const { return geom_->getObjectType(); }
   NODE_TYPE getNodeType() const { return geom_->getNodeType(); }
+  const AABB<S>& getAABB() const { return aabb_; }
+  // collision_object-inl.h:118-131: the local box moved by the translation
+  // when the rotation is the identity, else the cube of the geometry's
+  // bounding radius about its moved centre
+  void computeAABB() {
+    if (tf_.linear() == Matrix3<S>::Identity()) {
+      aabb_ = translate(geom_->aabb_local, tf_.translation());
+    } else {
+      const Vector3<S> c = tf_ * geom_->aabb_center;
+      const Vector3<S> d(geom_->aabb_radius, geom_->aabb_radius, geom_->aabb_radius);
+      aabb_ = AABB<S>(c - d, c + d);
+    }
+  }
+  void* getUserData() const { return user_data_; }
+  void setUserData(void* d) { user_data_ = d; }
 
  private:
   std::shared_ptr<CollisionGeometry<S>> geom_;
   Transform3<S> tf_;
+  AABB<S> aabb_;
+  void* user_data_ = nullptr;
 };
 
 // ------------------------------------------------------------- queries
@@ -581,6 +699,248 @@ struct CollideQuery {
 std::size_t collideBatch(const std::vector<CollideQuery>& queries,
                          const CollisionRequest<double>& request,
                          std::vector<CollisionResult<double>>& results);
+
+// ------------------------------------------------------------ broadphase
+// broadphase_collision_manager.h:57-66: a callback gets two objects and the
+// caller's data, and returns true to stop the broadphase.
+template <typename S>
+using CollisionCallBack = bool (*)(CollisionObject<S>* o1, CollisionObject<S>* o2, void* cdata);
+template <typename S>
+using DistanceCallBack = bool (*)(CollisionObject<S>* o1, CollisionObject<S>* o2, void* cdata,
+                                  S& dist);
+
+// default_broadphase_callbacks.h:60-103: collide() into one shared result
+// until the request is satisfied
+template <typename S>
+struct DefaultCollisionData {
+  CollisionRequest<S> request;
+  CollisionResult<S> result;
+  bool done{false};
+};
+template <typename S>
+bool DefaultCollisionFunction(CollisionObject<S>* o1, CollisionObject<S>* o2, void* data) {
+  auto* collision_data = static_cast<DefaultCollisionData<S>*>(data);
+  const CollisionRequest<S>& request = collision_data->request;
+  CollisionResult<S>& result = collision_data->result;
+  if (collision_data->done) return true;
+  collide(o1, o2, request, result);
+  if (!request.enable_cost && result.isCollision() &&
+      result.numContacts() >= request.num_max_contacts)
+    collision_data->done = true;
+  return collision_data->done;
+}
+
+// default_broadphase_callbacks.h:177-228: distance() into one shared result;
+// a pair in touch or collision stops the broadphase
+template <typename S>
+struct DefaultDistanceData {
+  DistanceRequest<S> request;
+  DistanceResult<S> result;
+  bool done{false};
+};
+template <typename S>
+bool DefaultDistanceFunction(CollisionObject<S>* o1, CollisionObject<S>* o2, void* data,
+                             S& dist) {
+  auto* cdata = static_cast<DefaultDistanceData<S>*>(data);
+  const DistanceRequest<S>& request = cdata->request;
+  DistanceResult<S>& result = cdata->result;
+  if (cdata->done) {
+    dist = result.min_distance;
+    return true;
+  }
+  distance(o1, o2, request, result);
+  dist = result.min_distance;
+  if (dist <= 0) return true;
+  return cdata->done;
+}
+
+// broadphase_collision_manager.h:77-122
+template <typename S>
+class BroadPhaseCollisionManager {
+ public:
+  virtual ~BroadPhaseCollisionManager() = default;
+  virtual void registerObjects(const std::vector<CollisionObject<S>*>& other_objs) {
+    for (auto* o : other_objs) registerObject(o);
+  }
+  virtual void registerObject(CollisionObject<S>* obj) = 0;
+  virtual void unregisterObject(CollisionObject<S>* obj) = 0;
+  virtual void setup() = 0;
+  virtual void update() = 0;
+  virtual void update(CollisionObject<S>* updated_obj) {
+    (void)updated_obj;
+    update();
+  }
+  virtual void update(const std::vector<CollisionObject<S>*>& updated_objs) {
+    (void)updated_objs;
+    update();
+  }
+  virtual void clear() = 0;
+  virtual void getObjects(std::vector<CollisionObject<S>*>& objs) const = 0;
+  virtual void collide(CollisionObject<S>* obj, void* cdata, CollisionCallBack<S> callback) const = 0;
+  virtual void distance(CollisionObject<S>* obj, void* cdata, DistanceCallBack<S> callback) const = 0;
+  virtual void collide(void* cdata, CollisionCallBack<S> callback) const = 0;
+  virtual void distance(void* cdata, DistanceCallBack<S> callback) const = 0;
+  virtual void collide(BroadPhaseCollisionManager* other_manager, void* cdata,
+                       CollisionCallBack<S> callback) const = 0;
+  virtual void distance(BroadPhaseCollisionManager* other_manager, void* cdata,
+                        DistanceCallBack<S> callback) const = 0;
+  virtual bool empty() const = 0;
+  virtual std::size_t size() const = 0;
+};
+
+namespace detail {
+using ObjectPair = std::pair<CollisionObject<double>*, CollisionObject<double>*>;
+// Pairs (i, j), i < j, of boxes that overlap (AABB::overlap), ascending,
+// found on the device; with groups only pairs across groups.  Returns false
+// when the device call fails.
+bool broadphase_pairs(const std::vector<AABB<double>>& boxes, const std::vector<int32_t>* groups,
+                      std::vector<std::pair<int32_t, int32_t>>& out);
+// DefaultCollisionFunction over the pairs in order, with the mesh pairs'
+// narrow phase run as one device batch: the same result as calling it pair
+// by pair.
+void default_collide_pairs(const std::vector<ObjectPair>& pairs, DefaultCollisionData<double>* data);
+}  // namespace detail
+
+// A BroadPhaseCollisionManager whose candidate pairs come from one device
+// sort-and-sweep over the objects' world AABBs (the boxes taken at setup()
+// and update(), as the reference's managers take them), handed to the
+// callback in ascending (i, j) registration order.  With
+// DefaultCollisionFunction the mesh pairs' narrow phase runs as one batch.
+// Distance queries visit pairs in ascending AABB distance and stop once the
+// callback's distance is at most the next pair's AABB distance.  The name of
+// FCL's default manager (broadphase_dynamic_AABB_tree.h) is kept for
+// drop-in use; the answers (contact counts of exhaustive requests, collision
+// flags, minimum distances) are the same, the visiting order is not.
+template <typename S>
+class DynamicAABBTreeCollisionManager : public BroadPhaseCollisionManager<S> {
+ public:
+  void registerObject(CollisionObject<S>* obj) override {
+    objs_.push_back(obj);
+    boxes_.push_back(obj->getAABB());
+  }
+  void unregisterObject(CollisionObject<S>* obj) override {
+    for (std::size_t i = 0; i < objs_.size(); ++i)
+      if (objs_[i] == obj) {
+        objs_.erase(objs_.begin() + static_cast<std::ptrdiff_t>(i));
+        boxes_.erase(boxes_.begin() + static_cast<std::ptrdiff_t>(i));
+        return;
+      }
+  }
+  void setup() override { update(); }
+  void update() override {
+    for (std::size_t i = 0; i < objs_.size(); ++i) boxes_[i] = objs_[i]->getAABB();
+  }
+  void clear() override {
+    objs_.clear();
+    boxes_.clear();
+  }
+  void getObjects(std::vector<CollisionObject<S>*>& objs) const override { objs = objs_; }
+  bool empty() const override { return objs_.empty(); }
+  std::size_t size() const override { return objs_.size(); }
+
+  void collide(void* cdata, CollisionCallBack<S> callback) const override {
+    std::vector<std::pair<int32_t, int32_t>> ij;
+    if (!detail::broadphase_pairs(boxes_, nullptr, ij)) return;
+    std::vector<detail::ObjectPair> ps;
+    for (const auto& p : ij) ps.emplace_back(objs_[p.first], objs_[p.second]);
+    run(ps, cdata, callback);
+  }
+  void collide(CollisionObject<S>* obj, void* cdata, CollisionCallBack<S> callback) const override {
+    std::vector<AABB<S>> boxes{obj->getAABB()};
+    boxes.insert(boxes.end(), boxes_.begin(), boxes_.end());
+    std::vector<int32_t> groups(boxes.size(), 1);
+    groups[0] = 0;
+    std::vector<std::pair<int32_t, int32_t>> ij;
+    if (!detail::broadphase_pairs(boxes, &groups, ij)) return;
+    std::vector<detail::ObjectPair> ps;
+    for (const auto& p : ij)
+      if (objs_[p.second - 1] != obj) ps.emplace_back(obj, objs_[p.second - 1]);
+    run(ps, cdata, callback);
+  }
+  void collide(BroadPhaseCollisionManager<S>* other_manager, void* cdata,
+               CollisionCallBack<S> callback) const override {
+    std::vector<CollisionObject<S>*> other;
+    other_manager->getObjects(other);
+    std::vector<AABB<S>> boxes = boxes_;
+    for (auto* o : other) boxes.push_back(o->getAABB());
+    std::vector<int32_t> groups(boxes.size(), 1);
+    for (std::size_t i = 0; i < objs_.size(); ++i) groups[i] = 0;
+    std::vector<std::pair<int32_t, int32_t>> ij;
+    if (!detail::broadphase_pairs(boxes, &groups, ij)) return;
+    const int32_t n = static_cast<int32_t>(objs_.size());
+    std::vector<detail::ObjectPair> ps;
+    for (const auto& p : ij) ps.emplace_back(objs_[p.first], other[p.second - n]);
+    run(ps, cdata, callback);
+  }
+
+  void distance(void* cdata, DistanceCallBack<S> callback) const override {
+    std::vector<detail::ObjectPair> ps;
+    for (std::size_t i = 0; i < objs_.size(); ++i)
+      for (std::size_t j = i + 1; j < objs_.size(); ++j) ps.emplace_back(objs_[i], objs_[j]);
+    run_distance(ps, cdata, callback);
+  }
+  void distance(CollisionObject<S>* obj, void* cdata, DistanceCallBack<S> callback) const override {
+    std::vector<detail::ObjectPair> ps;
+    for (auto* o : objs_)
+      if (o != obj) ps.emplace_back(obj, o);
+    run_distance(ps, cdata, callback);
+  }
+  void distance(BroadPhaseCollisionManager<S>* other_manager, void* cdata,
+                DistanceCallBack<S> callback) const override {
+    std::vector<CollisionObject<S>*> other;
+    other_manager->getObjects(other);
+    std::vector<detail::ObjectPair> ps;
+    for (auto* a : objs_)
+      for (auto* b : other) ps.emplace_back(a, b);
+    run_distance(ps, cdata, callback);
+  }
+
+ private:
+  static void run(const std::vector<detail::ObjectPair>& ps, void* cdata,
+                  CollisionCallBack<S> callback) {
+    if (callback == &DefaultCollisionFunction<S>) {
+      detail::default_collide_pairs(ps, static_cast<DefaultCollisionData<S>*>(cdata));
+      return;
+    }
+    for (const auto& p : ps)
+      if (callback(p.first, p.second, cdata)) return;
+  }
+  static S box_distance(const AABB<S>& a, const AABB<S>& b) {
+    S d = 0;
+    for (int k = 0; k < 3; ++k) {
+      const S g = std::max(a.min_[k] - b.max_[k], b.min_[k] - a.max_[k]);
+      if (g > 0) d += g * g;
+    }
+    return std::sqrt(d);
+  }
+  static void run_distance(std::vector<detail::ObjectPair> ps, void* cdata,
+                           DistanceCallBack<S> callback) {
+    std::vector<std::pair<S, std::size_t>> order;
+    for (std::size_t k = 0; k < ps.size(); ++k)
+      order.emplace_back(box_distance(ps[k].first->getAABB(), ps[k].second->getAABB()), k);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<S, std::size_t>& a, const std::pair<S, std::size_t>& b) {
+                       return a.first < b.first;
+                     });
+    S dist = std::numeric_limits<S>::max();
+    for (const auto& o : order) {
+      if (o.first >= dist) return;  // no later pair can come closer
+      if (callback(ps[o.second].first, ps[o.second].second, cdata, dist)) return;
+    }
+  }
+  std::vector<CollisionObject<S>*> objs_;
+  std::vector<AABB<S>> boxes_;
+};
+using DynamicAABBTreeCollisionManagerd = DynamicAABBTreeCollisionManager<double>;
+using BroadPhaseCollisionManagerd = BroadPhaseCollisionManager<double>;
+using DefaultCollisionDatad = DefaultCollisionData<double>;
+using DefaultDistanceDatad = DefaultDistanceData<double>;
+using CollisionObjectd = CollisionObject<double>;
+using CollisionGeometryd = CollisionGeometry<double>;
+using CollisionRequestd = CollisionRequest<double>;
+using CollisionResultd = CollisionResult<double>;
+using DistanceRequestd = DistanceRequest<double>;
+using DistanceResultd = DistanceResult<double>;
 
 }  // namespace fcl
 

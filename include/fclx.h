@@ -475,6 +475,24 @@ typedef struct {
 int fclx_collide_front_batch(fclx_context* ctx, const fclx_collide_front_batch_args* args,
                              fclx_front_set* fronts, void* stream);
 
+/* -------------------------------------------------------------- broadphase
+ * The candidate pairs of a broadphase manager (BroadPhaseCollisionManager,
+ * broadphase/broadphase_collision_manager.h:77-122, whose collide() hands
+ * every pair of objects with overlapping AABBs to its callback): all pairs
+ * (i, j), i < j, of objects whose world AABBs overlap (AABB::overlap,
+ * closed intervals, math/bv/AABB-inl.h), by sort and sweep on the device.
+ * aabbs: device, 6 doubles per object (lower x, y, z, upper x, y, z);
+ * groups: device, optional, a group id per object -- only pairs across
+ * groups are reported (a query set against a scene, or two managers);
+ * pairs: device, 2 int32 per pair, sorted ascending by (i, j);
+ * *num_pairs (device) receives the number of overlapping pairs -- when it
+ * exceeds pair_capacity (at most 2^31 - 1) only that many are stored (which
+ * ones is unspecified) and the call is repeated with more room.
+ * Asynchronous on the stream. */
+int fclx_broadphase_pairs(fclx_context* ctx, int32_t num_objects, const double* aabbs,
+                          const int32_t* groups, int32_t* pairs, int64_t pair_capacity,
+                          int64_t* num_pairs, void* stream);
+
 /* ---------------------------------------------------------------- serving
  * A request loop over one resident scene (no counterpart in the reference,
  * whose collide() is one synchronous call): planner requests -- a robot's

@@ -29,3 +29,13 @@ def test_cpp_shape_api():
                        timeout=240)
     print(r.stdout)
     assert r.returncode == 0 and "PASS" in r.stdout
+
+
+def test_cpp_broadphase():
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "test_broadphase")
+    assert os.path.exists(exe), "build the C++ tests first (__graft_entry__.build())"
+    print("case: C++ broadphase manager over the device sweep", flush=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0 and "PASS" in r.stdout
