@@ -1053,368 +1053,392 @@ __device__ inline bool sphere_cylinder(const Shape& sp, const Xf& X_FS, const Sh
   return true;
 }
 
-__device__ inline void line_closest(const D3& pa, const D3& ua, const D3& pb, const D3& ub,
-                                    double& alpha, double& beta) {
-  const D3 p = sub(pb, pa);
-  const double uaub = dot(ua, ub);
-  const double q1 = dot(ua, p);
-  const double q2 = -dot(ub, p);
-  double d = 1 - uaub * uaub;
-  if (d <= static_cast<double>(0.0001f)) {
-    alpha = 0;
-    beta = 0;
-  } else {
-    d = 1 / d;
-    alpha = (q1 + uaub * q2) * d;
-    beta = (uaub * q1 + q2) * d;
+// ------------------------------------------------------------- box-box
+// The box-box closed form FCL carries from ODE (box_box-inl.h:110-862,
+// boxBox2): a 15-axis separation search, then either one point between the
+// two closest edges or the face contact -- the incident face clipped to
+// the reference face's rectangle, points kept while behind the reference
+// face, at most four of them spread around their centroid.  Organised here
+// as small steps; each step's arithmetic and constants are the reference's,
+// so the contacts are bit-identical to the oracle's.
+
+// Parameters (ta, tb) of the closest points pa + ta ua and pb + tb ub of two
+// lines with unit directions; 0, 0 when they are nearly parallel (1 - c^2
+// at most 1e-4 in single precision).
+__device__ inline void closest_line_params(const D3& pa, const D3& ua, const D3& pb, const D3& ub,
+                                           double& ta, double& tb) {
+  const D3 gap = sub(pb, pa);
+  const double c = dot(ua, ub);
+  const double ga = dot(ua, gap);
+  const double gb = -dot(ub, gap);
+  const double det = 1 - c * c;
+  if (det <= static_cast<double>(0.0001f)) {
+    ta = 0;
+    tb = 0;
+    return;
   }
+  const double inv = 1 / det;
+  ta = (ga + c * gb) * inv;
+  tb = (c * ga + gb) * inv;
 }
 
-// box_box-inl.h:110-170: clip the quad p against the rectangle [-h, h]
-__device__ inline int intersect_rect_quad(const double h[2], double p[8], double ret[16]) {
-  double buffer[16];
-  double src[16];
-  int nq = 4, nr = 0;
-  // q/r are ping-pong buffers: 0 = p (as src), 1 = ret, 2 = buffer
-  for (int i = 0; i < 8; ++i) src[i] = p[i];
-  double* q = src;
-  double* r = ret;
-  bool q_is_ret = false;
-  for (int dir = 0; dir <= 1; ++dir) {
-    for (int sign = -1; sign <= 1; sign += 2) {
-      double* pq = q;
-      double* pr = r;
-      nr = 0;
-      for (int i = nq; i > 0; --i) {
-        if (sign * pq[dir] < h[dir]) {
-          pr[0] = pq[0];
-          pr[1] = pq[1];
-          pr += 2;
-          nr++;
-          if (nr & 8) {
-            q = r;
-            q_is_ret = (r == ret);
-            goto done;
-          }
-        }
-        double* nextq = (i > 1) ? pq + 2 : q;
-        if ((sign * pq[dir] < h[dir]) ^ (sign * nextq[dir] < h[dir])) {
-          pr[1 - dir] = pq[1 - dir] + (nextq[1 - dir] - pq[1 - dir]) / (nextq[dir] - pq[dir]) *
-                                          (sign * h[dir] - pq[dir]);
-          pr[dir] = sign * h[dir];
-          pr += 2;
-          nr++;
-          if (nr & 8) {
-            q = r;
-            q_is_ret = (r == ret);
-            goto done;
-          }
-        }
-        pq += 2;
+// One Sutherland-Hodgman pass over a polygon of n (x, y) points: the part
+// with sign * v[axis] < lim, each crossing edge contributing its crossing
+// point.  The output stops at 8 points (full is then set).
+__device__ inline int clip_pass(const double* in, int n, int axis, double sign, double lim,
+                                double* out, bool& full) {
+  const int other = 1 - axis;
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    const double* cur = in + 2 * i;
+    const double* nxt = in + 2 * (i + 1 < n ? i + 1 : 0);
+    const bool cur_in = sign * cur[axis] < lim;
+    if (cur_in) {
+      out[2 * m] = cur[0];
+      out[2 * m + 1] = cur[1];
+      if (++m == 8) {
+        full = true;
+        return m;
       }
-      q = r;
-      q_is_ret = (r == ret);
-      r = q_is_ret ? buffer : ret;
-      nq = nr;
+    }
+    if (cur_in != (sign * nxt[axis] < lim)) {
+      out[2 * m + other] =
+          cur[other] + (nxt[other] - cur[other]) / (nxt[axis] - cur[axis]) * (sign * lim - cur[axis]);
+      out[2 * m + axis] = sign * lim;
+      if (++m == 8) {
+        full = true;
+        return m;
+      }
     }
   }
-done:
-  if (!q_is_ret)
-    for (int i = 0; i < 2 * nr; ++i) ret[i] = q[i];
-  return nr;
+  return m;
 }
 
-__device__ inline void cull_points(int n, const double p[], int m, int i0, int iret[]) {
-  double a, cx, cy, q;
-  switch (n) {
-    case 1:
-      cx = p[0];
-      cy = p[1];
-      break;
-    case 2:
-      cx = 0.5 * (p[0] + p[2]);
-      cy = 0.5 * (p[1] + p[3]);
-      break;
-    default:
-      a = 0;
-      cx = 0;
-      cy = 0;
-      for (int i = 0; i < n - 1; ++i) {
-        q = p[i * 2] * p[i * 2 + 3] - p[i * 2 + 2] * p[i * 2 + 1];
-        a += q;
-        cx += q * (p[i * 2] + p[i * 2 + 2]);
-        cy += q * (p[i * 2 + 1] + p[i * 2 + 3]);
-      }
-      q = p[n * 2 - 2] * p[1] - p[0] * p[n * 2 - 1];
-      if (fabs(a + q) > 2.220446049250313e-16)
-        a = 1 / (3 * (a + q));
-      else
-        a = 1e18f;
-      cx = a * (cx + q * (p[n * 2 - 2] + p[0]));
-      cy = a * (cy + q * (p[n * 2 - 1] + p[1]));
+// The quad's part inside the rectangle [-h0, h0] x [-h1, h1]: passes
+// against -x, +x, -y, +y in turn (at most 8 points).
+__device__ inline int clip_quad_to_rect(const double h[2], const double quad[8], double out[16]) {
+  double a[16], b[16];
+  for (int k = 0; k < 8; ++k) a[k] = quad[k];
+  double* src = a;
+  double* dst = b;
+  int n = 4;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int axis = pass >> 1;
+    bool full = false;
+    n = clip_pass(src, n, axis, (pass & 1) ? 1.0 : -1.0, h[axis], dst, full);
+    double* t = src;
+    src = dst;
+    dst = t;
+    if (full) break;
   }
-  double A[8];
-  for (int i = 0; i < n; ++i) A[i] = atan2(p[i * 2 + 1] - cy, p[i * 2] - cx);
-  int avail[8];
-  for (int i = 0; i < n; ++i) avail[i] = 1;
-  avail[i0] = 0;
-  iret[0] = i0;
-  int k = 1;
+  for (int k = 0; k < 2 * n; ++k) out[k] = src[k];
+  return n;
+}
+
+// Centroid of a contact polygon of n (x, y) points: the point itself, the
+// midpoint of two, or the area-weighted centroid of the closed polygon (a
+// weight of 1e18 in single precision when its area vanishes).
+__device__ inline void polygon_centroid(int n, const double* p, double& cx, double& cy) {
+  if (n == 1) {
+    cx = p[0];
+    cy = p[1];
+    return;
+  }
+  if (n == 2) {
+    cx = 0.5 * (p[0] + p[2]);
+    cy = 0.5 * (p[1] + p[3]);
+    return;
+  }
+  double area = 0, sx = 0, sy = 0, w;
+  for (int i = 0; i + 1 < n; ++i) {
+    const double* u = p + 2 * i;
+    w = u[0] * u[3] - u[2] * u[1];
+    area += w;
+    sx += w * (u[0] + u[2]);
+    sy += w * (u[1] + u[3]);
+  }
+  const double* last = p + 2 * (n - 1);
+  w = last[0] * p[1] - p[0] * last[1];
+  const double scale = fabs(area + w) > 2.220446049250313e-16 ? 1 / (3 * (area + w))
+                                                               : static_cast<double>(1e18f);
+  cx = scale * (sx + w * (last[0] + p[0]));
+  cy = scale * (sy + w * (last[1] + p[1]));
+}
+
+// Of n contact points keep m: `first` (the deepest), then for each step of
+// 2 pi / m around the centroid the unused point whose angle is nearest.
+__device__ inline void spread_contacts(int n, const double* p, int m, int first, int* keep) {
+  double cx, cy;
+  polygon_centroid(n, p, cx, cy);
+  double ang[8];
+  bool used[8];
+  for (int i = 0; i < n; ++i) {
+    ang[i] = atan2(p[2 * i + 1] - cy, p[2 * i] - cx);
+    used[i] = false;
+  }
   const double pi = 3.14159265358979323846;
+  used[first] = true;
+  keep[0] = first;
   for (int j = 1; j < m; ++j) {
-    a = j * (2 * pi / m) + A[i0];
-    if (a > pi) a -= 2 * pi;
-    double maxdiff = 1e9, diff;
-    iret[k] = i0;
+    double target = j * (2 * pi / m) + ang[first];
+    if (target > pi) target -= 2 * pi;
+    double best = HUGE_VAL;  // every angular gap (at most pi) is below it
+    int pick = first;
     for (int i = 0; i < n; ++i) {
-      if (avail[i]) {
-        diff = fabs(A[i] - a);
-        if (diff > pi) diff = 2 * pi - diff;
-        if (diff < maxdiff) {
-          maxdiff = diff;
-          iret[k] = i;
-        }
+      if (used[i]) continue;
+      double gap = fabs(ang[i] - target);
+      if (gap > pi) gap = 2 * pi - gap;
+      if (gap < best) {
+        best = gap;
+        pick = i;
       }
     }
-    avail[iret[k]] = 0;
-    ++k;
+    used[pick] = true;
+    keep[j] = pick;
   }
 }
 
-// box_box-inl.h:253-862 (boxBox2 in the world frame, up to 4 contacts)
-__device__ inline bool box_box(const Shape& b1, const Xf& tf1, const Shape& b2, const Xf& tf2,
-                               bool want, Contact* out, int& n) {
-  const double fudge = 1.05;
-  const double* R1 = tf1.R;
-  const double* R2 = tf2.R;
-  const D3 T1 = tf1.t, T2 = tf2.t;
-  D3 normalC = mk(0, 0, 0);
-  double s, s2, l;
-  int invert_normal, code;
-  const D3 p = sub(T2, T1);
-  const D3 pp = rotT(R1, p);
-  const D3 A = mul(mk(b1.dims[0], b1.dims[1], b1.dims[2]), 0.5);
-  const D3 B = mul(mk(b2.dims[0], b2.dims[1], b2.dims[2]), 0.5);
-  double R[9], Q[9];
-  mulTN(R1, R2, R);
-  for (int i = 0; i < 9; ++i) Q[i] = fabs(R[i]);
-  int best_col = -1;
-  bool normal_from_R1 = true;
-  double tmp;
-  s = -1.7976931348623157e308;
-  invert_normal = 0;
-  code = 0;
-  for (int k = 0; k < 3; ++k) {
-    tmp = pp.v[k];
-    s2 = fabs(tmp) - (dot(mk(Q[3 * k], Q[3 * k + 1], Q[3 * k + 2]), B) + A.v[k]);
-    if (s2 > 0) return false;
-    if (s2 > s) {
-      s = s2;
-      best_col = k;
-      normal_from_R1 = true;
-      invert_normal = tmp < 0;
-      code = 1 + k;
+// The two boxes in box 1's frame: rel = R1^T R2, its absolute values, and
+// box 2's centre.
+struct BoxFrames {
+  const double* R1;
+  const double* R2;
+  D3 t1, t2, d;   // centres, d = t2 - t1 (world)
+  D3 d1;          // d in box 1's frame
+  D3 h1, h2;      // half extents
+  double rel[9], arel[9];
+};
+
+// The separating-axis search: false when some axis separates the boxes.
+// Otherwise the axis of least penetration (edge axes favoured by 1.05):
+// `feature` 1-3 box 1's faces, 4-6 box 2's, 7-15 edge pairs; `depth` its
+// penetration; `normal` in the world frame, from box 1 towards box 2.
+__device__ inline bool least_penetration(BoxFrames& f, int& feature, double& depth, D3& normal) {
+  const double* R = f.rel;
+  double* Q = f.arel;
+  const D3& d1 = f.d1;
+  double best = -1.7976931348623157e308;
+  int face = -1;
+  bool face_of_1 = true, flip = false;
+  feature = 0;
+  for (int k = 0; k < 3; ++k) {  // box 1's face normals
+    const double along = d1.v[k];
+    const double sep = fabs(along) - (dot(mk(Q[3 * k], Q[3 * k + 1], Q[3 * k + 2]), f.h2) + f.h1.v[k]);
+    if (sep > 0) return false;
+    if (sep > best) {
+      best = sep;
+      face = k;
+      face_of_1 = true;
+      flip = along < 0;
+      feature = 1 + k;
     }
   }
-  for (int k = 0; k < 3; ++k) {
-    tmp = dot(col(R2, k), p);
-    s2 = fabs(tmp) - (dot(mk(Q[k], Q[3 + k], Q[6 + k]), A) + B.v[k]);
-    if (s2 > 0) return false;
-    if (s2 > s) {
-      s = s2;
-      best_col = k;
-      normal_from_R1 = false;
-      invert_normal = tmp < 0;
-      code = 4 + k;
+  for (int k = 0; k < 3; ++k) {  // box 2's face normals
+    const double along = dot(col(f.R2, k), f.d);
+    const double sep = fabs(along) - (dot(mk(Q[k], Q[3 + k], Q[6 + k]), f.h1) + f.h2.v[k]);
+    if (sep > 0) return false;
+    if (sep > best) {
+      best = sep;
+      face = k;
+      face_of_1 = false;
+      flip = along < 0;
+      feature = 4 + k;
     }
   }
+  // edge axes: |rel| grown by 10 eps times the largest half extent (at least 1)
   const double eps = 2.220446049250313e-16;
   {
-    const double ma = A.v[0] > A.v[1] ? (A.v[0] > A.v[2] ? A.v[0] : A.v[2])
-                                      : (A.v[1] > A.v[2] ? A.v[1] : A.v[2]);
-    const double mb = B.v[0] > B.v[1] ? (B.v[0] > B.v[2] ? B.v[0] : B.v[2])
-                                      : (B.v[1] > B.v[2] ? B.v[1] : B.v[2]);
+    const double* a = f.h1.v;
+    const double* b = f.h2.v;
+    const double ma = a[0] > a[1] ? (a[0] > a[2] ? a[0] : a[2]) : (a[1] > a[2] ? a[1] : a[2]);
+    const double mb = b[0] > b[1] ? (b[0] > b[2] ? b[0] : b[2]) : (b[1] > b[2] ? b[1] : b[2]);
     double mm = ma > mb ? ma : mb;
     if (mm < 1.0) mm = 1.0;
-    const double sf = mm * 10.0 * eps;
-    for (int i = 0; i < 9; ++i) Q[i] += sf;
+    const double grow = mm * 10.0 * eps;
+    for (int i = 0; i < 9; ++i) Q[i] += grow;
   }
-  const double* a = A.v;
-  const double* b = B.v;
-  // the nine edge-edge axes, in the reference's order
-  const double etmp[9] = {pp.v[2] * R[3] - pp.v[1] * R[6], pp.v[2] * R[4] - pp.v[1] * R[7],
-                          pp.v[2] * R[5] - pp.v[1] * R[8], pp.v[0] * R[6] - pp.v[2] * R[0],
-                          pp.v[0] * R[7] - pp.v[2] * R[1], pp.v[0] * R[8] - pp.v[2] * R[2],
-                          pp.v[1] * R[0] - pp.v[0] * R[3], pp.v[1] * R[1] - pp.v[0] * R[4],
-                          pp.v[1] * R[2] - pp.v[0] * R[5]};
-  const double erad[9] = {
-      ((a[1] * Q[6] + a[2] * Q[3]) + b[1] * Q[2]) + b[2] * Q[1],
-      ((a[1] * Q[7] + a[2] * Q[4]) + b[0] * Q[2]) + b[2] * Q[0],
-      ((a[1] * Q[8] + a[2] * Q[5]) + b[0] * Q[1]) + b[1] * Q[0],
-      ((a[0] * Q[6] + a[2] * Q[0]) + b[1] * Q[5]) + b[2] * Q[4],
-      ((a[0] * Q[7] + a[2] * Q[1]) + b[0] * Q[5]) + b[2] * Q[3],
-      ((a[0] * Q[8] + a[2] * Q[2]) + b[0] * Q[4]) + b[1] * Q[3],
-      ((a[0] * Q[3] + a[1] * Q[0]) + b[1] * Q[8]) + b[2] * Q[7],
-      ((a[0] * Q[4] + a[1] * Q[1]) + b[0] * Q[8]) + b[2] * Q[6],
-      ((a[0] * Q[5] + a[1] * Q[2]) + b[0] * Q[7]) + b[1] * Q[6]};
-  const D3 eax[9] = {mk(0, -R[6], R[3]), mk(0, -R[7], R[4]), mk(0, -R[8], R[5]),
-                     mk(R[6], 0, -R[0]), mk(R[7], 0, -R[1]), mk(R[8], 0, -R[2]),
-                     mk(-R[3], R[0], 0), mk(-R[4], R[1], 0), mk(-R[5], R[2], 0)};
-  for (int k = 0; k < 9; ++k) {
-    tmp = etmp[k];
-    s2 = fabs(tmp) - erad[k];
-    if (s2 > 0) return false;
-    const D3 nv = eax[k];
-    l = nrm(nv);
-    if (l > eps) {
-      s2 /= l;
-      if (s2 * fudge > s) {
-        s = s2;
-        best_col = -1;
-        normalC = dvd(nv, l);
-        invert_normal = tmp < 0;
-        code = 7 + k;
+  const double* a = f.h1.v;
+  const double* b = f.h2.v;
+  D3 edge_axis = mk(0, 0, 0);
+  for (int i = 0; i < 3; ++i) {    // box 1's edge i
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+    for (int j = 0; j < 3; ++j) {  // box 2's edge j
+      const int j1 = j == 0 ? 1 : 0, j2 = j == 2 ? 1 : 2;
+      const double along = d1.v[i2] * R[3 * i1 + j] - d1.v[i1] * R[3 * i2 + j];
+      const double reach = ((a[i1] * Q[3 * i2 + j] + a[i2] * Q[3 * i1 + j]) + b[j1] * Q[3 * i + j2]) +
+                           b[j2] * Q[3 * i + j1];
+      double sep = fabs(along) - reach;
+      if (sep > 0) return false;
+      D3 ax;
+      ax.v[i] = 0;
+      ax.v[i1] = -R[3 * i2 + j];
+      ax.v[i2] = R[3 * i1 + j];
+      const double len = nrm(ax);
+      if (len > eps) {
+        sep /= len;
+        if (sep * 1.05 > best) {
+          best = sep;
+          face = -1;
+          edge_axis = dvd(ax, len);
+          flip = along < 0;
+          feature = 7 + 3 * i + j;
+        }
       }
     }
   }
-  if (!code) return false;
-  D3 normal = best_col != -1 ? col(normal_from_R1 ? R1 : R2, best_col) : rot(R1, normalC);
-  if (invert_normal) normal = neg(normal);
-  const double depth = -s;
+  if (!feature) return false;
+  normal = face != -1 ? col(face_of_1 ? f.R1 : f.R2, face) : rot(f.R1, edge_axis);
+  if (flip) normal = neg(normal);
+  depth = -best;
+  return true;
+}
+
+// An edge-edge feature: one contact half way between the closest points of
+// the two edge lines (each edge through the box corner furthest along the
+// normal's side).
+__device__ inline Contact edge_edge_contact(const BoxFrames& f, int feature, const D3& normal,
+                                            double depth) {
+  D3 pa = f.t1;
+  for (int j = 0; j < 3; ++j) {
+    const double sign = dot(col(f.R1, j), normal) > 0 ? 1 : -1;
+    pa = add(pa, mul(col(f.R1, j), f.h1.v[j] * sign));
+  }
+  D3 pb = f.t2;
+  for (int j = 0; j < 3; ++j) {
+    const double sign = dot(col(f.R2, j), normal) > 0 ? -1 : 1;
+    pb = add(pb, mul(col(f.R2, j), f.h2.v[j] * sign));
+  }
+  const D3 ua = col(f.R1, (feature - 7) / 3), ub = col(f.R2, (feature - 7) % 3);
+  double ta, tb;
+  closest_line_params(pa, ua, pb, ub, ta, tb);
+  pa = add(pa, mul(ua, ta));
+  pb = add(pb, mul(ub, tb));
+  Contact c;
+  c.normal = normal;
+  c.pos = mul(add(pa, pb), 0.5);
+  c.depth = depth;
+  return c;
+}
+
+// A face feature: the incident box's face most anti-parallel to the normal,
+// projected onto the reference face's plane, clipped to its rectangle; the
+// points behind the reference face are contacts (their depth below it), at
+// most 4 kept, the deepest first.
+__device__ inline int face_contacts(const BoxFrames& f, int feature, const D3& normal, bool want,
+                                    Contact* out) {
+  const bool ref1 = feature <= 3;  // reference face on box 1
+  const double* Ra = ref1 ? f.R1 : f.R2;
+  const double* Rb = ref1 ? f.R2 : f.R1;
+  const D3 Pa = ref1 ? f.t1 : f.t2, Pb = ref1 ? f.t2 : f.t1;
+  const D3 Sa = ref1 ? f.h1 : f.h2, Sb = ref1 ? f.h2 : f.h1;
+  const D3 nref = ref1 ? normal : neg(normal);
+  const D3 nb = rotT(Rb, nref);
+  const D3 anb = mk(fabs(nb.v[0]), fabs(nb.v[1]), fabs(nb.v[2]));
+  // the incident box's axis closest to the normal (lanr) and the other two
+  int lanr, e1, e2;
+  if (anb.v[1] > anb.v[0]) {
+    if (anb.v[1] > anb.v[2]) {
+      e1 = 0;
+      lanr = 1;
+      e2 = 2;
+    } else {
+      e1 = 0;
+      e2 = 1;
+      lanr = 2;
+    }
+  } else {
+    if (anb.v[0] > anb.v[2]) {
+      lanr = 0;
+      e1 = 1;
+      e2 = 2;
+    } else {
+      e1 = 0;
+      e2 = 1;
+      lanr = 2;
+    }
+  }
+  const D3 face_c = nb.v[lanr] < 0 ? add(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]))
+                                   : sub(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]));
+  const int ref_axis = ref1 ? feature - 1 : feature - 4;
+  const int u = ref_axis == 0 ? 1 : 0, v = ref_axis == 2 ? 1 : 2;  // the reference face's axes
+  const double c1 = dot(col(Ra, u), face_c), c2 = dot(col(Ra, v), face_c);
+  double m11 = dot(col(Rb, e1), col(Ra, u)), m12 = dot(col(Rb, e2), col(Ra, u));
+  double m21 = dot(col(Rb, e1), col(Ra, v)), m22 = dot(col(Rb, e2), col(Ra, v));
+  const double k1 = m11 * Sb.v[e1], k2 = m21 * Sb.v[e1], k3 = m12 * Sb.v[e2], k4 = m22 * Sb.v[e2];
+  const double quad[8] = {c1 - k1 - k3, c2 - k2 - k4, c1 - k1 + k3, c2 - k2 + k4,
+                          c1 + k1 + k3, c2 + k2 + k4, c1 + k1 - k3, c2 + k2 - k4};
+  const double rect[2] = {Sa.v[u], Sa.v[v]};
+  double clipped[16];
+  const int nclip = clip_quad_to_rect(rect, quad, clipped);
+  if (nclip < 1) return 0;
+  // back from the face's (u, v) coordinates to points on the incident face
+  const double inv_det = 1.f / (m11 * m22 - m12 * m21);
+  m11 *= inv_det;
+  m12 *= inv_det;
+  m21 *= inv_det;
+  m22 *= inv_det;
+  D3 pts[8];
+  double dep[8];
+  int cnt = 0;
+  for (int j = 0; j < nclip; ++j) {
+    const double du = clipped[2 * j] - c1, dv = clipped[2 * j + 1] - c2;
+    const double s1 = m22 * du - m12 * dv;
+    const double s2 = -m21 * du + m11 * dv;
+    pts[cnt] = add(add(face_c, mul(col(Rb, e1), s1)), mul(col(Rb, e2), s2));
+    dep[cnt] = Sa.v[ref_axis] - dot(nref, pts[cnt]);
+    if (dep[cnt] >= 0) {
+      clipped[2 * cnt] = clipped[2 * j];
+      clipped[2 * cnt + 1] = clipped[2 * j + 1];
+      ++cnt;
+    }
+  }
+  if (cnt < 1) return 0;
+  int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  if (cnt > 4) {
+    int deepest = 0;
+    for (int i = 1; i < cnt; ++i)
+      if (dep[i] > dep[deepest]) deepest = i;
+    spread_contacts(cnt, clipped, 4, deepest, order);
+    cnt = 4;
+  }
+  if (!want) return 0;
+  for (int j = 0; j < cnt; ++j) {
+    const int i = order[j];
+    out[j].normal = normal;
+    out[j].pos = feature < 4 ? add(add(pts[i], Pa), mul(normal, dep[i] / 2))
+                             : sub(add(pts[i], Pa), mul(normal, dep[i] / 2));
+    out[j].depth = dep[i];
+  }
+  return cnt;
+}
+
+// box_box-inl.h:253-862 (boxBox2, world frame, up to 4 contacts).
+__device__ inline bool box_box(const Shape& b1, const Xf& tf1, const Shape& b2, const Xf& tf2,
+                               bool want, Contact* out, int& n) {
+  BoxFrames f;
+  f.R1 = tf1.R;
+  f.R2 = tf2.R;
+  f.t1 = tf1.t;
+  f.t2 = tf2.t;
+  f.d = sub(f.t2, f.t1);
+  f.d1 = rotT(f.R1, f.d);
+  f.h1 = mul(mk(b1.dims[0], b1.dims[1], b1.dims[2]), 0.5);
+  f.h2 = mul(mk(b2.dims[0], b2.dims[1], b2.dims[2]), 0.5);
+  mulTN(f.R1, f.R2, f.rel);
+  for (int i = 0; i < 9; ++i) f.arel[i] = fabs(f.rel[i]);
+  int feature;
+  double depth;
+  D3 normal;
   n = 0;
-  if (code > 6) {
-    D3 pa = T1;
-    for (int j = 0; j < 3; ++j) {
-      const double sign = dot(col(R1, j), normal) > 0 ? 1 : -1;
-      pa = add(pa, mul(col(R1, j), A.v[j] * sign));
-    }
-    D3 pb = T2;
-    for (int j = 0; j < 3; ++j) {
-      const double sign = dot(col(R2, j), normal) > 0 ? -1 : 1;
-      pb = add(pb, mul(col(R2, j), B.v[j] * sign));
-    }
-    double alpha, beta;
-    const D3 ua = col(R1, (code - 7) / 3), ub = col(R2, (code - 7) % 3);
-    line_closest(pa, ua, pb, ub, alpha, beta);
-    pa = add(pa, mul(ua, alpha));
-    pb = add(pb, mul(ub, beta));
+  if (!least_penetration(f, feature, depth, normal)) return false;
+  if (feature > 6) {
     if (want) {
-      out[0].normal = normal;
-      out[0].pos = mul(add(pa, pb), 0.5);
-      out[0].depth = depth;
+      out[0] = edge_edge_contact(f, feature, normal, depth);
       n = 1;
     }
     return true;
   }
-  const double* Ra = code <= 3 ? R1 : R2;
-  const double* Rb = code <= 3 ? R2 : R1;
-  const D3 Pa = code <= 3 ? T1 : T2, Pb = code <= 3 ? T2 : T1;
-  const D3 Sa = code <= 3 ? A : B, Sb = code <= 3 ? B : A;
-  const D3 normal2 = code <= 3 ? normal : neg(normal);
-  const D3 nr = rotT(Rb, normal2);
-  const D3 anr = mk(fabs(nr.v[0]), fabs(nr.v[1]), fabs(nr.v[2]));
-  int lanr, a1, a2;
-  if (anr.v[1] > anr.v[0]) {
-    if (anr.v[1] > anr.v[2]) {
-      a1 = 0;
-      lanr = 1;
-      a2 = 2;
-    } else {
-      a1 = 0;
-      a2 = 1;
-      lanr = 2;
-    }
-  } else {
-    if (anr.v[0] > anr.v[2]) {
-      lanr = 0;
-      a1 = 1;
-      a2 = 2;
-    } else {
-      a1 = 0;
-      a2 = 1;
-      lanr = 2;
-    }
-  }
-  D3 center;
-  if (nr.v[lanr] < 0)
-    center = add(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]));
-  else
-    center = sub(sub(Pb, Pa), mul(col(Rb, lanr), Sb.v[lanr]));
-  const int codeN = code <= 3 ? code - 1 : code - 4;
-  int code1, code2;
-  if (codeN == 0) {
-    code1 = 1;
-    code2 = 2;
-  } else if (codeN == 1) {
-    code1 = 0;
-    code2 = 2;
-  } else {
-    code1 = 0;
-    code2 = 1;
-  }
-  double quad[8];
-  const double c1 = dot(col(Ra, code1), center), c2 = dot(col(Ra, code2), center);
-  double m11 = dot(col(Rb, a1), col(Ra, code1)), m12 = dot(col(Rb, a2), col(Ra, code1));
-  double m21 = dot(col(Rb, a1), col(Ra, code2)), m22 = dot(col(Rb, a2), col(Ra, code2));
-  const double k1 = m11 * Sb.v[a1], k2 = m21 * Sb.v[a1], k3 = m12 * Sb.v[a2],
-               k4 = m22 * Sb.v[a2];
-  quad[0] = c1 - k1 - k3;
-  quad[1] = c2 - k2 - k4;
-  quad[2] = c1 - k1 + k3;
-  quad[3] = c2 - k2 + k4;
-  quad[4] = c1 + k1 + k3;
-  quad[5] = c2 + k2 + k4;
-  quad[6] = c1 + k1 - k3;
-  quad[7] = c2 + k2 - k4;
-  const double rect[2] = {Sa.v[code1], Sa.v[code2]};
-  double ret[16];
-  const int n_intersect = intersect_rect_quad(rect, quad, ret);
-  if (n_intersect < 1) return true;  // intersecting, no contact point
-  D3 points[8];
-  double dep[8];
-  const double det1 = 1.f / (m11 * m22 - m12 * m21);
-  m11 *= det1;
-  m12 *= det1;
-  m21 *= det1;
-  m22 *= det1;
-  int cnum = 0;
-  for (int j = 0; j < n_intersect; ++j) {
-    const double kk1 = m22 * (ret[j * 2] - c1) - m12 * (ret[j * 2 + 1] - c2);
-    const double kk2 = -m21 * (ret[j * 2] - c1) + m11 * (ret[j * 2 + 1] - c2);
-    points[cnum] = add(add(center, mul(col(Rb, a1), kk1)), mul(col(Rb, a2), kk2));
-    dep[cnum] = Sa.v[codeN] - dot(normal2, points[cnum]);
-    if (dep[cnum] >= 0) {
-      ret[cnum * 2] = ret[j * 2];
-      ret[cnum * 2 + 1] = ret[j * 2 + 1];
-      cnum++;
-    }
-  }
-  if (cnum < 1) return true;
-  int maxc = 4;
-  if (maxc > cnum) maxc = cnum;
-  if (maxc < 1) maxc = 1;
-  int iret[8] = {0, 1, 2, 3, 4, 5, 6, 7};
-  if (cnum > maxc) {
-    int i1 = 0;
-    double maxdepth = dep[0];
-    for (int i = 1; i < cnum; ++i)
-      if (dep[i] > maxdepth) {
-        maxdepth = dep[i];
-        i1 = i;
-      }
-    cull_points(cnum, ret, maxc, i1, iret);
-    cnum = maxc;
-  }
-  if (want) {
-    for (int j = 0; j < cnum; ++j) {
-      const int i = iret[j];
-      out[j].normal = normal;
-      out[j].pos = code < 4 ? add(add(points[i], Pa), mul(normal, dep[i] / 2))
-                            : sub(add(points[i], Pa), mul(normal, dep[i] / 2));
-      out[j].depth = dep[i];
-    }
-    n = cnum;
-  }
+  n = face_contacts(f, feature, normal, want, out);
   return true;
 }
 
