@@ -42,6 +42,10 @@ struct MeshSlot {
   int32_t* level_nodes = nullptr;
   std::vector<int32_t> level_offsets;
   bool refitted = false;
+  // the BVs were last merged from their children (bottom-up refit): the
+  // reference's OBB and RSS merges need not contain every child, so the
+  // bounds are no longer lower bounds of every pair below them
+  bool merged = false;
 };
 
 template <typename T>
@@ -155,6 +159,13 @@ int max_depth(const fclx_context* ctx) {
   for (const auto& m : ctx->meshes)
     if (m.live) d = std::max(d, m.bvh.depth);
   return d;
+}
+
+// Some live mesh holds BVs merged by a bottom-up refit.
+bool any_merged(const fclx_context* ctx) {
+  for (const auto& m : ctx->meshes)
+    if (m.live && m.merged) return true;
+  return false;
 }
 
 size_t max_nodes(const fclx_context* ctx) {
@@ -523,6 +534,7 @@ int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
   HIP_TRY(ctx, hipMemcpy(&cmax, m.refit_cmax, sizeof(double), hipMemcpyDeviceToHost));
   m.dev.cmax = cmax;
   m.refitted = true;
+  m.merged = bottomup != 0;
   return table_sync(ctx);
 }
 
@@ -655,12 +667,14 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
     return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "unknown engine");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int depth, nmesh;
+  bool merged;
   const MeshDev* table;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     depth = max_depth(ctx);
     nmesh = static_cast<int>(ctx->meshes.size());
     table = ctx->d_table;
+    merged = any_merged(ctx);
   }
   if (2 * depth > kMaxStack)
     return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
@@ -683,8 +697,12 @@ int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, vo
     std::lock_guard<std::mutex> lock(ctx->ws_mu);
     tu = engine_tuning(ctx->ws).dist;
   }
-  hipError_t e = a->engine == FCLX_ENGINE_FAST ? launch_distance_wave(L, 2 * depth, tu, s)
-                                               : launch_distance_plain(L, 2 * depth, s);
+  // The FAST engine finds the reference's answer in any visit order only
+  // when every bound is a lower bound of the pairs below it; merged BVs
+  // need not be, and then only the reference's own order gives its answer.
+  hipError_t e = a->engine == FCLX_ENGINE_FAST && !merged
+                     ? launch_distance_wave(L, 2 * depth, tu, s)
+                     : launch_distance_plain(L, 2 * depth, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "distance launch");
   return FCLX_OK;
 }

@@ -210,7 +210,14 @@ int fclx_mesh_export_nodes(fclx_context* ctx, int32_t mesh_id, double* bv_data,
  * from their children, level by level; 0: every node fitted to its
  * triangles as the build does.  Runs on `stream` and returns once done (the
  * mesh's cull bound is updated on the host); queries on the mesh must not
- * run meanwhile.  fclx_mesh_export_nodes then reports the refit BVs. */
+ * run meanwhile.  fclx_mesh_export_nodes then reports the refit BVs.
+ * The reference's merges (OBB::operator+, RSS::operator+) need not contain
+ * both children, so after a bottom-up refit a BV's RSS distance is not
+ * always a lower bound of the triangle pairs below it, and distance()
+ * depends on the visit order: while any mesh of the context holds merged
+ * BVs, FCLX_ENGINE_FAST distance batches run the reference-order PLAIN
+ * kernel (a top-down refit clears this).  Collide batches are unaffected:
+ * their results do not depend on the visit order. */
 int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
                     int32_t bottomup, void* stream);
 

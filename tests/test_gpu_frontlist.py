@@ -61,8 +61,11 @@ def oracle_sequences(om, offsets, qm, tf1, tf2, cap, fronts=None):
 
 def check(ctx, om, ids, offsets, qm, tf1, tf2, cap, label, capacity=1 << 16, calls=1):
     """Runs the sequences in `calls` consecutive batches (the fronts carry
-    over between calls) and compares everything with the oracle."""
+    over between calls) and compares everything with the oracle; returns
+    the oracle's counts and num_bv_tests of every query (in qm order)."""
     fs = F.FrontSet(ctx, len(offsets) - 1, capacity)
+    all_c = np.zeros(len(qm), np.int32)
+    all_bv = np.zeros(len(qm), np.int64)
     try:
         fronts = None
         cuts = np.linspace(0, max(np.diff(offsets)), calls + 1).astype(int)
@@ -88,7 +91,9 @@ def check(ctx, om, ids, offsets, qm, tf1, tf2, cap, label, capacity=1 << 16, cal
             assert np.array_equal(grec, orec), label
             for s in range(len(sub_off) - 1):
                 assert np.array_equal(fs.read(s), fronts[s]), (label, s)
-        return oc, obv
+            all_c[sub_q] = oc
+            all_bv[sub_q] = obv
+        return all_c, all_bv
     finally:
         fs.close()
 
