@@ -281,9 +281,11 @@ class Context:
     # batched calls on device tensors
     def collide_batch_async(self, query_meshes, tf1, tf2, num_max_contacts, counts, pairs,
                             num_pairs, engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,
-                            stream=None, contact=None):
+                            stream=None, contact=None, noalloc=False):
         """contact: None, or (pos (K,3), normal (K,3), depth (K,)) float64 device
-        tensors for enable_contact (a record per contact)."""
+        tensors for enable_contact (a record per contact).  noalloc: call
+        fclx_collide_batch_noalloc (ENGINE_BOOL only; no device allocation,
+        its work area is shared by every call on the GPU)."""
         a = N.CollideBatchArgs()
         n = int(query_meshes.shape[0])
         a.num_queries = n
@@ -306,7 +308,8 @@ class Context:
             a.contact_depth = _req(contact[2], "contact depth", "f64", cap)
         _on_device(query_meshes, tf1, tf2, counts, pairs, num_pairs, bv_tests, leaf_tests,
                    *(contact or ()))
-        self._check(self._L.fclx_collide_batch(self._h, ctypes.byref(a), stream))
+        fn = self._L.fclx_collide_batch_noalloc if noalloc else self._L.fclx_collide_batch
+        self._check(fn(self._h, ctypes.byref(a), stream))
 
     def distance_batch_async(self, query_meshes, tf1, tf2, distance, points=None, ids=None,
                              engine=ENGINE_PLAIN, bv_tests=None, leaf_tests=None,

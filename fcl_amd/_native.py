@@ -139,6 +139,7 @@ SIGNATURES = {
     "fclx_bvh_build": (c_i32, [c_i32, P_dbl, c_i32, P_i32, c_i32, P_dbl, P_i32, P_dbl,
                                P_i32]),
     "fclx_collide_batch": (c_i32, [c_vp, ctypes.POINTER(CollideBatchArgs), c_vp]),
+    "fclx_collide_batch_noalloc": (c_i32, [c_vp, ctypes.POINTER(CollideBatchArgs), c_vp]),
     "fclx_distance_batch": (c_i32, [c_vp, ctypes.POINTER(DistanceBatchArgs), c_vp]),
     "fclx_sync": (c_i32, [c_vp, c_vp]),
     "fclx_shape_collide_batch": (c_i32, [c_vp, ctypes.POINTER(ShapeCollideBatchArgs), c_vp]),

@@ -180,6 +180,32 @@ constexpr size_t kFastMaxNodes = size_t(1) << 21;
 // ... and 22 bits per query.
 constexpr int32_t kFastMaxQueries = int32_t(1) << 22;
 
+// Work area of fclx_collide_batch_noalloc: the boolean kernel's range
+// counters and per-query stacks.  Each GPU the library runs on holds its
+// own copy for the life of the process.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): fclx_init resolves the device address of each GPU's copy with
+// hipGetSymbolAddress, with that GPU current, and every use goes through
+// that address.
+constexpr size_t kScratchWords = FCLX_NOALLOC_SCRATCH_BYTES / sizeof(unsigned long long);
+__device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Resolves the current device's copy of `scratch` once (device `device` is current).
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = static_cast<unsigned long long*>(p);
+}
+
 // Largest L1 norm of an OBB centre over the mesh's nodes; the FP32 cull's
 // translation error term scales with it.
 double centre_bound(const std::vector<HostNode>& nodes) {
@@ -219,6 +245,7 @@ int fclx_init(int32_t device, fclx_context** out) {
     return FCLX_ERR_DEVICE;
   }
   ctx->ws = engine_workspace_create();
+  resolve_scratch(device);
   *out = ctx;
   return FCLX_OK;
 }
@@ -656,6 +683,74 @@ extern "C" {
 int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* a, void* stream) {
   if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
   return fclx::collide_batch_status(ctx, a, stream, ctx->d_status);
+}
+
+int fclx_collide_batch_noalloc(fclx_context* ctx, const fclx_collide_batch_args* a,
+                               void* stream) {
+  if (!ctx || !a) return FCLX_ERR_INVALID_ARGUMENT;
+  if (a->engine != FCLX_ENGINE_BOOL)
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "the no-allocation collide runs FCLX_ENGINE_BOOL");
+  if (a->enable_contact)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "the no-allocation collide has no contact geometry");
+  if (a->num_queries < 0 || a->num_max_contacts < 1 || !a->counts || !a->num_pairs ||
+      (a->num_queries > 0 && (!a->query_meshes || !a->tf1 || !a->tf2)))
+    return fail(ctx, FCLX_ERR_INVALID_ARGUMENT, "invalid collide batch arguments");
+  if (a->num_queries >= kFastMaxQueries)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED,
+                "the no-allocation collide takes fewer than 2^22 queries per call");
+  unsigned long long* area = nullptr;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    area = g_scratch_ptr[ctx->device];
+    if (!area)
+      return fail(ctx, FCLX_ERR_DEVICE,
+                  std::string("no device address for the work area: ") +
+                      hipGetErrorString(g_scratch_err[ctx->device]));
+  } else {
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "device ordinal beyond the work-area table");
+  }
+  if (!ctx->ws) return fail(ctx, FCLX_ERR_DEVICE, "engine workspace missing");
+  int depth, nmesh;
+  size_t nodes;
+  const MeshDev* table;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    depth = max_depth(ctx);
+    nodes = max_nodes(ctx);
+    nmesh = static_cast<int>(ctx->meshes.size());
+    table = ctx->d_table;
+  }
+  if (2 * depth > kMaxStack)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "hierarchy deeper than the traversal stack bound");
+  if (nodes >= kFastMaxNodes)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED,
+                "the no-allocation collide takes meshes of fewer than 2^20 triangles");
+  EngineTuning tu;
+  {
+    std::lock_guard<std::mutex> lock(ctx->ws_mu);
+    tu = engine_tuning(ctx->ws);
+  }
+  if (engine_anyhit_min_words(tu, a->num_queries, 2 * depth) > kScratchWords)
+    return fail(ctx, FCLX_ERR_UNSUPPORTED, "the work area holds too few query stacks");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  zero_i64<<<1, 64, 0, s>>>(reinterpret_cast<long long*>(a->num_pairs), 1);
+  CollideLaunch L{};
+  L.meshes = table;
+  L.num_meshes = nmesh;
+  L.num_queries = a->num_queries;
+  L.query_meshes = a->query_meshes;
+  L.tf1 = a->tf1;
+  L.tf2 = a->tf2;
+  L.cap = a->num_max_contacts;
+  L.counts = a->counts;
+  L.num_pairs = reinterpret_cast<long long*>(a->num_pairs);
+  L.bv_tests = reinterpret_cast<long long*>(a->bv_tests);
+  L.leaf_tests = reinterpret_cast<long long*>(a->leaf_tests);
+  L.status = ctx->d_status;
+  const hipError_t e = engine_anyhit_collide_area(tu, L, 2 * depth, area, kScratchWords, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "no-allocation collide launch");
+  return FCLX_OK;
 }
 
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* a, void* stream) {

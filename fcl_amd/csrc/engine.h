@@ -45,6 +45,16 @@ struct EngineTuning {
 };
 EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 
+// The boolean kernel over a work area the caller holds (area_words 8-byte
+// words, 8-byte aligned): no allocation, no events; the grid is cut to the
+// workgroups whose stacks fit.  Calls that overlap on one area race.
+// hipErrorInvalidValue when not even one workgroup fits
+// (engine_anyhit_min_words gives that size).
+hipError_t engine_anyhit_collide_area(const EngineTuning& tu, const CollideLaunch& L,
+                                      int stack_need, unsigned long long* area,
+                                      size_t area_words, hipStream_t s);
+size_t engine_anyhit_min_words(const EngineTuning& tu, long long num_queries, int stack_need);
+
 constexpr int kOrderedCapMax = 64;
 
 struct EngineReport {

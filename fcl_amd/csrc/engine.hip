@@ -1919,7 +1919,85 @@ hipError_t fixup_keys(EngineWorkspace* ws, const CollideLaunch& L, unsigned long
   return hipGetLastError();
 }
 
+// Launch shape of the boolean kernel for a batch: slots per wavefront (K),
+// register budget (waves per SIMD), step width, stack entries per slot and
+// the grid.
+struct AnyShape {
+  int K, wps, width, slot_cap, wide_cap;
+  unsigned grid;
+  // 8-byte words of the work area: range counters, then one stack per slot
+  size_t words() const {
+    return kAnyAreaHead + static_cast<size_t>(grid) * kAnyWaves * K * slot_cap;
+  }
+};
+
+AnyShape anyhit_shape(const EngineTuning& tu, long long nq, int stack_need) {
+  AnyShape a;
+  // slots per wavefront: fewer for small batches, so that they still spread
+  // over many wavefronts
+  a.K = tu.bool_queries_per_wave > 0 ? tu.bool_queries_per_wave : nq < kAnySmallBatch ? 4 : 8;
+  a.wps = tu.bool_waves_per_simd;
+  a.width = std::max(tu.bool_width, kWave / a.K);
+  // as many wavefronts as fit at once (slots take queries from a counter),
+  // fewer for small batches
+  const long long nw = std::max<long long>(
+      1, std::min<long long>((nq + a.K - 1) / a.K, static_cast<long long>(device_cus()) * 4 * a.wps));
+  a.grid = static_cast<unsigned>((nw + kAnyWaves - 1) / kAnyWaves);
+  // A slot's stack stays sorted by BVTT level (the children of a step's
+  // tasks go on top, in order), and a step takes at most `width` tasks from
+  // its top, so it never holds more than 2 width entries of one level;
+  // children of unsure pairs come back later, hence the queue-sized margin.
+  a.slot_cap = std::max(kAnySlotCap, 4 * (stack_need + 2) + 2 * kAnyQueue);
+  a.wide_cap = a.slot_cap - (stack_need + 2) - 2 * kAnyQueue;
+  return a;
+}
+
+// The boolean kernel over `area` (a.words() words; its range counters are
+// cleared here by a kernel, not a memset, so the sequence can be captured).
+void anyhit_launch(const AnyShape& a, const EngineTuning& tu, const CollideLaunch& L,
+                   unsigned long long* area, unsigned long long* prof,
+                   unsigned long long* wave_prof, hipStream_t s) {
+  zero_u64<<<1, kAnyAreaHead, 0, s>>>(area, kAnyAreaHead);
+  const int lb = tu.bool_leaf_batch;
+  const int rf = tu.bool_refill;
+  const int K = a.K, wps = a.wps;
+#define FCLX_ANYHIT(k, w)                                                                     \
+  engine_anyhit<k, w><<<a.grid, kBlock, 0, s>>>(L, area, area + kAnyAreaHead, a.slot_cap,     \
+                                                a.wide_cap, a.width, lb, rf, prof, wave_prof)
+  if (K == 4)
+    wps >= 4 ? FCLX_ANYHIT(4, 4) : wps == 3 ? FCLX_ANYHIT(4, 3) : FCLX_ANYHIT(4, 2);
+  else if (K == 8)
+    wps >= 4 ? FCLX_ANYHIT(8, 4) : wps == 3 ? FCLX_ANYHIT(8, 3) : FCLX_ANYHIT(8, 2);
+  else if (K == 16)
+    wps >= 4 ? FCLX_ANYHIT(16, 4) : wps == 3 ? FCLX_ANYHIT(16, 3) : FCLX_ANYHIT(16, 2);
+  else
+    wps >= 3 ? FCLX_ANYHIT(32, 3) : FCLX_ANYHIT(32, 2);
+#undef FCLX_ANYHIT
+}
+
 }  // namespace
+
+hipError_t engine_anyhit_collide_area(const EngineTuning& tu, const CollideLaunch& L,
+                                      int stack_need, unsigned long long* area,
+                                      size_t area_words, hipStream_t s) {
+  if (!area) return hipErrorInvalidValue;
+  if (L.num_queries == 0) return hipSuccess;
+  AnyShape a = anyhit_shape(tu, L.num_queries, stack_need);
+  // whole workgroups of stacks that fit the area (slots take the queries
+  // from the counters, so fewer wavefronts only take longer)
+  const size_t per_block = static_cast<size_t>(kAnyWaves) * a.K * a.slot_cap;
+  const size_t fit = area_words > kAnyAreaHead ? (area_words - kAnyAreaHead) / per_block : 0;
+  if (fit == 0) return hipErrorInvalidValue;
+  a.grid = static_cast<unsigned>(std::min<size_t>(a.grid, fit));
+  anyhit_launch(a, tu, L, area, nullptr, nullptr, s);
+  return hipGetLastError();
+}
+
+size_t engine_anyhit_min_words(const EngineTuning& tu, long long num_queries, int stack_need) {
+  AnyShape a = anyhit_shape(tu, num_queries, stack_need);
+  a.grid = 1;
+  return a.words();
+}
 
 hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                                  hipStream_t s) {
@@ -1927,25 +2005,10 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   const long long nq = L.num_queries;
   if (nq == 0) return hipSuccess;
   const int cus = device_cus();
-  // slots per wavefront: fewer for small batches, so that they still spread
-  // over many wavefronts
-  const int K = ws->tune.bool_queries_per_wave > 0 ? ws->tune.bool_queries_per_wave
-                : nq < kAnySmallBatch                ? 4
-                                                     : 8;
-  const int wps = ws->tune.bool_waves_per_simd;
-  const int width = std::max(ws->tune.bool_width, kWave / K);
-  // as many wavefronts as fit at once (slots take queries from a counter),
-  // fewer for small batches
-  const long long nw = std::max<long long>(
-      1, std::min<long long>((nq + K - 1) / K, static_cast<long long>(cus) * 4 * wps));
-  const unsigned grid = static_cast<unsigned>((nw + kAnyWaves - 1) / kAnyWaves);
-  // A slot's stack stays sorted by BVTT level (the children of a step's
-  // tasks go on top, in order), and a step takes at most `width` tasks from
-  // its top, so it never holds more than 2 width entries of one level;
-  // children of unsure pairs come back later, hence the queue-sized margin.
-  const int slot_cap = std::max(kAnySlotCap, 4 * (stack_need + 2) + 2 * kAnyQueue);
-  const int wide_cap = slot_cap - (stack_need + 2) - 2 * kAnyQueue;
-  const size_t need = kAnyAreaHead + static_cast<size_t>(grid) * kAnyWaves * K * slot_cap;
+  const AnyShape a = anyhit_shape(ws->tune, nq, stack_need);
+  const int K = a.K, wps = a.wps, slot_cap = a.slot_cap;
+  const unsigned grid = a.grid;
+  const size_t need = a.words();
   const int slot = ws->any_next;
   ws->any_next = (slot + 1) % EngineWorkspace::kAnySlots;
   if (need > ws->any_cap[slot]) {
@@ -1969,27 +2032,13 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   }
   if (ws->any_used[slot]) ENG_TRY(hipStreamWaitEvent(s, ws->any_ev[slot], 0));
   unsigned long long* area = ws->any_stk[slot];
-  zero_u64<<<1, kAnyAreaHead, 0, s>>>(area, kAnyAreaHead);  // the range counters
-  const int lb = ws->tune.bool_leaf_batch;
-  const int rf = ws->tune.bool_refill;
   unsigned long long* prof = nullptr;
   if (ws->profile) {  // counters of this batch (the batch is waited for)
     prof = ws->counters + kCtrWork;
     zero_u64<<<1, 64, 0, s>>>(prof, 9);
     ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * static_cast<size_t>(grid) * kAnyWaves));
   }
-#define FCLX_ANYHIT(k, w) \
-  engine_anyhit<k, w><<<grid, kBlock, 0, s>>>(L, area, area + kAnyAreaHead, slot_cap, wide_cap, \
-                                              width, lb, rf, prof, ws->wave_prof)
-  if (K == 4)
-    wps >= 4 ? FCLX_ANYHIT(4, 4) : wps == 3 ? FCLX_ANYHIT(4, 3) : FCLX_ANYHIT(4, 2);
-  else if (K == 8)
-    wps >= 4 ? FCLX_ANYHIT(8, 4) : wps == 3 ? FCLX_ANYHIT(8, 3) : FCLX_ANYHIT(8, 2);
-  else if (K == 16)
-    wps >= 4 ? FCLX_ANYHIT(16, 4) : wps == 3 ? FCLX_ANYHIT(16, 3) : FCLX_ANYHIT(16, 2);
-  else
-    wps >= 3 ? FCLX_ANYHIT(32, 3) : FCLX_ANYHIT(32, 2);
-#undef FCLX_ANYHIT
+  anyhit_launch(a, ws->tune, L, area, prof, ws->wave_prof, s);
   ENG_TRY(hipGetLastError());
   ENG_TRY(hipEventRecord(ws->any_ev[slot], s));
   ws->any_used[slot] = true;

@@ -234,6 +234,28 @@ int fclx_collide_batch(fclx_context* ctx, const fclx_collide_batch_args* args,
                        void* stream);
 int fclx_distance_batch(fclx_context* ctx, const fclx_distance_batch_args* args,
                         void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory during a call -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc: fclx_collide_batch with
+ * FCLX_ENGINE_BOOL (counts[q] = min(collide() count, 1); *num_pairs set to
+ * 0; bv_tests / leaf_tests zeroed when given) that allocates nothing.  Its
+ * per-query stacks live in a fixed device array of
+ * FCLX_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs
+ * on until the process ends (fclx_init resolves its address); the grid is
+ * cut to the wavefronts whose stacks fit, so large batches take longer
+ * than with fclx_collide_batch, with the same answers.  Limits: engine
+ * FCLX_ENGINE_BOOL, enable_contact 0, fewer than 2^22 queries per call,
+ * meshes of fewer than 2^20 triangles and hierarchies within the traversal
+ * stack bound; anything else returns FCLX_ERR_UNSUPPORTED (or
+ * FCLX_ERR_INVALID_ARGUMENT) without launching.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race and give wrong answers; calls
+ * ordered on one stream, or separated by a synchronisation, are safe.
+ * The array keeps the last call's contents; each call clears the counters
+ * it needs. */
+#define FCLX_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int fclx_collide_batch_noalloc(fclx_context* ctx, const fclx_collide_batch_args* args,
+                               void* stream);
 /* Batched fcl::collide(mesh, tf1, shape, tf2) for a BVHModel<OBBRSS> and a
  * primitive (BOX, SPHERE, CAPSULE, CYLINDER, or CONVEX with >= 4 vertices)
  * with the GST_INDEP solver, enable_cost off: per query what
