@@ -351,6 +351,30 @@ def bench_distance(args):
         "algorithmic_bytes": alg_bytes,
         "batch_ms_events": gpu_ms,
     }
+    # the same batch with FP32 culling (certified FP32 lower bounds first,
+    # FP64 where both children of a node stay): same outputs, timed beside
+    fp32 = None
+    if tuned is None:
+        d_ref = dist.clone()
+        ctx.set_engine_param("dist_fp32_cull", 1)
+        run(True)
+        ctx.sync(s)
+        f_bv, f_leaf = int(bvt.sum().item()), int(lft.sum().item())
+        same = bool(torch.equal(dist, d_ref))
+        e2 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e2[0].record(stream)
+        for _ in range(args.steps):
+            run()
+        e2[1].record(stream)
+        ctx.sync(s)
+        ctx.set_engine_param("dist_fp32_cull", 0)
+        fp32 = {"batch_ms_events": e2[0].elapsed_time(e2[1]) / args.steps,
+                "bounds_per_query": f_bv / n, "triangle_tests_per_query": f_leaf / n,
+                "distances_equal": same}
+        log("fp32 culling: %.2f ms per batch, %.0f bounds (FP32 and FP64) and %.0f triangle "
+            "tests per query, distances equal %s" % (fp32["batch_ms_events"], f_bv / n,
+                                                     f_leaf / n, same))
+        roofline["fp32_cull"] = fp32
     d = dist.cpu().numpy()
     log("distance: %.2f ms/step (events %.2f), %.3g queries/s; %.0f RSS and %.0f triangle "
         "tests per query (the reference's: %.0f and %.0f); %d of %d touching" %

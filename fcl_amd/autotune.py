@@ -28,6 +28,7 @@ DIST_SPACE = (
     ("dist_waves_per_simd", (2, 3)),
     ("dist_width", (16, 32, 64)),
     ("dist_leaf_batch", (8, 16, 32, 64)),
+    ("dist_fp32_cull", (0, 1)),
 )
 
 MESHSHAPE_SPACE = (

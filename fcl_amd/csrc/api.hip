@@ -28,6 +28,7 @@ struct MeshSlot {
   RssNode64* rss64 = nullptr;
   double* size64 = nullptr;
   ObbNode32* obb32 = nullptr;
+  RssNode32* rss32 = nullptr;
   double* verts = nullptr;
   int32_t* tris = nullptr;
   double* tri_verts = nullptr;
@@ -36,7 +37,8 @@ struct MeshSlot {
   int64_t bytes = 0;
   MeshDev dev{};
   // device refit: scratch, the nodes level by level (bottom-up refit) and
-  // whether the device BVs differ from the host build (export reads them)
+  // whether the device BVs differ from the host build (export reads them);
+  // refit_cmax holds the two cull-error scales (OBB centres, RSS origins)
   int32_t* refit_order = nullptr;
   double* refit_cmax = nullptr;
   int32_t* level_nodes = nullptr;
@@ -122,6 +124,7 @@ void free_slot(MeshSlot& s) {
   (void)hipFree(s.rss64);
   (void)hipFree(s.size64);
   (void)hipFree(s.obb32);
+  (void)hipFree(s.rss32);
   (void)hipFree(s.verts);
   (void)hipFree(s.tri_verts);
   (void)hipFree(s.tris);
@@ -215,6 +218,15 @@ double centre_bound(const std::vector<HostNode>& nodes) {
   return mx;
 }
 
+// Largest L1 norm of an RSS origin over the mesh's nodes; the FP32 distance
+// bound's translation error term scales with it.
+double origin_bound(const std::vector<HostNode>& nodes) {
+  double mx = 0;
+  for (const auto& n : nodes)
+    mx = std::max(mx, std::fabs(n.rss_to[0]) + std::fabs(n.rss_to[1]) + std::fabs(n.rss_to[2]));
+  return mx;
+}
+
 }  // namespace
 
 namespace fclx {
@@ -297,6 +309,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   std::vector<ObbNode64> o64(nb);
   std::vector<RssNode64> r64(nb);
   std::vector<ObbNode32> o32(nb);
+  std::vector<RssNode32> r32(nb);
   std::vector<double> sz(nb);
   for (int i = 0; i < nb; ++i) {
     const HostNode& h = nodes[i];
@@ -321,6 +334,13 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
       f.extent[k] = static_cast<float>(h.obb_extent[k]);
     }
     f.first_child = h.first_child;
+    RssNode32& g = r32[i];
+    for (int k = 0; k < 9; ++k) g.axis[k] = static_cast<float>(h.rss_axis[k]);
+    for (int k = 0; k < 3; ++k) g.to[k] = static_cast<float>(h.rss_to[k]);
+    g.l[0] = static_cast<float>(h.rss_l[0]);
+    g.l[1] = static_cast<float>(h.rss_l[1]);
+    g.r = static_cast<float>(h.rss_r);
+    g.first_child = h.first_child;
     sz[i] = s.bvh.node_size(i);
   }
   // primitive ranges of the nodes and each triangle's position in the build
@@ -343,6 +363,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   (void)hipSetDevice(ctx->device);
   bool ok = dmalloc(&s.obb64, nb) == hipSuccess && dmalloc(&s.rss64, nb) == hipSuccess &&
             dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32, nb) == hipSuccess &&
+            dmalloc(&s.rss32, nb) == hipSuccess &&
             dmalloc(&s.verts, 3 * static_cast<size_t>(num_vertices)) == hipSuccess &&
             dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess &&
             dmalloc(&s.tri_verts, tv.size()) == hipSuccess &&
@@ -361,6 +382,8 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
              hipSuccess &&
          hipMemcpy(s.obb32, o32.data(), sizeof(ObbNode32) * nb, hipMemcpyHostToDevice) ==
              hipSuccess &&
+         hipMemcpy(s.rss32, r32.data(), sizeof(RssNode32) * nb, hipMemcpyHostToDevice) ==
+             hipSuccess &&
          hipMemcpy(s.verts, vertices, sizeof(double) * 3 * num_vertices,
                    hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(s.tris, tris, sizeof(int32_t) * 3 * num_tris, hipMemcpyHostToDevice) ==
@@ -372,7 +395,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
     return fail(ctx, FCLX_ERR_OUT_OF_MEMORY, "device allocation or upload failed");
   }
   s.bytes = static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
-                                        sizeof(ObbNode32)) +
+                                        sizeof(ObbNode32) + sizeof(RssNode32)) +
             24LL * num_vertices + 16LL * num_tris + 8LL * nb +
             8LL * kTriVertsStride * num_tris;
   s.live = true;
@@ -383,6 +406,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   d.rss64 = s.rss64;
   d.size64 = s.size64;
   d.obb32 = s.obb32;
+  d.rss32 = s.rss32;
   d.verts = s.verts;
   d.tris = s.tris;
   d.tri_verts = s.tri_verts;
@@ -393,6 +417,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
   d.bv_type = bv_type;
   d.pad = 0;
   d.cmax = centre_bound(nodes);
+  d.rcmax = origin_bound(nodes);
   int id = -1;
   for (size_t i = 0; i < ctx->meshes.size(); ++i)
     if (!ctx->meshes[i].live) {
@@ -514,7 +539,7 @@ int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
   const int nv = static_cast<int>(m.bvh.vertices.size() / 3);
   if (!m.refit_order) {
     HIP_TRY(ctx, dmalloc(&m.refit_order, static_cast<size_t>(nt)));
-    HIP_TRY(ctx, dmalloc(&m.refit_cmax, 1));
+    HIP_TRY(ctx, dmalloc(&m.refit_cmax, 2));
   }
   if (bottomup && !m.level_nodes) {
     // the nodes level by level from the root
@@ -548,7 +573,9 @@ int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
   L.rss64 = m.rss64;
   L.size64 = m.size64;
   L.obb32 = m.obb32;
+  L.rss32 = m.rss32;
   L.cmax = m.refit_cmax;
+  L.rcmax = m.refit_cmax + 1;
   L.num_verts = nv;
   L.num_tris = nt;
   L.num_bvs = nb;
@@ -557,9 +584,10 @@ int fclx_mesh_refit(fclx_context* ctx, int32_t mesh_id, const double* vertices,
   HIP_TRY(ctx, launch_refit(L, bottomup != 0, m.level_nodes, m.level_offsets.data(),
                             static_cast<int>(m.level_offsets.size()) - 1, s));
   HIP_TRY(ctx, hipStreamSynchronize(s));
-  double cmax = 0;
-  HIP_TRY(ctx, hipMemcpy(&cmax, m.refit_cmax, sizeof(double), hipMemcpyDeviceToHost));
-  m.dev.cmax = cmax;
+  double cmax[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpy(cmax, m.refit_cmax, 2 * sizeof(double), hipMemcpyDeviceToHost));
+  m.dev.cmax = cmax[0];
+  m.dev.rcmax = cmax[1];
   m.refitted = true;
   m.merged = bottomup != 0;
   return table_sync(ctx);
@@ -1013,6 +1041,7 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   else if (n == "meshshape_leaf_share") *value = t.meshshape_leaf_share;
   else if (n == "dist_width") *value = t.dist.width;
   else if (n == "dist_leaf_batch") *value = t.dist.leaf_batch;
+  else if (n == "dist_fp32_cull") *value = t.dist.fp32_cull;
   else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
   else if (n == "bool_waves_per_simd") *value = t.bool_waves_per_simd;
   else if (n == "bool_width") *value = t.bool_width;
@@ -1066,6 +1095,8 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.dist.width = static_cast<int>(value);
   } else if (n == "dist_leaf_batch" && in(1, 64)) {
     t.dist.leaf_batch = static_cast<int>(value);
+  } else if (n == "dist_fp32_cull" && in(0, 1)) {
+    t.dist.fp32_cull = static_cast<int>(value);
   } else if (n == "bool_queries_per_wave" &&
              (value == 0 || value == 4 || value == 8 || value == 16 || value == 32)) {
     t.bool_queries_per_wave = static_cast<int>(value);

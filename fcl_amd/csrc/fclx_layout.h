@@ -10,6 +10,8 @@
 //                    of the OBB extents; RSS: sqrt(l0^2+l1^2)+2r), computed on
 //                    the host so that no device sqrt enters a descent decision.
 //   obb32[n]   64 B: the OBB rounded to float, for conservative culling.
+//   rss32[n]   64 B: the RSS rounded to float, for conservative distance
+//                    lower bounds.
 //   verts      24 B each (x,y,z doubles); tris 12 B each (3 x int32).
 //   tri_verts 128 B per triangle: its three vertices (9 doubles) followed by
 //              padding, so a triangle test reads one line per triangle.
@@ -48,6 +50,15 @@ struct alignas(64) ObbNode32 {
 };
 static_assert(sizeof(ObbNode32) == 64, "ObbNode32 must be 64 B");
 
+struct alignas(64) RssNode32 {
+  float axis[9];
+  float to[3];
+  float l[2];
+  float r;
+  int32_t first_child;
+};
+static_assert(sizeof(RssNode32) == 64, "RssNode32 must be 64 B");
+
 // Per-mesh device descriptor (pointers into HBM).  Kernels receive an array
 // of these and index it with the query's mesh ids.
 constexpr int kTriVertsStride = 16;  // doubles per triangle in tri_verts
@@ -69,6 +80,8 @@ struct MeshDev {
   const int32_t* span;      // 2 per node: first_primitive, num_primitives
   const int32_t* prim_pos;  // per triangle: its position in the build's primitive order
   const double* tri_verts;  // kTriVertsStride per triangle: P1, P2, P3
+  const RssNode32* rss32;
+  double rcmax;         // max over nodes of |RSS origin|_1 (bounds the FP32 distance error)
 };
 
 }  // namespace fclx

@@ -158,7 +158,9 @@ struct RefitLaunch {
   RssNode64* rss64;
   double* size64;
   ObbNode32* obb32;
+  RssNode32* rss32;
   double* cmax;                // device scalar: largest |OBB centre|_1
+  double* rcmax;               // device scalar: largest |RSS origin|_1
   int32_t num_verts;
   int32_t num_tris;
   int32_t num_bvs;
@@ -187,6 +189,7 @@ struct DistanceTuning {
   int waves_per_simd = 2;    // 2, 3
   int width = 32;            // most bounds of one query per step
   int leaf_batch = 16;       // leaf pairs queued before their distances are taken
+  int fp32_cull = 0;         // 1: FP32 lower bounds first, FP64 where both children stay
 };
 hipError_t launch_distance_wave(const DistanceLaunch& L, int stack_need, const DistanceTuning& tu,
                                 hipStream_t s);

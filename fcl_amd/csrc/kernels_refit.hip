@@ -42,7 +42,10 @@ __global__ void refit_vertices(RefitLaunch L) {
   }
 }
 
-__global__ void refit_zero(RefitLaunch L) { *L.cmax = 0.0; }
+__global__ void refit_zero(RefitLaunch L) {
+  *L.cmax = 0.0;
+  *L.rcmax = 0.0;
+}
 
 __global__ void refit_order(RefitLaunch L) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -425,7 +428,8 @@ __global__ void __launch_bounds__(256) refit_bottomup_level(RefitLaunch L, const
 }
 
 // BV::size() (OBBRSS-inl.h:133-136 / OBB-inl.h:206-209; RSS-inl.h:409-412),
-// FP32 copies of the OBBs and the largest |OBB centre|_1
+// FP32 copies of the OBBs and RSSs, the largest |OBB centre|_1 and the
+// largest |RSS origin|_1
 __global__ void refit_finish(RefitLaunch L) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= L.num_bvs) return;
@@ -441,10 +445,19 @@ __global__ void refit_finish(RefitLaunch L) {
     f.to[k] = static_cast<float>(n.to[k]);
     f.extent[k] = static_cast<float>(n.extent[k]);
   }
+  RssNode32& g = L.rss32[b];
+  for (int k = 0; k < 9; ++k) g.axis[k] = static_cast<float>(r.axis[k]);
+  for (int k = 0; k < 3; ++k) g.to[k] = static_cast<float>(r.to[k]);
+  g.l[0] = static_cast<float>(r.l[0]);
+  g.l[1] = static_cast<float>(r.l[1]);
+  g.r = static_cast<float>(r.r);
   const double c = (fabs(n.to[0]) + fabs(n.to[1])) + fabs(n.to[2]);
+  const double rc = (fabs(r.to[0]) + fabs(r.to[1])) + fabs(r.to[2]);
   // non-negative doubles order as their bit patterns
   atomicMax(reinterpret_cast<unsigned long long*>(L.cmax),
             static_cast<unsigned long long>(__double_as_longlong(c)));
+  atomicMax(reinterpret_cast<unsigned long long*>(L.rcmax),
+            static_cast<unsigned long long>(__double_as_longlong(rc)));
 }
 
 }  // namespace

@@ -393,6 +393,12 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               (default 32)
  *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are
  *                               taken (default 16)
+ *   "dist_fp32_cull"    0, 1    1: FAST distance takes certified FP32 lower
+ *                               bounds of the RSS distances first (64-B
+ *                               float nodes) and FP64 bounds only where both
+ *                               children of a node stay; same outputs
+ *                               (default 0: FP64 bounds throughout, faster
+ *                               on MI355X)
  *   "epa_blocks_per_cu" 0..32   resident blocks per CU of the shape path's EPA
  *                               kernel (0: as many as fit); each lane holds
  *                               a 13 KiB EPA block in device memory

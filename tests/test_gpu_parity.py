@@ -328,18 +328,21 @@ def test_distance_parity(ctx, bv_type):
 
 
 DIST_DEFAULTS = {"dist_queries_per_wave": 4, "dist_waves_per_simd": 2, "dist_width": 32,
-                 "dist_leaf_batch": 16}
+                 "dist_leaf_batch": 16, "dist_fp32_cull": 0}
 
 
 @pytest.mark.parametrize("params", [
     {},
     {"dist_queries_per_wave": 2, "dist_waves_per_simd": 3, "dist_width": 64},
     {"dist_queries_per_wave": 8, "dist_width": 8, "dist_leaf_batch": 1},
-], ids=["default", "q2w3", "q8narrow"])
+    {"dist_fp32_cull": 1},
+    {"dist_fp32_cull": 1, "dist_queries_per_wave": 2, "dist_waves_per_simd": 3, "dist_width": 64},
+    {"dist_fp32_cull": 1, "dist_queries_per_wave": 8, "dist_width": 8, "dist_leaf_batch": 1},
+], ids=["default", "q2w3", "q8narrow", "fp32", "fp32_q2w3", "fp32_q8narrow"])
 def test_distance_fast_engine_tunables(ctx, params):
     """The FAST distance engine under other slot counts, widths and leaf
-    batches: the same outputs as the oracle's (OBBRSS meshes, 300 poses, a
-    third of them touching)."""
+    batches, with FP64 bounds and with FP32 culling: the same outputs as the
+    oracle's (OBBRSS meshes, 300 poses, a third of them touching)."""
     (V1, T1), (V2, T2) = F.icosphere(10, 0.15, 0xD6), F.icosphere(8, 0.15, 0xD7)
     o = [fclo.OracleMesh(V1, T1), fclo.OracleMesh(V2, T2)]
     g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]

@@ -139,12 +139,13 @@ def test_uneven_batch_bool_engine(ctx, uneven):
     assert np.array_equal(counts.cpu().numpy(), oc)
 
 
-@pytest.mark.parametrize("engine", [F.ENGINE_FAST, F.ENGINE_PLAIN])
-def test_distance_workload_full_size(ctx, engine):
+@pytest.mark.parametrize("engine,fp32", [(F.ENGINE_FAST, 0), (F.ENGINE_FAST, 1),
+                                         (F.ENGINE_PLAIN, 0)], ids=["fast", "fast_fp32", "plain"])
+def test_distance_workload_full_size(ctx, engine, fp32):
     """Mesh distance between the 20,480-triangle RSS icospheres, 1,000 of
     the workload's poses: distances, nearest points and ids bit-identical
     (statistics too for the PLAIN engine, which keeps the reference's
-    order)."""
+    order); the FAST engine with FP64 bounds and with FP32 culling."""
     w = W.mesh_distance_pair(1000)
     (V1, T1), (V2, T2) = w["meshes"]
     assert len(T1) == 20480
@@ -153,17 +154,22 @@ def test_distance_workload_full_size(ctx, engine):
     n = len(w["tf1"])
     od, opts, oids, obv, olf = fclo.distance_batch(o, w["query_meshes"], w["tf1"], w["tf2"],
                                                    threads=THREADS)
-    print("case: distance workload, 2x20480 tris, n=%d engine=%d" % (n, engine), flush=True)
+    print("case: distance workload, 2x20480 tris, n=%d engine=%d fp32_cull=%d" % (n, engine, fp32),
+          flush=True)
     dist = torch.zeros(n, dtype=torch.float64, device=DEV)
     pts = torch.zeros((n, 6), dtype=torch.float64, device=DEV)
     ids = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
     bvt = torch.zeros(n, dtype=torch.int64, device=DEV)
     lft = torch.zeros(n, dtype=torch.int64, device=DEV)
     s = torch.cuda.current_stream().cuda_stream
-    ctx.distance_batch_async(_t(np.tile(g, (n, 1)), torch.int32), _t(w["tf1"], torch.float64),
-                             _t(w["tf2"], torch.float64), dist, pts, ids, engine=engine,
-                             bv_tests=bvt, leaf_tests=lft, stream=s)
-    assert ctx.sync_status(s) == 0
+    ctx.set_engine_param("dist_fp32_cull", fp32)
+    try:
+        ctx.distance_batch_async(_t(np.tile(g, (n, 1)), torch.int32), _t(w["tf1"], torch.float64),
+                                 _t(w["tf2"], torch.float64), dist, pts, ids, engine=engine,
+                                 bv_tests=bvt, leaf_tests=lft, stream=s)
+        assert ctx.sync_status(s) == 0
+    finally:
+        ctx.set_engine_param("dist_fp32_cull", 0)
     assert np.array_equal(dist.cpu().numpy().view(np.uint64), od.view(np.uint64))
     assert np.array_equal(ids.cpu().numpy(), oids)
     assert np.array_equal(pts.cpu().numpy().view(np.uint64), opts.view(np.uint64))
