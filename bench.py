@@ -742,6 +742,20 @@ def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine,
     return srv, loads
 
 
+def pinned_pool(pool):
+    """The request pool's pose arrays in pinned host memory (as a serving
+    front end keeps its request buffers): the loop copies them to the device
+    straight from there."""
+    import torch
+
+    out = []
+    for P in pool:
+        t = torch.empty(P.shape, dtype=torch.float64, pin_memory=True)
+        t.numpy()[...] = P
+        out.append(t.numpy())
+    return out
+
+
 def serving_summary(rates, seconds, streams, no_cpu):
     """The serving workload inside the default benchmark line: the 500k-tri
     scene and 20-link robot, p50 / p99 at the given loads, and the booleans
@@ -753,7 +767,7 @@ def serving_summary(rates, seconds, streams, no_cpu):
 
     sV, sT = workloads.serving_scene()
     links = workloads.serving_robot()
-    pool = workloads.serving_requests(64)
+    pool = pinned_pool(workloads.serving_requests(64))
     ctx = F.Context(0)
     scene = ctx.mesh(sV, sT)
     link_ids = np.array([ctx.mesh(V, T) for V, T in links], np.int32)
@@ -800,6 +814,8 @@ def bench_serving(args):
     sV, sT = workloads.serving_scene()
     links = workloads.serving_robot()
     pool = workloads.serving_requests(args.serve_pool)
+    if not args.serve_pageable:
+        pool = pinned_pool(pool)
     log("serving workload: scene %d tris, %d links (%d-%d tris), pool of %d requests "
         "(mean %.0f waypoints), generated in %.1fs"
         % (len(sT), len(links), min(len(l[1]) for l in links), max(len(l[1]) for l in links),
@@ -942,6 +958,9 @@ def main():
                     help="arrival window per load")
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--serve-pageable", action="store_true",
+                    help="keep the request poses in pageable memory (the loop then packs "
+                         "them through its pinned staging) instead of pinned memory")
     ap.add_argument("--serve-engine", choices=["bool", "fast", "plain"], default="bool",
                     help="engine behind the request loop (bool: booleans only)")
     ap.add_argument("--prim-pairs", type=int, default=4_000_000)

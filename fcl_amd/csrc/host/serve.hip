@@ -5,11 +5,14 @@
 // filled once per query slot, link poses, counts) and pinned staging.
 // Whenever a slot is free it takes waiting requests, smallest first (oldest
 // first once one has waited 4 ms), as long as they fit the batch size and
-// the device-memory budget, copies
-// their poses in one transfer, launches one first-contact collide batch
-// (CollisionRequest(1, false) per link and waypoint), copies the counts back
-// and records an event; when the event has fired it writes each request's
-// results and reports its completion.  Slots run concurrently on the
+// the device-memory budget, moves their poses to the device (straight from
+// the caller's memory when it is pinned, else through the slot's pinned
+// staging, packed by the worker and its helper threads), sends a small
+// table (each request's first query, link count and link mesh ids) from
+// which a kernel writes every query's mesh ids, launches one first-contact
+// collide batch (CollisionRequest(1, false) per link and waypoint), copies
+// the counts back and records an event; when the event has fired it writes
+// each request's results and reports its completion.  Slots run concurrently on the
 // device, so later requests overlap earlier ones.  Every query of a batch is
 // independent of the others, so a request's result is the same alone and in
 // any mix.
@@ -43,12 +46,43 @@ double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
+// Host memory from hipHostMalloc (or torch's pinned allocator): the device
+// can read it directly.  Pageable memory is reported as not pinned.
+bool is_pinned(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
 constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
 constexpr auto kAgeLimit = std::chrono::milliseconds(4);  // then a request goes first
 constexpr int32_t kBatchDefault = 1 << 16;                 // queries per batch at most
 constexpr int64_t kSlotMin = 1 << 18;  // queries a slot holds without growing
+constexpr int kHelpers = 3;             // threads that pack poses beside the worker
+constexpr int64_t kPackChunk = 1 << 18;  // bytes of pose copy per packing job
 
 __global__ void clear_status(int32_t* status) { *status = 0; }
+
+// Query mesh ids of a batch from its request table: requests r = 0..R-1
+// with first[r] .. first[r+1] their queries (waypoint-major), nlink[r]
+// links whose mesh ids start at links[off[r]].
+__global__ void fill_ids(int32_t* qm, const int64_t* first, const int32_t* nlink,
+                         const int32_t* off, const int32_t* links, int nreq, int32_t scene,
+                         int64_t nq) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  int lo = 0, hi = nreq - 1;  // the last r with first[r] <= k
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  const int64_t j = k - first[lo];
+  qm[2 * k] = scene;
+  qm[2 * k + 1] = links[off[lo] + static_cast<int>(j % nlink[lo])];
+}
 
 __global__ void fill_pose(double* out, int n, fclx_server_config cfg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,6 +95,7 @@ struct Pending {
   fclx_request req;
   Clock::time_point t_submit;
   int64_t nq;
+  bool pinned;  // link_poses is pinned host memory: copied to the device directly
 };
 
 struct Slot {
@@ -73,7 +108,8 @@ struct Slot {
   int32_t* d_counts = nullptr;
   int64_t* d_num = nullptr;
   int32_t* d_status = nullptr;  // the batch's device status flags
-  int32_t* h_qm = nullptr;
+  char* d_table = nullptr;      // the batch's request table (fill_ids)
+  char* h_table = nullptr;
   double* h_tf2 = nullptr;
   int32_t* h_counts = nullptr;
   int32_t* h_status = nullptr;
@@ -92,11 +128,13 @@ void free_slot_buffers(Slot& s) {
   (void)hipFree(s.d_counts);
   (void)hipFree(s.d_num);
   (void)hipFree(s.d_status);
-  if (s.h_qm) (void)hipHostFree(s.h_qm);
+  (void)hipFree(s.d_table);
+  if (s.h_table) (void)hipHostFree(s.h_table);
+  s.d_table = s.h_table = nullptr;
   if (s.h_tf2) (void)hipHostFree(s.h_tf2);
   if (s.h_counts) (void)hipHostFree(s.h_counts);
   if (s.h_status) (void)hipHostFree(s.h_status);
-  s.d_qm = s.h_qm = s.d_counts = s.h_counts = s.d_status = s.h_status = nullptr;
+  s.d_qm = s.d_counts = s.h_counts = s.d_status = s.h_status = nullptr;
   s.d_tf1 = s.d_tf2 = s.h_tf2 = nullptr;
   s.d_num = nullptr;
   s.cap = 0;
@@ -119,9 +157,28 @@ struct fclx_server {
   int64_t in_flight_bytes = 0;
   int64_t queued_queries = 0;  // of the waiting requests
   int64_t batches = 0, queries = 0;
+  // pose packing: the worker posts copy jobs and takes a share of them
+  // itself; helpers take the rest
+  struct Job {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<std::thread> helpers;
+  std::mutex pack_mu;
+  std::condition_variable pack_cv, pack_done_cv;
+  std::vector<Job> jobs;
+  size_t job_next = 0, jobs_done = 0;
+  bool pack_stop = false;
 };
 
 namespace {
+
+// Bytes of a batch's request table for up to n queries: the firsts (8 B,
+// one more than the requests), link counts and offsets (4 B each), link
+// mesh ids (4 B each; every request has at least as many queries as
+// links, and at least one query).
+size_t table_bytes(int64_t n) { return static_cast<size_t>(8 + 24 * n + 64); }
 
 // Slot buffers for at least n queries (the slot is idle).
 hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
@@ -136,7 +193,8 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
       (e = hipMalloc(&s.d_num, 8)) != hipSuccess ||
       (e = hipMalloc(&s.d_status, 4)) != hipSuccess ||
       (e = hipHostMalloc(&s.h_status, 4)) != hipSuccess ||
-      (e = hipHostMalloc(&s.h_qm, 8 * cap)) != hipSuccess ||
+      (e = hipMalloc(&s.d_table, table_bytes(cap))) != hipSuccess ||
+      (e = hipHostMalloc(&s.h_table, table_bytes(cap))) != hipSuccess ||
       (e = hipHostMalloc(&s.h_tf2, 96 * cap)) != hipSuccess ||
       (e = hipHostMalloc(&s.h_counts, 4 * cap)) != hipSuccess) {
     free_slot_buffers(s);
@@ -149,27 +207,105 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
   return hipStreamSynchronize(s.stream);
 }
 
+void helper_main(fclx_server* S) {
+  std::unique_lock<std::mutex> lock(S->pack_mu);
+  while (true) {
+    S->pack_cv.wait(lock, [S] { return S->pack_stop || S->job_next < S->jobs.size(); });
+    if (S->pack_stop) return;
+    const fclx_server::Job j = S->jobs[S->job_next++];
+    lock.unlock();
+    std::memcpy(j.dst, j.src, j.bytes);
+    lock.lock();
+    if (++S->jobs_done == S->jobs.size()) S->pack_done_cv.notify_all();
+  }
+}
+
+// Runs the copy jobs on the worker and the helpers; returns when all are done.
+void run_jobs(fclx_server* S, std::vector<fclx_server::Job>& jobs) {
+  if (jobs.empty()) return;
+  std::unique_lock<std::mutex> lock(S->pack_mu);
+  S->jobs.swap(jobs);
+  S->job_next = 0;
+  S->jobs_done = 0;
+  S->pack_cv.notify_all();
+  while (S->job_next < S->jobs.size()) {
+    const fclx_server::Job j = S->jobs[S->job_next++];
+    lock.unlock();
+    std::memcpy(j.dst, j.src, j.bytes);
+    lock.lock();
+    ++S->jobs_done;
+  }
+  S->pack_done_cv.wait(lock, [S] { return S->jobs_done == S->jobs.size(); });
+  S->jobs.clear();
+}
+
 // Copies, launch and read-back of the slot's batch (filled by the caller).
 int slot_launch(fclx_server* S, Slot& s) {
   int64_t k = 0;
   s.first.clear();
-  for (const Pending& p : s.batch) {
+  // the request table: firsts, link counts, link offsets, link mesh ids
+  const int R = static_cast<int>(s.batch.size());
+  int64_t* first = reinterpret_cast<int64_t*>(s.h_table);
+  int32_t* nlink = reinterpret_cast<int32_t*>(first + R + 1);
+  int32_t* off = nlink + R;
+  int32_t* links = off + R;
+  int32_t nl = 0;
+  std::vector<fclx_server::Job> jobs;
+  for (int r = 0; r < R; ++r) {
+    const Pending& p = s.batch[r];
     s.first.push_back(k);
-    const int L = p.req.num_links;
-    std::memcpy(s.h_tf2 + 12 * k, p.req.link_poses, 96 * static_cast<size_t>(p.nq));
-    for (int64_t j = 0; j < p.nq; ++j) {
-      s.h_qm[2 * (k + j)] = S->cfg.scene_mesh;
-      s.h_qm[2 * (k + j) + 1] = p.req.link_meshes[j % L];
+    first[r] = k;
+    nlink[r] = p.req.num_links;
+    off[r] = nl;
+    std::memcpy(links + nl, p.req.link_meshes, 4 * static_cast<size_t>(p.req.num_links));
+    nl += p.req.num_links;
+    if (!p.pinned) {
+      const char* src = reinterpret_cast<const char*>(p.req.link_poses);
+      char* dst = reinterpret_cast<char*>(s.h_tf2 + 12 * k);
+      const size_t bytes = 96 * static_cast<size_t>(p.nq);
+      for (size_t o = 0; o < bytes; o += kPackChunk)
+        jobs.push_back({dst + o, src + o, std::min<size_t>(kPackChunk, bytes - o)});
     }
     k += p.nq;
   }
+  first[R] = k;
   s.queries = k;
+  const size_t tbytes = 8 * static_cast<size_t>(R + 1) + 4 * (2 * static_cast<size_t>(R) + nl);
+  run_jobs(S, jobs);
   if (fclx::take_injected_failure(S->ctx)) return FCLX_ERR_DEVICE;
   clear_status<<<1, 1, 0, s.stream>>>(s.d_status);
   if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
-      hipMemcpyAsync(s.d_qm, s.h_qm, 8 * k, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+      hipMemcpyAsync(s.d_table, s.h_table, tbytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
     return FCLX_ERR_DEVICE;
+  // poses: pinned requests straight from the caller's memory, the others
+  // from the staging in runs of consecutive requests
+  int64_t run_first = -1;
+  for (int r = 0; r <= R; ++r) {
+    const bool staged = r < R && !s.batch[r].pinned;
+    if (staged && run_first < 0) run_first = s.first[r];
+    if (!staged && run_first >= 0) {
+      const int64_t end = r < R ? s.first[r] : k;
+      if (hipMemcpyAsync(s.d_tf2 + 12 * run_first, s.h_tf2 + 12 * run_first,
+                         96 * static_cast<size_t>(end - run_first), hipMemcpyHostToDevice,
+                         s.stream) != hipSuccess)
+        return FCLX_ERR_DEVICE;
+      run_first = -1;
+    }
+    if (r < R && s.batch[r].pinned &&
+        hipMemcpyAsync(s.d_tf2 + 12 * s.first[r], s.batch[r].req.link_poses,
+                       96 * static_cast<size_t>(s.batch[r].nq), hipMemcpyHostToDevice,
+                       s.stream) != hipSuccess)
+      return FCLX_ERR_DEVICE;
+  }
+  {
+    const int64_t* d_first = reinterpret_cast<const int64_t*>(s.d_table);
+    const int32_t* d_nlink = reinterpret_cast<const int32_t*>(d_first + R + 1);
+    const int32_t* d_off = d_nlink + R;
+    const int32_t* d_links = d_off + R;
+    fill_ids<<<static_cast<unsigned>((k + 255) / 256), 256, 0, s.stream>>>(
+        s.d_qm, d_first, d_nlink, d_off, d_links, R, S->cfg.scene_mesh, k);
+    if (hipGetLastError() != hipSuccess) return FCLX_ERR_DEVICE;
+  }
   fclx_collide_batch_args a{};
   a.num_queries = static_cast<int32_t>(k);
   a.query_meshes = s.d_qm;
@@ -359,6 +495,7 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
       return FCLX_ERR_OUT_OF_MEMORY;
     }
   }
+  for (int i = 0; i < kHelpers; ++i) S->helpers.emplace_back(helper_main, S);
   S->worker = std::thread(worker_main, S);
   *out = S;
   return FCLX_OK;
@@ -380,6 +517,7 @@ int fclx_server_submit(fclx_server* S, const fclx_request* req) {
   p.req = *req;
   p.t_submit = Clock::now();
   p.nq = nq;
+  p.pinned = is_pinned(req->link_poses);
   {
     std::lock_guard<std::mutex> lock(S->mu);
     if (S->stop) return FCLX_ERR_INVALID_ARGUMENT;
@@ -462,6 +600,12 @@ int fclx_server_destroy(fclx_server* S) {
   }
   S->cv.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  {
+    std::lock_guard<std::mutex> lock(S->pack_mu);
+    S->pack_stop = true;
+  }
+  S->pack_cv.notify_all();
+  for (auto& h : S->helpers) h.join();
   (void)hipSetDevice(S->device);
   for (Slot& s : S->slots) {
     free_slot_buffers(s);

@@ -72,6 +72,30 @@ def test_same_alone_and_in_the_mix(ctx, setup, streams):
         srv.close()
 
 
+def test_pinned_and_pageable_requests(ctx, setup):
+    """Requests whose poses sit in pinned host memory are copied to the
+    device straight from it, the others through the loop's staging; a burst
+    mixing both kinds gives every request the oracle's booleans."""
+    import torch
+
+    scene, link_ids, reqs, expect = setup
+    pinned = []
+    for i, P in enumerate(reqs):
+        if i % 2 == 0:
+            t = torch.empty(P.shape, dtype=torch.float64, pin_memory=True)
+            t.numpy()[...] = P
+            pinned.append(t.numpy())
+        else:
+            pinned.append(P)
+    print("case: serving mix of pinned and pageable requests", flush=True)
+    srv = F.Server(ctx, scene, num_streams=3)
+    try:
+        mix, _, _ = srv.replay([(link_ids, P) for P in pinned], np.zeros(len(reqs)))
+        _check(mix, expect)
+    finally:
+        srv.close()
+
+
 def test_memory_budget_and_batch_size(ctx, setup):
     """A budget of about two average requests and batches of at most 100
     queries: requests wait for memory, large ones run alone, results stay
