@@ -707,7 +707,8 @@ def bench_first_contact(args):
     ctx.close()
 
 
-def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine, seed=11):
+def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine, seed=11,
+                    max_batch=0):
     """Replays Poisson request streams (one per rate, `seconds` long) from
     the pool through a fresh request loop; per rate the p50 / p99 request
     latency (arrival to results in host memory) and the queries/s."""
@@ -716,7 +717,7 @@ def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine,
     import fcl_amd as F
 
     nlink = len(link_ids)
-    srv = F.Server(ctx, scene, num_streams=streams, engine=engine)
+    srv = F.Server(ctx, scene, num_streams=streams, engine=engine, max_batch_queries=max_batch)
     srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
     rng = np.random.default_rng(seed)
     loads = []
@@ -730,10 +731,19 @@ def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine,
         nq = np.array([pool[i].shape[0] * nlink for i in idx])
         b0, q0 = srv.stats()
         _, lat, elapsed = srv.replay(reqs, arrivals)
+        que = srv.last_queue_ms
         b1, q1 = srv.stats()
         p50, p99 = np.percentile(lat, [50, 99])
+        svc = lat - que
+        worst = np.argsort(lat)[-4:][::-1]
         loads.append({"rate_per_s": rate, "requests": n, "queries": int(nq.sum()),
                       "p50_ms": float(p50), "p99_ms": float(p99), "max_ms": float(lat.max()),
+                      "first_ms": [float(x) for x in lat[:4]],
+                      "queue_p50_ms": float(np.percentile(que, 50)),
+                      "queue_p99_ms": float(np.percentile(que, 99)),
+                      "service_p99_ms": float(np.percentile(svc, 99)),
+                      "worst": [{"index": int(i), "queries": int(nq[i]), "latency_ms": float(lat[i]),
+                                 "queue_ms": float(que[i])} for i in worst],
                       "queries_per_s": float(nq.sum() / (elapsed / 1e3)),
                       "offered_queries_per_s": float(rate * nq.mean()),
                       "batches": b1 - b0, "elapsed_ms": elapsed})
@@ -887,7 +897,8 @@ def bench_serving(args):
     engine = {"bool": F.ENGINE_BOOL, "fast": F.ENGINE_FAST, "plain": F.ENGINE_PLAIN}[
         args.serve_engine]
     srv, loads = serving_measure(ctx, scene, link_ids, pool, args.serve_rates,
-                                 args.serve_seconds, args.serve_streams, engine)
+                                 args.serve_seconds, args.serve_streams, engine,
+                                 max_batch=args.serve_batch)
     # one request alone, the largest of the pool
     big = max(pool, key=lambda P: P.shape[0])
     _, lat1, _ = srv.replay([(link_ids, big)], [0.0])
@@ -957,6 +968,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per load")
     ap.add_argument("--serve-streams", type=int, default=3)
+    ap.add_argument("--serve-batch", type=int, default=0,
+                    help="queries per serving batch at most (0: the loop's default, 65536)")
     ap.add_argument("--serve-pool", type=int, default=64)
     ap.add_argument("--serve-pageable", action="store_true",
                     help="keep the request poses in pageable memory (the loop then packs "

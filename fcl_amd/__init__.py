@@ -605,9 +605,12 @@ class Server:
             keep.append(k)
         t = np.ascontiguousarray(arrival_ms, np.float64)
         lat = np.zeros(n)
+        que = np.zeros(n)
         el = ctypes.c_double(0)
         self._ctx._check(self._L.fclx_server_replay(self._h, arr, N.ptr(t, N.c_dbl), n,
-                                                    N.ptr(lat, N.c_dbl), ctypes.byref(el)))
+                                                    N.ptr(lat, N.c_dbl), ctypes.byref(el),
+                                                    N.ptr(que, N.c_dbl)))
+        self.last_queue_ms = que  # each request's submission-to-launch time
         return [k[2] for k in keep], lat, el.value
 
     def stats(self):

@@ -157,7 +157,8 @@ SIGNATURES = {
     "fclx_server_create": (c_i32, [c_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(c_vp)]),
     "fclx_server_submit": (c_i32, [c_vp, ctypes.POINTER(Request)]),
     "fclx_server_poll": (c_i32, [c_vp, ctypes.POINTER(Completion), c_i32, c_i32, P_i32]),
-    "fclx_server_replay": (c_i32, [c_vp, ctypes.POINTER(Request), P_dbl, c_i32, P_dbl, P_dbl]),
+    "fclx_server_replay": (c_i32, [c_vp, ctypes.POINTER(Request), P_dbl, c_i32, P_dbl, P_dbl,
+                                   P_dbl]),
     "fclx_server_stats": (c_i32, [c_vp, P_i64, P_i64]),
     "fclx_server_destroy": (c_i32, [c_vp]),
     "fclx_mesh_refit": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_vp]),

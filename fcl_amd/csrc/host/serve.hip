@@ -545,7 +545,7 @@ int fclx_server_poll(fclx_server* S, fclx_completion* out, int32_t max, int32_t 
 }
 
 int fclx_server_replay(fclx_server* S, const fclx_request* requests, const double* arrival_ms,
-                       int32_t n, double* latency_ms, double* elapsed_ms) {
+                       int32_t n, double* latency_ms, double* elapsed_ms, double* queue_ms) {
   if (!S || n < 0 || (n > 0 && (!requests || !arrival_ms || !latency_ms)))
     return FCLX_ERR_INVALID_ARGUMENT;
   const Clock::time_point t0 = Clock::now();
@@ -560,6 +560,7 @@ int fclx_server_replay(fclx_server* S, const fclx_request* requests, const doubl
       auto it = index.find(buf[i].tag);
       if (it == index.end()) continue;
       latency_ms[it->second] = buf[i].latency_ms;
+      if (queue_ms) queue_ms[it->second] = buf[i].queue_ms;
       if (buf[i].status != FCLX_OK) status = buf[i].status;
       ++got;
     }

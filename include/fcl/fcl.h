@@ -22,6 +22,7 @@
 #define FCL_AMD_FCL_H
 
 #include <algorithm>
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -57,10 +58,31 @@ struct Vector3 {
   }
   S squaredNorm() const { return dot(*this); }
   S norm() const { return std::sqrt(squaredNorm()); }
+  Vector3 normalized() const { return *this / norm(); }
+  void normalize() { *this = normalized(); }
+  static Vector3 UnitX() { return {1, 0, 0}; }
+  static Vector3 UnitY() { return {0, 1, 0}; }
+  static Vector3 UnitZ() { return {0, 0, 1}; }
+  static Vector3 Ones() { return {1, 1, 1}; }
+  static Vector3 Constant(S c) { return {c, c, c}; }
+  void setZero() { v[0] = v[1] = v[2] = 0; }
+  Vector3& operator*=(S s) { return *this = *this * s; }
+  Vector3& operator/=(S s) { return *this = *this / s; }
+  Vector3 cwiseAbs() const { return {std::abs(v[0]), std::abs(v[1]), std::abs(v[2])}; }
+  Vector3 cwiseProduct(const Vector3& o) const { return {v[0] * o[0], v[1] * o[1], v[2] * o[2]}; }
+  S maxCoeff() const { return std::max(v[0], std::max(v[1], v[2])); }
+  S minCoeff() const { return std::min(v[0], std::min(v[1], v[2])); }
+  bool isApprox(const Vector3& o, S prec = S(1e-12)) const {
+    return (*this - o).norm() <= prec * std::min(norm(), o.norm());
+  }
   S x() const { return v[0]; }
   S y() const { return v[1]; }
   S z() const { return v[2]; }
+  S& x() { return v[0]; }
+  S& y() { return v[1]; }
+  S& z() { return v[2]; }
   const S* data() const { return v; }
+  S* data() { return v; }
 };
 template <typename S>
 Vector3<S> operator*(S s, const Vector3<S>& a) {
@@ -68,9 +90,44 @@ Vector3<S> operator*(S s, const Vector3<S>& a) {
 }
 
 template <typename S>
+struct AngleAxis;
+template <typename S>
+struct Quaternion;
+
+template <typename S>
 struct Matrix3 {
   S m[9];  // row-major
   Matrix3() : m{0, 0, 0, 0, 0, 0, 0, 0, 0} {}
+  // rotations convert to their matrices, as Eigen's do
+  Matrix3(const AngleAxis<S>& aa);  // NOLINT(runtime/explicit)
+  Matrix3(const Quaternion<S>& q);  // NOLINT(runtime/explicit)
+  void setIdentity() { *this = Identity(); }
+  void setZero() { *this = Matrix3(); }
+  Matrix3 operator+(const Matrix3& o) const {
+    Matrix3 r;
+    for (int i = 0; i < 9; ++i) r.m[i] = m[i] + o.m[i];
+    return r;
+  }
+  Matrix3 operator-(const Matrix3& o) const {
+    Matrix3 r;
+    for (int i = 0; i < 9; ++i) r.m[i] = m[i] - o.m[i];
+    return r;
+  }
+  Matrix3 operator*(S s) const {
+    Matrix3 r;
+    for (int i = 0; i < 9; ++i) r.m[i] = m[i] * s;
+    return r;
+  }
+  S determinant() const { return row(0).dot(row(1).cross(row(2))); }
+  bool isApprox(const Matrix3& o, S prec = S(1e-12)) const {
+    S d = 0, a = 0, b = 0;
+    for (int i = 0; i < 9; ++i) {
+      d += (m[i] - o.m[i]) * (m[i] - o.m[i]);
+      a += m[i] * m[i];
+      b += o.m[i] * o.m[i];
+    }
+    return std::sqrt(d) <= prec * std::sqrt(std::min(a, b));
+  }
   static Matrix3 Identity() {
     Matrix3 r;
     r.m[0] = r.m[4] = r.m[8] = 1;
@@ -127,11 +184,157 @@ struct Transform3 {
     r.t = -(r.R * t);
     return r;
   }
+  // Eigen's in-place forms: translate / rotate act on the right (in the
+  // transform's own frame), pretranslate / prerotate on the left (world)
+  Transform3& translate(const Vector3<S>& v) {
+    t += R * v;
+    return *this;
+  }
+  Transform3& pretranslate(const Vector3<S>& v) {
+    t += v;
+    return *this;
+  }
+  Transform3& rotate(const Matrix3<S>& r) {
+    R = R * r;
+    return *this;
+  }
+  Transform3& prerotate(const Matrix3<S>& r) {
+    R = r * R;
+    t = r * t;
+    return *this;
+  }
+  Transform3 operator*(const Matrix3<S>& r) const {
+    Transform3 o = *this;
+    o.R = R * r;
+    return o;
+  }
+};
+
+// Rotation by `angle` about the unit vector `axis` (common/types.h: Eigen's
+// AngleAxis).
+template <typename S>
+struct AngleAxis {
+  S a = 0;
+  Vector3<S> ax = Vector3<S>::UnitX();
+  AngleAxis() = default;
+  AngleAxis(S angle, const Vector3<S>& axis) : a(angle), ax(axis) {}
+  S angle() const { return a; }
+  const Vector3<S>& axis() const { return ax; }
+  Matrix3<S> toRotationMatrix() const {
+    const S c = std::cos(a), s = std::sin(a), t = 1 - c;
+    const S x = ax[0], y = ax[1], z = ax[2];
+    Matrix3<S> r;
+    r(0, 0) = t * x * x + c;
+    r(0, 1) = t * x * y - s * z;
+    r(0, 2) = t * x * z + s * y;
+    r(1, 0) = t * x * y + s * z;
+    r(1, 1) = t * y * y + c;
+    r(1, 2) = t * y * z - s * x;
+    r(2, 0) = t * x * z - s * y;
+    r(2, 1) = t * y * z + s * x;
+    r(2, 2) = t * z * z + c;
+    return r;
+  }
+};
+
+// Unit quaternion w + xi + yj + zk (common/types.h: Eigen's Quaternion; the
+// constructor takes w first, as Eigen's does).
+template <typename S>
+struct Quaternion {
+  S q[4] = {1, 0, 0, 0};  // w, x, y, z
+  Quaternion() = default;
+  Quaternion(S w, S x, S y, S z) : q{w, x, y, z} {}
+  explicit Quaternion(const AngleAxis<S>& aa) {
+    const S h = aa.angle() / 2, s = std::sin(h);
+    q[0] = std::cos(h);
+    q[1] = aa.axis()[0] * s;
+    q[2] = aa.axis()[1] * s;
+    q[3] = aa.axis()[2] * s;
+  }
+  static Quaternion Identity() { return Quaternion(); }
+  S w() const { return q[0]; }
+  S x() const { return q[1]; }
+  S y() const { return q[2]; }
+  S z() const { return q[3]; }
+  S norm() const { return std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]); }
+  Quaternion normalized() const {
+    const S n = norm();
+    return {q[0] / n, q[1] / n, q[2] / n, q[3] / n};
+  }
+  void normalize() { *this = normalized(); }
+  Quaternion conjugate() const { return {q[0], -q[1], -q[2], -q[3]}; }
+  Quaternion inverse() const { return conjugate(); }
+  Quaternion operator*(const Quaternion& o) const {
+    return {q[0] * o.q[0] - q[1] * o.q[1] - q[2] * o.q[2] - q[3] * o.q[3],
+            q[0] * o.q[1] + q[1] * o.q[0] + q[2] * o.q[3] - q[3] * o.q[2],
+            q[0] * o.q[2] - q[1] * o.q[3] + q[2] * o.q[0] + q[3] * o.q[1],
+            q[0] * o.q[3] + q[1] * o.q[2] - q[2] * o.q[1] + q[3] * o.q[0]};
+  }
+  Matrix3<S> toRotationMatrix() const {
+    const S w = q[0], x = q[1], y = q[2], z = q[3];
+    Matrix3<S> r;
+    r(0, 0) = 1 - 2 * (y * y + z * z);
+    r(0, 1) = 2 * (x * y - w * z);
+    r(0, 2) = 2 * (x * z + w * y);
+    r(1, 0) = 2 * (x * y + w * z);
+    r(1, 1) = 1 - 2 * (x * x + z * z);
+    r(1, 2) = 2 * (y * z - w * x);
+    r(2, 0) = 2 * (x * z - w * y);
+    r(2, 1) = 2 * (y * z + w * x);
+    r(2, 2) = 1 - 2 * (x * x + y * y);
+    return r;
+  }
+  Vector3<S> operator*(const Vector3<S>& p) const { return toRotationMatrix() * p; }
+};
+
+template <typename S>
+Matrix3<S>::Matrix3(const AngleAxis<S>& aa) : Matrix3(aa.toRotationMatrix()) {}
+template <typename S>
+Matrix3<S>::Matrix3(const Quaternion<S>& q) : Matrix3(q.toRotationMatrix()) {}
+
+// Pure translation (common/types.h: Eigen's Translation<S, 3>).
+template <typename S>
+struct Translation3 {
+  Vector3<S> t;
+  Translation3() = default;
+  Translation3(S x, S y, S z) : t(x, y, z) {}
+  explicit Translation3(const Vector3<S>& v) : t(v) {}
+  const Vector3<S>& vector() const { return t; }
+  const Vector3<S>& translation() const { return t; }
+  Transform3<S> operator*(const Matrix3<S>& R) const {
+    Transform3<S> r;
+    r.R = R;
+    r.t = t;
+    return r;
+  }
+  Transform3<S> operator*(const AngleAxis<S>& aa) const { return *this * aa.toRotationMatrix(); }
+  Transform3<S> operator*(const Quaternion<S>& q) const { return *this * q.toRotationMatrix(); }
+  operator Transform3<S>() const {  // NOLINT(runtime/explicit)
+    Transform3<S> r;
+    r.t = t;
+    return r;
+  }
 };
 
 using Vector3d = Vector3<double>;
 using Matrix3d = Matrix3<double>;
 using Transform3d = Transform3<double>;
+using Quaterniond = Quaternion<double>;
+using AngleAxisd = AngleAxis<double>;
+using Translation3d = Translation3<double>;
+using Vector3f = Vector3<float>;
+using Matrix3f = Matrix3<float>;
+using Transform3f = Transform3<float>;
+using Quaternionf = Quaternion<float>;
+using AngleAxisf = AngleAxis<float>;
+using Translation3f = Translation3<float>;
+
+// math/constants.h
+template <typename S>
+struct constants {
+  static constexpr S pi() { return S(3.141592653589793238462643383279502884197169399L); }
+  static constexpr S eps() { return std::numeric_limits<S>::epsilon(); }
+};
 
 // Axis-aligned box (math/bv/AABB.h): the broadphase's bounding volume.
 // An empty box (the default) has min_ above max_, so adding a point sets it.

@@ -584,10 +584,12 @@ int fclx_server_poll(fclx_server* srv, fclx_completion* out, int32_t max, int32_
                      int32_t* n);
 /* Load replay for measurement: submits requests[i] at arrival_ms[i] after
  * the call (ascending), waits for all of them, and gives each request's
- * latency (submission to results) and the wall time of the whole replay. */
+ * latency (submission to results), optionally its time in the queue
+ * (submission to launch; queue_ms may be NULL) and the wall time of the
+ * whole replay. */
 int fclx_server_replay(fclx_server* srv, const fclx_request* requests,
                        const double* arrival_ms, int32_t n, double* latency_ms,
-                       double* elapsed_ms);
+                       double* elapsed_ms, double* queue_ms);
 /* Totals since creation: batches launched, queries run. */
 int fclx_server_stats(fclx_server* srv, int64_t* batches, int64_t* queries);
 /* Finishes every queued request, then stops the worker and frees it. */
