@@ -718,7 +718,9 @@ def serving_measure(ctx, scene, link_ids, pool, rates, seconds, streams, engine,
 
     nlink = len(link_ids)
     srv = F.Server(ctx, scene, num_streams=streams, engine=engine, max_batch_queries=max_batch)
-    srv.replay([(link_ids, P) for P in pool[:8]], np.zeros(8))  # warm-up
+    # warm-up: every pool request once, spread over 100 ms, so that first-use
+    # costs (pool buffers, launch shapes, clocks) fall before the timed loads
+    srv.replay([(link_ids, P) for P in pool], np.linspace(0, 100, len(pool)))
     rng = np.random.default_rng(seed)
     loads = []
     for rate in rates:

@@ -4,7 +4,7 @@
 // with its own HIP stream, device buffers (query mesh ids, the scene pose
 // filled once per query slot, link poses, counts) and pinned staging.
 // Whenever a slot is free it takes waiting requests, smallest first (oldest
-// first once one has waited 4 ms), as long as they fit the batch size and
+// first once one has waited 1.5 ms), as long as they fit the batch size and
 // the device-memory budget, moves their poses to the device (straight from
 // the caller's memory when it is pinned, else through the slot's pinned
 // staging, packed by the worker and its helper threads), sends a small
@@ -58,7 +58,7 @@ bool is_pinned(const void* p) {
 }
 
 constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
-constexpr auto kAgeLimit = std::chrono::milliseconds(4);  // then a request goes first
+constexpr auto kAgeLimit = std::chrono::microseconds(1500);  // then a request goes first
 constexpr int32_t kBatchDefault = 1 << 16;                 // queries per batch at most
 constexpr int64_t kSlotMin = 1 << 18;  // queries a slot holds without growing
 constexpr int kHelpers = 3;             // threads that pack poses beside the worker
@@ -494,6 +494,35 @@ int fclx_server_create(fclx_context* ctx, const fclx_server_config* cfg, fclx_se
       delete S;
       return FCLX_ERR_OUT_OF_MEMORY;
     }
+  }
+  // One batch of each launch shape the loop uses (4 query slots per
+  // wavefront below 65,536 queries, 8 from there): the first launch of a
+  // kernel costs milliseconds, which would otherwise fall inside the first
+  // large request's latency.  The queries are the scene against itself at
+  // its own pose, so each ends at its first leaf pair.
+  {
+    Slot& s = S->slots[0];
+    const int n = static_cast<int>(std::min<int64_t>(s.cap, 1 << 16));
+    std::vector<int32_t> ids(2 * static_cast<size_t>(n), cfg->scene_mesh);
+    bool ok = hipMemcpy(s.d_qm, ids.data(), 8 * static_cast<size_t>(n), hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(s.d_tf2, s.d_tf1, 96 * static_cast<size_t>(n),
+                        hipMemcpyDeviceToDevice) == hipSuccess;
+    for (const int m : {n, std::min(n, 1024)}) {
+      if (!ok) break;
+      fclx_collide_batch_args a{};
+      a.num_queries = m;
+      a.query_meshes = s.d_qm;
+      a.tf1 = s.d_tf1;
+      a.tf2 = s.d_tf2;
+      a.num_max_contacts = 1;
+      a.engine = S->cfg.engine;
+      a.counts = s.d_counts;
+      a.num_pairs = s.d_num;
+      ok = fclx::collide_batch_status(ctx, &a, s.stream, s.d_status) == FCLX_OK &&
+           hipStreamSynchronize(s.stream) == hipSuccess;
+    }
+    (void)hipGetLastError();
   }
   for (int i = 0; i < kHelpers; ++i) S->helpers.emplace_back(helper_main, S);
   S->worker = std::thread(worker_main, S);
