@@ -21,6 +21,7 @@ FAST_SPACE = (
     ("dfs_blocks_per_cu", (4, 6, 8)),
     ("idle_stop", (0.25, 0.5, 0.75)),
     ("stop_poll", (8, 16, 32)),
+    ("fast_persistent", (0, 1)),
 )
 
 DIST_SPACE = (

@@ -1042,6 +1042,8 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   else if (n == "dist_width") *value = t.dist.width;
   else if (n == "dist_leaf_batch") *value = t.dist.leaf_batch;
   else if (n == "dist_fp32_cull") *value = t.dist.fp32_cull;
+  else if (n == "fast_persistent") *value = t.fast_persistent;
+  else if (n == "fast_graph") *value = t.fast_graph;
   else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
   else if (n == "bool_waves_per_simd") *value = t.bool_waves_per_simd;
   else if (n == "bool_width") *value = t.bool_width;
@@ -1097,6 +1099,10 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.dist.leaf_batch = static_cast<int>(value);
   } else if (n == "dist_fp32_cull" && in(0, 1)) {
     t.dist.fp32_cull = static_cast<int>(value);
+  } else if (n == "fast_persistent" && in(0, 1)) {
+    t.fast_persistent = static_cast<int>(value);
+  } else if (n == "fast_graph" && in(0, 1)) {
+    t.fast_graph = static_cast<int>(value);
   } else if (n == "bool_queries_per_wave" &&
              (value == 0 || value == 4 || value == 8 || value == 16 || value == 32)) {
     t.bool_queries_per_wave = static_cast<int>(value);

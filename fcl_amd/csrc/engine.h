@@ -42,6 +42,10 @@ struct EngineTuning {
   int bool_leaf_batch = 64;       // leaf pairs queued before the boolean kernel tests them
   int bool_width = 32;            // most tasks of one query a boolean-kernel step tests
   int bool_refill = 4;            // free slots that make a boolean wavefront take new queries
+  int fast_persistent = 0;        // 1: exhaustive FAST batches as one persistent launch (the
+                                  // boolean kernel's exhaustive form) instead of rounds
+  int fast_graph = 1;             // 1: that launch sequence replayed from a captured HIP graph
+                                  // while its arguments stay the same
 };
 EngineTuning& engine_tuning(EngineWorkspace* ws);  // per context
 

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <hipcub/hipcub.hpp>
 #include <vector>
 
@@ -851,6 +852,10 @@ __global__ void engine_append_records(const int32_t* rec, long long nrec, const 
 }
 
 __global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void zero_i32_grid(int32_t* p, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 __global__ void zero_u64(unsigned long long* p, int n) {
   if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
@@ -993,6 +998,11 @@ struct AnyWave {
   unsigned long long lq[kAnyQueue];
   unsigned long long uq[kAnyQueue];
   unsigned char lane_slot[kWave];
+  // exhaustive mode: the slot's query, its contacts, BV tests, leaf tests
+  int qid[kSlots];
+  int cnt[kSlots];
+  unsigned long long nbv[kSlots];
+  unsigned long long nlf[kSlots];
 };
 
 // Inclusive sum over lanes 0..n-1 (n <= 32; valid in those lanes) with DPP
@@ -1014,7 +1024,14 @@ __device__ __forceinline__ int lane_total(int incl) {
   return __builtin_amdgcn_readlane(incl, n - 1);
 }
 
-template <int kSlots, int kWavesPerSimd>
+// kAll = false: booleans (a query ends at its first intersecting leaf
+// pair).  kAll = true: every intersecting leaf pair of every query is
+// recorded (one reservation per wavefront in the caller's pair buffer), with
+// the query's count and num_bv_tests / num_leaf_tests; nothing stops a
+// query early, so the BVTT nodes visited are the reference's, and a query
+// whose count reaches num_max_contacts is flagged (qflag, nflag) for the
+// depth-first prefix fix-up.
+template <int kSlots, int kWavesPerSimd, bool kAll>
 __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLaunch L,
                                                                        unsigned long long* area,
                                                                        unsigned long long* stk,
@@ -1022,7 +1039,9 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLa
                                                                        int width,
                                                                        int leaf_batch, int refill,
                                                                        unsigned long long* prof,
-                                                                       unsigned long long* wprof) {
+                                                                       unsigned long long* wprof,
+                                                                       int32_t* qflag,
+                                                                       unsigned long long* nflag) {
   static_assert(kSlots <= (1 << kAnySlotBits) && kSlots <= 32, "slot index must fit its field");
   __shared__ AnyWave<kSlots> waves_s[kAnyWaves];
   const int lane = lane_id();
@@ -1079,6 +1098,12 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLa
     W.found[lane] = 0;
     W.pend[lane] = 0;
     W.gen[lane] = (W.gen[lane] + 1) & kAnyGenMask;
+    if constexpr (kAll) {
+      W.qid[lane] = qq;
+      W.cnt[lane] = 0;
+      W.nbv[lane] = 0;
+      W.nlf[lane] = 0;
+    }
     return true;
   };
   for (;;) {
@@ -1089,7 +1114,18 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLa
         spv = W.sp[lane];
         const int f = W.found[lane];
         if (f != 0 || (spv == 0 && W.pend[lane] == 0)) {
-          L.counts[q] = f == 1 ? 1 : 0;
+          if constexpr (kAll) {
+            const int c = W.cnt[lane];
+            L.counts[q] = c;
+            if (L.bv_tests) L.bv_tests[q] = static_cast<long long>(W.nbv[lane]);
+            if (L.leaf_tests) L.leaf_tests[q] = static_cast<long long>(W.nlf[lane]);
+            if (c >= L.cap) {
+              qflag[q] = 1;
+              atomicAdd(nflag, 1ULL);
+            }
+          } else {
+            L.counts[q] = f == 1 ? 1 : 0;
+          }
           q = -1;
           spv = 0;
         }
@@ -1163,15 +1199,41 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLa
       // FP64 triangle tests
       const int n = nl < kWave ? nl : kWave;
       nl -= n;
-      bool skip = false;
+      bool skip = false, hit = false;
+      int hq = 0, h1 = 0, h2 = 0;
       if (lane < n) {
         int tag, t1, t2;
         unpack_task(W.lq[nl + lane], tag, t1, t2);
         const int slot = tag & ((1 << kAnySlotBits) - 1);
         skip = (tag >> kAnySlotBits) != W.gen[slot] || W.found[slot] != 0;
         if (!skip) {
-          if (leaf_any(W.x64.get(slot), t1, t2)) W.found[slot] = 1;
+          hit = leaf_any(W.x64.get(slot), t1, t2);
+          if constexpr (kAll) {
+            atomicAdd(&W.nlf[slot], 1ULL);
+            if (hit) {
+              atomicAdd(&W.cnt[slot], 1);
+              hq = W.qid[slot];
+              h1 = t1;
+              h2 = t2;
+            }
+          } else if (hit) {
+            W.found[slot] = 1;
+          }
           atomicSub(&W.pend[slot], 1);
+        }
+      }
+      if constexpr (kAll) {
+        // the hits' records, one reservation for the wavefront
+        const unsigned long long at = wave_reserve(
+            reinterpret_cast<unsigned long long*>(L.num_pairs), hit, 1);
+        if (hit) {
+          if (static_cast<long long>(at) < L.pair_capacity) {
+            L.pairs[3 * at] = hq;
+            L.pairs[3 * at + 1] = h1;
+            L.pairs[3 * at + 2] = h2;
+          } else if (L.pairs) {
+            atomicOr(L.status, kStatusPairOverflow);
+          }
         }
       }
       ++st[1];
@@ -1267,6 +1329,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_anyhit(CollideLa
           sp_max = nsp > sp_max ? nsp : sp_max;
         }
         W.pend[lane] += __popcll((lm | um) & own);
+        if constexpr (kAll) W.nbv[lane] += static_cast<unsigned long long>(take);
       }
       if (__ballot(emptied) != 0) check = true;
       ++st[0];
@@ -1508,6 +1571,14 @@ struct EngineWorkspace {
   hipEvent_t any_ev[kAnySlots] = {};
   bool any_used[kAnySlots] = {};
   int any_next = 0;
+  // work area of the exhaustive persistent form (one batch at a time)
+  unsigned long long* all_stk = nullptr;
+  size_t all_cap = 0;
+  // its launch sequence captured in a HIP graph, replayed while the launch
+  // arguments stay the same (the key); captured on a private stream
+  hipGraphExec_t all_graph = nullptr;
+  std::vector<unsigned long long> all_key;
+  hipStream_t cap_stream = nullptr;
   unsigned long long* ord_ctr = nullptr;  // kOrdSlots x 16
   hipEvent_t ord_ev[kOrdSlots] = {};
   bool ord_used[kOrdSlots] = {};
@@ -1561,6 +1632,9 @@ void engine_workspace_destroy(EngineWorkspace* ws) {
     if (ws->any_ev[i]) (void)hipEventDestroy(ws->any_ev[i]);
     (void)hipFree(ws->any_stk[i]);
   }
+  (void)hipFree(ws->all_stk);
+  if (ws->all_graph) (void)hipGraphExecDestroy(ws->all_graph);
+  if (ws->cap_stream) (void)hipStreamDestroy(ws->cap_stream);
   for (auto& e : ws->ev)
     if (e) (void)hipEventDestroy(e);
   delete ws;
@@ -1954,16 +2028,22 @@ AnyShape anyhit_shape(const EngineTuning& tu, long long nq, int stack_need) {
 
 // The boolean kernel over `area` (a.words() words; its range counters are
 // cleared here by a kernel, not a memset, so the sequence can be captured).
+// qflag / nflag: the exhaustive mode's flags (null for booleans).
 void anyhit_launch(const AnyShape& a, const EngineTuning& tu, const CollideLaunch& L,
                    unsigned long long* area, unsigned long long* prof,
-                   unsigned long long* wave_prof, hipStream_t s) {
+                   unsigned long long* wave_prof, hipStream_t s, int32_t* qflag = nullptr,
+                   unsigned long long* nflag = nullptr) {
   zero_u64<<<1, kAnyAreaHead, 0, s>>>(area, kAnyAreaHead);
   const int lb = tu.bool_leaf_batch;
   const int rf = tu.bool_refill;
   const int K = a.K, wps = a.wps;
 #define FCLX_ANYHIT(k, w)                                                                     \
-  engine_anyhit<k, w><<<a.grid, kBlock, 0, s>>>(L, area, area + kAnyAreaHead, a.slot_cap,     \
-                                                a.wide_cap, a.width, lb, rf, prof, wave_prof)
+  (qflag ? engine_anyhit<k, w, true><<<a.grid, kBlock, 0, s>>>(                               \
+               L, area, area + kAnyAreaHead, a.slot_cap, a.wide_cap, a.width, lb, rf, prof,   \
+               wave_prof, qflag, nflag)                                                       \
+         : engine_anyhit<k, w, false><<<a.grid, kBlock, 0, s>>>(                              \
+               L, area, area + kAnyAreaHead, a.slot_cap, a.wide_cap, a.width, lb, rf, prof,   \
+               wave_prof, qflag, nflag))
   if (K == 4)
     wps >= 4 ? FCLX_ANYHIT(4, 4) : wps == 3 ? FCLX_ANYHIT(4, 3) : FCLX_ANYHIT(4, 2);
   else if (K == 8)
@@ -2082,6 +2162,112 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   return hipSuccess;
 }
 
+// Exhaustive collide as one persistent launch fed by the query counters:
+// the exhaustive form of the boolean kernel (per-query slots, FP32 cull,
+// FP64 leaf and uncertain steps inline, every hit recorded), then the
+// depth-first prefix fix-up of the queries that reached the cap (only when
+// some did; it needs the host).
+hipError_t engine_persistent_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
+                                     hipStream_t s) {
+  const int nq = L.num_queries;
+  unsigned long long* C = ws->counters;
+  EngineReport& rep = ws->report;
+  rep = EngineReport{};
+  ws->spans.clear();
+  ws->ev_used = 0;
+  ENG_TRY(grow(ws->qflag, ws->qflag_cap, nq));
+  const AnyShape a = anyhit_shape(ws->tune, nq, stack_need);
+  ENG_TRY(grow(ws->all_stk, ws->all_cap, a.words()));
+  unsigned long long* prof = nullptr;
+  if (ws->profile) {
+    prof = C + kCtrWork;
+    ENG_TRY(grow(ws->wave_prof, ws->wave_prof_cap, 3 * static_cast<size_t>(a.grid) * kAnyWaves));
+  }
+  // the launch sequence: flags and counters cleared by kernels (no memset
+  // node, so the sequence can be captured), then the persistent kernel
+  auto enqueue = [&](hipStream_t q) {
+    zero_i32_grid<<<grid_for(nq), kBlock, 0, q>>>(ws->qflag, nq);
+    set_u64<<<1, 1, 0, q>>>(C + kCtrFlagged, 0);
+    if (prof) zero_u64<<<1, 64, 0, q>>>(prof, 9);
+    anyhit_launch(a, ws->tune, L, ws->all_stk, prof, ws->wave_prof, q, ws->qflag,
+                  C + kCtrFlagged);
+  };
+  size_t t = 0;
+  ENG_TRY(clock_begin(ws, s, t));
+  if (ws->tune.fast_graph && !prof) {
+    // replayed from a graph while every argument of the sequence is the same
+    const CollideLaunch* Lp = &L;
+    std::vector<unsigned long long> key(sizeof(CollideLaunch) / 8 + 12);
+    std::memcpy(key.data(), Lp, sizeof(CollideLaunch) / 8 * 8);
+    size_t k = sizeof(CollideLaunch) / 8;
+    key[k++] = reinterpret_cast<uintptr_t>(ws->all_stk);
+    key[k++] = reinterpret_cast<uintptr_t>(ws->qflag);
+    key[k++] = static_cast<unsigned long long>(a.grid);
+    key[k++] = static_cast<unsigned long long>(a.K);
+    key[k++] = static_cast<unsigned long long>(a.wps);
+    key[k++] = static_cast<unsigned long long>(a.width);
+    key[k++] = static_cast<unsigned long long>(a.slot_cap);
+    key[k++] = static_cast<unsigned long long>(a.wide_cap);
+    key[k++] = static_cast<unsigned long long>(ws->tune.bool_leaf_batch);
+    key[k++] = static_cast<unsigned long long>(ws->tune.bool_refill);
+    key[k++] = static_cast<unsigned long long>(nq);
+    key[k++] = reinterpret_cast<uintptr_t>(C);
+    if (!ws->all_graph || key != ws->all_key) {
+      if (!ws->cap_stream)
+        ENG_TRY(hipStreamCreateWithFlags(&ws->cap_stream, hipStreamNonBlocking));
+      if (ws->all_graph) {
+        ENG_TRY(hipGraphExecDestroy(ws->all_graph));
+        ws->all_graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      ENG_TRY(hipStreamBeginCapture(ws->cap_stream, hipStreamCaptureModeThreadLocal));
+      enqueue(ws->cap_stream);
+      const hipError_t ec = hipStreamEndCapture(ws->cap_stream, &g);
+      if (ec != hipSuccess) return ec;
+      const hipError_t ei = hipGraphInstantiate(&ws->all_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        ws->all_graph = nullptr;
+        return ei;
+      }
+      ws->all_key = key;
+    }
+    ENG_TRY(hipGraphLaunch(ws->all_graph, s));
+  } else {
+    enqueue(s);
+  }
+  ENG_TRY(hipGetLastError());
+  clock_end(ws, s, t, kPhaseDfs);
+  unsigned long long nflag = 0;
+  ENG_TRY(read_counter(ws, kCtrFlagged, nflag, s));
+  if (nflag > 0) {
+    ENG_TRY(clock_begin(ws, s, t));
+    if (L.bv_tests || L.leaf_tests || stack_need > 64)
+      ENG_TRY(fixup_flagged(ws, L, stack_need, nflag, s));
+    else
+      ENG_TRY(fixup_keys(ws, L, nflag, s));
+    clock_end(ws, s, t, kPhaseFixup);
+  }
+  rep.flagged = static_cast<int>(nflag);
+  rep.front_size = nq;
+  rep.dfs_waves = static_cast<int>(a.grid) * kAnyWaves;
+  rep.dfs_rounds = 1;
+  if (prof) {
+    unsigned long long h[9];
+    ENG_TRY(hipMemcpyAsync(h, prof, sizeof(h), hipMemcpyDeviceToHost, s));
+    ENG_TRY(hipStreamSynchronize(s));
+    rep.dfs_bv_tests = static_cast<long long>(h[3]);    // FP32 BV lanes
+    rep.dfs_leaf_tests = static_cast<long long>(h[4]);  // triangle tests
+    rep.fp64_fallbacks = static_cast<long long>(h[5]);  // FP64 BV tests
+    for (int k = 0; k < 3; ++k) {
+      rep.bool_steps[k] = static_cast<long long>(h[k]);
+      rep.bool_lanes[k] = static_cast<long long>(h[3 + k]);
+    }
+  }
+  ENG_TRY(clock_collect(ws));
+  return hipSuccess;
+}
+
 hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack_need,
                           hipStream_t s) {
   if (!ws) return hipErrorInvalidValue;
@@ -2108,6 +2294,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     return hipSuccess;
   }
   if (nq >= (1 << 22)) return hipErrorInvalidValue;  // task packing limit
+  if (ws->tune.fast_persistent) return engine_persistent_collide(ws, L, stack_need, s);
   const EngineTuning& tune = ws->tune;
   ENG_TRY(grow(ws->qx32, ws->qx32_cap, nq));
   ENG_TRY(grow(ws->qptr, ws->qptr_cap, nq));

@@ -393,6 +393,15 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               (default 32)
  *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are
  *                               taken (default 16)
+ *   "fast_persistent"   0, 1    1: FAST collide batches with caps above 64 run
+ *                               as one persistent launch (the BOOL kernel's
+ *                               exhaustive form: per-query slots fed from
+ *                               global query counters, FP64 steps inline)
+ *                               instead of depth-first rounds; same
+ *                               outputs (default 0: rounds, faster here)
+ *   "fast_graph"        0, 1    1: that launch sequence is captured once in a
+ *                               HIP graph and replayed while its arguments
+ *                               stay the same (default 1)
  *   "dist_fp32_cull"    0, 1    1: FAST distance takes certified FP32 lower
  *                               bounds of the RSS distances first (64-B
  *                               float nodes) and FP64 bounds only where both

@@ -154,14 +154,74 @@ def _set(p):
     return sorted(map(tuple, np.asarray(p).reshape(-1, 2).tolist()))
 
 
+@pytest.mark.parametrize("persistent", [0, 1], ids=["rounds", "persistent"])
 @pytest.mark.parametrize("cap,stats", [(100000, True), (70, True), (70, False)])
-def test_fast_engine_exhaustive(ctx, cap, stats):
+def test_fast_engine_exhaustive(ctx, cap, stats, persistent):
     """FAST engine, large caps: per query the same contact set and the same
     num_bv_tests / num_leaf_tests as the oracle; queries that reach the cap
     (cap=70 here) come back as the reference's depth-first prefix, by the
     exact re-run when statistics are requested and by depth-first keys
-    otherwise."""
-    _fast_engine_case(ctx, cap, stats)
+    otherwise.  Both forms: depth-first rounds, and one persistent launch
+    (fast_persistent)."""
+    ctx.set_engine_param("fast_persistent", persistent)
+    try:
+        _fast_engine_case(ctx, cap, stats)
+    finally:
+        ctx.set_engine_param("fast_persistent", 0)
+
+
+def test_fast_persistent_graph_replays(ctx):
+    """The persistent exhaustive form replayed from its captured HIP graph:
+    three replays of one batch, with other eager work (PLAIN and BOOL
+    batches, synchronisations) between them, then a different batch (a new
+    capture) and the first again, each equal to the same form run without
+    the graph (which the parity tests above compare with the oracle)."""
+    (V1, T1), (V2, T2) = F.icosphere(9, 0.15, 21), F.heightfield(24, 0.05, 22)
+    V2 = F.scale_to_radius(V2, 1.0)
+    g = [ctx.mesh(V1, T1), ctx.mesh(V2, T2)]
+    n = 700
+    qm = np.tile(np.array(g, np.int32), (n, 1))
+    A = F.poses(n, 0, 1.6, 0, 0xA7)
+    B = F.poses(n, 0, 1.6, 0, 0xA8)
+    d_qm = _t(qm, torch.int32)
+    d = {k: (_t(p[0], torch.float64), _t(p[1], torch.float64)) for k, p in (("A", A), ("B", B))}
+    counts = torch.zeros(n, dtype=torch.int32, device=DEV)
+    pairs = torch.zeros((1 << 20, 3), dtype=torch.int32, device=DEV)
+    num = torch.zeros(1, dtype=torch.int64, device=DEV)
+    s = torch.cuda.current_stream().cuda_stream
+
+    def run(key, graph):
+        ctx.set_engine_param("fast_graph", graph)
+        counts.fill_(-1)
+        ctx.collide_batch_async(d_qm, d[key][0], d[key][1], 100000, counts, pairs, num,
+                                engine=F.ENGINE_FAST, stream=s)
+        assert ctx.sync_status(s) == 0
+        k = int(num.item())
+        rec = pairs[:k].cpu().numpy()
+        return counts.cpu().numpy(), rec[np.lexsort((rec[:, 2], rec[:, 1], rec[:, 0]))]
+
+    def other_work():
+        c = torch.zeros(n, dtype=torch.int32, device=DEV)
+        p = torch.zeros((n * 64, 3), dtype=torch.int32, device=DEV)
+        m = torch.zeros(1, dtype=torch.int64, device=DEV)
+        for e in (F.ENGINE_PLAIN, F.ENGINE_BOOL):
+            ctx.collide_batch_async(d_qm, d["B"][0], d["B"][1], 64, c, p, m, engine=e, stream=s)
+            ctx.sync(s)
+
+    ctx.set_engine_param("fast_persistent", 1)
+    try:
+        print("case: persistent form, eager reference runs", flush=True)
+        want = {k: run(k, 0) for k in ("A", "B")}
+        assert want["A"][0].sum() > 0
+        for i, key in enumerate(("A", "A", "A", "B", "A")):
+            print("case: persistent form, graph replay %d of batch %s" % (i, key), flush=True)
+            got = run(key, 1)
+            assert np.array_equal(got[0], want[key][0])
+            assert np.array_equal(got[1], want[key][1])
+            other_work()
+    finally:
+        ctx.set_engine_param("fast_persistent", 0)
+        ctx.set_engine_param("fast_graph", 1)
 
 
 # "full": a tiny breadth-first front and deferred buffers of a few hundred

@@ -88,10 +88,20 @@ def _check_records(gc, rec, oc, off, orec, cap):
     return len(flagged)
 
 
+@pytest.mark.parametrize("persistent", [0, 1], ids=["rounds", "persistent"])
 @pytest.mark.parametrize("cap", [100000, 70])
-def test_uneven_batch_fast_engine(ctx, uneven, cap):
+def test_uneven_batch_fast_engine(ctx, uneven, cap, persistent):
     w, ids, om = uneven
-    print("case: uneven batch, 64 pairs, 10000 queries, FAST cap=%d" % cap, flush=True)
+    print("case: uneven batch, 64 pairs, 10000 queries, FAST cap=%d persistent=%d"
+          % (cap, persistent), flush=True)
+    ctx.set_engine_param("fast_persistent", persistent)
+    try:
+        _uneven_fast_case(ctx, w, ids, om, cap)
+    finally:
+        ctx.set_engine_param("fast_persistent", 0)
+
+
+def _uneven_fast_case(ctx, w, ids, om, cap):
     oc, off, orec = fclo.collide_batch_flat(om, w["query_meshes"], w["tf1"], w["tf2"], cap,
                                             threads=THREADS)
     # before the fix-up a query at the cap holds all its pairs, so the buffer
