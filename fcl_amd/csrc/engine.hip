@@ -475,6 +475,27 @@ __global__ void __launch_bounds__(kBlock) engine_uncertain(EngineArgs A) {
 }
 
 // ----------------------------------------------- persistent depth-first
+// firstOverSecond's size comparison (bvh_collision_traversal_node-inl.h:
+// 77-89; OBBRSS::size = |OBB extents|^2, OBB-inl.h:206-209) from the FP32
+// extents already loaded with the nodes.  Each FP32 sum of squares is
+// within 5u (u = 2^-24) of the FP64 size, relative; outside a 50u band the
+// FP32 order is the FP64 order, inside it (ties included) and for sizes
+// near the FP32 underflow the FP64 sizes decide.  Saves the two scattered
+// 8-B size loads of almost every descent.
+__device__ __forceinline__ bool first_larger(const ObbNode32& n1, const ObbNode32& n2,
+                                             const double* s1, const double* s2) {
+  const float a = fmaf(n1.extent[2], n1.extent[2],
+                       fmaf(n1.extent[1], n1.extent[1], n1.extent[0] * n1.extent[0]));
+  const float b = fmaf(n2.extent[2], n2.extent[2],
+                       fmaf(n2.extent[1], n2.extent[1], n2.extent[0] * n2.extent[0]));
+  constexpr float kBand = 50.0f * 5.9604644775390625e-08f;
+  if (a > 1e-30f && b > 1e-30f) {
+    if (a > b * (1.0f + kBand)) return true;
+    if (a < b * (1.0f - kBand)) return false;
+  }
+  return *s1 > *s2;
+}
+
 // Outcome of the FP32 pass over one task.
 enum : int { kStepNone = 0, kStepPush = 1, kStepLeaf = 2, kStepUncertain = 3 };
 
@@ -497,7 +518,7 @@ __device__ __forceinline__ int step32(const EngineArgs& A, unsigned long long ta
     c0 = pack_task(q, -(fc1 + 1), -(fc2 + 1));  // leaf candidate: (q, tri1, tri2)
     return kStepLeaf;
   }
-  if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+  if ((fc2 < 0) || (fc1 >= 0 && first_larger(n1, n2, P.s1 + b1, P.s2 + b2))) {
     c0 = pack_task(q, fc1, b2);
     c1 = pack_task(q, fc1 + 1, b2);
   } else {
@@ -955,7 +976,7 @@ __device__ __forceinline__ int cull_any(const QueryXf32& X, const QueryPtr& P, i
     c0 = pack_task(tag, -(fc1 + 1), -(fc2 + 1));
     return kAnyLeaf;
   }
-  if ((fc2 < 0) || (fc1 >= 0 && (P.s1[b1] > P.s2[b2]))) {
+  if ((fc2 < 0) || (fc1 >= 0 && first_larger(n1, n2, P.s1 + b1, P.s2 + b2))) {
     c0 = pack_task(tag, fc1, b2);
     c1 = pack_task(tag, fc1 + 1, b2);
   } else {
