@@ -1134,20 +1134,24 @@ def main():
         log("plain: %.1f ms, %.3g queries/s, exact match %s" % (plain_s * 1e3, nq / plain_s, exact))
 
     # ------------------------------------------------------------- roofline
-    # The depth-first kernel (FP32 only) does the BV tests of the exact
-    # traversal that the breadth-first phase left; its triangle tests are
-    # queued for engine_leaves and not charged here.  Its bound is HBM: the
-    # achieved rate is the bytes it moves (rocprofv3 counters of the same
-    # workload, profiles/engine_dfs_traffic.md) over its time in this run
-    # (HIP events, summed over the step's launches).  The algorithmic bytes
-    # (128 B per BV test) are given beside it: node records are re-used from
-    # L2, so their rate is not bounded by the HBM peak.  The VALU fraction
-    # prices the BV tests at 315 FP32 operations each.
+    # The depth-first kernel does the FP32 BV tests of the exact traversal
+    # that the breadth-first phase left, and the FP64 triangle tests of the
+    # leaf pairs it queues, 64 at a time (the remainder of each round's
+    # queues goes to engine_leaves and is not charged here).  Its bound is
+    # HBM: the achieved rate is the bytes it moves (rocprofv3 counters of the
+    # same workload, profiles/engine_dfs_traffic.md) over its time in this
+    # run (HIP events, summed over the step's launches).  The algorithmic
+    # bytes (128 B per BV test, 144 B per triangle test) are given beside it:
+    # node records are re-used from L2, so their rate is not bounded by the
+    # HBM peak.  The VALU fraction prices the BV tests at 315 FP32 operations
+    # and the triangle tests at 923 FP64 operations, each against its peak.
     t_dfs = rep["dfs_ms"] * 1e-3
-    alg_bytes = rep["dfs_bv_tests"] * BYTES_BV
+    inline_leaf = rep.get("dfs_inline_leaf_tests", 0)
+    alg_bytes = rep["dfs_bv_tests"] * BYTES_BV + inline_leaf * BYTES_LEAF
     valu_frac = None
     if t_dfs > 0:
-        valu_frac = rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 / t_dfs
+        valu_frac = (rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 +
+                     inline_leaf * OPS_LEAF / PEAK_FP64) / t_dfs
     traffic = measured_traffic(nq, args.pairs)
     achieved = traffic["bytes"] / t_dfs if traffic and t_dfs > 0 else None
     frac = achieved / PEAK_HBM if achieved else None
@@ -1172,7 +1176,8 @@ def main():
         "valu_peak_tflops": PEAK_FP32 / 1e12,
         "kernel_ms": rep["dfs_ms"],
         "bv_tests": rep["dfs_bv_tests"],
-        "leaf_tests_in_engine_leaves": rep["dfs_leaf_tests"],
+        "leaf_tests_in_kernel": inline_leaf,
+        "leaf_tests_in_engine_leaves": rep["dfs_leaf_tests"] - inline_leaf,
         "fp64_fallback_rate": rep["fp64_fallbacks"] / max(1, rep["dfs_bv_tests"] +
                                                            rep["expand_bv_tests"]),
         "phase_ms": {k: rep[k] for k in ("setup_ms", "expand_ms", "leaves_ms", "dfs_ms",

@@ -21,6 +21,8 @@ FAST_SPACE = (
     ("dfs_blocks_per_cu", (4, 6, 8)),
     ("idle_stop", (0.25, 0.5, 0.75)),
     ("stop_poll", (8, 16, 32)),
+    ("dfs_inline_leaves", (0, 1)),
+    ("dfs_stack_top", (256, 512)),
     ("fast_persistent", (0, 1)),
 )
 

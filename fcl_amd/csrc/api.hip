@@ -1043,6 +1043,8 @@ int fclx_get_engine_param(fclx_context* ctx, const char* name, double* value) {
   else if (n == "dist_leaf_batch") *value = t.dist.leaf_batch;
   else if (n == "dist_fp32_cull") *value = t.dist.fp32_cull;
   else if (n == "fast_persistent") *value = t.fast_persistent;
+  else if (n == "dfs_inline_leaves") *value = t.dfs_inline_leaves;
+  else if (n == "dfs_stack_top") *value = t.dfs_stack_top;
   else if (n == "fast_graph") *value = t.fast_graph;
   else if (n == "bool_queries_per_wave") *value = t.bool_queries_per_wave;
   else if (n == "bool_waves_per_simd") *value = t.bool_waves_per_simd;
@@ -1099,6 +1101,10 @@ int fclx_set_engine_param(fclx_context* ctx, const char* name, double value) {
     t.dist.leaf_batch = static_cast<int>(value);
   } else if (n == "dist_fp32_cull" && in(0, 1)) {
     t.dist.fp32_cull = static_cast<int>(value);
+  } else if (n == "dfs_inline_leaves" && in(0, 1)) {
+    t.dfs_inline_leaves = static_cast<int>(value);
+  } else if (n == "dfs_stack_top" && (value == 128 || value == 256 || value == 512)) {
+    t.dfs_stack_top = static_cast<int>(value);
   } else if (n == "fast_persistent" && in(0, 1)) {
     t.fast_persistent = static_cast<int>(value);
   } else if (n == "fast_graph" && in(0, 1)) {
@@ -1161,6 +1167,7 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   out->bool_end_p50_ms = r.bool_end_p50_ms;
   out->bool_end_p90_ms = r.bool_end_p90_ms;
   out->bool_span_ms = r.bool_span_ms;
+  out->dfs_inline_leaf_tests = r.dfs_inline_leaf_tests;
   return FCLX_OK;
 }
 

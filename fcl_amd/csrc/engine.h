@@ -36,6 +36,10 @@ struct EngineTuning {
   double idle_stop = 0.5;       // end a round once this share of wavefronts is idle
   int stop_poll = 16;           // depth-first steps between polls of the round's stop flag
   int dfs_waves_per_simd = 4;   // register budget of the depth-first kernel (4..6)
+  int dfs_inline_leaves = 1;    // 1: the depth-first kernel runs its triangle tests itself
+                                // (at 4 wavefronts per SIMD only)
+  int dfs_stack_top = 512;      // with inline triangle tests: stack entries per wavefront
+                                // kept in LDS (128, 256 or 512)
   int bool_queries_per_wave = 0;  // queries a boolean-engine wavefront keeps in flight (4, 8, 16,
                                   // 32; 0: 4 below 65536 queries, else 8)
   int bool_waves_per_simd = 4;    // register budget of the boolean kernel (2, 3, 4; 32 slots: 2, 3)
@@ -74,6 +78,7 @@ struct EngineReport {
   long long bool_steps[3], bool_lanes[3], bool_skipped, bool_rounds, bool_stack_max;
   double bool_wave_ms;  // mean wavefront lifetime
   double bool_end_p50_ms, bool_end_p90_ms, bool_span_ms;  // wavefront ends after the first start
+  long long dfs_inline_leaf_tests;  // of dfs_leaf_tests: run inside the depth-first kernel
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
 const EngineReport& engine_report(const EngineWorkspace* ws);

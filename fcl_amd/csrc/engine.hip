@@ -192,6 +192,7 @@ struct EngineArgs {
   unsigned long long* head;       // DFS refill cursor
   unsigned long long* stacks;
   int stack_cap;
+  int stack_top;                  // DFS stack entries kept in LDS (128..kStackTop)
   unsigned long long* work;  // kWork* counters
   unsigned long long* leaf_work;  // work counter the leaf kernel adds to
   TaskRanges in;                  // input of the leaf and uncertain kernels
@@ -212,7 +213,8 @@ struct EngineArgs {
   unsigned stop_poll;             // depth-first steps between polls of the stop flag
 };
 
-enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback };
+enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback,
+             kWorkDfsInline };
 
 // add popcount(pred) over the wavefront to *counter with one atomic
 __device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
@@ -562,19 +564,40 @@ __device__ __forceinline__ void dump_deferred(const EngineArgs& A, int w,
   for (int i = lane_id(); i < n; i += kWave) dst[s + i] = q[i];
 }
 
-// One depth-first round.  Every wavefront owns a LIFO stack in device
-// memory; each step pops up to 64 tasks (one per lane), runs the FP32 test,
-// pushes the children of overlapping internal pairs with one prefix sum and
-// queues overlapping leaf pairs and uncertain tasks in LDS, moved 64 at a
-// time to the deferred buffers for the FP64 kernels.  Idle wavefronts
-// refill from the front.  The round stops early when a deferred buffer is
-// full or when idle_limit wavefronts have run out of work; then every
-// wavefront appends its stack to front_out and its queue leftovers to the
-// dump regions, and the host runs the FP64 kernels and the next round.
-template <int kWavesPerSimd>
+// One depth-first round.  Every wavefront owns a LIFO stack; each step pops
+// up to 64 tasks (one per lane), runs the FP32 test, pushes the children of
+// overlapping internal pairs with one prefix sum and queues overlapping leaf
+// pairs and uncertain tasks in LDS.  Idle wavefronts refill from the front.
+// The round stops early when a deferred buffer is full or when idle_limit
+// wavefronts have run out of work; then every wavefront appends its stack to
+// front_out and its queue leftovers to the dump regions, and the host runs
+// the FP64 kernels and the next round.
+//
+// kInline = false: the stack lives in device memory, and leaf pairs and
+// uncertain tasks move 64 at a time to the deferred buffers for the FP64
+// kernels.
+//
+// kInline = true: the wavefront runs the FP64 triangle tests of its leaf
+// queue itself, 64 at a time, while the other wavefronts of the CU wait on
+// their node loads; hits are staged in LDS and written out in runs of up to
+// kHitStage records with one reservation on the record counter (a
+// reservation per step would serialise on that counter).  Only the queue's
+// remainder at the end of a round is deferred to engine_leaves.  The top
+// of the stack, up to A.stack_top <= kStackTop entries, lives in LDS (a ring
+// indexed by stack position): a pop reads LDS instead of waiting on device
+// memory, and only the deep part of the stack is written out (down to half
+// the limit at a time, read back when the LDS part runs low).
+constexpr int kHitStage = 256;
+constexpr int kStackTop = 512;
+template <int kWavesPerSimd, bool kInline>
 __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A) {
-  __shared__ unsigned long long lq_all[kWavesPerBlock][kLeafQ];
+  // with the triangle tests inline a queue never holds two full steps
+  constexpr int kLq = kInline ? 2 * kWave : kLeafQ;
+  constexpr int kTop = kInline ? kStackTop : 1;
+  __shared__ unsigned long long lq_all[kWavesPerBlock][kLq];
   __shared__ unsigned long long uq_all[kWavesPerBlock][kUncQ];
+  __shared__ int32_t hits_all[kInline ? kWavesPerBlock : 1][kInline ? 3 * kHitStage : 1];
+  __shared__ unsigned long long top_all[kInline ? kWavesPerBlock : 1][kTop];
   const int wave = threadIdx.x / kWave;
   const int lane = lane_id();
   const int xcd = static_cast<int>(blockIdx.x % kSub);
@@ -582,11 +605,47 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
   unsigned long long* uq = uq_all[wave];
   unsigned long long* stack =
       A.stacks + (static_cast<size_t>(blockIdx.x) * kWavesPerBlock + wave) * A.stack_cap;
-  int sp = 0;                 // wavefront-uniform
+  unsigned long long* top = top_all[kInline ? wave : 0];
+  // stack positions [0, base) are in device memory, [base, sp) in LDS
+  // (kInline; otherwise base stays 0 and every position is in device memory)
+  int sp = 0, base = 0;       // wavefront-uniform
   int nl = 0, nu = 0;         // queue fill, wavefront-uniform
   bool input_left = true;
   bool stopped = false;
   unsigned long long n_bv = 0, n_unc = 0;  // wavefront-uniform
+  unsigned long long n_leaf = 0;           // inline triangle tests
+  int nh = 0;                              // staged hit records (kInline)
+  int32_t* hits = hits_all[kInline ? wave : 0];
+  const CollideLaunch& L = A.L;
+  auto slot = [&](int pos) -> unsigned long long& {
+    if constexpr (kInline) return top[pos & (kTop - 1)];
+    else return stack[pos];
+  };
+  // staged records to the caller's buffer: one reservation for all of them
+  auto flush_hits = [&]() {
+    if constexpr (kInline) {
+      if (nh == 0) return;
+      unsigned long long b = 0;
+      if (lane == 0)
+        b = atomicAdd(reinterpret_cast<unsigned long long*>(L.num_pairs),
+                      static_cast<unsigned long long>(nh));
+      b = __shfl(b, 0);
+      bool over = false;
+      for (int r = lane; r < nh; r += kWave) {
+        const long long o = static_cast<long long>(b) + r;
+        if (o < L.pair_capacity) {
+          L.pairs[3 * o] = hits[3 * r];
+          L.pairs[3 * o + 1] = hits[3 * r + 1];
+          L.pairs[3 * o + 2] = hits[3 * r + 2];
+        } else {
+          over = true;
+        }
+      }
+      if (__ballot(over) != 0 && lane == 0 && L.pairs) atomicOr(L.status, kStatusPairOverflow);
+      nh = 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  };
   const unsigned long long t_start = A.wave_prof ? wall_clock64() : 0;
   unsigned step = 0;
 
@@ -599,8 +658,29 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
         break;
       }
     }
+    if constexpr (kInline) {
+      // a full step of queued leaf pairs: their FP64 triangle tests
+      if (nl >= kWave) {
+        nl -= kWave;
+        int q = 0, id1 = 0, id2 = 0;
+        const bool hit = leaf_hit(A, lq[nl + lane], q, id1, id2);
+        n_leaf += kWave;
+        const unsigned long long hm = __ballot(hit);
+        if (hit) {
+          const int r = nh + __popcll(hm & lanes_below());
+          hits[3 * r] = q;
+          hits[3 * r + 1] = id1;
+          hits[3 * r + 2] = id2;
+          count_hits(A, q);
+        }
+        nh += __popcll(hm);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (nh > kHitStage - kWave) flush_hits();
+        continue;
+      }
+    }
     // room for one full step in both queues
-    const bool full = (nl >= kLeafQ / 2 && !flush_deferred(A, 0, xcd, lq, nl, kLeafQ / 2)) ||
+    const bool full = (nl >= kLq / 2 && !flush_deferred(A, 0, xcd, lq, nl, kLq / 2)) ||
                       (nu >= kUncQ / 2 && !flush_deferred(A, 1, xcd, uq, nu, kUncQ / 2));
     if (full) {
       if (lane == 0) __hip_atomic_store(A.stop, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -608,6 +688,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
       break;
     }
     if (sp < kWave && input_left) {
+      // sp - base <= sp < 64: the new entries fit the LDS part
       const int want = kWave - sp;
       unsigned long long start = 0;
       if (lane == 0) start = atomicAdd(A.head, static_cast<unsigned long long>(want));
@@ -615,16 +696,28 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
       const long long avail = A.n_in - static_cast<long long>(start);
       const int got = avail <= 0 ? 0 : static_cast<int>(avail < want ? avail : want);
       if (got < want) input_left = false;
-      if (lane < got) stack[sp + lane] = A.front_in[start + lane];
+      if (lane < got) slot(sp + lane) = A.front_in[start + lane];
       sp += got;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (sp == 0) break;
     const int n = sp < kWave ? sp : kWave;
+    if constexpr (kInline) {
+      // the popped entries must be in LDS: read back up to half the limit
+      // (the LDS part holds fewer than 64 entries here, so with the limit
+      // in [128, kTop] that fits the ring and covers the pop)
+      if (sp - n < base) {
+        const int half = A.stack_top / 2;
+        const int r = base < half ? base : half;
+        for (int i = lane; i < r; i += kWave) slot(base - r + i) = stack[base - r + i];
+        base -= r;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
     sp -= n;
     const bool active = lane < n;
     unsigned long long task = 0, c0 = 0, c1 = 0;
-    if (active) task = stack[sp + lane];
+    if (active) task = slot(sp + lane);
     n_bv += n;
     const int r = active ? step32(A, task, c0, c1) : kStepNone;
     // children onto the stack (one prefix sum)
@@ -634,10 +727,23 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
       if (lane == 0) atomicOr(A.L.status, kStatusQueueOverflow);
       break;  // never write past the stack; the overflow is reported
     }
+    if constexpr (kInline) {
+      // keep [base, sp + npush) within the limit: write the deepest LDS
+      // entries out, 64 at a time, until at most half the limit is in use
+      // after the push (or all of them: then npush <= 128 entries remain)
+      if (sp + npush - base > A.stack_top) {
+        int w = sp + npush - base - A.stack_top / 2;
+        w = (w + kWave - 1) & ~(kWave - 1);
+        if (w > sp - base) w = sp - base;
+        for (int i = lane; i < w; i += kWave) stack[base + i] = slot(base + i);
+        base += w;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
     if (r == kStepPush) {
       const int o = sp + 2 * __popcll(pm & lanes_below());
-      stack[o] = c0;
-      stack[o + 1] = c1;
+      slot(o) = c0;
+      slot(o + 1) = c1;
     }
     sp += npush;
     // leaf pairs and uncertain tasks into the LDS queues
@@ -651,13 +757,14 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
     n_unc += __popcll(um);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
+  flush_hits();
   if (stopped) {
     // hand the stack to the next round
     if (sp > 0) {
       unsigned long long s = 0;
       if (lane == 0) s = atomicAdd(A.n_out, static_cast<unsigned long long>(sp));
       s = __shfl(s, 0);
-      for (int i = lane; i < sp; i += kWave) A.front_out[s + i] = stack[i];
+      for (int i = lane; i < sp; i += kWave) A.front_out[s + i] = i < base ? stack[i] : slot(i);
     }
     dump_deferred(A, 0, lq, nl);
     dump_deferred(A, 1, uq, nu);
@@ -680,6 +787,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
   if (lane == 0) {
     atomicAdd(A.work + kWorkDfsBV, n_bv);
     if (n_unc) atomicAdd(A.work + kWorkFallback, n_unc);
+    if (n_leaf) atomicAdd(A.work + kWorkDfsInline, n_leaf);
   }
 }
 
@@ -2401,9 +2509,11 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   // 3. depth-first rounds over the remaining front
   // register budget of the depth-first kernel: 4, 5 or 6 wavefronts per SIMD
   // (more waves hide more latency; fewer registers spill more)
-  auto dfs_kernel = tune.dfs_waves_per_simd >= 6   ? engine_dfs<6>
-                    : tune.dfs_waves_per_simd == 5 ? engine_dfs<5>
-                                                   : engine_dfs<4>;
+  // the inline triangle test needs the 128 registers of 4 wavefronts per SIMD
+  auto dfs_kernel = tune.dfs_waves_per_simd >= 6   ? engine_dfs<6, false>
+                    : tune.dfs_waves_per_simd == 5 ? engine_dfs<5, false>
+                    : tune.dfs_inline_leaves       ? engine_dfs<4, true>
+                                                   : engine_dfs<4, false>;
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dfs_kernel, kBlock, 0) != hipSuccess ||
       occ < 1)
@@ -2434,6 +2544,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
     D.head = C + kCtrHead;
     D.stacks = ws->stacks;
     D.stack_cap = stack_cap;
+    D.stack_top = tune.dfs_stack_top;
     D.wave_prof = ws->profile ? ws->wave_prof : nullptr;
     for (int w = 0; w < 2; ++w) {
       D.defer[w] = ws->defer[w];
@@ -2534,13 +2645,14 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   }
   rep.flagged = static_cast<int>(nflag);
   ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrWork, C + kCtrWork,
-                         5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+                         6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
   const unsigned long long* H = ws->host_counters + kCtrWork;
   rep.expand_bv_tests = static_cast<long long>(H[kWorkExpandBV]);
   rep.expand_leaf_tests = static_cast<long long>(H[kWorkLeafKernel]);
   rep.dfs_bv_tests = static_cast<long long>(H[kWorkDfsBV]);
-  rep.dfs_leaf_tests = static_cast<long long>(H[kWorkDfsLeaf]);
+  rep.dfs_leaf_tests = static_cast<long long>(H[kWorkDfsLeaf] + H[kWorkDfsInline]);
+  rep.dfs_inline_leaf_tests = static_cast<long long>(H[kWorkDfsInline]);
   rep.fp64_fallbacks = static_cast<long long>(H[kWorkFallback]);
   ENG_TRY(clock_collect(ws));
   return hipSuccess;

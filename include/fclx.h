@@ -354,6 +354,8 @@ typedef struct {
   double bool_end_p50_ms; /* wavefront ends after the first start: median, */
   double bool_end_p90_ms; /* 90th percentile */
   double bool_span_ms;    /* and last, ms */
+  int64_t dfs_inline_leaf_tests; /* of dfs_leaf_tests: those the depth-first kernel ran
+                                  * itself (the rest ran in the deferred leaf kernel) */
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
@@ -368,6 +370,13 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "idle_stop"         0..1    share of idle wavefronts that ends a round (0: never)
  *   "stop_poll"         1..1024 depth-first steps between checks for a round's end
  *   "dfs_waves_per_simd" 4..6   register budget of the depth-first kernel
+ *   "dfs_inline_leaves" 0, 1    1: at 4 wavefronts per SIMD the depth-first
+ *                               kernel runs the triangle tests of its leaf
+ *                               pairs itself and keeps the top of its stack
+ *                               in LDS (default 1); 0: leaf pairs deferred
+ *                               to a separate kernel, stack in device memory
+ *   "dfs_stack_top"     128, 256, 512  stack entries per wavefront kept in
+ *                               LDS with inline triangle tests (default 512)
  *   "bool_queries_per_wave" 0, 4, 8, 16, 32  queries a BOOL-engine wavefront
  *                               keeps in flight (default 0: 4 for batches
  *                               below 65536 queries, else 8)

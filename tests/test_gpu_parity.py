@@ -228,15 +228,22 @@ def test_fast_persistent_graph_replays(ctx):
 # entries, so depth-first rounds stop constantly because a buffer is full;
 # "idle": a wide front and one resident block per CU, and a round ends as
 # soon as one wavefront idles, so the other stacks are redistributed.
+_FULL = {"defer_leaf": 256, "defer_uncertain": 64, "front_target": 64, "max_levels": 2,
+         "dfs_blocks_per_cu": 1, "stop_poll": 1}
 STRESS = {
-    "full": {"defer_leaf": 256, "defer_uncertain": 64, "front_target": 64, "max_levels": 2,
-             "dfs_blocks_per_cu": 1, "stop_poll": 1},
+    # leaf pairs all deferred, so the small leaf buffer fills too
+    "full": dict(_FULL, dfs_inline_leaves=0),
+    # leaf pairs tested inside the depth-first kernel: only the uncertain
+    # buffer ends rounds, and inline hits and deferred ones share the output
+    # (and a small LDS stack top, so stacks spill to device memory and are
+    # read back, and stack hand-overs take both parts)
+    "full_inline": dict(_FULL, dfs_inline_leaves=1, dfs_stack_top=128),
     "idle": {"front_target": 100000, "idle_stop": 0.0005, "dfs_blocks_per_cu": 1,
              "stop_poll": 1},
 }
 
 
-@pytest.mark.parametrize("mode", ["full", "idle"])
+@pytest.mark.parametrize("mode", ["full", "full_inline", "idle"])
 @pytest.mark.parametrize("cap,stats", [(100000, True), (70, False)])
 def test_fast_engine_round_boundaries(mode, cap, stats):
     """The FAST-engine checks with tunables that end depth-first rounds early:
@@ -252,6 +259,8 @@ def test_fast_engine_round_boundaries(mode, cap, stats):
         c.close()
     if mode == "full":
         assert rep["dfs_rounds"] >= 5  # (how many depends on timing)
+    elif mode == "full_inline":
+        assert rep["dfs_rounds"] >= 2
     else:
         # rebalancing is armed only when the front gives every wavefront a
         # full step (one block of 4 wavefronts per CU here)
