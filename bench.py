@@ -1176,6 +1176,8 @@ def main():
         "valu_peak_tflops": PEAK_FP32 / 1e12,
         "kernel_ms": rep["dfs_ms"],
         "bv_tests": rep["dfs_bv_tests"],
+        "lanes_per_step": (rep["dfs_bv_tests"] / rep["dfs_wave_steps"]
+                           if rep.get("dfs_wave_steps") else None),
         "leaf_tests_in_kernel": inline_leaf,
         "leaf_tests_in_engine_leaves": rep["dfs_leaf_tests"] - inline_leaf,
         "fp64_fallback_rate": rep["fp64_fallbacks"] / max(1, rep["dfs_bv_tests"] +

@@ -113,7 +113,8 @@ class EngineReport(ctypes.Structure):
                 ("dfs_end_max_ms", c_dbl), ("bool_steps", c_i64 * 3), ("bool_lanes", c_i64 * 3),
                 ("bool_skipped", c_i64), ("bool_rounds", c_i64), ("bool_stack_max", c_i64),
                 ("bool_wave_ms", c_dbl), ("bool_end_p50_ms", c_dbl), ("bool_end_p90_ms", c_dbl),
-                ("bool_span_ms", c_dbl), ("dfs_inline_leaf_tests", c_i64)]
+                ("bool_span_ms", c_dbl), ("dfs_inline_leaf_tests", c_i64),
+                ("dfs_wave_steps", c_i64)]
 
 
 # name -> (restype, argtypes)

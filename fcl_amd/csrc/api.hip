@@ -1168,6 +1168,7 @@ int fclx_engine_last_report(fclx_context* ctx, fclx_engine_report* out) {
   out->bool_end_p90_ms = r.bool_end_p90_ms;
   out->bool_span_ms = r.bool_span_ms;
   out->dfs_inline_leaf_tests = r.dfs_inline_leaf_tests;
+  out->dfs_wave_steps = r.dfs_wave_steps;
   return FCLX_OK;
 }
 

@@ -79,6 +79,7 @@ struct EngineReport {
   double bool_wave_ms;  // mean wavefront lifetime
   double bool_end_p50_ms, bool_end_p90_ms, bool_span_ms;  // wavefront ends after the first start
   long long dfs_inline_leaf_tests;  // of dfs_leaf_tests: run inside the depth-first kernel
+  long long dfs_wave_steps;         // BV-test steps of the depth-first wavefronts
 };
 hipError_t engine_set_profiling(EngineWorkspace* ws, bool on);
 const EngineReport& engine_report(const EngineWorkspace* ws);

@@ -214,7 +214,7 @@ struct EngineArgs {
 };
 
 enum : int { kWorkExpandBV = 0, kWorkLeafKernel, kWorkDfsBV, kWorkDfsLeaf, kWorkFallback,
-             kWorkDfsInline };
+             kWorkDfsInline, kWorkDfsSteps };
 
 // add popcount(pred) over the wavefront to *counter with one atomic
 __device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
@@ -614,6 +614,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
   bool stopped = false;
   unsigned long long n_bv = 0, n_unc = 0;  // wavefront-uniform
   unsigned long long n_leaf = 0;           // inline triangle tests
+  unsigned long long n_steps = 0;          // BV-test steps (n_bv / n_steps: lanes used)
   int nh = 0;                              // staged hit records (kInline)
   int32_t* hits = hits_all[kInline ? wave : 0];
   const CollideLaunch& L = A.L;
@@ -719,6 +720,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
     unsigned long long task = 0, c0 = 0, c1 = 0;
     if (active) task = slot(sp + lane);
     n_bv += n;
+    ++n_steps;
     const int r = active ? step32(A, task, c0, c1) : kStepNone;
     // children onto the stack (one prefix sum)
     const unsigned long long pm = __ballot(r == kStepPush);
@@ -788,6 +790,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
     atomicAdd(A.work + kWorkDfsBV, n_bv);
     if (n_unc) atomicAdd(A.work + kWorkFallback, n_unc);
     if (n_leaf) atomicAdd(A.work + kWorkDfsInline, n_leaf);
+    if (n_steps) atomicAdd(A.work + kWorkDfsSteps, n_steps);
   }
 }
 
@@ -2645,7 +2648,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   }
   rep.flagged = static_cast<int>(nflag);
   ENG_TRY(hipMemcpyAsync(ws->host_counters + kCtrWork, C + kCtrWork,
-                         6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+                         7 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   ENG_TRY(hipStreamSynchronize(s));
   const unsigned long long* H = ws->host_counters + kCtrWork;
   rep.expand_bv_tests = static_cast<long long>(H[kWorkExpandBV]);
@@ -2653,6 +2656,7 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   rep.dfs_bv_tests = static_cast<long long>(H[kWorkDfsBV]);
   rep.dfs_leaf_tests = static_cast<long long>(H[kWorkDfsLeaf] + H[kWorkDfsInline]);
   rep.dfs_inline_leaf_tests = static_cast<long long>(H[kWorkDfsInline]);
+  rep.dfs_wave_steps = static_cast<long long>(H[kWorkDfsSteps]);
   rep.fp64_fallbacks = static_cast<long long>(H[kWorkFallback]);
   ENG_TRY(clock_collect(ws));
   return hipSuccess;

@@ -356,6 +356,8 @@ typedef struct {
   double bool_span_ms;    /* and last, ms */
   int64_t dfs_inline_leaf_tests; /* of dfs_leaf_tests: those the depth-first kernel ran
                                   * itself (the rest ran in the deferred leaf kernel) */
+  int64_t dfs_wave_steps;        /* BV-test steps of depth-first wavefronts (dfs_bv_tests /
+                                  * dfs_wave_steps: lanes a step used, of 64) */
 } fclx_engine_report;
 
 int fclx_set_profiling(fclx_context* ctx, int32_t enable);
