@@ -969,7 +969,7 @@ def main():
                     default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per load")
-    ap.add_argument("--serve-streams", type=int, default=3)
+    ap.add_argument("--serve-streams", type=int, default=6)
     ap.add_argument("--serve-batch", type=int, default=0,
                     help="queries per serving batch at most (0: the loop's default, 65536)")
     ap.add_argument("--serve-pool", type=int, default=64)

@@ -539,7 +539,7 @@ class Server:
     ((waypoints, links, 12) float64); its result is a (waypoints, links)
     uint8 array, 1 where the link collides with the scene."""
 
-    def __init__(self, ctx, scene_mesh, scene_pose=None, num_streams=2, hbm_budget=0,
+    def __init__(self, ctx, scene_mesh, scene_pose=None, num_streams=6, hbm_budget=0,
                  max_batch_queries=0, engine=ENGINE_BOOL):
         self._ctx = ctx
         self._L = ctx._L
