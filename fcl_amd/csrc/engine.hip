@@ -1697,7 +1697,7 @@ struct EngineWorkspace {
   // line each) and waits for the slot's previous launch to end
   static constexpr int kOrdSlots = 16;
   // stack areas of the boolean kernel, a ring guarded like the counters
-  static constexpr int kAnySlots = 4;
+  static constexpr int kAnySlots = 8;
   unsigned long long* any_stk[kAnySlots] = {};
   size_t any_cap[kAnySlots] = {};
   hipEvent_t any_ev[kAnySlots] = {};
@@ -2221,7 +2221,17 @@ hipError_t engine_anyhit_collide(EngineWorkspace* ws, const CollideLaunch& L, in
   const int K = a.K, wps = a.wps, slot_cap = a.slot_cap;
   const unsigned grid = a.grid;
   const size_t need = a.words();
-  const int slot = ws->any_next;
+  // the next area in turn whose last batch has finished (an area still in
+  // use makes this batch wait on the device for that batch, on whatever
+  // stream it runs); if every area is busy, the next in turn
+  int slot = ws->any_next;
+  for (int i = 0; i < EngineWorkspace::kAnySlots; ++i) {
+    const int c = (ws->any_next + i) % EngineWorkspace::kAnySlots;
+    if (!ws->any_used[c] || hipEventQuery(ws->any_ev[c]) == hipSuccess) {
+      slot = c;
+      break;
+    }
+  }
   ws->any_next = (slot + 1) % EngineWorkspace::kAnySlots;
   if (need > ws->any_cap[slot]) {
     // Every area of the ring grows at once, to what any batch with these

@@ -66,7 +66,7 @@ extern "C" {
  * records and no statistics (bv_tests / leaf_tests are zeroed); a wavefront
  * keeps 4 or 8 queries in flight, each descending nearly depth first on a
  * stack of its own, and stops a query at its first intersecting triangle
- * pair.  Batches on different streams may run at once (up to four without
+ * pair.  Batches on different streams may run at once (up to eight without
  * waiting for each other's work areas).  Batches
  * beyond 2^22 queries or over meshes of 2^20 or more triangles run the
  * PLAIN kernel with a cap of 1 (same answers). */
