@@ -17,8 +17,14 @@
 // the bound vertices' covariance), descends the mesh depth first with the
 // reference's child order and stops at num_max_contacts.  Without
 // enable_contact a GJK "inside" is the answer (EPA's status is never Failed
-// after it); with it the lane runs EPA in a block of device memory of its
-// own, so that grid is persistent and sized to the EPA memory it holds.
+// after it).  With it, GJK still decides the hit and the record's slot
+// (count, cap and order are GJK's), and EPA only supplies the slot's
+// contact: the lane queues GJK's final simplex with the slot, and a second
+// kernel runs EPA over the queue, each lane in a block of device memory of
+// its own, taking the next queued pair when its own ends.  (When the queue
+// for num_queries x num_max_contacts slots would not fit its budget, or
+// for meshes deeper than the LDS stack, the descending lane runs EPA
+// itself, in a grid sized to the EPA memory it holds.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +44,15 @@ constexpr int kMsStack = 64;  // pending right children in LDS: meshes up to thi
 constexpr int kMsDeepMax = 4096;
 constexpr int kMsDeepBlocksPerCu = 2;
 constexpr int kEpaBlocksPerCu = 2;
+constexpr size_t kEpaQueueBudget = size_t(4) << 30;  // bytes of queued EPA seeds at most
+
+// A hit whose contact EPA supplies: GJK's final simplex, the query, the
+// triangle and the record slot the contact goes to.
+struct MsEpaSeed {
+  shp::D3 w[4], d[4];
+  int32_t q, rank, prim, pad;
+  long long slot;
+};
 
 using shp::D3;
 using shp::mk;
@@ -434,6 +449,39 @@ __device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, c
   return hit;
 }
 
+// The queued form of leaf_test<true>: a sphere's contact from the closed
+// form; for the other shapes GJK alone, its final simplex in g and
+// need_epa set on a hit.
+__device__ __noinline__ bool leaf_test_gjk(const shp::Shape& s, const shp::Xf& T1,
+                                           const shp::Xf& T2, const double* tv, double tol,
+                                           shp::Contact& c, shp::Gjk& g, bool& need_epa) {
+  need_epa = false;
+  if (s.type == shp::kSphere) {
+    D3 P[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P[k] = shp::apply(T1, mk(tv[3 * k], tv[3 * k + 1], tv[3 * k + 2]));
+    D3 p, n;
+    double d;
+    const bool hit = sphere_triangle(s.dims[0], T2.t, P[0], P[1], P[2], p, d, n);
+    if (hit) {
+      c.pos = p;
+      c.normal = shp::neg(n);
+      c.depth = d;
+    }
+    return hit;
+  }
+  shp::Shape tri;
+  tri.type = shp::kTriangle;
+  tri.nv = 3;
+  tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
+  tri.verts = tv;
+  shp::MDiff m;
+  shp::make_mdiff(s, T2, tri, T1, m);
+  const bool hit = shp::gjk_evaluate(g, m, tol) == shp::kGjkInside;
+  need_epa = hit;
+  return hit;
+}
+
 // Lanes of a wavefront step together: a lane whose descent reached a leaf
 // that passed its BV test waits there, the others keep descending, and the
 // waiting lanes test their leaves together once they are a set share of the
@@ -445,16 +493,22 @@ __device__ __noinline__ bool leaf_test(const shp::Shape& s, const shp::Xf& T1, c
 // kDeep: the pending right children live in device memory (deep_stack, one
 // column of L.max_depth entries per lane, lane-interleaved) instead of LDS,
 // for contexts holding a mesh deeper than kMsStack levels.
-template <bool kContact, int kWaves, bool kDeep>
+//
+// kQueue (with kContact): hits that need EPA are queued (seeds, one
+// reservation per wavefront on nseed) instead of running EPA here.
+template <bool kContact, int kWaves, bool kDeep, bool kQueue = false>
 __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunch L,
                                                                   const int32_t* order,
                                                                   shp::EpaLane* lanes,
                                                                   shp::EpaWave* waves,
-                                                                  int* deep_stack) {
+                                                                  int* deep_stack,
+                                                                  MsEpaSeed* seeds = nullptr,
+                                                                  unsigned long long* nseed =
+                                                                      nullptr) {
   const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * kBlk;
   shp::Epa E;
-  if constexpr (kContact) {
+  if constexpr (kContact && !kQueue) {
     E.M = lanes + gid;
     E.S = waves + gid / 64;
     E.lane = threadIdx.x & 63;
@@ -546,18 +600,49 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
       pend = -1;
       ++nleaf;
       shp::Contact c;
-      const bool hit = leaf_test<kContact>(s, T1, T2, tris + static_cast<size_t>(prim) * kTriVertsStride,
-                                           L.tol, E, c);
+      bool hit, need_epa = false;
+      shp::Gjk g;
+      const double* tv = tris + static_cast<size_t>(prim) * kTriVertsStride;
+      if constexpr (kQueue)
+        hit = leaf_test_gjk(s, T1, T2, tv, L.tol, c, g, need_epa);
+      else
+        hit = leaf_test<kContact>(s, T1, T2, tv, L.tol, E, c);
+      if constexpr (kQueue) {
+        // the seeds of this leaf round: one reservation for the wavefront
+        const unsigned long long em = __ballot(need_epa);
+        if (em != 0) {
+          const int leader = __ffsll(static_cast<long long>(em)) - 1;
+          unsigned long long b = 0;
+          if (static_cast<int>(threadIdx.x & 63) == leader)
+            b = atomicAdd(nseed, static_cast<unsigned long long>(__popcll(em)));
+          b = __shfl(b, leader);
+          if (need_epa) {
+            MsEpaSeed& sd =
+                seeds[b + __popcll(em & ((1ULL << (threadIdx.x & 63)) - 1ULL))];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              sd.w[j] = g.w[j];
+              sd.d[j] = g.d[j];
+            }
+            sd.q = static_cast<int32_t>(q);
+            sd.rank = g.rank;
+            sd.prim = prim;
+            sd.slot = static_cast<long long>(q * L.cap + count);
+          }
+        }
+      }
       if (hit) {
         const size_t k = q * L.cap + count;
         if (L.prims) L.prims[k] = prim;
         if constexpr (kContact) {
+          if (!need_epa) {
 #pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            L.contact_pos[3 * k + j] = c.pos.v[j];
-            L.contact_normal[3 * k + j] = c.normal.v[j];
+            for (int j = 0; j < 3; ++j) {
+              L.contact_pos[3 * k + j] = c.pos.v[j];
+              L.contact_normal[3 * k + j] = c.normal.v[j];
+            }
+            L.contact_depth[k] = c.depth;
           }
-          L.contact_depth[k] = c.depth;
         }
         if (++count >= L.cap) {
           finish();
@@ -585,9 +670,118 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
   }
 }
 
+// EPA over the queued hits, each lane on its own pair one EPA loop pass per
+// trip, taking the wavefront's next entry when its own ends (the entries
+// come 64 at a time per wavefront from one counter), so that lanes do not
+// wait for the slowest pair of their wavefront.  The contact goes to the
+// slot the descent reserved, as leaf_test<true> writes it.
+template <int kWaves>
+__global__ void __launch_bounds__(kBlk, kWaves) meshshape_epa(MeshShapeLaunch L,
+                                                              const MsEpaSeed* seeds,
+                                                              const unsigned long long* nseed,
+                                                              unsigned long long* grab,
+                                                              shp::EpaLane* lanes,
+                                                              shp::EpaWave* waves) {
+  constexpr int kRefillAt = 16;
+  const int lane = threadIdx.x & 63;
+  const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
+  const long long total = static_cast<long long>(*nseed);
+  long long cursor = 0, stop = 0;  // the wavefront's entries not yet started
+  bool exhausted = total == 0;
+  shp::Epa E;
+  E.M = lanes + gid;
+  E.S = waves + gid / 64;
+  E.lane = lane;
+  shp::MDiff m;
+  shp::Xf T2;
+  long long slot = 0;
+  bool active = false;
+  while (true) {
+    bool finish = false, started = false;
+    const unsigned long long idle = __ballot(!active);
+    const int nidle = __popcll(idle);
+    if (cursor >= stop && !exhausted && (nidle == 64 || nidle >= kRefillAt)) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(grab, 64ULL);
+      base = __shfl(base, 0);
+      const long long b = static_cast<long long>(base);
+      cursor = b < total ? b : total;
+      stop = b + 64 < total ? b + 64 : total;
+      exhausted = b + 64 >= total;
+    }
+    if (cursor < stop && (nidle == 64 || nidle >= kRefillAt)) {
+      const long long v = cursor + __popcll(idle & ((1ULL << lane) - 1ULL));
+      cursor = cursor + nidle < stop ? cursor + nidle : stop;
+      if (!active && v < stop) {
+        const MsEpaSeed& sd = seeds[v];
+        const int q = sd.q;
+        shp::Shape s;
+        (void)load_query(L, q, s, kMsDeepMax);  // checked by the descent
+        const MeshDev& M = L.meshes[L.query_meshes[q]];
+        shp::Shape tri;
+        tri.type = shp::kTriangle;
+        tri.nv = 3;
+        tri.dims[0] = tri.dims[1] = tri.dims[2] = 0;
+        tri.verts = M.tri_verts + static_cast<size_t>(sd.prim) * kTriVertsStride;
+        const shp::Xf T1 = shp::xf_load(L.tf1 + 12 * static_cast<size_t>(q));
+        T2 = shp::xf_load(L.tf2 + 12 * static_cast<size_t>(q));
+        shp::make_mdiff(s, T2, tri, T1, m);
+        shp::Gjk g;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          g.w[j] = sd.w[j];
+          g.d[j] = sd.d[j];
+        }
+        g.rank = sd.rank;
+        slot = sd.slot;
+        active = shp::epa_start(E, g, m, L.tol);
+        finish = !active;
+        started = true;
+      }
+    }
+    if (active && !started) {
+      active = shp::epa_step(E, m, L.tol);
+      finish = !active;
+    }
+    if (finish) {
+      // the leaf's -normal undoes the solver's: EPA's normal stays
+      const shp::Contact c = shp::epa_finish(E, m, T2);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        L.contact_pos[3 * slot + j] = c.pos.v[j];
+        L.contact_normal[3 * slot + j] = c.normal.v[j];
+      }
+      L.contact_depth[slot] = c.depth;
+    }
+    if (exhausted && cursor >= stop && !__ballot(active)) break;
+  }
+}
+
 }  // namespace
 
 size_t mesh_shape_epa_lane_bytes() { return sizeof(shp::EpaLane) + sizeof(shp::EpaWave) / 64; }
+
+// the queued-EPA descent for a register budget of 1, 2 or 3 waves per SIMD
+void launch_queued(int waves, unsigned grid, hipStream_t s, const MeshShapeLaunch& L,
+                   const int32_t* order, MsEpaSeed* seeds, unsigned long long* nseed) {
+  switch (waves) {
+    case 1:
+      meshshape_collide<true, 1, false, true>
+          <<<grid, kBlk, 0, s>>>(L, order, nullptr, nullptr, nullptr, seeds, nseed);
+      break;
+    case 3:
+      meshshape_collide<true, 3, false, true>
+          <<<grid, kBlk, 0, s>>>(L, order, nullptr, nullptr, nullptr, seeds, nseed);
+      break;
+    default:
+      meshshape_collide<true, 2, false, true>
+          <<<grid, kBlk, 0, s>>>(L, order, nullptr, nullptr, nullptr, seeds, nseed);
+  }
+}
+
+__global__ void ms_zero_u64(unsigned long long* p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0;
+}
 
 template <bool kContact>
 void launch_variant(int waves, unsigned grid, hipStream_t s, const MeshShapeLaunch& L,
@@ -659,6 +853,45 @@ hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipSt
                           deep_stack);
     e = hipGetLastError();
     if (deep_stack) (void)hipFreeAsync(deep_stack, s);
+    if (order) (void)hipFreeAsync(order, s);
+    return e;
+  }
+  // contacts with EPA queued: the descent on every query at once, then EPA
+  // on a persistent grid sized to its memory
+  const size_t nslots = static_cast<size_t>(L.num_queries) * static_cast<size_t>(L.cap);
+  if (!deep && nslots * sizeof(MsEpaSeed) <= kEpaQueueBudget) {
+    MsEpaSeed* seeds = nullptr;
+    unsigned long long* ctr = nullptr;  // [0] seeds queued, [1] seeds handed out
+    e = hipMallocAsync(reinterpret_cast<void**>(&seeds), nslots * sizeof(MsEpaSeed), s);
+    if (e == hipSuccess) {
+      e = hipMallocAsync(reinterpret_cast<void**>(&ctr), 2 * sizeof(unsigned long long), s);
+      if (e != hipSuccess) (void)hipFreeAsync(seeds, s);
+    }
+    if (e != hipSuccess) {
+      if (order) (void)hipFreeAsync(order, s);
+      return e;
+    }
+    ms_zero_u64<<<1, 64, 0, s>>>(ctr);
+    launch_queued(waves, static_cast<unsigned>(blocks), s, L, order, seeds, ctr);
+    e = hipGetLastError();
+    const long long egrid = static_cast<long long>(cus) * kEpaBlocksPerCu;
+    const size_t nl = static_cast<size_t>(egrid) * kBlk;
+    shp::EpaLane* lanes = nullptr;
+    shp::EpaWave* wv = nullptr;
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&lanes), nl * sizeof(shp::EpaLane), s);
+    if (e == hipSuccess) {
+      e = hipMallocAsync(reinterpret_cast<void**>(&wv), nl / 64 * sizeof(shp::EpaWave), s);
+      if (e != hipSuccess) (void)hipFreeAsync(lanes, s);
+    }
+    if (e == hipSuccess) {
+      meshshape_epa<2><<<static_cast<unsigned>(egrid), kBlk, 0, s>>>(L, seeds, ctr, ctr + 1,
+                                                                        lanes, wv);
+      e = hipGetLastError();
+      (void)hipFreeAsync(lanes, s);
+      (void)hipFreeAsync(wv, s);
+    }
+    (void)hipFreeAsync(seeds, s);
+    (void)hipFreeAsync(ctr, s);
     if (order) (void)hipFreeAsync(order, s);
     return e;
   }
