@@ -26,7 +26,7 @@ struct EngineTuning {
   DistanceTuning dist;          // FAST distance engine
   int meshshape_waves_per_simd = 2;  // register budget of the mesh-vs-primitive kernel
   int meshshape_leaf_share = 0;      // percent of live lanes at leaves that starts a leaf round
-                                     // (0: 50 for booleans, 100 with contact geometry)
+                                     // (0: 50 for booleans, 75 with contact geometry)
   int max_levels = 24;          // breadth-first levels before a depth-first round
   long long front_target = 1 << 18;  // stop expanding once the front is this large (a full
                                      // step for every resident wavefront)

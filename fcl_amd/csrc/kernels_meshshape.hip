@@ -809,8 +809,9 @@ constexpr int kSortMin = 4096;
 hipError_t launch_mesh_shape_collide(const MeshShapeLaunch& in, int waves, hipStream_t s) {
   if (in.num_queries <= 0) return hipSuccess;
   MeshShapeLaunch L = in;
-  // measured: half the live lanes for booleans, all of them before EPA
-  if (L.leaf_share <= 0) L.leaf_share = L.contact ? 100 : 50;
+  // measured: half the live lanes for booleans, three quarters with contact
+  // geometry (whose leaf rounds queue EPA's work for a pass of its own)
+  if (L.leaf_share <= 0) L.leaf_share = L.contact ? 75 : 50;
   const long long blocks = (static_cast<long long>(L.num_queries) + kBlk - 1) / kBlk;
   // the order of queries grouped by shape kind (stream-ordered memory)
   int32_t* order = nullptr;

@@ -398,8 +398,8 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *                               kernel (default 2)
  *   "meshshape_leaf_share" 0..100  percent of a wavefront's live queries
  *                               waiting at leaves that starts their leaf
- *                               tests (default 0: 50 for booleans, 100
- *                               with contact geometry, where EPA follows)
+ *                               tests (default 0: 50 for booleans, 75
+ *                               with contact geometry)
  *   "dist_width"        1..64   most bounds of one query per distance step
  *                               (default 32)
  *   "dist_leaf_batch"   1..64   leaf pairs queued before their distances are
