@@ -643,26 +643,41 @@ __device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double 
 
 // epa-inl.h:193-209 findBest: the first face, in list order from the root,
 // whose d * d is below all before it -- the smallest d * d, ties to the
-// latest append; a NaN root is never displaced.
+// latest append; a NaN root is never displaced.  The scan reads eight
+// faces' stamps and distances at a time (all in bounds: kNfEpa is a
+// multiple of 8) before it compares them in slot order, so that their
+// loads are in flight together instead of one after another.
 __device__ inline int find_best(const Epa& E) {
+  static_assert(kNfEpa % 8 == 0, "the scan reads whole groups of eight faces");
   const EpaWave& S = *E.S;
   int root = -1, root_stamp = -1;
   double root_dsq = 0;
   int best = -1, best_stamp = -1;
   double best_dsq = 0;
-  for (int f = 0; f < E.hi; ++f) {
-    const int s = S.stamp[f][E.lane];
-    if (s < 0) continue;
-    const double v = S.dsq[f][E.lane];
-    if (s > root_stamp) {
-      root = f;
-      root_stamp = s;
-      root_dsq = v;
+  for (int f0 = 0; f0 < E.hi; f0 += 8) {
+    int st[8];
+    double dv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      st[k] = S.stamp[f0 + k][E.lane];
+      dv[k] = S.dsq[f0 + k][E.lane];
     }
-    if (v == v && (best < 0 || v < best_dsq || (v == best_dsq && s > best_stamp))) {
-      best = f;
-      best_dsq = v;
-      best_stamp = s;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int f = f0 + k;
+      const int s = f < E.hi ? st[k] : -1;
+      if (s < 0) continue;
+      const double v = dv[k];
+      if (s > root_stamp) {
+        root = f;
+        root_stamp = s;
+        root_dsq = v;
+      }
+      if (v == v && (best < 0 || v < best_dsq || (v == best_dsq && s > best_stamp))) {
+        best = f;
+        best_dsq = v;
+        best_stamp = s;
+      }
     }
   }
   return (root_dsq != root_dsq || best < 0) ? root : best;
