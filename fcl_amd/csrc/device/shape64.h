@@ -703,7 +703,8 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
   bool call = true;  // evaluate (f, e) as a fresh call
   while (true) {
     if (call) {
-      EpaFace& F = M.fc[f];
+      // the whole face record at once (one round trip, not one per field)
+      const EpaFace F = M.fc[f];
       result = false;
       if (F.pass != pass) {
         const int e1 = next3(e);
@@ -720,7 +721,7 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
             result = true;
           }
         } else {
-          F.pass = pass;
+          M.fc[f].pass = pass;
           // every frame marks a distinct face with this pass: at most kNfEpa
           if (sp >= kNfEpa) return false;
           M.xs[sp++] = static_cast<int16_t>(f | (e << 7) | (1 << 9));
@@ -827,19 +828,20 @@ __device__ inline bool epa_step(Epa& E, const MDiff& m, double tol) {
   const int w = kNvGjk + E.nextsv++;
   bool valid = true;
   const int best = E.best;
+  const EpaFace B = M.fc[best];  // the whole record at once
   M.fc[best].pass = ++E.pass;
-  const D3 bn = M.fc[best].n;
+  const D3 bn = B.n;
   const D3 wd = normalized(bn);
   const D3 ww = msupport(m, wd);
   M.dir[w] = wd;
   M.w[w] = ww;
-  const double wdist = dot(bn, ww) - M.fc[best].d;
+  const double wdist = dot(bn, ww) - B.d;
   if (!(wdist > tol)) {
     E.status = kEpaAccuracyReached;
     return false;
   }
-  for (int j = 0; j < 3 && valid; ++j)
-    valid &= expand(E, E.pass, w, M.fc[best].f[j], M.fc[best].e[j], h, tol);
+  // (expand changes other faces' links, never the best face's own)
+  for (int j = 0; j < 3 && valid; ++j) valid &= expand(E, E.pass, w, B.f[j], B.e[j], h, tol);
   if (!(valid && h.nf >= 3)) {
     E.status = kEpaInvalidHull;
     return false;
