@@ -142,15 +142,26 @@ __device__ inline D3 support(const Shape& s, const D3& d) {
       if (da > db) return dc > da ? c : a;
       return dc > db ? c : b;
     }
-    default: {  // convex
+    default: {  // convex: the first vertex of largest d . v
       double maxdot = -1.7976931348623157e308;
       D3 best = mk(0, 0, 0);
-      for (int i = 0; i < s.nv; ++i) {
-        const D3 p = mk(s.verts[3 * i], s.verts[3 * i + 1], s.verts[3 * i + 2]);
-        const double dt = dot(d, p);
-        if (dt > maxdot) {
-          best = p;
-          maxdot = dt;
+      // four vertices' loads at a time (indices clamped to the last vertex,
+      // so every load stays in the convex's range), compared in order
+      for (int i0 = 0; i0 < s.nv; i0 += 4) {
+        D3 p[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = i0 + k < s.nv ? i0 + k : s.nv - 1;
+          p[k] = mk(s.verts[3 * i], s.verts[3 * i + 1], s.verts[3 * i + 2]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (i0 + k >= s.nv) break;
+          const double dt = dot(d, p[k]);
+          if (dt > maxdot) {
+            best = p[k];
+            maxdot = dt;
+          }
         }
       }
       return best;
