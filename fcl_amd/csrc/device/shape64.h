@@ -593,9 +593,9 @@ __device__ __forceinline__ void bind(EpaLane& M, int fa, int ea, int fb, int eb)
   M.fc[fb].f[eb] = static_cast<int8_t>(fa);
 }
 
-// epa-inl.h:109-141 getEdgeDist (with the reference's distance formula)
-__device__ inline bool edge_dist(const EpaLane& M, const D3& n, int a, int b, double& dist) {
-  const D3 wa = M.w[a], wb = M.w[b];
+// epa-inl.h:109-141 getEdgeDist (with the reference's distance formula),
+// on the edge's two support points
+__device__ inline bool edge_dist(const D3& wa, const D3& wb, const D3& n, double& dist) {
   const D3 ba = sub(wb, wa);
   const D3 n_ab = cross(ba, n);
   if (dot(wa, n_ab) < 0) {
@@ -632,12 +632,14 @@ __device__ inline int new_face(Epa& E, int a, int b, int c, bool forced, double 
   F.c[0] = static_cast<int8_t>(a);
   F.c[1] = static_cast<int8_t>(b);
   F.c[2] = static_cast<int8_t>(c);
-  const D3 wa = M.w[a];
-  const D3 n = cross(sub(M.w[b], wa), sub(M.w[c], wa));
+  // the three support points once (the face writes above could alias them
+  // for the compiler, which would read them again for every use)
+  const D3 wa = M.w[a], wb = M.w[b], wc = M.w[c];
+  const D3 n = cross(sub(wb, wa), sub(wc, wa));
   const double l = nrm(n);
   if (l > tol) {
     double d;
-    if (!(edge_dist(M, n, a, b, d) || edge_dist(M, n, b, c, d) || edge_dist(M, n, c, a, d)))
+    if (!(edge_dist(wa, wb, n, d) || edge_dist(wb, wc, n, d) || edge_dist(wc, wa, n, d)))
       d = dot(wa, n) / l;
     F.d = d;
     E.S->dsq[f][E.lane] = d * d;
