@@ -564,6 +564,10 @@ struct Epa {
   EpaLane* M;
   EpaWave* S;
   int lane;
+  // the horizon walk's stack: frame i at xs[i * xs_stride] (the lane's own
+  // M->xs with stride 1, or a lane's column of an LDS array with stride 64)
+  int16_t* xs = nullptr;
+  int xs_stride = 1;
   int nfreed;      // entries on the freed stack
   int hi;          // slots handed out so far: [0, hi)
   int hull_count;
@@ -737,7 +741,7 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
           M.fc[f].pass = pass;
           // every frame marks a distinct face with this pass: at most kNfEpa
           if (sp >= kNfEpa) return false;
-          M.xs[sp++] = static_cast<int16_t>(f | (e << 7) | (1 << 9));
+          E.xs[sp++ * E.xs_stride] = static_cast<int16_t>(f | (e << 7) | (1 << 9));
           const int fe1 = F.f[e1], ee1 = F.e[e1];
           f = fe1;
           e = ee1;
@@ -748,14 +752,14 @@ __device__ inline bool expand(Epa& E, int pass, int w, int f0, int e0, Horizon& 
     }
     // return `result` to the parent frame
     if (sp == 0) return result;
-    const int top = M.xs[sp - 1];
+    const int top = E.xs[(sp - 1) * E.xs_stride];
     const int pf = top & 127, pe = (top >> 7) & 3;
     if (top >> 9 == 1) {
       if (!result) {
         --sp;  // parent returns false
         continue;
       }
-      M.xs[sp - 1] = static_cast<int16_t>(pf | (pe << 7) | (2 << 9));
+      E.xs[(sp - 1) * E.xs_stride] = static_cast<int16_t>(pf | (pe << 7) | (2 << 9));
       const int e2 = prev3(pe);
       f = M.fc[pf].f[e2];
       e = M.fc[pf].e[e2];

@@ -507,11 +507,15 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_collide(MeshShapeLaunc
                                                                       nullptr) {
   const size_t gid = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * kBlk;
+  // the horizon walk's stack in LDS, one column per thread
+  __shared__ int16_t xs_lds[(kContact && !kQueue) ? shp::kNfEpa : 1][kBlk];
   shp::Epa E;
   if constexpr (kContact && !kQueue) {
     E.M = lanes + gid;
     E.S = waves + gid / 64;
     E.lane = threadIdx.x & 63;
+    E.xs = &xs_lds[0][threadIdx.x];
+    E.xs_stride = kBlk;
   }
   size_t qq = gid;  // position in the (sorted) order
   size_t q = 0;     // the query at that position
@@ -688,10 +692,13 @@ __global__ void __launch_bounds__(kBlk, kWaves) meshshape_epa(MeshShapeLaunch L,
   const long long total = static_cast<long long>(*nseed);
   long long cursor = 0, stop = 0;  // the wavefront's entries not yet started
   bool exhausted = total == 0;
+  __shared__ int16_t xs_lds[shp::kNfEpa][kBlk];  // the horizon walk's stack
   shp::Epa E;
   E.M = lanes + gid;
   E.S = waves + gid / 64;
   E.lane = lane;
+  E.xs = &xs_lds[0][threadIdx.x];
+  E.xs_stride = kBlk;
   shp::MDiff m;
   shp::Xf T2;
   long long slot = 0;

@@ -300,10 +300,15 @@ __global__ void __launch_bounds__(kBlk, kWaves) shape_epa(ShapeLaunch L,
   const int wid = static_cast<int>(gid / 64);
   int cursor = 0, stop = 0;  // the wavefront's entries not yet started
   bool exhausted = false;
+  // the horizon walk's stack in LDS, one column per thread (a frame is
+  // popped right after the walk returns from a child, at LDS latency)
+  __shared__ int16_t xs_lds[shp::kNfEpa][kBlk];
   shp::Epa E;
   E.M = lanes + gid;
   E.S = waves + wid;
   E.lane = lane;
+  E.xs = &xs_lds[0][threadIdx.x];
+  E.xs_stride = kBlk;
   shp::MDiff m;
   int q = -1, gjk_passes = 0;
   bool active = false;
