@@ -204,6 +204,22 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
   fill_pose<<<static_cast<unsigned>((cap + 255) / 256), 256, 0, s.stream>>>(
       s.d_tf1, static_cast<int>(cap), S->cfg);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  // the stream once through each copy a batch issues (table and poses to
+  // the device, counts and status back): the first host-device copies on a
+  // stream set up the runtime's transfer path, which blocked the worker
+  // thread for 10-12 ms inside the first requests' latency
+  const size_t warm = static_cast<size_t>(std::min<int64_t>(cap, 1 << 16));
+  std::memset(s.h_tf2, 0, 96 * warm);
+  std::memset(s.h_table, 0, table_bytes(1));
+  if ((e = hipMemcpyAsync(s.d_table, s.h_table, table_bytes(1), hipMemcpyHostToDevice,
+                          s.stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * warm, hipMemcpyHostToDevice, s.stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(s.h_counts, s.d_counts, 4 * warm, hipMemcpyDeviceToHost, s.stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(s.h_status, s.d_status, 4, hipMemcpyDeviceToHost, s.stream)) !=
+          hipSuccess)
+    return e;
   return hipStreamSynchronize(s.stream);
 }
 
