@@ -10,10 +10,13 @@
 // staging, packed by the worker and its helper threads), sends a small
 // table (each request's first query, link count and link mesh ids) from
 // which a kernel writes every query's mesh ids, launches one first-contact
-// collide batch (CollisionRequest(1, false) per link and waypoint), copies
-// the counts back and records an event; when the event has fired it writes
-// each request's results and reports its completion.  Slots run concurrently on the
-// device, so later requests overlap earlier ones.  Every query of a batch is
+// collide batch (CollisionRequest(1, false) per link and waypoint), has a
+// kernel write the counts into the slot's pinned host buffer (read-back
+// copies queued behind the batch's kernels left the worker blocked in
+// later copy calls more often) and records an event; when the event has
+// fired it writes each request's results and reports its completion.
+// Slots run concurrently on the device, so later requests overlap earlier
+// ones.  Every query of a batch is
 // independent of the others, so a request's result is the same alone and in
 // any mix.
 #include <hip/hip_runtime.h>
@@ -64,15 +67,23 @@ constexpr int64_t kSlotMin = 1 << 18;  // queries a slot holds without growing
 constexpr int kHelpers = 3;             // threads that pack poses beside the worker
 constexpr int64_t kPackChunk = 1 << 18;  // bytes of pose copy per packing job
 
-__global__ void clear_status(int32_t* status) { *status = 0; }
+// A batch's counts and status flags into the slot's pinned host buffers,
+// written by the device (no copy command that waits on the batch's kernel)
+__global__ void publish_counts(const int32_t* counts, int32_t* h_counts, int64_t n,
+                               const int32_t* status, int32_t* h_status) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) h_counts[i] = counts[i];
+  if (i == 0) *h_status = *status;
+}
 
 // Query mesh ids of a batch from its request table: requests r = 0..R-1
 // with first[r] .. first[r+1] their queries (waypoint-major), nlink[r]
 // links whose mesh ids start at links[off[r]].
 __global__ void fill_ids(int32_t* qm, const int64_t* first, const int32_t* nlink,
                          const int32_t* off, const int32_t* links, int nreq, int32_t scene,
-                         int64_t nq) {
+                         int64_t nq, int32_t* status) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k == 0) *status = 0;  // the batch's status flags, before its collide kernels
   if (k >= nq) return;
   int lo = 0, hi = nreq - 1;  // the last r with first[r] <= k
   while (lo < hi) {
@@ -204,22 +215,22 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
   fill_pose<<<static_cast<unsigned>((cap + 255) / 256), 256, 0, s.stream>>>(
       s.d_tf1, static_cast<int>(cap), S->cfg);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // the stream once through each copy a batch issues (table and poses to
-  // the device, counts and status back): the first host-device copies on a
-  // stream set up the runtime's transfer path, which blocked the worker
-  // thread for 10-12 ms inside the first requests' latency
+  // the stream once through the transfers of a batch (table and poses to
+  // the device, counts and status written to host memory by a kernel): the
+  // first host-device copies on a stream set up the runtime's transfer
+  // path, which blocked the worker thread for 10-12 ms inside the first
+  // requests' latency
   const size_t warm = static_cast<size_t>(std::min<int64_t>(cap, 1 << 16));
   std::memset(s.h_tf2, 0, 96 * warm);
   std::memset(s.h_table, 0, table_bytes(1));
   if ((e = hipMemcpyAsync(s.d_table, s.h_table, table_bytes(1), hipMemcpyHostToDevice,
                           s.stream)) != hipSuccess ||
       (e = hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * warm, hipMemcpyHostToDevice, s.stream)) !=
-          hipSuccess ||
-      (e = hipMemcpyAsync(s.h_counts, s.d_counts, 4 * warm, hipMemcpyDeviceToHost, s.stream)) !=
-          hipSuccess ||
-      (e = hipMemcpyAsync(s.h_status, s.d_status, 4, hipMemcpyDeviceToHost, s.stream)) !=
           hipSuccess)
     return e;
+  publish_counts<<<static_cast<unsigned>((warm + 255) / 256), 256, 0, s.stream>>>(
+      s.d_counts, s.h_counts, static_cast<int64_t>(warm), s.d_status, s.h_status);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   return hipStreamSynchronize(s.stream);
 }
 
@@ -289,9 +300,8 @@ int slot_launch(fclx_server* S, Slot& s) {
   const size_t tbytes = 8 * static_cast<size_t>(R + 1) + 4 * (2 * static_cast<size_t>(R) + nl);
   run_jobs(S, jobs);
   if (fclx::take_injected_failure(S->ctx)) return FCLX_ERR_DEVICE;
-  clear_status<<<1, 1, 0, s.stream>>>(s.d_status);
-  if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(s.d_table, s.h_table, tbytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+  // the copies first: the slot's stream is idle, so they wait on nothing
+  if (hipMemcpyAsync(s.d_table, s.h_table, tbytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
     return FCLX_ERR_DEVICE;
   // poses: pinned requests straight from the caller's memory, the others
   // from the staging in runs of consecutive requests
@@ -319,7 +329,7 @@ int slot_launch(fclx_server* S, Slot& s) {
     const int32_t* d_off = d_nlink + R;
     const int32_t* d_links = d_off + R;
     fill_ids<<<static_cast<unsigned>((k + 255) / 256), 256, 0, s.stream>>>(
-        s.d_qm, d_first, d_nlink, d_off, d_links, R, S->cfg.scene_mesh, k);
+        s.d_qm, d_first, d_nlink, d_off, d_links, R, S->cfg.scene_mesh, k, s.d_status);
     if (hipGetLastError() != hipSuccess) return FCLX_ERR_DEVICE;
   }
   fclx_collide_batch_args a{};
@@ -335,11 +345,9 @@ int slot_launch(fclx_server* S, Slot& s) {
   a.num_pairs = s.d_num;
   int rc = fclx::collide_batch_status(S->ctx, &a, s.stream, s.d_status);
   if (rc != FCLX_OK) return rc;
-  if (hipMemcpyAsync(s.h_counts, s.d_counts, 4 * k, hipMemcpyDeviceToHost, s.stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(s.h_status, s.d_status, 4, hipMemcpyDeviceToHost, s.stream) !=
-          hipSuccess ||
-      hipEventRecord(s.done, s.stream) != hipSuccess)
+  publish_counts<<<static_cast<unsigned>((k + 255) / 256), 256, 0, s.stream>>>(
+      s.d_counts, s.h_counts, k, s.d_status, s.h_status);
+  if (hipGetLastError() != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess)
     return FCLX_ERR_DEVICE;
   return FCLX_OK;
 }
