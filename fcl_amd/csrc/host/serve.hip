@@ -50,14 +50,14 @@ double ms_since(Clock::time_point t0) {
 }
 
 // Host memory from hipHostMalloc (or torch's pinned allocator): the device
-// can read it directly.  Pageable memory is reported as not pinned.
-bool is_pinned(const void* p) {
+// reads it directly, at the address returned.  Pageable memory: nullptr.
+const double* pinned_device_address(const double* p) {
   hipPointerAttribute_t a{};
   if (hipPointerGetAttributes(&a, p) != hipSuccess) {
     (void)hipGetLastError();
-    return false;
+    return nullptr;
   }
-  return a.type == hipMemoryTypeHost;
+  return a.type == hipMemoryTypeHost ? static_cast<const double*>(a.devicePointer) : nullptr;
 }
 
 constexpr int64_t kQueryBytes = 8 + 96 + 96 + 4;  // ids, two poses, count
@@ -74,6 +74,16 @@ __global__ void publish_counts(const int32_t* counts, int32_t* h_counts, int64_t
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) h_counts[i] = counts[i];
   if (i == 0) *h_status = *status;
+}
+
+// n 16-B words from pinned host memory into device memory, read by the
+// device over the bus.  The loop moves every batch's input this way rather
+// than with copy commands: those blocked the worker thread for 6-7 ms a few
+// times a second under load.
+__global__ void gather_words(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
 }
 
 // Query mesh ids of a batch from its request table: requests r = 0..R-1
@@ -106,7 +116,8 @@ struct Pending {
   fclx_request req;
   Clock::time_point t_submit;
   int64_t nq;
-  bool pinned;  // link_poses is pinned host memory: copied to the device directly
+  bool pinned;                // link_poses is pinned host memory: read by the device directly
+  const double* dev_poses;    // its device address
 };
 
 struct Slot {
@@ -215,16 +226,16 @@ hipError_t slot_reserve(fclx_server* S, Slot& s, int64_t n) {
   fill_pose<<<static_cast<unsigned>((cap + 255) / 256), 256, 0, s.stream>>>(
       s.d_tf1, static_cast<int>(cap), S->cfg);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // the stream once through the transfers of a batch (table and poses to
-  // the device, counts and status written to host memory by a kernel): the
-  // first host-device copies on a stream set up the runtime's transfer
-  // path, which blocked the worker thread for 10-12 ms inside the first
-  // requests' latency
+  // the stream once through the transfers of a batch (poses read from
+  // pinned host memory, counts and status written to it, by kernels and by
+  // a copy command, the fallback for unaligned poses), so that nothing is
+  // set up inside the first requests' latency
   const size_t warm = static_cast<size_t>(std::min<int64_t>(cap, 1 << 16));
   std::memset(s.h_tf2, 0, 96 * warm);
-  std::memset(s.h_table, 0, table_bytes(1));
-  if ((e = hipMemcpyAsync(s.d_table, s.h_table, table_bytes(1), hipMemcpyHostToDevice,
-                          s.stream)) != hipSuccess ||
+  gather_words<<<static_cast<unsigned>(std::min<size_t>((6 * warm + 255) / 256, 1024)), 256, 0,
+                 s.stream>>>(reinterpret_cast<const uint4*>(s.h_tf2),
+                             reinterpret_cast<uint4*>(s.d_tf2), static_cast<int64_t>(6 * warm));
+  if ((e = hipGetLastError()) != hipSuccess ||
       (e = hipMemcpyAsync(s.d_tf2, s.h_tf2, 96 * warm, hipMemcpyHostToDevice, s.stream)) !=
           hipSuccess)
     return e;
@@ -300,9 +311,20 @@ int slot_launch(fclx_server* S, Slot& s) {
   const size_t tbytes = 8 * static_cast<size_t>(R + 1) + 4 * (2 * static_cast<size_t>(R) + nl);
   run_jobs(S, jobs);
   if (fclx::take_injected_failure(S->ctx)) return FCLX_ERR_DEVICE;
-  // the copies first: the slot's stream is idle, so they wait on nothing
-  if (hipMemcpyAsync(s.d_table, s.h_table, tbytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
-    return FCLX_ERR_DEVICE;
+  // pinned host memory (src: its device address) to the device, by the
+  // gather kernel when 16-B aligned (the table is rounded up to whole
+  // words: its buffers have room), else by a copy command
+  auto to_device = [&](void* dst, const void* src, size_t bytes) {
+    const int64_t n16 = static_cast<int64_t>((bytes + 15) / 16);
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0 && (bytes % 16 == 0 || src == s.h_table)) {
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n16 + 255) / 256, 1024));
+      gather_words<<<grid, 256, 0, s.stream>>>(static_cast<const uint4*>(src),
+                                               static_cast<uint4*>(dst), n16);
+      return hipGetLastError() == hipSuccess;
+    }
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s.stream) == hipSuccess;
+  };
+  if (!to_device(s.d_table, s.h_table, tbytes)) return FCLX_ERR_DEVICE;
   // poses: pinned requests straight from the caller's memory, the others
   // from the staging in runs of consecutive requests
   int64_t run_first = -1;
@@ -311,16 +333,14 @@ int slot_launch(fclx_server* S, Slot& s) {
     if (staged && run_first < 0) run_first = s.first[r];
     if (!staged && run_first >= 0) {
       const int64_t end = r < R ? s.first[r] : k;
-      if (hipMemcpyAsync(s.d_tf2 + 12 * run_first, s.h_tf2 + 12 * run_first,
-                         96 * static_cast<size_t>(end - run_first), hipMemcpyHostToDevice,
-                         s.stream) != hipSuccess)
+      if (!to_device(s.d_tf2 + 12 * run_first, s.h_tf2 + 12 * run_first,
+                     96 * static_cast<size_t>(end - run_first)))
         return FCLX_ERR_DEVICE;
       run_first = -1;
     }
     if (r < R && s.batch[r].pinned &&
-        hipMemcpyAsync(s.d_tf2 + 12 * s.first[r], s.batch[r].req.link_poses,
-                       96 * static_cast<size_t>(s.batch[r].nq), hipMemcpyHostToDevice,
-                       s.stream) != hipSuccess)
+        !to_device(s.d_tf2 + 12 * s.first[r], s.batch[r].dev_poses,
+                   96 * static_cast<size_t>(s.batch[r].nq)))
       return FCLX_ERR_DEVICE;
   }
   {
@@ -570,7 +590,8 @@ int fclx_server_submit(fclx_server* S, const fclx_request* req) {
   p.req = *req;
   p.t_submit = Clock::now();
   p.nq = nq;
-  p.pinned = is_pinned(req->link_poses);
+  p.dev_poses = pinned_device_address(req->link_poses);
+  p.pinned = p.dev_poses != nullptr;
   {
     std::lock_guard<std::mutex> lock(S->mu);
     if (S->stop) return FCLX_ERR_INVALID_ARGUMENT;

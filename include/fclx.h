@@ -556,7 +556,7 @@ int fclx_broadphase_pairs(fclx_context* ctx, int32_t num_objects, const double* 
  * the scene as the first object), each link against the scene at each
  * waypoint.  A worker thread forms a batch for a free stream slot from the
  * waiting requests -- the smallest first, the oldest first once one has
- * waited 4 ms -- that fit the batch size and the device-memory budget, so
+ * waited 1.5 ms -- that fit the batch size and the device-memory budget, so
  * requests that arrive while others are on the GPU join the next batch and
  * batches on different streams overlap on the device.  A request's result
  * does not depend on what it is batched with. */
@@ -582,7 +582,10 @@ typedef struct {
   int32_t num_waypoints;
   const int32_t* link_meshes;  /* host: a mesh id (OBBRSS) per link */
   const double* link_poses;    /* host: num_waypoints x num_links poses (12 each),
-                                  waypoint-major, world frame */
+                                  waypoint-major, world frame; pinned memory
+                                  (hipHostMalloc, 16-B aligned) is read by the
+                                  device directly, pageable memory is packed
+                                  into the loop's pinned staging first */
   uint8_t* result;             /* host: num_waypoints x num_links, written before
                                   the completion is reported: 1 = collides */
   int64_t tag;                 /* returned with the completion */
