@@ -70,6 +70,25 @@ def measured_traffic(queries, pairs):
     return None
 
 
+def measured_valu(queries, pairs):
+    """{"insts_per_step", "busy"} of engine_dfs for this workload size, or
+    None: from the line "valu: queries=Q pairs=P insts_per_step=I
+    busy_profiled=B" of the profile summary (rocprofv3 --pmc SQ_INSTS_VALU,
+    SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE; B is AMD's VALUBusy in that
+    run, as a fraction)."""
+    import re
+
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    for m in re.finditer(r"^valu: queries=(\d+) pairs=(\d+) insts_per_step=(\S+) "
+                         r"busy_profiled=(\S+)\s*$", text, re.M):
+        if int(m.group(1)) == queries and int(m.group(2)) == pairs:
+            return {"insts_per_step": float(m.group(3)), "busy": float(m.group(4))}
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -1153,6 +1172,7 @@ def main():
         valu_frac = (rep["dfs_bv_tests"] * OPS_BV / PEAK_FP32 +
                      inline_leaf * OPS_LEAF / PEAK_FP64) / t_dfs
     traffic = measured_traffic(nq, args.pairs)
+    valu = measured_valu(nq, args.pairs)
     achieved = traffic["bytes"] / t_dfs if traffic and t_dfs > 0 else None
     frac = achieved / PEAK_HBM if achieved else None
     if frac is not None and frac > 1:
@@ -1174,6 +1194,11 @@ def main():
         "algorithmic_rate_gbs": alg_bytes / t_dfs / 1e9 if t_dfs > 0 else None,
         "valu_frac": valu_frac,
         "valu_peak_tflops": PEAK_FP32 / 1e12,
+        # vector issue, the limit the kernel is closest to (counters of the
+        # same workload): wave instructions per step, and AMD's VALUBusy in
+        # the counter run (a kernel of independent FP32 FMAs reads 0.87)
+        "valu_wave_insts_per_step": valu["insts_per_step"] if valu else None,
+        "valu_busy_in_profiled_run": valu["busy"] if valu else None,
         "kernel_ms": rep["dfs_ms"],
         "bv_tests": rep["dfs_bv_tests"],
         "lanes_per_step": (rep["dfs_bv_tests"] / rep["dfs_wave_steps"]
