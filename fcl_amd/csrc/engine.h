@@ -40,11 +40,11 @@ struct EngineTuning {
                                 // (at 4 wavefronts per SIMD only)
   int dfs_stack_top = 512;      // with inline triangle tests: stack entries per wavefront
                                 // kept in LDS (128, 256 or 512)
-  int bool_queries_per_wave = 0;  // queries a boolean-engine wavefront keeps in flight (4, 8, 16,
+  int bool_queries_per_wave = 8;  // queries a boolean-engine wavefront keeps in flight (4, 8, 16,
                                   // 32; 0: 4 below 65536 queries, else 8)
   int bool_waves_per_simd = 4;    // register budget of the boolean kernel (2, 3, 4; 32 slots: 2, 3)
   int bool_leaf_batch = 64;       // leaf pairs queued before the boolean kernel tests them
-  int bool_width = 32;            // most tasks of one query a boolean-kernel step tests
+  int bool_width = 16;            // most tasks of one query a boolean-kernel step tests
   int bool_refill = 4;            // free slots that make a boolean wavefront take new queries
   int fast_persistent = 0;        // 1: exhaustive FAST batches as one persistent launch (the
                                   // boolean kernel's exhaustive form) instead of rounds

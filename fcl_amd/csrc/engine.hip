@@ -1112,7 +1112,8 @@ constexpr int kAnyQueue = 128;
 constexpr int kAnySlotBits = 5;  // slots per wavefront <= 32
 constexpr int kAnyXcds = 8;      // query ranges (one per XCD)
 constexpr int kAnyAreaHead = 16 * kAnyXcds;  // area: range counters, 128 B apart; then the stacks
-constexpr long long kAnySmallBatch = 65536;  // below: 4 slots per wavefront by default, else 8
+constexpr long long kAnySmallBatch = 65536;  // bool_queries_per_wave 0: 4 slots per wavefront
+                                             // below this many queries, else 8
 constexpr int kAnySlotCap = 1024;            // stack entries per slot (at least)
 constexpr int kAnyGenMask = (1 << (22 - kAnySlotBits)) - 1;
 
@@ -2139,8 +2140,10 @@ struct AnyShape {
 
 AnyShape anyhit_shape(const EngineTuning& tu, long long nq, int stack_need) {
   AnyShape a;
-  // slots per wavefront: fewer for small batches, so that they still spread
-  // over many wavefronts
+  // slots per wavefront: the tunable (8 by default: under serving load,
+  // where batches of several streams share the device, 8 beat 4 for small
+  // batches too), or with 0 fewer for small batches, so that a batch alone
+  // still spreads over many wavefronts
   a.K = tu.bool_queries_per_wave > 0 ? tu.bool_queries_per_wave : nq < kAnySmallBatch ? 4 : 8;
   a.wps = tu.bool_waves_per_simd;
   a.width = std::max(tu.bool_width, kWave / a.K);

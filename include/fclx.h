@@ -380,15 +380,16 @@ int fclx_set_profiling(fclx_context* ctx, int32_t enable);
  *   "dfs_stack_top"     128, 256, 512  stack entries per wavefront kept in
  *                               LDS with inline triangle tests (default 512)
  *   "bool_queries_per_wave" 0, 4, 8, 16, 32  queries a BOOL-engine wavefront
- *                               keeps in flight (default 0: 4 for batches
- *                               below 65536 queries, else 8)
+ *                               keeps in flight (default 8, measured best
+ *                               under serving load; 0: 4 for batches below
+ *                               65536 queries, else 8)
  *   "bool_waves_per_simd" 2..4  register budget of the BOOL-engine kernel
  *                               (32 queries per wavefront: at most 3)
  *   "bool_leaf_batch"   1..64   leaf pairs a BOOL-engine wavefront queues
  *                               before testing them
  *   "bool_width"        1..64   most tasks of one query a BOOL-engine step
  *                               tests (at least 64 / queries per wavefront;
- *                               default 32)
+ *                               default 16)
  *   "bool_refill"       1..32   free query slots that make a BOOL-engine
  *                               wavefront take new queries (default 4)
  *   "dist_queries_per_wave" 2, 4, 8  queries a FAST distance wavefront keeps

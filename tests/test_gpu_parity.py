@@ -547,8 +547,8 @@ def test_bool_engine_matches_oracle(ctx, c0_meshes, case):
     assert 0.1 < (oc > 0).mean() < 0.9
 
 
-BOOL_DEFAULTS = {"bool_queries_per_wave": 0, "bool_waves_per_simd": 4, "bool_leaf_batch": 64,
-                 "bool_width": 32, "bool_refill": 4}
+BOOL_DEFAULTS = {"bool_queries_per_wave": 8, "bool_waves_per_simd": 4, "bool_leaf_batch": 64,
+                 "bool_width": 16, "bool_refill": 4}
 
 
 @pytest.mark.parametrize("params", [
@@ -559,7 +559,8 @@ BOOL_DEFAULTS = {"bool_queries_per_wave": 0, "bool_waves_per_simd": 4, "bool_lea
     {"bool_queries_per_wave": 32, "bool_width": 64, "bool_waves_per_simd": 2, "bool_leaf_batch": 7},
     {"bool_width": 64, "bool_leaf_batch": 1, "bool_refill": 16},
     {"bool_width": 1, "bool_waves_per_simd": 2, "bool_leaf_batch": 17, "bool_refill": 32},
-], ids=["q8w3", "q16w8", "q4", "q32narrow", "q32wide", "wide_lb1", "narrow_lb17"])
+    {"bool_queries_per_wave": 0, "bool_width": 32},
+], ids=["q8w3", "q16w8", "q4", "q32narrow", "q32wide", "wide_lb1", "narrow_lb17", "auto"])
 def test_bool_engine_tunables(ctx, c0_meshes, params):
     """Every slot count, width, leaf batch and refill threshold gives the
     oracle's booleans, on 6000 and 1000 queries of the plain pair and on a
