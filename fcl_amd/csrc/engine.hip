@@ -501,17 +501,50 @@ __device__ __forceinline__ bool first_larger(const ObbNode32& n1, const ObbNode3
 // Outcome of the FP32 pass over one task.
 enum : int { kStepNone = 0, kStepPush = 1, kStepLeaf = 2, kStepUncertain = 3 };
 
+// A 64-B node record in four 16-B loads issued together (its child link
+// with the rest, not in a later load once the cull has passed).
+__device__ __forceinline__ ObbNode32 load_node32(const ObbNode32* p) {
+  const uint4* w = reinterpret_cast<const uint4*>(p);
+  const uint4 a = w[0], b = w[1], c = w[2], d = w[3];
+  ObbNode32 n;
+  uint4* o = reinterpret_cast<uint4*>(&n);
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+  o[3] = d;
+  return n;
+}
+
+// The query's 64-B FP32 transform record, likewise in four 16-B loads.
+__device__ __forceinline__ QueryXf32 load_xf32(const QueryXf32* p) {
+  static_assert(sizeof(QueryXf32) == 64, "four 16-B words");
+  const uint4* w = reinterpret_cast<const uint4*>(p);
+  const uint4 a = w[0], b = w[1], c = w[2], d = w[3];
+  QueryXf32 x;
+  uint4* o = reinterpret_cast<uint4*>(&x);
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+  o[3] = d;
+  return x;
+}
+
 // FP32-only BV test of task (q,b1,b2) and the descent rule; nothing FP64 is
 // inlined here, which keeps the depth-first kernel's register count low.
+// kStats: per-query BV-test counts are requested (the branch and its
+// pointer are compiled out of the kernel that runs without them).
+template <bool kStats>
 __device__ __forceinline__ int step32(const EngineArgs& A, unsigned long long task,
                                       unsigned long long& c0, unsigned long long& c1) {
   int q, b1, b2;
   unpack_task(task, q, b1, b2);
   const QueryPtr P = A.qptr[q];
-  const QueryXf32& X = A.qx32[q];
-  const ObbNode32& n1 = P.n1[b1];
-  const ObbNode32& n2 = P.n2[b2];
-  if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+  const QueryXf32 X = load_xf32(A.qx32 + q);
+  const ObbNode32 n1 = load_node32(P.n1 + b1);
+  const ObbNode32 n2 = load_node32(P.n2 + b2);
+  if constexpr (kStats) {
+    if (A.L.bv_tests) atomicAdd(reinterpret_cast<unsigned long long*>(&A.L.bv_tests[q]), 1ULL);
+  }
   const int r = obb_cull32(X, n1, n2);
   if (r == kCullDisjoint) return kStepNone;
   if (r == kCullUnsure) return kStepUncertain;
@@ -589,7 +622,7 @@ __device__ __forceinline__ void dump_deferred(const EngineArgs& A, int w,
 // the limit at a time, read back when the LDS part runs low).
 constexpr int kHitStage = 256;
 constexpr int kStackTop = 512;
-template <int kWavesPerSimd, bool kInline>
+template <int kWavesPerSimd, bool kInline, bool kStats = true>
 __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A) {
   // with the triangle tests inline a queue never holds two full steps
   constexpr int kLq = kInline ? 2 * kWave : kLeafQ;
@@ -721,7 +754,7 @@ __global__ void __launch_bounds__(kBlock, kWavesPerSimd) engine_dfs(EngineArgs A
     if (active) task = slot(sp + lane);
     n_bv += n;
     ++n_steps;
-    const int r = active ? step32(A, task, c0, c1) : kStepNone;
+    const int r = active ? step32<kStats>(A, task, c0, c1) : kStepNone;
     // children onto the stack (one prefix sum)
     const unsigned long long pm = __ballot(r == kStepPush);
     const int npush = 2 * __popcll(pm);
@@ -1077,8 +1110,8 @@ enum : int { kAnyNone = 0, kAnyPush, kAnyLeaf, kAnyUnsure };
 
 __device__ __forceinline__ int cull_any(const QueryXf32& X, const QueryPtr& P, int tag, int b1,
                                         int b2, unsigned long long& c0, unsigned long long& c1) {
-  const ObbNode32& n1 = P.n1[b1];
-  const ObbNode32& n2 = P.n2[b2];
+  const ObbNode32 n1 = load_node32(P.n1 + b1);
+  const ObbNode32 n2 = load_node32(P.n2 + b2);
   const int r = obb_cull32(X, n1, n2);
   if (r == kCullUnsure) return kAnyUnsure;
   if (r == kCullDisjoint) return kAnyNone;
@@ -2526,9 +2559,12 @@ hipError_t engine_collide(EngineWorkspace* ws, const CollideLaunch& L, int stack
   // register budget of the depth-first kernel: 4, 5 or 6 wavefronts per SIMD
   // (more waves hide more latency; fewer registers spill more)
   // the inline triangle test needs the 128 registers of 4 wavefronts per SIMD
+  // (without per-query statistics, the inline form runs the variant that
+  // leaves their counting out)
   auto dfs_kernel = tune.dfs_waves_per_simd >= 6   ? engine_dfs<6, false>
                     : tune.dfs_waves_per_simd == 5 ? engine_dfs<5, false>
-                    : tune.dfs_inline_leaves       ? engine_dfs<4, true>
+                    : tune.dfs_inline_leaves       ? (L.bv_tests ? engine_dfs<4, true>
+                                                                 : engine_dfs<4, true, false>)
                                                    : engine_dfs<4, false>;
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dfs_kernel, kBlock, 0) != hipSuccess ||
