@@ -311,20 +311,23 @@ int slot_launch(fclx_server* S, Slot& s) {
   const size_t tbytes = 8 * static_cast<size_t>(R + 1) + 4 * (2 * static_cast<size_t>(R) + nl);
   run_jobs(S, jobs);
   if (fclx::take_injected_failure(S->ctx)) return FCLX_ERR_DEVICE;
-  // pinned host memory (src: its device address) to the device, by the
-  // gather kernel when 16-B aligned (the table is rounded up to whole
-  // words: its buffers have room), else by a copy command
-  auto to_device = [&](void* dst, const void* src, size_t bytes) {
+  // pinned host memory to the device: by the gather kernel from its device
+  // address when 16-B aligned (the table is rounded up to whole words: its
+  // buffers have room), else by a copy command from its host address
+  auto to_device = [&](void* dst, const void* dev_src, const void* host_src, size_t bytes) {
     const int64_t n16 = static_cast<int64_t>((bytes + 15) / 16);
-    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0 && (bytes % 16 == 0 || src == s.h_table)) {
+    if ((reinterpret_cast<uintptr_t>(dev_src) & 15) == 0 &&
+        (bytes % 16 == 0 || host_src == s.h_table)) {
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n16 + 255) / 256, 1024));
-      gather_words<<<grid, 256, 0, s.stream>>>(static_cast<const uint4*>(src),
+      gather_words<<<grid, 256, 0, s.stream>>>(static_cast<const uint4*>(dev_src),
                                                static_cast<uint4*>(dst), n16);
       return hipGetLastError() == hipSuccess;
     }
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s.stream) == hipSuccess;
+    return hipMemcpyAsync(dst, host_src, bytes, hipMemcpyHostToDevice, s.stream) == hipSuccess;
   };
-  if (!to_device(s.d_table, s.h_table, tbytes)) return FCLX_ERR_DEVICE;
+  // the slot's own pinned buffers (hipHostMalloc) have one address for
+  // host and device
+  if (!to_device(s.d_table, s.h_table, s.h_table, tbytes)) return FCLX_ERR_DEVICE;
   // poses: pinned requests straight from the caller's memory, the others
   // from the staging in runs of consecutive requests
   int64_t run_first = -1;
@@ -334,12 +337,12 @@ int slot_launch(fclx_server* S, Slot& s) {
     if (!staged && run_first >= 0) {
       const int64_t end = r < R ? s.first[r] : k;
       if (!to_device(s.d_tf2 + 12 * run_first, s.h_tf2 + 12 * run_first,
-                     96 * static_cast<size_t>(end - run_first)))
+                     s.h_tf2 + 12 * run_first, 96 * static_cast<size_t>(end - run_first)))
         return FCLX_ERR_DEVICE;
       run_first = -1;
     }
     if (r < R && s.batch[r].pinned &&
-        !to_device(s.d_tf2 + 12 * s.first[r], s.batch[r].dev_poses,
+        !to_device(s.d_tf2 + 12 * s.first[r], s.batch[r].dev_poses, s.batch[r].req.link_poses,
                    96 * static_cast<size_t>(s.batch[r].nq)))
       return FCLX_ERR_DEVICE;
   }
