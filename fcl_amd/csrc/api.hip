@@ -27,7 +27,8 @@ struct MeshSlot {
   ObbNode64* obb64 = nullptr;
   RssNode64* rss64 = nullptr;
   double* size64 = nullptr;
-  ObbNode32* obb32 = nullptr;
+  ObbNode32* obb32 = nullptr;        // obb32_alloc + 1 (see fclx_mesh_create)
+  ObbNode32* obb32_alloc = nullptr;
   RssNode32* rss32 = nullptr;
   double* verts = nullptr;
   int32_t* tris = nullptr;
@@ -123,7 +124,7 @@ void free_slot(MeshSlot& s) {
   (void)hipFree(s.obb64);
   (void)hipFree(s.rss64);
   (void)hipFree(s.size64);
-  (void)hipFree(s.obb32);
+  (void)hipFree(s.obb32_alloc);
   (void)hipFree(s.rss32);
   (void)hipFree(s.verts);
   (void)hipFree(s.tri_verts);
@@ -361,14 +362,19 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
             vertices[3 * static_cast<size_t>(tris[3 * i + k]) + c];
   std::lock_guard<std::mutex> lock(ctx->mu);
   (void)hipSetDevice(ctx->device);
+  // The FP32 nodes start one 64-B record into their allocation (256-B
+  // aligned): node i sits at 64 (i + 1) bytes, so each pair of children
+  // (first_child odd) shares one 128-B line, which the two child tasks of a
+  // descent read in the same step.
   bool ok = dmalloc(&s.obb64, nb) == hipSuccess && dmalloc(&s.rss64, nb) == hipSuccess &&
-            dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32, nb) == hipSuccess &&
+            dmalloc(&s.size64, nb) == hipSuccess && dmalloc(&s.obb32_alloc, nb + 1) == hipSuccess &&
             dmalloc(&s.rss32, nb) == hipSuccess &&
             dmalloc(&s.verts, 3 * static_cast<size_t>(num_vertices)) == hipSuccess &&
             dmalloc(&s.tris, 3 * static_cast<size_t>(num_tris)) == hipSuccess &&
             dmalloc(&s.tri_verts, tv.size()) == hipSuccess &&
             dmalloc(&s.span, 2 * static_cast<size_t>(nb)) == hipSuccess &&
             dmalloc(&s.prim_pos, static_cast<size_t>(num_tris)) == hipSuccess;
+  if (ok) s.obb32 = s.obb32_alloc + 1;
   if (ok)
     ok = hipMemcpy(s.span, span.data(), sizeof(int32_t) * span.size(), hipMemcpyHostToDevice) ==
              hipSuccess &&
@@ -394,7 +400,7 @@ int fclx_mesh_create(fclx_context* ctx, int32_t bv_type, const double* vertices,
     free_slot(s);
     return fail(ctx, FCLX_ERR_OUT_OF_MEMORY, "device allocation or upload failed");
   }
-  s.bytes = static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
+  s.bytes = sizeof(ObbNode32) + static_cast<int64_t>(nb) * (sizeof(ObbNode64) + sizeof(RssNode64) + sizeof(double) +
                                         sizeof(ObbNode32) + sizeof(RssNode32)) +
             24LL * num_vertices + 16LL * num_tris + 8LL * nb +
             8LL * kTriVertsStride * num_tris;
